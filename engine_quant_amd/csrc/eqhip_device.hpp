@@ -1,0 +1,71 @@
+// Device-side building blocks of the exposure kernels (gfx950, FP64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace eqhip {
+
+constexpr double kDateTol = 1e-9;  // date matching tolerance, products/irs.rs:151-154,173
+
+// B(tau) = (1 - exp(-a tau)) / a, models/hull_white.rs:51-55 (same expression order).
+__device__ __forceinline__ double hw_b(double a, double tau) {
+    double e = exp(a * (-tau));
+    return ((-e) + 1.0) / a;
+}
+
+// A(tau), models/hull_white.rs:59-69.
+__device__ __forceinline__ double hw_a(double a, double b, double sigma, double tau, double bf) {
+    double a2 = a * a;
+    double sigma2 = sigma * sigma;
+    double term1 = ((bf + (-tau)) * (a2 * b - sigma2 * 0.5)) / a2;
+    double term2 = ((sigma2 * bf) * bf) / (a * 4.0);
+    return exp(term1 - term2);
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
+// SC'11): 10 rounds of two 32x32->64 multiplies, Weyl key schedule.
+__device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                              uint32_t k0, uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+    for (int round = 0; round < 10; ++round) {
+        if (round > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint32_t lo0 = 0xD2511F53u * c0;
+        uint32_t hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+// Two standard normals for (path, step pair): counter (pair, path_lo, path_hi, 0), key = seed.
+// u1 = ((w_a >> 11) + 1) 2^-53 in (0,1], u2 = (w_b >> 11) 2^-53 in [0,1); Box-Muller.
+__device__ __forceinline__ void philox_normal_pair(uint64_t seed, uint64_t path, uint32_t pair,
+                                                   double& z0, double& z1) {
+    uint32_t o[4];
+    philox4x32_10(pair, (uint32_t)path, (uint32_t)(path >> 32), 0u, (uint32_t)seed,
+                  (uint32_t)(seed >> 32), o);
+    uint64_t wa = ((uint64_t)o[1] << 32) | o[0];
+    uint64_t wb = ((uint64_t)o[3] << 32) | o[2];
+    const double two_m53 = 1.0 / 9007199254740992.0;
+    double u1 = (double)((wa >> 11) + 1u) * two_m53;
+    double u2 = (double)(wb >> 11) * two_m53;
+    double rad = sqrt(-2.0 * log(u1));
+    double th = 6.283185307179586476925286766559 * u2;
+    z0 = rad * cos(th);
+    z1 = rad * sin(th);
+}
+
+}  // namespace eqhip

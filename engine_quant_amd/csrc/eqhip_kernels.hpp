@@ -1,0 +1,94 @@
+// Kernel argument blocks and launch constants shared by the kernels and the host plan.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace eqhip {
+
+constexpr int kValueThreads = 256;   // 4 wave64s per valuation block
+constexpr int kItemsPerThread = 4;   // trade work items per thread in the valuation block
+constexpr int kTradesPerItem = 4;    // trades of one schedule class per work item
+constexpr int kSelectThreads = 256;  // EE / PFE select block
+constexpr double kDateTolHost = 1e-9; // date matching tolerance, products/irs.rs:151-154,173
+
+struct SimArgs {
+    double a, b, sigma, r0;
+    double dt, sqrt_dt;          // Euler
+    double decay, mean_c, sd;    // exact transition
+    uint64_t n_paths;
+    uint32_t n_steps;
+    uint64_t seed;
+    const int* slot_of_step;     // [n_steps + 1], -1 when r is not stored at that step
+    const double* Z;             // [n_steps][n_paths] host draws
+    double* R;                   // [n_slots][n_paths]
+};
+
+struct ValueArgs {
+    const double* R;
+    uint64_t n_paths;
+    uint64_t n_tiles;
+    int n_dates, D, C, S, Fmax;
+    int n_chunks, tiles_per_block;
+    const double2* AB;        // [n_dates][D] (A, B) of P(t_k, T_d)
+    const int* date_slot;     // [n_dates]
+    const int* date_dfirst;   // [n_dates] first live cash-flow date
+    const int* date_det;      // [n_dates] 1 when the date sits at grid index 0
+    const int4* cdesc;        // [n_dates][C] mode, first flow, end flow, fixing index
+    const int2* cdate;        // [n_dates][C] start/next date, maturity date
+    const int* flow_d;        // class flows: cash-flow date index
+    const double* flow_tau;   // class flows: accrual
+    const int* fix_off;       // [n_dates + 1]
+    const int* fix_slot;      // R slot of the fixing date
+    const double2* fix_AB;    // (A, B) of the fixing period
+    const int* set_off;       // [S + 1]
+    const int* set_cls;
+    const double* set_alpha;  // sum of notionals of the set's trades in the class
+    const double* set_beta;   // sum of notional * fixed rate
+    double* X;                // [S][n_dates][n_paths] max(V_set, 0)
+    double* set_mtm;          // optional [n_dates][S][n_paths]
+    int want_trade;
+    int n_items, n_trades;
+    const int* item_cls;
+    const int* item_first;
+    const int* item_count;
+    const double* tr_N;       // trades sorted by class
+    const double* tr_NK;
+    const int* tr_orig;
+    double* partial;          // [n_trades][n_dates][n_chunks]
+    double* path_mtm;         // optional [n_dates][n_trades][n_paths]
+};
+
+struct PvArgs {
+    const double* pillars;
+    const double* zeros;
+    int np;
+    int n_trades;
+    const int* flow_off;
+    const double* pay;
+    const double* acc;
+    const double* notional;
+    const double* fixed;
+    const double* start;
+    double bump;
+    double* pv;
+    double* dv01;
+    double* bucketed;
+    int bucketed_on;
+};
+
+template <int DISC, int RNG>
+__global__ void simulate_kernel(SimArgs g);
+__global__ void ab_kernel(const double* tau, int n, double a, double b, double sigma, double2* out);
+template <int PT>
+__global__ void value_kernel(ValueArgs g);
+__global__ void select_kernel(const double* X, int n_dates, uint64_t n_paths, const uint64_t* n_eff,
+                              const uint64_t* q_rank, double* ee, double* pfe);
+__global__ void trade_ee_kernel(const double* partial, int n_trades, int n_dates, int n_chunks,
+                                const uint64_t* n_eff, double* trade_ee);
+__global__ void cva_kernel(const double* ee, int n_profiles, int n_dates, const double* times,
+                           double a, double b, double sigma, double r0, double hazard,
+                           double recovery, double* cva);
+__global__ void pv_dv01_kernel(PvArgs g);
+
+}  // namespace eqhip

@@ -1,0 +1,297 @@
+// Measures of the exposure path and their GPU arm (cpp/engine/src/measure.cpp:20-462,
+// cpp/engine/src/engine.cpp:73-113). Every numeric result comes from libeqhip (include/eqhip.h).
+#include <cmath>
+#include <map>
+
+#include "eqhip.h"
+#include "engine.hpp"
+
+namespace engine {
+
+namespace {
+
+[[noreturn]] void throw_eqhip(int rc) {
+    char buf[2048];
+    eqhip_last_error(buf, sizeof(buf));
+    std::string msg = std::string("eqhip: ") + buf;
+    if (rc == EQHIP_E_INVALID) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+void check(int rc) {
+    if (rc != EQHIP_OK) throw_eqhip(rc);
+}
+
+// One eqhip context (one HIP stream) per host thread, created on first use.
+eqhip_context* gpu_context() {
+    struct Holder {
+        eqhip_context* ctx = nullptr;
+        ~Holder() {
+            if (ctx) eqhip_context_destroy(ctx);
+        }
+    };
+    thread_local Holder holder;
+    if (!holder.ctx) check(eqhip_context_create(0, &holder.ctx));
+    return holder.ctx;
+}
+
+// backend "cpu" is the reference's engine-core; this library is its GPU arm only (api.rs:151-159).
+void require_gpu(const ExecutionContext& execution) {
+    if (execution.backend() != "gpu")
+        throw std::invalid_argument(
+            "ExecutionContext backend '" + execution.backend() +
+            "' is served by the reference engine (engine-core); this library implements backend 'gpu'");
+}
+
+int disc_of(const ExecutionContext& execution) {
+    return execution.discretisation() == "exact" ? EQHIP_DISC_EXACT : EQHIP_DISC_EULER;
+}
+
+const HullWhite1FModel& require_hw1f(const IModel& model, const char* who) {
+    const auto* hw1 = dynamic_cast<const HullWhite1FModel*>(&model);
+    if (!hw1) throw std::invalid_argument(std::string(who) + ": modelo no soportado: " + model.type_name());
+    return *hw1;
+}
+
+const IrSwapProduct& require_irs(const IProduct& product, const char* who) {
+    const auto* irs = dynamic_cast<const IrSwapProduct*>(&product);
+    if (!irs) throw std::invalid_argument(std::string(who) + ": producto no soportado: " + product.type_name());
+    return *irs;
+}
+
+double par_rate_from_market(const MarketSnapshot& market, const IrSwapProduct& p) {
+    // measure.cpp:124-134
+    double numerator = market.discount_factor(p.start()) - market.discount_factor(p.payment_times().back());
+    double denominator = 0.0;
+    for (std::size_t i = 0; i < p.payment_times().size(); ++i)
+        denominator += p.accruals()[i] * market.discount_factor(p.payment_times()[i]);
+    return numerator / denominator;
+}
+
+double effective_fixed_rate(const MarketSnapshot& market, const IrSwapProduct& p) {
+    return p.use_par_rate() ? par_rate_from_market(market, p) : p.fixed_rate();
+}
+
+ExposureProfile compute_exposure_profile(const IModel& model, const IrSwapProduct& irs, const PricingContext& pricing,
+                                         const ExecutionContext& execution) {
+    const auto& hw1 = require_hw1f(model, "ExposureProfileMeasure");
+    return irs_hull_white_exposure_profile(execution, hw1.a(), hw1.b(), hw1.sigma(), hw1.r0(), irs.notional(),
+                                           irs.fixed_rate(), irs.use_par_rate(), irs.start(), irs.payment_times(),
+                                           irs.accruals(), reset_dates(irs), pricing.n_steps(), pricing.n_paths(),
+                                           pricing.seed());
+}
+
+}  // namespace
+
+std::vector<double> reset_dates(const IrSwapProduct& product) {
+    std::vector<double> dates;
+    dates.reserve(product.payment_times().size());
+    dates.push_back(product.start());
+    for (std::size_t i = 0; i + 1 < product.payment_times().size(); ++i) dates.push_back(product.payment_times()[i]);
+    return dates;
+}
+
+// ---- GPU arm of the FFI wrappers ---------------------------------------------------------------
+
+ExposureProfile irs_hull_white_exposure_profile(const ExecutionContext& execution, double a, double b, double sigma,
+                                                double r0, double notional, double fixed_rate, bool use_par_rate,
+                                                double start, const std::vector<double>& payment_times,
+                                                const std::vector<double>& accruals,
+                                                const std::vector<double>& monitoring_times, std::size_t n_steps,
+                                                std::uint64_t n_paths, std::uint64_t seed) {
+    require_gpu(execution);
+    std::size_t nd = monitoring_times.size();
+    ExposureProfile out;
+    out.times.resize(nd);
+    out.ee.resize(nd);
+    out.pfe_95.resize(nd);
+    int32_t n_out = 0;
+    check(eqhip_irs_hull_white_exposure_profile(
+        gpu_context(), a, b, sigma, r0, notional, fixed_rate, use_par_rate ? 1 : 0, start, payment_times.data(),
+        accruals.data(), (int32_t)payment_times.size(), monitoring_times.data(), (int32_t)nd, (uint32_t)n_steps,
+        n_paths, seed, EQHIP_RNG_PHILOX, disc_of(execution), nullptr, 0, out.times.data(), out.ee.data(),
+        out.pfe_95.data(), &n_out));
+    out.times.resize(n_out);
+    out.ee.resize(n_out);
+    out.pfe_95.resize(n_out);
+    return out;
+}
+
+std::vector<ExposureProfile> irs_hull_white_exposure_profile_batch(
+    const ExecutionContext& execution, double a, double b, double sigma, double r0,
+    const std::vector<double>& notionals, const std::vector<double>& fixed_rates, double start,
+    const std::vector<double>& payment_times, const std::vector<double>& accruals,
+    const std::vector<double>& monitoring_times, std::size_t n_steps, std::uint64_t n_paths, std::uint64_t seed) {
+    require_gpu(execution);
+    if (notionals.size() != fixed_rates.size())
+        throw std::invalid_argument("el lote requiere un fixed_rate por notional");
+    std::size_t nt = notionals.size(), nd = monitoring_times.size();
+    std::vector<double> times(nd), ee(nt * nd), pfe(nt * nd);
+    int32_t n_out = 0;
+    check(eqhip_irs_hull_white_exposure_profile_batch(
+        gpu_context(), a, b, sigma, r0, notionals.data(), fixed_rates.data(), (int32_t)nt, start,
+        payment_times.data(), accruals.data(), (int32_t)payment_times.size(), monitoring_times.data(), (int32_t)nd,
+        (uint32_t)n_steps, n_paths, seed, EQHIP_RNG_PHILOX, disc_of(execution), nullptr, 0, times.data(), ee.data(),
+        pfe.data(), &n_out));
+    std::vector<ExposureProfile> out(nt);
+    for (std::size_t j = 0; j < nt; ++j) {
+        out[j].times.assign(times.begin(), times.begin() + n_out);
+        out[j].ee.assign(ee.begin() + j * nd, ee.begin() + j * nd + n_out);
+        out[j].pfe_95.assign(pfe.begin() + j * nd, pfe.begin() + j * nd + n_out);
+    }
+    return out;
+}
+
+double unilateral_cva_from_exposure(const ExecutionContext& execution, double a, double b, double sigma, double r0,
+                                    const std::vector<double>& times, const std::vector<double>& ee,
+                                    double hazard_rate, double recovery_rate) {
+    require_gpu(execution);
+    if (times.size() != ee.size()) throw std::invalid_argument("times y ee deben tener el mismo largo");
+    double cva = 0.0;
+    check(eqhip_unilateral_cva_from_exposure(gpu_context(), a, b, sigma, r0, times.data(), ee.data(),
+                                             (int32_t)times.size(), hazard_rate, recovery_rate, &cva));
+    return cva;
+}
+
+std::vector<double> unilateral_cva_from_exposure_batch(const ExecutionContext& execution, double a, double b,
+                                                       double sigma, double r0,
+                                                       const std::vector<ExposureProfile>& profiles,
+                                                       double hazard_rate, double recovery_rate) {
+    require_gpu(execution);
+    std::vector<double> out(profiles.size());
+    if (profiles.empty()) return out;
+    const std::vector<double>& times = profiles.front().times;
+    std::vector<double> ee;
+    ee.reserve(profiles.size() * times.size());
+    for (const auto& p : profiles) {
+        if (p.times != times || p.ee.size() != times.size())
+            throw std::invalid_argument("unilateral_cva_from_exposure_batch: profiles must share their times");
+        ee.insert(ee.end(), p.ee.begin(), p.ee.end());
+    }
+    check(eqhip_unilateral_cva_from_exposure_batch(gpu_context(), a, b, sigma, r0, times.data(), ee.data(),
+                                                   (int32_t)profiles.size(), (int32_t)times.size(), hazard_rate,
+                                                   recovery_rate, out.data()));
+    return out;
+}
+
+// ---- batched helpers (measure.cpp:238-325) ----------------------------------------------------
+
+std::vector<ExposureProfile> compute_exposure_profile_batch(const IModel& model,
+                                                            const std::vector<const IrSwapProduct*>& products,
+                                                            const PricingContext& pricing,
+                                                            const ExecutionContext& execution) {
+    const auto& hw1 = require_hw1f(model, "ExposureProfileMeasure");
+    const IrSwapProduct& first = *products.front();
+    std::vector<double> notionals, fixed_rates;
+    for (const auto* p : products) {
+        notionals.push_back(p->notional());
+        fixed_rates.push_back(p->fixed_rate());
+    }
+    return irs_hull_white_exposure_profile_batch(execution, hw1.a(), hw1.b(), hw1.sigma(), hw1.r0(), notionals,
+                                                 fixed_rates, first.start(), first.payment_times(), first.accruals(),
+                                                 reset_dates(first), pricing.n_steps(), pricing.n_paths(),
+                                                 pricing.seed());
+}
+
+std::vector<double> compute_cva_from_exposure_batch(const IModel& model, const ExecutionContext& execution,
+                                                    const std::vector<ExposureProfile>& profiles,
+                                                    double hazard_rate, double recovery_rate) {
+    const auto& hw1 = require_hw1f(model, "UnilateralCvaMeasure");
+    return unilateral_cva_from_exposure_batch(execution, hw1.a(), hw1.b(), hw1.sigma(), hw1.r0(), profiles,
+                                              hazard_rate, recovery_rate);
+}
+
+PvDv01 compute_pv_dv01_batch(const MarketSnapshot& market, const std::vector<const IrSwapProduct*>& products,
+                             double bump, bool bucketed, const ExecutionContext& execution) {
+    require_gpu(execution);
+    std::size_t n = products.size(), np = market.pillars().size();
+    std::vector<int32_t> set_id(n), off(n + 1, 0);
+    std::vector<double> notional(n), fixed(n), start(n), pay, acc;
+    for (std::size_t j = 0; j < n; ++j) {
+        const IrSwapProduct& p = *products[j];
+        set_id[j] = (int32_t)j;
+        notional[j] = p.notional();
+        fixed[j] = effective_fixed_rate(market, p);  // fixed once under the base curve (measure.cpp:195-200)
+        start[j] = p.start();
+        pay.insert(pay.end(), p.payment_times().begin(), p.payment_times().end());
+        acc.insert(acc.end(), p.accruals().begin(), p.accruals().end());
+        off[j + 1] = (int32_t)pay.size();
+    }
+    eqhip_portfolio pf{(int32_t)n, (int32_t)n, set_id.data(), notional.data(), fixed.data(), start.data(),
+                       off.data(), pay.data(), acc.data()};
+    PvDv01 out;
+    out.pv.resize(n);
+    out.dv01.resize(n);
+    std::vector<double> bk(bucketed ? n * np : 0);
+    check(eqhip_pv_dv01(gpu_context(), market.pillars().data(), market.zero_rates().data(), (int32_t)np, &pf, bump,
+                        out.pv.data(), out.dv01.data(), bucketed ? bk.data() : nullptr));
+    if (bucketed)
+        for (std::size_t j = 0; j < n; ++j) out.bucketed.emplace_back(bk.begin() + j * np, bk.begin() + (j + 1) * np);
+    return out;
+}
+
+// ---- measures -----------------------------------------------------------------------------------
+
+MeasureResult ExposureProfileMeasure::evaluate(const IModel& model, const IProduct& product, const MarketSnapshot&,
+                                               const PricingContext& pricing,
+                                               const ExecutionContext& execution) const {
+    const auto& irs = require_irs(product, "ExposureProfileMeasure");
+    ExposureProfile profile = compute_exposure_profile(model, irs, pricing, execution);
+    MeasureResult r;
+    r.times = std::move(profile.times);
+    r.primary = std::move(profile.ee);
+    r.secondary = std::move(profile.pfe_95);
+    return r;
+}
+
+MeasureResult UnilateralCvaMeasure::evaluate(const IModel& model, const IProduct& product,
+                                             const MarketSnapshot& market, const PricingContext& pricing,
+                                             const ExecutionContext& execution) const {
+    ExposureProfileMeasure exposure(Params{});
+    MeasureResult r = exposure.evaluate(model, product, market, pricing, execution);
+    const auto& hw1 = require_hw1f(model, "UnilateralCvaMeasure");
+    r.scalar = unilateral_cva_from_exposure(execution, hw1.a(), hw1.b(), hw1.sigma(), hw1.r0(), r.times, r.primary,
+                                            market.hazard_rate(), market.recovery_rate());
+    r.has_scalar = true;
+    return r;
+}
+
+MeasureResult PresentValueMeasure::evaluate(const IModel&, const IProduct& product, const MarketSnapshot& market,
+                                            const PricingContext& pricing, const ExecutionContext& execution) const {
+    const auto& irs = require_irs(product, "PresentValueMeasure");
+    if (pricing.pricing_date() != 0.0)
+        throw std::invalid_argument("PresentValueMeasure: pricing_date != 0 is not supported by the GPU PV kernel");
+    PvDv01 v = compute_pv_dv01_batch(market, {&irs}, 0.0001, false, execution);
+    MeasureResult r;
+    r.has_scalar = true;
+    r.scalar = v.pv[0];
+    return r;
+}
+
+MeasureResult Dv01Measure::evaluate(const IModel&, const IProduct& product, const MarketSnapshot& market,
+                                    const PricingContext& pricing, const ExecutionContext& execution) const {
+    const auto& irs = require_irs(product, "Dv01Measure");
+    if (pricing.pricing_date() != 0.0)
+        throw std::invalid_argument("Dv01Measure: pricing_date != 0 is not supported by the GPU PV kernel");
+    PvDv01 v = compute_pv_dv01_batch(market, {&irs}, bump_, bucketed_, execution);
+    MeasureResult r;
+    if (bucketed_) {
+        r.times = market.pillars();
+        r.primary = v.bucketed[0];
+    } else {
+        r.has_scalar = true;
+        r.scalar = v.dv01[0];
+    }
+    return r;
+}
+
+void register_builtins(Registries& registries) {
+    registries.models.register_type<HullWhite1FModel>("HullWhite1F");
+    registries.products.register_type<IrSwapProduct>("IRSwap");
+    registries.measures.register_type<ExposureProfileMeasure>("ExposureProfile");
+    registries.measures.register_type<UnilateralCvaMeasure>("UnilateralCVA");
+    registries.measures.register_type<PresentValueMeasure>("PV");
+    registries.measures.register_type<Dv01Measure>("DV01");
+}
+
+}  // namespace engine

@@ -1,0 +1,323 @@
+"""ctypes binding of the C ABI in ``include/eqhip.h`` (libeqhip.so, HIP kernels for gfx950).
+
+This is the binding a Python maintainer of the reference would add beside its nanobind module;
+it loads the in-tree ``libeqhip.so`` and fails loudly when the library is missing. It never falls
+back to a CPU implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libeqhip.so")
+
+OK, E_INVALID, E_NO_DEVICE, E_HIP, E_CAPACITY = 0, 1, 2, 3, 4
+RNG_HOST_DRAWS, RNG_PHILOX = 0, 1
+DISC_EULER, DISC_EXACT = 0, 1
+
+_D, _I32, _U32, _U64 = ctypes.c_double, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint64
+_PD = ctypes.POINTER(ctypes.c_double)
+_PI32 = ctypes.POINTER(ctypes.c_int32)
+
+# Every symbol declared in include/eqhip.h.
+EXPORTED = [
+    "eqhip_is_gpu_backend_available", "eqhip_last_error", "eqhip_context_create",
+    "eqhip_context_destroy", "eqhip_irs_hull_white_exposure_profile",
+    "eqhip_irs_hull_white_exposure_profile_batch", "eqhip_unilateral_cva_from_exposure",
+    "eqhip_unilateral_cva_from_exposure_batch", "eqhip_pv_dv01", "eqhip_plan_create",
+    "eqhip_plan_execute", "eqhip_plan_timing", "eqhip_plan_destroy", "eqhip_exposure_run",
+]
+
+
+class EqhipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"eqhip error {code}: {msg}")
+        self.code = code
+
+
+class ModelHW1F(ctypes.Structure):
+    _fields_ = [("a", _D), ("b", _D), ("sigma", _D), ("r0", _D)]
+
+
+class PortfolioC(ctypes.Structure):
+    _fields_ = [("n_trades", _I32), ("n_sets", _I32), ("set_id", _PI32), ("notional", _PD),
+                ("fixed_rate", _PD), ("start", _PD), ("flow_offset", _PI32),
+                ("payment_times", _PD), ("accruals", _PD)]
+
+
+class McC(ctypes.Structure):
+    _fields_ = [("n_paths", _U64), ("n_steps", _U32), ("seed", _U64), ("rng", _I32), ("disc", _I32)]
+
+
+class CreditC(ctypes.Structure):
+    _fields_ = [("hazard_rate", _D), ("recovery_rate", _D)]
+
+
+class OutputsC(ctypes.Structure):
+    _fields_ = [("set_ee", _PD), ("set_pfe", _PD), ("set_cva", _PD), ("trade_ee", _PD),
+                ("trade_cva", _PD), ("path_mtm", _PD), ("set_mtm", _PD)]
+
+
+class TimingC(ctypes.Structure):
+    _fields_ = [("simulate_ms", _D), ("value_ms", _D), ("reduce_ms", _D), ("cva_ms", _D),
+                ("total_ms", _D), ("n_value_blocks", _U64), ("paths_per_tile", _U32),
+                ("tiles_per_block", _U32)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libeqhip.so (raises if it was not built: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.eqhip_is_gpu_backend_available.restype = ctypes.c_int
+        L.eqhip_last_error.restype = ctypes.c_size_t
+        L.eqhip_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+        L.eqhip_context_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.eqhip_context_destroy.argtypes = [ctypes.c_void_p]
+        prof = [ctypes.c_void_p, _D, _D, _D, _D]
+        tail = [_D, _PD, _PD, _I32, _PD, _I32, _U32, _U64, _U64, _I32, _I32, _PD, _I32, _PD, _PD, _PD, _PI32]
+        L.eqhip_irs_hull_white_exposure_profile.argtypes = prof + [_D, _D, ctypes.c_int] + tail
+        L.eqhip_irs_hull_white_exposure_profile_batch.argtypes = prof + [_PD, _PD, _I32] + tail
+        L.eqhip_unilateral_cva_from_exposure.argtypes = prof + [_PD, _PD, _I32, _D, _D, _PD]
+        L.eqhip_unilateral_cva_from_exposure_batch.argtypes = prof + [_PD, _PD, _I32, _I32, _D, _D, _PD]
+        L.eqhip_pv_dv01.argtypes = [ctypes.c_void_p, _PD, _PD, _I32, ctypes.POINTER(PortfolioC), _D, _PD,
+                                    _PD, _PD]
+        L.eqhip_plan_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(ModelHW1F),
+                                        ctypes.POINTER(PortfolioC), _PD, _I32, ctypes.POINTER(McC),
+                                        _I32, ctypes.POINTER(ctypes.c_void_p)]
+        L.eqhip_plan_execute.argtypes = [ctypes.c_void_p, _PD, ctypes.POINTER(CreditC),
+                                         ctypes.POINTER(OutputsC)]
+        L.eqhip_plan_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(TimingC)]
+        L.eqhip_plan_destroy.argtypes = [ctypes.c_void_p]
+        L.eqhip_exposure_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(ModelHW1F),
+                                         ctypes.POINTER(PortfolioC), _PD, _I32, ctypes.POINTER(McC), _PD,
+                                         ctypes.POINTER(CreditC), ctypes.POINTER(OutputsC)]
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(4096)
+    lib().eqhip_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def _check(code):
+    if code != OK:
+        raise EqhipError(code, last_error())
+
+
+def is_gpu_backend_available() -> bool:
+    return bool(lib().eqhip_is_gpu_backend_available())
+
+
+def _f64(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+def _dp(x):
+    return None if x is None else x.ctypes.data_as(_PD)
+
+
+@dataclass
+class Portfolio:
+    """SoA IRS portfolio (receiver swaps carry a negative notional)."""
+    set_id: np.ndarray
+    notional: np.ndarray
+    fixed_rate: np.ndarray
+    start: np.ndarray
+    flow_off: np.ndarray
+    pay: np.ndarray
+    acc: np.ndarray
+
+    @property
+    def n_trades(self):
+        return len(self.notional)
+
+    @property
+    def n_sets(self):
+        return int(self.set_id.max()) + 1 if len(self.set_id) else 0
+
+    def as_dict(self):
+        return dict(set_id=self.set_id, notional=self.notional, fixed_rate=self.fixed_rate,
+                    start=self.start, flow_off=self.flow_off, pay=self.pay, acc=self.acc)
+
+    def _c(self):
+        self.set_id = np.ascontiguousarray(self.set_id, dtype=np.int32)
+        self.flow_off = np.ascontiguousarray(self.flow_off, dtype=np.int32)
+        for name in ("notional", "fixed_rate", "start", "pay", "acc"):
+            setattr(self, name, _f64(getattr(self, name)))
+        return PortfolioC(self.n_trades, self.n_sets, self.set_id.ctypes.data_as(_PI32),
+                          _dp(self.notional), _dp(self.fixed_rate), _dp(self.start),
+                          self.flow_off.ctypes.data_as(_PI32), _dp(self.pay), _dp(self.acc))
+
+
+class Plan:
+    """Device-resident exposure job (eqhip_plan_create / execute / destroy)."""
+
+    def __init__(self, ctx, model, portfolio: Portfolio, dates, n_paths, n_steps, seed=0,
+                 rng=RNG_PHILOX, disc=DISC_EULER, want_trade_ee=False):
+        self.ctx = ctx
+        self.portfolio = portfolio
+        self.dates = _f64(dates)
+        self.n_paths, self.n_steps = int(n_paths), int(n_steps)
+        self.rng = rng
+        self.want_trade_ee = bool(want_trade_ee)
+        self._pc = portfolio._c()
+        m = ModelHW1F(*model)
+        mc = McC(self.n_paths, self.n_steps, int(seed), rng, disc)
+        h = ctypes.c_void_p()
+        _check(lib().eqhip_plan_create(ctx.handle, ctypes.byref(m), ctypes.byref(self._pc), _dp(self.dates),
+                                       len(self.dates), ctypes.byref(mc), int(want_trade_ee), ctypes.byref(h)))
+        self.handle = h
+
+    def execute(self, host_draws=None, hazard_rate=0.0, recovery_rate=0.0, want_cva=True,
+                want_path=False, want_set_mtm=False):
+        ns, nd, nt = self.portfolio.n_sets, len(self.dates), self.portfolio.n_trades
+        res = {"set_ee": np.zeros((ns, nd)), "set_pfe": np.zeros((ns, nd))}
+        if want_cva:
+            res["set_cva"] = np.zeros(ns)
+        if self.want_trade_ee:
+            res["trade_ee"] = np.zeros((nt, nd))
+            if want_cva:
+                res["trade_cva"] = np.zeros(nt)
+        if want_path:
+            res["path_mtm"] = np.zeros((nd, nt, self.n_paths))
+        if want_set_mtm:
+            res["set_mtm"] = np.zeros((nd, ns, self.n_paths))
+        out = OutputsC(*[_dp(res.get(k)) for k in ("set_ee", "set_pfe", "set_cva", "trade_ee", "trade_cva",
+                                                    "path_mtm", "set_mtm")])
+        z = None
+        if self.rng == RNG_HOST_DRAWS:
+            z = _f64(host_draws)
+            if z.shape != (self.n_steps, self.n_paths):
+                raise ValueError(f"host_draws must be [n_steps={self.n_steps}][n_paths={self.n_paths}]")
+        cr = CreditC(hazard_rate, recovery_rate)
+        _check(lib().eqhip_plan_execute(self.handle, _dp(z), ctypes.byref(cr), ctypes.byref(out)))
+        return res
+
+    def timing(self):
+        t = TimingC()
+        _check(lib().eqhip_plan_timing(self.handle, ctypes.byref(t)))
+        return {f: getattr(t, f) for f, _ in TimingC._fields_}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().eqhip_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """One HIP stream on one gfx950 device (eqhip_context)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _check(lib().eqhip_context_create(device, ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().eqhip_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # reference-shaped entry points -------------------------------------------------------
+    def exposure_profile(self, a, b, sigma, r0, notional, fixed_rate, use_par_rate, start, pay, acc,
+                         times, n_steps, n_paths, seed, rng=RNG_PHILOX, disc=DISC_EULER, host_draws=None,
+                         allow_any_date=False):
+        pay, acc, times = _f64(pay), _f64(acc), _f64(times)
+        nd = len(times)
+        ot, oe, op = np.zeros(nd), np.zeros(nd), np.zeros(nd)
+        n_out = ctypes.c_int32()
+        z = _f64(host_draws) if host_draws is not None else None
+        _check(lib().eqhip_irs_hull_white_exposure_profile(
+            self.handle, a, b, sigma, r0, notional, fixed_rate, int(bool(use_par_rate)), start, _dp(pay),
+            _dp(acc), len(pay), _dp(times), nd, n_steps, n_paths, seed, rng, disc, _dp(z),
+            int(allow_any_date), _dp(ot), _dp(oe), _dp(op), ctypes.byref(n_out)))
+        n = n_out.value
+        return ot[:n], oe[:n], op[:n]
+
+    def exposure_profile_batch(self, a, b, sigma, r0, notionals, fixed_rates, start, pay, acc, times,
+                               n_steps, n_paths, seed, rng=RNG_PHILOX, disc=DISC_EULER, host_draws=None,
+                               allow_any_date=False):
+        notionals, fixed_rates = _f64(notionals), _f64(fixed_rates)
+        pay, acc, times = _f64(pay), _f64(acc), _f64(times)
+        nt, nd = len(notionals), len(times)
+        ot, oe, op = np.zeros(nd), np.zeros((nt, nd)), np.zeros((nt, nd))
+        n_out = ctypes.c_int32()
+        z = _f64(host_draws) if host_draws is not None else None
+        _check(lib().eqhip_irs_hull_white_exposure_profile_batch(
+            self.handle, a, b, sigma, r0, _dp(notionals), _dp(fixed_rates), nt, start, _dp(pay), _dp(acc),
+            len(pay), _dp(times), nd, n_steps, n_paths, seed, rng, disc, _dp(z), int(allow_any_date), _dp(ot),
+            _dp(oe), _dp(op), ctypes.byref(n_out)))
+        n = n_out.value
+        return ot[:n], oe[:, :n], op[:, :n]
+
+    def unilateral_cva(self, a, b, sigma, r0, times, ee, hazard_rate, recovery_rate):
+        times, ee = _f64(times), _f64(ee)
+        out = ctypes.c_double()
+        _check(lib().eqhip_unilateral_cva_from_exposure(self.handle, a, b, sigma, r0, _dp(times), _dp(ee),
+                                                        len(times), hazard_rate, recovery_rate,
+                                                        ctypes.byref(out)))
+        return out.value
+
+    def unilateral_cva_batch(self, a, b, sigma, r0, times, ee, hazard_rate, recovery_rate):
+        times, ee = _f64(times), np.atleast_2d(_f64(ee))
+        out = np.zeros(ee.shape[0])
+        _check(lib().eqhip_unilateral_cva_from_exposure_batch(self.handle, a, b, sigma, r0, _dp(times),
+                                                              _dp(ee), ee.shape[0], len(times), hazard_rate,
+                                                              recovery_rate, _dp(out)))
+        return out
+
+    def pv_dv01(self, pillars, zero_rates, portfolio: Portfolio, bump=1e-4, bucketed=True):
+        pillars, zero_rates = _f64(pillars), _f64(zero_rates)
+        pc = portfolio._c()
+        nt = portfolio.n_trades
+        pv, dv = np.zeros(nt), np.zeros(nt)
+        bk = np.zeros((nt, len(pillars))) if bucketed else None
+        _check(lib().eqhip_pv_dv01(self.handle, _dp(pillars), _dp(zero_rates), len(pillars), ctypes.byref(pc),
+                                   bump, _dp(pv), _dp(dv), _dp(bk)))
+        return pv, dv, bk
+
+    # portfolio --------------------------------------------------------------------------------
+    def plan(self, model, portfolio, dates, n_paths, n_steps, **kw):
+        return Plan(self, model, portfolio, dates, n_paths, n_steps, **kw)
+
+    def exposure_run(self, model, portfolio, dates, n_paths, n_steps, seed=0, rng=RNG_PHILOX,
+                     disc=DISC_EULER, host_draws=None, hazard_rate=0.0, recovery_rate=0.0,
+                     want_trade_ee=False, want_path=False, want_set_mtm=False):
+        plan = Plan(self, model, portfolio, dates, n_paths, n_steps, seed=seed, rng=rng, disc=disc,
+                    want_trade_ee=want_trade_ee or want_path)
+        try:
+            return plan.execute(host_draws, hazard_rate, recovery_rate, True, want_path, want_set_mtm)
+        finally:
+            plan.close()
+
+
+def singleton_portfolio(notionals, fixed_rates, start, pay, acc):
+    """Homogeneous batch with one netting set per trade (the reference's per-trade profiles)."""
+    n = len(notionals)
+    m = len(pay)
+    return Portfolio(set_id=np.arange(n, dtype=np.int32), notional=_f64(notionals), fixed_rate=_f64(fixed_rates),
+                     start=np.full(n, float(start)), flow_off=np.arange(0, (n + 1) * m, m, dtype=np.int32),
+                     pay=np.tile(_f64(pay), n), acc=np.tile(_f64(acc), n))

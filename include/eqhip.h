@@ -1,0 +1,192 @@
+/*
+ * eqhip — C ABI of the MI355X (gfx950) Monte Carlo exposure library.
+ *
+ * This is the drop-in boundary under engine_quant's registry-driven pricing API: it replaces
+ * the cxx bridge functions the C++ measures call (rust/crates/engine-ffi/src/lib.rs:326-354,
+ * 412, 511-541) in their GPU arm (rust/crates/engine-core/src/api.rs:151-159, 229-236,
+ * 289-296, 342-349). Plain pointers and sizes only; every entry point returns 0 on success and
+ * a non-zero EQHIP_E_* code on failure, with a message readable through eqhip_last_error()
+ * (thread-local, like g_last_error in cpp/engine/src/abi.cpp:43-45). No exception crosses it.
+ *
+ * All arithmetic is FP64. Outputs are caller-allocated host buffers unless stated otherwise.
+ * A context owns one HIP stream on one device; use one context per host thread.
+ */
+#ifndef EQHIP_H
+#define EQHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EQHIP_OK 0
+#define EQHIP_E_INVALID 1   /* invalid argument (the reference's std::invalid_argument / assert!) */
+#define EQHIP_E_NO_DEVICE 2 /* backend "gpu" requested with no usable device: never a CPU run */
+#define EQHIP_E_HIP 3       /* HIP runtime error */
+#define EQHIP_E_CAPACITY 4  /* problem exceeds a documented capacity limit of this build */
+
+/* Random-number source of the short-rate shocks. */
+#define EQHIP_RNG_HOST_DRAWS 0 /* caller supplies Z[step][path] (shared-draw parity mode) */
+#define EQHIP_RNG_PHILOX 1     /* in-register Philox4x32-10 + Box-Muller, key = seed */
+
+/* Discretisation of dr = a(b - r)dt + sigma dW. */
+#define EQHIP_DISC_EULER 0 /* Euler-Maruyama, as kernel.rs:41-50 (reference semantics) */
+#define EQHIP_DISC_EXACT 1 /* exact Gaussian transition of the Vasicek/HW1F short rate */
+
+typedef struct eqhip_context eqhip_context;
+typedef struct eqhip_plan eqhip_plan;
+
+/* HullWhite1F model parameters, cpp/engine/src/model.cpp:39-43. */
+typedef struct {
+    double a, b, sigma, r0;
+} eqhip_model_hw1f;
+
+/* Portfolio of vanilla IRS in structure-of-arrays form. Trade j belongs to netting set
+ * set_id[j]; its schedule is payment_times/accruals[flow_offset[j] .. flow_offset[j+1]) with
+ * floating-leg start start[j]. Payer swaps have notional > 0, receivers notional < 0 (the
+ * reference has no direction flag, cpp/engine/src/product.cpp:7-18). */
+typedef struct {
+    int32_t n_trades;
+    int32_t n_sets;
+    const int32_t* set_id;
+    const double* notional;
+    const double* fixed_rate;
+    const double* start;
+    const int32_t* flow_offset;
+    const double* payment_times;
+    const double* accruals;
+} eqhip_portfolio;
+
+/* Monte Carlo settings (PricingContext n_paths/n_steps/seed, cpp/engine/src/pricing_context.cpp
+ * :7-18, plus the execution keys rng and discretisation). */
+typedef struct {
+    uint64_t n_paths;
+    uint32_t n_steps;
+    uint64_t seed;
+    int32_t rng;
+    int32_t disc;
+} eqhip_mc;
+
+/* Flat hazard / recovery (MarketSnapshot credit data, cpp/engine/include/engine/market.hpp). */
+typedef struct {
+    double hazard_rate, recovery_rate;
+} eqhip_credit;
+
+/* Results of one exposure run. set_ee / set_pfe are required; every other pointer may be NULL.
+ * Layouts are row-major: [n_sets][n_dates], [n_trades][n_dates], [n_dates][n_trades][n_paths]. */
+typedef struct {
+    double* set_ee;
+    double* set_pfe;
+    double* set_cva;
+    double* trade_ee;
+    double* trade_cva;
+    double* path_mtm; /* per-path, per-trade mark-to-market before max(.,0) (small runs only) */
+    double* set_mtm;  /* per-path netting-set value before max(.,0) (small runs only) */
+} eqhip_outputs;
+
+/* Per-kernel device times (milliseconds, HIP events on the context's stream) of the last
+ * eqhip_plan_execute on that plan. */
+typedef struct {
+    double simulate_ms;
+    double value_ms;
+    double reduce_ms; /* EE sums + PFE radix select + per-trade EE finalisation */
+    double cva_ms;
+    double total_ms;
+    uint64_t n_value_blocks;
+    uint32_t paths_per_tile;
+    uint32_t tiles_per_block;
+} eqhip_timing;
+
+/* ---- library / context -------------------------------------------------------------------- */
+
+/* Replaces engine::ffi::is_gpu_backend_available (engine-ffi/src/lib.rs:412, api.rs:362-364):
+ * here it is a runtime probe (a gfx950 device is visible), not a compile-time flag. */
+int eqhip_is_gpu_backend_available(void);
+
+/* Copies the calling thread's last error message into buf (always NUL-terminated). Returns the
+ * full message length. */
+size_t eqhip_last_error(char* buf, size_t len);
+
+int eqhip_context_create(int device, eqhip_context** out);
+int eqhip_context_destroy(eqhip_context* ctx);
+
+/* ---- reference-shaped entry points -------------------------------------------------------- */
+
+/* Replaces engine::ffi::irs_hull_white_exposure_profile (engine-ffi/src/lib.rs:326-342;
+ * api.rs:126-173): EE / PFE95 of one payer IRS at its monitoring dates. use_par_rate != 0
+ * sets the fixed rate to the model par rate at r0 (api.rs:53-57). host_draws ([n_steps][n_paths])
+ * is read only when rng == EQHIP_RNG_HOST_DRAWS. out_times/out_ee/out_pfe hold n_dates values;
+ * *n_out receives the number written (1 when every monitoring date is <= 0, exposure.rs:98-108).
+ * Monitoring dates must be reset dates of the swap (exposure.rs:85-90) unless allow_any_date. */
+int eqhip_irs_hull_white_exposure_profile(
+    eqhip_context* ctx, double a, double b, double sigma, double r0, double notional,
+    double fixed_rate, int use_par_rate, double start, const double* payment_times,
+    const double* accruals, int32_t n_flows, const double* monitoring_times, int32_t n_dates,
+    uint32_t n_steps, uint64_t n_paths, uint64_t seed, int32_t rng, int32_t disc,
+    const double* host_draws, int32_t allow_any_date, double* out_times, double* out_ee,
+    double* out_pfe, int32_t* n_out);
+
+/* Replaces engine::ffi::irs_hull_white_exposure_profile_batch (engine-ffi/src/lib.rs:511-527;
+ * api.rs:204-247): homogeneous batch (shared schedule), one simulation, one profile per trade.
+ * out_ee / out_pfe are [n_trades][n_dates]. */
+int eqhip_irs_hull_white_exposure_profile_batch(
+    eqhip_context* ctx, double a, double b, double sigma, double r0, const double* notionals,
+    const double* fixed_rates, int32_t n_trades, double start, const double* payment_times,
+    const double* accruals, int32_t n_flows, const double* monitoring_times, int32_t n_dates,
+    uint32_t n_steps, uint64_t n_paths, uint64_t seed, int32_t rng, int32_t disc,
+    const double* host_draws, int32_t allow_any_date, double* out_times, double* out_ee,
+    double* out_pfe, int32_t* n_out);
+
+/* Replaces engine::ffi::unilateral_cva_from_exposure (engine-ffi/src/lib.rs:344-354;
+ * api.rs:273-304; exposure.rs:242-275). */
+int eqhip_unilateral_cva_from_exposure(eqhip_context* ctx, double a, double b, double sigma,
+                                       double r0, const double* times, const double* ee,
+                                       int32_t n_dates, double hazard_rate, double recovery_rate,
+                                       double* out_cva);
+
+/* Replaces engine::ffi::unilateral_cva_from_exposure_batch (engine-ffi/src/lib.rs:532-541;
+ * api.rs:327-357): ee is [n_profiles][n_dates] over shared times. */
+int eqhip_unilateral_cva_from_exposure_batch(eqhip_context* ctx, double a, double b,
+                                             double sigma, double r0, const double* times,
+                                             const double* ee, int32_t n_profiles,
+                                             int32_t n_dates, double hazard_rate,
+                                             double recovery_rate, double* out_cva);
+
+/* PV and DV01 of every trade from the market zero curve, cpp/engine/src/measure.cpp:124-222
+ * (npv_from_market, compute_dv01, dv01_bucketed_from_market) with market.cpp:26-41,95-111.
+ * fixed_rate in the portfolio is the effective rate (par under the market for par swaps).
+ * out_bucketed ([n_trades][n_pillars]) may be NULL. */
+int eqhip_pv_dv01(eqhip_context* ctx, const double* pillars, const double* zero_rates,
+                  int32_t n_pillars, const eqhip_portfolio* portfolio, double bump, double* out_pv,
+                  double* out_dv01, double* out_bucketed);
+
+/* ---- portfolio / netting-set exposure ----------------------------------------------------- */
+
+/* Builds the device-resident plan of one exposure job: schedule classes, cash-flow date table,
+ * ZCB factor tables, work buffers. Monitoring dates are arbitrary (between-reset valuation
+ * carries the last floating fixing on each path). */
+int eqhip_plan_create(eqhip_context* ctx, const eqhip_model_hw1f* model,
+                      const eqhip_portfolio* portfolio, const double* dates, int32_t n_dates,
+                      const eqhip_mc* mc, int32_t want_trade_ee, eqhip_plan** out);
+
+/* Runs the job on resident data and copies the results into out. host_draws ([n_steps][n_paths])
+ * is read only in EQHIP_RNG_HOST_DRAWS mode. */
+int eqhip_plan_execute(eqhip_plan* plan, const double* host_draws, const eqhip_credit* credit,
+                       eqhip_outputs* out);
+
+int eqhip_plan_timing(const eqhip_plan* plan, eqhip_timing* out);
+int eqhip_plan_destroy(eqhip_plan* plan);
+
+/* One-shot convenience: plan_create + plan_execute + plan_destroy. */
+int eqhip_exposure_run(eqhip_context* ctx, const eqhip_model_hw1f* model,
+                       const eqhip_portfolio* portfolio, const double* dates, int32_t n_dates,
+                       const eqhip_mc* mc, const double* host_draws, const eqhip_credit* credit,
+                       eqhip_outputs* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,280 @@
+"""Python view of the oracle ``libeqref.so`` (CPU FP64 restatement of the reference path).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` as the checker / timed CPU baseline. The product package
+``engine_quant_amd`` never imports it.
+
+Every wrapper documents the reference file:line its C function restates (see ``eq_ref.c``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libeqref.so")
+_lib = None
+
+_D = ctypes.c_double
+_I = ctypes.c_int
+_U32 = ctypes.c_uint32
+_U64 = ctypes.c_uint64
+_PD = ctypes.POINTER(ctypes.c_double)
+_PI32 = ctypes.POINTER(ctypes.c_int32)
+_PU32 = ctypes.POINTER(ctypes.c_uint32)
+
+
+def build() -> str:
+    """Compile the oracle with its committed Makefile (gcc, no GPU needed)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.eqr_hw_b.restype = _D
+        L.eqr_hw_b.argtypes = [_D, _D]
+        L.eqr_hw_a.restype = _D
+        L.eqr_hw_a.argtypes = [_D, _D, _D, _D, _D]
+        L.eqr_zcb.restype = _D
+        L.eqr_zcb.argtypes = [_D] * 6
+        L.eqr_philox4x32_10.argtypes = [_PU32, _PU32, _PU32]
+        L.eqr_philox_normals.argtypes = [_U64, _U64, _U32, _PD]
+        L.eqr_philox_uniforms.argtypes = [_U64, _U64, _U32, _PD, _PD]
+        L.eqr_simulate.argtypes = [_D, _D, _D, _D, _D, _U32, _U64, _I, _PD, _PD]
+        L.eqr_is_reset_date.restype = _I
+        L.eqr_is_reset_date.argtypes = [_D, _PD, _I, _D]
+        L.eqr_par_rate.restype = _D
+        L.eqr_par_rate.argtypes = [_D, _D, _D, _D, _D, _PD, _PD, _I]
+        L.eqr_irs_npv_reset.restype = _D
+        L.eqr_irs_npv_reset.argtypes = [_D] * 8 + [_PD, _PD, _I]
+        L.eqr_irs_value.restype = _D
+        L.eqr_irs_value.argtypes = [_D] * 9 + [_PD, _PD, _I]
+        L.eqr_fixing_date.restype = _D
+        L.eqr_fixing_date.argtypes = [_D, _D, _PD, _I]
+        L.eqr_ee_pfe.argtypes = [_PD, _U64, _PD, _PD]
+        L.eqr_unilateral_cva.restype = _D
+        L.eqr_unilateral_cva.argtypes = [_D, _D, _D, _D, _PD, _PD, _I, _D, _D]
+        L.eqr_exposure_profile.restype = _I
+        L.eqr_exposure_profile.argtypes = [_D, _D, _D, _D, _D, _D, _I, _D, _PD, _PD, _I, _PD, _I,
+                                           _U32, _U64, _PD, _PD, _PD]
+        L.eqr_curve_zero_rate.restype = _D
+        L.eqr_curve_zero_rate.argtypes = [_PD, _PD, _I, _D]
+        L.eqr_curve_df.restype = _D
+        L.eqr_curve_df.argtypes = [_PD, _PD, _I, _D]
+        L.eqr_par_rate_from_market.restype = _D
+        L.eqr_par_rate_from_market.argtypes = [_PD, _PD, _I, _D, _PD, _PD, _I]
+        L.eqr_npv_from_market.restype = _D
+        L.eqr_npv_from_market.argtypes = [_PD, _PD, _I, _D, _D, _D, _PD, _PD, _I]
+        L.eqr_dv01.restype = _D
+        L.eqr_dv01.argtypes = [_PD, _PD, _I, _D, _D, _D, _PD, _PD, _I, _D]
+        L.eqr_dv01_bucketed.argtypes = [_PD, _PD, _I, _D, _D, _D, _PD, _PD, _I, _D, _PD]
+        L.eqr_portfolio_exposure.restype = _I
+        L.eqr_portfolio_exposure.argtypes = [
+            _D, _D, _D, _D, _I, _I, _PI32, _PD, _PD, _PD, _PI32, _PD, _PD, _PD, _I, _U32, _U64, _I,
+            _PD, _D, _D, _PD, _PD, _PD, _PD, _PD, _PD, _PD, _PD, _I]
+        L.eqr_max_threads.restype = _I
+        _lib = L
+    return _lib
+
+
+def _dp(x):
+    if x is None:
+        return None
+    return x.ctypes.data_as(_PD)
+
+
+def _f64(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+# --------------------------------------------------------------------------- model
+def hw_b(a, tau):
+    """B(tau), models/hull_white.rs:51-55."""
+    return lib().eqr_hw_b(a, tau)
+
+
+def hw_a(a, b, sigma, tau):
+    """A(tau), models/hull_white.rs:59-69."""
+    return lib().eqr_hw_a(a, b, sigma, tau, hw_b(a, tau))
+
+
+def zcb(a, b, sigma, r, t, maturity):
+    """P(t,T|r), models/hull_white.rs:75-80."""
+    return lib().eqr_zcb(a, b, sigma, r, t, maturity)
+
+
+# --------------------------------------------------------------------------- RNG
+def philox4x32_10(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*[int(v) & 0xFFFFFFFF for v in ctr])
+    k = (ctypes.c_uint32 * 2)(*[int(v) & 0xFFFFFFFF for v in key])
+    o = (ctypes.c_uint32 * 4)()
+    lib().eqr_philox4x32_10(c, k, o)
+    return [int(v) for v in o]
+
+
+def philox_uniforms(seed, path, pair):
+    u1, u2 = _D(), _D()
+    lib().eqr_philox_uniforms(seed, path, pair, ctypes.byref(u1), ctypes.byref(u2))
+    return u1.value, u2.value
+
+
+def philox_normals(seed, n_paths, n_steps):
+    """Normals Z[step][path] of the counter-based stream shared with the GPU."""
+    z = np.empty((n_steps, n_paths), dtype=np.float64)
+    lib().eqr_philox_normals(seed, n_paths, n_steps, _dp(z))
+    return z
+
+
+# --------------------------------------------------------------------------- simulation
+def simulate(a, b, sigma, r0, dt, z, scheme=0):
+    """Short-rate paths r[step][path] (kernel.rs:41-50, hull_white.rs:87-97; scheme 1 = exact)."""
+    z = _f64(z)
+    n_steps, n_paths = z.shape
+    r = np.empty((n_steps + 1, n_paths), dtype=np.float64)
+    lib().eqr_simulate(a, b, sigma, r0, dt, n_steps, n_paths, scheme, _dp(z), _dp(r))
+    return r
+
+
+# --------------------------------------------------------------------------- IRS
+def is_reset_date(start, pay, t):
+    p = _f64(pay)
+    return bool(lib().eqr_is_reset_date(start, _dp(p), len(p), t))
+
+
+def par_rate(a, b, sigma, r, start, pay, acc):
+    """irs.rs:126-145."""
+    p, q = _f64(pay), _f64(acc)
+    return lib().eqr_par_rate(a, b, sigma, r, start, _dp(p), _dp(q), len(p))
+
+
+def irs_npv_reset(a, b, sigma, r, t, notional, fixed_rate, start, pay, acc):
+    """remaining_from + npv, irs.rs:46-68,163-187."""
+    p, q = _f64(pay), _f64(acc)
+    return lib().eqr_irs_npv_reset(a, b, sigma, r, t, notional, fixed_rate, start, _dp(p), _dp(q),
+                                   len(p))
+
+
+def irs_value(a, b, sigma, r, t, r_fix, notional, fixed_rate, start, pay, acc):
+    p, q = _f64(pay), _f64(acc)
+    return lib().eqr_irs_value(a, b, sigma, r, t, r_fix, notional, fixed_rate, start, _dp(p),
+                               _dp(q), len(p))
+
+
+def fixing_date(t, start, pay):
+    p = _f64(pay)
+    return lib().eqr_fixing_date(t, start, _dp(p), len(p))
+
+
+def ee_pfe(x):
+    """exposure.rs:42-49."""
+    x = _f64(x).copy()
+    e, q = _D(), _D()
+    lib().eqr_ee_pfe(_dp(x), len(x), ctypes.byref(e), ctypes.byref(q))
+    return e.value, q.value
+
+
+def unilateral_cva(a, b, sigma, r0, times, ee, hazard_rate, recovery_rate):
+    """exposure.rs:242-275."""
+    t, e = _f64(times), _f64(ee)
+    return lib().eqr_unilateral_cva(a, b, sigma, r0, _dp(t), _dp(e), len(t), hazard_rate,
+                                    recovery_rate)
+
+
+def exposure_profile(a, b, sigma, r0, notional, fixed_rate, use_par_rate, start, pay, acc,
+                     times, n_steps, z):
+    """exposure.rs:73-138 on explicit draws z[step][path]; returns (times, ee, pfe)."""
+    p, q, t = _f64(pay), _f64(acc), _f64(times)
+    z = _f64(z)
+    n_paths = z.shape[1]
+    ee = np.zeros(len(t))
+    pfe = np.zeros(len(t))
+    n = lib().eqr_exposure_profile(a, b, sigma, r0, notional, fixed_rate, int(use_par_rate), start,
+                                   _dp(p), _dp(q), len(p), _dp(t), len(t), n_steps, n_paths,
+                                   _dp(z), _dp(ee), _dp(pfe))
+    return t[:n].copy(), ee[:n], pfe[:n]
+
+
+# --------------------------------------------------------------------------- market curve
+def curve_zero_rate(pillars, zeros, t):
+    p, z = _f64(pillars), _f64(zeros)
+    return lib().eqr_curve_zero_rate(_dp(p), _dp(z), len(p), t)
+
+
+def curve_df(pillars, zeros, t):
+    p, z = _f64(pillars), _f64(zeros)
+    return lib().eqr_curve_df(_dp(p), _dp(z), len(p), t)
+
+
+def par_rate_from_market(pillars, zeros, start, pay, acc):
+    p, z, a, c = _f64(pillars), _f64(zeros), _f64(pay), _f64(acc)
+    return lib().eqr_par_rate_from_market(_dp(p), _dp(z), len(p), start, _dp(a), _dp(c), len(a))
+
+
+def npv_from_market(pillars, zeros, notional, fixed_rate, start, pay, acc):
+    p, z, a, c = _f64(pillars), _f64(zeros), _f64(pay), _f64(acc)
+    return lib().eqr_npv_from_market(_dp(p), _dp(z), len(p), notional, fixed_rate, start, _dp(a),
+                                     _dp(c), len(a))
+
+
+def dv01(pillars, zeros, notional, fixed_rate, start, pay, acc, bump=1e-4):
+    p, z, a, c = _f64(pillars), _f64(zeros), _f64(pay), _f64(acc)
+    return lib().eqr_dv01(_dp(p), _dp(z), len(p), notional, fixed_rate, start, _dp(a), _dp(c),
+                          len(a), bump)
+
+
+def dv01_bucketed(pillars, zeros, notional, fixed_rate, start, pay, acc, bump=1e-4):
+    p, z, a, c = _f64(pillars), _f64(zeros), _f64(pay), _f64(acc)
+    out = np.zeros(len(p))
+    lib().eqr_dv01_bucketed(_dp(p), _dp(z), len(p), notional, fixed_rate, start, _dp(a), _dp(c),
+                            len(a), bump, _dp(out))
+    return out
+
+
+# --------------------------------------------------------------------------- portfolio
+def portfolio_exposure(model, port, dates, n_steps, z, scheme=0, hazard_rate=0.0,
+                       recovery_rate=0.0, want_trade=False, want_path=False, n_threads=0):
+    """Per-trade restatement of the netting-set exposure run.
+
+    ``model`` = (a, b, sigma, r0); ``port`` = dict with set_id, notional, fixed_rate, start,
+    flow_off, pay, acc (numpy arrays); ``z`` = normals [n_steps][n_paths].
+    """
+    a, b, sigma, r0 = model
+    set_id = np.ascontiguousarray(port["set_id"], dtype=np.int32)
+    flow_off = np.ascontiguousarray(port["flow_off"], dtype=np.int32)
+    notional, fixed, start = _f64(port["notional"]), _f64(port["fixed_rate"]), _f64(port["start"])
+    pay, acc = _f64(port["pay"]), _f64(port["acc"])
+    n_trades = len(notional)
+    n_sets = int(set_id.max()) + 1 if n_trades else 0
+    dates = _f64(dates)
+    z = _f64(z)
+    n_paths = z.shape[1]
+    nd = len(dates)
+    out = {
+        "set_ee": np.zeros((n_sets, nd)), "set_pfe": np.zeros((n_sets, nd)),
+        "set_cva": np.zeros(n_sets),
+    }
+    if want_trade:
+        out.update(trade_ee=np.zeros((n_trades, nd)), trade_pfe=np.zeros((n_trades, nd)),
+                   trade_cva=np.zeros(n_trades))
+    if want_path:
+        out.update(path_mtm=np.zeros((nd, n_trades, n_paths)), set_mtm=np.zeros((nd, n_sets, n_paths)))
+    lib().eqr_portfolio_exposure(
+        a, b, sigma, r0, n_trades, n_sets, set_id.ctypes.data_as(_PI32), _dp(notional), _dp(fixed),
+        _dp(start), flow_off.ctypes.data_as(_PI32), _dp(pay), _dp(acc), _dp(dates), nd, n_steps,
+        n_paths, scheme, _dp(z), hazard_rate, recovery_rate, _dp(out["set_ee"]),
+        _dp(out["set_pfe"]), _dp(out["set_cva"]), _dp(out.get("trade_ee")),
+        _dp(out.get("trade_pfe")), _dp(out.get("trade_cva")), _dp(out.get("path_mtm")),
+        _dp(out.get("set_mtm")), n_threads)
+    return out
+
+
+def max_threads():
+    return lib().eqr_max_threads()

@@ -1,0 +1,520 @@
+/*
+ * eq_ref — CPU FP64 restatement of engine_quant's Monte Carlo exposure path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the oracle the HIP path is checked against, and the CPU baseline
+ * timed beside it. The product never links, loads or calls this file's library.
+ *
+ * Compiled with -ffp-contract=off so that every product and sum is rounded exactly as the
+ * reference's Rust/Burn and C++ code rounds it (neither contracts a*b+c into an FMA).
+ * File:line citations are relative to the reference repository root.
+ */
+#include "eq_ref.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define EQR_TOL 1e-9 /* date-matching tolerance of products/irs.rs:151-154,173 */
+
+/* ------------------------------------------------------------------------------------------ */
+/* Hull-White 1F (Vasicek form)                                                               */
+/* ------------------------------------------------------------------------------------------ */
+
+/* B(tau) = (1 - exp(-a tau)) / a, built as Burn does it: (-(exp(a*(-tau)))) + 1, then / a.
+ * models/hull_white.rs:51-55 */
+double eqr_hw_b(double a, double tau) {
+    double neg_a_tau = a * (-tau);
+    double one_minus_exp = (-exp(neg_a_tau)) + 1.0;
+    return one_minus_exp / a;
+}
+
+/* A(tau) = exp(((B - tau)(a^2 b - sigma^2/2)) / a^2 - (sigma^2 B B) / (4a)).
+ * models/hull_white.rs:59-69 */
+double eqr_hw_a(double a, double b, double sigma, double tau, double bf) {
+    double a2 = a * a;
+    double sigma2 = sigma * sigma;
+    double term1 = ((bf + (-tau)) * (a2 * b - sigma2 * 0.5)) / a2;
+    double term2 = ((sigma2 * bf) * bf) / (a * 4.0);
+    return exp(term1 - term2);
+}
+
+/* P(t,T) = A(T-t) * exp(-(B(T-t) * r)). models/hull_white.rs:75-80 */
+double eqr_zcb(double a, double b, double sigma, double r, double t, double maturity) {
+    double tau = maturity - t;
+    double bf = eqr_hw_b(a, tau);
+    double af = eqr_hw_a(a, b, sigma, tau, bf);
+    return af * exp(-(bf * r));
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Philox4x32-10 + Box-Muller                                                                 */
+/* (no reference function: the reference's exposure path draws from burn-ndarray's global RNG,  */
+/*  exposure.rs:113-116; this counter-based stream replaces it in both the oracle and the GPU) */
+/* ------------------------------------------------------------------------------------------ */
+
+void eqr_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int round = 0; round < 10; ++round) {
+        if (round > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+void eqr_philox_uniforms(uint64_t seed, uint64_t path, uint32_t pair, double* u1, double* u2) {
+    uint32_t ctr[4] = {pair, (uint32_t)path, (uint32_t)(path >> 32), 0u};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t out[4];
+    eqr_philox4x32_10(ctr, key, out);
+    uint64_t wa = ((uint64_t)out[1] << 32) | out[0];
+    uint64_t wb = ((uint64_t)out[3] << 32) | out[2];
+    const double two_m53 = 1.0 / 9007199254740992.0;
+    *u1 = (double)((wa >> 11) + 1u) * two_m53; /* (0, 1] */
+    *u2 = (double)(wb >> 11) * two_m53;        /* [0, 1) */
+}
+
+void eqr_philox_normals(uint64_t seed, uint64_t n_paths, uint32_t n_steps, double* z) {
+    const double two_pi = 6.283185307179586476925286766559;
+    for (uint64_t p = 0; p < n_paths; ++p) {
+        for (uint32_t s = 0; s < n_steps; s += 2) {
+            double u1, u2;
+            eqr_philox_uniforms(seed, p, s / 2u, &u1, &u2);
+            double rad = sqrt(-2.0 * log(u1));
+            double th = two_pi * u2;
+            z[(uint64_t)s * n_paths + p] = rad * cos(th);
+            if (s + 1 < n_steps) z[(uint64_t)(s + 1) * n_paths + p] = rad * sin(th);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Simulation                                                                                 */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Euler: r' = (r + (a*(b - r))*dt) + sigma*(z*sqrt(dt)), kernel.rs:41-50, hull_white.rs:91-95.
+ * Exact: r' = (r*e + b*(1-e)) + sd*z with e = exp(-a dt), sd = sigma*sqrt((1-exp(-2a dt))/(2a)). */
+static void simulate_one(double a, double b, double sigma, double r0, double dt,
+                         uint32_t n_steps, uint64_t n_paths, uint64_t p, int scheme,
+                         const double* z, double* r_path /* [n_steps+1], stride 1 */) {
+    double r = r0;
+    r_path[0] = r;
+    if (scheme == 0) {
+        double sq = sqrt(dt);
+        for (uint32_t s = 0; s < n_steps; ++s) {
+            double drift = a * (b - r);
+            double dw = z[(uint64_t)s * n_paths + p] * sq;
+            r = (r + drift * dt) + sigma * dw;
+            r_path[s + 1] = r;
+        }
+    } else {
+        double e = exp(-(a * dt));
+        double mean_c = b * (1.0 - e);
+        double sd = sigma * sqrt((1.0 - exp(-(2.0 * a * dt))) / (2.0 * a));
+        for (uint32_t s = 0; s < n_steps; ++s) {
+            r = (r * e + mean_c) + sd * z[(uint64_t)s * n_paths + p];
+            r_path[s + 1] = r;
+        }
+    }
+}
+
+void eqr_simulate(double a, double b, double sigma, double r0, double dt, uint32_t n_steps,
+                  uint64_t n_paths, int scheme, const double* z, double* r_out) {
+    double* tmp = (double*)malloc(sizeof(double) * (n_steps + 1));
+    for (uint64_t p = 0; p < n_paths; ++p) {
+        simulate_one(a, b, sigma, r0, dt, n_steps, n_paths, p, scheme, z, tmp);
+        for (uint32_t s = 0; s <= n_steps; ++s) r_out[(uint64_t)s * n_paths + p] = tmp[s];
+    }
+    free(tmp);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* IRS                                                                                        */
+/* ------------------------------------------------------------------------------------------ */
+
+/* products/irs.rs:150-155 */
+int eqr_is_reset_date(double start, const double* pay, int m, double t) {
+    if (fabs(t - start) < EQR_TOL) return 1;
+    for (int i = 0; i + 1 < m; ++i)
+        if (fabs(pay[i] - t) < EQR_TOL) return 1;
+    return 0;
+}
+
+/* products/irs.rs:126-145: (P(s,s) - P(s,T_n)) / sum_i P(s,T_i)*tau_i, reduce in order. */
+double eqr_par_rate(double a, double b, double sigma, double r, double start,
+                    const double* pay, const double* acc, int m) {
+    double p_start = eqr_zcb(a, b, sigma, r, start, start);
+    double p_end = eqr_zcb(a, b, sigma, r, start, pay[m - 1]);
+    double numerator = p_start - p_end;
+    double den = 0.0;
+    for (int i = 0; i < m; ++i) {
+        double leg = eqr_zcb(a, b, sigma, r, start, pay[i]) * acc[i];
+        den = (i == 0) ? leg : den + leg;
+    }
+    return numerator / den;
+}
+
+/* remaining_from(t) (irs.rs:163-187) followed by npv (irs.rs:46-68):
+ * N*(P(t,t) - P(t,T_n)) - sum_{period start >= t-1e-9} (N*K)*(P(t,T_i)*tau_i). */
+double eqr_irs_npv_reset(double a, double b, double sigma, double r, double t, double notional,
+                         double fixed_rate, double start, const double* pay, const double* acc,
+                         int m) {
+    /* remaining_from keeps the periods whose start >= t - 1e-9 and sets start := t */
+    int first = -1;
+    double period_start = start;
+    for (int i = 0; i < m; ++i) {
+        if (period_start >= t - EQR_TOL) {
+            first = i;
+            break;
+        }
+        period_start = pay[i];
+    }
+    if (first < 0) return 0.0; /* no period left: the reference would reduce an empty leg */
+    double p_start = eqr_zcb(a, b, sigma, r, t, t);
+    double p_end = eqr_zcb(a, b, sigma, r, t, pay[m - 1]);
+    double floating_leg = notional * (p_start - p_end);
+    double fixed_leg = 0.0;
+    for (int i = first; i < m; ++i) {
+        double p_i = eqr_zcb(a, b, sigma, r, t, pay[i]);
+        double leg = (notional * fixed_rate) * (p_i * acc[i]);
+        fixed_leg = (i == first) ? leg : fixed_leg + leg;
+    }
+    return floating_leg - fixed_leg;
+}
+
+double eqr_fixing_date(double t, double start, const double* pay, int m) {
+    if (t <= start + EQR_TOL) return -1.0;       /* not started: no fixing yet */
+    if (t >= pay[m - 1] - EQR_TOL) return -1.0;  /* matured */
+    double prev = start;
+    for (int i = 0; i < m; ++i) {
+        if (pay[i] > t + EQR_TOL) {
+            if (fabs(t - prev) < EQR_TOL) return -1.0; /* reset date: fixing happens now */
+            return prev;
+        }
+        prev = pay[i];
+    }
+    return -1.0;
+}
+
+/* Extended valuation (no reference function beyond reset dates; at reset dates it is exactly
+ * eqr_irs_npv_reset). Cash flows alive at t are those with T_i > t + 1e-9.
+ *   t < start            : float = P(t,start) - P(t,T_n)      (forward-starting swap)
+ *   t at a reset date    : float = 1 - P(t,T_n)               (irs.rs:46-68 with start := t)
+ *   T_{k-1} < t < T_k    : float = P(t,T_k)/P(T_{k-1},T_k | r_fix) - P(t,T_n)
+ *   t >= T_n             : value 0
+ *   value = N*float - sum_{T_i > t} (N*K)*(P(t,T_i)*tau_i). */
+double eqr_irs_value(double a, double b, double sigma, double r, double t, double r_fix,
+                     double notional, double fixed_rate, double start, const double* pay,
+                     const double* acc, int m) {
+    if (t >= pay[m - 1] - EQR_TOL) return 0.0;
+    int first = 0;
+    while (first < m && !(pay[first] > t + EQR_TOL)) ++first;
+    double p_end = eqr_zcb(a, b, sigma, r, t, pay[m - 1]);
+    double fl;
+    if (fabs(t - start) < EQR_TOL) {
+        double p_start = eqr_zcb(a, b, sigma, r, t, t);
+        fl = p_start - p_end;
+    } else if (t < start) {
+        double p_start = eqr_zcb(a, b, sigma, r, t, start);
+        fl = p_start - p_end;
+    } else {
+        double prev = (first == 0) ? start : pay[first - 1];
+        if (fabs(t - prev) < EQR_TOL) {
+            double p_start = eqr_zcb(a, b, sigma, r, t, t);
+            fl = p_start - p_end;
+        } else {
+            double p_next = eqr_zcb(a, b, sigma, r, t, pay[first]);
+            double p_fix = eqr_zcb(a, b, sigma, r_fix, prev, pay[first]);
+            fl = p_next / p_fix - p_end;
+        }
+    }
+    double floating_leg = notional * fl;
+    double fixed_leg = 0.0;
+    for (int i = first; i < m; ++i) {
+        double p_i = eqr_zcb(a, b, sigma, r, t, pay[i]);
+        double leg = (notional * fixed_rate) * (p_i * acc[i]);
+        fixed_leg = (i == first) ? leg : fixed_leg + leg;
+    }
+    return floating_leg - fixed_leg;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* EE / PFE95, exposure.rs:42-49                                                              */
+/* ------------------------------------------------------------------------------------------ */
+
+static int cmp_double(const void* x, const void* y) {
+    double a = *(const double*)x, b = *(const double*)y;
+    return (a > b) - (a < b);
+}
+
+/* mean = sum in input order / n; PFE = sorted[min(ceil(0.95 n) - 1, n - 1)] (saturating). */
+void eqr_ee_pfe(double* exposures, uint64_t n, double* ee, double* pfe) {
+    double sum = 0.0;
+    for (uint64_t i = 0; i < n; ++i) sum += exposures[i];
+    *ee = sum / (double)n;
+    qsort(exposures, (size_t)n, sizeof(double), cmp_double);
+    uint64_t q = (uint64_t)ceil(0.95 * (double)n);
+    q = (q == 0) ? 0 : q - 1;
+    if (q > n - 1) q = n - 1;
+    *pfe = exposures[q];
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* CVA, exposure.rs:242-275                                                                   */
+/* ------------------------------------------------------------------------------------------ */
+
+double eqr_unilateral_cva(double a, double b, double sigma, double r0, const double* times,
+                          const double* ee, int n, double hazard_rate, double recovery_rate) {
+    double cva = 0.0;
+    double prev_survival = 1.0;
+    for (int i = 0; i < n; ++i) {
+        double t = times[i];
+        double survival = exp(-hazard_rate * t);
+        double default_prob = prev_survival - survival;
+        double disc = eqr_zcb(a, b, sigma, r0, 0.0, t);
+        cva += (1.0 - recovery_rate) * ee[i] * default_prob * disc;
+        prev_survival = survival;
+    }
+    return cva;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Reference exposure profile of one swap, exposure.rs:73-138, api.rs:42-66,97-118            */
+/* ------------------------------------------------------------------------------------------ */
+
+int eqr_exposure_profile(double a, double b, double sigma, double r0, double notional,
+                         double fixed_rate, int use_par_rate, double start, const double* pay,
+                         const double* acc, int m, const double* times, int n_dates,
+                         uint32_t n_steps, uint64_t n_paths, const double* z, double* ee,
+                         double* pfe) {
+    double k = use_par_rate ? eqr_par_rate(a, b, sigma, r0, start, pay, acc, m) : fixed_rate;
+    double t_max = -INFINITY;
+    for (int i = 0; i < n_dates; ++i) t_max = fmax(t_max, times[i]);
+    if (t_max <= 0.0) {
+        double v = eqr_irs_npv_reset(a, b, sigma, r0, times[0], notional, k, start, pay, acc, m);
+        v = fmax(v, 0.0);
+        ee[0] = v;
+        pfe[0] = v;
+        return 1;
+    }
+    double dt = t_max / (double)n_steps;
+    double* rp = (double*)malloc(sizeof(double) * (n_steps + 1));
+    double* r_all = (double*)malloc(sizeof(double) * (n_steps + 1) * n_paths);
+    for (uint64_t p = 0; p < n_paths; ++p) {
+        simulate_one(a, b, sigma, r0, dt, n_steps, n_paths, p, 0, z, rp);
+        for (uint32_t s = 0; s <= n_steps; ++s) r_all[(uint64_t)s * n_paths + p] = rp[s];
+    }
+    double* x = (double*)malloc(sizeof(double) * n_paths);
+    for (int d = 0; d < n_dates; ++d) {
+        double t = times[d];
+        uint64_t idx = (uint64_t)round(t / dt);
+        uint64_t n = (idx == 0) ? 1 : n_paths; /* path[0] is r0 with shape [1] */
+        for (uint64_t p = 0; p < n; ++p) {
+            double v = eqr_irs_npv_reset(a, b, sigma, r_all[idx * n_paths + p], t, notional, k,
+                                         start, pay, acc, m);
+            x[p] = fmax(v, 0.0);
+        }
+        eqr_ee_pfe(x, n, &ee[d], &pfe[d]);
+    }
+    free(x);
+    free(r_all);
+    free(rp);
+    return n_dates;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Market curve, market.cpp:26-41,95-111, measure.cpp:124-222                                 */
+/* ------------------------------------------------------------------------------------------ */
+
+double eqr_curve_zero_rate(const double* pillars, const double* zeros, int np, double t) {
+    if (t <= pillars[0]) return zeros[0];
+    if (t >= pillars[np - 1]) return zeros[np - 1];
+    int i = 0; /* lower_bound: first pillar >= t */
+    while (i < np && pillars[i] < t) ++i;
+    if (i == 0) i = 1;
+    double t0 = pillars[i - 1], t1 = pillars[i];
+    double z0 = zeros[i - 1], z1 = zeros[i];
+    double w = (t - t0) / (t1 - t0);
+    return z0 + w * (z1 - z0);
+}
+
+double eqr_curve_df(const double* pillars, const double* zeros, int np, double t) {
+    return exp(-eqr_curve_zero_rate(pillars, zeros, np, t) * t);
+}
+
+double eqr_par_rate_from_market(const double* pillars, const double* zeros, int np, double start,
+                                const double* pay, const double* acc, int m) {
+    double numerator = eqr_curve_df(pillars, zeros, np, start) -
+                       eqr_curve_df(pillars, zeros, np, pay[m - 1]);
+    double denominator = 0.0;
+    for (int i = 0; i < m; ++i) denominator += acc[i] * eqr_curve_df(pillars, zeros, np, pay[i]);
+    return numerator / denominator;
+}
+
+/* valuation_time = 0: discount_at_valuation = DF(0) = 1 (measure.cpp:160-184). */
+double eqr_npv_from_market(const double* pillars, const double* zeros, int np, double notional,
+                           double fixed_rate, double start, const double* pay, const double* acc,
+                           int m) {
+    double dv = eqr_curve_df(pillars, zeros, np, 0.0);
+    double p_start = eqr_curve_df(pillars, zeros, np, start) / dv;
+    double p_end = eqr_curve_df(pillars, zeros, np, pay[m - 1]) / dv;
+    double floating_leg = notional * (p_start - p_end);
+    double fixed_leg = 0.0;
+    for (int i = 0; i < m; ++i)
+        fixed_leg += notional * fixed_rate * acc[i] * (eqr_curve_df(pillars, zeros, np, pay[i]) / dv);
+    return floating_leg - fixed_leg;
+}
+
+double eqr_dv01(const double* pillars, const double* zeros, int np, double notional,
+                double fixed_rate, double start, const double* pay, const double* acc, int m,
+                double bump) {
+    double* bumped = (double*)malloc(sizeof(double) * np);
+    for (int i = 0; i < np; ++i) bumped[i] = zeros[i] + bump;
+    double base = eqr_npv_from_market(pillars, zeros, np, notional, fixed_rate, start, pay, acc, m);
+    double up = eqr_npv_from_market(pillars, bumped, np, notional, fixed_rate, start, pay, acc, m);
+    free(bumped);
+    return up - base;
+}
+
+void eqr_dv01_bucketed(const double* pillars, const double* zeros, int np, double notional,
+                       double fixed_rate, double start, const double* pay, const double* acc,
+                       int m, double bump, double* out) {
+    double* bumped = (double*)malloc(sizeof(double) * np);
+    double base = eqr_npv_from_market(pillars, zeros, np, notional, fixed_rate, start, pay, acc, m);
+    for (int j = 0; j < np; ++j) {
+        memcpy(bumped, zeros, sizeof(double) * np);
+        bumped[j] += bump;
+        out[j] = eqr_npv_from_market(pillars, bumped, np, notional, fixed_rate, start, pay, acc, m) -
+                 base;
+    }
+    free(bumped);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Portfolio / netting-set exposure                                                           */
+/* ------------------------------------------------------------------------------------------ */
+
+int eqr_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+int eqr_portfolio_exposure(double a, double b, double sigma, double r0, int n_trades, int n_sets,
+                           const int32_t* set_id, const double* notional,
+                           const double* fixed_rate, const double* start,
+                           const int32_t* flow_off, const double* pay, const double* acc,
+                           const double* dates, int n_dates, uint32_t n_steps, uint64_t n_paths,
+                           int scheme, const double* z, double hazard_rate, double recovery_rate,
+                           double* set_ee, double* set_pfe, double* set_cva, double* trade_ee,
+                           double* trade_pfe, double* trade_cva, double* path_mtm,
+                           double* set_mtm, int n_threads) {
+    double t_max = -INFINITY;
+    for (int k = 0; k < n_dates; ++k) t_max = fmax(t_max, dates[k]);
+    int deterministic = !(t_max > 0.0);
+    double dt = deterministic ? 1.0 : t_max / (double)n_steps;
+    uint64_t* idx = (uint64_t*)malloc(sizeof(uint64_t) * (n_dates > 0 ? n_dates : 1));
+    for (int k = 0; k < n_dates; ++k) idx[k] = deterministic ? 0 : (uint64_t)round(dates[k] / dt);
+
+    /* exposures: set rows [n_sets][n_dates][n_paths], trade rows likewise when requested */
+    int want_trade = (trade_ee != NULL) || (trade_pfe != NULL) || (trade_cva != NULL);
+    double* xs = (double*)malloc(sizeof(double) * (size_t)n_sets * n_dates * n_paths);
+    double* xt = want_trade ? (double*)malloc(sizeof(double) * (size_t)n_trades * n_dates * n_paths)
+                            : NULL;
+    uint32_t steps = deterministic ? 0 : n_steps;
+
+#ifdef _OPENMP
+    if (n_threads <= 0) n_threads = omp_get_max_threads();
+#pragma omp parallel num_threads(n_threads)
+#endif
+    {
+        double* rp = (double*)malloc(sizeof(double) * (steps + 1));
+        double* vset = (double*)malloc(sizeof(double) * (n_sets > 0 ? n_sets : 1));
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+        for (int64_t pp = 0; pp < (int64_t)n_paths; ++pp) {
+            uint64_t p = (uint64_t)pp;
+            simulate_one(a, b, sigma, r0, dt, steps, n_paths, p, scheme, z, rp);
+            for (int k = 0; k < n_dates; ++k) {
+                double t = dates[k];
+                double r = rp[idx[k]];
+                for (int s = 0; s < n_sets; ++s) vset[s] = 0.0;
+                for (int j = 0; j < n_trades; ++j) {
+                    const double* pj = pay + flow_off[j];
+                    const double* aj = acc + flow_off[j];
+                    int m = flow_off[j + 1] - flow_off[j];
+                    double t_fix = eqr_fixing_date(t, start[j], pj, m);
+                    double r_fix = 0.0;
+                    if (t_fix >= 0.0) r_fix = rp[deterministic ? 0 : (uint64_t)round(t_fix / dt)];
+                    double v = eqr_irs_value(a, b, sigma, r, t, r_fix, notional[j], fixed_rate[j],
+                                             start[j], pj, aj, m);
+                    vset[set_id[j]] += v;
+                    if (xt) xt[((size_t)j * n_dates + k) * n_paths + p] = fmax(v, 0.0);
+                    if (path_mtm) path_mtm[((size_t)k * n_trades + j) * n_paths + p] = v;
+                }
+                for (int s = 0; s < n_sets; ++s) {
+                    xs[((size_t)s * n_dates + k) * n_paths + p] = fmax(vset[s], 0.0);
+                    if (set_mtm) set_mtm[((size_t)k * n_sets + s) * n_paths + p] = vset[s];
+                }
+            }
+        }
+        free(vset);
+        free(rp);
+    }
+
+    double* scratch = (double*)malloc(sizeof(double) * n_paths);
+    for (int s = 0; s < n_sets; ++s) {
+        for (int k = 0; k < n_dates; ++k) {
+            uint64_t n = (idx[k] == 0) ? 1 : n_paths;
+            memcpy(scratch, xs + ((size_t)s * n_dates + k) * n_paths, sizeof(double) * n);
+            eqr_ee_pfe(scratch, n, &set_ee[(size_t)s * n_dates + k], &set_pfe[(size_t)s * n_dates + k]);
+        }
+        if (set_cva)
+            set_cva[s] = eqr_unilateral_cva(a, b, sigma, r0, dates, set_ee + (size_t)s * n_dates,
+                                            n_dates, hazard_rate, recovery_rate);
+    }
+    if (want_trade) {
+        double* tee = (double*)malloc(sizeof(double) * (size_t)n_trades * n_dates);
+        for (int j = 0; j < n_trades; ++j) {
+            for (int k = 0; k < n_dates; ++k) {
+                uint64_t n = (idx[k] == 0) ? 1 : n_paths;
+                double e, q;
+                memcpy(scratch, xt + ((size_t)j * n_dates + k) * n_paths, sizeof(double) * n);
+                eqr_ee_pfe(scratch, n, &e, &q);
+                tee[(size_t)j * n_dates + k] = e;
+                if (trade_ee) trade_ee[(size_t)j * n_dates + k] = e;
+                if (trade_pfe) trade_pfe[(size_t)j * n_dates + k] = q;
+            }
+            if (trade_cva)
+                trade_cva[j] = eqr_unilateral_cva(a, b, sigma, r0, dates, tee + (size_t)j * n_dates,
+                                                  n_dates, hazard_rate, recovery_rate);
+        }
+        free(tee);
+    }
+    free(scratch);
+    free(xt);
+    free(xs);
+    free(idx);
+    return 0;
+}

@@ -1,0 +1,119 @@
+/*
+ * eq_ref — CPU FP64 restatement of engine_quant's Monte Carlo exposure path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the oracle the HIP path is checked against.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and only
+ * as the checker or the timed CPU baseline. The product (engine_quant_amd, libeqhip) never
+ * links, loads or calls it.
+ *
+ * Every function cites the reference file:line whose arithmetic it restates; the expression
+ * order of the reference is kept so that the restatement is a faithful CPU twin.
+ *
+ * Parity pins (tests/test_oracle.py): reference golden values (PV/DV01 exact, EE/PFE/CVA of the
+ * seed-7 case within 3 Monte Carlo standard errors), closed forms (ZCB vs numerical integral,
+ * Euler conditional mean, MC ZCB), Philox4x32-10 known-answer vectors, brute-force order
+ * statistics, the martingale property of the between-reset floating leg, and the
+ * single-trade-netting-set reduction.
+ */
+#ifndef EQ_REF_H
+#define EQ_REF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Hull-White 1F (Vasicek form), models/hull_white.rs:50-80 ---- */
+double eqr_hw_b(double a, double tau);
+double eqr_hw_a(double a, double b, double sigma, double tau, double bf);
+double eqr_zcb(double a, double b, double sigma, double r, double t, double maturity);
+
+/* ---- Philox4x32-10 (Salmon et al., SC'11) + Box-Muller normals ---- */
+void eqr_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* Normals Z[step][path] for steps [0,n_steps), paths [0,n_paths): counter = (step/2, path_lo,
+ * path_hi, 0), key = (seed_lo, seed_hi). */
+void eqr_philox_normals(uint64_t seed, uint64_t n_paths, uint32_t n_steps, double* z);
+/* The two uniforms of one Philox draw, as used by Box-Muller: u1 in (0,1], u2 in [0,1). */
+void eqr_philox_uniforms(uint64_t seed, uint64_t path, uint32_t pair, double* u1, double* u2);
+
+/* ---- short-rate simulation, kernel.rs:41-50, hull_white.rs:87-97 ----
+ * scheme 0 = Euler-Maruyama (reference), 1 = exact discretisation.
+ * z: [n_steps][n_paths] standard normals; r_out: [n_steps+1][n_paths]. */
+void eqr_simulate(double a, double b, double sigma, double r0, double dt, uint32_t n_steps,
+                  uint64_t n_paths, int scheme, const double* z, double* r_out);
+
+/* ---- vanilla payer IRS, products/irs.rs ---- */
+int eqr_is_reset_date(double start, const double* pay, int m, double t);
+double eqr_par_rate(double a, double b, double sigma, double r, double start,
+                    const double* pay, const double* acc, int m);
+/* NPV at a reset date t (reference semantics: remaining_from(t) then npv, irs.rs:46-68,163-187). */
+double eqr_irs_npv_reset(double a, double b, double sigma, double r, double t, double notional,
+                         double fixed_rate, double start, const double* pay, const double* acc, int m);
+/* Extended valuation at any date t (between resets the floating leg carries the fixing taken
+ * at the start of the period in progress, from r_fix = short rate at that date). */
+double eqr_irs_value(double a, double b, double sigma, double r, double t, double r_fix,
+                     double notional, double fixed_rate, double start, const double* pay,
+                     const double* acc, int m);
+/* Start date of the period in progress at t, or -1 when no fixing is needed at t. */
+double eqr_fixing_date(double t, double start, const double* pay, int m);
+
+/* ---- exposure aggregation, exposure.rs:42-49 ---- */
+void eqr_ee_pfe(double* exposures, uint64_t n, double* ee, double* pfe);
+
+/* ---- CVA, exposure.rs:242-275 ---- */
+double eqr_unilateral_cva(double a, double b, double sigma, double r0, const double* times,
+                          const double* ee, int n, double hazard_rate, double recovery_rate);
+
+/* ---- reference exposure profile of one swap, exposure.rs:73-138 + api.rs:42-66 ----
+ * use_par_rate: K = par rate under the model at r0 (api.rs:53-57).
+ * z: [n_steps][n_paths]. Returns the number of dates written (1 for the t_max<=0 branch). */
+int eqr_exposure_profile(double a, double b, double sigma, double r0, double notional,
+                         double fixed_rate, int use_par_rate, double start, const double* pay,
+                         const double* acc, int m, const double* times, int n_dates,
+                         uint32_t n_steps, uint64_t n_paths, const double* z, double* ee,
+                         double* pfe);
+
+/* ---- market curve PV / DV01, market.cpp:26-41,95-111, measure.cpp:124-222 ---- */
+double eqr_curve_zero_rate(const double* pillars, const double* zeros, int np, double t);
+double eqr_curve_df(const double* pillars, const double* zeros, int np, double t);
+double eqr_par_rate_from_market(const double* pillars, const double* zeros, int np, double start,
+                                const double* pay, const double* acc, int m);
+double eqr_npv_from_market(const double* pillars, const double* zeros, int np, double notional,
+                           double fixed_rate, double start, const double* pay, const double* acc,
+                           int m);
+/* DV01 = NPV(z+bump) - NPV(z) at the same effective fixed rate; bucketed: one pillar at a time. */
+double eqr_dv01(const double* pillars, const double* zeros, int np, double notional,
+                double fixed_rate, double start, const double* pay, const double* acc, int m,
+                double bump);
+void eqr_dv01_bucketed(const double* pillars, const double* zeros, int np, double notional,
+                       double fixed_rate, double start, const double* pay, const double* acc,
+                       int m, double bump, double* out);
+
+/* ---- portfolio / netting-set exposure (extended semantics, per-trade restatement) ----
+ * trades j: set_id[j], notional[j] (signed: receiver < 0), fixed_rate[j], start[j], schedule
+ * flows pay/acc[flow_off[j] .. flow_off[j+1]).
+ * dates: monitoring times; grid dt = max(dates)/n_steps; r at date = r[round(t/dt)].
+ * Outputs (nullable unless noted):
+ *   set_ee, set_pfe [n_sets][n_dates] (required); set_cva [n_sets];
+ *   trade_ee, trade_pfe [n_trades][n_dates]; trade_cva [n_trades];
+ *   path_mtm [n_dates][n_trades][n_paths]; set_mtm [n_dates][n_sets][n_paths].
+ * n_threads: OpenMP threads over paths (<=0: library default). */
+int eqr_portfolio_exposure(double a, double b, double sigma, double r0, int n_trades, int n_sets,
+                           const int32_t* set_id, const double* notional,
+                           const double* fixed_rate, const double* start,
+                           const int32_t* flow_off, const double* pay, const double* acc,
+                           const double* dates, int n_dates, uint32_t n_steps, uint64_t n_paths,
+                           int scheme, const double* z, double hazard_rate, double recovery_rate,
+                           double* set_ee, double* set_pfe, double* set_cva, double* trade_ee,
+                           double* trade_pfe, double* trade_cva, double* path_mtm,
+                           double* set_mtm, int n_threads);
+
+/* Maximum OpenMP threads the oracle would use (for the CPU-baseline report). */
+int eqr_max_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,42 @@
+"""Shared test inputs (small portfolios with mixed schedules) and comparison helpers."""
+import numpy as np
+
+from engine_quant_amd.eqhip import Portfolio
+
+MODEL = (0.10, 0.03, 0.01, 0.02)
+
+
+def small_portfolio(n_trades=12, n_sets=3, seed=3, max_years=6, start=0.0):
+    rng = np.random.default_rng(seed)
+    pays, accs, off, fixed = [], [], [0], []
+    for j in range(n_trades):
+        f = int(rng.choice([1, 2, 4]))
+        n_per = int(rng.integers(1, max_years * f + 1))
+        s0 = start if j % 5 else start + (1.0 / f) * int(rng.integers(0, 3))
+        pay = s0 + np.arange(1, n_per + 1) / f
+        pays.append(pay)
+        accs.append(np.full(n_per, 1.0 / f))
+        off.append(off[-1] + n_per)
+        fixed.append(0.02 + rng.normal(0, 0.005))
+    starts = np.array([pays[j][0] - accs[j][0] for j in range(n_trades)])
+    notional = np.exp(rng.normal(np.log(1e6), 0.7, n_trades)) * np.where(rng.uniform(size=n_trades) < 0.5, 1, -1)
+    return Portfolio(set_id=rng.integers(0, n_sets, n_trades).astype(np.int32), notional=notional,
+                     fixed_rate=np.array(fixed), start=starts, flow_off=np.array(off, dtype=np.int32),
+                     pay=np.concatenate(pays), acc=np.concatenate(accs))
+
+
+def leg_scale(port, model, dates):
+    """|notional| * (1 + sum of accruals * |K|) per trade: the magnitude of the swap's legs, the
+    scale of its mark-to-market rounding."""
+    scale = np.zeros(port.n_trades)
+    for j in range(port.n_trades):
+        f0, f1 = port.flow_off[j], port.flow_off[j + 1]
+        scale[j] = abs(port.notional[j]) * (2.0 + np.sum(port.acc[f0:f1]) * abs(port.fixed_rate[j]))
+    return scale
+
+
+def assert_rel(actual, expected, rel, scale=None, what=""):
+    actual, expected = np.asarray(actual), np.asarray(expected)
+    s = np.maximum(np.abs(expected), 1.0) if scale is None else scale
+    err = np.max(np.abs(actual - expected) / s) if actual.size else 0.0
+    assert err <= rel, f"{what}: max scaled error {err:.3e} > {rel:.1e}"
