@@ -1,0 +1,198 @@
+"""Throughput of the HW1F swap exposure path on one MI355X.
+
+One step = one full exposure job on resident data: Philox4x32-10 + exact HW1F simulation of all
+paths, per-path mark-to-market of every swap at every monitoring date, netting-set EE / PFE95 /
+CVA, per-trade EE / CVA, plus the PV / DV01 / bucketed-DV01 ladder of every trade. The workload is
+the one-netting-set portfolio: 1000 seeded synthetic swaps, 100k paths x 360 monthly dates, FP64.
+
+Prints one JSON line (rank 0). ``--gpus`` above 1 is refused: this build measures one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "trade-paths-dates/sec for HW1F swap EE/PFE/CVA (GPU vs CPU); CVA err vs FP64"
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
+EXP_FLOPS = 20            # FP64 exp as range reduction + degree-~10 FMA polynomial + scaling
+
+
+def algorithmic_work(port, dates, n_paths, n_sets):
+    """Algorithmic FP64 FLOPs and bytes of one value_kernel launch (the dominant kernel).
+
+    Units and per-unit figures:
+      * live (trade, path, date):        5 FLOPs   V = N*F - (N K)*A (mul + FMA), max, add
+      * live (class, path, date):        2 m + 2   annuity FMAs over the m remaining flows,
+                                                   floating leg (sub, + div mid-period)
+      * live (set, class, path, date):   4 FLOPs   V_s += alpha F - beta A (2 FMAs)
+      * live (cash-flow date, path, date): exp + 2 FLOPs   P = A exp(-B r)
+    Bytes: r read (8 B per path-date) + exposure write (8 B per set-path-date).
+    """
+    tol = 1e-9
+    off = port.flow_off
+    # schedule classes
+    keys = {}
+    cls = np.empty(port.n_trades, dtype=np.int64)
+    for j in range(port.n_trades):
+        k = (float(port.start[j]), tuple(port.pay[off[j]:off[j + 1]]), tuple(port.acc[off[j]:off[j + 1]]))
+        cls[j] = keys.setdefault(k, len(keys))
+    classes = list(keys.keys())
+    mats = np.array([c[1][-1] for c in classes])
+    cflows = [np.array(c[1]) for c in classes]
+    n_cls_trades = np.bincount(cls, minlength=len(classes))
+    set_cls = len({(int(port.set_id[j]), int(cls[j])) for j in range(port.n_trades)})
+    cf_dates = np.unique(np.concatenate([port.pay, port.start]))
+    flops = 0.0
+    u_live = 0.0
+    for t in dates:
+        live = mats > t + tol
+        rem = np.array([np.sum(f > t + tol) for f in cflows])
+        u_live += float(np.sum(n_cls_trades[live]))
+        per_path = (5.0 * np.sum(n_cls_trades[live]) + np.sum(2.0 * rem[live] + 2.0)
+                    + 4.0 * set_cls * np.mean(live) + (EXP_FLOPS + 2.0) * np.sum(cf_dates > t + tol))
+        flops += per_path
+    flops *= n_paths
+    u_live *= n_paths
+    bytes_ = 8.0 * n_paths * len(dates) * (1 + n_sets)
+    return flops, bytes_, u_live
+
+
+def cpu_baseline(port, dates, n_steps, model, seed, disc, target_s=15.0):
+    """The oracle (CPU FP64 restatement of the reference, OpenMP over paths) on a bounded path sample
+    of the same workload; returns (units/s, threads, sample description, oracle CVA)."""
+    import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    threads = min(threads, 16)
+    # calibrate on a tiny sample, then size the timed sample to ~target_s seconds
+    probe = 8
+    z = oracle.philox_normals(seed, probe, n_steps)
+    t0 = time.perf_counter()
+    oracle.portfolio_exposure(model, port.as_dict(), dates, n_steps, z, scheme=disc, want_trade=True,
+                              n_threads=threads)
+    dt = time.perf_counter() - t0
+    n_sample = int(max(threads, min(4096, probe * target_s / max(dt, 1e-6))))
+    n_sample = max(threads, (n_sample // threads) * threads)
+    z = oracle.philox_normals(seed, n_sample, n_steps)
+    t0 = time.perf_counter()
+    ref = oracle.portfolio_exposure(model, port.as_dict(), dates, n_steps, z, scheme=disc, hazard_rate=0.02,
+                                    recovery_rate=0.4, want_trade=True, n_threads=threads)
+    dt = time.perf_counter() - t0
+    units = port.n_trades * n_sample * len(dates)
+    return units / dt, threads, n_sample, dt, ref
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--trades", type=int, default=1000)
+    ap.add_argument("--paths", type=int, default=100_000)
+    ap.add_argument("--years", type=int, default=30)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--disc", choices=["exact", "euler"], default="exact")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
+              "(paths would shard by global path index; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from engine_quant_amd import eqhip
+    from engine_quant_amd.portfolios import MODEL, monthly_dates, swap_portfolio, zero_curve
+
+    pillars, zeros = zero_curve(args.seed)
+    port = swap_portfolio(args.trades, seed=args.seed, pillars=pillars, zeros=zeros)
+    dates = monthly_dates(args.years)
+    n_steps = len(dates)
+    disc = eqhip.DISC_EXACT if args.disc == "exact" else eqhip.DISC_EULER
+    hazard, recovery = 0.02, 0.4
+
+    ctx = eqhip.Context(0)
+    plan = ctx.plan(MODEL, port, dates, args.paths, n_steps, seed=args.seed, rng=eqhip.RNG_PHILOX, disc=disc,
+                    want_trade_ee=True)
+
+    def step():
+        res = plan.execute(None, hazard, recovery, want_cva=True)
+        pv, dv, bk = ctx.pv_dv01(pillars, zeros, port, bump=1e-4, bucketed=True)
+        return res, pv, dv
+
+    for _ in range(args.warmup):
+        step()
+    value_ms, total_kernel_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res, pv, dv = step()
+        tm = plan.timing()
+        value_ms.append(tm["value_ms"])
+        total_kernel_ms.append(tm["total_ms"])
+        print(f"# step done: value_kernel {tm['value_ms']:.3f} ms, exposure kernels {tm['total_ms']:.3f} ms",
+              file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0  # every step ends with a stream synchronisation
+    ms_per_step = 1000.0 * elapsed / args.steps
+    units = port.n_trades * args.paths * len(dates)
+    value = units / (ms_per_step / 1000.0)
+
+    flops, bytes_, u_live = algorithmic_work(port, dates, args.paths, port.n_sets)
+    v_ms = float(np.mean(value_ms))
+    achieved = flops / (v_ms / 1000.0) / 1e12
+    tm = plan.timing()
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "trade-paths-dates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "hw1f-irs-one-netting-set", "n_trades": args.trades, "n_paths": args.paths,
+                   "n_dates": len(dates), "n_steps": n_steps, "dates": "monthly", "rng": "philox4x32-10",
+                   "discretisation": args.disc, "seed": args.seed, "n_sets": port.n_sets,
+                   "outputs": "set EE/PFE95/CVA, trade EE/CVA, PV/DV01/bucketed DV01 (20 pillars)",
+                   "live_units": u_live},
+        "roofline": {"kernel": "value_kernel", "bound": "fp64-valu", "achieved": achieved,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": None, "algorithmic_flops": flops, "algorithmic_bytes": bytes_,
+                     "value_kernel_ms": v_ms, "exposure_kernels_ms": float(np.mean(total_kernel_ms)),
+                     "paths_per_tile": tm["paths_per_tile"], "tiles_per_block": tm["tiles_per_block"],
+                     "blocks": tm["n_value_blocks"]},
+        "set_cva": float(res["set_cva"][0]),
+    }
+    if not args.no_cpu:
+        rate, threads, n_sample, cpu_s, ref = cpu_baseline(port, dates, n_steps, MODEL, args.seed, disc,
+                                                           args.cpu_seconds)
+        # the same paths on the GPU (counter-based stream: path p's draws do not depend on n_paths)
+        small = ctx.exposure_run(MODEL, port, dates, n_sample, n_steps, seed=args.seed, rng=eqhip.RNG_PHILOX,
+                                 disc=disc, hazard_rate=hazard, recovery_rate=recovery, want_trade_ee=True)
+        cva_err = abs(small["set_cva"][0] - ref["set_cva"][0]) / max(abs(ref["set_cva"][0]), 1e-300)
+        out["cpu_baseline"] = {"value": rate, "unit": "trade-paths-dates/s", "cores": threads, "kind": "port",
+                               "sample": f"same portfolio and dates, first {n_sample} Philox paths "
+                                         f"({cpu_s:.1f} s of oracle time, OpenMP over paths)"}
+        out["speedup_vs_cpu"] = value / rate
+        out["cva_rel_err_vs_fp64_oracle"] = cva_err
+        out["trade_ee_max_rel_err_vs_oracle"] = float(
+            np.max(np.abs(small["trade_ee"] - ref["trade_ee"]) / np.maximum(np.abs(ref["trade_ee"]).max(), 1e-300)))
+    plan.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

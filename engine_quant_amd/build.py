@@ -23,7 +23,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", 
 
 EQHIP_SOURCES = ["eqhip_kernels.hip", "eqhip_api.hip"]
 HOST_SOURCES = ["host/params.cpp", "host/market.cpp", "host/model_product.cpp", "host/contexts.cpp",
-                "host/measures.cpp", "host/price.cpp", "host/abi.cpp"]
+                "host/measures.cpp", "host/price.cpp"]
 PY_SOURCES = ["py/engine_py.cpp"]
 
 LIBEQHIP = os.path.join(PKG, "libeqhip.so")

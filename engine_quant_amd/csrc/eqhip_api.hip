@@ -122,20 +122,23 @@ struct eqhip_plan {
     eqhip_context* ctx = nullptr;
     eqhip_model_hw1f model{};
     eqhip_mc mc{};
-    int n_dates = 0, n_trades = 0, S = 0, C = 0, D = 0, Fmax = 0, n_slots = 0;
+    int n_dates = 0, n_trades = 0, S = 0, C = 0, NF = 0, D = 0, Fmax = 0, n_slots = 0;
     bool det = false;
     double dt = 1.0;
-    int PT = 32, tiles_per_block = 1, n_chunks = 1;
+    int tiles_per_block = 1, n_chunks = 1;
     uint64_t n_tiles = 0;
     int want_trade = 0;
     int n_items = 0;
+    int set_mode = 0;
     std::vector<double> dates;
     size_t lds_bytes = 0;
 
-    DevBuf slot_of_step, R, Z, AB, date_slot, date_dfirst, date_det, cdesc, cdate, flow_d,
-        flow_tau, fix_off, fix_slot, fix_AB, set_off, set_cls, set_alpha, set_beta, X, set_mtm,
-        item_cls, item_first, item_count, tr_N, tr_NK, tr_orig, partial, path_mtm, trade_ee,
-        trade_cva, set_ee, set_pfe, set_cva, neff, qrank, d_dates;
+    DevBuf slot_of_step, R, Z, AB, date_slot, date_dfirst, date_det, fdesc, fam_off, flow_d, flow_tau,
+        flow_cls, fix_off, fix_slot, fix_AB, set_off, set_cls, set_alpha, set_beta, X, items, cls_mat, tr_K,
+        tr_absN, tr_N, tr_NK, tr_orig, partial, trade_ee, trade_cva, set_ee, set_pfe, set_cva, neff, qrank,
+        d_dates, vplan, units;
+    ValuePlan vp{};  // host copy of the device descriptor (per-run pointers patched at launch)
+    std::vector<int4> unit_list;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     eqhip_timing timing{};
 
@@ -180,29 +183,24 @@ void validate_portfolio(const eqhip_portfolio* pf) {
 }
 
 void choose_launch(eqhip_plan& P) {
-    auto lds = [&](int pt) { return (size_t)8 * pt * (1 + P.D + P.Fmax + 2 * P.C); };
-    int pts[3] = {64, 32, 16};
-    P.PT = 0;
-    for (int pt : pts) {
-        if (lds(pt) <= P.ctx->lds_limit) {
-            P.PT = pt;
-            break;
-        }
-    }
-    if (P.PT == 0)
-        raise(EQHIP_E_CAPACITY,
-              "portfolio too diverse for one plan: " + std::to_string(P.D) + " cash-flow dates and " +
-                  std::to_string(P.C) + " schedule classes need " + std::to_string(lds(16)) +
-                  " B of LDS at 16 paths per tile (limit " + std::to_string(P.ctx->lds_limit) + ")");
-    // keep PT = 64 only when it still leaves room for two blocks per CU
-    if (P.PT == 64 && lds(64) > P.ctx->lds_limit / 2 && lds(32) <= P.ctx->lds_limit / 2) P.PT = 32;
-    P.lds_bytes = lds(P.PT);
-    P.n_tiles = (P.mc.n_paths + P.PT - 1) / P.PT;
-    uint64_t target_chunks = std::max<uint64_t>(1, (4096 + P.n_dates - 1) / P.n_dates);
+    P.set_mode = (P.S <= kMaxSplitSets) ? 0 : 1;
+    size_t lds = (size_t)8 * kPathsPerTile * (1 + P.D + P.Fmax) + (size_t)16 * kPathsPerTile * P.C +
+                 (P.set_mode == 0 ? (size_t)8 * kPathsPerTile * kValueWaves * P.S : 0);
+    if (lds > P.ctx->lds_limit)
+        raise(EQHIP_E_CAPACITY, "portfolio too diverse for one plan: " + std::to_string(P.D) +
+                                    " cash-flow dates and " + std::to_string(P.C) + " schedule classes need " +
+                                    std::to_string(lds) + " B of LDS (limit " + std::to_string(P.ctx->lds_limit) +
+                                    "); split it into several plans");
+    P.lds_bytes = lds;
+    P.n_tiles = (P.mc.n_paths + kPathsPerTile - 1) / kPathsPerTile;
+    // enough units for ~8 blocks per CU (one 512-thread block per CU at this LDS footprint)
+    uint64_t target_chunks = std::max<uint64_t>(1, (2048 + P.n_dates - 1) / P.n_dates);
     uint64_t tpb = std::max<uint64_t>(1, (P.n_tiles + target_chunks - 1) / target_chunks);
     P.tiles_per_block = (int)tpb;
     P.n_chunks = (int)((P.n_tiles + tpb - 1) / tpb);
 }
+
+void build_value_plan(eqhip_plan& P);
 
 void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfolio* pf,
                 const double* dates, int n_dates, const eqhip_mc* mc, int want_trade) {
@@ -265,16 +263,46 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
     P.D = (int)T.size();
     auto didx = [&](double t) { return (int)(std::lower_bound(T.begin(), T.end(), t) - T.begin()); };
 
-    // class flows
-    std::vector<int> cls_flow_off(P.C + 1, 0), flow_d;
-    std::vector<double> flow_tau;
-    for (int c = 0; c < P.C; ++c) {
-        for (size_t i = 0; i < classes[c].pay.size(); ++i) {
-            flow_d.push_back(didx(classes[c].pay[i]));
-            flow_tau.push_back(classes[c].acc[i]);
+    // schedule families: classes whose (start, payment, accrual) sequences are prefixes of one
+    // representative schedule share one running annuity sum
+    std::vector<int> cls_order(P.C);
+    for (int c = 0; c < P.C; ++c) cls_order[c] = c;
+    std::stable_sort(cls_order.begin(), cls_order.end(),
+                     [&](int x, int y) { return classes[x].pay.size() > classes[y].pay.size(); });
+    std::vector<int> fam_rep;               // representative class of each family
+    std::vector<int> cls_fam(P.C), cls_len(P.C);
+    for (int c : cls_order) {
+        const ClassKey& K = classes[c];
+        const size_t m = K.pay.size();
+        int fam = -1;
+        for (size_t F = 0; F < fam_rep.size() && fam < 0; ++F) {
+            const ClassKey& R = classes[fam_rep[F]];
+            if (R.start == K.start && std::equal(K.pay.begin(), K.pay.end(), R.pay.begin()) &&
+                std::equal(K.acc.begin(), K.acc.end(), R.acc.begin()))
+                fam = (int)F;
         }
-        cls_flow_off[c + 1] = (int)flow_d.size();
+        if (fam < 0) {
+            fam = (int)fam_rep.size();
+            fam_rep.push_back(c);
+        }
+        cls_fam[c] = fam;
+        cls_len[c] = (int)m;
     }
+    P.NF = (int)fam_rep.size();
+    std::vector<int> fam_off(P.NF + 1, 0), flow_d, flow_cls;
+    std::vector<double> flow_tau;
+    for (int F = 0; F < P.NF; ++F) {
+        const ClassKey& R = classes[fam_rep[F]];
+        for (size_t i = 0; i < R.pay.size(); ++i) {
+            flow_d.push_back(didx(R.pay[i]));
+            flow_tau.push_back(R.acc[i]);
+            flow_cls.push_back(-1);
+        }
+        fam_off[F + 1] = (int)flow_d.size();
+    }
+    for (int c = 0; c < P.C; ++c) flow_cls[fam_off[cls_fam[c]] + cls_len[c] - 1] = c;
+    std::vector<double> cls_mat(P.C);
+    for (int c = 0; c < P.C; ++c) cls_mat[c] = classes[c].pay.back();
 
     // slots: grid indices at which r is stored
     std::map<int64_t, int> slot_of;
@@ -284,8 +312,7 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
         return it->second;
     };
     std::vector<int> date_slot(n_dates), date_dfirst(n_dates), date_det(n_dates);
-    std::vector<int4> cdesc((size_t)n_dates * P.C);
-    std::vector<int2> cdate((size_t)n_dates * P.C);
+    std::vector<int4> fdesc((size_t)n_dates * P.NF);
     std::vector<int> fix_off(n_dates + 1, 0), fix_slot;
     std::vector<double> fix_tau;
     std::vector<double> ab_tau((size_t)n_dates * P.D, -1.0);
@@ -298,29 +325,28 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
         date_dfirst[k] = dfirst;
         for (int d = dfirst; d < P.D; ++d) ab_tau[(size_t)k * P.D + d] = T[d] - t;
         std::map<std::pair<int, double>, int> fix_here;
-        for (int c = 0; c < P.C; ++c) {
-            const ClassKey& K = classes[c];
+        // a family is alive while its representative (longest) schedule is; every live member
+        // class shares the representative's period state at t (same start, same earlier flows)
+        for (int F = 0; F < P.NF; ++F) {
+            const ClassKey& K = classes[fam_rep[F]];
             const int m = (int)K.pay.size();
-            int4 cd = make_int4(0, 0, 0, 0);
-            int2 dd = make_int2(0, 0);
+            int4 fd = make_int4(0, 0, 0, 0);
             if (t < K.pay[m - 1] - kDateTolHost) {
                 int first = 0;
                 while (first < m && !(K.pay[first] > t + kDateTolHost)) ++first;
-                cd.y = cls_flow_off[c] + first;
-                cd.z = cls_flow_off[c + 1];
-                dd.y = didx(K.pay[m - 1]);
+                fd.y = first;
                 if (std::fabs(t - K.start) < kDateTolHost) {
-                    cd.x = 1;
+                    fd.x = 1;
                 } else if (t < K.start) {
-                    cd.x = 2;
-                    dd.x = didx(K.start);
+                    fd.x = 2;
+                    fd.z = didx(K.start);
                 } else {
                     double prev = (first == 0) ? K.start : K.pay[first - 1];
                     if (std::fabs(t - prev) < kDateTolHost) {
-                        cd.x = 1;
+                        fd.x = 1;
                     } else {
-                        cd.x = 3;
-                        dd.x = didx(K.pay[first]);
+                        fd.x = 3;
+                        fd.z = didx(K.pay[first]);
                         int fs = slot(grid_of(prev));
                         double ftau = K.pay[first] - prev;
                         auto key = std::make_pair(fs, ftau);
@@ -330,12 +356,11 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
                             fix_slot.push_back(fs);
                             fix_tau.push_back(ftau);
                         }
-                        cd.w = it->second;
+                        fd.w = it->second;
                     }
                 }
             }
-            cdesc[(size_t)k * P.C + c] = cd;
-            cdate[(size_t)k * P.C + c] = dd;
+            fdesc[(size_t)k * P.NF + F] = fd;
         }
         fix_off[k + 1] = (int)fix_slot.size();
         P.Fmax = std::max(P.Fmax, (int)fix_here.size());
@@ -363,28 +388,33 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
         set_off[s + 1] = (int)set_cls.size();
     }
 
-    // per-trade work items: trades sorted by class, up to kTradesPerItem per item
-    std::vector<int> item_cls, item_first, item_count, tr_orig;
-    std::vector<double> tr_N, tr_NK;
+    // per-trade work items: trades sorted by (class, sign), up to kTradesPerItem per item
+    std::vector<int4> items;
+    std::vector<int> tr_orig;
+    std::vector<double> tr_K, tr_absN, tr_N, tr_NK;
     if (P.want_trade) {
+        auto sgn = [&](int j) { return pf->notional[j] < 0.0 ? -1 : 1; };
         std::vector<int> order(pf->n_trades);
         for (int j = 0; j < pf->n_trades; ++j) order[j] = j;
-        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return trade_cls[x] < trade_cls[y]; });
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+            return std::make_pair(trade_cls[x], sgn(x)) < std::make_pair(trade_cls[y], sgn(y));
+        });
         for (int i = 0; i < pf->n_trades;) {
-            int c = trade_cls[order[i]];
-            int n = 0;
-            item_cls.push_back(c);
-            item_first.push_back(i);
-            while (i < pf->n_trades && trade_cls[order[i]] == c && n < kTradesPerItem) {
-                tr_orig.push_back(order[i]);
-                tr_N.push_back(pf->notional[order[i]]);
-                tr_NK.push_back(pf->notional[order[i]] * pf->fixed_rate[order[i]]);
+            const int c = trade_cls[order[i]], sg = sgn(order[i]);
+            int4 item = make_int4(c, i, 0, sg);
+            while (i < pf->n_trades && trade_cls[order[i]] == c && sgn(order[i]) == sg && item.z < kTradesPerItem) {
+                const int j = order[i];
+                tr_orig.push_back(j);
+                tr_K.push_back(pf->fixed_rate[j]);
+                tr_absN.push_back(std::fabs(pf->notional[j]));
+                tr_N.push_back(pf->notional[j]);
+                tr_NK.push_back(pf->notional[j] * pf->fixed_rate[j]);
                 ++i;
-                ++n;
+                ++item.z;
             }
-            item_count.push_back(n);
+            items.push_back(item);
         }
-        P.n_items = (int)item_cls.size();
+        P.n_items = (int)items.size();
         if (P.n_items > kValueThreads * kItemsPerThread)
             raise(EQHIP_E_CAPACITY, "per-trade EE supports at most " +
                                         std::to_string(kValueThreads * kItemsPerThread) +
@@ -419,10 +449,12 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
     upload(P.date_slot, date_slot, st);
     upload(P.date_dfirst, date_dfirst, st);
     upload(P.date_det, date_det, st);
-    upload(P.cdesc, cdesc, st);
-    upload(P.cdate, cdate, st);
+    upload(P.fdesc, fdesc, st);
+    upload(P.fam_off, fam_off, st);
     upload(P.flow_d, flow_d, st);
     upload(P.flow_tau, flow_tau, st);
+    upload(P.flow_cls, flow_cls, st);
+    upload(P.cls_mat, cls_mat, st);
     upload(P.fix_off, fix_off, st);
     upload(P.fix_slot, fix_slot, st);
     upload(P.set_off, set_off, st);
@@ -433,9 +465,9 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
     upload(P.qrank, qrank, st);
     upload(P.d_dates, P.dates, st);
     if (P.want_trade) {
-        upload(P.item_cls, item_cls, st);
-        upload(P.item_first, item_first, st);
-        upload(P.item_count, item_count, st);
+        upload(P.items, items, st);
+        upload(P.tr_K, tr_K, st);
+        upload(P.tr_absN, tr_absN, st);
         upload(P.tr_N, tr_N, st);
         upload(P.tr_NK, tr_NK, st);
         upload(P.tr_orig, tr_orig, st);
@@ -469,29 +501,36 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
     P.set_pfe.alloc((size_t)P.S * n_dates * 8);
     P.set_cva.alloc((size_t)P.S * 8);
     if (mc->rng == EQHIP_RNG_HOST_DRAWS) P.Z.alloc((size_t)mc->n_steps * mc->n_paths * 8);
+    build_value_plan(P);
     for (auto& e : P.ev) HIPCHK(hipEventCreate(&e));
 }
 
-void launch_value(eqhip_plan& P, double* path_mtm, double* set_mtm) {
-    ValueArgs g{};
+// Device descriptor of a plan (pointers into the plan's buffers) and its work units.
+void build_value_plan(eqhip_plan& P) {
+    ValuePlan& g = P.vp;
+    g = ValuePlan{};
     g.R = P.R.as<double>();
     g.n_paths = P.mc.n_paths;
     g.n_tiles = P.n_tiles;
     g.n_dates = P.n_dates;
     g.D = P.D;
     g.C = P.C;
+    g.NF = P.NF;
     g.S = P.S;
     g.Fmax = P.Fmax;
     g.n_chunks = P.n_chunks;
     g.tiles_per_block = P.tiles_per_block;
+    g.set_mode = P.set_mode;
+    g.dates = P.d_dates.as<double>();
     g.AB = P.AB.as<double2>();
     g.date_slot = P.date_slot.as<int>();
     g.date_dfirst = P.date_dfirst.as<int>();
     g.date_det = P.date_det.as<int>();
-    g.cdesc = P.cdesc.as<int4>();
-    g.cdate = P.cdate.as<int2>();
+    g.fdesc = P.fdesc.as<int4>();
+    g.fam_off = P.fam_off.as<int>();
     g.flow_d = P.flow_d.as<int>();
     g.flow_tau = P.flow_tau.as<double>();
+    g.flow_cls = P.flow_cls.as<int>();
     g.fix_off = P.fix_off.as<int>();
     g.fix_slot = P.fix_slot.as<int>();
     g.fix_AB = P.fix_AB.as<double2>();
@@ -500,25 +539,35 @@ void launch_value(eqhip_plan& P, double* path_mtm, double* set_mtm) {
     g.set_alpha = P.set_alpha.as<double>();
     g.set_beta = P.set_beta.as<double>();
     g.X = P.X.as<double>();
-    g.set_mtm = set_mtm;
     g.want_trade = P.want_trade;
     g.n_items = P.n_items;
     g.n_trades = P.n_trades;
-    g.item_cls = P.item_cls.as<int>();
-    g.item_first = P.item_first.as<int>();
-    g.item_count = P.item_count.as<int>();
+    g.items = P.items.as<int4>();
+    g.cls_mat = P.cls_mat.as<double>();
+    g.tr_K = P.tr_K.as<double>();
+    g.tr_absN = P.tr_absN.as<double>();
     g.tr_N = P.tr_N.as<double>();
     g.tr_NK = P.tr_NK.as<double>();
     g.tr_orig = P.tr_orig.as<int>();
     g.partial = P.partial.as<double>();
-    g.path_mtm = path_mtm;
-    dim3 grid((unsigned)((uint64_t)P.n_dates * P.n_chunks));
+    P.vplan.alloc(sizeof(ValuePlan));
+    HIPCHK(hipMemcpy(P.vplan.p, &g, sizeof(ValuePlan), hipMemcpyHostToDevice));
+    P.unit_list.clear();
+    for (int k = 0; k < P.n_dates; ++k)
+        for (int c = 0; c < P.n_chunks; ++c) P.unit_list.push_back(make_int4(0, k, c, 0));
+    upload(P.units, P.unit_list, P.ctx->stream);
+}
+
+void launch_value(eqhip_plan& P, double* path_mtm, double* set_mtm) {
     hipStream_t st = P.ctx->stream;
-    switch (P.PT) {
-        case 64: hipLaunchKernelGGL(value_kernel<64>, grid, dim3(kValueThreads), P.lds_bytes, st, g); break;
-        case 32: hipLaunchKernelGGL(value_kernel<32>, grid, dim3(kValueThreads), P.lds_bytes, st, g); break;
-        default: hipLaunchKernelGGL(value_kernel<16>, grid, dim3(kValueThreads), P.lds_bytes, st, g); break;
+    const bool patched = path_mtm || set_mtm || P.vp.path_mtm || P.vp.set_mtm;
+    if (patched) {  // per-run debug outputs live in the descriptor
+        P.vp.path_mtm = path_mtm;
+        P.vp.set_mtm = set_mtm;
+        HIPCHK(hipMemcpy(P.vplan.p, &P.vp, sizeof(ValuePlan), hipMemcpyHostToDevice));
     }
+    hipLaunchKernelGGL(value_kernel, dim3((unsigned)P.unit_list.size()), dim3(kValueThreads), P.lds_bytes, st,
+                       P.vplan.as<const ValuePlan>(), P.units.as<const int4>());
     HIPCHK(hipGetLastError());
 }
 
@@ -617,7 +666,7 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
     P.timing.cva_ms = ms[3];
     P.timing.total_ms = (double)ms[0] + ms[1] + ms[2] + ms[3];
     P.timing.n_value_blocks = (uint64_t)P.n_dates * P.n_chunks;
-    P.timing.paths_per_tile = (uint32_t)P.PT;
+    P.timing.paths_per_tile = (uint32_t)kPathsPerTile;
     P.timing.tiles_per_block = (uint32_t)P.tiles_per_block;
 }
 
@@ -721,7 +770,12 @@ int eqhip_context_create(int device, eqhip_context** out) {
         ctx->device = device;
         HIPCHK(hipSetDevice(device));
         HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-        ctx->lds_limit = std::min<size_t>(65536, prop.sharedMemPerBlock);
+        ctx->lds_limit = prop.sharedMemPerBlock;  // 160 KB per workgroup on gfx950
+        if (hipFuncSetAttribute((const void*)value_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)ctx->lds_limit) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->lds_limit = std::min<size_t>(ctx->lds_limit, 65536);
+        }
         *out = ctx.release();
     });
 }

@@ -80,52 +80,69 @@ __global__ void ab_kernel(const double* tau, int n, double a, double b, double s
 }
 
 // ---------------------------------------------------------------------------------------------
-// Valuation.
+// Valuation. One block = one work unit (plan, date k, chunk of path tiles); one lane = one path.
+// Per tile of 64 paths:
+//   A  P(t_k, T_d) = A exp(-B r) for every live cash-flow date d      -> LDS sP[d][lane]
+//   B  one wave per schedule family: running annuity over the family's live flows; at the last
+//      flow of each class, (float leg, annuity) of that class          -> LDS sC[c][lane]
+//   C  netting sets: V_s = sum_c alpha_sc float_c - beta_sc annuity_c   -> X[s][k][path]
+//   D  trades: x = s (float - K annuity), acc += max(x, 0) over the paths of the chunk
+// Schedule families hold classes whose schedules are prefixes of one another, so one running sum
+// yields every class's annuity, bit-identical to summing that class's flows in order.
 // ---------------------------------------------------------------------------------------------
-template <int PT>
-__global__ void __launch_bounds__(kValueThreads) value_kernel(ValueArgs g) {
+__global__ void __launch_bounds__(kValueThreads) value_kernel(const ValuePlan* plans, const int4* units) {
     extern __shared__ double smem[];
-    double* sR = smem;                    // [PT]
-    double* sP = sR + PT;                 // [D][PT]     P(t_k, T_d) for live d
-    double* sF = sP + (size_t)g.D * PT;   // [Fmax][PT]  fixing bonds P(T_{k-1}, T_k | r_fix)
-    double2* sC = (double2*)(sF + (size_t)g.Fmax * PT);  // [C][PT]  (float, annuity)
-
+    const int4 u = units[blockIdx.x];
+    const ValuePlan& g = plans[u.x];
+    const int k = u.y;
+    const int chunk = u.z;
     const int tid = threadIdx.x;
-    const int k = blockIdx.x / g.n_chunks;
-    const int chunk = blockIdx.x - k * g.n_chunks;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+
+    double* sR = smem;                                     // [64]
+    double* sP = sR + kPathsPerTile;                       // [D][64]
+    double* sF = sP + (size_t)g.D * kPathsPerTile;         // [Fmax][64]
+    double2* sC = (double2*)(sF + (size_t)g.Fmax * kPathsPerTile);  // [C][64]
+    double* sPart = (double*)(sC + (size_t)g.C * kPathsPerTile);    // [S][waves][64] (split mode)
+
+    const double t = g.dates[k];
     const int dfirst = g.date_dfirst[k];
     const int n_live = g.D - dfirst;
     const int fbeg = g.fix_off[k];
     const int nfix = g.fix_off[k + 1] - fbeg;
     const int det = g.date_det[k];
-    const uint64_t slot_base = (uint64_t)g.date_slot[k] * g.n_paths;
+    const uint64_t n_paths = g.n_paths;
+    const double* Rk = g.R + (uint64_t)g.date_slot[k] * n_paths;
     const double2* ab = g.AB + (size_t)k * g.D;
-    const int4* cdesc = g.cdesc + (size_t)k * g.C;
-    const int2* cdate = g.cdate + (size_t)k * g.C;
+    const int4* fdesc = g.fdesc + (size_t)k * g.NF;
 
-    // per-trade accumulators: thread owns items tid + i*blockDim (i < kItemsPerThread), each up
-    // to kTradesPerItem trades of one class
+    // trade work items owned by this thread
     double acc[kItemsPerThread][kTradesPerItem];
-    double tN[kItemsPerThread][kTradesPerItem];
-    double tNK[kItemsPerThread][kTradesPerItem];
-    int icls[kItemsPerThread], icnt[kItemsPerThread];
+    double tK[kItemsPerThread][kTradesPerItem];
+    int icls[kItemsPerThread], icnt[kItemsPerThread], ifirst[kItemsPerThread];
+    double isgn[kItemsPerThread];
 #pragma unroll
     for (int i = 0; i < kItemsPerThread; ++i) {
-        int it = tid + i * kValueThreads;
+        const int it = tid + i * kValueThreads;
         icnt[i] = 0;
         icls[i] = 0;
+        ifirst[i] = 0;
+        isgn[i] = 1.0;
         if (g.want_trade && it < g.n_items) {
-            icls[i] = g.item_cls[it];
-            icnt[i] = g.item_count[it];
-            int f = g.item_first[it];
-#pragma unroll
-            for (int q = 0; q < kTradesPerItem; ++q) {
-                tN[i][q] = (q < icnt[i]) ? g.tr_N[f + q] : 0.0;
-                tNK[i][q] = (q < icnt[i]) ? g.tr_NK[f + q] : 0.0;
-            }
+            int4 item = g.items[it];
+            icls[i] = item.x;
+            ifirst[i] = item.y;
+            isgn[i] = (double)item.w;
+            // the class is alive at t when its last payment is still to come
+            icnt[i] = (g.cls_mat[item.x] > t + kDateTol) ? item.z : 0;
+            if (g.path_mtm) icnt[i] = item.z;
         }
 #pragma unroll
-        for (int q = 0; q < kTradesPerItem; ++q) acc[i][q] = 0.0;
+        for (int q = 0; q < kTradesPerItem; ++q) {
+            acc[i][q] = 0.0;
+            tK[i][q] = (q < icnt[i]) ? g.tr_K[ifirst[i] + q] : 0.0;
+        }
     }
 
     const uint64_t tile0 = (uint64_t)chunk * g.tiles_per_block;
@@ -133,88 +150,117 @@ __global__ void __launch_bounds__(kValueThreads) value_kernel(ValueArgs g) {
     if (tile_end > g.n_tiles) tile_end = g.n_tiles;
 
     for (uint64_t tile = tile0; tile < tile_end; ++tile) {
-        const uint64_t p0 = tile * PT;
-        __syncthreads();  // previous tile's readers are done with the LDS
-        if (tid < PT) {
-            uint64_t gp = p0 + tid;
-            sR[tid] = (gp < g.n_paths) ? g.R[slot_base + gp] : 0.0;
-        }
-        // fixing bonds of the periods in progress
-        for (int idx = tid; idx < nfix * PT; idx += kValueThreads) {
-            int f = idx / PT, p = idx - f * PT;
-            uint64_t gp = p0 + p;
-            double rf = (gp < g.n_paths) ? g.R[(uint64_t)g.fix_slot[fbeg + f] * g.n_paths + gp] : 0.0;
+        const uint64_t p0 = tile * kPathsPerTile;
+        const uint64_t gp = p0 + lane;
+        const bool valid = gp < n_paths;
+        __syncthreads();
+        if (tid < kPathsPerTile) sR[lane] = valid ? Rk[gp] : 0.0;
+        for (int f = wave; f < nfix; f += kValueWaves) {
+            double rf = valid ? g.R[(uint64_t)g.fix_slot[fbeg + f] * n_paths + gp] : 0.0;
             double2 fab = g.fix_AB[fbeg + f];
-            sF[f * PT + p] = fab.x * exp(-(fab.y * rf));
+            sF[f * kPathsPerTile + lane] = fab.x * exp(-(fab.y * rf));
         }
         __syncthreads();
-        // P(t_k, T_d) for every live cash-flow date
-        for (int idx = tid; idx < n_live * PT; idx += kValueThreads) {
-            int dl = idx / PT, p = idx - dl * PT;
-            int d = dfirst + dl;
-            double2 v = ab[d];
-            sP[d * PT + p] = v.x * exp(-(v.y * sR[p]));
+        // A: zero-coupon bonds of the live cash-flow dates
+        {
+            const double r = sR[lane];
+            for (int dl = wave; dl < n_live; dl += kValueWaves) {
+                const int d = dfirst + dl;
+                const double2 v = ab[d];
+                sP[d * kPathsPerTile + lane] = v.x * exp(-(v.y * r));
+            }
         }
         __syncthreads();
-        // schedule classes: floating-leg factor and fixed-leg annuity
-        for (int idx = tid; idx < g.C * PT; idx += kValueThreads) {
-            int c = idx / PT, p = idx - c * PT;
-            int4 cd = cdesc[c];  // x mode, y first flow, z end flow, w fixing index
-            double fl = 0.0, ann = 0.0;
-            if (cd.x != 0) {
-                for (int f = cd.y; f < cd.z; ++f) ann += sP[g.flow_d[f] * PT + p] * g.flow_tau[f];
-                int2 dd = cdate[c];  // x start/next date, y maturity date
-                double st;
-                if (cd.x == 1) st = 1.0;
-                else if (cd.x == 2) st = sP[dd.x * PT + p];
-                else st = sP[dd.x * PT + p] / sF[cd.w * PT + p];
-                fl = st - sP[dd.y * PT + p];
+        // B: schedule families
+        for (int F = wave; F < g.NF; F += kValueWaves) {
+            const int4 fd = fdesc[F];
+            const int f0 = g.fam_off[F];
+            const int f1 = g.fam_off[F + 1];
+            const int first = (fd.x == 0) ? f1 : f0 + fd.y;
+            for (int i = f0; i < first; ++i) {  // classes already matured
+                const int c = g.flow_cls[i];
+                if (c >= 0) sC[c * kPathsPerTile + lane] = make_double2(0.0, 0.0);
             }
-            sC[c * PT + p] = make_double2(fl, ann);
+            if (fd.x == 0) continue;
+            double st;
+            if (fd.x == 1) st = 1.0;
+            else if (fd.x == 2) st = sP[fd.z * kPathsPerTile + lane];
+            else st = sP[fd.z * kPathsPerTile + lane] / sF[fd.w * kPathsPerTile + lane];
+            double ann = 0.0;
+            for (int i = first; i < f1; ++i) {
+                const double P = sP[g.flow_d[i] * kPathsPerTile + lane];
+                ann += P * g.flow_tau[i];
+                const int c = g.flow_cls[i];
+                if (c >= 0) sC[c * kPathsPerTile + lane] = make_double2(st - P, ann);
+            }
         }
         __syncthreads();
-        // netting sets: V_s = sum_c alpha_sc * float_c - beta_sc * annuity_c
-        for (int idx = tid; idx < g.S * PT; idx += kValueThreads) {
-            int s = idx / PT, p = idx - s * PT;
-            uint64_t gp = p0 + p;
-            double v = 0.0;
-            for (int e = g.set_off[s]; e < g.set_off[s + 1]; ++e) {
-                double2 c = sC[g.set_cls[e] * PT + p];
-                v += g.set_alpha[e] * c.x - g.set_beta[e] * c.y;
+        // C: netting sets
+        if (g.set_mode == 0) {
+            for (int s = 0; s < g.S; ++s) {
+                const int beg = g.set_off[s], end = g.set_off[s + 1];
+                const int per = (end - beg + kValueWaves - 1) / kValueWaves;
+                const int cb = beg + wave * per;
+                const int ce = min(end, cb + per);
+                double v = 0.0;
+                for (int e = cb; e < ce; ++e) {
+                    const double2 c2 = sC[g.set_cls[e] * kPathsPerTile + lane];
+                    v += g.set_alpha[e] * c2.x - g.set_beta[e] * c2.y;
+                }
+                sPart[(s * kValueWaves + wave) * kPathsPerTile + lane] = v;
             }
-            if (gp < g.n_paths) {
-                g.X[((uint64_t)s * g.n_dates + k) * g.n_paths + gp] = (v > 0.0) ? v : 0.0;
-                if (g.set_mtm) g.set_mtm[((uint64_t)k * g.S + s) * g.n_paths + gp] = v;
+        } else {
+            for (int s = wave; s < g.S; s += kValueWaves) {
+                double v = 0.0;
+                for (int e = g.set_off[s]; e < g.set_off[s + 1]; ++e) {
+                    const double2 c2 = sC[g.set_cls[e] * kPathsPerTile + lane];
+                    v += g.set_alpha[e] * c2.x - g.set_beta[e] * c2.y;
+                }
+                if (valid) {
+                    g.X[((uint64_t)s * g.n_dates + k) * n_paths + gp] = (v > 0.0) ? v : 0.0;
+                    if (g.set_mtm) g.set_mtm[((uint64_t)k * g.S + s) * n_paths + gp] = v;
+                }
             }
         }
-        // per-trade EE partial sums (no barrier needed: reads sC only)
+        // D: per-trade exposure sums (reads sC only)
         if (g.want_trade) {
-            uint64_t np_tile = g.n_paths - p0;
-            const int pvalid = (np_tile < (uint64_t)PT) ? (int)np_tile : PT;
-            int pend = pvalid;
-            if (det) pend = (p0 == 0) ? 1 : 0;  // date at grid index 0: EE over path 0 only
+            const uint64_t left = n_paths - p0;
+            const int pvalid = (left < (uint64_t)kPathsPerTile) ? (int)left : kPathsPerTile;
+            const int pend = det ? ((p0 == 0) ? 1 : 0) : pvalid;  // grid index 0: path 0 only
 #pragma unroll
             for (int i = 0; i < kItemsPerThread; ++i) {
                 if (icnt[i] == 0) continue;
-                const double2* row = sC + icls[i] * PT;
+                const double2* row = sC + icls[i] * kPathsPerTile;
+                const double sg = isgn[i];
                 for (int p = 0; p < pend; ++p) {
-                    double2 c = row[p];
+                    const double2 c2 = row[p];
+                    const double sfl = sg * c2.x;
+                    const double sann = sg * c2.y;
 #pragma unroll
                     for (int q = 0; q < kTradesPerItem; ++q) {
-                        double v = tN[i][q] * c.x - tNK[i][q] * c.y;
-                        acc[i][q] += (v > 0.0) ? v : 0.0;
+                        const double x = fma(-tK[i][q], sann, sfl);
+                        acc[i][q] += (x > 0.0) ? x : 0.0;
                     }
                 }
                 if (g.path_mtm) {
-                    int f = g.item_first[tid + i * kValueThreads];
                     for (int q = 0; q < icnt[i]; ++q) {
-                        int orig = g.tr_orig[f + q];
-                        for (int p = 0; p < pvalid; ++p) {
-                            double2 c = row[p];
-                            g.path_mtm[((uint64_t)k * g.n_trades + orig) * g.n_paths + p0 + p] =
-                                tN[i][q] * c.x - tNK[i][q] * c.y;
-                        }
+                        const int j = ifirst[i] + q;
+                        const double N = g.tr_N[j], NK = g.tr_NK[j];
+                        double* dst = g.path_mtm + ((uint64_t)k * g.n_trades + g.tr_orig[j]) * n_paths + p0;
+                        for (int p = 0; p < pvalid; ++p) dst[p] = N * row[p].x - NK * row[p].y;
                     }
+                }
+            }
+        }
+        if (g.set_mode == 0) {
+            __syncthreads();
+            if (tid < g.S * kPathsPerTile) {
+                const int s = tid >> 6;
+                double v = 0.0;
+                for (int w = 0; w < kValueWaves; ++w) v += sPart[(s * kValueWaves + w) * kPathsPerTile + lane];
+                if (valid) {
+                    g.X[((uint64_t)s * g.n_dates + k) * n_paths + gp] = (v > 0.0) ? v : 0.0;
+                    if (g.set_mtm) g.set_mtm[((uint64_t)k * g.S + s) * n_paths + gp] = v;
                 }
             }
         }
@@ -222,22 +268,21 @@ __global__ void __launch_bounds__(kValueThreads) value_kernel(ValueArgs g) {
     if (g.want_trade) {
 #pragma unroll
         for (int i = 0; i < kItemsPerThread; ++i) {
-            if (icnt[i] == 0) continue;
-            int f = g.item_first[tid + i * kValueThreads];
+            const int it = tid + i * kValueThreads;
+            if (!(it < g.n_items)) continue;
+            const int4 item = g.items[it];
 #pragma unroll
             for (int q = 0; q < kTradesPerItem; ++q) {
-                if (q < icnt[i]) {
-                    int orig = g.tr_orig[f + q];
-                    g.partial[((uint64_t)orig * g.n_dates + k) * g.n_chunks + chunk] = acc[i][q];
+                if (q < item.z) {
+                    const int j = item.y + q;
+                    // a matured class contributes exact zeros
+                    const double e = (icnt[i] == 0) ? 0.0 : g.tr_absN[j] * acc[i][q];
+                    g.partial[((uint64_t)g.tr_orig[j] * g.n_dates + k) * g.n_chunks + chunk] = e;
                 }
             }
         }
     }
 }
-
-template __global__ void value_kernel<16>(ValueArgs);
-template __global__ void value_kernel<32>(ValueArgs);
-template __global__ void value_kernel<64>(ValueArgs);
 
 // ---------------------------------------------------------------------------------------------
 // EE + PFE95 of one (set, date) row. EE = (fixed-order sum) / n. PFE = the element of rank

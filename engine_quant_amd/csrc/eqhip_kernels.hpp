@@ -6,9 +6,12 @@
 
 namespace eqhip {
 
-constexpr int kValueThreads = 256;   // 4 wave64s per valuation block
-constexpr int kItemsPerThread = 4;   // trade work items per thread in the valuation block
-constexpr int kTradesPerItem = 4;    // trades of one schedule class per work item
+constexpr int kValueThreads = 512;   // 8 wave64s per valuation block
+constexpr int kValueWaves = kValueThreads / 64;
+constexpr int kPathsPerTile = 64;    // one path per lane
+constexpr int kItemsPerThread = 2;   // trade work items per thread in the valuation block
+constexpr int kTradesPerItem = 4;    // trades of one schedule class and sign per work item
+constexpr int kMaxSplitSets = 8;     // netting sets whose class sums are split across the waves
 constexpr int kSelectThreads = 256;  // EE / PFE select block
 constexpr double kDateTolHost = 1e-9; // date matching tolerance, products/irs.rs:151-154,173
 
@@ -24,38 +27,44 @@ struct SimArgs {
     double* R;                   // [n_slots][n_paths]
 };
 
-struct ValueArgs {
-    const double* R;
+// One exposure plan as the valuation kernel sees it. A launch takes an array of plans and a list
+// of work units (plan, date, path chunk), so one launch can serve many netting-set plans.
+struct ValuePlan {
+    const double* R;          // [n_slots][n_paths]
     uint64_t n_paths;
-    uint64_t n_tiles;
-    int n_dates, D, C, S, Fmax;
+    uint64_t n_tiles;         // ceil(n_paths / kPathsPerTile)
+    int n_dates, D, C, NF, S, Fmax;
     int n_chunks, tiles_per_block;
+    int set_mode;             // 0: S <= kMaxSplitSets, class sums split across waves; 1: one wave per set
+    const double* dates;      // [n_dates]
     const double2* AB;        // [n_dates][D] (A, B) of P(t_k, T_d)
     const int* date_slot;     // [n_dates]
     const int* date_dfirst;   // [n_dates] first live cash-flow date
     const int* date_det;      // [n_dates] 1 when the date sits at grid index 0
-    const int4* cdesc;        // [n_dates][C] mode, first flow, end flow, fixing index
-    const int2* cdate;        // [n_dates][C] start/next date, maturity date
-    const int* flow_d;        // class flows: cash-flow date index
-    const double* flow_tau;   // class flows: accrual
+    const int4* fdesc;        // [n_dates][NF] mode (0 dead, 1 reset, 2 forward, 3 mid), first live flow,
+                              //   start/next date, fixing index
+    const int* fam_off;       // [NF + 1] family flow ranges
+    const int* flow_d;        // family flows: cash-flow date index
+    const double* flow_tau;   //               accrual
+    const int* flow_cls;      //               class whose last flow this is, or -1
     const int* fix_off;       // [n_dates + 1]
     const int* fix_slot;      // R slot of the fixing date
     const double2* fix_AB;    // (A, B) of the fixing period
-    const int* set_off;       // [S + 1]
+    const int* set_off;       // [S + 1] CSR of (class, alpha, beta)
     const int* set_cls;
     const double* set_alpha;  // sum of notionals of the set's trades in the class
     const double* set_beta;   // sum of notional * fixed rate
     double* X;                // [S][n_dates][n_paths] max(V_set, 0)
     double* set_mtm;          // optional [n_dates][S][n_paths]
-    int want_trade;
-    int n_items, n_trades;
-    const int* item_cls;
-    const int* item_first;
-    const int* item_count;
-    const double* tr_N;       // trades sorted by class
+    int want_trade, n_items, n_trades, pad_;
+    const int4* items;        // (class, first sorted trade, count, sign)
+    const double* cls_mat;    // [C] maturity of the class
+    const double* tr_K;       // sorted trades: fixed rate, |notional|, notional, notional * rate
+    const double* tr_absN;
+    const double* tr_N;
     const double* tr_NK;
-    const int* tr_orig;
-    double* partial;          // [n_trades][n_dates][n_chunks]
+    const int* tr_orig;       // sorted -> caller trade index
+    double* partial;          // [n_trades][n_dates][n_chunks] per-trade EE partial sums
     double* path_mtm;         // optional [n_dates][n_trades][n_paths]
 };
 
@@ -80,8 +89,7 @@ struct PvArgs {
 template <int DISC, int RNG>
 __global__ void simulate_kernel(SimArgs g);
 __global__ void ab_kernel(const double* tau, int n, double a, double b, double sigma, double2* out);
-template <int PT>
-__global__ void value_kernel(ValueArgs g);
+__global__ void value_kernel(const ValuePlan* plans, const int4* units);
 __global__ void select_kernel(const double* X, int n_dates, uint64_t n_paths, const uint64_t* n_eff,
                               const uint64_t* q_rank, double* ee, double* pfe);
 __global__ void trade_ee_kernel(const double* partial, int n_trades, int n_dates, int n_chunks,
