@@ -39,7 +39,9 @@ def _run(cmd):
 
 def build_eqhip() -> str:
     srcs = [os.path.join(CSRC, s) for s in EQHIP_SOURCES]
-    _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", f"-I{CSRC}", *srcs, "-o", LIBEQHIP])
+    tmp = LIBEQHIP + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", f"-I{CSRC}", *srcs, "-o", tmp])
+    os.replace(tmp, LIBEQHIP)  # atomic: a concurrent reader never sees a partial library
     return LIBEQHIP
 
 
@@ -57,7 +59,8 @@ def build_engine() -> str:
     py_inc = sysconfig.get_paths()["include"]
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-fvisibility=hidden",
           f"-I{INCLUDE}", f"-I{CSRC}", f"-I{py_inc}", f"-I{pybind11.get_include()}", *srcs,
-          f"-L{PKG}", "-leqhip", "-Wl,-rpath,$ORIGIN", "-o", out])
+          f"-L{PKG}", "-leqhip", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
     return out
 
 

@@ -130,14 +130,24 @@ struct eqhip_plan {
     int want_trade = 0;
     int n_items = 0;
     int set_mode = 0;
+    uint64_t target_units = 2048;   // work units the launch aims for (chunking of the path tiles)
+    const double* r_shared = nullptr;  // batch-owned short-rate buffer (all grid steps), or null
+    uint64_t r_stride = 0;
     std::vector<double> dates;
     size_t lds_bytes = 0;
 
     DevBuf slot_of_step, R, Z, AB, date_slot, date_dfirst, date_det, fdesc, fam_off, flow_d, flow_tau,
         flow_cls, fix_off, fix_slot, fix_AB, set_off, set_cls, set_alpha, set_beta, X, items, cls_mat, tr_K,
         tr_absN, tr_N, tr_NK, tr_orig, partial, trade_ee, trade_cva, set_ee, set_pfe, set_cva, neff, qrank,
-        d_dates, vplan, units;
+        d_dates, vplan, units, rows;
     ValuePlan vp{};  // host copy of the device descriptor (per-run pointers patched at launch)
+    // host copies of the plan tables (used by the adjoint)
+    std::vector<double> h_T;
+    std::vector<int4> h_fdesc;
+    std::vector<int> h_fam_off, h_flow_d, h_flow_cls, h_set_off, h_set_cls, h_cls_fam, h_cls_len, h_date_dfirst,
+        h_fix_off, h_fix_slot, h_date_grid, h_slot_grid;
+    std::vector<double> h_flow_tau, h_set_alpha, h_set_beta, h_fix_tau;
+    std::vector<int64_t> h_fix_grid;  // grid step of each fixing entry
     std::vector<int4> unit_list;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     eqhip_timing timing{};
@@ -194,7 +204,7 @@ void choose_launch(eqhip_plan& P) {
     P.lds_bytes = lds;
     P.n_tiles = (P.mc.n_paths + kPathsPerTile - 1) / kPathsPerTile;
     // enough units for ~8 blocks per CU (one 512-thread block per CU at this LDS footprint)
-    uint64_t target_chunks = std::max<uint64_t>(1, (2048 + P.n_dates - 1) / P.n_dates);
+    uint64_t target_chunks = std::max<uint64_t>(1, (P.target_units + P.n_dates - 1) / P.n_dates);
     uint64_t tpb = std::max<uint64_t>(1, (P.n_tiles + target_chunks - 1) / target_chunks);
     P.tiles_per_block = (int)tpb;
     P.n_chunks = (int)((P.n_tiles + tpb - 1) / tpb);
@@ -202,8 +212,10 @@ void choose_launch(eqhip_plan& P) {
 
 void build_value_plan(eqhip_plan& P);
 
+// all_steps: r is stored at every grid step (slot = step), so plans of a batch can share one
+// simulation; the batch then owns the short-rate buffer.
 void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfolio* pf,
-                const double* dates, int n_dates, const eqhip_mc* mc, int want_trade) {
+                const double* dates, int n_dates, const eqhip_mc* mc, int want_trade, bool all_steps = false) {
     if (!model || !mc) raise(EQHIP_E_INVALID, "model / mc is null");
     if (!(model->a != 0.0)) raise(EQHIP_E_INVALID, "HullWhite1F: a must be non-zero");
     validate_portfolio(pf);
@@ -306,6 +318,8 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
 
     // slots: grid indices at which r is stored
     std::map<int64_t, int> slot_of;
+    if (all_steps)
+        for (int64_t g = 0; g <= (int64_t)steps; ++g) slot_of.emplace(g, (int)g);
     auto slot = [&](int64_t g) {
         auto it = slot_of.find(g);
         if (it == slot_of.end()) it = slot_of.emplace(g, (int)slot_of.size()).first;
@@ -437,7 +451,7 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
     // memory check for the large buffers
     size_t free_b = 0, total_b = 0;
     HIPCHK(hipMemGetInfo(&free_b, &total_b));
-    double need = 8.0 * ((double)P.n_slots * mc->n_paths + (double)P.S * n_dates * mc->n_paths +
+    double need = 8.0 * ((all_steps ? 0.0 : (double)P.n_slots * mc->n_paths) + (double)P.S * n_dates * mc->n_paths +
                          (P.want_trade ? (double)P.n_trades * n_dates * P.n_chunks : 0.0) +
                          (mc->rng == EQHIP_RNG_HOST_DRAWS ? (double)mc->n_steps * mc->n_paths : 0.0));
     if (need > 0.9 * (double)free_b)
@@ -495,12 +509,32 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
         }
         HIPCHK(hipStreamSynchronize(st));  // tau buffers die here
     }
-    P.R.alloc((size_t)P.n_slots * mc->n_paths * 8);
+    if (!all_steps) P.R.alloc((size_t)P.n_slots * mc->n_paths * 8);
     P.X.alloc((size_t)P.S * n_dates * mc->n_paths * 8);
     P.set_ee.alloc((size_t)P.S * n_dates * 8);
     P.set_pfe.alloc((size_t)P.S * n_dates * 8);
     P.set_cva.alloc((size_t)P.S * 8);
     if (mc->rng == EQHIP_RNG_HOST_DRAWS) P.Z.alloc((size_t)mc->n_steps * mc->n_paths * 8);
+    P.h_T = T;
+    P.h_fdesc = fdesc;
+    P.h_fam_off = fam_off;
+    P.h_flow_d = flow_d;
+    P.h_flow_cls = flow_cls;
+    P.h_flow_tau = flow_tau;
+    P.h_set_off = set_off;
+    P.h_set_cls = set_cls;
+    P.h_set_alpha = set_alpha;
+    P.h_set_beta = set_beta;
+    P.h_cls_fam = cls_fam;
+    P.h_cls_len = cls_len;
+    P.h_date_dfirst = date_dfirst;
+    P.h_fix_off = fix_off;
+    P.h_fix_slot = fix_slot;
+    P.h_fix_tau = fix_tau;
+    P.h_date_grid.resize(n_dates);
+    for (int k = 0; k < n_dates; ++k) P.h_date_grid[k] = (int)grid_of(P.dates[k]);
+    P.h_slot_grid.assign(P.n_slots, 0);
+    for (auto& kv : slot_of) P.h_slot_grid[kv.second] = (int)kv.first;
     build_value_plan(P);
     for (auto& e : P.ev) HIPCHK(hipEventCreate(&e));
 }
@@ -509,7 +543,8 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
 void build_value_plan(eqhip_plan& P) {
     ValuePlan& g = P.vp;
     g = ValuePlan{};
-    g.R = P.R.as<double>();
+    g.R = P.r_shared ? P.r_shared : P.R.as<double>();
+    g.r_stride = P.r_shared ? P.r_stride : P.mc.n_paths;
     g.n_paths = P.mc.n_paths;
     g.n_tiles = P.n_tiles;
     g.n_dates = P.n_dates;
@@ -556,6 +591,17 @@ void build_value_plan(eqhip_plan& P) {
     for (int k = 0; k < P.n_dates; ++k)
         for (int c = 0; c < P.n_chunks; ++c) P.unit_list.push_back(make_int4(0, k, c, 0));
     upload(P.units, P.unit_list, P.ctx->stream);
+    std::vector<uint64_t> neff(P.n_dates), qrank(P.n_dates);
+    HIPCHK(hipMemcpy(neff.data(), P.neff.p, P.n_dates * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(qrank.data(), P.qrank.p, P.n_dates * 8, hipMemcpyDeviceToHost));
+    std::vector<SelectRow> rows;
+    for (int s = 0; s < P.S; ++s)
+        for (int k = 0; k < P.n_dates; ++k) {
+            const size_t r = (size_t)s * P.n_dates + k;
+            rows.push_back(SelectRow{P.X.as<double>() + r * P.mc.n_paths, neff[k], qrank[k],
+                                     P.set_ee.as<double>() + r, P.set_pfe.as<double>() + r});
+        }
+    upload(P.rows, rows, P.ctx->stream);
 }
 
 void launch_value(eqhip_plan& P, double* path_mtm, double* set_mtm) {
@@ -621,8 +667,7 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
     launch_value(P, dpath.as<double>(), dset.as<double>());
     HIPCHK(hipEventRecord(P.ev[2], st));
     hipLaunchKernelGGL(select_kernel, dim3((unsigned)(P.S * P.n_dates)), dim3(kSelectThreads), 0, st,
-                       P.X.as<double>(), P.n_dates, np, P.neff.as<uint64_t>(), P.qrank.as<uint64_t>(),
-                       P.set_ee.as<double>(), P.set_pfe.as<double>());
+                       P.rows.as<const SelectRow>());
     HIPCHK(hipGetLastError());
     if (P.want_trade) {
         int n = P.n_trades * P.n_dates;
@@ -723,6 +768,487 @@ void profile_batch(eqhip_context* ctx, double a, double b, double sigma, double 
         }
     for (int k = 0; k < nd; ++k) out_times[k] = times[k];
     if (n_out) *n_out = nd;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// Pathwise adjoint of one netting set's CVA. V_s at date k is written as
+//   V_s = C0_k + sum_d nu_{k,d} P(t_k, T_d) + sum_{families mid-period} alpha_F P(t_k,T_next)/P_fix
+// (reset floating legs give the constant, forward starts and maturities and annuity flows give nu);
+// the adjoint kernel differentiates that per path and sweeps the short rate backwards.
+// ---------------------------------------------------------------------------------------------
+void sens_table(eqhip_context* ctx, const std::vector<double>& tau, const eqhip_model_hw1f& m, DevBuf& out) {
+    DevBuf dt;
+    upload(dt, tau, ctx->stream);
+    out.alloc(std::max<size_t>(1, tau.size()) * sizeof(ZcbSens));
+    int n = (int)tau.size();
+    if (n > 0) {
+        hipLaunchKernelGGL(zcb_sens_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, dt.as<double>(), n,
+                           m.a, m.b, m.sigma, out.as<ZcbSens>());
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+}
+
+void cva_adjoint(eqhip_plan& P, const double* host_draws, int set, const eqhip_credit* credit, double* out_cva,
+                 double* out_grad, double* out_check) {
+    if (set < 0 || set >= P.S) raise(EQHIP_E_INVALID, "set index out of range");
+    if (!credit || !out_cva || !out_grad) raise(EQHIP_E_INVALID, "credit / out_cva / out_grad are required");
+    if (P.det) raise(EQHIP_E_INVALID, "the adjoint needs at least one monitoring date after t = 0");
+    const int nd = P.n_dates, D = P.D;
+    const uint32_t steps = P.mc.n_steps;
+    std::vector<int> date_of_step(steps + 1, -1);
+    for (int k = 0; k < nd; ++k) {
+        int g = P.h_date_grid[k];
+        if (date_of_step[g] >= 0) raise(EQHIP_E_INVALID, "the adjoint needs one grid step per monitoring date");
+        date_of_step[g] = k;
+    }
+    // forward exposure run: EE of every set (also fills R / X on the device)
+    std::vector<double> ee((size_t)P.S * nd), pfe((size_t)P.S * nd), cva(P.S);
+    eqhip_outputs o{};
+    o.set_ee = ee.data();
+    o.set_pfe = pfe.data();
+    o.set_cva = cva.data();
+    execute_plan(P, host_draws, credit, &o);
+
+    // per-date linear form of V_s
+    std::vector<double> C0(nd, 0.0), nu((size_t)nd * D, 0.0), mid_alpha;
+    std::vector<int> mid_off(nd + 1, 0);
+    std::vector<int4> mid;
+    std::vector<double> fix_tau_adj;
+    std::map<int64_t, int> gslot;  // fixing grid step -> G row
+    std::vector<int64_t> cp_rows;  // extra R rows (fixing steps off the checkpoint grid)
+    const int K = kAdjCheckpoint;
+    const int n_cp = (int)(steps / K) + 1;
+    auto rrow = [&](int64_t step) -> int {  // row of the forward buffer holding r at this step
+        if (step % K == 0) return (int)(step / K);
+        auto it = std::find(cp_rows.begin(), cp_rows.end(), step);
+        if (it == cp_rows.end()) {
+            cp_rows.push_back(step);
+            return n_cp + (int)cp_rows.size() - 1;
+        }
+        return n_cp + (int)(it - cp_rows.begin());
+    };
+    std::vector<double> cls_alpha(P.C, 0.0), cls_beta(P.C, 0.0);
+    for (int e = P.h_set_off[set]; e < P.h_set_off[set + 1]; ++e) {
+        cls_alpha[P.h_set_cls[e]] = P.h_set_alpha[e];
+        cls_beta[P.h_set_cls[e]] = P.h_set_beta[e];
+    }
+    for (int k = 0; k < nd; ++k) {
+        std::map<std::pair<int, int>, int> mid_here;  // (family) -> index
+        for (int F = 0; F < P.NF; ++F) {
+            const int4 fd = P.h_fdesc[(size_t)k * P.NF + F];
+            if (fd.x == 0) continue;
+            const int f0 = P.h_fam_off[F], f1 = P.h_fam_off[F + 1];
+            double alpha_mid = 0.0;
+            (void)f1;
+            for (int c = 0; c < P.C; ++c) {
+                if (P.h_cls_fam[c] != F || (cls_alpha[c] == 0.0 && cls_beta[c] == 0.0)) continue;
+                const int last = P.h_cls_len[c] - 1;
+                if (last < fd.y) continue;  // matured
+                const double al = cls_alpha[c], be = cls_beta[c];
+                nu[(size_t)k * D + P.h_flow_d[f0 + last]] -= al;
+                for (int i = fd.y; i <= last; ++i) nu[(size_t)k * D + P.h_flow_d[f0 + i]] -= be * P.h_flow_tau[f0 + i];
+                if (fd.x == 1) C0[k] += al;
+                else if (fd.x == 2) nu[(size_t)k * D + fd.z] += al;
+                else alpha_mid += al;
+            }
+            if (fd.x == 3 && alpha_mid != 0.0) {
+                const int fe = P.h_fix_off[k] + fd.w;              // fixing entry of this date
+                const int64_t fstep = P.h_slot_grid[P.h_fix_slot[fe]];
+                auto it = gslot.find(fstep);
+                if (it == gslot.end()) it = gslot.emplace(fstep, (int)gslot.size()).first;
+                mid.push_back(make_int4(fd.z, rrow(fstep), (int)fix_tau_adj.size(), it->second));
+                fix_tau_adj.push_back(P.h_fix_tau[fe]);
+                mid_alpha.push_back(alpha_mid);
+            }
+        }
+        mid_off[k + 1] = (int)mid.size();
+    }
+    // mid entries address R rows through .y and G rows through .w; the kernel reads .y for r_fix
+    std::vector<int> fixslot_of_step(steps + 1, -1);
+    for (auto& kv : gslot) fixslot_of_step[(size_t)kv.first] = kv.second;
+    const int n_rows = n_cp + (int)cp_rows.size();
+
+    // weights w_k = (1-R)(S_{k-1}-S_k) P(0,t_k) / n_eff(k) and the direct CVA terms
+    std::vector<double> tk(P.dates);
+    DevBuf d_sens0;
+    sens_table(P.ctx, tk, P.model, d_sens0);
+    std::vector<ZcbSens> s0(nd);
+    HIPCHK(hipMemcpy(s0.data(), d_sens0.p, nd * sizeof(ZcbSens), hipMemcpyDeviceToHost));
+    const double h = credit->hazard_rate, R = credit->recovery_rate, r0 = P.model.r0;
+    std::vector<double> wk(nd), neffv(nd);
+    double cva_s = 0.0, dcva[6] = {0, 0, 0, 0, 0, 0};
+    double prev_s = 1.0, prev_t = 0.0;
+    for (int k = 0; k < nd; ++k) {
+        const double t = P.dates[k];
+        const double S = std::exp(-h * t);
+        const double dp = prev_s - S;
+        const ZcbSens z = s0[k];
+        const double P0 = z.A * std::exp(-(z.B * r0));
+        const double e = ee[(size_t)set * nd + k];
+        const double n_eff = (P.h_date_grid[k] == 0) ? 1.0 : (double)P.mc.n_paths;
+        wk[k] = (1.0 - R) * dp * P0 / n_eff;
+        cva_s += (1.0 - R) * e * dp * P0;
+        const double c = (1.0 - R) * dp * P0;
+        dcva[0] += c * (z.lnA_a - r0 * z.B_a) * e;
+        dcva[1] += c * z.lnA_b * e;
+        dcva[2] += c * z.lnA_s * e;
+        dcva[3] += c * (-z.B) * e;
+        dcva[4] += (1.0 - R) * e * P0 * (-prev_t * prev_s + t * S);
+        dcva[5] += -e * dp * P0;
+        prev_s = S;
+        prev_t = t;
+    }
+
+    // device tables
+    std::vector<double> tau((size_t)nd * D, -1.0);
+    for (int k = 0; k < nd; ++k)
+        for (int d = P.h_date_dfirst[k]; d < D; ++d) tau[(size_t)k * D + d] = P.h_T[d] - P.dates[k];
+    DevBuf d_zs, d_fzs, d_C0, d_nu, d_mid_off, d_mid, d_mid_alpha, d_wk, d_dos, d_fos, d_buf, d_G, d_part, d_slot;
+    sens_table(P.ctx, tau, P.model, d_zs);
+    sens_table(P.ctx, fix_tau_adj, P.model, d_fzs);
+    upload(d_C0, C0, P.ctx->stream);
+    upload(d_nu, nu, P.ctx->stream);
+    upload(d_mid_off, mid_off, P.ctx->stream);
+    // kernel reads r_fix from row .y and writes G row .w: pack as (next, rrow, sens, grow)
+    std::vector<int4> mid_dev = mid;
+    upload(d_mid, mid_dev, P.ctx->stream);
+    upload(d_mid_alpha, mid_alpha, P.ctx->stream);
+    upload(d_wk, wk, P.ctx->stream);
+    upload(d_dos, date_of_step, P.ctx->stream);
+    upload(d_fos, fixslot_of_step, P.ctx->stream);
+    // forward sweep: r at checkpoints (rows 0..n_cp-1) and at off-grid fixing steps
+    std::vector<int> slot_of_step(steps + 1, -1);
+    for (int c = 0; c < n_cp; ++c) slot_of_step[(size_t)c * K] = c;
+    for (size_t i = 0; i < cp_rows.size(); ++i) slot_of_step[(size_t)cp_rows[i]] = n_cp + (int)i;
+    upload(d_slot, slot_of_step, P.ctx->stream);
+    const uint64_t np = P.mc.n_paths;
+    d_buf.alloc((size_t)n_rows * np * 8);
+    d_G.alloc((size_t)std::max<size_t>(1, gslot.size()) * np * 8);
+    hipStream_t st = P.ctx->stream;
+    SimArgs sa{};
+    sa.a = P.model.a;
+    sa.b = P.model.b;
+    sa.sigma = P.model.sigma;
+    sa.r0 = P.model.r0;
+    sa.dt = P.dt;
+    sa.sqrt_dt = std::sqrt(P.dt);
+    sa.decay = std::exp(-(P.model.a * P.dt));
+    sa.mean_c = P.model.b * (1.0 - sa.decay);
+    const double e2 = std::exp(-(2.0 * P.model.a * P.dt));
+    const double q = (1.0 - e2) / (2.0 * P.model.a);
+    sa.sd = P.model.sigma * std::sqrt(q);
+    sa.n_paths = np;
+    sa.n_steps = steps;
+    sa.seed = P.mc.seed;
+    sa.slot_of_step = d_slot.as<int>();
+    sa.Z = P.Z.as<double>();
+    sa.R = d_buf.as<double>();
+    dim3 sgrid((unsigned)((np + 255) / 256));
+    const int key = P.mc.disc * 2 + P.mc.rng;
+    switch (key) {
+        case 0: hipLaunchKernelGGL((simulate_kernel<0, 0>), sgrid, dim3(256), 0, st, sa); break;
+        case 1: hipLaunchKernelGGL((simulate_kernel<0, 1>), sgrid, dim3(256), 0, st, sa); break;
+        case 2: hipLaunchKernelGGL((simulate_kernel<1, 0>), sgrid, dim3(256), 0, st, sa); break;
+        default: hipLaunchKernelGGL((simulate_kernel<1, 1>), sgrid, dim3(256), 0, st, sa); break;
+    }
+    HIPCHK(hipGetLastError());
+
+    AdjArgs g{};
+    g.a = sa.a;
+    g.b = sa.b;
+    g.sigma = sa.sigma;
+    g.r0 = sa.r0;
+    g.dt = sa.dt;
+    g.sqrt_dt = sa.sqrt_dt;
+    g.decay = sa.decay;
+    g.mean_c = sa.mean_c;
+    g.sd = sa.sd;
+    const double a = P.model.a;
+    const double dq_da = (2.0 * a * P.dt * e2 - (1.0 - e2)) / (2.0 * a * a);
+    g.dsd_da = (q > 0.0) ? P.model.sigma * dq_da / (2.0 * std::sqrt(q)) : 0.0;
+    g.dsd_ds = std::sqrt(q);
+    g.disc = P.mc.disc;
+    g.rng = P.mc.rng;
+    g.seed = P.mc.seed;
+    g.Z = P.Z.as<double>();
+    g.n_paths = np;
+    g.n_steps = steps;
+    g.CP = d_buf.as<double>();
+    g.RF = d_buf.as<double>();
+    g.G = d_G.as<double>();
+    g.n_fix_slots = (int)gslot.size();
+    g.date_of_step = d_dos.as<int>();
+    g.fixslot_of_step = d_fos.as<int>();
+    g.D = D;
+    g.date_dfirst = P.date_dfirst.as<int>();
+    g.C0 = d_C0.as<double>();
+    g.nu = d_nu.as<double>();
+    g.zs = d_zs.as<ZcbSens>();
+    g.mid_off = d_mid_off.as<int>();
+    g.mid = d_mid.as<int4>();
+    g.mid_alpha = d_mid_alpha.as<double>();
+    g.fix_zs = d_fzs.as<ZcbSens>();
+    g.wk = d_wk.as<double>();
+    g.date_det = P.date_det.as<int>();
+    const unsigned blocks = (unsigned)((np + kAdjThreads - 1) / kAdjThreads);
+    d_part.alloc((size_t)blocks * 5 * 8);
+    g.partial = d_part.as<double>();
+    switch (key) {
+        case 0: hipLaunchKernelGGL((adjoint_kernel<0, 0>), dim3(blocks), dim3(kAdjThreads), 0, st, g); break;
+        case 1: hipLaunchKernelGGL((adjoint_kernel<0, 1>), dim3(blocks), dim3(kAdjThreads), 0, st, g); break;
+        case 2: hipLaunchKernelGGL((adjoint_kernel<1, 0>), dim3(blocks), dim3(kAdjThreads), 0, st, g); break;
+        default: hipLaunchKernelGGL((adjoint_kernel<1, 1>), dim3(blocks), dim3(kAdjThreads), 0, st, g); break;
+    }
+    HIPCHK(hipGetLastError());
+    std::vector<double> part((size_t)blocks * 5);
+    HIPCHK(hipMemcpyAsync(part.data(), d_part.p, part.size() * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    double sum[5] = {0, 0, 0, 0, 0};
+    for (unsigned b = 0; b < blocks; ++b)
+        for (int c = 0; c < 5; ++c) sum[c] += part[(size_t)b * 5 + c];
+    for (int c = 0; c < 4; ++c) dcva[c] += sum[c];
+    *out_cva = cva_s;
+    for (int c = 0; c < 6; ++c) out_grad[c] = dcva[c];
+    if (out_check) *out_check = sum[4];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Batched netting sets: many exposure jobs (own portfolio and path count; shared model, dates,
+// grid, seed, rng and discretisation) in one simulation, one valuation launch and one select
+// launch. Job j's paths are the first n_j paths of the shared simulation, so every job sees the
+// shocks an unbatched run would draw: set-level results are bitwise those of unbatched runs.
+// ---------------------------------------------------------------------------------------------
+struct eqhip_batch {
+    eqhip_context* ctx = nullptr;
+    eqhip_model_hw1f model{};
+    std::vector<std::unique_ptr<eqhip_plan>> jobs;
+    uint64_t max_paths = 0;
+    uint32_t n_steps = 0;
+    uint64_t seed = 0;
+    int rng = 0, disc = 0;
+    double dt = 1.0;
+    bool det = false;
+    DevBuf R, slot_of_step, plans, units, rows, counter;
+    std::vector<int4> unit_list;
+    size_t lds_bytes = 0;
+    int persistent = 1;
+    uint64_t target_units_per_cu = 8;
+    bool tuned = false;
+    std::vector<std::string> tune_log;
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    eqhip_timing timing{};
+    ~eqhip_batch() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+namespace {
+
+int device_cus(eqhip_context* ctx) {
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, ctx->device));
+    return prop.multiProcessorCount;
+}
+
+// (Re)tiles every job for the batch's chunking target and uploads the joint descriptors, unit
+// list (longest units first) and select rows.
+void layout_batch(eqhip_batch& B) {
+    const int cus = device_cus(B.ctx);
+    std::vector<ValuePlan> vps;
+    std::vector<SelectRow> rows;
+    std::vector<std::pair<double, int4>> costed;
+    B.lds_bytes = 0;
+    uint64_t total_tiles = 0;
+    for (auto& J : B.jobs) total_tiles += J->n_tiles * (uint64_t)J->n_dates;
+    for (size_t j = 0; j < B.jobs.size(); ++j) {
+        eqhip_plan& P = *B.jobs[j];
+        // share of the batch's units proportional to the job's tile-dates
+        double share = (double)(P.n_tiles * (uint64_t)P.n_dates) / (double)std::max<uint64_t>(1, total_tiles);
+        P.target_units = std::max<uint64_t>(1, (uint64_t)(share * (double)(B.target_units_per_cu * cus)));
+        choose_launch(P);
+        if (P.want_trade) P.partial.alloc((size_t)P.n_trades * P.n_dates * P.n_chunks * 8);
+        P.r_shared = B.R.as<double>();
+        P.r_stride = B.max_paths;
+        build_value_plan(P);
+        vps.push_back(P.vp);
+        B.lds_bytes = std::max(B.lds_bytes, P.lds_bytes);
+        for (const int4& u : P.unit_list) {
+            int4 v = u;
+            v.x = (int)j;
+            uint64_t t0 = (uint64_t)u.z * P.tiles_per_block;
+            uint64_t tiles = std::min<uint64_t>(P.tiles_per_block, P.n_tiles - t0);
+            double live = (double)P.n_trades + 1.0;  // cost ~ tiles x (trades + classes)
+            costed.emplace_back((double)tiles * live, v);
+        }
+        std::vector<uint64_t> neff(P.n_dates), qrank(P.n_dates);
+        HIPCHK(hipMemcpy(neff.data(), P.neff.p, P.n_dates * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(qrank.data(), P.qrank.p, P.n_dates * 8, hipMemcpyDeviceToHost));
+        for (int s = 0; s < P.S; ++s)
+            for (int k = 0; k < P.n_dates; ++k) {
+                const size_t r = (size_t)s * P.n_dates + k;
+                rows.push_back(SelectRow{P.X.as<double>() + r * P.mc.n_paths, neff[k], qrank[k],
+                                         P.set_ee.as<double>() + r, P.set_pfe.as<double>() + r});
+            }
+    }
+    std::stable_sort(costed.begin(), costed.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+    B.unit_list.clear();
+    for (auto& c : costed) B.unit_list.push_back(c.second);
+    upload(B.plans, vps, B.ctx->stream);
+    upload(B.units, B.unit_list, B.ctx->stream);
+    upload(B.rows, rows, B.ctx->stream);
+}
+
+void launch_batch_value(eqhip_batch& B) {
+    hipStream_t st = B.ctx->stream;
+    const int n = (int)B.unit_list.size();
+    if (B.persistent) {
+        hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, st, B.counter.as<unsigned int>());
+        const int cus = device_cus(B.ctx);
+        const int blocks_per_cu = B.lds_bytes * 2 <= B.ctx->lds_limit ? 2 : 1;
+        const int grid = std::min(n, cus * blocks_per_cu);
+        hipLaunchKernelGGL(value_kernel_persistent, dim3(grid), dim3(kValueThreads), B.lds_bytes, st,
+                           B.plans.as<const ValuePlan>(), B.units.as<const int4>(), n, B.counter.as<unsigned int>());
+    } else {
+        hipLaunchKernelGGL(value_kernel, dim3(n), dim3(kValueThreads), B.lds_bytes, st, B.plans.as<const ValuePlan>(),
+                           B.units.as<const int4>());
+    }
+    HIPCHK(hipGetLastError());
+}
+
+void simulate_batch(eqhip_batch& B) {
+    SimArgs s{};
+    s.a = B.model.a;
+    s.b = B.model.b;
+    s.sigma = B.model.sigma;
+    s.r0 = B.model.r0;
+    s.dt = B.dt;
+    s.sqrt_dt = std::sqrt(B.dt);
+    s.decay = std::exp(-(B.model.a * B.dt));
+    s.mean_c = B.model.b * (1.0 - s.decay);
+    s.sd = B.model.sigma * std::sqrt((1.0 - std::exp(-(2.0 * B.model.a * B.dt))) / (2.0 * B.model.a));
+    s.n_paths = B.max_paths;
+    s.n_steps = B.det ? 0u : B.n_steps;
+    s.seed = B.seed;
+    s.slot_of_step = B.slot_of_step.as<int>();
+    s.Z = nullptr;
+    s.R = B.R.as<double>();
+    dim3 grid((unsigned)((B.max_paths + 255) / 256));
+    hipStream_t st = B.ctx->stream;
+    if (B.disc == EQHIP_DISC_EULER) hipLaunchKernelGGL((simulate_kernel<0, 1>), grid, dim3(256), 0, st, s);
+    else hipLaunchKernelGGL((simulate_kernel<1, 1>), grid, dim3(256), 0, st, s);
+    HIPCHK(hipGetLastError());
+}
+
+// Autotuner: times the valuation launch for each (units per CU, persistent) candidate on the
+// batch's own data and keeps the fastest.
+void autotune_batch(eqhip_batch& B) {
+    hipStream_t st = B.ctx->stream;
+    simulate_batch(B);
+    const uint64_t unit_targets[] = {4, 8, 16};
+    const int modes[] = {1, 0};
+    double best = 1e300;
+    uint64_t best_t = B.target_units_per_cu;
+    int best_m = B.persistent;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    for (uint64_t t : unit_targets) {
+        B.target_units_per_cu = t;
+        layout_batch(B);
+        for (int m : modes) {
+            B.persistent = m;
+            launch_batch_value(B);  // warm
+            HIPCHK(hipEventRecord(e0, st));
+            launch_batch_value(B);
+            HIPCHK(hipEventRecord(e1, st));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            B.tune_log.push_back("units/CU=" + std::to_string(t) + (m ? " persistent" : " per-unit") + ": " +
+                                 std::to_string(ms) + " ms");
+            if (ms < best) {
+                best = ms;
+                best_t = t;
+                best_m = m;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    B.target_units_per_cu = best_t;
+    B.persistent = best_m;
+    layout_batch(B);
+    B.tuned = true;
+}
+
+void execute_batch(eqhip_batch& B, const eqhip_credit* credit, eqhip_outputs* outs) {
+    if (!outs) raise(EQHIP_E_INVALID, "outputs are required");
+    for (size_t j = 0; j < B.jobs.size(); ++j) {
+        if (!outs[j].set_ee || !outs[j].set_pfe) raise(EQHIP_E_INVALID, "outputs set_ee / set_pfe are required");
+        if ((outs[j].trade_ee || outs[j].trade_cva) && !B.jobs[j]->want_trade)
+            raise(EQHIP_E_INVALID, "per-trade outputs need a batch created with want_trade_ee");
+        if (outs[j].path_mtm || outs[j].set_mtm) raise(EQHIP_E_INVALID, "per-path outputs are not produced by batches");
+    }
+    if (!B.tuned) autotune_batch(B);
+    hipStream_t st = B.ctx->stream;
+    HIPCHK(hipSetDevice(B.ctx->device));
+    HIPCHK(hipEventRecord(B.ev[0], st));
+    simulate_batch(B);
+    HIPCHK(hipEventRecord(B.ev[1], st));
+    launch_batch_value(B);
+    HIPCHK(hipEventRecord(B.ev[2], st));
+    unsigned n_rows = 0;
+    for (auto& J : B.jobs) n_rows += (unsigned)(J->S * J->n_dates);
+    hipLaunchKernelGGL(select_kernel, dim3(n_rows), dim3(kSelectThreads), 0, st, B.rows.as<const SelectRow>());
+    HIPCHK(hipGetLastError());
+    for (auto& J : B.jobs) {
+        if (!J->want_trade) continue;
+        int n = J->n_trades * J->n_dates;
+        hipLaunchKernelGGL(trade_ee_kernel, dim3((n + 255) / 256), dim3(256), 0, st, J->partial.as<double>(),
+                           J->n_trades, J->n_dates, J->n_chunks, J->neff.as<uint64_t>(), J->trade_ee.as<double>());
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(B.ev[3], st));
+    double h = credit ? credit->hazard_rate : 0.0, rr = credit ? credit->recovery_rate : 0.0;
+    for (size_t j = 0; j < B.jobs.size(); ++j) {
+        eqhip_plan& P = *B.jobs[j];
+        if (outs[j].set_cva)
+            hipLaunchKernelGGL(cva_kernel, dim3((P.S + 63) / 64), dim3(64), 0, st, P.set_ee.as<double>(), P.S,
+                               P.n_dates, P.d_dates.as<double>(), P.model.a, P.model.b, P.model.sigma, P.model.r0, h,
+                               rr, P.set_cva.as<double>());
+        if (outs[j].trade_cva)
+            hipLaunchKernelGGL(cva_kernel, dim3((P.n_trades + 63) / 64), dim3(64), 0, st, P.trade_ee.as<double>(),
+                               P.n_trades, P.n_dates, P.d_dates.as<double>(), P.model.a, P.model.b, P.model.sigma,
+                               P.model.r0, h, rr, P.trade_cva.as<double>());
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(B.ev[4], st));
+    for (size_t j = 0; j < B.jobs.size(); ++j) {
+        eqhip_plan& P = *B.jobs[j];
+        auto d2h = [&](void* dst, const DevBuf& src, size_t bytes) {
+            if (dst) HIPCHK(hipMemcpyAsync(dst, src.p, bytes, hipMemcpyDeviceToHost, st));
+        };
+        d2h(outs[j].set_ee, P.set_ee, (size_t)P.S * P.n_dates * 8);
+        d2h(outs[j].set_pfe, P.set_pfe, (size_t)P.S * P.n_dates * 8);
+        d2h(outs[j].set_cva, P.set_cva, (size_t)P.S * 8);
+        d2h(outs[j].trade_ee, P.trade_ee, (size_t)P.n_trades * P.n_dates * 8);
+        d2h(outs[j].trade_cva, P.trade_cva, (size_t)P.n_trades * 8);
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    float ms[4];
+    for (int i = 0; i < 4; ++i) HIPCHK(hipEventElapsedTime(&ms[i], B.ev[i], B.ev[i + 1]));
+    B.timing.simulate_ms = ms[0];
+    B.timing.value_ms = ms[1];
+    B.timing.reduce_ms = ms[2];
+    B.timing.cva_ms = ms[3];
+    B.timing.total_ms = (double)ms[0] + ms[1] + ms[2] + ms[3];
+    B.timing.n_value_blocks = B.unit_list.size();
+    B.timing.paths_per_tile = (uint32_t)kPathsPerTile;
+    B.timing.tiles_per_block = (uint32_t)B.target_units_per_cu;
 }
 
 }  // namespace
@@ -945,6 +1471,83 @@ int eqhip_pv_dv01(eqhip_context* ctx, const double* pillars, const double* zero_
         if (out_bucketed)
             HIPCHK(hipMemcpyAsync(out_bucketed, dbk.p, (size_t)n * n_pillars * 8, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
+int eqhip_batch_create(eqhip_context* ctx, const eqhip_model_hw1f* model, int32_t n_jobs,
+                       const eqhip_portfolio* portfolios, const uint64_t* n_paths, const double* dates,
+                       int32_t n_dates, uint32_t n_steps, uint64_t seed, int32_t rng, int32_t disc,
+                       int32_t want_trade_ee, eqhip_batch** out) {
+    return guarded([&] {
+        if (!out) raise(EQHIP_E_INVALID, "out is null");
+        *out = nullptr;
+        check_ctx(ctx);
+        if (n_jobs < 1 || !portfolios || !n_paths) raise(EQHIP_E_INVALID, "a batch needs at least one job");
+        if (rng != EQHIP_RNG_PHILOX) raise(EQHIP_E_INVALID, "batches draw from the Philox stream (rng = EQHIP_RNG_PHILOX)");
+        auto B = std::make_unique<eqhip_batch>();
+        B->ctx = ctx;
+        B->model = model ? *model : eqhip_model_hw1f{};
+        B->n_steps = n_steps;
+        B->seed = seed;
+        B->rng = rng;
+        B->disc = disc;
+        for (int j = 0; j < n_jobs; ++j) B->max_paths = std::max<uint64_t>(B->max_paths, n_paths[j]);
+        for (int j = 0; j < n_jobs; ++j) {
+            auto P = std::make_unique<eqhip_plan>();
+            P->ctx = ctx;
+            eqhip_mc mc{n_paths[j], n_steps, seed, rng, disc};
+            build_plan(*P, model, &portfolios[j], dates, n_dates, &mc, want_trade_ee, true);
+            B->det = P->det;
+            B->dt = P->dt;
+            B->jobs.push_back(std::move(P));
+        }
+        const uint32_t steps = B->det ? 0u : n_steps;
+        std::vector<int> slot_of_step(steps + 1);
+        for (uint32_t g = 0; g <= steps; ++g) slot_of_step[g] = (int)g;
+        upload(B->slot_of_step, slot_of_step, ctx->stream);
+        B->R.alloc((size_t)(steps + 1) * B->max_paths * 8);
+        B->counter.alloc(sizeof(unsigned int));
+        for (auto& e : B->ev) HIPCHK(hipEventCreate(&e));
+        layout_batch(*B);
+        *out = B.release();
+    });
+}
+
+int eqhip_batch_execute(eqhip_batch* batch, const eqhip_credit* credit, eqhip_outputs* outs) {
+    return guarded([&] {
+        if (!batch) raise(EQHIP_E_INVALID, "batch is null");
+        execute_batch(*batch, credit, outs);
+    });
+}
+
+int eqhip_batch_timing(const eqhip_batch* batch, eqhip_timing* out, char* tune_log, size_t len) {
+    return guarded([&] {
+        if (!batch || !out) raise(EQHIP_E_INVALID, "null argument");
+        *out = batch->timing;
+        if (tune_log && len) {
+            std::string log = std::string(batch->persistent ? "chosen: persistent" : "chosen: per-unit") +
+                              ", units/CU=" + std::to_string(batch->target_units_per_cu);
+            for (auto& l : batch->tune_log) log += "\n" + l;
+            size_t n = std::min(len - 1, log.size());
+            std::memcpy(tune_log, log.data(), n);
+            tune_log[n] = '\0';
+        }
+    });
+}
+
+int eqhip_batch_destroy(eqhip_batch* batch) {
+    return guarded([&] {
+        if (!batch) return;
+        (void)hipSetDevice(batch->ctx->device);
+        delete batch;
+    });
+}
+
+int eqhip_cva_adjoint(eqhip_plan* plan, const double* host_draws, int32_t set, const eqhip_credit* credit,
+                      double* out_cva, double* out_grad, double* out_check) {
+    return guarded([&] {
+        if (!plan) raise(EQHIP_E_INVALID, "plan is null");
+        cva_adjoint(*plan, host_draws, set, credit, out_cva, out_grad, out_check);
     });
 }
 

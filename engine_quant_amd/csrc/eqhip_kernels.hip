@@ -79,6 +79,181 @@ __global__ void ab_kernel(const double* tau, int n, double a, double b, double s
     out[i] = v;
 }
 
+// A, B and their parameter derivatives (models/hull_white.rs:50-80 differentiated):
+//   B   = (1 - e^{-a tau}) / a,            B_a   = (tau e^{-a tau} - B) / a
+//   lnA = (B - tau)(b - s^2/(2a^2)) - s^2 B^2 / (4a)
+//   lnA_b = B - tau,   lnA_s = -(B - tau) s / a^2 - s B^2 / (2a)
+//   lnA_a = B_a (b - s^2/(2a^2)) + (B - tau) s^2 / a^3 - s^2 (2 a B B_a - B^2) / (4 a^2)
+__device__ __forceinline__ ZcbSens zcb_sens(double a, double b, double s, double tau) {
+    ZcbSens z;
+    const double e = exp(-(a * tau));
+    const double B = hw_b(a, tau);
+    z.B = B;
+    z.A = hw_a(a, b, s, tau, B);
+    z.B_a = (tau * e - B) / a;
+    const double s2 = s * s, a2 = a * a;
+    z.lnA_b = B - tau;
+    z.lnA_s = -(B - tau) * s / a2 - s * B * B / (2.0 * a);
+    z.lnA_a = z.B_a * (b - s2 / (2.0 * a2)) + (B - tau) * s2 / (a2 * a) - s2 * (2.0 * a * B * z.B_a - B * B) / (4.0 * a2);
+    return z;
+}
+
+__global__ void zcb_sens_kernel(const double* tau, int n, double a, double b, double sigma, ZcbSens* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double t = tau[i];
+    out[i] = (t < 0.0) ? ZcbSens{0.0, 0.0, 0.0, 0.0, 0.0, 0.0} : zcb_sens(a, b, sigma, t);
+}
+
+// Reverse sweep of one path. The CVA estimator is sum_k w_k sum_p max(V_s(p,k), 0); its pathwise
+// derivative flows through r at every monitoring date (and at the fixing dates of periods in
+// progress) back to r0 and, at every step, into a, b, sigma. Short rates are recomputed per
+// segment of kAdjCheckpoint steps from the stored checkpoint, shocks from the counter-based stream.
+template <int DISC, int RNG>
+__device__ void adjoint_path(const AdjArgs& g, uint64_t p, double acc[5]) {
+    constexpr int K = kAdjCheckpoint;
+    const uint64_t n = g.n_paths;
+    for (int f = 0; f < g.n_fix_slots; ++f) g.G[(uint64_t)f * n + p] = 0.0;
+    double rbar = 0.0, abar = 0.0, bbar = 0.0, sbar = 0.0, vsum = 0.0;
+    const int n_seg = (int)((g.n_steps + K - 1) / K);
+
+    auto date_adjoint = [&](int k, double r) {
+        double w = g.wk[k];
+        if (g.date_det[k] && p != 0) w = 0.0;
+        if (w == 0.0) return;
+        double V = g.C0[k], gr = 0.0, ga = 0.0, gb = 0.0, gs = 0.0;
+        const ZcbSens* zk = g.zs + (size_t)k * g.D;
+        const double* nk = g.nu + (size_t)k * g.D;
+        for (int d = g.date_dfirst[k]; d < g.D; ++d) {
+            const double nu = nk[d];
+            if (nu == 0.0) continue;
+            const ZcbSens z = zk[d];
+            const double P = z.A * exp(-(z.B * r));
+            const double nP = nu * P;
+            V += nP;
+            gr -= nP * z.B;
+            ga += nP * (z.lnA_a - r * z.B_a);
+            gb += nP * z.lnA_b;
+            gs += nP * z.lnA_s;
+        }
+        double grf[4];
+        int fslot[4];
+        int nm = 0;
+        for (int m = g.mid_off[k]; m < g.mid_off[k + 1] && nm < 4; ++m, ++nm) {
+            const int4 md = g.mid[m];
+            const ZcbSens zn = zk[md.x];
+            const ZcbSens zf = g.fix_zs[md.z];
+            const double rf = g.RF[(uint64_t)md.y * n + p];
+            const double Pn = zn.A * exp(-(zn.B * r));
+            const double Pf = zf.A * exp(-(zf.B * rf));
+            const double term = g.mid_alpha[m] * (Pn / Pf);
+            V += term;
+            gr -= term * zn.B;
+            ga += term * ((zn.lnA_a - r * zn.B_a) - (zf.lnA_a - rf * zf.B_a));
+            gb += term * (zn.lnA_b - zf.lnA_b);
+            gs += term * (zn.lnA_s - zf.lnA_s);
+            grf[nm] = term * zf.B;
+            fslot[nm] = md.w;
+        }
+        if (V > 0.0) {
+            vsum += w * V;
+            rbar += w * gr;
+            abar += w * ga;
+            bbar += w * gb;
+            sbar += w * gs;
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                if (m < nm) g.G[(uint64_t)fslot[m] * n + p] += w * grf[m];
+        }
+    };
+    auto step_inputs = [&](int j) {  // adjoints arriving at grid step j
+        const int k = g.date_of_step[j];
+        return k;
+    };
+
+    for (int seg = n_seg - 1; seg >= 0; --seg) {
+        const int j0 = seg * K;
+        const int len = min(K, (int)g.n_steps - j0);
+        double rl[K + 1], zl[K];
+        rl[0] = g.CP[(uint64_t)seg * n + p];
+        double z_next = 0.0;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            if (i < len) {
+                const int j = j0 + i;
+                double z;
+                if (RNG == 0) {
+                    z = g.Z[(uint64_t)j * n + p];
+                } else if ((j & 1) == 0) {
+                    philox_normal_pair(g.seed, p, (uint32_t)(j >> 1), z, z_next);
+                } else {
+                    z = z_next;
+                }
+                zl[i] = z;
+                const double r = rl[i];
+                if (DISC == 0) rl[i + 1] = (r + (g.a * (g.b - r)) * g.dt) + g.sigma * (z * g.sqrt_dt);
+                else rl[i + 1] = (r * g.decay + g.mean_c) + g.sd * z;
+            }
+        }
+#pragma unroll
+        for (int i = K; i >= 1; --i) {
+            if (i <= len) {
+                const int j = j0 + i;
+                const int k = step_inputs(j);
+                if (k >= 0) date_adjoint(k, rl[i]);
+                const int fs = g.fixslot_of_step[j];
+                if (fs >= 0) rbar += g.G[(uint64_t)fs * n + p];
+                // back through r_j = phi(r_{j-1}, z_{j-1})
+                const double r = rl[i - 1], z = zl[i - 1];
+                if (DISC == 0) {
+                    abar += rbar * ((g.b - r) * g.dt);
+                    bbar += rbar * (g.a * g.dt);
+                    sbar += rbar * (z * g.sqrt_dt);
+                    rbar *= (1.0 - g.a * g.dt);
+                } else {
+                    abar += rbar * (g.dt * g.decay * (g.b - r) + z * g.dsd_da);
+                    bbar += rbar * (1.0 - g.decay);
+                    sbar += rbar * (z * g.dsd_ds);
+                    rbar *= g.decay;
+                }
+            }
+        }
+    }
+    {  // grid step 0: r = r0
+        const int k = step_inputs(0);
+        if (k >= 0) date_adjoint(k, g.r0);
+        const int fs = g.fixslot_of_step[0];
+        if (fs >= 0) rbar += g.G[(uint64_t)fs * n + p];
+    }
+    acc[0] += abar;
+    acc[1] += bbar;
+    acc[2] += sbar;
+    acc[3] += rbar;
+    acc[4] += vsum;
+}
+
+template <int DISC, int RNG>
+__global__ void __launch_bounds__(kAdjThreads) adjoint_kernel(AdjArgs g) {
+    __shared__ double red[5][kAdjThreads];
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (p < g.n_paths) adjoint_path<DISC, RNG>(g, p, acc);
+    for (int c = 0; c < 5; ++c) red[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int w = kAdjThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int c = 0; c < 5; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 5; ++c) g.partial[(uint64_t)blockIdx.x * 5 + c] = red[c][0];
+}
+
+template __global__ void adjoint_kernel<0, 0>(AdjArgs);
+template __global__ void adjoint_kernel<0, 1>(AdjArgs);
+template __global__ void adjoint_kernel<1, 0>(AdjArgs);
+template __global__ void adjoint_kernel<1, 1>(AdjArgs);
+
 // ---------------------------------------------------------------------------------------------
 // Valuation. One block = one work unit (plan, date k, chunk of path tiles); one lane = one path.
 // Per tile of 64 paths:
@@ -90,12 +265,7 @@ __global__ void ab_kernel(const double* tau, int n, double a, double b, double s
 // Schedule families hold classes whose schedules are prefixes of one another, so one running sum
 // yields every class's annuity, bit-identical to summing that class's flows in order.
 // ---------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(kValueThreads) value_kernel(const ValuePlan* plans, const int4* units) {
-    extern __shared__ double smem[];
-    const int4 u = units[blockIdx.x];
-    const ValuePlan& g = plans[u.x];
-    const int k = u.y;
-    const int chunk = u.z;
+__device__ __forceinline__ void value_unit(const ValuePlan& g, const int k, const int chunk, double* smem) {
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -113,7 +283,7 @@ __global__ void __launch_bounds__(kValueThreads) value_kernel(const ValuePlan* p
     const int nfix = g.fix_off[k + 1] - fbeg;
     const int det = g.date_det[k];
     const uint64_t n_paths = g.n_paths;
-    const double* Rk = g.R + (uint64_t)g.date_slot[k] * n_paths;
+    const double* Rk = g.R + (uint64_t)g.date_slot[k] * g.r_stride;
     const double2* ab = g.AB + (size_t)k * g.D;
     const int4* fdesc = g.fdesc + (size_t)k * g.NF;
 
@@ -156,7 +326,7 @@ __global__ void __launch_bounds__(kValueThreads) value_kernel(const ValuePlan* p
         __syncthreads();
         if (tid < kPathsPerTile) sR[lane] = valid ? Rk[gp] : 0.0;
         for (int f = wave; f < nfix; f += kValueWaves) {
-            double rf = valid ? g.R[(uint64_t)g.fix_slot[fbeg + f] * n_paths + gp] : 0.0;
+            double rf = valid ? g.R[(uint64_t)g.fix_slot[fbeg + f] * g.r_stride + gp] : 0.0;
             double2 fab = g.fix_AB[fbeg + f];
             sF[f * kPathsPerTile + lane] = fab.x * exp(-(fab.y * rf));
         }
@@ -284,83 +454,169 @@ __global__ void __launch_bounds__(kValueThreads) value_kernel(const ValuePlan* p
     }
 }
 
+// One block per work unit; the hardware dispatcher hands out units in list order.
+__global__ void __launch_bounds__(kValueThreads) value_kernel(const ValuePlan* plans, const int4* units) {
+    extern __shared__ double smem[];
+    const int4 u = units[blockIdx.x];
+    value_unit(plans[u.x], u.y, u.z, smem);
+}
+
+// Persistent variant: a fixed grid of blocks pulls work units from a device counter until the
+// list is exhausted (every block exits once the counter passes n_units).
+__global__ void __launch_bounds__(kValueThreads) value_kernel_persistent(const ValuePlan* plans, const int4* units,
+                                                                         int n_units, unsigned int* counter) {
+    extern __shared__ double smem[];
+    __shared__ int s_unit;
+    while (true) {
+        __syncthreads();  // every thread is done with the previous unit and with s_unit
+        if (threadIdx.x == 0) s_unit = (int)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int idx = s_unit;
+        if (idx >= n_units) break;
+        const int4 u = units[idx];
+        value_unit(plans[u.x], u.y, u.z, smem);
+    }
+}
+
+// Sets the persistent launch's work counter to zero (a kernel, so it can sit in a captured graph).
+__global__ void reset_counter_kernel(unsigned int* counter) { *counter = 0u; }
+
 // ---------------------------------------------------------------------------------------------
 // EE + PFE95 of one (set, date) row. EE = (fixed-order sum) / n. PFE = the element of rank
-// q = min(ceil(0.95 n) - 1, n - 1) of the ascending order (exposure.rs:42-49), found exactly by
-// an MSB-first radix select over the IEEE-754 bit patterns (monotone for x >= +0.0).
+// q = min(ceil(0.95 n) - 1, n - 1) of the ascending order (exposure.rs:42-49), found exactly on
+// the IEEE-754 bit patterns (monotone for x >= +0.0): 12-bit MSB radix passes narrow the key
+// prefix (the first pass is fused with the EE sum); once at most kSelectCand candidates share the
+// prefix they are compacted into LDS and ranked by a bitonic sort.
 // ---------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(kSelectThreads) select_kernel(const double* X, int n_dates,
-                                                                uint64_t n_paths,
-                                                                const uint64_t* n_eff,
-                                                                const uint64_t* q_rank, double* ee,
-                                                                double* pfe) {
-    __shared__ unsigned int hist[2048];
-    __shared__ unsigned int part[kSelectThreads];
+namespace {
+
+// Block-wide: the bin holding `rank` among hist[0..4096) (counts), as (bin, count before it).
+__device__ void locate_rank(const unsigned int* hist, unsigned int* scan, unsigned long long rank,
+                            unsigned int* s_bin, unsigned long long* s_before) {
+    const int tid = threadIdx.x;
+    const int per = 4096 / kSelectThreads;
+    unsigned int local = 0;
+    for (int b = 0; b < per; ++b) local += hist[tid * per + b];
+    scan[tid] = local;
+    __syncthreads();
+    for (int off = 1; off < kSelectThreads; off <<= 1) {  // inclusive Hillis-Steele scan
+        unsigned int v = (tid >= off) ? scan[tid - off] : 0u;
+        __syncthreads();
+        scan[tid] += v;
+        __syncthreads();
+    }
+    const unsigned long long incl = scan[tid];
+    const unsigned long long excl = incl - local;
+    if (rank >= excl && rank < incl) {
+        unsigned long long cum = excl;
+        int b = tid * per;
+        for (;; ++b) {
+            if (cum + hist[b] > rank) break;
+            cum += hist[b];
+        }
+        *s_bin = (unsigned int)b;
+        *s_before = cum;
+    }
+    __syncthreads();
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow* rows) {
+    __shared__ unsigned int hist[4096];
+    __shared__ unsigned int scan[kSelectThreads];
     __shared__ double red[kSelectThreads];
-    __shared__ unsigned long long s_prefix;
-    __shared__ unsigned long long s_rank;
+    __shared__ unsigned long long cand[kSelectCand];
+    __shared__ unsigned long long s_prefix, s_rank, s_before;
+    __shared__ unsigned int s_bin, s_count, s_ncand;
 
     const int tid = threadIdx.x;
-    const int row = blockIdx.x;
-    const int k = row % n_dates;
-    const uint64_t n = n_eff[k];
-    const unsigned long long* x = (const unsigned long long*)(X + (uint64_t)row * n_paths);
+    const SelectRow row = rows[blockIdx.x];
+    const uint64_t n = row.n;
+    const unsigned long long* x = (const unsigned long long*)row.x;
 
+    for (int b = tid; b < 4096; b += kSelectThreads) hist[b] = 0u;
+    __syncthreads();
+    // pass 1: EE sum and histogram of bits 63..52
     double sum = 0.0;
-    for (uint64_t i = tid; i < n; i += kSelectThreads) sum += __longlong_as_double(x[i]);
+    for (uint64_t i = tid; i < n; i += kSelectThreads) {
+        const unsigned long long key = x[i];
+        sum += __longlong_as_double(key);
+        atomicAdd(&hist[key >> 52], 1u);
+    }
     red[tid] = sum;
     __syncthreads();
     for (int w = kSelectThreads / 2; w > 0; w >>= 1) {
         if (tid < w) red[tid] = red[tid] + red[tid + w];
         __syncthreads();
     }
+    if (tid == 0) *row.ee = red[0] / (double)n;
+    locate_rank(hist, scan, row.q, &s_bin, &s_before);
     if (tid == 0) {
-        ee[row] = red[0] / (double)n;
-        s_prefix = 0ull;
-        s_rank = q_rank[k];
+        s_prefix = (unsigned long long)s_bin << 52;
+        s_rank = row.q - s_before;
+        s_count = hist[s_bin];
     }
     __syncthreads();
 
-    const int shifts[6] = {53, 42, 31, 20, 9, 0};
-    const int widths[6] = {11, 11, 11, 11, 11, 9};
-    for (int pass = 0; pass < 6; ++pass) {
-        const int sh = shifts[pass];
-        const int wd = widths[pass];
-        const int hi = sh + wd;  // bits >= hi are fixed by the prefix
+    int hi = 52;  // bits >= hi are fixed
+    while (true) {
         const unsigned long long prefix = s_prefix;
-        const unsigned int nb = 1u << wd;
-        for (unsigned int b = tid; b < nb; b += kSelectThreads) hist[b] = 0u;
+        const unsigned int count = s_count;
+        if (count <= (unsigned int)kSelectCand || hi == 0) break;
+        const int wd = hi >= 12 ? 12 : hi;
+        const int sh = hi - wd;
+        for (int b = tid; b < 4096; b += kSelectThreads) hist[b] = 0u;
         __syncthreads();
         for (uint64_t i = tid; i < n; i += kSelectThreads) {
-            unsigned long long key = x[i];
-            bool match = (hi >= 64) || ((key >> hi) == (prefix >> hi));
-            if (match) atomicAdd(&hist[(key >> sh) & (nb - 1u)], 1u);
+            const unsigned long long key = x[i];
+            if ((key >> hi) == (prefix >> hi)) atomicAdd(&hist[(key >> sh) & ((1u << wd) - 1u)], 1u);
         }
         __syncthreads();
-        const unsigned int per = nb / kSelectThreads;  // 8 bins per thread (2 in the last pass)
-        unsigned int local = 0;
-        for (unsigned int b = 0; b < per; ++b) local += hist[tid * per + b];
-        part[tid] = local;
-        __syncthreads();
+        locate_rank(hist, scan, s_rank, &s_bin, &s_before);
         if (tid == 0) {
-            unsigned long long rank = s_rank;
-            unsigned long long cum = 0;
-            int t = 0;
-            for (; t < kSelectThreads; ++t) {
-                if (cum + part[t] > rank) break;
-                cum += part[t];
-            }
-            unsigned int b = t * per;
-            for (;; ++b) {
-                if (cum + hist[b] > rank) break;
-                cum += hist[b];
-            }
-            s_prefix = prefix | ((unsigned long long)b << sh);
-            s_rank = rank - cum;
+            s_prefix = prefix | ((unsigned long long)s_bin << sh);
+            s_rank = s_rank - s_before;
+            s_count = hist[s_bin];
         }
         __syncthreads();
+        hi = sh;
     }
-    if (tid == 0) pfe[row] = __longlong_as_double(s_prefix);
+    if (hi == 0) {  // every bit fixed: the prefix is the key
+        if (tid == 0) *row.pfe = __longlong_as_double(s_prefix);
+        return;
+    }
+    // compact the candidates sharing the prefix and rank them exactly
+    if (tid == 0) s_ncand = 0u;
+    __syncthreads();
+    const unsigned long long prefix = s_prefix;
+    for (uint64_t i = tid; i < n; i += kSelectThreads) {
+        const unsigned long long key = x[i];
+        if ((key >> hi) == (prefix >> hi)) cand[atomicAdd(&s_ncand, 1u)] = key;
+    }
+    __syncthreads();
+    const unsigned int nc = s_ncand;
+    unsigned int m = 1;
+    while (m < nc) m <<= 1;
+    for (unsigned int i = nc + tid; i < m; i += kSelectThreads) cand[i] = ~0ull;
+    __syncthreads();
+    for (unsigned int size = 2; size <= m; size <<= 1) {
+        for (unsigned int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (unsigned int i = tid; i < m; i += kSelectThreads) {
+                const unsigned int j = i ^ stride;
+                if (j > i) {
+                    const bool up = (i & size) == 0;
+                    const unsigned long long a = cand[i], b = cand[j];
+                    if ((a > b) == up) {
+                        cand[i] = b;
+                        cand[j] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (tid == 0) *row.pfe = __longlong_as_double(cand[s_rank]);
 }
 
 // Per-trade EE: partial sums over chunks in chunk order, divided by the effective path count.

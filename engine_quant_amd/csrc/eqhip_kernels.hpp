@@ -12,7 +12,8 @@ constexpr int kPathsPerTile = 64;    // one path per lane
 constexpr int kItemsPerThread = 2;   // trade work items per thread in the valuation block
 constexpr int kTradesPerItem = 4;    // trades of one schedule class and sign per work item
 constexpr int kMaxSplitSets = 8;     // netting sets whose class sums are split across the waves
-constexpr int kSelectThreads = 256;  // EE / PFE select block
+constexpr int kSelectThreads = 1024;  // EE / PFE select block
+constexpr int kSelectCand = 4096;     // candidates ranked in LDS by the select kernel
 constexpr double kDateTolHost = 1e-9; // date matching tolerance, products/irs.rs:151-154,173
 
 struct SimArgs {
@@ -30,7 +31,8 @@ struct SimArgs {
 // One exposure plan as the valuation kernel sees it. A launch takes an array of plans and a list
 // of work units (plan, date, path chunk), so one launch can serve many netting-set plans.
 struct ValuePlan {
-    const double* R;          // [n_slots][n_paths]
+    const double* R;          // [n_slots][r_stride]
+    uint64_t r_stride;        // row stride of R (>= n_paths when R is shared by a batch)
     uint64_t n_paths;
     uint64_t n_tiles;         // ceil(n_paths / kPathsPerTile)
     int n_dates, D, C, NF, S, Fmax;
@@ -86,12 +88,62 @@ struct PvArgs {
     int bucketed_on;
 };
 
+// Pathwise adjoint of CVA for one netting set (reverse sweep with short-rate checkpoints).
+constexpr int kAdjCheckpoint = 12;   // grid steps between stored short rates
+constexpr int kAdjThreads = 256;
+
+// A, B of P(t,T) = A exp(-B r) and their derivatives in a, b, sigma (B does not depend on b, sigma).
+struct ZcbSens {
+    double A, B, B_a, lnA_a, lnA_b, lnA_s;
+};
+
+struct AdjArgs {
+    double a, b, sigma, r0;
+    double dt, sqrt_dt, decay, mean_c, sd, dsd_da, dsd_ds;
+    int disc, rng;
+    uint64_t seed;
+    const double* Z;              // host draws [n_steps][n_paths] (rng == host draws)
+    uint64_t n_paths;
+    uint32_t n_steps;
+    const double* CP;             // [n_steps / K + 1][n_paths] r at steps 0, K, 2K, ...
+    const double* RF;             // [n_fix_slots][n_paths] r at the fixing steps
+    double* G;                    // [n_fix_slots][n_paths] pending adjoints of the fixing rates
+    int n_fix_slots;
+    const int* date_of_step;      // [n_steps + 1] monitoring date index or -1
+    const int* fixslot_of_step;   // [n_steps + 1] fixing slot or -1
+    int D;
+    const int* date_dfirst;       // [n_dates]
+    const double* C0;             // [n_dates] constant part of V_s (reset floating legs)
+    const double* nu;             // [n_dates][D] V_s = C0 + sum_d nu_d P_d + mid-period terms
+    const ZcbSens* zs;            // [n_dates][D]
+    const int* mid_off;           // [n_dates + 1]
+    const int4* mid;              // (next cash-flow date, fixing slot, fixing sens index, -)
+    const double* mid_alpha;      // sum of notionals of the family's trades in the set
+    const ZcbSens* fix_zs;        // fixing-period sensitivities
+    const double* wk;             // [n_dates] (1 - R)(S_{k-1} - S_k) P(0,t_k) / n_eff(k)
+    const int* date_det;          // [n_dates]
+    double* partial;              // [blocks][5] d/da, d/db, d/dsigma, d/dr0, sum of w V+
+};
+
 template <int DISC, int RNG>
 __global__ void simulate_kernel(SimArgs g);
+__global__ void zcb_sens_kernel(const double* tau, int n, double a, double b, double sigma, ZcbSens* out);
+template <int DISC, int RNG>
+__global__ void adjoint_kernel(AdjArgs g);
 __global__ void ab_kernel(const double* tau, int n, double a, double b, double sigma, double2* out);
 __global__ void value_kernel(const ValuePlan* plans, const int4* units);
-__global__ void select_kernel(const double* X, int n_dates, uint64_t n_paths, const uint64_t* n_eff,
-                              const uint64_t* q_rank, double* ee, double* pfe);
+__global__ void value_kernel_persistent(const ValuePlan* plans, const int4* units, int n_units,
+                                        unsigned int* counter);
+__global__ void reset_counter_kernel(unsigned int* counter);
+// One EE / PFE95 row: n values at x, rank q, outputs ee / pfe (device pointers).
+struct SelectRow {
+    const double* x;
+    uint64_t n;
+    uint64_t q;
+    double* ee;
+    double* pfe;
+};
+__global__ void select_kernel(const SelectRow* rows);
 __global__ void trade_ee_kernel(const double* partial, int n_trades, int n_dates, int n_chunks,
                                 const uint64_t* n_eff, double* trade_ee);
 __global__ void cva_kernel(const double* ee, int n_profiles, int n_dates, const double* times,

@@ -30,6 +30,8 @@ EXPORTED = [
     "eqhip_irs_hull_white_exposure_profile_batch", "eqhip_unilateral_cva_from_exposure",
     "eqhip_unilateral_cva_from_exposure_batch", "eqhip_pv_dv01", "eqhip_plan_create",
     "eqhip_plan_execute", "eqhip_plan_timing", "eqhip_plan_destroy", "eqhip_exposure_run",
+    "eqhip_batch_create", "eqhip_batch_execute", "eqhip_batch_timing", "eqhip_batch_destroy",
+    "eqhip_cva_adjoint",
 ]
 
 
@@ -101,6 +103,14 @@ def lib():
         L.eqhip_exposure_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(ModelHW1F),
                                          ctypes.POINTER(PortfolioC), _PD, _I32, ctypes.POINTER(McC), _PD,
                                          ctypes.POINTER(CreditC), ctypes.POINTER(OutputsC)]
+        L.eqhip_batch_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(ModelHW1F), _I32,
+                                         ctypes.POINTER(PortfolioC), ctypes.POINTER(ctypes.c_uint64), _PD, _I32,
+                                         _U32, _U64, _I32, _I32, _I32, ctypes.POINTER(ctypes.c_void_p)]
+        L.eqhip_batch_execute.argtypes = [ctypes.c_void_p, ctypes.POINTER(CreditC), ctypes.POINTER(OutputsC)]
+        L.eqhip_batch_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(TimingC), ctypes.c_char_p,
+                                         ctypes.c_size_t]
+        L.eqhip_batch_destroy.argtypes = [ctypes.c_void_p]
+        L.eqhip_cva_adjoint.argtypes = [ctypes.c_void_p, _PD, _I32, ctypes.POINTER(CreditC), _PD, _PD, _PD]
         _lib = L
     return _lib
 
@@ -204,6 +214,18 @@ class Plan:
         cr = CreditC(hazard_rate, recovery_rate)
         _check(lib().eqhip_plan_execute(self.handle, _dp(z), ctypes.byref(cr), ctypes.byref(out)))
         return res
+
+    def cva_adjoint(self, set_index=0, hazard_rate=0.0, recovery_rate=0.0, host_draws=None):
+        """CVA of one netting set and dCVA/d(a, b, sigma, r0, hazard, recovery) by pathwise adjoint."""
+        z = None
+        if self.rng == RNG_HOST_DRAWS:
+            z = _f64(host_draws)
+        cva, check = ctypes.c_double(), ctypes.c_double()
+        grad = np.zeros(6)
+        cr = CreditC(hazard_rate, recovery_rate)
+        _check(lib().eqhip_cva_adjoint(self.handle, _dp(z), set_index, ctypes.byref(cr), ctypes.byref(cva),
+                                       _dp(grad), ctypes.byref(check)))
+        return cva.value, dict(zip(("a", "b", "sigma", "r0", "hazard_rate", "recovery_rate"), grad)), check.value
 
     def timing(self):
         t = TimingC()
@@ -312,6 +334,60 @@ class Context:
             return plan.execute(host_draws, hazard_rate, recovery_rate, True, want_path, want_set_mtm)
         finally:
             plan.close()
+
+
+class Batch:
+    """Many netting-set jobs in one simulation and one valuation launch (eqhip_batch_*)."""
+
+    def __init__(self, ctx, model, jobs, dates, n_steps, seed=0, disc=DISC_EULER, want_trade_ee=False):
+        """jobs: list of (Portfolio, n_paths)."""
+        self.ctx = ctx
+        self.jobs = jobs
+        self.dates = _f64(dates)
+        self.want_trade_ee = bool(want_trade_ee)
+        self._pcs = (PortfolioC * len(jobs))(*[p._c() for p, _ in jobs])
+        self._np = (ctypes.c_uint64 * len(jobs))(*[int(n) for _, n in jobs])
+        m = ModelHW1F(*model)
+        h = ctypes.c_void_p()
+        _check(lib().eqhip_batch_create(ctx.handle, ctypes.byref(m), len(jobs), self._pcs, self._np,
+                                        _dp(self.dates), len(self.dates), int(n_steps), int(seed), RNG_PHILOX,
+                                        disc, int(want_trade_ee), ctypes.byref(h)))
+        self.handle = h
+
+    def execute(self, hazard_rate=0.0, recovery_rate=0.0):
+        nd = len(self.dates)
+        results, outs = [], (OutputsC * len(self.jobs))()
+        for j, (p, _) in enumerate(self.jobs):
+            r = {"set_ee": np.zeros((p.n_sets, nd)), "set_pfe": np.zeros((p.n_sets, nd)),
+                 "set_cva": np.zeros(p.n_sets)}
+            if self.want_trade_ee:
+                r["trade_ee"] = np.zeros((p.n_trades, nd))
+                r["trade_cva"] = np.zeros(p.n_trades)
+            outs[j] = OutputsC(*[_dp(r.get(k)) for k in ("set_ee", "set_pfe", "set_cva", "trade_ee", "trade_cva",
+                                                         "path_mtm", "set_mtm")])
+            results.append(r)
+        cr = CreditC(hazard_rate, recovery_rate)
+        _check(lib().eqhip_batch_execute(self.handle, ctypes.byref(cr), outs))
+        return results
+
+    def timing(self):
+        t = TimingC()
+        buf = ctypes.create_string_buffer(8192)
+        _check(lib().eqhip_batch_timing(self.handle, ctypes.byref(t), buf, len(buf)))
+        d = {f: getattr(t, f) for f, _ in TimingC._fields_}
+        d["tune_log"] = buf.value.decode()
+        return d
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().eqhip_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def singleton_portfolio(notionals, fixed_rates, start, pay, acc):

@@ -185,6 +185,40 @@ int eqhip_exposure_run(eqhip_context* ctx, const eqhip_model_hw1f* model,
                        const eqhip_mc* mc, const double* host_draws, const eqhip_credit* credit,
                        eqhip_outputs* out);
 
+/* ---- sensitivities ---------------------------------------------------------------------------- */
+
+/* Pathwise adjoint of netting set `set`'s unilateral CVA (exposure.rs:242-275 over the plan's EE):
+ * runs the plan's exposure job, then a reverse sweep per path that stores the short rate only every
+ * 12 grid steps (plus the fixing dates) and re-simulates each segment backwards.
+ * out_cva: the CVA; out_grad[6]: dCVA / d{a, b, sigma, r0, hazard_rate, recovery_rate};
+ * out_check (nullable): sum_k w_k sum_p max(V,0) recomputed by the adjoint pass (equals out_cva up
+ * to rounding). Curve-pillar sensitivities of this CVA are identically zero: it discounts with the
+ * model bond P(0,t|r0), not the market curve (exposure.rs:264-275). Requires one grid step per
+ * monitoring date. The reference computes CVA Greeks by bump-and-revalue with common random
+ * numbers (cpp/engine/src/greeks.cpp:160-169). */
+int eqhip_cva_adjoint(eqhip_plan* plan, const double* host_draws, int32_t set, const eqhip_credit* credit,
+                      double* out_cva, double* out_grad, double* out_check);
+
+/* ---- batched netting sets ------------------------------------------------------------------ */
+
+/* Many exposure jobs (each its own portfolio and path count; shared model, monitoring dates, grid,
+ * seed and discretisation, Philox draws) as one simulation, one valuation launch over every job's
+ * work units (persistent or one block per unit, chosen by an autotuner on first execute) and one
+ * select launch. Job j uses the first n_paths[j] paths of the shared stream, so its set-level
+ * EE / PFE95 / CVA are bitwise those of an unbatched plan with the same settings.
+ * No counterpart in the reference, which prices one calendar group at a time
+ * (cpp/engine/src/price.cpp:534-578). */
+typedef struct eqhip_batch eqhip_batch;
+int eqhip_batch_create(eqhip_context* ctx, const eqhip_model_hw1f* model, int32_t n_jobs,
+                       const eqhip_portfolio* portfolios, const uint64_t* n_paths, const double* dates,
+                       int32_t n_dates, uint32_t n_steps, uint64_t seed, int32_t rng, int32_t disc,
+                       int32_t want_trade_ee, eqhip_batch** out);
+/* outs[n_jobs]; per-path outputs are not produced by batches. */
+int eqhip_batch_execute(eqhip_batch* batch, const eqhip_credit* credit, eqhip_outputs* outs);
+/* Timing of the last execute; tune_log (nullable) receives the autotuner's choice and trials. */
+int eqhip_batch_timing(const eqhip_batch* batch, eqhip_timing* out, char* tune_log, size_t len);
+int eqhip_batch_destroy(eqhip_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

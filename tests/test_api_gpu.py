@@ -112,7 +112,10 @@ def test_price_batch_matches_scalar_prices():
     for b, t in zip(batch, trades):
         single = eng.price(t, names, model, market, pr, ex)
         for n in names:
-            assert np.allclose(b.measures[n].primary, single[n].primary, rtol=1e-12, atol=1e-9)
+            # price() returns UnilateralCVA with its exposure profile attached (measure.cpp:348-365),
+            # price_batch() with the scalar only (price.cpp:212-224): compare what both carry
+            if n != "UnilateralCVA":
+                assert np.allclose(b.measures[n].primary, single[n].primary, rtol=1e-12, atol=1e-9)
             assert math.isclose(b.measures[n].scalar, single[n].scalar, rel_tol=1e-12, abs_tol=1e-9)
 
 
