@@ -103,11 +103,24 @@ def main():
     ap.add_argument("--disc", choices=["exact", "euler"], default="exact")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device-to-device copy of BYTES (hardware-counter scale factor)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 8, "copies": 3}), flush=True)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
-              "(paths would shard by global path index; see DESIGN.md)", file=sys.stderr)
+              "(paths would shard by global path index across GPUs)", file=sys.stderr)
         sys.exit(2)
 
     from engine_quant_amd import eqhip

@@ -129,6 +129,7 @@ struct eqhip_plan {
     uint64_t n_tiles = 0;
     int want_trade = 0;
     int n_items = 0;
+    int n_slices = 1;
     int set_mode = 0;
     uint64_t target_units = 2048;   // work units the launch aims for (chunking of the path tiles)
     const double* r_shared = nullptr;  // batch-owned short-rate buffer (all grid steps), or null
@@ -429,6 +430,18 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
             items.push_back(item);
         }
         P.n_items = (int)items.size();
+        // path slices per item: fill the block's work units as evenly as possible
+        double best = -1.0;
+        for (int sl = 1; sl <= 8; ++sl) {
+            const int units = P.n_items * sl;
+            if (units > kValueThreads * kItemsPerThread) break;
+            const int rounds = (units + kValueThreads - 1) / kValueThreads;
+            const double util = (double)units / (double)(rounds * kValueThreads);
+            if (util > best + 1e-9) {
+                best = util;
+                P.n_slices = sl;
+            }
+        }
         if (P.n_items > kValueThreads * kItemsPerThread)
             raise(EQHIP_E_CAPACITY, "per-trade EE supports at most " +
                                         std::to_string(kValueThreads * kItemsPerThread) +
@@ -485,7 +498,7 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
         upload(P.tr_N, tr_N, st);
         upload(P.tr_NK, tr_NK, st);
         upload(P.tr_orig, tr_orig, st);
-        P.partial.alloc((size_t)P.n_trades * n_dates * P.n_chunks * 8);
+        P.partial.alloc((size_t)P.n_trades * n_dates * P.n_chunks * P.n_slices * 8);
         P.trade_ee.alloc((size_t)P.n_trades * n_dates * 8);
         P.trade_cva.alloc((size_t)P.n_trades * 8);
     }
@@ -577,6 +590,7 @@ void build_value_plan(eqhip_plan& P) {
     g.want_trade = P.want_trade;
     g.n_items = P.n_items;
     g.n_trades = P.n_trades;
+    g.n_slices = P.n_slices;
     g.items = P.items.as<int4>();
     g.cls_mat = P.cls_mat.as<double>();
     g.tr_K = P.tr_K.as<double>();
@@ -672,7 +686,7 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
     if (P.want_trade) {
         int n = P.n_trades * P.n_dates;
         hipLaunchKernelGGL(trade_ee_kernel, dim3((n + 255) / 256), dim3(256), 0, st, P.partial.as<double>(),
-                           P.n_trades, P.n_dates, P.n_chunks, P.neff.as<uint64_t>(), P.trade_ee.as<double>());
+                           P.n_trades, P.n_dates, P.n_chunks * P.n_slices, P.neff.as<uint64_t>(), P.trade_ee.as<double>());
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(P.ev[3], st));
@@ -1070,7 +1084,7 @@ void layout_batch(eqhip_batch& B) {
         double share = (double)(P.n_tiles * (uint64_t)P.n_dates) / (double)std::max<uint64_t>(1, total_tiles);
         P.target_units = std::max<uint64_t>(1, (uint64_t)(share * (double)(B.target_units_per_cu * cus)));
         choose_launch(P);
-        if (P.want_trade) P.partial.alloc((size_t)P.n_trades * P.n_dates * P.n_chunks * 8);
+        if (P.want_trade) P.partial.alloc((size_t)P.n_trades * P.n_dates * P.n_chunks * P.n_slices * 8);
         P.r_shared = B.R.as<double>();
         P.r_stride = B.max_paths;
         build_value_plan(P);
@@ -1209,7 +1223,7 @@ void execute_batch(eqhip_batch& B, const eqhip_credit* credit, eqhip_outputs* ou
         if (!J->want_trade) continue;
         int n = J->n_trades * J->n_dates;
         hipLaunchKernelGGL(trade_ee_kernel, dim3((n + 255) / 256), dim3(256), 0, st, J->partial.as<double>(),
-                           J->n_trades, J->n_dates, J->n_chunks, J->neff.as<uint64_t>(), J->trade_ee.as<double>());
+                           J->n_trades, J->n_dates, J->n_chunks * J->n_slices, J->neff.as<uint64_t>(), J->trade_ee.as<double>());
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(B.ev[3], st));

@@ -265,7 +265,10 @@ template __global__ void adjoint_kernel<1, 1>(AdjArgs);
 // Schedule families hold classes whose schedules are prefixes of one another, so one running sum
 // yields every class's annuity, bit-identical to summing that class's flows in order.
 // ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ void value_unit(const ValuePlan& g, const int k, const int chunk, double* smem) {
+__device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, const int chunk, double* smem) {
+    // a private copy of the descriptor: the kernel's own stores cannot alias it, so its fields stay
+    // in scalar registers instead of being reloaded after every global store
+    const ValuePlan g = plan;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -273,7 +276,14 @@ __device__ __forceinline__ void value_unit(const ValuePlan& g, const int k, cons
     double* sR = smem;                                     // [64]
     double* sP = sR + kPathsPerTile;                       // [D][64]
     double* sF = sP + (size_t)g.D * kPathsPerTile;         // [Fmax][64]
-    double2* sC = (double2*)(sF + (size_t)g.Fmax * kPathsPerTile);  // [C][64]
+    double2* sC = (double2*)(sF + (size_t)g.Fmax * kPathsPerTile);  // [64][C] path-major
+    const int CS = g.C;
+    const int* __restrict__ flow_d = g.flow_d;
+    const double* __restrict__ flow_tau = g.flow_tau;
+    const int* __restrict__ flow_cls = g.flow_cls;
+    const int* __restrict__ set_cls = g.set_cls;
+    const double* __restrict__ set_alpha = g.set_alpha;
+    const double* __restrict__ set_beta = g.set_beta;
     double* sPart = (double*)(sC + (size_t)g.C * kPathsPerTile);    // [S][waves][64] (split mode)
 
     const double t = g.dates[k];
@@ -292,9 +302,14 @@ __device__ __forceinline__ void value_unit(const ValuePlan& g, const int k, cons
     double tK[kItemsPerThread][kTradesPerItem];
     int icls[kItemsPerThread], icnt[kItemsPerThread], ifirst[kItemsPerThread];
     double isgn[kItemsPerThread];
+    int islice_b[kItemsPerThread], islice_e[kItemsPerThread];
 #pragma unroll
     for (int i = 0; i < kItemsPerThread; ++i) {
-        const int it = tid + i * kValueThreads;
+        const int u = tid + i * kValueThreads;  // unit = item * n_slices + slice
+        const int it = u / g.n_slices;
+        const int sl = u - it * g.n_slices;
+        islice_b[i] = (sl * kPathsPerTile) / g.n_slices;
+        islice_e[i] = ((sl + 1) * kPathsPerTile) / g.n_slices;
         icnt[i] = 0;
         icls[i] = 0;
         ifirst[i] = 0;
@@ -348,8 +363,8 @@ __device__ __forceinline__ void value_unit(const ValuePlan& g, const int k, cons
             const int f1 = g.fam_off[F + 1];
             const int first = (fd.x == 0) ? f1 : f0 + fd.y;
             for (int i = f0; i < first; ++i) {  // classes already matured
-                const int c = g.flow_cls[i];
-                if (c >= 0) sC[c * kPathsPerTile + lane] = make_double2(0.0, 0.0);
+                const int c = flow_cls[i];
+                if (c >= 0) sC[lane * CS + c] = make_double2(0.0, 0.0);
             }
             if (fd.x == 0) continue;
             double st;
@@ -357,11 +372,23 @@ __device__ __forceinline__ void value_unit(const ValuePlan& g, const int k, cons
             else if (fd.x == 2) st = sP[fd.z * kPathsPerTile + lane];
             else st = sP[fd.z * kPathsPerTile + lane] / sF[fd.w * kPathsPerTile + lane];
             double ann = 0.0;
-            for (int i = first; i < f1; ++i) {
-                const double P = sP[g.flow_d[i] * kPathsPerTile + lane];
-                ann += P * g.flow_tau[i];
-                const int c = g.flow_cls[i];
-                if (c >= 0) sC[c * kPathsPerTile + lane] = make_double2(st - P, ann);
+            int i = first;
+            for (; i + 4 <= f1; i += 4) {  // four independent LDS reads in flight, then the chain
+                double P[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) P[u] = sP[flow_d[i + u] * kPathsPerTile + lane];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    ann += P[u] * flow_tau[i + u];
+                    const int c = flow_cls[i + u];
+                    if (c >= 0) sC[lane * CS + c] = make_double2(st - P[u], ann);
+                }
+            }
+            for (; i < f1; ++i) {
+                const double P = sP[flow_d[i] * kPathsPerTile + lane];
+                ann += P * flow_tau[i];
+                const int c = flow_cls[i];
+                if (c >= 0) sC[lane * CS + c] = make_double2(st - P, ann);
             }
         }
         __syncthreads();
@@ -373,9 +400,17 @@ __device__ __forceinline__ void value_unit(const ValuePlan& g, const int k, cons
                 const int cb = beg + wave * per;
                 const int ce = min(end, cb + per);
                 double v = 0.0;
-                for (int e = cb; e < ce; ++e) {
-                    const double2 c2 = sC[g.set_cls[e] * kPathsPerTile + lane];
-                    v += g.set_alpha[e] * c2.x - g.set_beta[e] * c2.y;
+                int e = cb;
+                for (; e + 4 <= ce; e += 4) {
+                    double2 c2[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) c2[u] = sC[lane * CS + set_cls[e + u]];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) v += set_alpha[e + u] * c2[u].x - set_beta[e + u] * c2[u].y;
+                }
+                for (; e < ce; ++e) {
+                    const double2 c2 = sC[lane * CS + set_cls[e]];
+                    v += set_alpha[e] * c2.x - set_beta[e] * c2.y;
                 }
                 sPart[(s * kValueWaves + wave) * kPathsPerTile + lane] = v;
             }
@@ -383,8 +418,8 @@ __device__ __forceinline__ void value_unit(const ValuePlan& g, const int k, cons
             for (int s = wave; s < g.S; s += kValueWaves) {
                 double v = 0.0;
                 for (int e = g.set_off[s]; e < g.set_off[s + 1]; ++e) {
-                    const double2 c2 = sC[g.set_cls[e] * kPathsPerTile + lane];
-                    v += g.set_alpha[e] * c2.x - g.set_beta[e] * c2.y;
+                    const double2 c2 = sC[lane * CS + set_cls[e]];
+                    v += set_alpha[e] * c2.x - set_beta[e] * c2.y;
                 }
                 if (valid) {
                     g.X[((uint64_t)s * g.n_dates + k) * n_paths + gp] = (v > 0.0) ? v : 0.0;
@@ -392,18 +427,20 @@ __device__ __forceinline__ void value_unit(const ValuePlan& g, const int k, cons
                 }
             }
         }
-        // D: per-trade exposure sums (reads sC only)
+        // D: per-trade exposure sums (reads sC only); work unit = (item, slice of the tile's paths)
         if (g.want_trade) {
             const uint64_t left = n_paths - p0;
             const int pvalid = (left < (uint64_t)kPathsPerTile) ? (int)left : kPathsPerTile;
-            const int pend = det ? ((p0 == 0) ? 1 : 0) : pvalid;  // grid index 0: path 0 only
 #pragma unroll
             for (int i = 0; i < kItemsPerThread; ++i) {
                 if (icnt[i] == 0) continue;
-                const double2* row = sC + icls[i] * kPathsPerTile;
+                const double2* col = sC + icls[i];
                 const double sg = isgn[i];
-                for (int p = 0; p < pend; ++p) {
-                    const double2 c2 = row[p];
+                int pb = islice_b[i], pe = min(islice_e[i], pvalid);
+                if (det) pe = (p0 == 0 && pb == 0) ? 1 : pb;  // grid index 0: path 0 only
+#pragma unroll 4
+                for (int p = pb; p < pe; ++p) {
+                    const double2 c2 = col[p * CS];
                     const double sfl = sg * c2.x;
                     const double sann = sg * c2.y;
 #pragma unroll
@@ -417,7 +454,8 @@ __device__ __forceinline__ void value_unit(const ValuePlan& g, const int k, cons
                         const int j = ifirst[i] + q;
                         const double N = g.tr_N[j], NK = g.tr_NK[j];
                         double* dst = g.path_mtm + ((uint64_t)k * g.n_trades + g.tr_orig[j]) * n_paths + p0;
-                        for (int p = 0; p < pvalid; ++p) dst[p] = N * row[p].x - NK * row[p].y;
+                        for (int p = islice_b[i]; p < min(islice_e[i], pvalid); ++p)
+                            dst[p] = N * col[p * CS].x - NK * col[p * CS].y;
                     }
                 }
             }
@@ -438,16 +476,19 @@ __device__ __forceinline__ void value_unit(const ValuePlan& g, const int k, cons
     if (g.want_trade) {
 #pragma unroll
         for (int i = 0; i < kItemsPerThread; ++i) {
-            const int it = tid + i * kValueThreads;
+            const int u = tid + i * kValueThreads;
+            const int it = u / g.n_slices;
+            const int sl = u - it * g.n_slices;
             if (!(it < g.n_items)) continue;
             const int4 item = g.items[it];
+            const int n_parts = g.n_chunks * g.n_slices;
 #pragma unroll
             for (int q = 0; q < kTradesPerItem; ++q) {
                 if (q < item.z) {
                     const int j = item.y + q;
                     // a matured class contributes exact zeros
                     const double e = (icnt[i] == 0) ? 0.0 : g.tr_absN[j] * acc[i][q];
-                    g.partial[((uint64_t)g.tr_orig[j] * g.n_dates + k) * g.n_chunks + chunk] = e;
+                    g.partial[((uint64_t)g.tr_orig[j] * g.n_dates + k) * n_parts + chunk * g.n_slices + sl] = e;
                 }
             }
         }

@@ -58,7 +58,7 @@ struct ValuePlan {
     const double* set_beta;   // sum of notional * fixed rate
     double* X;                // [S][n_dates][n_paths] max(V_set, 0)
     double* set_mtm;          // optional [n_dates][S][n_paths]
-    int want_trade, n_items, n_trades, pad_;
+    int want_trade, n_items, n_trades, n_slices;  // n_slices: path slices per item (work units)
     const int4* items;        // (class, first sorted trade, count, sign)
     const double* cls_mat;    // [C] maturity of the class
     const double* tr_K;       // sorted trades: fixed rate, |notional|, notional, notional * rate
@@ -66,7 +66,7 @@ struct ValuePlan {
     const double* tr_N;
     const double* tr_NK;
     const int* tr_orig;       // sorted -> caller trade index
-    double* partial;          // [n_trades][n_dates][n_chunks] per-trade EE partial sums
+    double* partial;          // [n_trades][n_dates][n_chunks * n_slices] per-trade EE partial sums
     double* path_mtm;         // optional [n_dates][n_trades][n_paths]
 };
 

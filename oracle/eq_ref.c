@@ -518,3 +518,193 @@ int eqr_portfolio_exposure(double a, double b, double sigma, double r0, int n_tr
     free(idx);
     return 0;
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Bermudan payer swaptions, Longstaff-Schwartz (no reference function: the reference's LSM is  */
+/* the quadratic GBM one of payoff/lsm.rs:73-107; this restates the same regression method for   */
+/* HW1F with a degree-3 basis in the standardized short rate)                                   */
+/* ------------------------------------------------------------------------------------------ */
+
+/* r and the trapezoid log-numeraire log B(t) = int_0^t r ds (hull_white.rs:114-120) on the exact
+ * grid: out arrays [n_steps+1][n_paths]. */
+void eqr_berm_simulate(double a, double b, double sigma, double r0, double dt, uint32_t n_steps,
+                       uint64_t n_paths, const double* z, double* r_out, double* lb_out) {
+    eqr_simulate(a, b, sigma, r0, dt, n_steps, n_paths, 1, z, r_out);
+    for (uint64_t p = 0; p < n_paths; ++p) {
+        double acc = 0.0;
+        lb_out[p] = 0.0;
+        for (uint32_t s = 0; s < n_steps; ++s) {
+            acc += (r_out[(uint64_t)s * n_paths + p] + r_out[(uint64_t)(s + 1) * n_paths + p]) * (0.5 * dt);
+            lb_out[(uint64_t)(s + 1) * n_paths + p] = acc;
+        }
+    }
+}
+
+/* standardized short rate x = (r - E[r_t]) / sd(r_t) */
+double eqr_berm_x(double a, double b, double sigma, double r0, double t, double r) {
+    double m = b + (r0 - b) * exp(-a * t);
+    double s = sigma * sqrt((1.0 - exp(-2.0 * a * t)) / (2.0 * a));
+    return (r - m) / s;
+}
+
+/* payer swap entered at an exercise date: N[(1 - P(T,T+L)) - K sum_{m=1..L} P(T,T+m)] */
+static double berm_exercise_value(double a, double b, double sigma, double r, int L, double n, double k) {
+    double ann = 0.0, pl = 0.0;
+    for (int m = 1; m <= L; ++m) {
+        pl = eqr_zcb(a, b, sigma, r, 0.0, (double)m);
+        ann += pl;
+    }
+    return n * ((1.0 - pl) - k * ann);
+}
+
+/* solve the 4x4 normal equations (Gaussian elimination with partial pivoting); returns 0 if singular */
+static int solve4(double M[4][4], double v[4], double c[4]) {
+    double A[4][5];
+    for (int i = 0; i < 4; ++i) {
+        for (int j = 0; j < 4; ++j) A[i][j] = M[i][j];
+        A[i][4] = v[i];
+    }
+    for (int col = 0; col < 4; ++col) {
+        int piv = col;
+        for (int i = col + 1; i < 4; ++i)
+            if (fabs(A[i][col]) > fabs(A[piv][col])) piv = i;
+        if (fabs(A[piv][col]) < 1e-300) return 0;
+        for (int j = 0; j < 5; ++j) {
+            double t = A[col][j];
+            A[col][j] = A[piv][j];
+            A[piv][j] = t;
+        }
+        for (int i = col + 1; i < 4; ++i) {
+            double f = A[i][col] / A[col][col];
+            for (int j = col; j < 5; ++j) A[i][j] -= f * A[col][j];
+        }
+    }
+    for (int i = 3; i >= 0; --i) {
+        double s = A[i][4];
+        for (int j = i + 1; j < 4; ++j) s -= A[i][j] * c[j];
+        c[i] = s / A[i][i];
+    }
+    return 1;
+}
+
+/* Exercise policy on training paths: backward over the trade's exercise years E..E+L-1 (annual);
+ * at each, regress the discounted continuation cashflow on [1,x,x^2,x^3] over in-the-money paths,
+ * exercise where U > 0 and U > C. r, lb: [n_steps+1][n_paths] with steps_per_year steps per year.
+ * ex_coef: [n_trades][max_ex][4]; tau_year: [n_trades][n_paths] (-1 = never); y0: discounted to 0. */
+int eqr_berm_policy(double a, double b, double sigma, double r0, int n_trades, const double* notional,
+                    const double* strike, const int32_t* first_ex, const int32_t* n_ex, int max_ex,
+                    uint32_t steps_per_year, uint64_t n_paths, const double* r, const double* lb,
+                    double* ex_coef, int32_t* tau_year, double* y0, double* price) {
+    for (int j = 0; j < n_trades; ++j) {
+        int32_t* tau = tau_year + (size_t)j * n_paths;
+        double* y = y0 + (size_t)j * n_paths;
+        for (uint64_t p = 0; p < n_paths; ++p) { tau[p] = -1; y[p] = 0.0; }
+        int E = first_ex[j], L = n_ex[j];
+        for (int e = L - 1; e >= 0; --e) {
+            int T = E + e;                    /* exercise year */
+            int rem = L - e;                  /* years of the underlying swap left */
+            uint64_t step = (uint64_t)T * steps_per_year;
+            double M[4][4] = {{0}}, v[4] = {0}, c[4] = {0, 0, 0, 0};
+            uint64_t n_itm = 0;
+            for (uint64_t p = 0; p < n_paths; ++p) {
+                double rr = r[step * n_paths + p];
+                double u = berm_exercise_value(a, b, sigma, rr, rem, notional[j], strike[j]);
+                if (!(u > 0.0)) continue;
+                double x = eqr_berm_x(a, b, sigma, r0, (double)T, rr);
+                double phi[4] = {1.0, x, x * x, x * x * x};
+                double yy = y[p] * exp(lb[step * n_paths + p]);
+                for (int i = 0; i < 4; ++i) {
+                    for (int k = 0; k < 4; ++k) M[i][k] += phi[i] * phi[k];
+                    v[i] += phi[i] * yy;
+                }
+                ++n_itm;
+            }
+            int fit = (e < L - 1) && n_itm >= 8 && solve4(M, v, c);
+            if (!fit) c[0] = c[1] = c[2] = c[3] = 0.0;
+            for (int i = 0; i < 4; ++i) ex_coef[((size_t)j * max_ex + e) * 4 + i] = c[i];
+            for (uint64_t p = 0; p < n_paths; ++p) {
+                double rr = r[step * n_paths + p];
+                double u = berm_exercise_value(a, b, sigma, rr, rem, notional[j], strike[j]);
+                if (!(u > 0.0)) continue;
+                double x = eqr_berm_x(a, b, sigma, r0, (double)T, rr);
+                double cont = c[0] + x * (c[1] + x * (c[2] + x * c[3]));
+                if (u > cont) {
+                    tau[p] = T;
+                    y[p] = u * exp(-lb[step * n_paths + p]);
+                }
+            }
+        }
+        double s = 0.0;
+        for (uint64_t p = 0; p < n_paths; ++p) s += y[p];
+        price[j] = s / (double)n_paths;
+    }
+    return 0;
+}
+
+/* Forward application of a fitted policy to fresh paths (same outputs as eqr_berm_policy). */
+int eqr_berm_apply(double a, double b, double sigma, double r0, int n_trades, const double* notional,
+                   const double* strike, const int32_t* first_ex, const int32_t* n_ex, int max_ex,
+                   uint32_t steps_per_year, uint64_t n_paths, const double* r, const double* lb,
+                   const double* ex_coef, int32_t* tau_year, double* y0) {
+    for (int j = 0; j < n_trades; ++j) {
+        for (uint64_t p = 0; p < n_paths; ++p) {
+            int32_t t_ex = -1;
+            double yy = 0.0;
+            for (int e = 0; e < n_ex[j]; ++e) {
+                int T = first_ex[j] + e;
+                uint64_t step = (uint64_t)T * steps_per_year;
+                double rr = r[step * n_paths + p];
+                double u = berm_exercise_value(a, b, sigma, rr, n_ex[j] - e, notional[j], strike[j]);
+                if (!(u > 0.0)) continue;
+                double x = eqr_berm_x(a, b, sigma, r0, (double)T, rr);
+                const double* c = ex_coef + ((size_t)j * max_ex + e) * 4;
+                double cont = c[0] + x * (c[1] + x * (c[2] + x * c[3]));
+                if (u > cont) {
+                    t_ex = T;
+                    yy = u * exp(-lb[step * n_paths + p]);
+                    break;
+                }
+            }
+            tau_year[(size_t)j * n_paths + p] = t_ex;
+            y0[(size_t)j * n_paths + p] = yy;
+        }
+    }
+    return 0;
+}
+
+/* Continuation-value regression at monitoring date t (grid step `step`) over the paths alive at t
+ * (not exercised before t): target y0 * B(t) on [1,x,x^2,x^3]. coef[4]; returns alive count. */
+uint64_t eqr_berm_judge_fit(double a, double b, double sigma, double r0, uint64_t n_paths,
+                            uint64_t step, double t, const double* r, const double* lb,
+                            const int32_t* tau_year, const double* y0, double* coef) {
+    double M[4][4] = {{0}}, v[4] = {0}, c[4] = {0, 0, 0, 0};
+    uint64_t n = 0;
+    for (uint64_t p = 0; p < n_paths; ++p) {
+        if (tau_year[p] >= 0 && (double)tau_year[p] < t - 1e-9) continue;  /* exercised before t */
+        double rr = r[step * n_paths + p];
+        double x = eqr_berm_x(a, b, sigma, r0, t, rr);
+        double phi[4] = {1.0, x, x * x, x * x * x};
+        double yy = y0[p] * exp(lb[step * n_paths + p]);
+        for (int i = 0; i < 4; ++i) {
+            for (int k = 0; k < 4; ++k) M[i][k] += phi[i] * phi[k];
+            v[i] += phi[i] * yy;
+        }
+        ++n;
+    }
+    if (!(n >= 8 && solve4(M, v, c))) c[0] = c[1] = c[2] = c[3] = 0.0;
+    for (int i = 0; i < 4; ++i) coef[i] = c[i];
+    return n;
+}
+
+/* EE at t from a continuation polynomial: (1/n) sum_{alive} max(C(x), 0). */
+double eqr_berm_ee(double a, double b, double sigma, double r0, uint64_t n_paths, uint64_t step,
+                   double t, const double* r, const int32_t* tau_year, const double* coef) {
+    double s = 0.0;
+    for (uint64_t p = 0; p < n_paths; ++p) {
+        if (tau_year[p] >= 0 && (double)tau_year[p] < t - 1e-9) continue;
+        double x = eqr_berm_x(a, b, sigma, r0, t, r[step * n_paths + p]);
+        double c = coef[0] + x * (coef[1] + x * (coef[2] + x * coef[3]));
+        s += (c > 0.0) ? c : 0.0;
+    }
+    return s / (double)n_paths;
+}

@@ -109,6 +109,24 @@ int eqr_portfolio_exposure(double a, double b, double sigma, double r0, int n_tr
                            double* trade_pfe, double* trade_cva, double* path_mtm,
                            double* set_mtm, int n_threads);
 
+/* ---- Bermudan payer swaptions, Longstaff-Schwartz with a degree-3 basis ---- */
+void eqr_berm_simulate(double a, double b, double sigma, double r0, double dt, uint32_t n_steps,
+                       uint64_t n_paths, const double* z, double* r_out, double* lb_out);
+double eqr_berm_x(double a, double b, double sigma, double r0, double t, double r);
+int eqr_berm_policy(double a, double b, double sigma, double r0, int n_trades, const double* notional,
+                    const double* strike, const int32_t* first_ex, const int32_t* n_ex, int max_ex,
+                    uint32_t steps_per_year, uint64_t n_paths, const double* r, const double* lb,
+                    double* ex_coef, int32_t* tau_year, double* y0, double* price);
+int eqr_berm_apply(double a, double b, double sigma, double r0, int n_trades, const double* notional,
+                   const double* strike, const int32_t* first_ex, const int32_t* n_ex, int max_ex,
+                   uint32_t steps_per_year, uint64_t n_paths, const double* r, const double* lb,
+                   const double* ex_coef, int32_t* tau_year, double* y0);
+uint64_t eqr_berm_judge_fit(double a, double b, double sigma, double r0, uint64_t n_paths,
+                            uint64_t step, double t, const double* r, const double* lb,
+                            const int32_t* tau_year, const double* y0, double* coef);
+double eqr_berm_ee(double a, double b, double sigma, double r0, uint64_t n_paths, uint64_t step,
+                   double t, const double* r, const int32_t* tau_year, const double* coef);
+
 /* Maximum OpenMP threads the oracle would use (for the CPU-baseline report). */
 int eqr_max_threads(void);
 
