@@ -66,6 +66,26 @@ def algorithmic_work(port, dates, n_paths, n_sets):
     return flops, bytes_, u_live
 
 
+PROFILE = os.path.join(ROOT, "profiles", "value_kernel_hw1f_1000swaps_100kpaths_360dates.md")
+
+
+def measured_traffic(kernel="value_kernel"):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    of the default workload (each counter scaled by what it reads for a plain 1 GiB copy)."""
+    try:
+        with open(PROFILE) as f:
+            in_traffic = False
+            for line in f:
+                if line.startswith("## "):
+                    in_traffic = line.startswith("## HBM traffic")
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if in_traffic and len(cells) == 6 and cells[0].startswith(f"`eqhip::{kernel}("):
+                    return float(cells[5]) * 1e6
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(port, dates, n_steps, model, seed, disc, target_s=15.0):
     """The oracle (CPU FP64 restatement of the reference, OpenMP over paths) on a bounded path sample
     of the same workload; returns (units/s, threads, sample description, oracle CVA)."""
@@ -162,6 +182,8 @@ def main():
     v_ms = float(np.mean(value_ms))
     achieved = flops / (v_ms / 1000.0) / 1e12
     tm = plan.timing()
+    default_workload = (args.trades, args.paths, args.years, args.seed, args.disc) == (1000, 100_000, 30, 7, "exact")
+    traffic = measured_traffic() if default_workload else None
     out = {
         "metric": METRIC,
         "value": value,
@@ -180,9 +202,12 @@ def main():
                    "discretisation": args.disc, "seed": args.seed, "n_sets": port.n_sets,
                    "outputs": "set EE/PFE95/CVA, trade EE/CVA, PV/DV01/bucketed DV01 (20 pillars)",
                    "live_units": u_live},
-        "roofline": {"kernel": "value_kernel", "bound": "fp64-valu", "achieved": achieved,
+        "roofline": {"kernel": "value_kernel", "bound": "mfma", "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": None, "algorithmic_flops": flops, "algorithmic_bytes": bytes_,
+                     "traffic": traffic,
+                     "compute": "fp64 on the vector ALUs (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
+                     "traffic_source": os.path.relpath(PROFILE, ROOT) if traffic else None,
+                     "algorithmic_flops": flops, "algorithmic_bytes": bytes_,
                      "value_kernel_ms": v_ms, "exposure_kernels_ms": float(np.mean(total_kernel_ms)),
                      "paths_per_tile": tm["paths_per_tile"], "tiles_per_block": tm["tiles_per_block"],
                      "blocks": tm["n_value_blocks"]},

@@ -488,7 +488,7 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                     const int j = item.y + q;
                     // a matured class contributes exact zeros
                     const double e = (icnt[i] == 0) ? 0.0 : g.tr_absN[j] * acc[i][q];
-                    g.partial[((uint64_t)g.tr_orig[j] * g.n_dates + k) * n_parts + chunk * g.n_slices + sl] = e;
+                    g.partial[((uint64_t)k * n_parts + chunk * g.n_slices + sl) * g.n_trades + g.tr_orig[j]] = e;
                 }
             }
         }
@@ -661,15 +661,18 @@ __global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow*
 }
 
 // Per-trade EE: partial sums over chunks in chunk order, divided by the effective path count.
+// partial is [date][part][trade], so the lanes of a wave (consecutive trades) read contiguously;
+// trade_ee is [trade][date].
 __global__ void trade_ee_kernel(const double* partial, int n_trades, int n_dates, int n_chunks,
                                 const uint64_t* n_eff, double* trade_ee) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_trades * n_dates) return;
-    int k = i % n_dates;
-    const double* p = partial + (uint64_t)i * n_chunks;
+    const int k = i / n_trades;
+    const int j = i - k * n_trades;
+    const double* p = partial + (uint64_t)k * n_chunks * n_trades + j;
     double s = 0.0;
-    for (int c = 0; c < n_chunks; ++c) s += p[c];
-    trade_ee[i] = s / (double)n_eff[k];
+    for (int c = 0; c < n_chunks; ++c) s += p[(uint64_t)c * n_trades];
+    trade_ee[(uint64_t)j * n_dates + k] = s / (double)n_eff[k];
 }
 
 // CVA = sum_k (1-R) EE_k (S_{k-1} - S_k) P(0,t_k | r0), S = exp(-h t), exposure.rs:242-275.

@@ -66,7 +66,7 @@ struct ValuePlan {
     const double* tr_N;
     const double* tr_NK;
     const int* tr_orig;       // sorted -> caller trade index
-    double* partial;          // [n_trades][n_dates][n_chunks * n_slices] per-trade EE partial sums
+    double* partial;          // [n_dates][n_chunks * n_slices][n_trades] per-trade EE partial sums
     double* path_mtm;         // optional [n_dates][n_trades][n_paths]
 };
 
