@@ -13,73 +13,21 @@
 #include <vector>
 
 #include "eqhip.h"
+#include "eqhip_host.hpp"
 #include "eqhip_kernels.hpp"
 
 using namespace eqhip;
 
+namespace eqhip_host {
+std::string& last_error() {
+    thread_local std::string msg;
+    return msg;
+}
+}  // namespace eqhip_host
+
+using namespace eqhip_host;
+
 namespace {
-
-thread_local std::string g_last_error;
-
-struct Error : std::runtime_error {
-    int code;
-    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-[[noreturn]] void raise(int code, const std::string& msg) { throw Error(code, msg); }
-
-void hip_check(hipError_t e, const char* what) {
-    if (e != hipSuccess) raise(EQHIP_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
-}
-#define HIPCHK(x) hip_check((x), #x)
-
-template <class F>
-int guarded(F&& f) {
-    try {
-        f();
-        return EQHIP_OK;
-    } catch (const Error& e) {
-        g_last_error = e.what();
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        g_last_error = "out of host memory";
-        return EQHIP_E_CAPACITY;
-    } catch (const std::exception& e) {
-        g_last_error = e.what();
-        return EQHIP_E_INVALID;
-    }
-}
-
-// Device allocation owned by a plan; freed when the plan dies.
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
-    void alloc(size_t n) {
-        if (p) {
-            (void)hipFree(p);
-            p = nullptr;
-        }
-        bytes = n;
-        if (n) HIPCHK(hipMalloc(&p, n));
-    }
-    template <class T>
-    T* as() const {
-        return static_cast<T*>(p);
-    }
-};
-
-// Synchronous copy: the source vector may be a temporary of the caller.
-template <class T>
-void upload(DevBuf& b, const std::vector<T>& v, hipStream_t) {
-    b.alloc(std::max<size_t>(1, v.size()) * sizeof(T));
-    if (!v.empty()) HIPCHK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-}
 
 // Host twin of the affine ZCB (models/hull_white.rs:50-80), used only for the par rate of
 // use_par_rate swaps (api.rs:53-57, irs.rs:126-145), a deterministic set-up scalar.
@@ -112,14 +60,9 @@ bool is_reset_date(double start, const double* pay, int m, double t) {
 
 }  // namespace
 
-struct eqhip_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    size_t lds_limit = 65536;
-};
-
 struct eqhip_plan {
     eqhip_context* ctx = nullptr;
+    int device = 0;  // kept so that destroying the plan never touches its context
     eqhip_model_hw1f model{};
     eqhip_mc mc{};
     int n_dates = 0, n_trades = 0, S = 0, C = 0, NF = 0, D = 0, Fmax = 0, n_slots = 0;
@@ -769,6 +712,7 @@ void profile_batch(eqhip_context* ctx, double a, double b, double sigma, double 
     eqhip_mc mc{n_paths, n_steps, seed, rng, disc};
     eqhip_plan P;
     P.ctx = ctx;
+    P.device = ctx->device;
     build_plan(P, &model, &pf, times, nd, &mc, 0);
     std::vector<double> ee((size_t)n_trades * nd), pfe((size_t)n_trades * nd);
     eqhip_outputs out{};
@@ -1037,6 +981,7 @@ void cva_adjoint(eqhip_plan& P, const double* host_draws, int set, const eqhip_c
 // ---------------------------------------------------------------------------------------------
 struct eqhip_batch {
     eqhip_context* ctx = nullptr;
+    int device = 0;
     eqhip_model_hw1f model{};
     std::vector<std::unique_ptr<eqhip_plan>> jobs;
     uint64_t max_paths = 0;
@@ -1285,11 +1230,11 @@ int eqhip_is_gpu_backend_available(void) {
 
 size_t eqhip_last_error(char* buf, size_t len) {
     if (buf && len) {
-        size_t n = std::min(len - 1, g_last_error.size());
-        std::memcpy(buf, g_last_error.data(), n);
+        size_t n = std::min(len - 1, last_error().size());
+        std::memcpy(buf, last_error().data(), n);
         buf[n] = '\0';
     }
-    return g_last_error.size();
+    return last_error().size();
 }
 
 int eqhip_context_create(int device, eqhip_context** out) {
@@ -1338,6 +1283,7 @@ int eqhip_plan_create(eqhip_context* ctx, const eqhip_model_hw1f* model, const e
         check_ctx(ctx);
         auto P = std::make_unique<eqhip_plan>();
         P->ctx = ctx;
+        P->device = ctx->device;
         build_plan(*P, model, portfolio, dates, n_dates, mc, want_trade_ee);
         *out = P.release();
     });
@@ -1361,7 +1307,7 @@ int eqhip_plan_timing(const eqhip_plan* plan, eqhip_timing* out) {
 int eqhip_plan_destroy(eqhip_plan* plan) {
     return guarded([&] {
         if (!plan) return;
-        (void)hipSetDevice(plan->ctx->device);
+        (void)hipSetDevice(plan->device);
         delete plan;
     });
 }
@@ -1373,6 +1319,7 @@ int eqhip_exposure_run(eqhip_context* ctx, const eqhip_model_hw1f* model, const 
         check_ctx(ctx);
         eqhip_plan P;
         P.ctx = ctx;
+    P.device = ctx->device;
         int want_trade = out && (out->trade_ee || out->trade_cva || out->path_mtm);
         build_plan(P, model, portfolio, dates, n_dates, mc, want_trade);
         execute_plan(P, host_draws, credit, out);
@@ -1500,6 +1447,7 @@ int eqhip_batch_create(eqhip_context* ctx, const eqhip_model_hw1f* model, int32_
         if (rng != EQHIP_RNG_PHILOX) raise(EQHIP_E_INVALID, "batches draw from the Philox stream (rng = EQHIP_RNG_PHILOX)");
         auto B = std::make_unique<eqhip_batch>();
         B->ctx = ctx;
+        B->device = ctx->device;
         B->model = model ? *model : eqhip_model_hw1f{};
         B->n_steps = n_steps;
         B->seed = seed;
@@ -1509,6 +1457,7 @@ int eqhip_batch_create(eqhip_context* ctx, const eqhip_model_hw1f* model, int32_
         for (int j = 0; j < n_jobs; ++j) {
             auto P = std::make_unique<eqhip_plan>();
             P->ctx = ctx;
+            P->device = ctx->device;
             eqhip_mc mc{n_paths[j], n_steps, seed, rng, disc};
             build_plan(*P, model, &portfolios[j], dates, n_dates, &mc, want_trade_ee, true);
             B->det = P->det;
@@ -1552,7 +1501,7 @@ int eqhip_batch_timing(const eqhip_batch* batch, eqhip_timing* out, char* tune_l
 int eqhip_batch_destroy(eqhip_batch* batch) {
     return guarded([&] {
         if (!batch) return;
-        (void)hipSetDevice(batch->ctx->device);
+        (void)hipSetDevice(batch->device);
         delete batch;
     });
 }

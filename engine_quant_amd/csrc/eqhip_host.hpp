@@ -1,0 +1,86 @@
+// Host-side helpers shared by the translation units of libeqhip: error capture for the C ABI
+// (status code + thread-local message), owned device buffers and the context object.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "eqhip.h"
+
+namespace eqhip_host {
+
+// The calling thread's last error message (defined once, in eqhip_api.hip).
+std::string& last_error();
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void raise(int code, const std::string& msg) { throw Error(code, msg); }
+
+inline void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) raise(EQHIP_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPCHK(x) ::eqhip_host::hip_check((x), #x)
+
+// Runs f, mapping exceptions to EQHIP_* status codes and the thread-local message.
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return EQHIP_OK;
+    } catch (const Error& e) {
+        last_error() = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        last_error() = "out of host memory";
+        return EQHIP_E_CAPACITY;
+    } catch (const std::exception& e) {
+        last_error() = e.what();
+        return EQHIP_E_INVALID;
+    }
+}
+
+// Device allocation owned by its holder; freed when the holder dies.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void alloc(size_t n) {
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+        }
+        bytes = n;
+        if (n) HIPCHK(hipMalloc(&p, n));
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+// Synchronous copy: the source vector may be a temporary of the caller.
+template <class T>
+void upload(DevBuf& b, const std::vector<T>& v, hipStream_t) {
+    b.alloc(std::max<size_t>(1, v.size()) * sizeof(T));
+    if (!v.empty()) HIPCHK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+}  // namespace eqhip_host
+
+struct eqhip_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t lds_limit = 65536;
+};

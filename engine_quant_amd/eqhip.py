@@ -31,7 +31,10 @@ EXPORTED = [
     "eqhip_unilateral_cva_from_exposure_batch", "eqhip_pv_dv01", "eqhip_plan_create",
     "eqhip_plan_execute", "eqhip_plan_timing", "eqhip_plan_destroy", "eqhip_exposure_run",
     "eqhip_batch_create", "eqhip_batch_execute", "eqhip_batch_timing", "eqhip_batch_destroy",
-    "eqhip_cva_adjoint",
+    "eqhip_cva_adjoint", "eqhip_berm_create", "eqhip_berm_fit_policy", "eqhip_berm_pathwise",
+    "eqhip_berm_exposure_lsm", "eqhip_berm_train", "eqhip_berm_exposure_mlp", "eqhip_berm_mse",
+    "eqhip_berm_get_params", "eqhip_berm_set_params", "eqhip_berm_mlp_grad", "eqhip_berm_mlp_forward",
+    "eqhip_berm_adam_step", "eqhip_berm_get_timing", "eqhip_berm_destroy",
 ]
 
 

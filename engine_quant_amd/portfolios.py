@@ -77,3 +77,28 @@ def batched_sets(n_sets=64, seed=7, trade_range=(10, 2000), path_range=(10_000, 
         sets.append((swap_portfolio(int(n_tr[s]), seed=seed * 1000 + s, pillars=pillars, zeros=zeros),
                      int(n_pa[s])))
     return sets
+
+
+def _model_zcb(model, tau):
+    a, b, sigma, r0 = model
+    tau = np.asarray(tau, dtype=np.float64)
+    bf = (1.0 - np.exp(-a * tau)) / a
+    af = np.exp(((bf - tau) * (a * a * b - sigma * sigma / 2.0)) / (a * a) - sigma * sigma * bf * bf / (4.0 * a))
+    return af * np.exp(-bf * r0)
+
+
+def bermudan_book(n_trades=200, seed=7, model=MODEL):
+    """Bermudan payer swaptions: first exercise E ~ U{1..10}y, underlying tenor L ~ U{5..20}y (annual
+    exercise E..E+L-1 into the co-terminal swap ending at E+L, annual fixed payments), notional
+    lognormal(ln 1e7, 1), strike = forward swap rate under the model at r0 + N(0, 50bp)."""
+    rng = np.random.default_rng(seed + 500)
+    first = rng.integers(1, 11, size=n_trades).astype(np.int32)
+    tenor = rng.integers(5, 21, size=n_trades).astype(np.int32)
+    notional = np.exp(rng.normal(np.log(1e7), 1.0, size=n_trades))
+    shock = rng.normal(0.0, 0.0050, size=n_trades)
+    strike = np.empty(n_trades)
+    for j in range(n_trades):
+        E, L = int(first[j]), int(tenor[j])
+        p = _model_zcb(model, np.arange(E, E + L + 1, dtype=np.float64))
+        strike[j] = (p[0] - p[-1]) / np.sum(p[1:]) + shock[j]
+    return {"notional": notional, "strike": strike, "first_ex": first, "n_ex": tenor}

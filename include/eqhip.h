@@ -219,6 +219,88 @@ int eqhip_batch_execute(eqhip_batch* batch, const eqhip_credit* credit, eqhip_ou
 int eqhip_batch_timing(const eqhip_batch* batch, eqhip_timing* out, char* tune_log, size_t len);
 int eqhip_batch_destroy(eqhip_batch* batch);
 
+/* ---- Bermudan payer swaptions and the learned continuation value ---------------------------
+ * No reference counterpart: the reference's only regression method is the quadratic
+ * Longstaff-Schwartz fit of rust/crates/engine-core/src/payoff/lsm.rs:73-107 (GBM payoffs). These
+ * entry points hold a book of Bermudan payer swaptions under the same HW1F model, fit a degree-3
+ * LSM exercise policy, and estimate exposures either by the per-date polynomial regression (the
+ * fallback and judge) or by a per-trade continuation network 3 -> 64 -> 64 -> 64 -> 1 (tanh) on
+ * (short rate, log numeraire, time), trained and evaluated with BF16 matrix-core products and FP32
+ * accumulation. Cash settlement: exercising at year T pays
+ * N [(1 - P(T, M)) - K sum_{m=1..M-T} P(T, T+m)], M = first_exercise + n_exercise, and ends the
+ * trade, so exposure counts only paths not exercised before t. */
+typedef struct eqhip_berm eqhip_berm;
+
+typedef struct {
+    int32_t n_trades;
+    const double* notional;
+    const double* strike;
+    const int32_t* first_exercise; /* E >= 1 (years) */
+    const int32_t* n_exercise;     /* exercise years E .. E + n_exercise - 1, annual */
+} eqhip_berm_book;
+
+typedef struct {
+    uint64_t n_train, n_infer;      /* paths of the training set and of the inference set */
+    uint32_t steps_per_year;        /* grid; monitoring dates are k / steps_per_year, k = 1..n_steps */
+    uint32_t horizon_years;         /* n_steps = steps_per_year * horizon_years, >= every maturity */
+    uint64_t seed_train, seed_infer;
+    int32_t rng;                    /* EQHIP_RNG_PHILOX, or EQHIP_RNG_HOST_DRAWS (z_train, z_infer) */
+} eqhip_berm_mc;
+
+typedef struct {
+    uint32_t steps;                 /* Adam steps */
+    uint32_t batch;                 /* samples per network per step, multiple of 64 */
+    float lr, lr_final;             /* cosine decay from lr to lr_final */
+    float beta1, beta2, eps;
+    uint64_t seed;                  /* initial weights and sample draws */
+    /* inputs u0 = (r - r_shift) r_scale, u1 = lb lb_scale + lb_shift, u2 = t t_scale + t_shift;
+     * the network's output is the continuation value per unit notional times y_scale */
+    double r_shift, r_scale, lb_scale, lb_shift, t_scale, t_shift, y_scale;
+} eqhip_berm_mlp_cfg;
+
+typedef struct {
+    double simulate_ms, policy_ms, apply_ms, judge_ms, train_ms, infer_ms;
+} eqhip_berm_timing;
+
+/* Simulates both path sets (exact scheme, trapezoid log-numeraire) and keeps them resident.
+ * z_train [n_steps][n_train] and z_infer [n_steps][n_infer] are read only in host-draw mode. */
+int eqhip_berm_create(eqhip_context* ctx, const eqhip_model_hw1f* model, const eqhip_berm_book* book,
+                      const eqhip_berm_mc* mc, const double* z_train, const double* z_infer,
+                      eqhip_berm** out);
+/* LSM exercise policy fitted backwards on the training set, then applied to the inference set.
+ * Nullable outputs: ex_coef [n_trades][max n_exercise][4], price_train / price_infer [n_trades]. */
+int eqhip_berm_fit_policy(eqhip_berm* berm, double* ex_coef, double* price_train, double* price_infer);
+/* Exercise years and discounted pathwise cashflows of set 0 (training) or 1 (inference):
+ * tau [n_trades][n_paths] (-1 = never exercised), y0 [n_trades][n_paths]; either may be NULL. */
+int eqhip_berm_pathwise(eqhip_berm* berm, int32_t set, int32_t* tau, double* y0);
+/* Polynomial judge on set 0 or 1: per (trade, date) degree-3 regression of the pathwise value on
+ * the standardised short rate over the alive paths; ee [n_trades][n_steps], coef (nullable)
+ * [n_trades][n_steps][4]. The coefficients are kept for eqhip_berm_mse. */
+int eqhip_berm_exposure_lsm(eqhip_berm* berm, int32_t set, double* ee, double* coef);
+/* Initialises (seeded) and trains the networks on the training set with Adam. loss_curve
+ * (nullable) [steps]: mean over trades of each step's minibatch loss. */
+int eqhip_berm_train(eqhip_berm* berm, const eqhip_berm_mlp_cfg* cfg, float* loss_curve);
+/* Network exposure on set 0 or 1: ee [n_trades][n_steps]. */
+int eqhip_berm_exposure_mlp(eqhip_berm* berm, int32_t set, double* ee);
+/* Out-of-sample accuracy on set 1 against the pathwise values: per-trade mean squared error per
+ * unit notional of the network and of the polynomials last fitted by eqhip_berm_exposure_lsm. */
+int eqhip_berm_mse(eqhip_berm* berm, double* mse_mlp, double* mse_poly);
+/* Network parameters, float32 [n_trades][8641] (W1 [64][4] incl. bias, W2, b2, W3, b3, w4, b4). */
+int eqhip_berm_get_params(eqhip_berm* berm, float* params);
+int eqhip_berm_set_params(eqhip_berm* berm, const eqhip_berm_mlp_cfg* cfg, const float* params);
+/* Loss and gradient of every network on explicit samples (n per network, multiple of 64):
+ * u [n_trades][n][3], target / mask [n_trades][n]; loss [n_trades], grad [n_trades][8641]. */
+int eqhip_berm_mlp_grad(eqhip_berm* berm, const float* u, const float* target, const float* mask, uint32_t n,
+                        float* loss, float* grad);
+/* Network outputs on explicit inputs: u [n_trades][n][3] -> y [n_trades][n]. */
+int eqhip_berm_mlp_forward(eqhip_berm* berm, const float* u, uint32_t n, float* y);
+/* One Adam step of every network from the gradient of eqhip_berm_mlp_grad's samples (for tests);
+ * step counts from 1. */
+int eqhip_berm_adam_step(eqhip_berm* berm, const float* u, const float* target, const float* mask, uint32_t n,
+                         uint32_t step, float lr);
+int eqhip_berm_get_timing(const eqhip_berm* berm, eqhip_berm_timing* out);
+int eqhip_berm_destroy(eqhip_berm* berm);
+
 #ifdef __cplusplus
 }
 #endif

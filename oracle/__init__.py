@@ -81,6 +81,19 @@ def lib():
             _D, _D, _D, _D, _I, _I, _PI32, _PD, _PD, _PD, _PI32, _PD, _PD, _PD, _I, _U32, _U64, _I,
             _PD, _D, _D, _PD, _PD, _PD, _PD, _PD, _PD, _PD, _PD, _I]
         L.eqr_max_threads.restype = _I
+        L.eqr_berm_simulate.argtypes = [_D, _D, _D, _D, _D, _U32, _U64, _PD, _PD, _PD]
+        L.eqr_berm_x.restype = _D
+        L.eqr_berm_x.argtypes = [_D] * 6
+        L.eqr_berm_policy.restype = _I
+        L.eqr_berm_policy.argtypes = [_D, _D, _D, _D, _I, _PD, _PD, _PI32, _PI32, _I, _U32, _U64, _PD, _PD,
+                                      _PD, _PI32, _PD, _PD]
+        L.eqr_berm_apply.restype = _I
+        L.eqr_berm_apply.argtypes = [_D, _D, _D, _D, _I, _PD, _PD, _PI32, _PI32, _I, _U32, _U64, _PD, _PD,
+                                     _PD, _PI32, _PD]
+        L.eqr_berm_judge_fit.restype = _U64
+        L.eqr_berm_judge_fit.argtypes = [_D, _D, _D, _D, _U64, _U64, _D, _PD, _PD, _PI32, _PD, _PD]
+        L.eqr_berm_ee.restype = _D
+        L.eqr_berm_ee.argtypes = [_D, _D, _D, _D, _U64, _U64, _D, _PD, _PI32, _PD]
         _lib = L
     return _lib
 
@@ -278,3 +291,80 @@ def portfolio_exposure(model, port, dates, n_steps, z, scheme=0, hazard_rate=0.0
 
 def max_threads():
     return lib().eqr_max_threads()
+
+
+# --------------------------------------------------------------------------- Bermudan swaptions
+def _i32(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.int32))
+
+
+def _book(book):
+    return (_f64(book["notional"]), _f64(book["strike"]), _i32(book["first_ex"]), _i32(book["n_ex"]))
+
+
+def berm_simulate(model, dt, z):
+    """Exact-scheme short rate and trapezoid log-numeraire (hull_white.rs:114-120) on one grid:
+    z [n_steps][n_paths] -> (r, lb), each [n_steps+1][n_paths]."""
+    a, b, sigma, r0 = model
+    z = _f64(z)
+    n_steps, n_paths = z.shape
+    r = np.empty((n_steps + 1, n_paths))
+    lb = np.empty((n_steps + 1, n_paths))
+    lib().eqr_berm_simulate(a, b, sigma, r0, dt, n_steps, n_paths, _dp(z), _dp(r), _dp(lb))
+    return r, lb
+
+
+def berm_x(model, t, r):
+    a, b, sigma, r0 = model
+    return lib().eqr_berm_x(a, b, sigma, r0, t, r)
+
+
+def berm_policy(model, book, steps_per_year, r, lb):
+    """Longstaff-Schwartz exercise policy (degree-3 basis in the standardised short rate) fitted on
+    the given paths; returns ex_coef [n][max_ex][4], tau (exercise year, -1 never) and y0 (pathwise
+    cashflow discounted to 0) [n][n_paths], and the in-sample price [n]."""
+    a, b, sigma, r0 = model
+    notional, strike, first_ex, n_ex = _book(book)
+    n = len(notional)
+    max_ex = int(n_ex.max())
+    r, lb = _f64(r), _f64(lb)
+    n_paths = r.shape[1]
+    coef = np.zeros((n, max_ex, 4))
+    tau = np.zeros((n, n_paths), dtype=np.int32)
+    y0 = np.zeros((n, n_paths))
+    price = np.zeros(n)
+    lib().eqr_berm_policy(a, b, sigma, r0, n, _dp(notional), _dp(strike), first_ex.ctypes.data_as(_PI32),
+                          n_ex.ctypes.data_as(_PI32), max_ex, steps_per_year, n_paths, _dp(r), _dp(lb),
+                          _dp(coef), tau.ctypes.data_as(_PI32), _dp(y0), _dp(price))
+    return {"ex_coef": coef, "tau": tau, "y0": y0, "price": price}
+
+
+def berm_apply(model, book, steps_per_year, r, lb, ex_coef):
+    """Forward application of a fitted policy to other paths: (tau, y0), each [n][n_paths]."""
+    a, b, sigma, r0 = model
+    notional, strike, first_ex, n_ex = _book(book)
+    n = len(notional)
+    ex_coef = _f64(ex_coef)
+    r, lb = _f64(r), _f64(lb)
+    n_paths = r.shape[1]
+    tau = np.zeros((n, n_paths), dtype=np.int32)
+    y0 = np.zeros((n, n_paths))
+    lib().eqr_berm_apply(a, b, sigma, r0, n, _dp(notional), _dp(strike), first_ex.ctypes.data_as(_PI32),
+                         n_ex.ctypes.data_as(_PI32), ex_coef.shape[1], steps_per_year, n_paths, _dp(r), _dp(lb),
+                         _dp(ex_coef), tau.ctypes.data_as(_PI32), _dp(y0))
+    return tau, y0
+
+
+def berm_judge(model, steps_per_year, r, lb, tau, y0, step):
+    """Continuation regression of one trade at grid step `step` over its alive paths:
+    (coef[4], n_alive, EE = mean over all paths of alive * max(C(x), 0))."""
+    a, b, sigma, r0 = model
+    r, lb = _f64(r), _f64(lb)
+    tau, y0 = _i32(tau), _f64(y0)
+    n_paths = r.shape[1]
+    t = step / float(steps_per_year)
+    coef = np.zeros(4)
+    n = lib().eqr_berm_judge_fit(a, b, sigma, r0, n_paths, step, t, _dp(r), _dp(lb),
+                                 tau.ctypes.data_as(_PI32), _dp(y0), _dp(coef))
+    ee = lib().eqr_berm_ee(a, b, sigma, r0, n_paths, step, t, _dp(r), tau.ctypes.data_as(_PI32), _dp(coef))
+    return coef, int(n), ee
