@@ -111,6 +111,107 @@ def cpu_baseline(port, dates, n_steps, model, seed, disc, target_s=15.0):
     return units / dt, threads, n_sample, dt, ref
 
 
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix
+# network FLOPs per evaluated sample: 3->64 (+bias), 64->64, 64->64, 64->1
+MLP_FLOPS = 2 * (3 * 64 + 64 * 64 + 64 * 64 + 64)
+
+
+def run_bermudan(args):
+    """Learned continuation value on the Bermudan book: 200 payer swaptions, LSM policy on 2^18
+    training paths, polynomial judge and BF16 network trained on the GPU, network exposure on 2^20
+    fresh paths x 120 quarterly dates. The timed step is the network's exposure pass."""
+    from engine_quant_amd import eqhip
+    from engine_quant_amd.berm import BermudanBook, MlpConfig
+    from engine_quant_amd.portfolios import MODEL, bermudan_book
+
+    spy, hz = 4, 30
+    bk = bermudan_book(args.berm_trades, seed=args.seed)
+    ctx = eqhip.Context(0)
+    B = BermudanBook(ctx, MODEL, bk["notional"], bk["strike"], bk["first_ex"], bk["n_ex"], args.berm_train,
+                     args.berm_infer, spy, hz, seed_train=args.seed * 101, seed_infer=args.seed * 211)
+    pol = B.fit_policy()
+    t0 = time.perf_counter()
+    ee_lsm, _ = B.exposure_lsm(1)
+    judge_ms = 1000.0 * (time.perf_counter() - t0)
+    B.exposure_lsm(0)  # out-of-sample polynomials for the accuracy comparison
+    cfg = MlpConfig.for_model(MODEL, hz, steps=args.berm_steps, batch=2048, lr=4e-3)
+    curve = B.train(cfg)
+    train_ms = B.timing()["train_ms"]
+    for _ in range(args.warmup):
+        B.exposure_mlp(1)
+    infer = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        ee_net = B.exposure_mlp(1)
+        infer.append(1000.0 * (time.perf_counter() - t0))
+        print(f"# step done: network exposure {infer[-1]:.1f} ms", file=sys.stderr, flush=True)
+    mse_net, mse_poly = B.mse()
+    tau, _ = B.pathwise(1, want_y0=False)
+    kmax = (bk["first_ex"] + bk["n_ex"] - 1) * spy
+    alive = 0
+    for j in range(B.n_trades):
+        t = tau[j]
+        exercised_step = np.where(t < 0, 10 ** 9, t * spy)
+        counts = np.bincount(np.minimum(exercised_step, kmax[j] + 1), minlength=kmax[j] + 2)
+        # alive at k <=> exercise step >= k: paths with step >= k
+        ge = np.cumsum(counts[::-1])[::-1]
+        alive += int(ge[1:kmax[j] + 1].sum())
+    del tau
+    units = int(kmax.sum()) * args.berm_infer
+    ms = float(np.median(infer))
+    dates = np.arange(1, spy * hz + 1) / spy
+    cva_net = ctx.unilateral_cva_batch(*MODEL, dates, ee_net, 0.02, 0.4)
+    cva_lsm = ctx.unilateral_cva_batch(*MODEL, dates, ee_lsm, 0.02, 0.4)
+    port_net, port_lsm = ee_net.sum(0), ee_lsm.sum(0)
+    achieved = alive * MLP_FLOPS / (ms / 1000.0) / 1e12
+    out = {
+        "metric": "trade-paths-dates/sec of the learned continuation-value exposure (Bermudan book, BF16 MFMA)",
+        "value": units / (ms / 1000.0), "unit": "trade-paths-dates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 (fp32 accumulate); fp64 paths", "data": "synthetic",
+        "config": {"workload": "bermudan-surrogate", "n_trades": B.n_trades, "n_train": args.berm_train,
+                   "n_infer": args.berm_infer, "dates": "quarterly x 120", "network": "3-64-64-64-1 tanh",
+                   "train_steps": args.berm_steps, "batch": 2048, "seed": args.seed},
+        "roofline": {"kernel": "mlp_eval_kernel<0>", "bound": "mfma", "achieved": achieved,
+                     "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None,
+                     "algorithmic_flops": alive * MLP_FLOPS, "alive_samples": alive},
+        "accuracy": {
+            "portfolio_ee_l1_rel_err": float(np.abs(port_net - port_lsm).sum() / port_lsm.sum()),
+            "portfolio_ee_max_rel_err_of_peak": float(np.abs(port_net - port_lsm).max() / port_lsm.max()),
+            "notional_weighted_trade_ee_abs_err": float(np.abs(ee_net - ee_lsm).sum() / ee_lsm.sum()),
+            "portfolio_cva_rel_err": float(abs(cva_net.sum() - cva_lsm.sum()) / cva_lsm.sum()),
+            "oos_mse_per_unit_notional": {"network": float(mse_net.mean()), "cubic": float(mse_poly.mean()),
+                                          "trades_where_network_better": float((mse_net < mse_poly).mean())}},
+        "timing_ms": {"simulate": B.timing()["simulate_ms"], "lsm_policy": B.timing()["policy_ms"],
+                      "policy_apply": B.timing()["apply_ms"], "lsm_judge_exposure": judge_ms, "train": train_ms,
+                      "network_exposure": ms},
+        "lsm_judge_rate": units / (judge_ms / 1000.0),
+        "price_mean": float(pol["price_infer"].mean()), "loss_first_last": [float(curve[:50].mean()),
+                                                                            float(curve[-200:].mean())],
+    }
+    if not args.no_cpu:
+        import oracle
+        n_cpu, trades = 1 << 17, list(range(32))
+        z = oracle.philox_normals(args.seed, n_cpu, spy * hz)
+        r, lb = oracle.berm_simulate(MODEL, 1.0 / spy, z)
+        sub = {k: np.asarray(v)[trades] for k, v in bk.items()}
+        t0 = time.perf_counter()
+        t_c, y_c = oracle.berm_apply(MODEL, sub, spy, r, lb, pol["ex_coef"][trades])
+        cpu_units = 0
+        for i, j in enumerate(trades):
+            for k in range(1, int(kmax[j]) + 1):
+                oracle.berm_judge(MODEL, spy, r, lb, t_c[i], y_c[i], k)
+            cpu_units += int(kmax[j]) * n_cpu
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cpu_units / cpu_s, "unit": "trade-paths-dates/s", "cores": 1, "kind": "port",
+                               "sample": f"plain LSM exposure (policy application + per-date cubic regression) of "
+                                         f"trades 0-31 on {n_cpu} paths, {cpu_s:.1f} s, one thread"}
+        out["speedup_vs_cpu_lsm"] = out["value"] / out["cpu_baseline"]["value"]
+    B.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +224,13 @@ def main():
     ap.add_argument("--disc", choices=["exact", "euler"], default="exact")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--workload", choices=["swaps", "bermudan"], default="swaps",
+                    help="swaps: the one-netting-set IRS exposure job (default); bermudan: the learned "
+                         "continuation-value exposure of the Bermudan book (its own JSON line)")
+    ap.add_argument("--berm-trades", type=int, default=200)
+    ap.add_argument("--berm-train", type=int, default=1 << 18)
+    ap.add_argument("--berm-infer", type=int, default=1 << 20)
+    ap.add_argument("--berm-steps", type=int, default=6000, help="Adam steps of the network training")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (hardware-counter scale factor)")
     args = ap.parse_args()
@@ -142,6 +250,10 @@ def main():
         print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
               "(paths would shard by global path index across GPUs)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "bermudan":
+        run_bermudan(args)
+        return
 
     from engine_quant_amd import eqhip
     from engine_quant_amd.portfolios import MODEL, monthly_dates, swap_portfolio, zero_curve

@@ -165,10 +165,10 @@ class BermudanBook:
         _check(_lib().eqhip_berm_fit_policy(self.handle, _dp(coef), _dp(pt), _dp(pi)))
         return {"ex_coef": coef, "price_train": pt, "price_infer": pi}
 
-    def pathwise(self, set_index):
+    def pathwise(self, set_index, want_y0=True):
         n = self.n_train if set_index == 0 else self.n_infer
         tau = np.zeros((self.n_trades, n), dtype=np.int32)
-        y0 = np.zeros((self.n_trades, n))
+        y0 = np.zeros((self.n_trades, n)) if want_y0 else None
         _check(_lib().eqhip_berm_pathwise(self.handle, set_index, tau.ctypes.data_as(_PI32), _dp(y0)))
         return tau, y0
 

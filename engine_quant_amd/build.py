@@ -18,8 +18,9 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 # Packed FP32 instructions are disabled for every HIP translation unit of this library.
+# Matrix-core results go straight to VGPRs (no accumulator-register reads in the network kernels).
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature",
-             "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-result"]
+             "-Xclang", "-packed-fp32-ops", "-mllvm", "-amdgpu-mfma-vgpr-form", "-Wall", "-Wno-unused-result"]
 
 EQHIP_SOURCES = ["eqhip_kernels.hip", "eqhip_api.hip", "eqhip_berm_kernels.hip", "eqhip_berm.hip"]
 HOST_SOURCES = ["host/params.cpp", "host/market.cpp", "host/model_product.cpp", "host/contexts.cpp",

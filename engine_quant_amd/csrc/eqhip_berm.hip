@@ -60,7 +60,7 @@ struct eqhip_berm {
     eqhip_berm_mlp_cfg cfg{};
     bool have_net = false;
     uint32_t adam_t = 0;
-    DevBuf master, m1, v1, frags, grad_part, loss_part, loss_step, grad_out, xs_u, xs_t, xs_m;
+    DevBuf master, m1, v1, frags, fwdvec, grad_part, loss_part, loss_step, grad_out, xs_u, xs_t, xs_m;
     eqhip_berm_timing timing{};
 
     Book book() const {
@@ -231,6 +231,7 @@ void alloc_net(eqhip_berm& B) {
     B.m1.alloc(np * 4);
     B.v1.alloc(np * 4);
     B.frags.alloc((size_t)B.n_trades * kNFrag * 64 * 16);
+    B.fwdvec.alloc((size_t)B.n_trades * kFwdStride * 4);
     B.grad_part.alloc(np * kWaves * 4);
     B.loss_part.alloc((size_t)B.n_trades * kWaves * 4);
     B.loss_step.alloc((size_t)B.n_trades * 4);
@@ -244,6 +245,9 @@ void alloc_net(eqhip_berm& B) {
 void pack(eqhip_berm& B) {
     hipLaunchKernelGGL(mlp_pack_kernel, dim3(kNFrag, B.n_trades), dim3(64), 0, B.st(), B.master.as<const float>(),
                        B.frags.as<uint4>());
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(mlp_fold_kernel, dim3(B.n_trades), dim3(256), 0, B.st(), B.master.as<const float>(),
+                       B.fwdvec.as<float>());
     HIPCHK(hipGetLastError());
 }
 
@@ -284,7 +288,7 @@ void stage_samples(eqhip_berm& B, const float* u, const float* target, const flo
 
 void train_explicit(eqhip_berm& B, uint32_t n) {
     hipLaunchKernelGGL(mlp_train_kernel<1>, dim3(B.n_trades), dim3(kThreads), 0, B.st(), B.book(), B.sets[0].view(),
-                       B.norm(), B.frags.as<const uint4>(), B.master.as<const float>(), (int)(n / 64), 0u,
+                       B.norm(), B.frags.as<const uint4>(), B.fwdvec.as<const float>(), (int)(n / 64), 0u,
                        (uint64_t)0, B.xs_u.as<const float>(), B.xs_t.as<const float>(), B.xs_m.as<const float>(),
                        B.grad_part.as<float>(), B.loss_part.as<float>());
     HIPCHK(hipGetLastError());
@@ -423,7 +427,7 @@ int eqhip_berm_train(eqhip_berm* B, const eqhip_berm_mlp_cfg* cfg, float* loss_c
         const int tpw = (int)(cfg->batch / 64);
         for (uint32_t s = 0; s < cfg->steps; ++s) {
             hipLaunchKernelGGL(mlp_train_kernel<0>, dim3(B->n_trades), dim3(kThreads), 0, B->st(), B->book(),
-                               B->sets[0].view(), B->norm(), B->frags.as<const uint4>(), B->master.as<const float>(),
+                               B->sets[0].view(), B->norm(), B->frags.as<const uint4>(), B->fwdvec.as<const float>(),
                                tpw, s, cfg->seed, nullptr, nullptr, nullptr, B->grad_part.as<float>(),
                                B->loss_part.as<float>());
             HIPCHK(hipGetLastError());
@@ -458,7 +462,7 @@ int eqhip_berm_exposure_mlp(eqhip_berm* B, int32_t set, double* ee) {
         const int rows = B->n_trades * B->n_steps;
         ensure_part(*B, B->part_bytes(set, 1));
         hipLaunchKernelGGL(mlp_eval_kernel<0>, dim3(S.n_chunks, B->n_trades), dim3(kThreads), 0, B->st(), B->book(),
-                           S.view(), B->norm(), B->frags.as<const uint4>(), B->master.as<const float>(), nullptr,
+                           S.view(), B->norm(), B->frags.as<const uint4>(), B->fwdvec.as<const float>(), nullptr,
                            B->d_part.as<double>());
         HIPCHK(hipGetLastError());
         std::vector<double> scale(B->n_trades);
@@ -486,7 +490,7 @@ int eqhip_berm_mse(eqhip_berm* B, double* mse_mlp, double* mse_poly) {
         const int rows = B->n_trades * B->n_steps;
         ensure_part(*B, B->part_bytes(1, 3));
         hipLaunchKernelGGL(mlp_eval_kernel<1>, dim3(S.n_chunks, B->n_trades), dim3(kThreads), 0, B->st(), B->book(),
-                           S.view(), B->norm(), B->frags.as<const uint4>(), B->master.as<const float>(),
+                           S.view(), B->norm(), B->frags.as<const uint4>(), B->fwdvec.as<const float>(),
                            B->sets[poly_set].judge_coef.as<const double>(), B->d_part.as<double>());
         HIPCHK(hipGetLastError());
         DevBuf d;
@@ -580,7 +584,7 @@ int eqhip_berm_mlp_forward(eqhip_berm* B, const float* u, uint32_t n, float* y) 
         dy.alloc(m * 4);
         HIPCHK(hipMemcpy(du.p, u, m * 12, hipMemcpyHostToDevice));
         hipLaunchKernelGGL(mlp_forward_kernel, dim3((n + 63) / 64, B->n_trades), dim3(64), 0, B->st(), (int)n,
-                           B->frags.as<const uint4>(), B->master.as<const float>(), du.as<const float>(),
+                           B->frags.as<const uint4>(), B->fwdvec.as<const float>(), du.as<const float>(),
                            dy.as<float>());
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(y, dy.p, m * 4, hipMemcpyDeviceToHost, B->st()));

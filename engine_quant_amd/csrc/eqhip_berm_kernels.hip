@@ -373,10 +373,12 @@ __global__ void berm_finalize(int n_rows, int n_chunks, const double* part, int 
 // ---------------------------------------------------------------------------------------------
 namespace {
 
-__device__ __forceinline__ float tanh_fast(float x) {
-    // 1 - 2 / (exp(2x) + 1) with the hardware exp2 and reciprocal (saturates to +-1 at overflow)
-    const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);
-    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+// r = 1 / (1 + 2^zs) for zs = 2 log2(e) z, so that tanh(z) = 1 - 2r: the forward operands carry the
+// factor 2 log2(e), and the next layer absorbs the affine map r -> 1 - 2r (weights -2W, bias
+// b + W 1), leaving the hardware exp2, one add and the reciprocal per activation. Saturates to
+// r = 0 / 1 (tanh = +-1) when 2^zs overflows / underflows.
+__device__ __forceinline__ float act_r(float zs) {
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(zs));
 }
 
 __device__ __forceinline__ bf16x8 as_bf8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
@@ -401,13 +403,15 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
 }
 
 // Forward operands of one network held in registers by every lane of a wave.
+// Forward operands of one network held in registers by every lane of a wave: the scaled BF16
+// weight fragments and the folded FP32 vectors (b2', b3', w4' = -2 w4, b4').
 struct FwdW {
     bf16x8 w1[4], w2[8], w3[8];
     f32x4 b2[4], b3[4], w4[4];
     float b4;
 };
 
-__device__ __forceinline__ void load_fwd(FwdW& W, const uint4* fr, const float* th, int l) {
+__device__ __forceinline__ void load_fwd(FwdW& W, const uint4* fr, const float* fv, int l) {
     const int g = l >> 4;
 #pragma unroll
     for (int f = 0; f < 4; ++f) W.w1[f] = as_bf8(fr[(kFragW1 + f) * 64 + l]);
@@ -421,48 +425,49 @@ __device__ __forceinline__ void load_fwd(FwdW& W, const uint4* fr, const float* 
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int f = 16 * R + 4 * g + i;
-            W.b2[R][i] = th[OFF_B2 + f];
-            W.b3[R][i] = th[OFF_B3 + f];
-            W.w4[R][i] = th[OFF_W4 + f];
+            W.b2[R][i] = fv[FV_B2 + f];
+            W.b3[R][i] = fv[FV_B3 + f];
+            W.w4[R][i] = fv[FV_W4 + f];
         }
-    W.b4 = th[OFF_B4];
+    W.b4 = fv[FV_B4];
 }
 
 // One 16-sample tile: bx is layer 1's B operand (lane group 0 holds u0, u1, u2, 1 of sample l%16).
-// Returns y of sample l%16 in every lane; h1..h3 are the FP32 activations in accumulator layout.
-__device__ __forceinline__ float mlp_fwd(const FwdW& W, bf16x8 bx, f32x4 (&h1)[4], f32x4 (&h2)[4],
-                                         f32x4 (&h3)[4]) {
+// Returns y of sample l%16 in every lane; r1..r3 are the FP32 activations r = (1 - tanh) / 2 in
+// accumulator layout.
+__device__ __forceinline__ float mlp_fwd(const FwdW& W, bf16x8 bx, f32x4 (&r1)[4], f32x4 (&r2)[4],
+                                         f32x4 (&r3)[4]) {
 #pragma unroll
     for (int R = 0; R < 4; ++R) {
         f32x4 z = {0.f, 0.f, 0.f, 0.f};
         z = mfma32(W.w1[R], bx, z);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) h1[R][i] = tanh_fast(z[i]);
+        for (int i = 0; i < 4; ++i) r1[R][i] = act_r(z[i]);
     }
-    bf16x8 q0 = pack_b(h1[0], h1[1]), q1 = pack_b(h1[2], h1[3]);
+    bf16x8 q0 = pack_b(r1[0], r1[1]), q1 = pack_b(r1[2], r1[3]);
 #pragma unroll
     for (int R = 0; R < 4; ++R) {
         f32x4 z = W.b2[R];
         z = mfma32(W.w2[2 * R], q0, z);
         z = mfma32(W.w2[2 * R + 1], q1, z);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) h2[R][i] = tanh_fast(z[i]);
+        for (int i = 0; i < 4; ++i) r2[R][i] = act_r(z[i]);
     }
-    q0 = pack_b(h2[0], h2[1]);
-    q1 = pack_b(h2[2], h2[3]);
+    q0 = pack_b(r2[0], r2[1]);
+    q1 = pack_b(r2[2], r2[3]);
 #pragma unroll
     for (int R = 0; R < 4; ++R) {
         f32x4 z = W.b3[R];
         z = mfma32(W.w3[2 * R], q0, z);
         z = mfma32(W.w3[2 * R + 1], q1, z);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) h3[R][i] = tanh_fast(z[i]);
+        for (int i = 0; i < 4; ++i) r3[R][i] = act_r(z[i]);
     }
     float part = 0.f;
 #pragma unroll
     for (int R = 0; R < 4; ++R)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) part += W.w4[R][i] * h3[R][i];
+        for (int i = 0; i < 4; ++i) part += W.w4[R][i] * r3[R][i];
     part += __shfl_xor(part, 16);
     part += __shfl_xor(part, 32);
     return part + W.b4;
@@ -536,22 +541,45 @@ __global__ void mlp_pack_kernel(const float* master, uint4* frags) {
     const int m = l & 15, g = l >> 4;
     const float* th = master + (size_t)net * kParamStride;
     bf16x8 v;
-    if (f < kFragW2) {
+    if (f < kFragW2) {  // layer 1 (incl. its bias column), scaled by 2 log2(e)
 #pragma unroll
-        for (int i = 0; i < 8; ++i) v[i] = (__bf16)((g == 0 && i < 4) ? th[OFF_W1 + (16 * f + m) * 4 + i] : 0.0f);
+        for (int i = 0; i < 8; ++i)
+            v[i] = (__bf16)((g == 0 && i < 4) ? kActScale * th[OFF_W1 + (16 * f + m) * 4 + i] : 0.0f);
     } else {
         const int idx = f - kFragW2;
         const int mat = idx >> 3, R = (idx & 7) >> 1, Q = idx & 1;
         const int off = (mat == 0 || mat == 2) ? OFF_W2 : OFF_W3;
-        const bool trans = mat >= 2;
+        const bool trans = mat >= 2;  // backward operands W^T are unscaled
+        const float scale = trans ? 1.0f : -2.0f * kActScale;
         const int row = 16 * R + m;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int kf = 32 * Q + (i < 4 ? 4 * g + i : 16 + 4 * g + (i - 4));
-            v[i] = (__bf16)(trans ? th[off + kf * kH + row] : th[off + row * kH + kf]);
+            v[i] = (__bf16)(scale * (trans ? th[off + kf * kH + row] : th[off + row * kH + kf]));
         }
     }
     frags[((size_t)net * kNFrag + f) * 64 + l] = __builtin_bit_cast(uint4, v);
+}
+
+// Folded FP32 forward vectors of every network: b' = 2 log2(e) (b + W 1) for layers 2 and 3,
+// w4' = -2 w4, b4' = b4 + sum(w4). Grid n_nets, 256 threads.
+__global__ void mlp_fold_kernel(const float* master, float* fwdvec) {
+    const int net = blockIdx.x, t = threadIdx.x;
+    const float* th = master + (size_t)net * kParamStride;
+    float* fv = fwdvec + (size_t)net * kFwdStride;
+    if (t < 128) {
+        const int f = t & 63;
+        const int ow = (t < 64) ? OFF_W2 : OFF_W3, ob = (t < 64) ? OFF_B2 : OFF_B3;
+        float rs = 0.0f;
+        for (int k = 0; k < kH; ++k) rs += th[ow + f * kH + k];
+        fv[(t < 64 ? FV_B2 : FV_B3) + f] = kActScale * (th[ob + f] + rs);
+    } else if (t < 192) {
+        fv[FV_W4 + (t - 128)] = -2.0f * th[OFF_W4 + (t - 128)];
+    } else if (t == 192) {
+        float s = 0.0f;
+        for (int k = 0; k < kH; ++k) s += th[OFF_W4 + k];
+        fv[FV_B4] = th[OFF_B4] + s;
+    }
 }
 
 // One training step of every network (one block each): forward, backward and the weight-gradient
@@ -559,7 +587,7 @@ __global__ void mlp_pack_kernel(const float* master, uint4* frags) {
 // (counter = sample, step, net), SRC 1 reads explicit samples.
 template <int SRC>
 __global__ void __launch_bounds__(kThreads) mlp_train_kernel(Book bk, PathSet ps, Norm nm, const uint4* frags,
-                                                             const float* master, int tiles_per_wave, uint32_t step,
+                                                             const float* fwdvec, int tiles_per_wave, uint32_t step,
                                                              uint64_t seed, const float* xu, const float* xt,
                                                              const float* xm, float* grad_part, float* loss_part) {
     const int j = blockIdx.x;
@@ -570,11 +598,10 @@ __global__ void __launch_bounds__(kThreads) mlp_train_kernel(Book bk, PathSet ps
     __shared__ __attribute__((aligned(16))) __bf16 sH[kWaves][kH * 16];
     __shared__ __attribute__((aligned(16))) __bf16 sU[kWaves][16 * 16];
     const uint4* fr = frags + (size_t)j * kNFrag * 64;
-    const float* th = master + (size_t)j * kParamStride;
     for (int q = threadIdx.x; q < 16 * 64; q += kThreads) sWT[q] = fr[kFragW2T * 64 + q];
     for (int q = l; q < 16 * 16; q += 64) sU[w][q] = (__bf16)0.0f;
     FwdW W;
-    load_fwd(W, fr, th, l);
+    load_fwd(W, fr, fwdvec + (size_t)j * kFwdStride, l);
     f32x4 dW1[4], dW2[16], dW3[16], gb2[4], gb3[4], gw4[4];
     const f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -619,6 +646,14 @@ __global__ void __launch_bounds__(kThreads) mlp_train_kernel(Book bk, PathSet ps
         }
         f32x4 h1[4], h2[4], h3[4];
         const float y = mlp_fwd(W, input_operand(g, u0, u1, u2), h1, h2, h3);
+#pragma unroll
+        for (int R = 0; R < 4; ++R)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {  // r -> tanh = 1 - 2r
+                h1[R][q] = fmaf(-2.0f, h1[R][q], 1.0f);
+                h2[R][q] = fmaf(-2.0f, h2[R][q], 1.0f);
+                h3[R][q] = fmaf(-2.0f, h3[R][q], 1.0f);
+            }
         const float res = (y - tgt) * msk;
         const float dy = inv_b2 * res;
         if (g == 0) {
@@ -631,7 +666,7 @@ __global__ void __launch_bounds__(kThreads) mlp_train_kernel(Book bk, PathSet ps
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 gw4[R][q] += h3[R][q] * dy;
-                dz[R][q] = W.w4[R][q] * dy * (1.0f - h3[R][q] * h3[R][q]);
+                dz[R][q] = (-0.5f * W.w4[R][q]) * dy * (1.0f - h3[R][q] * h3[R][q]);  // w4 = -w4'/2
                 gb3[R][q] += dz[R][q];
             }
         outer_acc<4>(dz, h2, sZ[w], sH[w], dW3, l, true);
@@ -744,13 +779,13 @@ __global__ void mlp_adam_kernel(int n_nets, float* master, float* m1, float* v1,
 // Grid (chunks, trades); 4 waves per block, each a quarter of the chunk.
 template <int MODE>
 __global__ void __launch_bounds__(kThreads) mlp_eval_kernel(Book bk, PathSet ps, Norm nm, const uint4* frags,
-                                                            const float* master, const double* poly, double* part) {
+                                                            const float* fwdvec, const double* poly, double* part) {
     const int j = blockIdx.y, c = blockIdx.x;
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 4, n16 = l & 15;
     constexpr int NQ = (MODE == 0) ? 1 : 3;
     __shared__ double red[kWaves][NQ];
     FwdW W;
-    load_fwd(W, frags + (size_t)j * kNFrag * 64, master + (size_t)j * kParamStride, l);
+    load_fwd(W, frags + (size_t)j * kNFrag * 64, fwdvec + (size_t)j * kFwdStride, l);
     const uint64_t n = ps.n;
     const uint64_t q0 = (uint64_t)c * kChunk + (uint64_t)w * (kChunk / kWaves);
     const uint64_t q1 = (q0 + kChunk / kWaves < n) ? q0 + kChunk / kWaves : n;
@@ -793,25 +828,42 @@ __global__ void __launch_bounds__(kThreads) mlp_eval_kernel(Book bk, PathSet ps,
                 u1 = (float)(lb * nm.lb_scale + nm.lb_shift);
             }
             float my_y = 0.f;
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const float a0 = __shfl(u0, 16 * s + n16), a1 = __shfl(u1, 16 * s + n16);
-                f32x4 h1[4], h2[4], h3[4];
-                const float y = mlp_fwd(W, input_operand(g, a0, a1, u2), h1, h2, h3);
-                if (g == s) my_y = y;  // lane l holds sample 16 (l/16) + l%16 = l of this 64-path tile
-            }
-            if (!alive) continue;
-            if (MODE == 0) {
-                acc[0] += (double)fmaxf(my_y, 0.0f);
+            if constexpr (MODE == 0) {
+                // only the sum over alive paths is needed: move their inputs to the first n_alive lanes
+                // (a bijective forward permute, dead lanes behind) and run ceil(n_alive / 16) tiles
+                const uint64_t amask = __ballot(alive);
+                const int n_alive = __popcll(amask);
+                const int rank_alive = __builtin_amdgcn_mbcnt_hi((uint32_t)(amask >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)amask, 0u));
+                const int pos = alive ? rank_alive : n_alive + (l - rank_alive);
+                const float v0 = __int_as_float(__builtin_amdgcn_ds_permute(pos << 2, __float_as_int(u0)));
+                const float v1 = __int_as_float(__builtin_amdgcn_ds_permute(pos << 2, __float_as_int(u1)));
+                const int n_sub = (n_alive + 15) >> 4;
+                for (int s = 0; s < n_sub; ++s) {
+                    const float a0 = __shfl(v0, 16 * s + n16), a1 = __shfl(v1, 16 * s + n16);
+                    f32x4 h1[4], h2[4], h3[4];
+                    const float y = mlp_fwd(W, input_operand(g, a0, a1, u2), h1, h2, h3);
+                    if (g == s) my_y = y;  // lane l holds compact sample l
+                }
+                if (l < n_alive) acc[0] += (double)fmaxf(my_y, 0.0f);
             } else {
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const float a0 = __shfl(u0, 16 * s + n16), a1 = __shfl(u1, 16 * s + n16);
+                    f32x4 h1[4], h2[4], h3[4];
+                    const float y = mlp_fwd(W, input_operand(g, a0, a1, u2), h1, h2, h3);
+                    if (g == s) my_y = y;  // lane l holds sample 16 (l/16) + l%16 = l of this 64-path tile
+                }
+                if (alive) {
 #pragma clang fp contract(off)
-                const double tgt = ps.y0[(uint64_t)j * n + p] * ps.BN[row + p] * inv_scale;
-                const double x = ps.X[row + p];
-                const double cp = (c0 + x * (c1 + x * (c2 + x * c3))) * inv_scale;
-                const double e1 = (double)my_y - tgt, e2 = cp - tgt;
-                acc[0] += e1 * e1;
-                acc[1] += e2 * e2;
-                acc[2] += 1.0;
+                    const double tgt = ps.y0[(uint64_t)j * n + p] * ps.BN[row + p] * inv_scale;
+                    const double x = ps.X[row + p];
+                    const double cp = (c0 + x * (c1 + x * (c2 + x * c3))) * inv_scale;
+                    const double e1 = (double)my_y - tgt, e2 = cp - tgt;
+                    acc[0] += e1 * e1;
+                    acc[1] += e2 * e2;
+                    acc[2] += 1.0;
+                }
             }
         }
 #pragma unroll
@@ -833,11 +885,11 @@ template __global__ void mlp_eval_kernel<0>(Book, PathSet, Norm, const uint4*, c
 template __global__ void mlp_eval_kernel<1>(Book, PathSet, Norm, const uint4*, const float*, const double*, double*);
 
 // Outputs on explicit inputs xu [net][n][3] -> y [net][n]; grid (ceil(n/64), nets), one wave per block.
-__global__ void __launch_bounds__(64) mlp_forward_kernel(int n, const uint4* frags, const float* master,
+__global__ void __launch_bounds__(64) mlp_forward_kernel(int n, const uint4* frags, const float* fwdvec,
                                                          const float* xu, float* y) {
     const int j = blockIdx.y, l = threadIdx.x, g = l >> 4, n16 = l & 15;
     FwdW W;
-    load_fwd(W, frags + (size_t)j * kNFrag * 64, master + (size_t)j * kParamStride, l);
+    load_fwd(W, frags + (size_t)j * kNFrag * 64, fwdvec + (size_t)j * kFwdStride, l);
     const int p = blockIdx.x * 64 + l;
     float u0 = 0.f, u1 = 0.f, u2 = 0.f;
     if (p < n) {

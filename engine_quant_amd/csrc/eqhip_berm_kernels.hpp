@@ -26,6 +26,10 @@ constexpr int OFF_W1 = 0, OFF_W2 = 256, OFF_B2 = 4352, OFF_W3 = 4416, OFF_B3 = 8
               OFF_B4 = 8640;
 // BF16 operand fragments per network, 64 lanes x 16 bytes each.
 constexpr int kFragW1 = 0, kFragW2 = 4, kFragW3 = 12, kFragW2T = 20, kFragW3T = 28, kNFrag = 36;
+// Forward activation: r = 1 / (1 + 2^zs), tanh = 1 - 2r, with zs = kActScale z (kActScale = 2 log2 e)
+// carried by the forward weights; folded FP32 forward vectors per network (stride kFwdStride).
+constexpr float kActScale = 2.8853900817779268f;
+constexpr int kFwdStride = 256, FV_B2 = 0, FV_B3 = 64, FV_W4 = 128, FV_B4 = 192;
 
 constexpr int kThreads = 256;      // every Bermudan kernel runs 4 waves per block
 constexpr int kWaves = 4;
@@ -88,17 +92,18 @@ __global__ void berm_finalize(int n_rows, int n_chunks, const double* part, int 
 // network
 __global__ void mlp_init_kernel(int n_nets, uint64_t seed, float* master);
 __global__ void mlp_pack_kernel(const float* master, uint4* frags);
+__global__ void mlp_fold_kernel(const float* master, float* fwdvec);
 template <int SRC>
-__global__ void mlp_train_kernel(Book bk, PathSet ps, Norm nm, const uint4* frags, const float* master,
+__global__ void mlp_train_kernel(Book bk, PathSet ps, Norm nm, const uint4* frags, const float* fwdvec,
                                  int tiles_per_wave, uint32_t step, uint64_t seed, const float* xu,
                                  const float* xt, const float* xm, float* grad_part, float* loss_part);
 __global__ void mlp_adam_kernel(int n_nets, float* master, float* m1, float* v1, const float* grad_part,
                                 float lr, float beta1, float beta2, float eps, float bc1, float bc2,
                                 int do_update, float* grad_out, const float* loss_part, float* loss_out);
 template <int MODE>
-__global__ void mlp_eval_kernel(Book bk, PathSet ps, Norm nm, const uint4* frags, const float* master,
+__global__ void mlp_eval_kernel(Book bk, PathSet ps, Norm nm, const uint4* frags, const float* fwdvec,
                                 const double* poly, double* part);
-__global__ void mlp_forward_kernel(int n, const uint4* frags, const float* master, const float* xu, float* y);
+__global__ void mlp_forward_kernel(int n, const uint4* frags, const float* fwdvec, const float* xu, float* y);
 
 }  // namespace berm
 }  // namespace eqhip
