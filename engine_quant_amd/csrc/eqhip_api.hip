@@ -1435,6 +1435,57 @@ int eqhip_pv_dv01(eqhip_context* ctx, const double* pillars, const double* zero_
     });
 }
 
+int eqhip_pv_sensitivities(eqhip_context* ctx, const double* pillars, const double* zero_rates, int32_t n_pillars,
+                           const eqhip_portfolio* pf, const eqhip_model_hw1f* model, double* out_pillar_delta,
+                           double* out_model_npv, double* out_model_greeks) {
+    return guarded([&] {
+        check_ctx(ctx);
+        validate_portfolio(pf);
+        if (n_pillars < 1 || !pillars || !zero_rates) raise(EQHIP_E_INVALID, "Curve: necesita al menos un pillar");
+        for (int i = 1; i < n_pillars; ++i)
+            if (!(pillars[i] > pillars[i - 1])) raise(EQHIP_E_INVALID, "Curve: pillars debe ser estrictamente creciente");
+        if (!out_pillar_delta) raise(EQHIP_E_INVALID, "out_pillar_delta is required");
+        if (model && (!out_model_npv || !out_model_greeks))
+            raise(EQHIP_E_INVALID, "out_model_npv / out_model_greeks are required with a model");
+        if (model && !(model->a > 0.0)) raise(EQHIP_E_INVALID, "HW1F needs a > 0");
+        hipStream_t st = ctx->stream;
+        const int n = pf->n_trades, nf = pf->flow_offset[n];
+        DevBuf dp, dz, doff, dpay, dacc, dn, dk, ds, dd, dm, dg;
+        upload(dp, std::vector<double>(pillars, pillars + n_pillars), st);
+        upload(dz, std::vector<double>(zero_rates, zero_rates + n_pillars), st);
+        upload(doff, std::vector<int>(pf->flow_offset, pf->flow_offset + n + 1), st);
+        upload(dpay, std::vector<double>(pf->payment_times, pf->payment_times + nf), st);
+        upload(dacc, std::vector<double>(pf->accruals, pf->accruals + nf), st);
+        upload(dn, std::vector<double>(pf->notional, pf->notional + n), st);
+        upload(dk, std::vector<double>(pf->fixed_rate, pf->fixed_rate + n), st);
+        upload(ds, std::vector<double>(pf->start, pf->start + n), st);
+        dd.alloc((size_t)n * n_pillars * 8);
+        dm.alloc((size_t)n * 8);
+        dg.alloc((size_t)n * 4 * 8);
+        PvArgs g{};
+        g.pillars = dp.as<double>();
+        g.zeros = dz.as<double>();
+        g.np = n_pillars;
+        g.n_trades = n;
+        g.flow_off = doff.as<int>();
+        g.pay = dpay.as<double>();
+        g.acc = dacc.as<double>();
+        g.notional = dn.as<double>();
+        g.fixed = dk.as<double>();
+        g.start = ds.as<double>();
+        const eqhip_model_hw1f mm = model ? *model : eqhip_model_hw1f{1.0, 0.0, 0.0, 0.0};
+        hipLaunchKernelGGL(pv_sens_kernel, dim3((n + 127) / 128), dim3(128), 0, st, g, mm.a, mm.b, mm.sigma, mm.r0,
+                           model ? 1 : 0, dd.as<double>(), dm.as<double>(), dg.as<double>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out_pillar_delta, dd.p, (size_t)n * n_pillars * 8, hipMemcpyDeviceToHost, st));
+        if (model) {
+            HIPCHK(hipMemcpyAsync(out_model_npv, dm.p, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(out_model_greeks, dg.p, (size_t)n * 32, hipMemcpyDeviceToHost, st));
+        }
+        HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
 int eqhip_batch_create(eqhip_context* ctx, const eqhip_model_hw1f* model, int32_t n_jobs,
                        const eqhip_portfolio* portfolios, const uint64_t* n_paths, const double* dates,
                        int32_t n_dates, uint32_t n_steps, uint64_t seed, int32_t rng, int32_t disc,

@@ -762,4 +762,78 @@ __global__ void pv_dv01_kernel(PvArgs g) {
     else if (g.bucketed) g.bucketed[(uint64_t)j * g.np + (v - 2)] = bumped - base;
 }
 
+// Interpolation weights of the zero rate at t on the pillars (the adjoint of curve_df's
+// interpolation): weight w1 on pillar i1 and 1 - w1 on pillar i0 (i0 == i1 outside the pillars).
+__device__ void curve_weights(const double* pillars, int np, double t, int& i0, int& i1, double& w1) {
+#pragma clang fp contract(off)
+    if (t <= pillars[0]) {
+        i0 = i1 = 0;
+        w1 = 1.0;
+    } else if (t >= pillars[np - 1]) {
+        i0 = i1 = np - 1;
+        w1 = 1.0;
+    } else {
+        int i = 0;
+        while (i < np && pillars[i] < t) ++i;
+        if (i == 0) i = 1;
+        i0 = i - 1;
+        i1 = i;
+        w1 = (t - pillars[i0]) / (pillars[i1] - pillars[i0]);
+    }
+}
+
+// Analytic sensitivities of one trade's PV (one thread per trade):
+//   dPV/dz_p = sum_t c_t (-t DF(t)) dz(t)/dz_p over the cash-flow dates t with PV coefficients c_t
+//   (N at the start, -N at the last payment, -N K tau_i at every payment; DF(0) = 1);
+//   model NPV at t = 0 and its derivatives in a, b, sigma, r0 through P(0,t) = A(t) exp(-B(t) r0).
+__global__ void pv_sens_kernel(PvArgs g, double a, double b, double sigma, double r0, int want_model,
+                               double* pillar_delta, double* model_npv, double* model_greeks) {
+#pragma clang fp contract(off)
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= g.n_trades) return;
+    const int f0 = g.flow_off[j], m = g.flow_off[j + 1] - f0;
+    const double N = g.notional[j], K = g.fixed[j];
+    double* dz = pillar_delta + (size_t)j * g.np;
+    for (int p = 0; p < g.np; ++p) dz[p] = 0.0;
+    double npv = 0.0, ga = 0.0, gb = 0.0, gs = 0.0, gr = 0.0;
+    auto add_flow = [&](double t, double c) {
+        if (t > 0.0) {
+            int i0, i1;
+            double w1;
+            curve_weights(g.pillars, g.np, t, i0, i1, w1);
+            const double d = c * (-t * curve_df(g.pillars, g.zeros, g.np, -1, 0.0, t));
+            if (i1 != i0) {
+                dz[i0] += d * (1.0 - w1);
+                dz[i1] += d * w1;
+            } else {
+                dz[i0] += d;  // flat extrapolation: the whole weight on the end pillar
+            }
+        }
+        if (want_model) {
+            if (t <= 0.0) {
+                npv += c;  // P(0,0) = 1, no parameter dependence
+                return;
+            }
+            const ZcbSens z = zcb_sens(a, b, sigma, t);
+            const double P = z.A * exp(-(z.B * r0));
+            const double cP = c * P;
+            npv += cP;
+            ga += cP * (z.lnA_a - r0 * z.B_a);
+            gb += cP * z.lnA_b;
+            gs += cP * z.lnA_s;
+            gr -= cP * z.B;
+        }
+    };
+    add_flow(g.start[j], N);
+    add_flow(g.pay[f0 + m - 1], -N);
+    for (int i = 0; i < m; ++i) add_flow(g.pay[f0 + i], -(N * K * g.acc[f0 + i]));
+    if (want_model) {
+        model_npv[j] = npv;
+        model_greeks[(size_t)j * 4 + 0] = ga;
+        model_greeks[(size_t)j * 4 + 1] = gb;
+        model_greeks[(size_t)j * 4 + 2] = gs;
+        model_greeks[(size_t)j * 4 + 3] = gr;
+    }
+}
+
 }  // namespace eqhip

@@ -150,5 +150,7 @@ __global__ void cva_kernel(const double* ee, int n_profiles, int n_dates, const 
                            double a, double b, double sigma, double r0, double hazard,
                            double recovery, double* cva);
 __global__ void pv_dv01_kernel(PvArgs g);
+__global__ void pv_sens_kernel(PvArgs g, double a, double b, double sigma, double r0, int want_model,
+                               double* pillar_delta, double* model_npv, double* model_greeks);
 
 }  // namespace eqhip

@@ -31,7 +31,7 @@ EXPORTED = [
     "eqhip_unilateral_cva_from_exposure_batch", "eqhip_pv_dv01", "eqhip_plan_create",
     "eqhip_plan_execute", "eqhip_plan_timing", "eqhip_plan_destroy", "eqhip_exposure_run",
     "eqhip_batch_create", "eqhip_batch_execute", "eqhip_batch_timing", "eqhip_batch_destroy",
-    "eqhip_cva_adjoint", "eqhip_berm_create", "eqhip_berm_fit_policy", "eqhip_berm_pathwise",
+    "eqhip_cva_adjoint", "eqhip_pv_sensitivities", "eqhip_berm_create", "eqhip_berm_fit_policy", "eqhip_berm_pathwise",
     "eqhip_berm_exposure_lsm", "eqhip_berm_train", "eqhip_berm_exposure_mlp", "eqhip_berm_mse",
     "eqhip_berm_get_params", "eqhip_berm_set_params", "eqhip_berm_mlp_grad", "eqhip_berm_mlp_forward",
     "eqhip_berm_adam_step", "eqhip_berm_get_timing", "eqhip_berm_destroy",
@@ -114,6 +114,8 @@ def lib():
                                          ctypes.c_size_t]
         L.eqhip_batch_destroy.argtypes = [ctypes.c_void_p]
         L.eqhip_cva_adjoint.argtypes = [ctypes.c_void_p, _PD, _I32, ctypes.POINTER(CreditC), _PD, _PD, _PD]
+        L.eqhip_pv_sensitivities.argtypes = [ctypes.c_void_p, _PD, _PD, _I32, ctypes.POINTER(PortfolioC),
+                                             ctypes.POINTER(ModelHW1F), _PD, _PD, _PD]
         _lib = L
     return _lib
 
@@ -323,6 +325,20 @@ class Context:
         _check(lib().eqhip_pv_dv01(self.handle, _dp(pillars), _dp(zero_rates), len(pillars), ctypes.byref(pc),
                                    bump, _dp(pv), _dp(dv), _dp(bk)))
         return pv, dv, bk
+
+    def pv_sensitivities(self, pillars, zero_rates, portfolio: Portfolio, model=None):
+        """Analytic dPV/dz per pillar [n_trades][n_pillars]; with a model also the model NPV at t = 0
+        and its (a, b, sigma, r0) derivatives [n_trades][4]."""
+        pillars, zero_rates = _f64(pillars), _f64(zero_rates)
+        pc = portfolio._c()
+        nt = portfolio.n_trades
+        dz = np.zeros((nt, len(pillars)))
+        npv = np.zeros(nt) if model is not None else None
+        gr = np.zeros((nt, 4)) if model is not None else None
+        m = ctypes.byref(ModelHW1F(*model)) if model is not None else None
+        _check(lib().eqhip_pv_sensitivities(self.handle, _dp(pillars), _dp(zero_rates), len(pillars),
+                                            ctypes.byref(pc), m, _dp(dz), _dp(npv), _dp(gr)))
+        return dz, npv, gr
 
     # portfolio --------------------------------------------------------------------------------
     def plan(self, model, portfolio, dates, n_paths, n_steps, **kw):

@@ -162,6 +162,18 @@ int eqhip_pv_dv01(eqhip_context* ctx, const double* pillars, const double* zero_
                   int32_t n_pillars, const eqhip_portfolio* portfolio, double bump, double* out_pv,
                   double* out_dv01, double* out_bucketed);
 
+/* Analytic first-order sensitivities of every trade's PV:
+ * out_pillar_delta [n_trades][n_pillars] = dPV/dz_p, the derivative of npv_from_market
+ * (cpp/engine/src/measure.cpp:160-184) with respect to each pillar zero rate, which the one-sided
+ * bucketed DV01 of measure.cpp:211-222 approximates as bump * dPV/dz_p; with a model (nullable),
+ * out_model_npv [n_trades] is the HW1F model NPV at t = 0 and out_model_greeks [n_trades][4] its
+ * derivatives in (a, b, sigma, r0), the quantities the reference obtains by reverse-mode AD
+ * (rust/crates/engine-core/src/api.rs:394-415, 519-552). */
+int eqhip_pv_sensitivities(eqhip_context* ctx, const double* pillars, const double* zero_rates,
+                           int32_t n_pillars, const eqhip_portfolio* portfolio,
+                           const eqhip_model_hw1f* model, double* out_pillar_delta,
+                           double* out_model_npv, double* out_model_greeks);
+
 /* ---- portfolio / netting-set exposure ----------------------------------------------------- */
 
 /* Builds the device-resident plan of one exposure job: schedule classes, cash-flow date table,

@@ -74,3 +74,39 @@ def test_adjoint_matches_oracle_differences_on_shared_draws(ctx):
         fd = (up - dn) / (2 * h)
         assert abs(grad[name] - fd) <= 2e-3 * abs(fd) + 1e-9, (name, grad[name], fd)
     plan.close()
+
+
+def test_pv_pillar_and_model_sensitivities(ctx):
+    """Analytic dPV/dz (all 20 pillars) against central differences of the oracle's market PV and
+    against the one-sided bucketed DV01 (measure.cpp:211-222, bump 1bp, not divided by h); the model
+    NPV Greeks against central differences of the oracle's model NPV at t = 0 (api.rs:394-415)."""
+    from engine_quant_amd.portfolios import swap_portfolio, zero_curve
+    pillars, zeros = zero_curve(7)
+    port = swap_portfolio(40, seed=9, pillars=pillars, zeros=zeros)
+    dz, npv, greeks = ctx.pv_sensitivities(pillars, zeros, port, MODEL)
+    _, _, bk = ctx.pv_dv01(pillars, zeros, port, bump=1e-4, bucketed=True)
+    off = port.flow_off
+    h = 1e-6
+    for j in range(port.n_trades):
+        pay, acc = port.pay[off[j]:off[j + 1]], port.acc[off[j]:off[j + 1]]
+        N, K = port.notional[j], port.fixed_rate[j]
+        for p in range(len(pillars)):
+            up, dn = zeros.copy(), zeros.copy()
+            up[p] += h
+            dn[p] -= h
+            fd = (oracle.npv_from_market(pillars, up, N, K, 0.0, pay, acc)
+                  - oracle.npv_from_market(pillars, dn, N, K, 0.0, pay, acc)) / (2 * h)
+            assert abs(dz[j, p] - fd) <= 1e-6 * abs(N) * 30, (j, p, dz[j, p], fd)
+            # the reference's bucketed DV01 is the one-sided difference: bump * dPV/dz + O(bump^2)
+            assert abs(bk[j, p] - 1e-4 * dz[j, p]) <= 5e-3 * abs(bk[j, p]) + 1e-6 * abs(N), (j, p)
+        base = oracle.irs_npv_reset(*MODEL, 0.0, N, K, 0.0, pay, acc)
+        assert abs(npv[j] - base) <= 1e-9 * abs(N)
+        for i in range(4):
+            mu, md = list(MODEL), list(MODEL)
+            mu[i] += h
+            md[i] -= h
+            a_u, b_u, s_u, r_u = mu
+            a_d, b_d, s_d, r_d = md
+            fd = (oracle.irs_npv_reset(a_u, b_u, s_u, r_u, 0.0, N, K, 0.0, pay, acc)
+                  - oracle.irs_npv_reset(a_d, b_d, s_d, r_d, 0.0, N, K, 0.0, pay, acc)) / (2 * h)
+            assert abs(greeks[j, i] - fd) <= 1e-6 * abs(fd) + 1e-4 * abs(N) * 1e-4, (j, i, greeks[j, i], fd)
