@@ -80,6 +80,8 @@ struct eqhip_plan {
     std::vector<double> dates;
     size_t lds_bytes = 0;
 
+    int Lmax = 0, Zmax = 0;
+    DevBuf lflow_off, lflow, ltau, lseg, zcls_off, zcls;
     DevBuf slot_of_step, R, Z, AB, date_slot, date_dfirst, date_det, fdesc, fam_off, flow_d, flow_tau,
         flow_cls, fix_off, fix_slot, fix_AB, set_off, set_cls, set_alpha, set_beta, X, items, cls_mat, tr_K,
         tr_absN, tr_N, tr_NK, tr_orig, partial, trade_ee, trade_cva, set_ee, set_pfe, set_cva, neff, qrank,
@@ -138,8 +140,7 @@ void validate_portfolio(const eqhip_portfolio* pf) {
 
 void choose_launch(eqhip_plan& P) {
     P.set_mode = (P.S <= kMaxSplitSets) ? 0 : 1;
-    size_t lds = (size_t)8 * kPathsPerTile * (1 + P.D + P.Fmax) + (size_t)16 * kPathsPerTile * P.C +
-                 (P.set_mode == 0 ? (size_t)8 * kPathsPerTile * kValueWaves * P.S : 0);
+    size_t lds = value_lds_bytes(P.D, P.Fmax, P.C, P.set_mode == 0 ? P.S : 0, P.Lmax, P.Zmax, P.NF);
     if (lds > P.ctx->lds_limit)
         raise(EQHIP_E_CAPACITY, "portfolio too diverse for one plan: " + std::to_string(P.D) +
                                     " cash-flow dates and " + std::to_string(P.C) + " schedule classes need " +
@@ -323,6 +324,50 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
         fix_off[k + 1] = (int)fix_slot.size();
         P.Fmax = std::max(P.Fmax, (int)fix_here.size());
     }
+    // per-date live-flow lists (families in order), their wave segments, and the matured classes
+    std::vector<int> lflow_off(n_dates + 1, 0), zcls_off(n_dates + 1, 0), zcls;
+    std::vector<int4> lseg((size_t)n_dates * kValueWaves);
+    std::vector<int4> lflow;
+    std::vector<double> ltau;
+    P.Lmax = P.Zmax = 0;
+    for (int k = 0; k < n_dates; ++k) {
+        const int l0 = (int)lflow.size(), z0 = (int)zcls.size();
+        for (int F = 0; F < P.NF; ++F) {
+            const int4 fd = fdesc[(size_t)k * P.NF + F];
+            const int f0 = fam_off[F], f1 = fam_off[F + 1];
+            const int first = (fd.x == 0) ? f1 : f0 + fd.y;
+            for (int i = f0; i < first; ++i)
+                if (flow_cls[i] >= 0) zcls.push_back(flow_cls[i]);
+            for (int i = first; i < f1; ++i) {
+                lflow.push_back(make_int4(flow_d[i], flow_cls[i], F, 0));
+                ltau.push_back(flow_tau[i]);
+            }
+        }
+        lflow_off[k + 1] = (int)lflow.size();
+        zcls_off[k + 1] = (int)zcls.size();
+        const int L = (int)lflow.size() - l0;
+        // segment w = [w L / waves, (w+1) L / waves); it takes the totals of the earlier non-empty
+        // segments that end in the family its first flow belongs to (a family spans consecutive segments)
+        int sb[kValueWaves + 1];
+        for (int w = 0; w <= kValueWaves; ++w) sb[w] = (w * L) / kValueWaves;
+        auto fam_of = [&](int i) { return lflow[(size_t)l0 + i].z; };
+        for (int w = 0; w < kValueWaves; ++w) {
+            int mask = 0, n_carry = 0;
+            if (sb[w + 1] > sb[w]) {
+                const int fam0 = fam_of(sb[w]);
+                for (int w2 = w - 1; w2 >= 0; --w2) {
+                    if (sb[w2 + 1] == sb[w2]) continue;  // empty
+                    if (fam_of(sb[w2 + 1] - 1) != fam0) break;
+                    mask |= 1 << w2;
+                }
+                if (mask)
+                    for (int i = sb[w]; i < sb[w + 1] && fam_of(i) == fam0; ++i) ++n_carry;
+            }
+            lseg[(size_t)k * kValueWaves + w] = make_int4(sb[w], sb[w + 1], mask, n_carry);
+        }
+        P.Lmax = std::max(P.Lmax, L);
+        P.Zmax = std::max(P.Zmax, (int)zcls.size() - z0);
+    }
     P.n_slots = (int)slot_of.size();
     std::vector<int> slot_of_step(steps + 1, -1);
     for (auto& kv : slot_of) slot_of_step[(size_t)kv.first] = kv.second;
@@ -424,6 +469,12 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
     upload(P.flow_d, flow_d, st);
     upload(P.flow_tau, flow_tau, st);
     upload(P.flow_cls, flow_cls, st);
+    upload(P.lflow_off, lflow_off, st);
+    upload(P.lflow, lflow, st);
+    upload(P.ltau, ltau, st);
+    upload(P.lseg, lseg, st);
+    upload(P.zcls_off, zcls_off, st);
+    upload(P.zcls, zcls, st);
     upload(P.cls_mat, cls_mat, st);
     upload(P.fix_off, fix_off, st);
     upload(P.fix_slot, fix_slot, st);
@@ -522,6 +573,14 @@ void build_value_plan(eqhip_plan& P) {
     g.flow_d = P.flow_d.as<int>();
     g.flow_tau = P.flow_tau.as<double>();
     g.flow_cls = P.flow_cls.as<int>();
+    g.Lmax = P.Lmax;
+    g.Zmax = P.Zmax;
+    g.lflow_off = P.lflow_off.as<int>();
+    g.lflow = P.lflow.as<int4>();
+    g.ltau = P.ltau.as<double>();
+    g.lseg = P.lseg.as<int4>();
+    g.zcls_off = P.zcls_off.as<int>();
+    g.zcls = P.zcls.as<int>();
     g.fix_off = P.fix_off.as<int>();
     g.fix_slot = P.fix_slot.as<int>();
     g.fix_AB = P.fix_AB.as<double2>();

@@ -265,6 +265,17 @@ template __global__ void adjoint_kernel<1, 1>(AdjArgs);
 // Schedule families hold classes whose schedules are prefixes of one another, so one running sum
 // yields every class's annuity, bit-identical to summing that class's flows in order.
 // ---------------------------------------------------------------------------------------------
+// Values that are the same in every lane (LDS broadcasts of the date's flow list) moved to scalar
+// registers, so branches on them are scalar and addresses built from them are uniform.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int4 uni(int4 v) { return make_int4(uni(v.x), uni(v.y), uni(v.z), uni(v.w)); }
+__device__ __forceinline__ double uni(double v) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);
+    const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32));
+    return __builtin_bit_cast(double, lo | (hi << 32));
+}
+
 __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, const int chunk, double* smem) {
     // a private copy of the descriptor: the kernel's own stores cannot alias it, so its fields stay
     // in scalar registers instead of being reloaded after every global store
@@ -278,13 +289,15 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     double* sF = sP + (size_t)g.D * kPathsPerTile;         // [Fmax][64]
     double2* sC = (double2*)(sF + (size_t)g.Fmax * kPathsPerTile);  // [64][C] path-major
     const int CS = g.C;
-    const int* __restrict__ flow_d = g.flow_d;
-    const double* __restrict__ flow_tau = g.flow_tau;
-    const int* __restrict__ flow_cls = g.flow_cls;
     const int* __restrict__ set_cls = g.set_cls;
     const double* __restrict__ set_alpha = g.set_alpha;
     const double* __restrict__ set_beta = g.set_beta;
     double* sPart = (double*)(sC + (size_t)g.C * kPathsPerTile);    // [S][waves][64] (split mode)
+    double* sSeg = sPart + (g.set_mode == 0 ? (size_t)g.S * kValueWaves * kPathsPerTile : 0);  // [waves][64]
+    double* sLT = sSeg + (size_t)kValueWaves * kPathsPerTile;       // [Lmax] live-flow accruals
+    int4* sLF = (int4*)(sLT + ((g.Lmax + 1) & ~1));                 // [Lmax] live flows
+    int4* sFD = sLF + g.Lmax;                                       // [NF] the date's family descriptors
+    int* sZc = (int*)(sFD + g.NF);                                  // [Zmax] matured classes
 
     const double t = g.dates[k];
     const int dfirst = g.date_dfirst[k];
@@ -330,18 +343,42 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
         }
     }
 
+    // the date's live flows and matured classes, staged once for all tiles of the unit
+    const int lb = g.lflow_off[k], n_lf = g.lflow_off[k + 1] - lb;
+    const int zb = g.zcls_off[k], n_zc = g.zcls_off[k + 1] - zb;
+    const int4 segd = g.lseg[k * kValueWaves + wave];
+    const int seg_b = segd.x, seg_e = segd.y, carry_mask = segd.z, n_carry = segd.w;
+    __syncthreads();  // the previous unit of a persistent block may still read this LDS
+    for (int i = tid; i < n_lf; i += kValueThreads) {
+        sLF[i] = g.lflow[lb + i];
+        sLT[i] = g.ltau[lb + i];
+    }
+    for (int i = tid; i < n_zc; i += kValueThreads) sZc[i] = g.zcls[zb + i];
+    for (int i = tid; i < g.NF; i += kValueThreads) sFD[i] = fdesc[i];
+
     const uint64_t tile0 = (uint64_t)chunk * g.tiles_per_block;
     uint64_t tile_end = tile0 + g.tiles_per_block;
     if (tile_end > g.n_tiles) tile_end = g.n_tiles;
 
+    // the short rates of the next tile are loaded one tile ahead (wave 0: r at t_k; wave f < nfix:
+    // r at fixing f), so that no tile starts by waiting on HBM
+    const double* Rfix = (wave < nfix) ? g.R + (uint64_t)g.fix_slot[fbeg + wave] * g.r_stride : nullptr;
+    double r_next = 0.0, rf_next = 0.0;
+    if (tile0 < tile_end) {
+        const uint64_t gq = tile0 * kPathsPerTile + lane;
+        if (gq < n_paths) {
+            if (wave == 0) r_next = Rk[gq];
+            if (Rfix) rf_next = Rfix[gq];
+        }
+    }
     for (uint64_t tile = tile0; tile < tile_end; ++tile) {
         const uint64_t p0 = tile * kPathsPerTile;
         const uint64_t gp = p0 + lane;
         const bool valid = gp < n_paths;
         __syncthreads();
-        if (tid < kPathsPerTile) sR[lane] = valid ? Rk[gp] : 0.0;
+        if (wave == 0) sR[lane] = r_next;
         for (int f = wave; f < nfix; f += kValueWaves) {
-            double rf = valid ? g.R[(uint64_t)g.fix_slot[fbeg + f] * g.r_stride + gp] : 0.0;
+            double rf = (f == wave) ? rf_next : (valid ? g.R[(uint64_t)g.fix_slot[fbeg + f] * g.r_stride + gp] : 0.0);
             double2 fab = g.fix_AB[fbeg + f];
             sF[f * kPathsPerTile + lane] = fab.x * exp(-(fab.y * rf));
         }
@@ -356,39 +393,80 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
             }
         }
         __syncthreads();
-        // B: schedule families
-        for (int F = wave; F < g.NF; F += kValueWaves) {
-            const int4 fd = fdesc[F];
-            const int f0 = g.fam_off[F];
-            const int f1 = g.fam_off[F + 1];
-            const int first = (fd.x == 0) ? f1 : f0 + fd.y;
-            for (int i = f0; i < first; ++i) {  // classes already matured
-                const int c = flow_cls[i];
-                if (c >= 0) sC[lane * CS + c] = make_double2(0.0, 0.0);
-            }
-            if (fd.x == 0) continue;
-            double st;
-            if (fd.x == 1) st = 1.0;
-            else if (fd.x == 2) st = sP[fd.z * kPathsPerTile + lane];
-            else st = sP[fd.z * kPathsPerTile + lane] / sF[fd.w * kPathsPerTile + lane];
-            double ann = 0.0;
-            int i = first;
-            for (; i + 4 <= f1; i += 4) {  // four independent LDS reads in flight, then the chain
-                double P[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) P[u] = sP[flow_d[i + u] * kPathsPerTile + lane];
+        // B: schedule families. Every class's (floating leg, annuity) is a running sum over its
+        // family's live flows; the date's flows are split into one contiguous segment per wave, each
+        // summed from zero, and a second pass adds the totals of the earlier segments of the same family.
+        {  // next tile's short rates: in flight during the first pass, waited for at the next barrier
+            const uint64_t gq = gp + kPathsPerTile;
+            const bool vq = (tile + 1 < tile_end) && gq < n_paths;
+            if (wave == 0) r_next = vq ? Rk[gq] : 0.0;
+            if (Rfix) rf_next = vq ? Rfix[gq] : 0.0;
+        }
+        for (int i = wave; i < n_zc; i += kValueWaves) sC[lane * CS + uni(sZc[i])] = make_double2(0.0, 0.0);
+        {
+            double run = 0.0, st = 0.0;
+            int fam = -1;
+            auto family_start = [&](int f) {  // P(t, start) or the carried fixing of the period in progress
+                const int4 fd = uni(sFD[f]);
+                if (fd.x == 1) return 1.0;
+                if (fd.x == 2) return sP[fd.z * kPathsPerTile + lane];
+                return sP[fd.z * kPathsPerTile + lane] / sF[fd.w * kPathsPerTile + lane];
+            };
+            int i = seg_b;
+            for (; i + 4 <= seg_e; i += 4) {  // all LDS reads of four flows first, then the chain
+                int4 e[4];
+                double tau[4], P[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    ann += P[u] * flow_tau[i + u];
-                    const int c = flow_cls[i + u];
-                    if (c >= 0) sC[lane * CS + c] = make_double2(st - P[u], ann);
+                    e[u] = uni(sLF[i + u]);
+                    tau[u] = uni(sLT[i + u]);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) P[u] = sP[e[u].x * kPathsPerTile + lane];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (e[u].z != fam) {
+                        fam = e[u].z;
+                        run = 0.0;
+                        st = family_start(fam);
+                    }
+                    run += P[u] * tau[u];
+                    if (e[u].y >= 0) sC[lane * CS + e[u].y] = make_double2(st - P[u], run);
                 }
             }
-            for (; i < f1; ++i) {
-                const double P = sP[flow_d[i] * kPathsPerTile + lane];
-                ann += P * flow_tau[i];
-                const int c = flow_cls[i];
-                if (c >= 0) sC[lane * CS + c] = make_double2(st - P, ann);
+            for (; i < seg_e; ++i) {
+                const int4 e = uni(sLF[i]);
+                if (e.z != fam) {
+                    fam = e.z;
+                    run = 0.0;
+                    st = family_start(fam);
+                }
+                const double P = sP[e.x * kPathsPerTile + lane];
+                run += P * uni(sLT[i]);
+                if (e.y >= 0) sC[lane * CS + e.y] = make_double2(st - P, run);
+            }
+            sSeg[wave * kPathsPerTile + lane] = run;
+        }
+        __syncthreads();
+        if (carry_mask) {  // the totals of the earlier segments of this segment's first family
+            double carry = 0.0;
+            bool any = false;
+            for (int w2 = 0; w2 < wave; ++w2) {
+                if (!((carry_mask >> w2) & 1)) continue;
+                const double v = sSeg[w2 * kPathsPerTile + lane];
+                carry = any ? carry + v : v;
+                any = true;
+            }
+            for (int i = seg_b; i < seg_b + n_carry; i += 4) {  // reads of four classes, then the writes
+                int cls[4];
+                double a[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) cls[u] = (i + u < seg_b + n_carry) ? uni(sLF[i + u].y) : -1;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) a[u] = (cls[u] >= 0) ? sC[lane * CS + cls[u]].y : 0.0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (cls[u] >= 0) sC[lane * CS + cls[u]].y = carry + a[u];
             }
         }
         __syncthreads();
@@ -400,18 +478,29 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                 const int cb = beg + wave * per;
                 const int ce = min(end, cb + per);
                 double v = 0.0;
-                int e = cb;
-                for (; e + 4 <= ce; e += 4) {
-                    double2 c2[4];
+                // the set CSR from LDS when staged, else from global memory (separate loops keep the
+                // address spaces known to the compiler)
+                auto set_sum = [&](const int* __restrict__ scls, const double* __restrict__ salpha,
+                                   const double* __restrict__ sbeta) {
+                    int e = cb;
+                    for (; e + 4 <= ce; e += 4) {
+                        double2 c2[4];
+                        double al[4], be[4];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) c2[u] = sC[lane * CS + set_cls[e + u]];
+                        for (int u = 0; u < 4; ++u) {
+                            al[u] = salpha[e + u];
+                            be[u] = sbeta[e + u];
+                            c2[u] = sC[lane * CS + scls[e + u]];
+                        }
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) v += set_alpha[e + u] * c2[u].x - set_beta[e + u] * c2[u].y;
-                }
-                for (; e < ce; ++e) {
-                    const double2 c2 = sC[lane * CS + set_cls[e]];
-                    v += set_alpha[e] * c2.x - set_beta[e] * c2.y;
-                }
+                        for (int u = 0; u < 4; ++u) v += al[u] * c2[u].x - be[u] * c2[u].y;
+                    }
+                    for (; e < ce; ++e) {
+                        const double2 c2 = sC[lane * CS + scls[e]];
+                        v += salpha[e] * c2.x - sbeta[e] * c2.y;
+                    }
+                };
+                set_sum(set_cls, set_alpha, set_beta);
                 sPart[(s * kValueWaves + wave) * kPathsPerTile + lane] = v;
             }
         } else {

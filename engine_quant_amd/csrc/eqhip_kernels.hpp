@@ -49,6 +49,16 @@ struct ValuePlan {
     const int* flow_d;        // family flows: cash-flow date index
     const double* flow_tau;   //               accrual
     const int* flow_cls;      //               class whose last flow this is, or -1
+    // per date: the live flows of every family in family order, split into kValueWaves segments
+    // (one per wave), and the classes already matured
+    int Lmax, Zmax;           // largest live-flow / matured-class list over the dates
+    const int* lflow_off;     // [n_dates + 1]
+    const int4* lflow;        // (cash-flow date, class or -1, family, 0)
+    const double* ltau;       // accrual of the flow
+    const int4* lseg;         // [n_dates][kValueWaves] (begin, end, mask of the earlier segments whose
+                              //   totals carry into this one, number of leading flows that take the carry)
+    const int* zcls_off;      // [n_dates + 1]
+    const int* zcls;          // matured classes
     const int* fix_off;       // [n_dates + 1]
     const int* fix_slot;      // R slot of the fixing date
     const double2* fix_AB;    // (A, B) of the fixing period
@@ -69,6 +79,17 @@ struct ValuePlan {
     double* partial;          // [n_dates][n_chunks * n_slices][n_trades] per-trade EE partial sums
     double* path_mtm;         // optional [n_dates][n_trades][n_paths]
 };
+
+// Dynamic LDS of one value_kernel block: sR [64], sP [D][64], sF [Fmax][64], sC [64][C] (double2),
+// set partials [S_split][waves][64], segment totals [waves][64], then the date's live-flow accruals
+// [Lmax] (padded to even), flows [Lmax] (int4)
+// ... followed by the date's family descriptors [NF] (int4).
+__host__ __device__ inline size_t value_lds_bytes(int D, int Fmax, int C, int S_split, int Lmax, int Zmax, int NF) {
+    const size_t lmax_even = (size_t)((Lmax + 1) & ~1);
+    return 8 * ((size_t)kPathsPerTile * (1 + D + Fmax) + 2 * (size_t)kPathsPerTile * C +
+                (size_t)kPathsPerTile * kValueWaves * S_split + (size_t)kPathsPerTile * kValueWaves + lmax_even) +
+           16 * (size_t)Lmax + 16 * (size_t)NF + 4 * (size_t)Zmax;
+}
 
 struct PvArgs {
     const double* pillars;
