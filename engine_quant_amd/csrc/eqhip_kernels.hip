@@ -355,6 +355,9 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     }
     for (int i = tid; i < n_zc; i += kValueThreads) sZc[i] = g.zcls[zb + i];
     for (int i = tid; i < g.NF; i += kValueThreads) sFD[i] = fdesc[i];
+    __syncthreads();
+    // classes already matured at t_k: zeroed once, no live flow writes them during this unit
+    for (int i = wave; i < n_zc; i += kValueWaves) sC[lane * CS + uni(sZc[i])] = make_double2(0.0, 0.0);
 
     const uint64_t tile0 = (uint64_t)chunk * g.tiles_per_block;
     uint64_t tile_end = tile0 + g.tiles_per_block;
@@ -402,7 +405,6 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
             if (wave == 0) r_next = vq ? Rk[gq] : 0.0;
             if (Rfix) rf_next = vq ? Rfix[gq] : 0.0;
         }
-        for (int i = wave; i < n_zc; i += kValueWaves) sC[lane * CS + uni(sZc[i])] = make_double2(0.0, 0.0);
         {
             double run = 0.0, st = 0.0;
             int fam = -1;

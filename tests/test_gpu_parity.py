@@ -124,3 +124,24 @@ def test_invalid_monitoring_date_is_rejected(ctx):
     with pytest.raises(eqhip.EqhipError) as e:
         ctx.exposure_profile(A, B, SIG, R0, 1e6, 0.02, False, 0.0, PAY5, ACC5, [0.5], 10, 100, 1)
     assert e.value.code == eqhip.E_INVALID
+
+
+def test_many_tiles_per_unit_shared_draws(ctx):
+    """A launch whose work units span several 64-path tiles (the bench-size regime: short rates are
+    prefetched a tile ahead and the date's flow list is staged once per unit) against the oracle."""
+    port = small_portfolio(n_trades=30, n_sets=2, seed=8)
+    dates = np.arange(1, 7) / 3.0
+    n_paths, n_steps = 120_000, 24
+    z = np.random.default_rng(3).standard_normal((n_steps, n_paths))
+    plan = ctx.plan(MODEL, port, dates, n_paths, n_steps, rng=eqhip.RNG_HOST_DRAWS, disc=eqhip.DISC_EXACT,
+                    want_trade_ee=True)
+    got = plan.execute(z, 0.02, 0.4)
+    assert plan.timing()["tiles_per_block"] > 1
+    plan.close()
+    ref = oracle.portfolio_exposure(MODEL, port.as_dict(), dates, n_steps, z, scheme=1, hazard_rate=0.02,
+                                    recovery_rate=0.4, want_trade=True)
+    scale = leg_scale(port, MODEL, dates)
+    set_scale = np.array([scale[port.set_id == s].sum() for s in range(port.n_sets)])
+    assert_rel(got["set_ee"], ref["set_ee"], 1e-10, scale=set_scale[:, None], what="set EE")
+    assert_rel(got["set_pfe"], ref["set_pfe"], 1e-10, scale=set_scale[:, None], what="set PFE")
+    assert_rel(got["trade_ee"], ref["trade_ee"], 1e-10, scale=scale[:, None], what="trade EE")
