@@ -391,16 +391,23 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
         set_off[s + 1] = (int)set_cls.size();
     }
 
-    // per-trade work items: trades sorted by (class, sign), up to kTradesPerItem per item
+    // per-trade work items: trades sorted by (last date of the class, class, sign), up to
+    // kTradesPerItem per item. Ordering by maturity keeps the items still alive at a date a
+    // contiguous suffix of the list, so the waves whose items have all matured skip the date.
     std::vector<int4> items;
     std::vector<int> tr_orig;
     std::vector<double> tr_K, tr_absN, tr_N, tr_NK;
     if (P.want_trade) {
         auto sgn = [&](int j) { return pf->notional[j] < 0.0 ? -1 : 1; };
+        auto last_t = [&](int j) {
+            const ClassKey& K = classes[trade_cls[j]];
+            return K.pay.empty() ? K.start : K.pay.back();
+        };
         std::vector<int> order(pf->n_trades);
         for (int j = 0; j < pf->n_trades; ++j) order[j] = j;
         std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-            return std::make_pair(trade_cls[x], sgn(x)) < std::make_pair(trade_cls[y], sgn(y));
+            return std::make_tuple(last_t(x), trade_cls[x], sgn(x)) <
+                   std::make_tuple(last_t(y), trade_cls[y], sgn(y));
         });
         for (int i = 0; i < pf->n_trades;) {
             const int c = trade_cls[order[i]], sg = sgn(order[i]);

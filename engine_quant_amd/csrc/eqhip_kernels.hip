@@ -451,14 +451,18 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
         }
         __syncthreads();
         if (carry_mask) {  // the totals of the earlier segments of this segment's first family
+            double tot[kValueWaves - 1];
+#pragma unroll
+            for (int w2 = 0; w2 < kValueWaves - 1; ++w2)  // independent reads, issued together
+                tot[w2] = ((carry_mask >> w2) & 1) ? sSeg[w2 * kPathsPerTile + lane] : 0.0;
             double carry = 0.0;
             bool any = false;
-            for (int w2 = 0; w2 < wave; ++w2) {
-                if (!((carry_mask >> w2) & 1)) continue;
-                const double v = sSeg[w2 * kPathsPerTile + lane];
-                carry = any ? carry + v : v;
-                any = true;
-            }
+#pragma unroll
+            for (int w2 = 0; w2 < kValueWaves - 1; ++w2)
+                if ((carry_mask >> w2) & 1) {
+                    carry = any ? carry + tot[w2] : tot[w2];
+                    any = true;
+                }
             for (int i = seg_b; i < seg_b + n_carry; i += 4) {  // reads of four classes, then the writes
                 int cls[4];
                 double a[4];
