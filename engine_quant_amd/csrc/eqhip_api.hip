@@ -83,7 +83,7 @@ struct eqhip_plan {
     int Lmax = 0, Zmax = 0;
     DevBuf lflow_off, lflow, ltau, lseg, zcls_off, zcls;
     DevBuf slot_of_step, R, Z, AB, date_slot, date_dfirst, date_det, fdesc, fam_off, flow_d, flow_tau,
-        flow_cls, fix_off, fix_slot, fix_AB, set_off, set_cls, set_alpha, set_beta, X, items, cls_mat, tr_K,
+        flow_cls, fix_off, fix_slot, fix_AB, set_off, set_nlive, set_cls, set_alpha, set_beta, X, items, cls_mat, tr_K,
         tr_absN, tr_N, tr_NK, tr_orig, partial, trade_ee, trade_cva, set_ee, set_pfe, set_cva, neff, qrank,
         d_dates, vplan, units, rows;
     ValuePlan vp{};  // host copy of the device descriptor (per-run pointers patched at launch)
@@ -380,16 +380,29 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
         e.first += n;
         e.second += n * pf->fixed_rate[j];
     }
+    // Each set lists its classes latest maturity first. A class is alive at t while its last
+    // payment is after t (cls_mat > t + tol, the rule of the matured-class lists above), so the
+    // classes alive at a date are a prefix of the set's list, of length set_nlive[date][set].
     std::vector<int> set_off(P.S + 1, 0), set_cls;
     std::vector<double> set_alpha, set_beta;
     for (int s = 0; s < P.S; ++s) {
-        for (auto& kv : set_cls_sum[s]) {
+        std::vector<std::pair<int, std::pair<double, double>>> ent(set_cls_sum[s].begin(), set_cls_sum[s].end());
+        std::stable_sort(ent.begin(), ent.end(),
+                         [&](const auto& x, const auto& y) { return cls_mat[x.first] > cls_mat[y.first]; });
+        for (auto& kv : ent) {
             set_cls.push_back(kv.first);
             set_alpha.push_back(kv.second.first);
             set_beta.push_back(kv.second.second);
         }
         set_off[s + 1] = (int)set_cls.size();
     }
+    std::vector<int> set_nlive((size_t)n_dates * P.S);
+    for (int k = 0; k < n_dates; ++k)
+        for (int s = 0; s < P.S; ++s) {
+            int n = 0;
+            while (set_off[s] + n < set_off[s + 1] && cls_mat[set_cls[set_off[s] + n]] > P.dates[k] + kDateTolHost) ++n;
+            set_nlive[(size_t)k * P.S + s] = n;
+        }
 
     // per-trade work items: trades sorted by (last date of the class, class, sign), up to
     // kTradesPerItem per item. Ordering by maturity keeps the items still alive at a date a
@@ -486,6 +499,7 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
     upload(P.fix_off, fix_off, st);
     upload(P.fix_slot, fix_slot, st);
     upload(P.set_off, set_off, st);
+    upload(P.set_nlive, set_nlive, st);
     upload(P.set_cls, set_cls, st);
     upload(P.set_alpha, set_alpha, st);
     upload(P.set_beta, set_beta, st);
@@ -592,6 +606,7 @@ void build_value_plan(eqhip_plan& P) {
     g.fix_slot = P.fix_slot.as<int>();
     g.fix_AB = P.fix_AB.as<double2>();
     g.set_off = P.set_off.as<int>();
+    g.set_nlive = P.set_nlive.as<int>();
     g.set_cls = P.set_cls.as<int>();
     g.set_alpha = P.set_alpha.as<double>();
     g.set_beta = P.set_beta.as<double>();

@@ -479,13 +479,12 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
         // C: netting sets
         if (g.set_mode == 0) {
             for (int s = 0; s < g.S; ++s) {
-                const int beg = g.set_off[s], end = g.set_off[s + 1];
+                const int beg = g.set_off[s], end = beg + g.set_nlive[(size_t)k * g.S + s];
                 const int per = (end - beg + kValueWaves - 1) / kValueWaves;
                 const int cb = beg + wave * per;
                 const int ce = min(end, cb + per);
                 double v = 0.0;
-                // the set CSR from LDS when staged, else from global memory (separate loops keep the
-                // address spaces known to the compiler)
+                // the set's classes alive at t, split evenly over the waves
                 auto set_sum = [&](const int* __restrict__ scls, const double* __restrict__ salpha,
                                    const double* __restrict__ sbeta) {
                     int e = cb;
@@ -512,7 +511,8 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
         } else {
             for (int s = wave; s < g.S; s += kValueWaves) {
                 double v = 0.0;
-                for (int e = g.set_off[s]; e < g.set_off[s + 1]; ++e) {
+                const int beg = g.set_off[s], end = beg + g.set_nlive[(size_t)k * g.S + s];
+                for (int e = beg; e < end; ++e) {
                     const double2 c2 = sC[lane * CS + set_cls[e]];
                     v += set_alpha[e] * c2.x - set_beta[e] * c2.y;
                 }

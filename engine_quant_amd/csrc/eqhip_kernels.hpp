@@ -62,7 +62,8 @@ struct ValuePlan {
     const int* fix_off;       // [n_dates + 1]
     const int* fix_slot;      // R slot of the fixing date
     const double2* fix_AB;    // (A, B) of the fixing period
-    const int* set_off;       // [S + 1] CSR of (class, alpha, beta)
+    const int* set_off;       // [S + 1] CSR of (class, alpha, beta), each set's classes latest maturity first
+    const int* set_nlive;     // [n_dates][S] classes of the set still alive at the date (a prefix)
     const int* set_cls;
     const double* set_alpha;  // sum of notionals of the set's trades in the class
     const double* set_beta;   // sum of notional * fixed rate
