@@ -74,6 +74,7 @@ struct eqhip_plan {
     int n_items = 0;
     int n_slices = 1;
     int set_mode = 0;
+    int n_set_entries = 0;
     uint64_t target_units = 2048;   // work units the launch aims for (chunking of the path tiles)
     const double* r_shared = nullptr;  // batch-owned short-rate buffer (all grid steps), or null
     uint64_t r_stride = 0;
@@ -139,8 +140,12 @@ void validate_portfolio(const eqhip_portfolio* pf) {
 }
 
 void choose_launch(eqhip_plan& P) {
-    P.set_mode = (P.S <= kMaxSplitSets) ? 0 : 1;
-    size_t lds = value_lds_bytes(P.D, P.Fmax, P.C, P.set_mode == 0 ? P.S : 0, P.Lmax, P.Zmax, P.NF);
+    // split mode stages the set entries in LDS; when they do not fit, one wave per set reads them
+    P.set_mode = (P.S <= kMaxSplitSets &&
+                  value_lds_bytes(P.D, P.Fmax, P.C, P.S, P.Lmax, P.Zmax, P.NF, P.n_set_entries) <= P.ctx->lds_limit)
+                     ? 0 : 1;
+    size_t lds = value_lds_bytes(P.D, P.Fmax, P.C, P.set_mode == 0 ? P.S : 0, P.Lmax, P.Zmax, P.NF,
+                                 P.set_mode == 0 ? P.n_set_entries : 0);
     if (lds > P.ctx->lds_limit)
         raise(EQHIP_E_CAPACITY, "portfolio too diverse for one plan: " + std::to_string(P.D) +
                                     " cash-flow dates and " + std::to_string(P.C) + " schedule classes need " +
@@ -396,6 +401,7 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
         }
         set_off[s + 1] = (int)set_cls.size();
     }
+    P.n_set_entries = (int)set_cls.size();
     std::vector<int> set_nlive((size_t)n_dates * P.S);
     for (int k = 0; k < n_dates; ++k)
         for (int s = 0; s < P.S; ++s) {
@@ -584,6 +590,7 @@ void build_value_plan(eqhip_plan& P) {
     g.n_chunks = P.n_chunks;
     g.tiles_per_block = P.tiles_per_block;
     g.set_mode = P.set_mode;
+    g.n_set_entries = P.n_set_entries;
     g.dates = P.d_dates.as<double>();
     g.AB = P.AB.as<double2>();
     g.date_slot = P.date_slot.as<int>();

@@ -276,6 +276,16 @@ __device__ __forceinline__ double uni(double v) {
     return __builtin_bit_cast(double, lo | (hi << 32));
 }
 
+__device__ __forceinline__ int2 uni2(int2 v) { return make_int2(uni(v.x), uni(v.y)); }
+
+// Global memory touched inside the tile loop is accessed through pointers in the global address
+// space. Generic (flat) accesses also count against the LDS wait counter, so every LDS wait after a
+// flat load or store would wait for that memory access too.
+template <class T> using gptr = const __attribute__((address_space(1))) T*;
+template <class T> using gmut = __attribute__((address_space(1))) T*;
+template <class T> __device__ __forceinline__ gptr<T> glb(const T* p) { return (gptr<T>)p; }
+template <class T> __device__ __forceinline__ gmut<T> glb(T* p) { return (gmut<T>)p; }
+
 __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, const int chunk, double* smem) {
     // a private copy of the descriptor: the kernel's own stores cannot alias it, so its fields stay
     // in scalar registers instead of being reloaded after every global store
@@ -289,15 +299,17 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     double* sF = sP + (size_t)g.D * kPathsPerTile;         // [Fmax][64]
     double2* sC = (double2*)(sF + (size_t)g.Fmax * kPathsPerTile);  // [64][C] path-major
     const int CS = g.C;
-    const int* __restrict__ set_cls = g.set_cls;
-    const double* __restrict__ set_alpha = g.set_alpha;
-    const double* __restrict__ set_beta = g.set_beta;
     double* sPart = (double*)(sC + (size_t)g.C * kPathsPerTile);    // [S][waves][64] (split mode)
     double* sSeg = sPart + (g.set_mode == 0 ? (size_t)g.S * kValueWaves * kPathsPerTile : 0);  // [waves][64]
     double* sLT = sSeg + (size_t)kValueWaves * kPathsPerTile;       // [Lmax] live-flow accruals
-    int4* sLF = (int4*)(sLT + ((g.Lmax + 1) & ~1));                 // [Lmax] live flows
+    double2* sAB = (double2*)(sLT + ((g.Lmax + 1) & ~1));           // [D] (A, B) of P(t_k, T_d)
+    const int E_split = (g.set_mode == 0) ? g.n_set_entries : 0;
+    double2* sSAB = sAB + g.D;                                      // [E_split] set entries (alpha, beta)
+    int4* sLF = (int4*)(sSAB + E_split);                            // [Lmax] live flows
     int4* sFD = sLF + g.Lmax;                                       // [NF] the date's family descriptors
-    int* sZc = (int*)(sFD + g.NF);                                  // [Zmax] matured classes
+    int* sSCl = (int*)(sFD + g.NF);                                 // [E_split] set entries' classes
+    int* sZc = sSCl + E_split;                                      // [Zmax] matured classes
+    int2* sSet = (int2*)(sZc + g.Zmax + ((E_split + g.Zmax) & 1));   // [S_split] live entry range per set
 
     const double t = g.dates[k];
     const int dfirst = g.date_dfirst[k];
@@ -306,7 +318,7 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     const int nfix = g.fix_off[k + 1] - fbeg;
     const int det = g.date_det[k];
     const uint64_t n_paths = g.n_paths;
-    const double* Rk = g.R + (uint64_t)g.date_slot[k] * g.r_stride;
+    const gptr<double> Rk = glb(g.R + (uint64_t)g.date_slot[k] * g.r_stride);
     const double2* ab = g.AB + (size_t)k * g.D;
     const int4* fdesc = g.fdesc + (size_t)k * g.NF;
 
@@ -355,6 +367,14 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     }
     for (int i = tid; i < n_zc; i += kValueThreads) sZc[i] = g.zcls[zb + i];
     for (int i = tid; i < g.NF; i += kValueThreads) sFD[i] = fdesc[i];
+    for (int i = dfirst + tid; i < g.D; i += kValueThreads) sAB[i] = ab[i];
+    for (int i = tid; i < E_split; i += kValueThreads) {
+        sSCl[i] = g.set_cls[i];
+        sSAB[i] = make_double2(g.set_alpha[i], g.set_beta[i]);
+    }
+    if (g.set_mode == 0)
+        for (int s = tid; s < g.S; s += kValueThreads)
+            sSet[s] = make_int2(g.set_off[s], g.set_off[s] + g.set_nlive[(size_t)k * g.S + s]);
     __syncthreads();
     // classes already matured at t_k: zeroed once, no live flow writes them during this unit
     for (int i = wave; i < n_zc; i += kValueWaves) sC[lane * CS + uni(sZc[i])] = make_double2(0.0, 0.0);
@@ -365,7 +385,8 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
 
     // the short rates of the next tile are loaded one tile ahead (wave 0: r at t_k; wave f < nfix:
     // r at fixing f), so that no tile starts by waiting on HBM
-    const double* Rfix = (wave < nfix) ? g.R + (uint64_t)g.fix_slot[fbeg + wave] * g.r_stride : nullptr;
+    const gptr<double> Rfix = (wave < nfix) ? glb(g.R + (uint64_t)g.fix_slot[fbeg + wave] * g.r_stride) : nullptr;
+    const double2 fab_w = (wave < nfix) ? g.fix_AB[fbeg + wave] : make_double2(0.0, 0.0);
     double r_next = 0.0, rf_next = 0.0;
     if (tile0 < tile_end) {
         const uint64_t gq = tile0 * kPathsPerTile + lane;
@@ -381,8 +402,13 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
         __syncthreads();
         if (wave == 0) sR[lane] = r_next;
         for (int f = wave; f < nfix; f += kValueWaves) {
-            double rf = (f == wave) ? rf_next : (valid ? g.R[(uint64_t)g.fix_slot[fbeg + f] * g.r_stride + gp] : 0.0);
-            double2 fab = g.fix_AB[fbeg + f];
+            double rf = rf_next;
+            double2 fab = fab_w;
+            if (f != wave) {  // more fixings than waves
+                rf = valid ? glb(g.R)[(uint64_t)glb(g.fix_slot)[fbeg + f] * g.r_stride + gp] : 0.0;
+                const gptr<double> fp = glb((const double*)(g.fix_AB + fbeg + f));
+                fab = make_double2(fp[0], fp[1]);
+            }
             sF[f * kPathsPerTile + lane] = fab.x * exp(-(fab.y * rf));
         }
         __syncthreads();
@@ -391,7 +417,7 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
             const double r = sR[lane];
             for (int dl = wave; dl < n_live; dl += kValueWaves) {
                 const int d = dfirst + dl;
-                const double2 v = ab[d];
+                const double2 v = make_double2(uni(sAB[d].x), uni(sAB[d].y));
                 sP[d * kPathsPerTile + lane] = v.x * exp(-(v.y * r));
             }
         }
@@ -479,46 +505,43 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
         // C: netting sets
         if (g.set_mode == 0) {
             for (int s = 0; s < g.S; ++s) {
-                const int beg = g.set_off[s], end = beg + g.set_nlive[(size_t)k * g.S + s];
-                const int per = (end - beg + kValueWaves - 1) / kValueWaves;
-                const int cb = beg + wave * per;
-                const int ce = min(end, cb + per);
+                // the set's classes alive at t (a prefix of its entries), split evenly over the waves
+                const int2 rng = uni2(sSet[s]);
+                const int per = (rng.y - rng.x + kValueWaves - 1) / kValueWaves;
+                const int cb = rng.x + wave * per;
+                const int ce = min(rng.y, cb + per);
                 double v = 0.0;
-                // the set's classes alive at t, split evenly over the waves
-                auto set_sum = [&](const int* __restrict__ scls, const double* __restrict__ salpha,
-                                   const double* __restrict__ sbeta) {
-                    int e = cb;
-                    for (; e + 4 <= ce; e += 4) {
-                        double2 c2[4];
-                        double al[4], be[4];
+                int e = cb;
+                for (; e + 4 <= ce; e += 4) {
+                    double2 c2[4], ab4[4];
+                    int cl[4];
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            al[u] = salpha[e + u];
-                            be[u] = sbeta[e + u];
-                            c2[u] = sC[lane * CS + scls[e + u]];
-                        }
+                    for (int u = 0; u < 4; ++u) cl[u] = uni(sSCl[e + u]);
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) v += al[u] * c2[u].x - be[u] * c2[u].y;
+                    for (int u = 0; u < 4; ++u) {
+                        ab4[u] = make_double2(uni(sSAB[e + u].x), uni(sSAB[e + u].y));
+                        c2[u] = sC[lane * CS + cl[u]];
                     }
-                    for (; e < ce; ++e) {
-                        const double2 c2 = sC[lane * CS + scls[e]];
-                        v += salpha[e] * c2.x - sbeta[e] * c2.y;
-                    }
-                };
-                set_sum(set_cls, set_alpha, set_beta);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) v += ab4[u].x * c2[u].x - ab4[u].y * c2[u].y;
+                }
+                for (; e < ce; ++e) {
+                    const double2 c2 = sC[lane * CS + uni(sSCl[e])];
+                    v += uni(sSAB[e].x) * c2.x - uni(sSAB[e].y) * c2.y;
+                }
                 sPart[(s * kValueWaves + wave) * kPathsPerTile + lane] = v;
             }
         } else {
             for (int s = wave; s < g.S; s += kValueWaves) {
                 double v = 0.0;
-                const int beg = g.set_off[s], end = beg + g.set_nlive[(size_t)k * g.S + s];
+                const int beg = glb(g.set_off)[s], end = beg + glb(g.set_nlive)[(size_t)k * g.S + s];
                 for (int e = beg; e < end; ++e) {
-                    const double2 c2 = sC[lane * CS + set_cls[e]];
-                    v += set_alpha[e] * c2.x - set_beta[e] * c2.y;
+                    const double2 c2 = sC[lane * CS + glb(g.set_cls)[e]];
+                    v += glb(g.set_alpha)[e] * c2.x - glb(g.set_beta)[e] * c2.y;
                 }
                 if (valid) {
-                    g.X[((uint64_t)s * g.n_dates + k) * n_paths + gp] = (v > 0.0) ? v : 0.0;
-                    if (g.set_mtm) g.set_mtm[((uint64_t)k * g.S + s) * n_paths + gp] = v;
+                    glb(g.X)[((uint64_t)s * g.n_dates + k) * n_paths + gp] = (v > 0.0) ? v : 0.0;
+                    if (g.set_mtm) glb(g.set_mtm)[((uint64_t)k * g.S + s) * n_paths + gp] = v;
                 }
             }
         }
@@ -548,7 +571,7 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                     for (int q = 0; q < icnt[i]; ++q) {
                         const int j = ifirst[i] + q;
                         const double N = g.tr_N[j], NK = g.tr_NK[j];
-                        double* dst = g.path_mtm + ((uint64_t)k * g.n_trades + g.tr_orig[j]) * n_paths + p0;
+                        const gmut<double> dst = glb(g.path_mtm + ((uint64_t)k * g.n_trades + g.tr_orig[j]) * n_paths + p0);
                         for (int p = islice_b[i]; p < min(islice_e[i], pvalid); ++p)
                             dst[p] = N * col[p * CS].x - NK * col[p * CS].y;
                     }
@@ -562,8 +585,8 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                 double v = 0.0;
                 for (int w = 0; w < kValueWaves; ++w) v += sPart[(s * kValueWaves + w) * kPathsPerTile + lane];
                 if (valid) {
-                    g.X[((uint64_t)s * g.n_dates + k) * n_paths + gp] = (v > 0.0) ? v : 0.0;
-                    if (g.set_mtm) g.set_mtm[((uint64_t)k * g.S + s) * n_paths + gp] = v;
+                    glb(g.X)[((uint64_t)s * g.n_dates + k) * n_paths + gp] = (v > 0.0) ? v : 0.0;
+                    if (g.set_mtm) glb(g.set_mtm)[((uint64_t)k * g.S + s) * n_paths + gp] = v;
                 }
             }
         }
@@ -583,7 +606,7 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                     const int j = item.y + q;
                     // a matured class contributes exact zeros
                     const double e = (icnt[i] == 0) ? 0.0 : g.tr_absN[j] * acc[i][q];
-                    g.partial[((uint64_t)k * n_parts + chunk * g.n_slices + sl) * g.n_trades + g.tr_orig[j]] = e;
+                    glb(g.partial)[((uint64_t)k * n_parts + chunk * g.n_slices + sl) * g.n_trades + g.tr_orig[j]] = e;
                 }
             }
         }

@@ -145,3 +145,49 @@ def test_many_tiles_per_unit_shared_draws(ctx):
     assert_rel(got["set_ee"], ref["set_ee"], 1e-10, scale=set_scale[:, None], what="set EE")
     assert_rel(got["set_pfe"], ref["set_pfe"], 1e-10, scale=set_scale[:, None], what="set PFE")
     assert_rel(got["trade_ee"], ref["trade_ee"], 1e-10, scale=scale[:, None], what="trade EE")
+
+
+def staggered_annual_portfolio(n_sets=12, seed=6):
+    """Annual swaps starting one month apart (12 families, one per start month), two per start
+    (2y and 3y), trade j in set j mod n_sets. Between resets every family is in a period fixed at
+    a different month, so a monitoring date carries 12 fixings."""
+    rng = np.random.default_rng(seed)
+    pays, accs, off, starts = [], [], [0], []
+    for m in range(12):
+        for years in (2, 3):
+            s0 = m / 12.0
+            pays.append(s0 + np.arange(1, years + 1, dtype=np.float64))
+            accs.append(np.ones(years))
+            off.append(off[-1] + years)
+            starts.append(s0)
+    n = len(starts)
+    notional = np.exp(rng.normal(np.log(1e6), 0.5, n)) * np.where(rng.uniform(size=n) < 0.5, 1, -1)
+    return eqhip.Portfolio(set_id=(np.arange(n) % n_sets).astype(np.int32), notional=notional,
+                           fixed_rate=0.02 + rng.normal(0, 0.004, n), start=np.array(starts),
+                           flow_off=np.array(off, dtype=np.int32), pay=np.concatenate(pays), acc=np.concatenate(accs))
+
+
+def test_many_sets_and_more_fixings_than_waves(ctx):
+    """More netting sets than the split limit (one wave per set) and more fixings at a date than
+    the block has waves, against the oracle, with per-path checks."""
+    port = staggered_annual_portfolio(n_sets=12)
+    assert port.n_sets > 8
+    dates = np.concatenate([[0.0], (np.arange(35) + 0.5) / 12.0])  # between the monthly resets
+    n_paths, n_steps = 900, 69  # dt = 1/24: every reset and date on the grid
+    t = dates[20]
+    mid = {round(s0 + np.floor(t - s0), 9) for s0 in port.start if s0 < t}
+    assert len(mid) > 8  # fixings at this date
+    z = np.random.default_rng(4).standard_normal((n_steps, n_paths))
+    got = ctx.exposure_run(MODEL, port, dates, n_paths, n_steps, rng=eqhip.RNG_HOST_DRAWS, disc=eqhip.DISC_EXACT,
+                           host_draws=z, hazard_rate=0.02, recovery_rate=0.4, want_trade_ee=True,
+                           want_path=True, want_set_mtm=True)
+    ref = oracle.portfolio_exposure(MODEL, port.as_dict(), dates, n_steps, z, scheme=1, hazard_rate=0.02,
+                                    recovery_rate=0.4, want_trade=True, want_path=True)
+    scale = leg_scale(port, MODEL, dates)
+    set_scale = np.array([scale[port.set_id == s].sum() for s in range(port.n_sets)])
+    assert_rel(got["path_mtm"], ref["path_mtm"], 1e-12, scale=scale[None, :, None], what="per-path MtM")
+    assert_rel(got["set_mtm"], ref["set_mtm"], 1e-12, scale=set_scale[None, :, None], what="set MtM")
+    assert_rel(got["set_ee"], ref["set_ee"], 1e-10, scale=set_scale[:, None], what="set EE")
+    assert_rel(got["set_pfe"], ref["set_pfe"], 1e-10, scale=set_scale[:, None], what="set PFE")
+    assert_rel(got["set_cva"], ref["set_cva"], 1e-10, scale=set_scale, what="set CVA")
+    assert_rel(got["trade_ee"], ref["trade_ee"], 1e-10, scale=scale[:, None], what="trade EE")
