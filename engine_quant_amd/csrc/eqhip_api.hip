@@ -72,7 +72,6 @@ struct eqhip_plan {
     uint64_t n_tiles = 0;
     int want_trade = 0;
     int n_items = 0;
-    int n_slices = 1;
     int set_mode = 0;
     int n_set_entries = 0;
     uint64_t target_units = 2048;   // work units the launch aims for (chunking of the path tiles)
@@ -85,7 +84,7 @@ struct eqhip_plan {
     DevBuf lflow_off, lflow, ltau, lseg, zcls_off, zcls;
     DevBuf slot_of_step, R, Z, AB, date_slot, date_dfirst, date_det, fdesc, fam_off, flow_d, flow_tau,
         flow_cls, fix_off, fix_slot, fix_AB, set_off, set_nlive, set_cls, set_alpha, set_beta, X, items, cls_mat, tr_K,
-        tr_absN, tr_N, tr_NK, tr_orig, partial, trade_ee, trade_cva, set_ee, set_pfe, set_cva, neff, qrank,
+        tr_absN, tr_N, tr_NK, tr_orig, item_first, partial, trade_ee, trade_cva, set_ee, set_pfe, set_cva, neff, qrank,
         d_dates, vplan, units, rows;
     ValuePlan vp{};  // host copy of the device descriptor (per-run pointers patched at launch)
     // host copies of the plan tables (used by the adjoint)
@@ -151,6 +150,8 @@ void choose_launch(eqhip_plan& P) {
                                     " cash-flow dates and " + std::to_string(P.C) + " schedule classes need " +
                                     std::to_string(lds) + " B of LDS (limit " + std::to_string(P.ctx->lds_limit) +
                                     "); split it into several plans");
+    // per-trade sums of a unit's work slots are reduced through LDS at the end of the unit
+    if (P.want_trade) lds = std::max(lds, value_trade_scratch_bytes());
     P.lds_bytes = lds;
     P.n_tiles = (P.mc.n_paths + kPathsPerTile - 1) / kPathsPerTile;
     // enough units for ~8 blocks per CU (one 512-thread block per CU at this LDS footprint)
@@ -414,7 +415,7 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
     // kTradesPerItem per item. Ordering by maturity keeps the items still alive at a date a
     // contiguous suffix of the list, so the waves whose items have all matured skip the date.
     std::vector<int4> items;
-    std::vector<int> tr_orig;
+    std::vector<int> tr_orig, item_first(n_dates, 0);
     std::vector<double> tr_K, tr_absN, tr_N, tr_NK;
     if (P.want_trade) {
         auto sgn = [&](int j) { return pf->notional[j] < 0.0 ? -1 : 1; };
@@ -444,17 +445,11 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
             items.push_back(item);
         }
         P.n_items = (int)items.size();
-        // path slices per item: fill the block's work units as evenly as possible
-        double best = -1.0;
-        for (int sl = 1; sl <= 8; ++sl) {
-            const int units = P.n_items * sl;
-            if (units > kValueThreads * kItemsPerThread) break;
-            const int rounds = (units + kValueThreads - 1) / kValueThreads;
-            const double util = (double)units / (double)(rounds * kValueThreads);
-            if (util > best + 1e-9) {
-                best = util;
-                P.n_slices = sl;
-            }
+        // first item alive at each date: the items alive at a date are a suffix of the list
+        for (int k = 0; k < n_dates; ++k) {
+            int f = 0;
+            while (f < P.n_items && !(cls_mat[items[f].x] > P.dates[k] + kDateTolHost)) ++f;
+            item_first[k] = f;
         }
         if (P.n_items > kValueThreads * kItemsPerThread)
             raise(EQHIP_E_CAPACITY, "per-trade EE supports at most " +
@@ -519,7 +514,8 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
         upload(P.tr_N, tr_N, st);
         upload(P.tr_NK, tr_NK, st);
         upload(P.tr_orig, tr_orig, st);
-        P.partial.alloc((size_t)P.n_trades * n_dates * P.n_chunks * P.n_slices * 8);
+        upload(P.item_first, item_first, st);
+        P.partial.alloc((size_t)P.n_trades * n_dates * P.n_chunks * 8);
         P.trade_ee.alloc((size_t)P.n_trades * n_dates * 8);
         P.trade_cva.alloc((size_t)P.n_trades * 8);
     }
@@ -621,7 +617,7 @@ void build_value_plan(eqhip_plan& P) {
     g.want_trade = P.want_trade;
     g.n_items = P.n_items;
     g.n_trades = P.n_trades;
-    g.n_slices = P.n_slices;
+    g.item_first = P.item_first.as<int>();
     g.items = P.items.as<int4>();
     g.cls_mat = P.cls_mat.as<double>();
     g.tr_K = P.tr_K.as<double>();
@@ -717,7 +713,7 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
     if (P.want_trade) {
         int n = P.n_trades * P.n_dates;
         hipLaunchKernelGGL(trade_ee_kernel, dim3((n + 255) / 256), dim3(256), 0, st, P.partial.as<double>(),
-                           P.n_trades, P.n_dates, P.n_chunks * P.n_slices, P.neff.as<uint64_t>(), P.trade_ee.as<double>());
+                           P.n_trades, P.n_dates, P.n_chunks, P.neff.as<uint64_t>(), P.trade_ee.as<double>());
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(P.ev[3], st));
@@ -1117,7 +1113,7 @@ void layout_batch(eqhip_batch& B) {
         double share = (double)(P.n_tiles * (uint64_t)P.n_dates) / (double)std::max<uint64_t>(1, total_tiles);
         P.target_units = std::max<uint64_t>(1, (uint64_t)(share * (double)(B.target_units_per_cu * cus)));
         choose_launch(P);
-        if (P.want_trade) P.partial.alloc((size_t)P.n_trades * P.n_dates * P.n_chunks * P.n_slices * 8);
+        if (P.want_trade) P.partial.alloc((size_t)P.n_trades * P.n_dates * P.n_chunks * 8);
         P.r_shared = B.R.as<double>();
         P.r_stride = B.max_paths;
         build_value_plan(P);
@@ -1256,7 +1252,7 @@ void execute_batch(eqhip_batch& B, const eqhip_credit* credit, eqhip_outputs* ou
         if (!J->want_trade) continue;
         int n = J->n_trades * J->n_dates;
         hipLaunchKernelGGL(trade_ee_kernel, dim3((n + 255) / 256), dim3(256), 0, st, J->partial.as<double>(),
-                           J->n_trades, J->n_dates, J->n_chunks * J->n_slices, J->neff.as<uint64_t>(), J->trade_ee.as<double>());
+                           J->n_trades, J->n_dates, J->n_chunks, J->neff.as<uint64_t>(), J->trade_ee.as<double>());
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(B.ev[3], st));

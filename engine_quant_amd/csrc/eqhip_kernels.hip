@@ -322,7 +322,14 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     const double2* ab = g.AB + (size_t)k * g.D;
     const int4* fdesc = g.fdesc + (size_t)k * g.NF;
 
-    // trade work items owned by this thread
+    // Per-trade work. The items alive at t_k are a suffix of the maturity-sorted list (every item
+    // when per-path values are requested); each gets ns equal slices of a tile's paths, ns as large
+    // as the block's kTradeSlots work slots allow, so that the date's work is spread over every wave.
+    // Slot u = (live item u / ns, slice u % ns); thread tid holds slots tid + i * kValueThreads.
+    const int item0 = !g.want_trade ? g.n_items : (g.path_mtm ? 0 : g.item_first[k]);
+    const int n_li = g.n_items - item0;
+    const int ns = (n_li > 0) ? max(1, min(kPathsPerTile, kTradeSlots / n_li)) : 1;
+    const int n_slots = n_li * ns;
     double acc[kItemsPerThread][kTradesPerItem];
     double tK[kItemsPerThread][kTradesPerItem];
     int icls[kItemsPerThread], icnt[kItemsPerThread], ifirst[kItemsPerThread];
@@ -330,23 +337,21 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     int islice_b[kItemsPerThread], islice_e[kItemsPerThread];
 #pragma unroll
     for (int i = 0; i < kItemsPerThread; ++i) {
-        const int u = tid + i * kValueThreads;  // unit = item * n_slices + slice
-        const int it = u / g.n_slices;
-        const int sl = u - it * g.n_slices;
-        islice_b[i] = (sl * kPathsPerTile) / g.n_slices;
-        islice_e[i] = ((sl + 1) * kPathsPerTile) / g.n_slices;
+        const int u = tid + i * kValueThreads;
+        const int li = u / ns;
+        const int sl = u - li * ns;
+        islice_b[i] = (sl * kPathsPerTile) / ns;
+        islice_e[i] = ((sl + 1) * kPathsPerTile) / ns;
         icnt[i] = 0;
         icls[i] = 0;
         ifirst[i] = 0;
         isgn[i] = 1.0;
-        if (g.want_trade && it < g.n_items) {
-            int4 item = g.items[it];
+        if (u < n_slots) {
+            const int4 item = g.items[item0 + li];
             icls[i] = item.x;
             ifirst[i] = item.y;
             isgn[i] = (double)item.w;
-            // the class is alive at t when its last payment is still to come
-            icnt[i] = (g.cls_mat[item.x] > t + kDateTol) ? item.z : 0;
-            if (g.path_mtm) icnt[i] = item.z;
+            icnt[i] = item.z;
         }
 #pragma unroll
         for (int q = 0; q < kTradesPerItem; ++q) {
@@ -592,23 +597,32 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
         }
     }
     if (g.want_trade) {
+        // the slots' sums through LDS, then one fixed-order sum over the slices of each trade
+        __syncthreads();
+        double* sAcc = smem;  // [kTradeSlots][kTradesPerItem]
 #pragma unroll
         for (int i = 0; i < kItemsPerThread; ++i) {
             const int u = tid + i * kValueThreads;
-            const int it = u / g.n_slices;
-            const int sl = u - it * g.n_slices;
-            if (!(it < g.n_items)) continue;
-            const int4 item = g.items[it];
-            const int n_parts = g.n_chunks * g.n_slices;
+            if (u < n_slots)
 #pragma unroll
-            for (int q = 0; q < kTradesPerItem; ++q) {
-                if (q < item.z) {
-                    const int j = item.y + q;
-                    // a matured class contributes exact zeros
-                    const double e = (icnt[i] == 0) ? 0.0 : g.tr_absN[j] * acc[i][q];
-                    glb(g.partial)[((uint64_t)k * n_parts + chunk * g.n_slices + sl) * g.n_trades + g.tr_orig[j]] = e;
-                }
+                for (int q = 0; q < kTradesPerItem; ++q) sAcc[u * kTradesPerItem + q] = acc[i][q];
+        }
+        __syncthreads();
+        const gmut<double> part = glb(g.partial + ((uint64_t)k * g.n_chunks + chunk) * g.n_trades);
+        for (int w = tid; w < g.n_items * kTradesPerItem; w += kValueThreads) {
+            const int it = w / kTradesPerItem;
+            const int q = w - it * kTradesPerItem;
+            const int4 item = g.items[it];
+            if (q >= item.z) continue;
+            const int j = item.y + q;
+            double e = 0.0;  // a matured class contributes exact zeros
+            if (it >= item0) {
+                const double* a = sAcc + (it - item0) * ns * kTradesPerItem + q;
+                double sum = 0.0;
+                for (int sl = 0; sl < ns; ++sl) sum += a[sl * kTradesPerItem];
+                e = g.tr_absN[j] * sum;
             }
+            part[g.tr_orig[j]] = e;
         }
     }
 }

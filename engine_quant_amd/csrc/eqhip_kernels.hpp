@@ -11,6 +11,7 @@ constexpr int kValueWaves = kValueThreads / 64;
 constexpr int kPathsPerTile = 64;    // one path per lane
 constexpr int kItemsPerThread = 2;   // trade work items per thread in the valuation block
 constexpr int kTradesPerItem = 4;    // trades of one schedule class and sign per work item
+constexpr int kTradeSlots = kValueThreads * kItemsPerThread;  // (item, path slice) work slots per block
 constexpr int kMaxSplitSets = 8;     // netting sets whose class sums are split across the waves
 constexpr int kSelectThreads = 1024;  // EE / PFE select block
 constexpr int kSelectCand = 4096;     // candidates ranked in LDS by the select kernel
@@ -71,15 +72,16 @@ struct ValuePlan {
     const double* set_beta;   // sum of notional * fixed rate
     double* X;                // [S][n_dates][n_paths] max(V_set, 0)
     double* set_mtm;          // optional [n_dates][S][n_paths]
-    int want_trade, n_items, n_trades, n_slices;  // n_slices: path slices per item (work units)
-    const int4* items;        // (class, first sorted trade, count, sign)
+    int want_trade, n_items, n_trades;
+    const int4* items;        // (class, first sorted trade, count, sign), latest maturity last
+    const int* item_first;    // [n_dates] first item alive at the date (the live items are a suffix)
     const double* cls_mat;    // [C] maturity of the class
     const double* tr_K;       // sorted trades: fixed rate, |notional|, notional, notional * rate
     const double* tr_absN;
     const double* tr_N;
     const double* tr_NK;
     const int* tr_orig;       // sorted -> caller trade index
-    double* partial;          // [n_dates][n_chunks * n_slices][n_trades] per-trade EE partial sums
+    double* partial;          // [n_dates][n_chunks][n_trades] per-trade EE partial sums
     double* path_mtm;         // optional [n_dates][n_trades][n_paths]
 };
 
@@ -98,6 +100,9 @@ __host__ __device__ inline size_t value_lds_bytes(int D, int Fmax, int C, int S_
            16 * ((size_t)D + E_split + Lmax + NF) + 4 * ((size_t)E_split + Zmax + ((E_split + Zmax) & 1)) +
            8 * (size_t)S_split;
 }
+
+// LDS that holds the per-slot trade sums of a unit for their reduction (floor of the allocation)
+__host__ __device__ inline size_t value_trade_scratch_bytes() { return 8 * (size_t)kTradeSlots * kTradesPerItem; }
 
 struct PvArgs {
     const double* pillars;
