@@ -154,7 +154,7 @@ void choose_launch(eqhip_plan& P) {
     if (P.want_trade) lds = std::max(lds, value_trade_scratch_bytes());
     P.lds_bytes = lds;
     P.n_tiles = (P.mc.n_paths + kPathsPerTile - 1) / kPathsPerTile;
-    // enough units for ~8 blocks per CU (one 512-thread block per CU at this LDS footprint)
+    // enough units for ~8 blocks per CU (one 1024-thread block per CU at this LDS footprint)
     uint64_t target_chunks = std::max<uint64_t>(1, (P.target_units + P.n_dates - 1) / P.n_dates);
     uint64_t tpb = std::max<uint64_t>(1, (P.n_tiles + target_chunks - 1) / target_chunks);
     P.tiles_per_block = (int)tpb;
@@ -352,24 +352,25 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
         lflow_off[k + 1] = (int)lflow.size();
         zcls_off[k + 1] = (int)zcls.size();
         const int L = (int)lflow.size() - l0;
-        // segment w = [w L / waves, (w+1) L / waves); it takes the totals of the earlier non-empty
-        // segments that end in the family its first flow belongs to (a family spans consecutive segments)
+        // segment w = [w L / waves, (w+1) L / waves); it takes the totals of the earlier segments
+        // w_lo .. w-1 that end in the family its first flow belongs to (a family spans consecutive
+        // segments; an empty segment in between totals zero)
         int sb[kValueWaves + 1];
         for (int w = 0; w <= kValueWaves; ++w) sb[w] = (w * L) / kValueWaves;
         auto fam_of = [&](int i) { return lflow[(size_t)l0 + i].z; };
         for (int w = 0; w < kValueWaves; ++w) {
-            int mask = 0, n_carry = 0;
+            int w_lo = w, n_carry = 0;
             if (sb[w + 1] > sb[w]) {
                 const int fam0 = fam_of(sb[w]);
                 for (int w2 = w - 1; w2 >= 0; --w2) {
                     if (sb[w2 + 1] == sb[w2]) continue;  // empty
                     if (fam_of(sb[w2 + 1] - 1) != fam0) break;
-                    mask |= 1 << w2;
+                    w_lo = w2;
                 }
-                if (mask)
+                if (w_lo < w)
                     for (int i = sb[w]; i < sb[w + 1] && fam_of(i) == fam0; ++i) ++n_carry;
             }
-            lseg[(size_t)k * kValueWaves + w] = make_int4(sb[w], sb[w + 1], mask, n_carry);
+            lseg[(size_t)k * kValueWaves + w] = make_int4(sb[w], sb[w + 1], w_lo, n_carry);
         }
         P.Lmax = std::max(P.Lmax, L);
         P.Zmax = std::max(P.Zmax, (int)zcls.size() - z0);

@@ -364,7 +364,7 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     const int lb = g.lflow_off[k], n_lf = g.lflow_off[k + 1] - lb;
     const int zb = g.zcls_off[k], n_zc = g.zcls_off[k + 1] - zb;
     const int4 segd = g.lseg[k * kValueWaves + wave];
-    const int seg_b = segd.x, seg_e = segd.y, carry_mask = segd.z, n_carry = segd.w;
+    const int seg_b = segd.x, seg_e = segd.y, carry_lo = segd.z, n_carry = segd.w;
     __syncthreads();  // the previous unit of a persistent block may still read this LDS
     for (int i = tid; i < n_lf; i += kValueThreads) {
         sLF[i] = g.lflow[lb + i];
@@ -481,19 +481,9 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
             sSeg[wave * kPathsPerTile + lane] = run;
         }
         __syncthreads();
-        if (carry_mask) {  // the totals of the earlier segments of this segment's first family
-            double tot[kValueWaves - 1];
-#pragma unroll
-            for (int w2 = 0; w2 < kValueWaves - 1; ++w2)  // independent reads, issued together
-                tot[w2] = ((carry_mask >> w2) & 1) ? sSeg[w2 * kPathsPerTile + lane] : 0.0;
+        if (n_carry > 0) {  // the totals of the earlier segments of this segment's first family
             double carry = 0.0;
-            bool any = false;
-#pragma unroll
-            for (int w2 = 0; w2 < kValueWaves - 1; ++w2)
-                if ((carry_mask >> w2) & 1) {
-                    carry = any ? carry + tot[w2] : tot[w2];
-                    any = true;
-                }
+            for (int w2 = carry_lo; w2 < wave; ++w2) carry += sSeg[w2 * kPathsPerTile + lane];
             for (int i = seg_b; i < seg_b + n_carry; i += 4) {  // reads of four classes, then the writes
                 int cls[4];
                 double a[4];

@@ -6,10 +6,10 @@
 
 namespace eqhip {
 
-constexpr int kValueThreads = 512;   // 8 wave64s per valuation block
+constexpr int kValueThreads = 1024;  // 16 wave64s per valuation block
 constexpr int kValueWaves = kValueThreads / 64;
 constexpr int kPathsPerTile = 64;    // one path per lane
-constexpr int kItemsPerThread = 2;   // trade work items per thread in the valuation block
+constexpr int kItemsPerThread = 1;   // trade work items per thread in the valuation block
 constexpr int kTradesPerItem = 4;    // trades of one schedule class and sign per work item
 constexpr int kTradeSlots = kValueThreads * kItemsPerThread;  // (item, path slice) work slots per block
 constexpr int kMaxSplitSets = 8;     // netting sets whose class sums are split across the waves
@@ -58,7 +58,7 @@ struct ValuePlan {
     const int* lflow_off;     // [n_dates + 1]
     const int4* lflow;        // (cash-flow date, class or -1, family, 0)
     const double* ltau;       // accrual of the flow
-    const int4* lseg;         // [n_dates][kValueWaves] (begin, end, mask of the earlier segments whose
+    const int4* lseg;         // [n_dates][kValueWaves] (begin, end, first of the earlier segments whose
                               //   totals carry into this one, number of leading flows that take the carry)
     const int* zcls_off;      // [n_dates + 1]
     const int* zcls;          // matured classes
