@@ -87,6 +87,7 @@ struct eqhip_plan {
         tr_absN, tr_N, tr_NK, tr_orig, item_first, partial, trade_ee, trade_cva, set_ee, set_pfe, set_cva, neff, qrank,
         d_dates, vplan, units, rows;
     ValuePlan vp{};  // host copy of the device descriptor (per-run pointers patched at launch)
+    PinnedBuf h_out; // page-locked staging of the outputs
     // host copies of the plan tables (used by the adjoint)
     std::vector<double> h_T;
     std::vector<int4> h_fdesc;
@@ -733,17 +734,34 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
     }
     HIPCHK(hipEventRecord(P.ev[4], st));
 
-    auto d2h = [&](void* dst, const DevBuf& src, size_t bytes) {
-        if (dst) HIPCHK(hipMemcpyAsync(dst, src.p, bytes, hipMemcpyDeviceToHost, st));
+    // profiles and CVAs come back through the plan's page-locked staging buffer (DMA-speed copies,
+    // then host copies into the caller's arrays); the optional per-path debug outputs go directly
+    struct Out {
+        void* dst;
+        const DevBuf* src;
+        size_t bytes, off;
     };
-    d2h(out->set_ee, P.set_ee, (size_t)P.S * P.n_dates * 8);
-    d2h(out->set_pfe, P.set_pfe, (size_t)P.S * P.n_dates * 8);
-    d2h(out->set_cva, P.set_cva, (size_t)P.S * 8);
-    d2h(out->trade_ee, P.trade_ee, (size_t)P.n_trades * P.n_dates * 8);
-    d2h(out->trade_cva, P.trade_cva, (size_t)P.n_trades * 8);
-    d2h(out->path_mtm, dpath, (size_t)P.n_dates * P.n_trades * np * 8);
-    d2h(out->set_mtm, dset, (size_t)P.n_dates * P.S * np * 8);
+    Out outs[5] = {{out->set_ee, &P.set_ee, (size_t)P.S * P.n_dates * 8, 0},
+                   {out->set_pfe, &P.set_pfe, (size_t)P.S * P.n_dates * 8, 0},
+                   {out->set_cva, &P.set_cva, (size_t)P.S * 8, 0},
+                   {out->trade_ee, &P.trade_ee, (size_t)P.n_trades * P.n_dates * 8, 0},
+                   {out->trade_cva, &P.trade_cva, (size_t)P.n_trades * 8, 0}};
+    size_t staged = 0;
+    for (Out& o : outs)
+        if (o.dst) {
+            o.off = staged;
+            staged += (o.bytes + 255) & ~(size_t)255;
+        }
+    reserve(P.h_out, staged);
+    for (const Out& o : outs)
+        if (o.dst) HIPCHK(hipMemcpyAsync(P.h_out.as<char>() + o.off, o.src->p, o.bytes, hipMemcpyDeviceToHost, st));
+    if (out->path_mtm)
+        HIPCHK(hipMemcpyAsync(out->path_mtm, dpath.p, (size_t)P.n_dates * P.n_trades * np * 8, hipMemcpyDeviceToHost, st));
+    if (out->set_mtm)
+        HIPCHK(hipMemcpyAsync(out->set_mtm, dset.p, (size_t)P.n_dates * P.S * np * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    for (const Out& o : outs)
+        if (o.dst) std::memcpy(o.dst, P.h_out.as<char>() + o.off, o.bytes);
 
     float ms[4];
     for (int i = 0; i < 4; ++i) HIPCHK(hipEventElapsedTime(&ms[i], P.ev[i], P.ev[i + 1]));
@@ -1480,43 +1498,60 @@ int eqhip_pv_dv01(eqhip_context* ctx, const double* pillars, const double* zero_
             if (!(pillars[i] > pillars[i - 1])) raise(EQHIP_E_INVALID, "Curve: pillars debe ser estrictamente creciente");
         if (!out_pv || !out_dv01) raise(EQHIP_E_INVALID, "out_pv / out_dv01 are required");
         hipStream_t st = ctx->stream;
-        int n = pf->n_trades, nf = pf->flow_offset[n];
-        DevBuf dp, dz, doff, dpay, dacc, dn, dk, ds, dpv, ddv, dbk;
-        upload(dp, std::vector<double>(pillars, pillars + n_pillars), st);
-        upload(dz, std::vector<double>(zero_rates, zero_rates + n_pillars), st);
-        upload(doff, std::vector<int>(pf->flow_offset, pf->flow_offset + n + 1), st);
-        upload(dpay, std::vector<double>(pf->payment_times, pf->payment_times + nf), st);
-        upload(dacc, std::vector<double>(pf->accruals, pf->accruals + nf), st);
-        upload(dn, std::vector<double>(pf->notional, pf->notional + n), st);
-        upload(dk, std::vector<double>(pf->fixed_rate, pf->fixed_rate + n), st);
-        upload(ds, std::vector<double>(pf->start, pf->start + n), st);
-        dpv.alloc((size_t)n * 8);
-        ddv.alloc((size_t)n * 8);
-        if (out_bucketed) dbk.alloc((size_t)n * n_pillars * 8);
+        HIPCHK(hipSetDevice(ctx->device));
+        const int n = pf->n_trades, nf = pf->flow_offset[n], np = n_pillars;
+        // inputs, then outputs, packed at 256-byte offsets: one upload and one download
+        std::lock_guard<std::mutex> lock(ctx->staging_lock);
+        size_t off = 0;
+        auto place = [&](size_t bytes) {
+            const size_t o = off;
+            off += (bytes + 255) & ~(size_t)255;
+            return o;
+        };
+        const size_t o_p = place((size_t)np * 8), o_z = place((size_t)np * 8), o_off = place((size_t)(n + 1) * 4),
+                     o_pay = place((size_t)nf * 8), o_acc = place((size_t)nf * 8), o_n = place((size_t)n * 8),
+                     o_k = place((size_t)n * 8), o_s = place((size_t)n * 8);
+        const size_t in_bytes = off;
+        const size_t o_pv = place((size_t)n * 8), o_dv = place((size_t)n * 8),
+                     o_bk = place(out_bucketed ? (size_t)n * np * 8 : 0);
+        const size_t out_bytes = off - in_bytes;
+        reserve(ctx->host_stage, off);
+        reserve(ctx->dev_stage, off);
+        char* h = ctx->host_stage.as<char>();
+        char* d = ctx->dev_stage.as<char>();
+        std::memcpy(h + o_p, pillars, (size_t)np * 8);
+        std::memcpy(h + o_z, zero_rates, (size_t)np * 8);
+        std::memcpy(h + o_off, pf->flow_offset, (size_t)(n + 1) * 4);
+        std::memcpy(h + o_pay, pf->payment_times, (size_t)nf * 8);
+        std::memcpy(h + o_acc, pf->accruals, (size_t)nf * 8);
+        std::memcpy(h + o_n, pf->notional, (size_t)n * 8);
+        std::memcpy(h + o_k, pf->fixed_rate, (size_t)n * 8);
+        std::memcpy(h + o_s, pf->start, (size_t)n * 8);
+        HIPCHK(hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, st));
         PvArgs g{};
-        g.pillars = dp.as<double>();
-        g.zeros = dz.as<double>();
-        g.np = n_pillars;
+        g.pillars = (const double*)(d + o_p);
+        g.zeros = (const double*)(d + o_z);
+        g.np = np;
         g.n_trades = n;
-        g.flow_off = doff.as<int>();
-        g.pay = dpay.as<double>();
-        g.acc = dacc.as<double>();
-        g.notional = dn.as<double>();
-        g.fixed = dk.as<double>();
-        g.start = ds.as<double>();
+        g.flow_off = (const int*)(d + o_off);
+        g.pay = (const double*)(d + o_pay);
+        g.acc = (const double*)(d + o_acc);
+        g.notional = (const double*)(d + o_n);
+        g.fixed = (const double*)(d + o_k);
+        g.start = (const double*)(d + o_s);
         g.bump = bump;
-        g.pv = dpv.as<double>();
-        g.dv01 = ddv.as<double>();
-        g.bucketed = out_bucketed ? dbk.as<double>() : nullptr;
+        g.pv = (double*)(d + o_pv);
+        g.dv01 = (double*)(d + o_dv);
+        g.bucketed = out_bucketed ? (double*)(d + o_bk) : nullptr;
         g.bucketed_on = out_bucketed ? 1 : 0;
-        int total = n * (2 + n_pillars);
+        int total = n * (2 + np);
         hipLaunchKernelGGL(pv_dv01_kernel, dim3((total + 127) / 128), dim3(128), 0, st, g);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(out_pv, dpv.p, (size_t)n * 8, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(out_dv01, ddv.p, (size_t)n * 8, hipMemcpyDeviceToHost, st));
-        if (out_bucketed)
-            HIPCHK(hipMemcpyAsync(out_bucketed, dbk.p, (size_t)n * n_pillars * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(h + in_bytes, d + in_bytes, out_bytes, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+        std::memcpy(out_pv, h + o_pv, (size_t)n * 8);
+        std::memcpy(out_dv01, h + o_dv, (size_t)n * 8);
+        if (out_bucketed) std::memcpy(out_bucketed, h + o_bk, (size_t)n * np * 8);
     });
 }
 

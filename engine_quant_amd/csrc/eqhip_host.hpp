@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -70,6 +71,37 @@ struct DevBuf {
     }
 };
 
+// Page-locked host allocation (hipHostMalloc) owned by its holder: copies between it and the
+// device run at DMA speed and can be asynchronous.
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    void alloc(size_t n) {
+        if (p) {
+            (void)hipHostFree(p);
+            p = nullptr;
+        }
+        bytes = n;
+        if (n) HIPCHK(hipHostMalloc(&p, n, hipHostMallocDefault));
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+// Grows a buffer to at least n bytes (contents are not kept).
+template <class B>
+void reserve(B& b, size_t n) {
+    if (b.bytes < n) b.alloc(std::max(n, b.bytes + b.bytes / 2));
+}
+
 // Synchronous copy: the source vector may be a temporary of the caller.
 template <class T>
 void upload(DevBuf& b, const std::vector<T>& v, hipStream_t) {
@@ -83,4 +115,9 @@ struct eqhip_context {
     int device = 0;
     hipStream_t stream = nullptr;
     size_t lds_limit = 65536;
+    // staging for the stateless entry points (PV / DV01 and its sensitivities): one packed upload,
+    // one kernel, one packed download per call; the lock makes calls on one context take turns
+    std::mutex staging_lock;
+    eqhip_host::PinnedBuf host_stage;
+    eqhip_host::DevBuf dev_stage;
 };
