@@ -685,6 +685,23 @@ __device__ void locate_rank(const unsigned int* hist, unsigned int* scan, unsign
 
 }  // namespace
 
+// Visits x[i] for i = tid, tid + T, tid + 2T, ... in that order, with eight loads in flight per
+// thread (a pass over a row is latency-bound with one).
+template <class F>
+__device__ __forceinline__ void select_pass(const unsigned long long* x, uint64_t n, int tid, F f) {
+    constexpr int U = 8;
+    const gptr<unsigned long long> gx = glb(x);
+    uint64_t i = tid;
+    for (; i + (uint64_t)(U - 1) * kSelectThreads < n; i += (uint64_t)U * kSelectThreads) {
+        unsigned long long k[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) k[u] = gx[i + (uint64_t)u * kSelectThreads];
+#pragma unroll
+        for (int u = 0; u < U; ++u) f(k[u]);
+    }
+    for (; i < n; i += kSelectThreads) f(gx[i]);
+}
+
 __global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow* rows) {
     __shared__ unsigned int hist[4096];
     __shared__ unsigned int scan[kSelectThreads];
@@ -702,11 +719,10 @@ __global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow*
     __syncthreads();
     // pass 1: EE sum and histogram of bits 63..52
     double sum = 0.0;
-    for (uint64_t i = tid; i < n; i += kSelectThreads) {
-        const unsigned long long key = x[i];
+    select_pass(x, n, tid, [&](unsigned long long key) {
         sum += __longlong_as_double(key);
         atomicAdd(&hist[key >> 52], 1u);
-    }
+    });
     red[tid] = sum;
     __syncthreads();
     for (int w = kSelectThreads / 2; w > 0; w >>= 1) {
@@ -731,10 +747,9 @@ __global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow*
         const int sh = hi - wd;
         for (int b = tid; b < 4096; b += kSelectThreads) hist[b] = 0u;
         __syncthreads();
-        for (uint64_t i = tid; i < n; i += kSelectThreads) {
-            const unsigned long long key = x[i];
+        select_pass(x, n, tid, [&](unsigned long long key) {
             if ((key >> hi) == (prefix >> hi)) atomicAdd(&hist[(key >> sh) & ((1u << wd) - 1u)], 1u);
-        }
+        });
         __syncthreads();
         locate_rank(hist, scan, s_rank, &s_bin, &s_before);
         if (tid == 0) {
@@ -753,10 +768,9 @@ __global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow*
     if (tid == 0) s_ncand = 0u;
     __syncthreads();
     const unsigned long long prefix = s_prefix;
-    for (uint64_t i = tid; i < n; i += kSelectThreads) {
-        const unsigned long long key = x[i];
+    select_pass(x, n, tid, [&](unsigned long long key) {
         if ((key >> hi) == (prefix >> hi)) cand[atomicAdd(&s_ncand, 1u)] = key;
-    }
+    });
     __syncthreads();
     const unsigned int nc = s_ncand;
     unsigned int m = 1;
