@@ -718,11 +718,17 @@ __global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow*
     for (int b = tid; b < 4096; b += kSelectThreads) hist[b] = 0u;
     __syncthreads();
     // pass 1: EE sum and histogram of bits 63..52
+    // keys in bin 0 (zero exposures, the common case) are counted per thread, not by LDS atomics
+    // that would all hit one address
     double sum = 0.0;
+    unsigned int c0 = 0u;
     select_pass(x, n, tid, [&](unsigned long long key) {
         sum += __longlong_as_double(key);
-        atomicAdd(&hist[key >> 52], 1u);
+        const unsigned int b = (unsigned int)(key >> 52);
+        if (b == 0u) ++c0;
+        else atomicAdd(&hist[b], 1u);
     });
+    if (c0) atomicAdd(&hist[0], c0);
     red[tid] = sum;
     __syncthreads();
     for (int w = kSelectThreads / 2; w > 0; w >>= 1) {
@@ -747,9 +753,15 @@ __global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow*
         const int sh = hi - wd;
         for (int b = tid; b < 4096; b += kSelectThreads) hist[b] = 0u;
         __syncthreads();
+        unsigned int c0 = 0u;
         select_pass(x, n, tid, [&](unsigned long long key) {
-            if ((key >> hi) == (prefix >> hi)) atomicAdd(&hist[(key >> sh) & ((1u << wd) - 1u)], 1u);
+            if ((key >> hi) == (prefix >> hi)) {
+                const unsigned int b = (unsigned int)((key >> sh) & ((1u << wd) - 1u));
+                if (b == 0u) ++c0;
+                else atomicAdd(&hist[b], 1u);
+            }
         });
+        if (c0) atomicAdd(&hist[0], c0);
         __syncthreads();
         locate_rank(hist, scan, s_rank, &s_bin, &s_before);
         if (tid == 0) {
