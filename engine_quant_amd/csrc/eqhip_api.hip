@@ -88,6 +88,7 @@ struct eqhip_plan {
         d_dates, vplan, units, rows;
     ValuePlan vp{};  // host copy of the device descriptor (per-run pointers patched at launch)
     PinnedBuf h_out; // page-locked staging of the outputs
+    DevBuf cva_w;    // per-date CVA factors
     // host copies of the plan tables (used by the adjoint)
     std::vector<double> h_T;
     std::vector<int4> h_fdesc;
@@ -660,6 +661,21 @@ void launch_value(eqhip_plan& P, double* path_mtm, double* set_mtm) {
     HIPCHK(hipGetLastError());
 }
 
+// Unilateral CVA of n_profiles EE rows ([profile][date]) on the stream: the per-date factors once
+// into w (n_dates double2), then one fixed-order sum per profile.
+void launch_cva_weights(hipStream_t st, const double* d_times, int n_dates, const eqhip_model_hw1f& m, double hazard,
+                        double2* w) {
+    hipLaunchKernelGGL(cva_weights_kernel, dim3((n_dates + 127) / 128), dim3(128), 0, st, d_times, n_dates, m.a, m.b,
+                       m.sigma, m.r0, hazard, w);
+    HIPCHK(hipGetLastError());
+}
+void launch_cva_sum(hipStream_t st, const double* ee, int n_profiles, int n_dates, const double2* w, double recovery,
+                    double* out) {
+    hipLaunchKernelGGL(cva_kernel, dim3((n_profiles + 63) / 64), dim3(64), 0, st, ee, n_profiles, n_dates, w, recovery,
+                       out);
+    HIPCHK(hipGetLastError());
+}
+
 void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* credit,
                   eqhip_outputs* out) {
     if (!out || !out->set_ee || !out->set_pfe) raise(EQHIP_E_INVALID, "outputs set_ee / set_pfe are required");
@@ -720,18 +736,15 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
     }
     HIPCHK(hipEventRecord(P.ev[3], st));
     double h = credit ? credit->hazard_rate : 0.0, rr = credit ? credit->recovery_rate : 0.0;
-    if (out->set_cva) {
-        hipLaunchKernelGGL(cva_kernel, dim3((P.S + 63) / 64), dim3(64), 0, st, P.set_ee.as<double>(), P.S,
-                           P.n_dates, P.d_dates.as<double>(), P.model.a, P.model.b, P.model.sigma,
-                           P.model.r0, h, rr, P.set_cva.as<double>());
-        HIPCHK(hipGetLastError());
+    if (out->set_cva || out->trade_cva) {
+        reserve(P.cva_w, (size_t)P.n_dates * 16);
+        launch_cva_weights(st, P.d_dates.as<double>(), P.n_dates, P.model, h, P.cva_w.as<double2>());
     }
-    if (out->trade_cva) {
-        hipLaunchKernelGGL(cva_kernel, dim3((P.n_trades + 63) / 64), dim3(64), 0, st, P.trade_ee.as<double>(),
-                           P.n_trades, P.n_dates, P.d_dates.as<double>(), P.model.a, P.model.b,
-                           P.model.sigma, P.model.r0, h, rr, P.trade_cva.as<double>());
-        HIPCHK(hipGetLastError());
-    }
+    if (out->set_cva)
+        launch_cva_sum(st, P.set_ee.as<double>(), P.S, P.n_dates, P.cva_w.as<double2>(), rr, P.set_cva.as<double>());
+    if (out->trade_cva)
+        launch_cva_sum(st, P.trade_ee.as<double>(), P.n_trades, P.n_dates, P.cva_w.as<double2>(), rr,
+                       P.trade_cva.as<double>());
     HIPCHK(hipEventRecord(P.ev[4], st));
 
     // profiles and CVAs come back through the plan's page-locked staging buffer (DMA-speed copies,
@@ -1278,15 +1291,15 @@ void execute_batch(eqhip_batch& B, const eqhip_credit* credit, eqhip_outputs* ou
     double h = credit ? credit->hazard_rate : 0.0, rr = credit ? credit->recovery_rate : 0.0;
     for (size_t j = 0; j < B.jobs.size(); ++j) {
         eqhip_plan& P = *B.jobs[j];
+        if (outs[j].set_cva || outs[j].trade_cva) {
+            reserve(P.cva_w, (size_t)P.n_dates * 16);
+            launch_cva_weights(st, P.d_dates.as<double>(), P.n_dates, P.model, h, P.cva_w.as<double2>());
+        }
         if (outs[j].set_cva)
-            hipLaunchKernelGGL(cva_kernel, dim3((P.S + 63) / 64), dim3(64), 0, st, P.set_ee.as<double>(), P.S,
-                               P.n_dates, P.d_dates.as<double>(), P.model.a, P.model.b, P.model.sigma, P.model.r0, h,
-                               rr, P.set_cva.as<double>());
+            launch_cva_sum(st, P.set_ee.as<double>(), P.S, P.n_dates, P.cva_w.as<double2>(), rr, P.set_cva.as<double>());
         if (outs[j].trade_cva)
-            hipLaunchKernelGGL(cva_kernel, dim3((P.n_trades + 63) / 64), dim3(64), 0, st, P.trade_ee.as<double>(),
-                               P.n_trades, P.n_dates, P.d_dates.as<double>(), P.model.a, P.model.b, P.model.sigma,
-                               P.model.r0, h, rr, P.trade_cva.as<double>());
-        HIPCHK(hipGetLastError());
+            launch_cva_sum(st, P.trade_ee.as<double>(), P.n_trades, P.n_dates, P.cva_w.as<double2>(), rr,
+                           P.trade_cva.as<double>());
     }
     HIPCHK(hipEventRecord(B.ev[4], st));
     for (size_t j = 0; j < B.jobs.size(); ++j) {
@@ -1469,13 +1482,14 @@ int eqhip_unilateral_cva_from_exposure_batch(eqhip_context* ctx, double a, doubl
             raise(EQHIP_E_INVALID, "invalid CVA arguments");
         if (n_profiles == 0) return;
         hipStream_t st = ctx->stream;
-        DevBuf dt, de, dc;
+        DevBuf dt, de, dc, dw;
         upload(dt, std::vector<double>(times, times + n_dates), st);
         upload(de, std::vector<double>(ee, ee + (size_t)n_profiles * n_dates), st);
         dc.alloc((size_t)n_profiles * 8);
-        hipLaunchKernelGGL(cva_kernel, dim3((n_profiles + 63) / 64), dim3(64), 0, st, de.as<double>(), n_profiles,
-                           n_dates, dt.as<double>(), a, b, sigma, r0, hazard_rate, recovery_rate, dc.as<double>());
-        HIPCHK(hipGetLastError());
+        dw.alloc(std::max<size_t>(1, (size_t)n_dates) * 16);
+        const eqhip_model_hw1f m{a, b, sigma, r0};
+        if (n_dates > 0) launch_cva_weights(st, dt.as<double>(), n_dates, m, hazard_rate, dw.as<double2>());
+        launch_cva_sum(st, de.as<double>(), n_profiles, n_dates, dw.as<double2>(), recovery_rate, dc.as<double>());
         HIPCHK(hipMemcpyAsync(out_cva, dc.p, (size_t)n_profiles * 8, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
     });

@@ -824,23 +824,32 @@ __global__ void trade_ee_kernel(const double* partial, int n_trades, int n_dates
 }
 
 // CVA = sum_k (1-R) EE_k (S_{k-1} - S_k) P(0,t_k | r0), S = exp(-h t), exposure.rs:242-275.
-__global__ void cva_kernel(const double* ee, int n_profiles, int n_dates, const double* times,
-                           double a, double b, double sigma, double r0, double hazard,
-                           double recovery, double* cva) {
+// The per-date factors of that sum, shared by every profile: (S_{k-1} - S_k, P(0,t_k | r0)).
+__global__ void cva_weights_kernel(const double* times, int n_dates, double a, double b, double sigma,
+                                   double r0, double hazard, double2* w) {
 #pragma clang fp contract(off)
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_dates) return;
+    const double t = times[k];
+    const double survival = exp(-hazard * t);
+    const double prev_survival = (k == 0) ? 1.0 : exp(-hazard * times[k - 1]);
+    const double default_prob = prev_survival - survival;
+    const double tau = t - 0.0;
+    const double bf = hw_b(a, tau);
+    const double disc = hw_a(a, b, sigma, tau, bf) * exp(-(bf * r0));
+    w[k] = make_double2(default_prob, disc);
+}
+
+// One profile per thread, the dates in order, each term as (1-R) * EE_k * dS_k * P_k.
+__global__ void cva_kernel(const double* ee, int n_profiles, int n_dates, const double2* w, double recovery,
+                           double* cva) {
+#pragma clang fp contract(off)
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n_profiles) return;
-    double acc = 0.0, prev_survival = 1.0;
-    for (int k = 0; k < n_dates; ++k) {
-        double t = times[k];
-        double survival = exp(-hazard * t);
-        double default_prob = prev_survival - survival;
-        double tau = t - 0.0;
-        double bf = hw_b(a, tau);
-        double disc = hw_a(a, b, sigma, tau, bf) * exp(-(bf * r0));
-        acc += (1.0 - recovery) * ee[(uint64_t)j * n_dates + k] * default_prob * disc;
-        prev_survival = survival;
-    }
+    const gptr<double> e = glb(ee + (uint64_t)j * n_dates);
+    const gptr<double> wd = glb((const double*)w);
+    double acc = 0.0;
+    for (int k = 0; k < n_dates; ++k) acc += (1.0 - recovery) * e[k] * wd[2 * k] * wd[2 * k + 1];
     cva[j] = acc;
 }
 

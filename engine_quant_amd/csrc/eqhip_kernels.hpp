@@ -180,9 +180,10 @@ struct SelectRow {
 __global__ void select_kernel(const SelectRow* rows);
 __global__ void trade_ee_kernel(const double* partial, int n_trades, int n_dates, int n_chunks,
                                 const uint64_t* n_eff, double* trade_ee);
-__global__ void cva_kernel(const double* ee, int n_profiles, int n_dates, const double* times,
-                           double a, double b, double sigma, double r0, double hazard,
-                           double recovery, double* cva);
+__global__ void cva_weights_kernel(const double* times, int n_dates, double a, double b, double sigma,
+                                   double r0, double hazard, double2* w);
+__global__ void cva_kernel(const double* ee, int n_profiles, int n_dates, const double2* w, double recovery,
+                           double* cva);
 __global__ void pv_dv01_kernel(PvArgs g);
 __global__ void pv_sens_kernel(PvArgs g, double a, double b, double sigma, double r0, int want_model,
                                double* pillar_delta, double* model_npv, double* model_greeks);
