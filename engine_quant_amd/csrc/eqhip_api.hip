@@ -73,14 +73,12 @@ struct eqhip_plan {
     int want_trade = 0;
     int n_items = 0;
     int set_mode = 0;
-    int n_set_entries = 0;
     uint64_t target_units = 2048;   // work units the launch aims for (chunking of the path tiles)
     const double* r_shared = nullptr;  // batch-owned short-rate buffer (all grid steps), or null
     uint64_t r_stride = 0;
     std::vector<double> dates;
     size_t lds_bytes = 0;
 
-    int Lmax = 0, Zmax = 0;
     DevBuf lflow_off, lflow, ltau, lseg, zcls_off, zcls;
     DevBuf slot_of_step, R, Z, AB, date_slot, date_dfirst, date_det, fdesc, fam_off, flow_d, flow_tau,
         flow_cls, fix_off, fix_slot, fix_AB, set_off, set_nlive, set_cls, set_alpha, set_beta, X, items, cls_mat, tr_K,
@@ -141,12 +139,9 @@ void validate_portfolio(const eqhip_portfolio* pf) {
 }
 
 void choose_launch(eqhip_plan& P) {
-    // split mode stages the set entries in LDS; when they do not fit, one wave per set reads them
-    P.set_mode = (P.S <= kMaxSplitSets &&
-                  value_lds_bytes(P.D, P.Fmax, P.C, P.S, P.Lmax, P.Zmax, P.NF, P.n_set_entries) <= P.ctx->lds_limit)
-                     ? 0 : 1;
-    size_t lds = value_lds_bytes(P.D, P.Fmax, P.C, P.set_mode == 0 ? P.S : 0, P.Lmax, P.Zmax, P.NF,
-                                 P.set_mode == 0 ? P.n_set_entries : 0);
+    // split mode keeps one partial sum per (set, wave) in LDS; when those do not fit, one wave per set
+    P.set_mode = (P.S <= kMaxSplitSets && value_lds_bytes(P.D, P.Fmax, P.C, P.S) <= P.ctx->lds_limit) ? 0 : 1;
+    size_t lds = value_lds_bytes(P.D, P.Fmax, P.C, P.set_mode == 0 ? P.S : 0);
     if (lds > P.ctx->lds_limit)
         raise(EQHIP_E_CAPACITY, "portfolio too diverse for one plan: " + std::to_string(P.D) +
                                     " cash-flow dates and " + std::to_string(P.C) + " schedule classes need " +
@@ -337,9 +332,8 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
     std::vector<int4> lseg((size_t)n_dates * kValueWaves);
     std::vector<int4> lflow;
     std::vector<double> ltau;
-    P.Lmax = P.Zmax = 0;
     for (int k = 0; k < n_dates; ++k) {
-        const int l0 = (int)lflow.size(), z0 = (int)zcls.size();
+        const int l0 = (int)lflow.size();
         for (int F = 0; F < P.NF; ++F) {
             const int4 fd = fdesc[(size_t)k * P.NF + F];
             const int f0 = fam_off[F], f1 = fam_off[F + 1];
@@ -374,8 +368,6 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
             }
             lseg[(size_t)k * kValueWaves + w] = make_int4(sb[w], sb[w + 1], w_lo, n_carry);
         }
-        P.Lmax = std::max(P.Lmax, L);
-        P.Zmax = std::max(P.Zmax, (int)zcls.size() - z0);
     }
     P.n_slots = (int)slot_of.size();
     std::vector<int> slot_of_step(steps + 1, -1);
@@ -405,7 +397,6 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
         }
         set_off[s + 1] = (int)set_cls.size();
     }
-    P.n_set_entries = (int)set_cls.size();
     std::vector<int> set_nlive((size_t)n_dates * P.S);
     for (int k = 0; k < n_dates; ++k)
         for (int s = 0; s < P.S; ++s) {
@@ -589,7 +580,6 @@ void build_value_plan(eqhip_plan& P) {
     g.n_chunks = P.n_chunks;
     g.tiles_per_block = P.tiles_per_block;
     g.set_mode = P.set_mode;
-    g.n_set_entries = P.n_set_entries;
     g.dates = P.d_dates.as<double>();
     g.AB = P.AB.as<double2>();
     g.date_slot = P.date_slot.as<int>();
@@ -600,8 +590,6 @@ void build_value_plan(eqhip_plan& P) {
     g.flow_d = P.flow_d.as<int>();
     g.flow_tau = P.flow_tau.as<double>();
     g.flow_cls = P.flow_cls.as<int>();
-    g.Lmax = P.Lmax;
-    g.Zmax = P.Zmax;
     g.lflow_off = P.lflow_off.as<int>();
     g.lflow = P.lflow.as<int4>();
     g.ltau = P.ltau.as<double>();

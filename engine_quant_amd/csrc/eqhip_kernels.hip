@@ -276,8 +276,6 @@ __device__ __forceinline__ double uni(double v) {
     return __builtin_bit_cast(double, lo | (hi << 32));
 }
 
-__device__ __forceinline__ int2 uni2(int2 v) { return make_int2(uni(v.x), uni(v.y)); }
-
 // Global memory touched inside the tile loop is accessed through pointers in the global address
 // space. Generic (flat) accesses also count against the LDS wait counter, so every LDS wait after a
 // flat load or store would wait for that memory access too.
@@ -285,11 +283,16 @@ template <class T> using gptr = const __attribute__((address_space(1))) T*;
 template <class T> using gmut = __attribute__((address_space(1))) T*;
 template <class T> __device__ __forceinline__ gptr<T> glb(const T* p) { return (gptr<T>)p; }
 template <class T> __device__ __forceinline__ gmut<T> glb(T* p) { return (gmut<T>)p; }
+// The per-date descriptor tables (flows, accruals, family descriptors, ZCB coefficients, set
+// entries) are read-only and indexed by wave-uniform values: read through the constant address
+// space they are scalar loads into SGPRs, served by the scalar cache after a unit's first tile.
+template <class T> using cptr = const __attribute__((address_space(4))) T*;
+template <class T> __device__ __forceinline__ cptr<T> cst(const T* p) { return (cptr<T>)p; }
 
 __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, const int chunk, double* smem) {
-    // a private copy of the descriptor: the kernel's own stores cannot alias it, so its fields stay
-    // in scalar registers instead of being reloaded after every global store
-    const ValuePlan g = plan;
+    // the descriptor read through the constant address space: no store of the kernel can alias it,
+    // so a field is a scalar load wherever it is used instead of a register held for the whole unit
+    const __attribute__((address_space(4))) ValuePlan& g = *cst(&plan);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -301,15 +304,6 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     const int CS = g.C;
     double* sPart = (double*)(sC + (size_t)g.C * kPathsPerTile);    // [S][waves][64] (split mode)
     double* sSeg = sPart + (g.set_mode == 0 ? (size_t)g.S * kValueWaves * kPathsPerTile : 0);  // [waves][64]
-    double* sLT = sSeg + (size_t)kValueWaves * kPathsPerTile;       // [Lmax] live-flow accruals
-    double2* sAB = (double2*)(sLT + ((g.Lmax + 1) & ~1));           // [D] (A, B) of P(t_k, T_d)
-    const int E_split = (g.set_mode == 0) ? g.n_set_entries : 0;
-    double2* sSAB = sAB + g.D;                                      // [E_split] set entries (alpha, beta)
-    int4* sLF = (int4*)(sSAB + E_split);                            // [Lmax] live flows
-    int4* sFD = sLF + g.Lmax;                                       // [NF] the date's family descriptors
-    int* sSCl = (int*)(sFD + g.NF);                                 // [E_split] set entries' classes
-    int* sZc = sSCl + E_split;                                      // [Zmax] matured classes
-    int2* sSet = (int2*)(sZc + g.Zmax + ((E_split + g.Zmax) & 1));   // [S_split] live entry range per set
 
     const double t = g.dates[k];
     const int dfirst = g.date_dfirst[k];
@@ -319,8 +313,8 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     const int det = g.date_det[k];
     const uint64_t n_paths = g.n_paths;
     const gptr<double> Rk = glb(g.R + (uint64_t)g.date_slot[k] * g.r_stride);
-    const double2* ab = g.AB + (size_t)k * g.D;
-    const int4* fdesc = g.fdesc + (size_t)k * g.NF;
+    const cptr<double> abd = cst((const double*)(g.AB + (size_t)k * g.D));  // (A, B) pairs
+    const cptr<int> fdc = cst((const int*)(g.fdesc + (size_t)k * g.NF));     // int4 per family
 
     // Per-trade work. The items alive at t_k are a suffix of the maturity-sorted list (every item
     // when per-path values are requested); each gets ns equal slices of a tile's paths, ns as large
@@ -360,29 +354,16 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
         }
     }
 
-    // the date's live flows and matured classes, staged once for all tiles of the unit
-    const int lb = g.lflow_off[k], n_lf = g.lflow_off[k + 1] - lb;
+    // the date's live flows (cash-flow date, class or -1, family) and accruals, this wave's segment
+    const int lb = g.lflow_off[k];
+    const cptr<int> lf = cst((const int*)(g.lflow + lb));  // int4 per flow
+    const cptr<double> lt = cst(g.ltau + lb);
     const int zb = g.zcls_off[k], n_zc = g.zcls_off[k + 1] - zb;
     const int4 segd = g.lseg[k * kValueWaves + wave];
-    const int seg_b = segd.x, seg_e = segd.y, carry_lo = segd.z, n_carry = segd.w;
+    const int seg_b = uni(segd.x), seg_e = uni(segd.y), carry_lo = uni(segd.z), n_carry = uni(segd.w);
     __syncthreads();  // the previous unit of a persistent block may still read this LDS
-    for (int i = tid; i < n_lf; i += kValueThreads) {
-        sLF[i] = g.lflow[lb + i];
-        sLT[i] = g.ltau[lb + i];
-    }
-    for (int i = tid; i < n_zc; i += kValueThreads) sZc[i] = g.zcls[zb + i];
-    for (int i = tid; i < g.NF; i += kValueThreads) sFD[i] = fdesc[i];
-    for (int i = dfirst + tid; i < g.D; i += kValueThreads) sAB[i] = ab[i];
-    for (int i = tid; i < E_split; i += kValueThreads) {
-        sSCl[i] = g.set_cls[i];
-        sSAB[i] = make_double2(g.set_alpha[i], g.set_beta[i]);
-    }
-    if (g.set_mode == 0)
-        for (int s = tid; s < g.S; s += kValueThreads)
-            sSet[s] = make_int2(g.set_off[s], g.set_off[s] + g.set_nlive[(size_t)k * g.S + s]);
-    __syncthreads();
     // classes already matured at t_k: zeroed once, no live flow writes them during this unit
-    for (int i = wave; i < n_zc; i += kValueWaves) sC[lane * CS + uni(sZc[i])] = make_double2(0.0, 0.0);
+    for (int i = wave; i < n_zc; i += kValueWaves) sC[lane * CS + cst(g.zcls)[zb + i]] = make_double2(0.0, 0.0);
 
     const uint64_t tile0 = (uint64_t)chunk * g.tiles_per_block;
     uint64_t tile_end = tile0 + g.tiles_per_block;
@@ -422,8 +403,7 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
             const double r = sR[lane];
             for (int dl = wave; dl < n_live; dl += kValueWaves) {
                 const int d = dfirst + dl;
-                const double2 v = make_double2(uni(sAB[d].x), uni(sAB[d].y));
-                sP[d * kPathsPerTile + lane] = v.x * exp(-(v.y * r));
+                sP[d * kPathsPerTile + lane] = abd[2 * d] * exp(-(abd[2 * d + 1] * r));
             }
         }
         __syncthreads();
@@ -440,7 +420,7 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
             double run = 0.0, st = 0.0;
             int fam = -1;
             auto family_start = [&](int f) {  // P(t, start) or the carried fixing of the period in progress
-                const int4 fd = uni(sFD[f]);
+                const int4 fd = make_int4(fdc[4 * f], 0, fdc[4 * f + 2], fdc[4 * f + 3]);
                 if (fd.x == 1) return 1.0;
                 if (fd.x == 2) return sP[fd.z * kPathsPerTile + lane];
                 return sP[fd.z * kPathsPerTile + lane] / sF[fd.w * kPathsPerTile + lane];
@@ -451,8 +431,8 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                 double tau[4], P[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    e[u] = uni(sLF[i + u]);
-                    tau[u] = uni(sLT[i + u]);
+                    e[u] = make_int4(lf[4 * (i + u)], lf[4 * (i + u) + 1], lf[4 * (i + u) + 2], 0);
+                    tau[u] = lt[i + u];
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) P[u] = sP[e[u].x * kPathsPerTile + lane];
@@ -468,14 +448,14 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                 }
             }
             for (; i < seg_e; ++i) {
-                const int4 e = uni(sLF[i]);
+                const int4 e = make_int4(lf[4 * i], lf[4 * i + 1], lf[4 * i + 2], 0);
                 if (e.z != fam) {
                     fam = e.z;
                     run = 0.0;
                     st = family_start(fam);
                 }
                 const double P = sP[e.x * kPathsPerTile + lane];
-                run += P * uni(sLT[i]);
+                run += P * lt[i];
                 if (e.y >= 0) sC[lane * CS + e.y] = make_double2(st - P, run);
             }
             sSeg[wave * kPathsPerTile + lane] = run;
@@ -488,7 +468,7 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                 int cls[4];
                 double a[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) cls[u] = (i + u < seg_b + n_carry) ? uni(sLF[i + u].y) : -1;
+                for (int u = 0; u < 4; ++u) cls[u] = (i + u < seg_b + n_carry) ? lf[4 * (i + u) + 1] : -1;
 #pragma unroll
                 for (int u = 0; u < 4; ++u) a[u] = (cls[u] >= 0) ? sC[lane * CS + cls[u]].y : 0.0;
 #pragma unroll
@@ -501,38 +481,35 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
         if (g.set_mode == 0) {
             for (int s = 0; s < g.S; ++s) {
                 // the set's classes alive at t (a prefix of its entries), split evenly over the waves
-                const int2 rng = uni2(sSet[s]);
-                const int per = (rng.y - rng.x + kValueWaves - 1) / kValueWaves;
-                const int cb = rng.x + wave * per;
-                const int ce = min(rng.y, cb + per);
+                const int beg = cst(g.set_off)[s];
+                const int end = beg + cst(g.set_nlive)[(size_t)k * g.S + s];
+                const int per = (end - beg + kValueWaves - 1) / kValueWaves;
+                const int cb = beg + wave * per;
+                const int ce = min(end, cb + per);
+                const cptr<int> scl = cst(g.set_cls);
+                const cptr<double> sal = cst(g.set_alpha), sbe = cst(g.set_beta);
                 double v = 0.0;
                 int e = cb;
                 for (; e + 4 <= ce; e += 4) {
-                    double2 c2[4], ab4[4];
-                    int cl[4];
+                    double2 c2[4];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) cl[u] = uni(sSCl[e + u]);
+                    for (int u = 0; u < 4; ++u) c2[u] = sC[lane * CS + scl[e + u]];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        ab4[u] = make_double2(uni(sSAB[e + u].x), uni(sSAB[e + u].y));
-                        c2[u] = sC[lane * CS + cl[u]];
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) v += ab4[u].x * c2[u].x - ab4[u].y * c2[u].y;
+                    for (int u = 0; u < 4; ++u) v += sal[e + u] * c2[u].x - sbe[e + u] * c2[u].y;
                 }
                 for (; e < ce; ++e) {
-                    const double2 c2 = sC[lane * CS + uni(sSCl[e])];
-                    v += uni(sSAB[e].x) * c2.x - uni(sSAB[e].y) * c2.y;
+                    const double2 c2 = sC[lane * CS + scl[e]];
+                    v += sal[e] * c2.x - sbe[e] * c2.y;
                 }
                 sPart[(s * kValueWaves + wave) * kPathsPerTile + lane] = v;
             }
         } else {
             for (int s = wave; s < g.S; s += kValueWaves) {
                 double v = 0.0;
-                const int beg = glb(g.set_off)[s], end = beg + glb(g.set_nlive)[(size_t)k * g.S + s];
+                const int beg = cst(g.set_off)[s], end = beg + cst(g.set_nlive)[(size_t)k * g.S + s];
                 for (int e = beg; e < end; ++e) {
-                    const double2 c2 = sC[lane * CS + glb(g.set_cls)[e]];
-                    v += glb(g.set_alpha)[e] * c2.x - glb(g.set_beta)[e] * c2.y;
+                    const double2 c2 = sC[lane * CS + cst(g.set_cls)[e]];
+                    v += cst(g.set_alpha)[e] * c2.x - cst(g.set_beta)[e] * c2.y;
                 }
                 if (valid) {
                     glb(g.X)[((uint64_t)s * g.n_dates + k) * n_paths + gp] = (v > 0.0) ? v : 0.0;

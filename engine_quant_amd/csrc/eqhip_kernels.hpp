@@ -38,9 +38,7 @@ struct ValuePlan {
     uint64_t n_tiles;         // ceil(n_paths / kPathsPerTile)
     int n_dates, D, C, NF, S, Fmax;
     int n_chunks, tiles_per_block;
-    int set_mode;             // 0: S <= kMaxSplitSets, class sums split across waves, set entries staged
-                              //    in LDS; 1: one wave per set
-    int n_set_entries;        // set_off[S]
+    int set_mode;             // 0: S <= kMaxSplitSets, class sums split across waves; 1: one wave per set
     const double* dates;      // [n_dates]
     const double2* AB;        // [n_dates][D] (A, B) of P(t_k, T_d)
     const int* date_slot;     // [n_dates]
@@ -54,7 +52,6 @@ struct ValuePlan {
     const int* flow_cls;      //               class whose last flow this is, or -1
     // per date: the live flows of every family in family order, split into kValueWaves segments
     // (one per wave), and the classes already matured
-    int Lmax, Zmax;           // largest live-flow / matured-class list over the dates
     const int* lflow_off;     // [n_dates + 1]
     const int4* lflow;        // (cash-flow date, class or -1, family, 0)
     const double* ltau;       // accrual of the flow
@@ -86,19 +83,11 @@ struct ValuePlan {
 };
 
 // Dynamic LDS of one value_kernel block: sR [64], sP [D][64], sF [Fmax][64], sC [64][C] (double2),
-// set partials [S_split][waves][64], segment totals [waves][64], then the date's live-flow accruals
-// [Lmax] (padded to even), the date's (A, B) of the cash-flow dates [D] (double2), the set entries'
-// (alpha, beta) [E_split] (double2), live flows [Lmax] (int4), family descriptors [NF] (int4), the
-// set entries' classes [E_split] and the matured classes [Zmax] (int), padded to 8 bytes, and the
-// date's live entry range of each set [S_split] (int2).
-// E_split: netting-set entries staged in LDS (all entries in split mode, none otherwise)
-__host__ __device__ inline size_t value_lds_bytes(int D, int Fmax, int C, int S_split, int Lmax, int Zmax, int NF,
-                                                  int E_split) {
-    const size_t lmax_even = (size_t)((Lmax + 1) & ~1);
+// set partials [S_split][waves][64] and segment totals [waves][64] (the per-date descriptor tables
+// are read from global memory through the scalar cache).
+__host__ __device__ inline size_t value_lds_bytes(int D, int Fmax, int C, int S_split) {
     return 8 * ((size_t)kPathsPerTile * (1 + D + Fmax) + 2 * (size_t)kPathsPerTile * C +
-                (size_t)kPathsPerTile * kValueWaves * S_split + (size_t)kPathsPerTile * kValueWaves + lmax_even) +
-           16 * ((size_t)D + E_split + Lmax + NF) + 4 * ((size_t)E_split + Zmax + ((E_split + Zmax) & 1)) +
-           8 * (size_t)S_split;
+                (size_t)kPathsPerTile * kValueWaves * S_split + (size_t)kPathsPerTile * kValueWaves);
 }
 
 // LDS that holds the per-slot trade sums of a unit for their reduction (floor of the allocation)
