@@ -350,7 +350,7 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
 #pragma unroll
         for (int q = 0; q < kTradesPerItem; ++q) {
             acc[i][q] = 0.0;
-            tK[i][q] = (q < icnt[i]) ? g.tr_K[ifirst[i] + q] : 0.0;
+            tK[i][q] = (q < icnt[i]) ? isgn[i] * g.tr_K[ifirst[i] + q] : 0.0;  // sign * K
         }
     }
 
@@ -463,7 +463,15 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
         __syncthreads();
         if (n_carry > 0) {  // the totals of the earlier segments of this segment's first family
             double carry = 0.0;
-            for (int w2 = carry_lo; w2 < wave; ++w2) carry += sSeg[w2 * kPathsPerTile + lane];
+            int w2 = carry_lo;
+            for (; w2 + 4 <= wave; w2 += 4) {  // four reads in flight, added in order
+                double t4[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) t4[u] = sSeg[(w2 + u) * kPathsPerTile + lane];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) carry += t4[u];
+            }
+            for (; w2 < wave; ++w2) carry += sSeg[w2 * kPathsPerTile + lane];
             for (int i = seg_b; i < seg_b + n_carry; i += 4) {  // reads of four classes, then the writes
                 int cls[4];
                 double a[4];
@@ -531,11 +539,10 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
 #pragma unroll 4
                 for (int p = pb; p < pe; ++p) {
                     const double2 c2 = col[p * CS];
-                    const double sfl = sg * c2.x;
-                    const double sann = sg * c2.y;
+                    const double sfl = sg * c2.x;  // sign * F - (sign * K) A, exact sign flips
 #pragma unroll
                     for (int q = 0; q < kTradesPerItem; ++q) {
-                        const double x = fma(-tK[i][q], sann, sfl);
+                        const double x = fma(-tK[i][q], c2.y, sfl);
                         acc[i][q] += (x > 0.0) ? x : 0.0;
                     }
                 }
