@@ -297,8 +297,7 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
-    double* sR = smem;                                     // [64]
-    double* sP = sR + kPathsPerTile;                       // [D][64]
+    double* sP = smem;                                     // [D][64]
     double* sF = sP + (size_t)g.D * kPathsPerTile;         // [Fmax][64]
     double2* sC = (double2*)(sF + (size_t)g.Fmax * kPathsPerTile);  // [64][C] path-major
     const int CS = g.C;
@@ -369,51 +368,52 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     uint64_t tile_end = tile0 + g.tiles_per_block;
     if (tile_end > g.n_tiles) tile_end = g.n_tiles;
 
-    // the short rates of the next tile are loaded one tile ahead (wave 0: r at t_k; wave f < nfix:
-    // r at fixing f), so that no tile starts by waiting on HBM
+    // The short rates of the next tile are loaded one tile ahead (every wave: r at t_k for its own
+    // lanes; wave f < nfix: r at fixing f), so that no tile starts by waiting on HBM.
     const gptr<double> Rfix = (wave < nfix) ? glb(g.R + (uint64_t)g.fix_slot[fbeg + wave] * g.r_stride) : nullptr;
     const double2 fab_w = (wave < nfix) ? g.fix_AB[fbeg + wave] : make_double2(0.0, 0.0);
     double r_next = 0.0, rf_next = 0.0;
     if (tile0 < tile_end) {
         const uint64_t gq = tile0 * kPathsPerTile + lane;
         if (gq < n_paths) {
-            if (wave == 0) r_next = Rk[gq];
+            r_next = Rk[gq];
             if (Rfix) rf_next = Rfix[gq];
         }
     }
+    // Per tile: A | B pass 1 | B pass 2 | C, D | set reduction, with a barrier after each of the
+    // first four. A of the next tile needs none before it: sP and sF were last read in pass 1.
     for (uint64_t tile = tile0; tile < tile_end; ++tile) {
         const uint64_t p0 = tile * kPathsPerTile;
         const uint64_t gp = p0 + lane;
         const bool valid = gp < n_paths;
-        __syncthreads();
-        if (wave == 0) sR[lane] = r_next;
-        for (int f = wave; f < nfix; f += kValueWaves) {
-            double rf = rf_next;
-            double2 fab = fab_w;
-            if (f != wave) {  // more fixings than waves
-                rf = valid ? glb(g.R)[(uint64_t)glb(g.fix_slot)[fbeg + f] * g.r_stride + gp] : 0.0;
-                const gptr<double> fp = glb((const double*)(g.fix_AB + fbeg + f));
-                fab = make_double2(fp[0], fp[1]);
-            }
-            sF[f * kPathsPerTile + lane] = fab.x * exp(-(fab.y * rf));
-        }
-        __syncthreads();
-        // A: zero-coupon bonds of the live cash-flow dates
+        // A: one list over the waves, the fixings of periods in progress (entry f < nfix, taken by
+        // wave f) and then the zero-coupon bonds of the live cash-flow dates
         {
-            const double r = sR[lane];
-            for (int dl = wave; dl < n_live; dl += kValueWaves) {
-                const int d = dfirst + dl;
-                sP[d * kPathsPerTile + lane] = abd[2 * d] * exp(-(abd[2 * d + 1] * r));
+            const double r = r_next;
+            for (int e = wave; e < nfix + n_live; e += kValueWaves) {
+                if (e < nfix) {
+                    double rf = rf_next;
+                    double2 fab = fab_w;
+                    if (e != wave) {  // more fixings than waves
+                        rf = valid ? glb(g.R)[(uint64_t)glb(g.fix_slot)[fbeg + e] * g.r_stride + gp] : 0.0;
+                        const gptr<double> fp = glb((const double*)(g.fix_AB + fbeg + e));
+                        fab = make_double2(fp[0], fp[1]);
+                    }
+                    sF[e * kPathsPerTile + lane] = fab.x * exp(-(fab.y * rf));
+                } else {
+                    const int d = dfirst + (e - nfix);
+                    sP[d * kPathsPerTile + lane] = abd[2 * d] * exp(-(abd[2 * d + 1] * r));
+                }
             }
         }
         __syncthreads();
         // B: schedule families. Every class's (floating leg, annuity) is a running sum over its
         // family's live flows; the date's flows are split into one contiguous segment per wave, each
         // summed from zero, and a second pass adds the totals of the earlier segments of the same family.
-        {  // next tile's short rates: in flight during the first pass, waited for at the next barrier
+        {  // next tile's short rates: in flight for the rest of this tile, used by the next A
             const uint64_t gq = gp + kPathsPerTile;
             const bool vq = (tile + 1 < tile_end) && gq < n_paths;
-            if (wave == 0) r_next = vq ? Rk[gq] : 0.0;
+            r_next = vq ? Rk[gq] : 0.0;
             if (Rfix) rf_next = vq ? Rfix[gq] : 0.0;
         }
         {

@@ -82,11 +82,11 @@ struct ValuePlan {
     double* path_mtm;         // optional [n_dates][n_trades][n_paths]
 };
 
-// Dynamic LDS of one value_kernel block: sR [64], sP [D][64], sF [Fmax][64], sC [64][C] (double2),
+// Dynamic LDS of one value_kernel block: sP [D][64], sF [Fmax][64], sC [64][C] (double2),
 // set partials [S_split][waves][64] and segment totals [waves][64] (the per-date descriptor tables
 // are read from global memory through the scalar cache).
 __host__ __device__ inline size_t value_lds_bytes(int D, int Fmax, int C, int S_split) {
-    return 8 * ((size_t)kPathsPerTile * (1 + D + Fmax) + 2 * (size_t)kPathsPerTile * C +
+    return 8 * ((size_t)kPathsPerTile * (D + Fmax) + 2 * (size_t)kPathsPerTile * C +
                 (size_t)kPathsPerTile * kValueWaves * S_split + (size_t)kPathsPerTile * kValueWaves);
 }
 
