@@ -298,7 +298,7 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
     double* sP = smem;                                     // [D][64]
-    double* sF = sP + (size_t)g.D * kPathsPerTile;         // [Fmax][64]
+    double* sF = sP + (size_t)g.D * kPathsPerTile;         // [Fmax][64] reciprocal fixing factors
     double2* sC = (double2*)(sF + (size_t)g.Fmax * kPathsPerTile);  // [64][C] path-major
     const int CS = g.C;
     double* sPart = (double*)(sC + (size_t)g.C * kPathsPerTile);    // [S][waves][64] (split mode)
@@ -371,7 +371,8 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     // The short rates of the next tile are loaded one tile ahead (every wave: r at t_k for its own
     // lanes; wave f < nfix: r at fixing f), so that no tile starts by waiting on HBM.
     const gptr<double> Rfix = (wave < nfix) ? glb(g.R + (uint64_t)g.fix_slot[fbeg + wave] * g.r_stride) : nullptr;
-    const double2 fab_w = (wave < nfix) ? g.fix_AB[fbeg + wave] : make_double2(0.0, 0.0);
+    const double2 fab_w = (wave < nfix) ? g.fix_AB[fbeg + wave] : make_double2(1.0, 0.0);
+    const double inv_a_w = 1.0 / fab_w.x;  // once per unit
     double r_next = 0.0, rf_next = 0.0;
     if (tile0 < tile_end) {
         const uint64_t gq = tile0 * kPathsPerTile + lane;
@@ -387,19 +388,20 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
         const uint64_t gp = p0 + lane;
         const bool valid = gp < n_paths;
         // A: one list over the waves, the fixings of periods in progress (entry f < nfix, taken by
-        // wave f) and then the zero-coupon bonds of the live cash-flow dates
+        // wave f; sF holds 1 / P(T_fix, T_next | r_fix) = exp(B r_fix) / A, so that a family start
+        // multiplies) and then the zero-coupon bonds of the live cash-flow dates
         {
             const double r = r_next;
             for (int e = wave; e < nfix + n_live; e += kValueWaves) {
                 if (e < nfix) {
-                    double rf = rf_next;
-                    double2 fab = fab_w;
+                    double rf = rf_next, inv_a = inv_a_w, b = fab_w.y;
                     if (e != wave) {  // more fixings than waves
                         rf = valid ? glb(g.R)[(uint64_t)glb(g.fix_slot)[fbeg + e] * g.r_stride + gp] : 0.0;
                         const gptr<double> fp = glb((const double*)(g.fix_AB + fbeg + e));
-                        fab = make_double2(fp[0], fp[1]);
+                        inv_a = 1.0 / fp[0];
+                        b = fp[1];
                     }
-                    sF[e * kPathsPerTile + lane] = fab.x * exp(-(fab.y * rf));
+                    sF[e * kPathsPerTile + lane] = inv_a * exp(b * rf);
                 } else {
                     const int d = dfirst + (e - nfix);
                     sP[d * kPathsPerTile + lane] = abd[2 * d] * exp(-(abd[2 * d + 1] * r));
@@ -423,7 +425,7 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                 const int4 fd = make_int4(fdc[4 * f], 0, fdc[4 * f + 2], fdc[4 * f + 3]);
                 if (fd.x == 1) return 1.0;
                 if (fd.x == 2) return sP[fd.z * kPathsPerTile + lane];
-                return sP[fd.z * kPathsPerTile + lane] / sF[fd.w * kPathsPerTile + lane];
+                return sP[fd.z * kPathsPerTile + lane] * sF[fd.w * kPathsPerTile + lane];
             };
             int i = seg_b;
             for (; i + 4 <= seg_e; i += 4) {  // all LDS reads of four flows first, then the chain
