@@ -211,6 +211,23 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
         trade_cls[j] = it->second;
     }
     P.C = (int)classes.size();
+    // classes numbered by maturity (then first appearance): the maturity-sorted trade items then
+    // run through consecutive class ids, and their sC columns through consecutive LDS banks
+    {
+        auto last_of = [&](int c) { return classes[c].pay.empty() ? classes[c].start : classes[c].pay.back(); };
+        std::vector<int> by_mat(P.C);
+        for (int c = 0; c < P.C; ++c) by_mat[c] = c;
+        std::stable_sort(by_mat.begin(), by_mat.end(), [&](int x, int y) { return last_of(x) < last_of(y); });
+        std::vector<int> new_id(P.C);
+        std::vector<ClassKey> sorted;
+        sorted.reserve(P.C);
+        for (int c = 0; c < P.C; ++c) {
+            new_id[by_mat[c]] = c;
+            sorted.push_back(classes[by_mat[c]]);
+        }
+        classes.swap(sorted);
+        for (int& c : trade_cls) c = new_id[c];
+    }
 
     // cash-flow date table
     std::vector<double> T;

@@ -318,7 +318,9 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     // Per-trade work. The items alive at t_k are a suffix of the maturity-sorted list (every item
     // when per-path values are requested); each gets ns equal slices of a tile's paths, ns as large
     // as the block's kTradeSlots work slots allow, so that the date's work is spread over every wave.
-    // Slot u = (live item u / ns, slice u % ns); thread tid holds slots tid + i * kValueThreads.
+    // Slot u = (slice u / n_li, live item u % n_li): the lanes of a wave read one path row of sC at
+    // consecutive items, whose class ids are consecutive (classes are numbered by maturity), so their
+    // reads fall in consecutive banks or share an address. Thread tid holds slots tid + i * kValueThreads.
     const int item0 = !g.want_trade ? g.n_items : (g.path_mtm ? 0 : g.item_first[k]);
     const int n_li = g.n_items - item0;
     const int ns = (n_li > 0) ? max(1, min(kPathsPerTile, kTradeSlots / n_li)) : 1;
@@ -331,8 +333,8 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
 #pragma unroll
     for (int i = 0; i < kItemsPerThread; ++i) {
         const int u = tid + i * kValueThreads;
-        const int li = u / ns;
-        const int sl = u - li * ns;
+        const int sl = (n_li > 0) ? u / n_li : 0;
+        const int li = u - sl * n_li;
         islice_b[i] = (sl * kPathsPerTile) / ns;
         islice_e[i] = ((sl + 1) * kPathsPerTile) / ns;
         icnt[i] = 0;
@@ -593,9 +595,9 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
             const int j = item.y + q;
             double e = 0.0;  // a matured class contributes exact zeros
             if (it >= item0) {
-                const double* a = sAcc + (it - item0) * ns * kTradesPerItem + q;
+                const double* a = sAcc + (it - item0) * kTradesPerItem + q;
                 double sum = 0.0;
-                for (int sl = 0; sl < ns; ++sl) sum += a[sl * kTradesPerItem];
+                for (int sl = 0; sl < ns; ++sl) sum += a[sl * n_li * kTradesPerItem];
                 e = g.tr_absN[j] * sum;
             }
             part[g.tr_orig[j]] = e;
