@@ -383,38 +383,40 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
             if (Rfix) rf_next = Rfix[gq];
         }
     }
-    // Per tile: A | B pass 1 | B pass 2 | C, D | set reduction, with a barrier after each of the
-    // first four. A of the next tile needs none before it: sP and sF were last read in pass 1.
+    // A: one list over the waves, the fixings of periods in progress (entry f < nfix, taken by wave f;
+    // sF holds 1 / P(T_fix, T_next | r_fix) = exp(B r_fix) / A, so that a family start multiplies) and
+    // then the zero-coupon bonds of the live cash-flow dates, for the tile starting at path pa.
+    auto stage_a = [&](uint64_t pa, double r, double rf_w) {
+        const uint64_t gpa = pa + lane;
+        for (int e = wave; e < nfix + n_live; e += kValueWaves) {
+            if (e < nfix) {
+                double rf = rf_w, inv_a = inv_a_w, b = fab_w.y;
+                if (e != wave) {  // more fixings than waves
+                    rf = (gpa < n_paths) ? glb(g.R)[(uint64_t)glb(g.fix_slot)[fbeg + e] * g.r_stride + gpa] : 0.0;
+                    const gptr<double> fp = glb((const double*)(g.fix_AB + fbeg + e));
+                    inv_a = 1.0 / fp[0];
+                    b = fp[1];
+                }
+                sF[e * kPathsPerTile + lane] = inv_a * exp(b * rf);
+            } else {
+                const int d = dfirst + (e - nfix);
+                sP[d * kPathsPerTile + lane] = abd[2 * d] * exp(-(abd[2 * d + 1] * r));
+            }
+        }
+    };
+    if (tile0 < tile_end) stage_a(tile0 * kPathsPerTile, r_next, rf_next);
+    __syncthreads();
+    // Per tile: B pass 1 | B pass 2 | C, D and A of the next tile | set reduction, with a barrier after
+    // each of the first three. A of the next tile can run beside C and D: it writes only sP and sF,
+    // which this tile last read in pass 1.
     for (uint64_t tile = tile0; tile < tile_end; ++tile) {
         const uint64_t p0 = tile * kPathsPerTile;
         const uint64_t gp = p0 + lane;
         const bool valid = gp < n_paths;
-        // A: one list over the waves, the fixings of periods in progress (entry f < nfix, taken by
-        // wave f; sF holds 1 / P(T_fix, T_next | r_fix) = exp(B r_fix) / A, so that a family start
-        // multiplies) and then the zero-coupon bonds of the live cash-flow dates
-        {
-            const double r = r_next;
-            for (int e = wave; e < nfix + n_live; e += kValueWaves) {
-                if (e < nfix) {
-                    double rf = rf_next, inv_a = inv_a_w, b = fab_w.y;
-                    if (e != wave) {  // more fixings than waves
-                        rf = valid ? glb(g.R)[(uint64_t)glb(g.fix_slot)[fbeg + e] * g.r_stride + gp] : 0.0;
-                        const gptr<double> fp = glb((const double*)(g.fix_AB + fbeg + e));
-                        inv_a = 1.0 / fp[0];
-                        b = fp[1];
-                    }
-                    sF[e * kPathsPerTile + lane] = inv_a * exp(b * rf);
-                } else {
-                    const int d = dfirst + (e - nfix);
-                    sP[d * kPathsPerTile + lane] = abd[2 * d] * exp(-(abd[2 * d + 1] * r));
-                }
-            }
-        }
-        __syncthreads();
         // B: schedule families. Every class's (floating leg, annuity) is a running sum over its
         // family's live flows; the date's flows are split into one contiguous segment per wave, each
         // summed from zero, and a second pass adds the totals of the earlier segments of the same family.
-        {  // next tile's short rates: in flight for the rest of this tile, used by the next A
+        {  // next tile's short rates: in flight during passes 1 and 2, used by the next tile's A
             const uint64_t gq = gp + kPathsPerTile;
             const bool vq = (tile + 1 < tile_end) && gq < n_paths;
             r_next = vq ? Rk[gq] : 0.0;
@@ -561,8 +563,9 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                 }
             }
         }
+        if (tile + 1 < tile_end) stage_a(p0 + kPathsPerTile, r_next, rf_next);
+        __syncthreads();
         if (g.set_mode == 0) {
-            __syncthreads();
             if (tid < g.S * kPathsPerTile) {
                 const int s = tid >> 6;
                 double v = 0.0;
