@@ -23,7 +23,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", 
              "-Xclang", "-packed-fp32-ops", "-mllvm", "-amdgpu-mfma-vgpr-form", "-Wall", "-Wno-unused-result"]
 
 EQHIP_SOURCES = ["eqhip_kernels.hip", "eqhip_api.hip", "eqhip_berm_kernels.hip", "eqhip_berm.hip"]
-HOST_SOURCES = ["host/params.cpp", "host/market.cpp", "host/model_product.cpp", "host/contexts.cpp",
+HOST_SOURCES = ["host/params.cpp", "host/market.cpp", "host/model_product.cpp", "host/contexts.cpp", "host/greeks.cpp",
                 "host/measures.cpp", "host/price.cpp"]
 PY_SOURCES = ["py/engine_py.cpp"]
 

@@ -97,6 +97,7 @@ private:
 
 MarketSnapshot bump_market_parallel(const MarketSnapshot& market, double bump);
 MarketSnapshot bump_market_pillar(const MarketSnapshot& market, std::size_t pillar_index, double bump);
+MarketSnapshot bump_market_credit(const MarketSnapshot& market, const std::string& name, double bump);
 
 // ---- model.hpp:41-58, product.hpp:49-69 -------------------------------------------------------
 class IModel {
@@ -289,6 +290,94 @@ struct Registries {
 };
 
 void register_builtins(Registries& registries);
+
+// ---- greeks.hpp:17-320 ------------------------------------------------------------------------
+// Sensitivities of any registered measure by bump-and-reval on the GPU measures (the Philox stream
+// is counter-based, so the bumped runs see the same shocks: common random numbers by construction),
+// and, for UnilateralCVA of a fixed-rate swap under HullWhite1F, pathwise by the CVA adjoint.
+namespace greeks {
+
+enum class RiskFactorKind { ModelParameter, CurveParallel, CurvePillar, CreditParameter, TimeShift };
+
+struct RiskFactor {
+    RiskFactorKind kind = RiskFactorKind::ModelParameter;
+    std::string scope;
+    std::string name;
+    std::optional<std::size_t> pillar_index;
+};
+
+RiskFactor parse_risk_factor(const std::string& text);
+std::string to_string(const RiskFactor& risk_factor);
+
+struct GreekOrder {
+    int order = 1;
+    std::optional<RiskFactor> cross_factor;
+};
+
+enum class GreekMethod { Auto, BumpAndReval, Pathwise, AadReverse, LikelihoodRatioHessian, AadForwardOverForward };
+
+GreekMethod parse_greek_method(const std::string& text);
+std::string to_string(GreekMethod method);
+
+struct GreekRequest {
+    std::string metric_name;
+    Params metric_params;
+    RiskFactor risk_factor;
+    GreekOrder order;
+    GreekMethod method = GreekMethod::Auto;
+    std::optional<double> bump_override;
+};
+
+struct GreekResult {
+    bool has_scalar = false;
+    double value = 0.0;
+    std::vector<double> times;
+    std::vector<double> primary;
+    std::vector<double> secondary;
+    GreekOrder order;
+    RiskFactor risk_factor;
+    GreekMethod method_used = GreekMethod::BumpAndReval;
+    std::optional<double> bump_used;
+    std::vector<std::string> warnings;
+};
+
+GreekResult compute_greek(const Registries& registries, const GreekRequest& request, const IModel& model,
+                          const IProduct& product, const MarketSnapshot& market, const PricingContext& pricing,
+                          const ExecutionContext& execution);
+
+struct GreeksReport {
+    std::vector<GreekResult> greeks;
+    std::vector<std::string> skipped;
+};
+
+GreeksReport compute_all_greeks(const Registries& registries, const std::string& metric_name,
+                                const Params& metric_params, const IModel& model, const IProduct& product,
+                                const MarketSnapshot& market, const PricingContext& pricing,
+                                const ExecutionContext& execution, bool include_curve_buckets = false,
+                                bool include_second_order = false);
+
+}  // namespace greeks
+
+struct CvaAdjoint {
+    double cva = 0.0;
+    double grad[6] = {0, 0, 0, 0, 0, 0};  // d/d{a, b, sigma, r0, hazard_rate, recovery_rate}
+};
+CvaAdjoint unilateral_cva_adjoint(const IModel& model, const IProduct& product, const MarketSnapshot& market,
+                                  const PricingContext& pricing, const ExecutionContext& execution);
+
+// The "Greek" measure (greeks.hpp:304-318): a thin wrapper over compute_greek; the inner metric's
+// own parameters arrive as "metric.<key>".
+class GreekMeasure : public IMeasure {
+public:
+    GreekMeasure(const Params& params, const Registries& registries);
+    std::string type_name() const override { return "Greek"; }
+    MeasureResult evaluate(const IModel& model, const IProduct& product, const MarketSnapshot& market,
+                           const PricingContext& pricing, const ExecutionContext& execution) const override;
+
+private:
+    greeks::GreekRequest request_;
+    const Registries& registries_;
+};
 
 // ---- price.hpp --------------------------------------------------------------------------------
 struct PriceResultEntry {

@@ -285,6 +285,40 @@ MeasureResult Dv01Measure::evaluate(const IModel&, const IProduct& product, cons
     return r;
 }
 
+// UnilateralCVA of one fixed-rate swap as a singleton netting set on the measure's grid (reset-date
+// monitoring, the pricing context's steps / paths / seed), with its pathwise derivatives from the
+// GPU adjoint (eqhip_cva_adjoint): grad = dCVA / d{a, b, sigma, r0, hazard_rate, recovery_rate}.
+CvaAdjoint unilateral_cva_adjoint(const IModel& model, const IProduct& product, const MarketSnapshot& market,
+                                  const PricingContext& pricing, const ExecutionContext& execution) {
+    require_gpu(execution);
+    const auto& hw1 = require_hw1f(model, "UnilateralCVA (pathwise)");
+    const auto& irs = require_irs(product, "UnilateralCVA (pathwise)");
+    if (irs.use_par_rate())
+        throw std::invalid_argument(
+            "compute_greek: metodo 'pathwise' de 'UnilateralCVA' requiere un fixed_rate explicito: con tasa par la "
+            "tasa fija depende de los parametros del modelo y el adjunto la mantiene fija (use method='auto' o "
+            "'bump_and_reval')");
+    const std::vector<double> dates = reset_dates(irs);
+    const eqhip_model_hw1f m{hw1.a(), hw1.b(), hw1.sigma(), hw1.r0()};
+    const int32_t set_id = 0;
+    const double notional = irs.notional(), k = irs.fixed_rate(), start = irs.start();
+    const int32_t off[2] = {0, (int32_t)irs.payment_times().size()};
+    const eqhip_portfolio pf{1, 1, &set_id, &notional, &k, &start, off, irs.payment_times().data(),
+                             irs.accruals().data()};
+    const eqhip_mc mc{pricing.n_paths(), (uint32_t)pricing.n_steps(), pricing.seed(), EQHIP_RNG_PHILOX,
+                      disc_of(execution)};
+    eqhip_plan* plan = nullptr;
+    check(eqhip_plan_create(gpu_context(), &m, &pf, dates.data(), (int32_t)dates.size(), &mc, 0, &plan));
+    struct Guard {
+        eqhip_plan* p;
+        ~Guard() { eqhip_plan_destroy(p); }
+    } guard{plan};
+    const eqhip_credit credit{market.hazard_rate(), market.recovery_rate()};
+    CvaAdjoint out{};
+    check(eqhip_cva_adjoint(plan, nullptr, 0, &credit, &out.cva, out.grad, nullptr));
+    return out;
+}
+
 void register_builtins(Registries& registries) {
     registries.models.register_type<HullWhite1FModel>("HullWhite1F");
     registries.products.register_type<IrSwapProduct>("IRSwap");
@@ -292,6 +326,10 @@ void register_builtins(Registries& registries) {
     registries.measures.register_type<UnilateralCvaMeasure>("UnilateralCVA");
     registries.measures.register_type<PresentValueMeasure>("PV");
     registries.measures.register_type<Dv01Measure>("DV01");
+    // the Greek measure resolves its inner metric by name, so it keeps the registries (bootstrap.cpp:33-39)
+    const Registries* regs = &registries;
+    registries.measures.register_factory(
+        "Greek", [regs](const Params& params) { return std::make_unique<GreekMeasure>(params, *regs); });
 }
 
 }  // namespace engine

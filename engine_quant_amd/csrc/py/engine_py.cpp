@@ -104,6 +104,18 @@ public:
         return out;
     }
 
+    // engine_py_ext.cpp:298-312: the automatic Greek sweep of one metric
+    engine::greeks::GreeksReport all_greeks(const engine::IProduct& product, const std::string& metric_name,
+                                            const engine::IModel& model, const engine::MarketSnapshot& market,
+                                            const engine::PricingContext& pricing,
+                                            const engine::ExecutionContext& execution, const py::dict& metric_params,
+                                            bool include_curve_buckets, bool include_second_order) const {
+        engine::Params mp = dict_to_params(metric_params);
+        py::gil_scoped_release release;
+        return engine::greeks::compute_all_greeks(registries_, metric_name, mp, model, product, market, pricing,
+                                                  execution, include_curve_buckets, include_second_order);
+    }
+
 private:
     engine::Registries registries_;
 };
@@ -157,6 +169,31 @@ PYBIND11_MODULE(_engine, m) {
         .def_property_readonly("rng", &engine::ExecutionContext::rng)
         .def_property_readonly("discretisation", &engine::ExecutionContext::discretisation);
 
+    py::class_<engine::greeks::GreekResult>(m, "GreekResult")
+        .def_readonly("has_scalar", &engine::greeks::GreekResult::has_scalar)
+        .def_readonly("value", &engine::greeks::GreekResult::value)
+        .def_readonly("times", &engine::greeks::GreekResult::times)
+        .def_readonly("primary", &engine::greeks::GreekResult::primary)
+        .def_readonly("secondary", &engine::greeks::GreekResult::secondary)
+        .def_readonly("bump_used", &engine::greeks::GreekResult::bump_used)
+        .def_readonly("warnings", &engine::greeks::GreekResult::warnings)
+        .def_property_readonly("order", [](const engine::greeks::GreekResult& g) { return g.order.order; })
+        .def_property_readonly("risk_factor",
+                               [](const engine::greeks::GreekResult& g) { return engine::greeks::to_string(g.risk_factor); })
+        .def_property_readonly("method_used",
+                               [](const engine::greeks::GreekResult& g) { return engine::greeks::to_string(g.method_used); })
+        .def("__repr__", [](const engine::greeks::GreekResult& g) {
+            return "<GreekResult " + engine::greeks::to_string(g.risk_factor) + " order=" + std::to_string(g.order.order) +
+                   " method=" + engine::greeks::to_string(g.method_used) + ">";
+        });
+    py::class_<engine::greeks::GreeksReport>(m, "GreeksReport")
+        .def_readonly("greeks", &engine::greeks::GreeksReport::greeks)
+        .def_readonly("skipped", &engine::greeks::GreeksReport::skipped)
+        .def("__repr__", [](const engine::greeks::GreeksReport& r) {
+            return "<GreeksReport greeks=" + std::to_string(r.greeks.size()) + " skipped=" + std::to_string(r.skipped.size()) +
+                   ">";
+        });
+
     py::class_<Engine>(m, "Engine")
         .def(py::init<>())
         .def("list_models", &Engine::list_models)
@@ -181,7 +218,10 @@ PYBIND11_MODULE(_engine, m) {
                  return e.batch(true, products, measures, model, market, pricing, execution);
              },
              py::arg("products"), py::arg("measures"), py::arg("model"), py::arg("market"), py::arg("pricing"),
-             py::arg("execution"));
+             py::arg("execution"))
+        .def("all_greeks", &Engine::all_greeks, py::arg("product"), py::arg("metric_name"), py::arg("model"),
+             py::arg("market"), py::arg("pricing"), py::arg("execution"), py::arg("metric_params") = py::dict(),
+             py::arg("include_curve_buckets") = false, py::arg("include_second_order") = false);
 
     py::register_exception_translator([](std::exception_ptr p) {
         try {

@@ -45,7 +45,7 @@ def test_host_contexts_validate_like_the_reference():
         eng.create_model("NoExiste", {})
     with pytest.raises(ValueError):
         eng.create_product("IRSwap", {"notional": 1.0, "payment_times": [1.0, 2.0], "accruals": [1.0]})
-    assert sorted(eng.list_measures()) == ["DV01", "ExpectedExposure", "ExposureProfile", "PFE95", "PV",
+    assert sorted(eng.list_measures()) == ["DV01", "ExpectedExposure", "ExposureProfile", "Greek", "PFE95", "PV",
                                            "UnilateralCVA"]
 
 
@@ -72,3 +72,39 @@ def test_synthetic_portfolio_generator_shape():
     assert len(monthly_dates()) == 360
     sets = batched_sets(4, seed=1)
     assert all(10 <= p.n_trades <= 2000 and 10_000 <= n <= 200_000 for p, n in sets)
+
+
+def test_greek_measure_validates_like_the_reference():
+    """greeks.cpp:954-1010, 1062-1120: malformed risk factors, orders and methods are refused before
+    anything runs."""
+    eng = engine.Engine()
+    model = eng.create_model("HullWhite1F", {"a": 0.1, "b": 0.03, "sigma": 0.01, "r0": 0.02})
+    product = eng.create_product("IRSwap", {"notional": 1e6, "fixed_rate": 0.02, "payment_times": [1.0, 2.0],
+                                            "accruals": [1.0, 1.0]})
+    market = engine.MarketSnapshot(pillars=[1.0], zero_rates=[0.02], hazard_rate=0.02, recovery_rate=0.4)
+    pricing = engine.PricingContext({"n_paths": 10.0, "n_steps": 10.0, "seed": 1.0})
+    gpu = engine.ExecutionContext({"backend": "gpu"})
+
+    def price(**params):
+        return eng.price(product, [("Greek", params)], model, market, pricing, gpu)
+
+    for rf, msg in (("a", "formato"), ("model.", "requiere"), ("curve.pillar:-1", "entero"),
+                    ("credit.spread", "desconocido"), ("time.rho", "desconocido"), ("fx.spot", "scope")):
+        with pytest.raises(ValueError, match=msg):
+            price(metric="UnilateralCVA", risk_factor=rf)
+    with pytest.raises(ValueError, match="order debe ser 1 o 2"):
+        price(metric="UnilateralCVA", risk_factor="model.a", order=3.0)
+    with pytest.raises(ValueError, match="tercera derivada"):
+        price(metric="UnilateralCVA", risk_factor="model.a", order=2.0, cross_factor="model.b")
+    with pytest.raises(ValueError, match="time.theta"):
+        price(metric="UnilateralCVA", risk_factor="time.theta")
+    with pytest.raises(ValueError, match="ModelParameter"):
+        price(metric="UnilateralCVA", risk_factor="credit.hazard_rate", method="pathwise")
+    with pytest.raises(ValueError, match="GreekMethod"):
+        price(metric="UnilateralCVA", risk_factor="model.a", method="finite")
+    with pytest.raises(ValueError, match="no disponible"):
+        price(metric="UnilateralCVA", risk_factor="model.a", method="aad")
+    with pytest.raises(ValueError, match="no tiene el parametro"):
+        price(metric="UnilateralCVA", risk_factor="model.kappa")
+    with pytest.raises(ValueError, match="fuera de rango"):
+        price(metric="UnilateralCVA", risk_factor="curve.pillar:3")
