@@ -51,6 +51,40 @@ double host_par_rate(double a, double b, double sigma, double r, double start, c
     return num / den;
 }
 
+// Host twin of the G2++ bond at the initial state (0, 0) (models/hull_white_2f.rs:66-106), for the
+// par rate of use_par_rate swaps (api.rs build_irs_swap_2f, irs.rs:126-145).
+double host_hw2_var(double z, double vol, double tau) {
+    const double inv_z = std::pow(z, -1.0);
+    const double e1 = std::exp(z * (-tau));
+    const double e2 = std::exp(z * (-2.0 * tau));
+    const double bracket = (((inv_z * 2.0) * e1 - (inv_z * 0.5) * e2) - inv_z * 1.5) + tau;
+    return ((vol * vol) / (z * z)) * bracket;
+}
+double host_zcb2_at_origin(const eqhip_model_hw2f& m, double tau) {
+    const double apb = m.a + m.b;
+    const double ta = (std::exp(m.a * (-tau)) + (-1.0)) / m.a;
+    const double tb = (std::exp(m.b * (-tau)) + (-1.0)) / m.b;
+    const double tab = (std::exp(apb * (-tau)) + (-1.0)) / apb;
+    const double cross = (((m.sigma * m.eta) * (2.0 * m.rho)) / (m.a * m.b)) * (((ta + tau) + tb) - tab);
+    const double v = (host_hw2_var(m.a, m.sigma, tau) + host_hw2_var(m.b, m.eta, tau)) + cross;
+    return std::exp(m.r0 * (-tau) + v * 0.5);
+}
+double host_par_rate2(const eqhip_model_hw2f& m, double start, const double* pay, const double* acc, int n) {
+    const double num = host_zcb2_at_origin(m, start - start) - host_zcb2_at_origin(m, pay[n - 1] - start);
+    double den = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double leg = host_zcb2_at_origin(m, pay[i] - start) * acc[i];
+        den = (i == 0) ? leg : den + leg;
+    }
+    return num / den;
+}
+
+void validate_hw2f(const eqhip_model_hw2f& m) {
+    if (!(m.a != 0.0) || !(m.b != 0.0) || !(m.a + m.b != 0.0))
+        raise(EQHIP_E_INVALID, "HullWhite2F: a, b and a + b must be non-zero");
+    if (!(std::fabs(m.rho) <= 1.0)) raise(EQHIP_E_INVALID, "HullWhite2F: rho must lie in [-1, 1]");
+}
+
 bool is_reset_date(double start, const double* pay, int m, double t) {
     if (std::fabs(t - start) < kDateTolHost) return true;
     for (int i = 0; i + 1 < m; ++i)
@@ -64,6 +98,8 @@ struct eqhip_plan {
     eqhip_context* ctx = nullptr;
     int device = 0;  // kept so that destroying the plan never touches its context
     eqhip_model_hw1f model{};
+    int factors = 1;           // 2: G2++ plan (model2; state (x, y) in R, R2)
+    eqhip_model_hw2f model2{};
     eqhip_mc mc{};
     int n_dates = 0, n_trades = 0, S = 0, C = 0, NF = 0, D = 0, Fmax = 0, n_slots = 0;
     bool det = false;
@@ -83,7 +119,7 @@ struct eqhip_plan {
     DevBuf slot_of_step, R, Z, AB, date_slot, date_dfirst, date_det, fdesc, fam_off, flow_d, flow_tau,
         flow_cls, fix_off, fix_slot, fix_AB, set_off, set_nlive, set_cls, set_alpha, set_beta, X, items, cls_mat, tr_K,
         tr_absN, tr_N, tr_NK, tr_orig, item_first, partial, trade_ee, trade_cva, set_ee, set_pfe, set_cva, neff, qrank,
-        d_dates, vplan, units, rows;
+        d_dates, vplan, units, rows, R2, AB2, fix_B2;
     ValuePlan vp{};  // host copy of the device descriptor (per-run pointers patched at launch)
     PinnedBuf h_out; // page-locked staging of the outputs
     DevBuf cva_w;    // per-date CVA factors
@@ -164,8 +200,14 @@ void build_value_plan(eqhip_plan& P);
 // simulation; the batch then owns the short-rate buffer.
 void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfolio* pf,
                 const double* dates, int n_dates, const eqhip_mc* mc, int want_trade, bool all_steps = false) {
-    if (!model || !mc) raise(EQHIP_E_INVALID, "model / mc is null");
-    if (!(model->a != 0.0)) raise(EQHIP_E_INVALID, "HullWhite1F: a must be non-zero");
+    // a two-factor plan arrives with factors = 2 and model2 set, and no one-factor model
+    if (!mc) raise(EQHIP_E_INVALID, "mc is null");
+    if (P.factors == 2) {
+        validate_hw2f(P.model2);
+    } else {
+        if (!model) raise(EQHIP_E_INVALID, "model is null");
+        if (!(model->a != 0.0)) raise(EQHIP_E_INVALID, "HullWhite1F: a must be non-zero");
+    }
     validate_portfolio(pf);
     if (n_dates < 1 || !dates) raise(EQHIP_E_INVALID, "monitoring_times vacío: at least one date is required");
     if (mc->n_paths < 1) raise(EQHIP_E_INVALID, "PricingContext: n_paths must be > 0");
@@ -174,7 +216,7 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
     if (mc->disc != EQHIP_DISC_EULER && mc->disc != EQHIP_DISC_EXACT) raise(EQHIP_E_INVALID, "unknown discretisation");
     if (mc->n_paths > (uint64_t)1 << 40) raise(EQHIP_E_CAPACITY, "n_paths too large");
 
-    P.model = *model;
+    if (P.factors != 2) P.model = *model;
     P.mc = *mc;
     P.n_dates = n_dates;
     P.dates.assign(dates, dates + n_dates);
@@ -299,7 +341,10 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
         const double t = P.dates[k];
         const int64_t g = grid_of(t);
         date_slot[k] = slot(g);
-        date_det[k] = (g == 0) ? 1 : 0;
+        // grid index 0 has a single row under HW1F (exposure.rs:118-126: path[0] is r0 alone) but
+        // n_paths equal rows under G2++ (exposure.rs:346-352: path[0] is n_paths zero states),
+        // except when every date is <= 0 (one deterministic value, exposure.rs:325-337)
+        date_det[k] = (g == 0 && (P.factors == 1 || P.det)) ? 1 : 0;
         int dfirst = (int)(std::upper_bound(T.begin(), T.end(), t + kDateTolHost) - T.begin());
         date_dfirst[k] = dfirst;
         for (int d = dfirst; d < P.D; ++d) ab_tau[(size_t)k * P.D + d] = T[d] - t;
@@ -484,9 +529,10 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
     // memory check for the large buffers
     size_t free_b = 0, total_b = 0;
     HIPCHK(hipMemGetInfo(&free_b, &total_b));
-    double need = 8.0 * ((all_steps ? 0.0 : (double)P.n_slots * mc->n_paths) + (double)P.S * n_dates * mc->n_paths +
+    double need = 8.0 * ((all_steps ? 0.0 : (double)P.factors * P.n_slots * mc->n_paths) +
+                         (double)P.S * n_dates * mc->n_paths +
                          (P.want_trade ? (double)P.n_trades * n_dates * P.n_chunks : 0.0) +
-                         (mc->rng == EQHIP_RNG_HOST_DRAWS ? (double)mc->n_steps * mc->n_paths : 0.0));
+                         (mc->rng == EQHIP_RNG_HOST_DRAWS ? (double)P.factors * mc->n_steps * mc->n_paths : 0.0));
     if (need > 0.9 * (double)free_b)
         raise(EQHIP_E_CAPACITY, "exposure plan needs " + std::to_string(need / 1e9) + " GB of device memory, " +
                                     std::to_string(free_b / 1e9) + " GB free");
@@ -536,26 +582,40 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
         upload(tau, ab_tau, st);
         P.AB.alloc(ab_tau.size() * sizeof(double2));
         int n = (int)ab_tau.size();
-        hipLaunchKernelGGL(ab_kernel, dim3((n + 255) / 256), dim3(256), 0, st, tau.as<double>(), n,
-                           P.model.a, P.model.b, P.model.sigma, P.AB.as<double2>());
+        const eqhip_model_hw2f& m2 = P.model2;
+        if (P.factors == 2) {  // (c, B(a,tau)) and B(b,tau)
+            P.AB2.alloc(ab_tau.size() * sizeof(double));
+            hipLaunchKernelGGL(ab2_kernel, dim3((n + 255) / 256), dim3(256), 0, st, tau.as<double>(), n, m2.a, m2.b,
+                               m2.sigma, m2.eta, m2.rho, m2.r0, P.AB.as<double2>(), P.AB2.as<double>());
+        } else {
+            hipLaunchKernelGGL(ab_kernel, dim3((n + 255) / 256), dim3(256), 0, st, tau.as<double>(), n,
+                               P.model.a, P.model.b, P.model.sigma, P.AB.as<double2>());
+        }
         HIPCHK(hipGetLastError());
         DevBuf ftau;
         upload(ftau, fix_tau, st);
         P.fix_AB.alloc(std::max<size_t>(1, fix_tau.size()) * sizeof(double2));
+        if (P.factors == 2) P.fix_B2.alloc(std::max<size_t>(1, fix_tau.size()) * sizeof(double));
         int nf = (int)fix_tau.size();
         if (nf > 0) {
-            hipLaunchKernelGGL(ab_kernel, dim3((nf + 255) / 256), dim3(256), 0, st, ftau.as<double>(), nf,
-                               P.model.a, P.model.b, P.model.sigma, P.fix_AB.as<double2>());
+            if (P.factors == 2)
+                hipLaunchKernelGGL(ab2_kernel, dim3((nf + 255) / 256), dim3(256), 0, st, ftau.as<double>(), nf, m2.a,
+                                   m2.b, m2.sigma, m2.eta, m2.rho, m2.r0, P.fix_AB.as<double2>(),
+                                   P.fix_B2.as<double>());
+            else
+                hipLaunchKernelGGL(ab_kernel, dim3((nf + 255) / 256), dim3(256), 0, st, ftau.as<double>(), nf,
+                                   P.model.a, P.model.b, P.model.sigma, P.fix_AB.as<double2>());
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipStreamSynchronize(st));  // tau buffers die here
     }
     if (!all_steps) P.R.alloc((size_t)P.n_slots * mc->n_paths * 8);
+    if (P.factors == 2) P.R2.alloc((size_t)P.n_slots * mc->n_paths * 8);
     P.X.alloc((size_t)P.S * n_dates * mc->n_paths * 8);
     P.set_ee.alloc((size_t)P.S * n_dates * 8);
     P.set_pfe.alloc((size_t)P.S * n_dates * 8);
     P.set_cva.alloc((size_t)P.S * 8);
-    if (mc->rng == EQHIP_RNG_HOST_DRAWS) P.Z.alloc((size_t)mc->n_steps * mc->n_paths * 8);
+    if (mc->rng == EQHIP_RNG_HOST_DRAWS) P.Z.alloc((size_t)P.factors * mc->n_steps * mc->n_paths * 8);
     P.h_T = T;
     P.h_fdesc = fdesc;
     P.h_fam_off = fam_off;
@@ -597,6 +657,10 @@ void build_value_plan(eqhip_plan& P) {
     g.n_chunks = P.n_chunks;
     g.tiles_per_block = P.tiles_per_block;
     g.set_mode = P.set_mode;
+    g.two_factor = (P.factors == 2) ? 1 : 0;
+    g.R2 = P.R2.as<double>();
+    g.AB2 = P.AB2.as<double>();
+    g.fix_B2 = P.fix_B2.as<double>();
     g.dates = P.d_dates.as<double>();
     g.AB = P.AB.as<double2>();
     g.date_slot = P.date_slot.as<int>();
@@ -661,8 +725,12 @@ void launch_value(eqhip_plan& P, double* path_mtm, double* set_mtm) {
         P.vp.set_mtm = set_mtm;
         HIPCHK(hipMemcpy(P.vplan.p, &P.vp, sizeof(ValuePlan), hipMemcpyHostToDevice));
     }
-    hipLaunchKernelGGL(value_kernel, dim3((unsigned)P.unit_list.size()), dim3(kValueThreads), P.lds_bytes, st,
-                       P.vplan.as<const ValuePlan>(), P.units.as<const int4>());
+    if (P.factors == 2)
+        hipLaunchKernelGGL(value2_kernel, dim3((unsigned)P.unit_list.size()), dim3(kValueThreads), P.lds_bytes, st,
+                           P.vplan.as<const ValuePlan>(), P.units.as<const int4>());
+    else
+        hipLaunchKernelGGL(value_kernel, dim3((unsigned)P.unit_list.size()), dim3(kValueThreads), P.lds_bytes, st,
+                           P.vplan.as<const ValuePlan>(), P.units.as<const int4>());
     HIPCHK(hipGetLastError());
 }
 
@@ -674,10 +742,57 @@ void launch_cva_weights(hipStream_t st, const double* d_times, int n_dates, cons
                        m.sigma, m.r0, hazard, w);
     HIPCHK(hipGetLastError());
 }
+void launch_cva_weights2(hipStream_t st, const double* d_times, int n_dates, const eqhip_model_hw2f& m, double hazard,
+                         double2* w) {
+    hipLaunchKernelGGL(cva_weights2_kernel, dim3((n_dates + 127) / 128), dim3(128), 0, st, d_times, n_dates, m.a, m.b,
+                       m.sigma, m.eta, m.rho, m.r0, hazard, w);
+    HIPCHK(hipGetLastError());
+}
 void launch_cva_sum(hipStream_t st, const double* ee, int n_profiles, int n_dates, const double2* w, double recovery,
                     double* out) {
     hipLaunchKernelGGL(cva_kernel, dim3((n_profiles + 63) / 64), dim3(64), 0, st, ee, n_profiles, n_dates, w, recovery,
                        out);
+    HIPCHK(hipGetLastError());
+}
+
+// G2++ paths of a plan into (R, R2). Euler follows hull_white_2f.rs:116-140; the exact scheme
+// draws the pair's Gaussian transition over dt (correlation c of the two increments).
+void launch_simulate2(eqhip_plan& P) {
+    const eqhip_model_hw2f& m = P.model2;
+    const double dt = P.dt;
+    Sim2Args s{};
+    s.a = m.a;
+    s.b = m.b;
+    s.sigma = m.sigma;
+    s.eta = m.eta;
+    s.rho = m.rho;
+    s.dt = dt;
+    s.sqrt_dt = std::sqrt(dt);
+    s.sqrt_1mrho2 = std::sqrt(1.0 - m.rho * m.rho);
+    s.ex = std::exp(-(m.a * dt));
+    s.ey = std::exp(-(m.b * dt));
+    s.sdx = m.sigma * std::sqrt((1.0 - std::exp(-(2.0 * m.a * dt))) / (2.0 * m.a));
+    s.sdy = m.eta * std::sqrt((1.0 - std::exp(-(2.0 * m.b * dt))) / (2.0 * m.b));
+    const double cov = m.rho * m.sigma * m.eta * (1.0 - std::exp(-((m.a + m.b) * dt))) / (m.a + m.b);
+    double c = (s.sdx > 0.0 && s.sdy > 0.0) ? cov / (s.sdx * s.sdy) : 0.0;
+    c = std::min(1.0, std::max(-1.0, c));
+    s.c_xy = c;
+    s.sqrt_1mc2 = std::sqrt(1.0 - c * c);
+    s.n_paths = P.mc.n_paths;
+    s.n_steps = P.det ? 0u : P.mc.n_steps;
+    s.seed = P.mc.seed;
+    s.slot_of_step = P.slot_of_step.as<int>();
+    s.Z = P.Z.as<double>();
+    s.X = P.R.as<double>();
+    s.Y = P.R2.as<double>();
+    dim3 grid((unsigned)((P.mc.n_paths + 255) / 256));
+    hipStream_t st = P.ctx->stream;
+    switch (P.mc.disc * 2 + P.mc.rng) {
+        case 0: hipLaunchKernelGGL((simulate2_kernel<0, 0>), grid, dim3(256), 0, st, s); break;
+        case 1: hipLaunchKernelGGL((simulate2_kernel<0, 1>), grid, dim3(256), 0, st, s); break;
+        case 2: hipLaunchKernelGGL((simulate2_kernel<1, 0>), grid, dim3(256), 0, st, s); break;
+        default: hipLaunchKernelGGL((simulate2_kernel<1, 1>), grid, dim3(256), 0, st, s); break;
+    }
     HIPCHK(hipGetLastError());
 }
 
@@ -692,7 +807,8 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
     const uint64_t np = P.mc.n_paths;
     if (P.mc.rng == EQHIP_RNG_HOST_DRAWS) {
         if (!host_draws) raise(EQHIP_E_INVALID, "host_draws required in EQHIP_RNG_HOST_DRAWS mode");
-        HIPCHK(hipMemcpyAsync(P.Z.p, host_draws, (size_t)P.mc.n_steps * np * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(P.Z.p, host_draws, (size_t)P.factors * P.mc.n_steps * np * 8, hipMemcpyHostToDevice,
+                              st));
     }
     DevBuf dpath, dset;
     if (out->path_mtm) {
@@ -702,6 +818,9 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
     if (out->set_mtm) dset.alloc((size_t)P.n_dates * P.S * np * 8);
 
     HIPCHK(hipEventRecord(P.ev[0], st));
+    if (P.factors == 2) {
+        launch_simulate2(P);
+    } else {
     SimArgs s{};
     s.a = P.model.a;
     s.b = P.model.b;
@@ -727,6 +846,7 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
         default: hipLaunchKernelGGL((simulate_kernel<1, 1>), sgrid, dim3(256), 0, st, s); break;
     }
     HIPCHK(hipGetLastError());
+    }
     HIPCHK(hipEventRecord(P.ev[1], st));
     launch_value(P, dpath.as<double>(), dset.as<double>());
     HIPCHK(hipEventRecord(P.ev[2], st));
@@ -743,7 +863,10 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
     double h = credit ? credit->hazard_rate : 0.0, rr = credit ? credit->recovery_rate : 0.0;
     if (out->set_cva || out->trade_cva) {
         reserve(P.cva_w, (size_t)P.n_dates * 16);
-        launch_cva_weights(st, P.d_dates.as<double>(), P.n_dates, P.model, h, P.cva_w.as<double2>());
+        if (P.factors == 2)
+            launch_cva_weights2(st, P.d_dates.as<double>(), P.n_dates, P.model2, h, P.cva_w.as<double2>());
+        else
+            launch_cva_weights(st, P.d_dates.as<double>(), P.n_dates, P.model, h, P.cva_w.as<double2>());
     }
     if (out->set_cva)
         launch_cva_sum(st, P.set_ee.as<double>(), P.S, P.n_dates, P.cva_w.as<double2>(), rr, P.set_cva.as<double>());
@@ -806,7 +929,7 @@ void profile_batch(eqhip_context* ctx, double a, double b, double sigma, double 
                    const double* pay, const double* acc, int m, const double* times, int n_dates,
                    uint32_t n_steps, uint64_t n_paths, uint64_t seed, int rng, int disc,
                    const double* host_draws, int allow_any_date, double* out_times, double* out_ee,
-                   double* out_pfe, int32_t* n_out) {
+                   double* out_pfe, int32_t* n_out, const eqhip_model_hw2f* model2 = nullptr) {
     check_ctx(ctx);
     if (n_dates < 1 || !times) raise(EQHIP_E_INVALID, "monitoring_times vacío");
     if (n_steps < 1) raise(EQHIP_E_INVALID, "n_steps debe ser al menos 1");
@@ -817,7 +940,7 @@ void profile_batch(eqhip_context* ctx, double a, double b, double sigma, double 
                 raise(EQHIP_E_INVALID, "t=" + std::to_string(times[k]) + " no es una fecha de reseteo válida del swap");
     double t_max = -INFINITY;
     for (int k = 0; k < n_dates; ++k) t_max = std::max(t_max, times[k]);
-    int nd = (t_max <= 0.0) ? 1 : n_dates;  // exposure.rs:98-108 returns only the first date
+    int nd = (t_max <= 0.0) ? 1 : n_dates;  // exposure.rs:98-108 (and 325-337) return only the first date
     std::vector<int32_t> set_id(n_trades), off(n_trades + 1);
     std::vector<double> st(n_trades, start), P_(n_trades * (size_t)m), A_(n_trades * (size_t)m);
     for (int j = 0; j < n_trades; ++j) {
@@ -834,7 +957,11 @@ void profile_batch(eqhip_context* ctx, double a, double b, double sigma, double 
     eqhip_plan P;
     P.ctx = ctx;
     P.device = ctx->device;
-    build_plan(P, &model, &pf, times, nd, &mc, 0);
+    if (model2) {
+        P.factors = 2;
+        P.model2 = *model2;
+    }
+    build_plan(P, model2 ? nullptr : &model, &pf, times, nd, &mc, 0);
     std::vector<double> ee((size_t)n_trades * nd), pfe((size_t)n_trades * nd);
     eqhip_outputs out{};
     out.set_ee = ee.data();
@@ -874,6 +1001,7 @@ void cva_adjoint(eqhip_plan& P, const double* host_draws, int set, const eqhip_c
                  double* out_grad, double* out_check) {
     if (set < 0 || set >= P.S) raise(EQHIP_E_INVALID, "set index out of range");
     if (!credit || !out_cva || !out_grad) raise(EQHIP_E_INVALID, "credit / out_cva / out_grad are required");
+    if (P.factors != 1) raise(EQHIP_E_INVALID, "the CVA adjoint serves HullWhite1F plans only");
     if (P.det) raise(EQHIP_E_INVALID, "the adjoint needs at least one monitoring date after t = 0");
     const int nd = P.n_dates, D = P.D;
     const uint32_t steps = P.mc.n_steps;
@@ -1378,6 +1506,8 @@ int eqhip_context_create(int device, eqhip_context** out) {
         HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         ctx->lds_limit = prop.sharedMemPerBlock;  // 160 KB per workgroup on gfx950
         if (hipFuncSetAttribute((const void*)value_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)ctx->lds_limit) != hipSuccess ||
+            hipFuncSetAttribute((const void*)value2_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)ctx->lds_limit) != hipSuccess) {
             (void)hipGetLastError();
             ctx->lds_limit = std::min<size_t>(ctx->lds_limit, 65536);
@@ -1406,6 +1536,24 @@ int eqhip_plan_create(eqhip_context* ctx, const eqhip_model_hw1f* model, const e
         P->ctx = ctx;
         P->device = ctx->device;
         build_plan(*P, model, portfolio, dates, n_dates, mc, want_trade_ee);
+        *out = P.release();
+    });
+}
+
+int eqhip_plan_create_2f(eqhip_context* ctx, const eqhip_model_hw2f* model, const eqhip_portfolio* portfolio,
+                         const double* dates, int32_t n_dates, const eqhip_mc* mc, int32_t want_trade_ee,
+                         eqhip_plan** out) {
+    return guarded([&] {
+        if (!out) raise(EQHIP_E_INVALID, "out is null");
+        *out = nullptr;
+        check_ctx(ctx);
+        if (!model) raise(EQHIP_E_INVALID, "model is null");
+        auto P = std::make_unique<eqhip_plan>();
+        P->ctx = ctx;
+        P->device = ctx->device;
+        P->factors = 2;
+        P->model2 = *model;
+        build_plan(*P, nullptr, portfolio, dates, n_dates, mc, want_trade_ee);
         *out = P.release();
     });
 }
@@ -1505,6 +1653,71 @@ int eqhip_unilateral_cva_from_exposure(eqhip_context* ctx, double a, double b, d
                                        double hazard_rate, double recovery_rate, double* out_cva) {
     return eqhip_unilateral_cva_from_exposure_batch(ctx, a, b, sigma, r0, times, ee, 1, n_dates, hazard_rate,
                                                     recovery_rate, out_cva);
+}
+
+int eqhip_irs_hull_white_2f_exposure_profile(
+    eqhip_context* ctx, double a, double b, double sigma, double eta, double rho, double r0, double notional,
+    double fixed_rate, int use_par_rate, double start, const double* payment_times, const double* accruals,
+    int32_t n_flows, const double* monitoring_times, int32_t n_dates, uint32_t n_steps, uint64_t n_paths,
+    uint64_t seed, int32_t rng, int32_t disc, const double* host_draws, int32_t allow_any_date, double* out_times,
+    double* out_ee, double* out_pfe, int32_t* n_out) {
+    return guarded([&] {
+        if (n_flows < 1 || !payment_times || !accruals) raise(EQHIP_E_INVALID, "un swap necesita al menos un periodo");
+        const eqhip_model_hw2f m{a, b, sigma, eta, rho, r0};
+        validate_hw2f(m);
+        double k = use_par_rate ? host_par_rate2(m, start, payment_times, accruals, n_flows) : fixed_rate;
+        profile_batch(ctx, a, b, sigma, r0, &notional, &k, 1, start, payment_times, accruals, n_flows,
+                      monitoring_times, n_dates, n_steps, n_paths, seed, rng, disc, host_draws, allow_any_date,
+                      out_times, out_ee, out_pfe, n_out, &m);
+    });
+}
+
+int eqhip_irs_hull_white_2f_exposure_profile_batch(
+    eqhip_context* ctx, double a, double b, double sigma, double eta, double rho, double r0, const double* notionals,
+    const double* fixed_rates, int32_t n_trades, double start, const double* payment_times, const double* accruals,
+    int32_t n_flows, const double* monitoring_times, int32_t n_dates, uint32_t n_steps, uint64_t n_paths,
+    uint64_t seed, int32_t rng, int32_t disc, const double* host_draws, int32_t allow_any_date, double* out_times,
+    double* out_ee, double* out_pfe, int32_t* n_out) {
+    return guarded([&] {
+        if (n_trades < 1 || !notionals || !fixed_rates)
+            raise(EQHIP_E_INVALID, "el lote requiere al menos un trade con notional y fixed_rate");
+        const eqhip_model_hw2f m{a, b, sigma, eta, rho, r0};
+        profile_batch(ctx, a, b, sigma, r0, notionals, fixed_rates, n_trades, start, payment_times, accruals, n_flows,
+                      monitoring_times, n_dates, n_steps, n_paths, seed, rng, disc, host_draws, allow_any_date,
+                      out_times, out_ee, out_pfe, n_out, &m);
+    });
+}
+
+int eqhip_unilateral_cva_from_exposure_2f_batch(eqhip_context* ctx, double a, double b, double sigma, double eta,
+                                                double rho, double r0, const double* times, const double* ee,
+                                                int32_t n_profiles, int32_t n_dates, double hazard_rate,
+                                                double recovery_rate, double* out_cva) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (n_profiles < 0 || n_dates < 0 || (n_dates > 0 && (!times || !ee)) || !out_cva)
+            raise(EQHIP_E_INVALID, "invalid CVA arguments");
+        const eqhip_model_hw2f m{a, b, sigma, eta, rho, r0};
+        validate_hw2f(m);
+        if (n_profiles == 0) return;
+        hipStream_t st = ctx->stream;
+        DevBuf dt, de, dc, dw;
+        upload(dt, std::vector<double>(times, times + n_dates), st);
+        upload(de, std::vector<double>(ee, ee + (size_t)n_profiles * n_dates), st);
+        dc.alloc((size_t)n_profiles * 8);
+        dw.alloc(std::max<size_t>(1, (size_t)n_dates) * 16);
+        if (n_dates > 0) launch_cva_weights2(st, dt.as<double>(), n_dates, m, hazard_rate, dw.as<double2>());
+        launch_cva_sum(st, de.as<double>(), n_profiles, n_dates, dw.as<double2>(), recovery_rate, dc.as<double>());
+        HIPCHK(hipMemcpyAsync(out_cva, dc.p, (size_t)n_profiles * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
+int eqhip_unilateral_cva_from_exposure_2f(eqhip_context* ctx, double a, double b, double sigma, double eta,
+                                          double rho, double r0, const double* times, const double* ee,
+                                          int32_t n_dates, double hazard_rate, double recovery_rate,
+                                          double* out_cva) {
+    return eqhip_unilateral_cva_from_exposure_2f_batch(ctx, a, b, sigma, eta, rho, r0, times, ee, 1, n_dates,
+                                                       hazard_rate, recovery_rate, out_cva);
 }
 
 int eqhip_pv_dv01(eqhip_context* ctx, const double* pillars, const double* zero_rates, int32_t n_pillars,

@@ -23,6 +23,33 @@ __device__ __forceinline__ double hw_a(double a, double b, double sigma, double 
     return exp(term1 - term2);
 }
 
+// G2++ (models/hull_white_2f.rs:58-113, Brigo-Mercurio eq. 4.11), same expression order.
+// Own variance of a factor with reversion z and volatility vol.
+__device__ __forceinline__ double hw2_var(double z, double vol, double tau) {
+    const double inv_z = pow(z, -1.0);
+    const double e1 = exp(z * (-tau));
+    const double e2 = exp(z * (-2.0 * tau));
+    const double bracket = (((inv_z * 2.0) * e1 - (inv_z * 0.5) * e2) - inv_z * 1.5) + tau;
+    return ((vol * vol) / (z * z)) * bracket;
+}
+// Covariance term of the two factors.
+__device__ __forceinline__ double hw2_cross(double a, double b, double sigma, double eta, double rho, double tau) {
+    const double apb = a + b;
+    const double ta = (exp(a * (-tau)) + (-1.0)) / a;
+    const double tb = (exp(b * (-tau)) + (-1.0)) / b;
+    const double tab = (exp(apb * (-tau)) + (-1.0)) / apb;
+    const double bracket = ((ta + tau) + tb) - tab;
+    const double coef = ((sigma * eta) * (2.0 * rho)) / (a * b);
+    return coef * bracket;
+}
+// The state-free part of ln P(t, t + tau): -r0 tau + V(tau) / 2, so that
+// P = exp((c - B(a,tau) x) - B(b,tau) y).
+__device__ __forceinline__ double hw2_c(double a, double b, double sigma, double eta, double rho, double r0,
+                                        double tau) {
+    const double v = (hw2_var(a, sigma, tau) + hw2_var(b, eta, tau)) + hw2_cross(a, b, sigma, eta, rho, tau);
+    return r0 * (-tau) + v * 0.5;
+}
+
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
 // SC'11): 10 rounds of two 32x32->64 multiplies, Weyl key schedule.
 __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
@@ -50,12 +77,14 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
     out[3] = c3;
 }
 
-// Two standard normals for (path, step pair): counter (pair, path_lo, path_hi, 0), key = seed.
-// u1 = ((w_a >> 11) + 1) 2^-53 in (0,1], u2 = (w_b >> 11) 2^-53 in [0,1); Box-Muller.
+// Two standard normals for (path, step pair): counter (pair, path_lo, path_hi, stream), key = seed.
+// u1 = ((w_a >> 11) + 1) 2^-53 in (0,1], u2 = (w_b >> 11) 2^-53 in [0,1); Box-Muller. Stream 0 is
+// the one-factor model's (two consecutive steps per pair); stream 1 the two-factor model's (the two
+// factors' shocks of one step per pair).
 __device__ __forceinline__ void philox_normal_pair(uint64_t seed, uint64_t path, uint32_t pair,
-                                                   double& z0, double& z1) {
+                                                   double& z0, double& z1, uint32_t stream = 0u) {
     uint32_t o[4];
-    philox4x32_10(pair, (uint32_t)path, (uint32_t)(path >> 32), 0u, (uint32_t)seed,
+    philox4x32_10(pair, (uint32_t)path, (uint32_t)(path >> 32), stream, (uint32_t)seed,
                   (uint32_t)(seed >> 32), o);
     uint64_t wa = ((uint64_t)o[1] << 32) | o[0];
     uint64_t wb = ((uint64_t)o[3] << 32) | o[2];

@@ -60,6 +60,64 @@ template __global__ void simulate_kernel<0, 1>(SimArgs);
 template __global__ void simulate_kernel<1, 0>(SimArgs);
 template __global__ void simulate_kernel<1, 1>(SimArgs);
 
+// G2++ simulation (see Sim2Args). One Philox pair per step gives both factors' shocks.
+template <int DISC, int RNG>
+__global__ void __launch_bounds__(256) simulate2_kernel(Sim2Args g) {
+#pragma clang fp contract(off)
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.n_paths) return;
+    const uint64_t n = g.n_paths;
+    double x = 0.0, y = 0.0;
+    for (uint32_t s = 0;; ++s) {
+        const int slot = g.slot_of_step[s];
+        if (slot >= 0) {
+            g.X[(uint64_t)slot * n + p] = x;
+            g.Y[(uint64_t)slot * n + p] = y;
+        }
+        if (s == g.n_steps) break;
+        double zx, zy;
+        if (RNG == 0) {
+            zx = g.Z[(uint64_t)s * n + p];
+            zy = g.Z[((uint64_t)g.n_steps + s) * n + p];
+        } else {
+            philox_normal_pair(g.seed, p, s, zx, zy, 1u);
+        }
+        if (DISC == 0) {
+            const double zc = zy * g.sqrt_1mrho2 + zx * g.rho;
+            const double drift_x = (-g.a) * x;
+            const double drift_y = (-g.b) * y;
+            x = (x + drift_x * g.dt) + g.sigma * (zx * g.sqrt_dt);
+            y = (y + drift_y * g.dt) + g.eta * (zc * g.sqrt_dt);
+        } else {
+            const double zc = g.c_xy * zx + g.sqrt_1mc2 * zy;
+            x = x * g.ex + g.sdx * zx;
+            y = y * g.ey + g.sdy * zc;
+        }
+    }
+}
+
+template __global__ void simulate2_kernel<0, 0>(Sim2Args);
+template __global__ void simulate2_kernel<0, 1>(Sim2Args);
+template __global__ void simulate2_kernel<1, 0>(Sim2Args);
+template __global__ void simulate2_kernel<1, 1>(Sim2Args);
+
+// G2++ bond factors for a flat list of year fractions: (c, B(a,tau)) and B(b,tau), with
+// P(t, t+tau) = exp((c - B(a,tau) x) - B(b,tau) y) (tau < 0 entries are unused: zeros).
+__global__ void ab2_kernel(const double* tau, int n, double a, double b, double sigma, double eta, double rho,
+                           double r0, double2* c_b1, double* b2) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double t = tau[i];
+    if (t < 0.0) {
+        c_b1[i] = make_double2(0.0, 0.0);
+        b2[i] = 0.0;
+        return;
+    }
+    c_b1[i] = make_double2(hw2_c(a, b, sigma, eta, rho, r0, t), hw_b(a, t));
+    b2[i] = hw_b(b, t);
+}
+
 // A(tau), B(tau) for a flat list of year fractions (tau <= 0 entries are unused: A=B=0).
 __global__ void ab_kernel(const double* tau, int n, double a, double b, double sigma,
                           double2* out) {
@@ -289,6 +347,9 @@ template <class T> __device__ __forceinline__ gmut<T> glb(T* p) { return (gmut<T
 template <class T> using cptr = const __attribute__((address_space(4))) T*;
 template <class T> __device__ __forceinline__ cptr<T> cst(const T* p) { return (cptr<T>)p; }
 
+// NFAC = 2: G2++ plans; the state is (x, y) in (R, R2) and the bond factors are
+// P = exp((c - B(a,tau) x) - B(b,tau) y), with (c, B(a,tau)) in AB / fix_AB and B(b,tau) in AB2 / fix_B2.
+template <int NFAC>
 __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, const int chunk, double* smem) {
     // the descriptor read through the constant address space: no store of the kernel can alias it,
     // so a field is a scalar load wherever it is used instead of a register held for the whole unit
@@ -375,19 +436,50 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     const gptr<double> Rfix = (wave < nfix) ? glb(g.R + (uint64_t)g.fix_slot[fbeg + wave] * g.r_stride) : nullptr;
     const double2 fab_w = (wave < nfix) ? g.fix_AB[fbeg + wave] : make_double2(1.0, 0.0);
     const double inv_a_w = 1.0 / fab_w.x;  // once per unit
-    double r_next = 0.0, rf_next = 0.0;
+    // two factors: the second state at t_k and at fixing `wave`, and that fixing's B(b, tau)
+    const gptr<double> Yk = (NFAC == 2) ? glb(g.R2 + (uint64_t)g.date_slot[k] * g.r_stride) : nullptr;
+    const gptr<double> Yfix =
+        (NFAC == 2 && wave < nfix) ? glb(g.R2 + (uint64_t)g.fix_slot[fbeg + wave] * g.r_stride) : nullptr;
+    const double fb2_w = (NFAC == 2 && wave < nfix) ? g.fix_B2[fbeg + wave] : 0.0;
+    double r_next = 0.0, rf_next = 0.0, y_next = 0.0, yf_next = 0.0;
     if (tile0 < tile_end) {
         const uint64_t gq = tile0 * kPathsPerTile + lane;
         if (gq < n_paths) {
             r_next = Rk[gq];
             if (Rfix) rf_next = Rfix[gq];
+            if (NFAC == 2) {
+                y_next = Yk[gq];
+                if (Yfix) yf_next = Yfix[gq];
+            }
         }
     }
     // A: one list over the waves, the fixings of periods in progress (entry f < nfix, taken by wave f;
     // sF holds 1 / P(T_fix, T_next | r_fix) = exp(B r_fix) / A, so that a family start multiplies) and
     // then the zero-coupon bonds of the live cash-flow dates, for the tile starting at path pa.
-    auto stage_a = [&](uint64_t pa, double r, double rf_w) {
+    auto stage_a = [&](uint64_t pa, double r, double rf_w, double y, double yf_w) {
         const uint64_t gpa = pa + lane;
+        if (NFAC == 2) {  // sF = 1 / P(T_fix, T_next | x_f, y_f), sP = P(t_k, T_d | x, y)
+            const cptr<double> b2d = cst(g.AB2 + (size_t)k * g.D);
+            for (int e = wave; e < nfix + n_live; e += kValueWaves) {
+                if (e < nfix) {
+                    double xf = rf_w, yf = yf_w, c = fab_w.x, b1 = fab_w.y, b2 = fb2_w;
+                    if (e != wave) {  // more fixings than waves
+                        const uint64_t row = (uint64_t)glb(g.fix_slot)[fbeg + e] * g.r_stride;
+                        xf = (gpa < n_paths) ? glb(g.R)[row + gpa] : 0.0;
+                        yf = (gpa < n_paths) ? glb(g.R2)[row + gpa] : 0.0;
+                        const gptr<double> fp = glb((const double*)(g.fix_AB + fbeg + e));
+                        c = fp[0];
+                        b1 = fp[1];
+                        b2 = glb(g.fix_B2)[fbeg + e];
+                    }
+                    sF[e * kPathsPerTile + lane] = exp(-((c - b1 * xf) - b2 * yf));
+                } else {
+                    const int d = dfirst + (e - nfix);
+                    sP[d * kPathsPerTile + lane] = exp((abd[2 * d] - abd[2 * d + 1] * r) - b2d[d] * y);
+                }
+            }
+            return;
+        }
         for (int e = wave; e < nfix + n_live; e += kValueWaves) {
             if (e < nfix) {
                 double rf = rf_w, inv_a = inv_a_w, b = fab_w.y;
@@ -404,7 +496,7 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
             }
         }
     };
-    if (tile0 < tile_end) stage_a(tile0 * kPathsPerTile, r_next, rf_next);
+    if (tile0 < tile_end) stage_a(tile0 * kPathsPerTile, r_next, rf_next, y_next, yf_next);
     __syncthreads();
     // Per tile: B pass 1 | B pass 2 | C, D and A of the next tile | set reduction, with a barrier after
     // each of the first three. A of the next tile can run beside C and D: it writes only sP and sF,
@@ -421,6 +513,10 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
             const bool vq = (tile + 1 < tile_end) && gq < n_paths;
             r_next = vq ? Rk[gq] : 0.0;
             if (Rfix) rf_next = vq ? Rfix[gq] : 0.0;
+            if (NFAC == 2) {
+                y_next = vq ? Yk[gq] : 0.0;
+                if (Yfix) yf_next = vq ? Yfix[gq] : 0.0;
+            }
         }
         {
             double run = 0.0, st = 0.0;
@@ -563,7 +659,7 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                 }
             }
         }
-        if (tile + 1 < tile_end) stage_a(p0 + kPathsPerTile, r_next, rf_next);
+        if (tile + 1 < tile_end) stage_a(p0 + kPathsPerTile, r_next, rf_next, y_next, yf_next);
         __syncthreads();
         if (g.set_mode == 0) {
             if (tid < g.S * kPathsPerTile) {
@@ -612,7 +708,14 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
 __global__ void __launch_bounds__(kValueThreads) value_kernel(const ValuePlan* plans, const int4* units) {
     extern __shared__ double smem[];
     const int4 u = units[blockIdx.x];
-    value_unit(plans[u.x], u.y, u.z, smem);
+    value_unit<1>(plans[u.x], u.y, u.z, smem);
+}
+
+// The same for two-factor (G2++) plans.
+__global__ void __launch_bounds__(kValueThreads) value2_kernel(const ValuePlan* plans, const int4* units) {
+    extern __shared__ double smem[];
+    const int4 u = units[blockIdx.x];
+    value_unit<2>(plans[u.x], u.y, u.z, smem);
 }
 
 // Persistent variant: a fixed grid of blocks pulls work units from a device counter until the
@@ -628,7 +731,7 @@ __global__ void __launch_bounds__(kValueThreads) value_kernel_persistent(const V
         const int idx = s_unit;
         if (idx >= n_units) break;
         const int4 u = units[idx];
-        value_unit(plans[u.x], u.y, u.z, smem);
+        value_unit<1>(plans[u.x], u.y, u.z, smem);
     }
 }
 
@@ -828,6 +931,21 @@ __global__ void cva_weights_kernel(const double* times, int n_dates, double a, d
     const double tau = t - 0.0;
     const double bf = hw_b(a, tau);
     const double disc = hw_a(a, b, sigma, tau, bf) * exp(-(bf * r0));
+    w[k] = make_double2(default_prob, disc);
+}
+
+// G2++ CVA factors: the discount is the model bond at the initial state (0, 0),
+// exposure.rs:470-479 (unilateral_cva_2f), i.e. exp(c(t)).
+__global__ void cva_weights2_kernel(const double* times, int n_dates, double a, double b, double sigma, double eta,
+                                    double rho, double r0, double hazard, double2* w) {
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_dates) return;
+    const double t = times[k];
+    const double survival = exp(-hazard * t);
+    const double prev_survival = (k == 0) ? 1.0 : exp(-hazard * times[k - 1]);
+    const double default_prob = prev_survival - survival;
+    const double disc = exp(hw2_c(a, b, sigma, eta, rho, r0, t - 0.0));
     w[k] = make_double2(default_prob, disc);
 }
 

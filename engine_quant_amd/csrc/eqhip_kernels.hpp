@@ -29,6 +29,23 @@ struct SimArgs {
     double* R;                   // [n_slots][n_paths]
 };
 
+// G2++ (models/hull_white_2f.rs:1-147): x and y start at 0, r = x + y + r0. Euler (the reference's
+// scheme): x' = (x + (-a x) dt) + sigma (zx sqrt(dt)), y' likewise with b, eta and
+// zy = zy' sqrt(1 - rho^2) + zx rho. Exact: each factor's own transition, the two Gaussian
+// increments correlated by their exact covariance.
+struct Sim2Args {
+    double a, b, sigma, eta, rho;
+    double dt, sqrt_dt, sqrt_1mrho2;                  // Euler
+    double ex, ey, sdx, sdy, c_xy, sqrt_1mc2;         // exact
+    uint64_t n_paths;
+    uint32_t n_steps;
+    uint64_t seed;
+    const int* slot_of_step;
+    const double* Z;  // host draws [2][n_steps][n_paths]: every x shock, then every y shock
+    double* X;        // [n_slots][n_paths]
+    double* Y;        // [n_slots][n_paths]
+};
+
 // One exposure plan as the valuation kernel sees it. A launch takes an array of plans and a list
 // of work units (plan, date, path chunk), so one launch can serve many netting-set plans.
 struct ValuePlan {
@@ -39,6 +56,11 @@ struct ValuePlan {
     int n_dates, D, C, NF, S, Fmax;
     int n_chunks, tiles_per_block;
     int set_mode;             // 0: S <= kMaxSplitSets, class sums split across waves; 1: one wave per set
+    int two_factor;           // 1: G2++ (R holds x, R2 holds y; P = exp((c - B1 x) - B2 y) with AB = (c, B1),
+                              //    AB2 = B2; fixings likewise with fix_AB, fix_B2)
+    const double* R2;         // [n_slots][r_stride] second factor (two-factor plans)
+    const double* AB2;        // [n_dates][D]
+    const double* fix_B2;
     const double* dates;      // [n_dates]
     const double2* AB;        // [n_dates][D] (A, B) of P(t_k, T_d)
     const int* date_slot;     // [n_dates]
@@ -154,7 +176,14 @@ __global__ void zcb_sens_kernel(const double* tau, int n, double a, double b, do
 template <int DISC, int RNG>
 __global__ void adjoint_kernel(AdjArgs g);
 __global__ void ab_kernel(const double* tau, int n, double a, double b, double sigma, double2* out);
+template <int DISC, int RNG>
+__global__ void simulate2_kernel(Sim2Args g);
+__global__ void ab2_kernel(const double* tau, int n, double a, double b, double sigma, double eta, double rho,
+                           double r0, double2* c_b1, double* b2);
+__global__ void cva_weights2_kernel(const double* times, int n_dates, double a, double b, double sigma, double eta,
+                                    double rho, double r0, double hazard, double2* w);
 __global__ void value_kernel(const ValuePlan* plans, const int4* units);
+__global__ void value2_kernel(const ValuePlan* plans, const int4* units);
 __global__ void value_kernel_persistent(const ValuePlan* plans, const int4* units, int n_units,
                                         unsigned int* counter);
 __global__ void reset_counter_kernel(unsigned int* counter);

@@ -121,6 +121,23 @@ private:
     double a_, b_, sigma_, r0_;
 };
 
+// model.cpp:54-71: G2++ with r = x + y + r0, state (x, y) from (0, 0).
+class HullWhite2FModel : public IModel {
+public:
+    explicit HullWhite2FModel(const Params& params);
+    std::string type_name() const override { return "HullWhite2F"; }
+    Params to_params() const override;
+    double a() const { return a_; }
+    double b() const { return b_; }
+    double sigma() const { return sigma_; }
+    double eta() const { return eta_; }
+    double rho() const { return rho_; }
+    double r0() const { return r0_; }
+
+private:
+    double a_, b_, sigma_, eta_, rho_, r0_;
+};
+
 class IProduct {
 public:
     virtual ~IProduct() = default;
@@ -204,6 +221,26 @@ std::vector<double> unilateral_cva_from_exposure_batch(const ExecutionContext& e
                                                        double sigma, double r0,
                                                        const std::vector<ExposureProfile>& profiles,
                                                        double hazard_rate, double recovery_rate);
+// engine.cpp:226-266 and their batch forms (the HullWhite2F arm)
+ExposureProfile irs_hull_white_2f_exposure_profile(const ExecutionContext& execution, double a, double b,
+                                                   double sigma, double eta, double rho, double r0, double notional,
+                                                   double fixed_rate, bool use_par_rate, double start,
+                                                   const std::vector<double>& payment_times,
+                                                   const std::vector<double>& accruals,
+                                                   const std::vector<double>& monitoring_times, std::size_t n_steps,
+                                                   std::uint64_t n_paths, std::uint64_t seed);
+std::vector<ExposureProfile> irs_hull_white_2f_exposure_profile_batch(
+    const ExecutionContext& execution, double a, double b, double sigma, double eta, double rho, double r0,
+    const std::vector<double>& notionals, const std::vector<double>& fixed_rates, double start,
+    const std::vector<double>& payment_times, const std::vector<double>& accruals,
+    const std::vector<double>& monitoring_times, std::size_t n_steps, std::uint64_t n_paths, std::uint64_t seed);
+double unilateral_cva_from_exposure_2f(const ExecutionContext& execution, double a, double b, double sigma,
+                                       double eta, double rho, double r0, const std::vector<double>& times,
+                                       const std::vector<double>& ee, double hazard_rate, double recovery_rate);
+std::vector<double> unilateral_cva_from_exposure_2f_batch(const ExecutionContext& execution, double a, double b,
+                                                          double sigma, double eta, double rho, double r0,
+                                                          const std::vector<ExposureProfile>& profiles,
+                                                          double hazard_rate, double recovery_rate);
 
 // ---- measure.hpp:30-172 -----------------------------------------------------------------------
 struct MeasureResult {

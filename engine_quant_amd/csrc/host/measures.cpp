@@ -72,8 +72,14 @@ double effective_fixed_rate(const MarketSnapshot& market, const IrSwapProduct& p
     return p.use_par_rate() ? par_rate_from_market(market, p) : p.fixed_rate();
 }
 
+// measure.cpp:36-65: the model's own exposure wrapper
 ExposureProfile compute_exposure_profile(const IModel& model, const IrSwapProduct& irs, const PricingContext& pricing,
                                          const ExecutionContext& execution) {
+    if (const auto* hw2 = dynamic_cast<const HullWhite2FModel*>(&model))
+        return irs_hull_white_2f_exposure_profile(execution, hw2->a(), hw2->b(), hw2->sigma(), hw2->eta(), hw2->rho(),
+                                                  hw2->r0(), irs.notional(), irs.fixed_rate(), irs.use_par_rate(),
+                                                  irs.start(), irs.payment_times(), irs.accruals(), reset_dates(irs),
+                                                  pricing.n_steps(), pricing.n_paths(), pricing.seed());
     const auto& hw1 = require_hw1f(model, "ExposureProfileMeasure");
     return irs_hull_white_exposure_profile(execution, hw1.a(), hw1.b(), hw1.sigma(), hw1.r0(), irs.notional(),
                                            irs.fixed_rate(), irs.use_par_rate(), irs.start(), irs.payment_times(),
@@ -174,19 +180,106 @@ std::vector<double> unilateral_cva_from_exposure_batch(const ExecutionContext& e
     return out;
 }
 
+ExposureProfile irs_hull_white_2f_exposure_profile(const ExecutionContext& execution, double a, double b,
+                                                   double sigma, double eta, double rho, double r0, double notional,
+                                                   double fixed_rate, bool use_par_rate, double start,
+                                                   const std::vector<double>& payment_times,
+                                                   const std::vector<double>& accruals,
+                                                   const std::vector<double>& monitoring_times, std::size_t n_steps,
+                                                   std::uint64_t n_paths, std::uint64_t seed) {
+    require_gpu(execution);
+    std::size_t nd = monitoring_times.size();
+    ExposureProfile out;
+    out.times.resize(nd);
+    out.ee.resize(nd);
+    out.pfe_95.resize(nd);
+    int32_t n_out = 0;
+    check(eqhip_irs_hull_white_2f_exposure_profile(
+        gpu_context(), a, b, sigma, eta, rho, r0, notional, fixed_rate, use_par_rate ? 1 : 0, start,
+        payment_times.data(), accruals.data(), (int32_t)payment_times.size(), monitoring_times.data(), (int32_t)nd,
+        (uint32_t)n_steps, n_paths, seed, EQHIP_RNG_PHILOX, disc_of(execution), nullptr, 0, out.times.data(),
+        out.ee.data(), out.pfe_95.data(), &n_out));
+    out.times.resize(n_out);
+    out.ee.resize(n_out);
+    out.pfe_95.resize(n_out);
+    return out;
+}
+
+std::vector<ExposureProfile> irs_hull_white_2f_exposure_profile_batch(
+    const ExecutionContext& execution, double a, double b, double sigma, double eta, double rho, double r0,
+    const std::vector<double>& notionals, const std::vector<double>& fixed_rates, double start,
+    const std::vector<double>& payment_times, const std::vector<double>& accruals,
+    const std::vector<double>& monitoring_times, std::size_t n_steps, std::uint64_t n_paths, std::uint64_t seed) {
+    require_gpu(execution);
+    if (notionals.size() != fixed_rates.size())
+        throw std::invalid_argument("el lote requiere un fixed_rate por notional");
+    std::size_t nt = notionals.size(), nd = monitoring_times.size();
+    std::vector<double> times(nd), ee(nt * nd), pfe(nt * nd);
+    int32_t n_out = 0;
+    check(eqhip_irs_hull_white_2f_exposure_profile_batch(
+        gpu_context(), a, b, sigma, eta, rho, r0, notionals.data(), fixed_rates.data(), (int32_t)nt, start,
+        payment_times.data(), accruals.data(), (int32_t)payment_times.size(), monitoring_times.data(), (int32_t)nd,
+        (uint32_t)n_steps, n_paths, seed, EQHIP_RNG_PHILOX, disc_of(execution), nullptr, 0, times.data(), ee.data(),
+        pfe.data(), &n_out));
+    std::vector<ExposureProfile> out(nt);
+    for (std::size_t j = 0; j < nt; ++j) {
+        out[j].times.assign(times.begin(), times.begin() + n_out);
+        out[j].ee.assign(ee.begin() + j * nd, ee.begin() + j * nd + n_out);
+        out[j].pfe_95.assign(pfe.begin() + j * nd, pfe.begin() + j * nd + n_out);
+    }
+    return out;
+}
+
+double unilateral_cva_from_exposure_2f(const ExecutionContext& execution, double a, double b, double sigma,
+                                       double eta, double rho, double r0, const std::vector<double>& times,
+                                       const std::vector<double>& ee, double hazard_rate, double recovery_rate) {
+    require_gpu(execution);
+    if (times.size() != ee.size()) throw std::invalid_argument("times y ee deben tener el mismo largo");
+    double cva = 0.0;
+    check(eqhip_unilateral_cva_from_exposure_2f(gpu_context(), a, b, sigma, eta, rho, r0, times.data(), ee.data(),
+                                                (int32_t)times.size(), hazard_rate, recovery_rate, &cva));
+    return cva;
+}
+
+std::vector<double> unilateral_cva_from_exposure_2f_batch(const ExecutionContext& execution, double a, double b,
+                                                          double sigma, double eta, double rho, double r0,
+                                                          const std::vector<ExposureProfile>& profiles,
+                                                          double hazard_rate, double recovery_rate) {
+    require_gpu(execution);
+    std::vector<double> out(profiles.size());
+    if (profiles.empty()) return out;
+    const std::vector<double>& times = profiles.front().times;
+    std::vector<double> ee;
+    ee.reserve(profiles.size() * times.size());
+    for (const auto& p : profiles) {
+        if (p.times != times || p.ee.size() != times.size())
+            throw std::invalid_argument("unilateral_cva_from_exposure_2f_batch: profiles must share their times");
+        ee.insert(ee.end(), p.ee.begin(), p.ee.end());
+    }
+    check(eqhip_unilateral_cva_from_exposure_2f_batch(gpu_context(), a, b, sigma, eta, rho, r0, times.data(),
+                                                      ee.data(), (int32_t)profiles.size(), (int32_t)times.size(),
+                                                      hazard_rate, recovery_rate, out.data()));
+    return out;
+}
+
 // ---- batched helpers (measure.cpp:238-325) ----------------------------------------------------
 
 std::vector<ExposureProfile> compute_exposure_profile_batch(const IModel& model,
                                                             const std::vector<const IrSwapProduct*>& products,
                                                             const PricingContext& pricing,
                                                             const ExecutionContext& execution) {
-    const auto& hw1 = require_hw1f(model, "ExposureProfileMeasure");
     const IrSwapProduct& first = *products.front();
     std::vector<double> notionals, fixed_rates;
     for (const auto* p : products) {
         notionals.push_back(p->notional());
         fixed_rates.push_back(p->fixed_rate());
     }
+    if (const auto* hw2 = dynamic_cast<const HullWhite2FModel*>(&model))
+        return irs_hull_white_2f_exposure_profile_batch(execution, hw2->a(), hw2->b(), hw2->sigma(), hw2->eta(),
+                                                        hw2->rho(), hw2->r0(), notionals, fixed_rates, first.start(),
+                                                        first.payment_times(), first.accruals(), reset_dates(first),
+                                                        pricing.n_steps(), pricing.n_paths(), pricing.seed());
+    const auto& hw1 = require_hw1f(model, "ExposureProfileMeasure");
     return irs_hull_white_exposure_profile_batch(execution, hw1.a(), hw1.b(), hw1.sigma(), hw1.r0(), notionals,
                                                  fixed_rates, first.start(), first.payment_times(), first.accruals(),
                                                  reset_dates(first), pricing.n_steps(), pricing.n_paths(),
@@ -196,6 +289,9 @@ std::vector<ExposureProfile> compute_exposure_profile_batch(const IModel& model,
 std::vector<double> compute_cva_from_exposure_batch(const IModel& model, const ExecutionContext& execution,
                                                     const std::vector<ExposureProfile>& profiles,
                                                     double hazard_rate, double recovery_rate) {
+    if (const auto* hw2 = dynamic_cast<const HullWhite2FModel*>(&model))
+        return unilateral_cva_from_exposure_2f_batch(execution, hw2->a(), hw2->b(), hw2->sigma(), hw2->eta(),
+                                                     hw2->rho(), hw2->r0(), profiles, hazard_rate, recovery_rate);
     const auto& hw1 = require_hw1f(model, "UnilateralCvaMeasure");
     return unilateral_cva_from_exposure_batch(execution, hw1.a(), hw1.b(), hw1.sigma(), hw1.r0(), profiles,
                                               hazard_rate, recovery_rate);
@@ -249,9 +345,15 @@ MeasureResult UnilateralCvaMeasure::evaluate(const IModel& model, const IProduct
                                              const ExecutionContext& execution) const {
     ExposureProfileMeasure exposure(Params{});
     MeasureResult r = exposure.evaluate(model, product, market, pricing, execution);
-    const auto& hw1 = require_hw1f(model, "UnilateralCvaMeasure");
-    r.scalar = unilateral_cva_from_exposure(execution, hw1.a(), hw1.b(), hw1.sigma(), hw1.r0(), r.times, r.primary,
-                                            market.hazard_rate(), market.recovery_rate());
+    if (const auto* hw2 = dynamic_cast<const HullWhite2FModel*>(&model)) {  // measure.cpp:79-85
+        r.scalar = unilateral_cva_from_exposure_2f(execution, hw2->a(), hw2->b(), hw2->sigma(), hw2->eta(),
+                                                   hw2->rho(), hw2->r0(), r.times, r.primary, market.hazard_rate(),
+                                                   market.recovery_rate());
+    } else {
+        const auto& hw1 = require_hw1f(model, "UnilateralCvaMeasure");
+        r.scalar = unilateral_cva_from_exposure(execution, hw1.a(), hw1.b(), hw1.sigma(), hw1.r0(), r.times,
+                                                r.primary, market.hazard_rate(), market.recovery_rate());
+    }
     r.has_scalar = true;
     return r;
 }
@@ -321,6 +423,7 @@ CvaAdjoint unilateral_cva_adjoint(const IModel& model, const IProduct& product, 
 
 void register_builtins(Registries& registries) {
     registries.models.register_type<HullWhite1FModel>("HullWhite1F");
+    registries.models.register_type<HullWhite2FModel>("HullWhite2F");
     registries.products.register_type<IrSwapProduct>("IRSwap");
     registries.measures.register_type<ExposureProfileMeasure>("ExposureProfile");
     registries.measures.register_type<UnilateralCvaMeasure>("UnilateralCVA");

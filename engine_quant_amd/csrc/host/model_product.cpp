@@ -1,4 +1,4 @@
-// HullWhite1FModel (cpp/engine/src/model.cpp:39-52) and IrSwapProduct (product.cpp:7-25).
+// HullWhite1FModel, HullWhite2FModel (cpp/engine/src/model.cpp:39-71) and IrSwapProduct (product.cpp:7-25).
 #include "engine.hpp"
 
 namespace engine {
@@ -8,6 +8,14 @@ HullWhite1FModel::HullWhite1FModel(const Params& params)
       r0_(get_double(params, "r0")) {}
 
 Params HullWhite1FModel::to_params() const { return Params{{"a", a_}, {"b", b_}, {"sigma", sigma_}, {"r0", r0_}}; }
+
+HullWhite2FModel::HullWhite2FModel(const Params& params)
+    : a_(get_double(params, "a")), b_(get_double(params, "b")), sigma_(get_double(params, "sigma")),
+      eta_(get_double(params, "eta")), rho_(get_double(params, "rho")), r0_(get_double(params, "r0")) {}
+
+Params HullWhite2FModel::to_params() const {
+    return Params{{"a", a_}, {"b", b_}, {"sigma", sigma_}, {"eta", eta_}, {"rho", rho_}, {"r0", r0_}};
+}
 
 // fixed_rate absent => par swap (use_par_rate), product.cpp:10.
 IrSwapProduct::IrSwapProduct(const Params& params)

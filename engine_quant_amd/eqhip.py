@@ -35,6 +35,8 @@ EXPORTED = [
     "eqhip_berm_exposure_lsm", "eqhip_berm_train", "eqhip_berm_exposure_mlp", "eqhip_berm_mse",
     "eqhip_berm_get_params", "eqhip_berm_set_params", "eqhip_berm_mlp_grad", "eqhip_berm_mlp_forward",
     "eqhip_berm_adam_step", "eqhip_berm_get_timing", "eqhip_berm_destroy",
+    "eqhip_irs_hull_white_2f_exposure_profile", "eqhip_irs_hull_white_2f_exposure_profile_batch",
+    "eqhip_unilateral_cva_from_exposure_2f", "eqhip_unilateral_cva_from_exposure_2f_batch", "eqhip_plan_create_2f",
 ]
 
 
@@ -46,6 +48,10 @@ class EqhipError(RuntimeError):
 
 class ModelHW1F(ctypes.Structure):
     _fields_ = [("a", _D), ("b", _D), ("sigma", _D), ("r0", _D)]
+
+
+class ModelHW2F(ctypes.Structure):
+    _fields_ = [("a", _D), ("b", _D), ("sigma", _D), ("eta", _D), ("rho", _D), ("r0", _D)]
 
 
 class PortfolioC(ctypes.Structure):
@@ -94,6 +100,13 @@ def lib():
         L.eqhip_irs_hull_white_exposure_profile_batch.argtypes = prof + [_PD, _PD, _I32] + tail
         L.eqhip_unilateral_cva_from_exposure.argtypes = prof + [_PD, _PD, _I32, _D, _D, _PD]
         L.eqhip_unilateral_cva_from_exposure_batch.argtypes = prof + [_PD, _PD, _I32, _I32, _D, _D, _PD]
+        prof2 = [ctypes.c_void_p, _D, _D, _D, _D, _D, _D]
+        L.eqhip_irs_hull_white_2f_exposure_profile.argtypes = prof2 + [_D, _D, ctypes.c_int] + tail
+        L.eqhip_irs_hull_white_2f_exposure_profile_batch.argtypes = prof2 + [_PD, _PD, _I32] + tail
+        L.eqhip_unilateral_cva_from_exposure_2f.argtypes = prof2 + [_PD, _PD, _I32, _D, _D, _PD]
+        L.eqhip_unilateral_cva_from_exposure_2f_batch.argtypes = prof2 + [_PD, _PD, _I32, _I32, _D, _D, _PD]
+        L.eqhip_plan_create_2f.argtypes = [ctypes.c_void_p, ctypes.POINTER(ModelHW2F), ctypes.POINTER(PortfolioC),
+                                           _PD, _I32, ctypes.POINTER(McC), _I32, ctypes.POINTER(ctypes.c_void_p)]
         L.eqhip_pv_dv01.argtypes = [ctypes.c_void_p, _PD, _PD, _I32, ctypes.POINTER(PortfolioC), _D, _PD,
                                     _PD, _PD]
         L.eqhip_plan_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(ModelHW1F),
@@ -177,7 +190,11 @@ class Portfolio:
 
 
 class Plan:
-    """Device-resident exposure job (eqhip_plan_create / execute / destroy)."""
+    """Device-resident exposure job (eqhip_plan_create / execute / destroy).
+
+    ``model`` is (a, b, sigma, r0) for HullWhite1F or (a, b, sigma, eta, rho, r0) for HullWhite2F
+    (eqhip_plan_create_2f); a two-factor plan takes host draws [2][n_steps][n_paths].
+    """
 
     def __init__(self, ctx, model, portfolio: Portfolio, dates, n_paths, n_steps, seed=0,
                  rng=RNG_PHILOX, disc=DISC_EULER, want_trade_ee=False):
@@ -188,11 +205,13 @@ class Plan:
         self.rng = rng
         self.want_trade_ee = bool(want_trade_ee)
         self._pc = portfolio._c()
-        m = ModelHW1F(*model)
+        self.factors = 2 if len(model) == 6 else 1
+        m = ModelHW2F(*model) if self.factors == 2 else ModelHW1F(*model)
+        create = lib().eqhip_plan_create_2f if self.factors == 2 else lib().eqhip_plan_create
         mc = McC(self.n_paths, self.n_steps, int(seed), rng, disc)
         h = ctypes.c_void_p()
-        _check(lib().eqhip_plan_create(ctx.handle, ctypes.byref(m), ctypes.byref(self._pc), _dp(self.dates),
-                                       len(self.dates), ctypes.byref(mc), int(want_trade_ee), ctypes.byref(h)))
+        _check(create(ctx.handle, ctypes.byref(m), ctypes.byref(self._pc), _dp(self.dates), len(self.dates),
+                      ctypes.byref(mc), int(want_trade_ee), ctypes.byref(h)))
         self.handle = h
 
     def execute(self, host_draws=None, hazard_rate=0.0, recovery_rate=0.0, want_cva=True,
@@ -214,8 +233,9 @@ class Plan:
         z = None
         if self.rng == RNG_HOST_DRAWS:
             z = _f64(host_draws)
-            if z.shape != (self.n_steps, self.n_paths):
-                raise ValueError(f"host_draws must be [n_steps={self.n_steps}][n_paths={self.n_paths}]")
+            shape = (self.n_steps, self.n_paths) if self.factors == 1 else (2, self.n_steps, self.n_paths)
+            if z.shape != shape:
+                raise ValueError(f"host_draws must have shape {shape}")
         cr = CreditC(hazard_rate, recovery_rate)
         _check(lib().eqhip_plan_execute(self.handle, _dp(z), ctypes.byref(cr), ctypes.byref(out)))
         return res
@@ -314,6 +334,53 @@ class Context:
         _check(lib().eqhip_unilateral_cva_from_exposure_batch(self.handle, a, b, sigma, r0, _dp(times),
                                                               _dp(ee), ee.shape[0], len(times), hazard_rate,
                                                               recovery_rate, _dp(out)))
+        return out
+
+    def exposure_profile_2f(self, a, b, sigma, eta, rho, r0, notional, fixed_rate, use_par_rate, start, pay, acc,
+                            times, n_steps, n_paths, seed, rng=RNG_PHILOX, disc=DISC_EULER, host_draws=None,
+                            allow_any_date=False):
+        pay, acc, times = _f64(pay), _f64(acc), _f64(times)
+        nd = len(times)
+        ot, oe, op = np.zeros(nd), np.zeros(nd), np.zeros(nd)
+        n_out = ctypes.c_int32()
+        z = _f64(host_draws) if host_draws is not None else None
+        _check(lib().eqhip_irs_hull_white_2f_exposure_profile(
+            self.handle, a, b, sigma, eta, rho, r0, notional, fixed_rate, int(bool(use_par_rate)), start, _dp(pay),
+            _dp(acc), len(pay), _dp(times), nd, n_steps, n_paths, seed, rng, disc, _dp(z), int(allow_any_date),
+            _dp(ot), _dp(oe), _dp(op), ctypes.byref(n_out)))
+        n = n_out.value
+        return ot[:n], oe[:n], op[:n]
+
+    def exposure_profile_2f_batch(self, a, b, sigma, eta, rho, r0, notionals, fixed_rates, start, pay, acc, times,
+                                  n_steps, n_paths, seed, rng=RNG_PHILOX, disc=DISC_EULER, host_draws=None,
+                                  allow_any_date=False):
+        notionals, fixed_rates = _f64(notionals), _f64(fixed_rates)
+        pay, acc, times = _f64(pay), _f64(acc), _f64(times)
+        nt, nd = len(notionals), len(times)
+        ot, oe, op = np.zeros(nd), np.zeros((nt, nd)), np.zeros((nt, nd))
+        n_out = ctypes.c_int32()
+        z = _f64(host_draws) if host_draws is not None else None
+        _check(lib().eqhip_irs_hull_white_2f_exposure_profile_batch(
+            self.handle, a, b, sigma, eta, rho, r0, _dp(notionals), _dp(fixed_rates), nt, start, _dp(pay),
+            _dp(acc), len(pay), _dp(times), nd, n_steps, n_paths, seed, rng, disc, _dp(z), int(allow_any_date),
+            _dp(ot), _dp(oe), _dp(op), ctypes.byref(n_out)))
+        n = n_out.value
+        return ot[:n], oe[:, :n], op[:, :n]
+
+    def unilateral_cva_2f(self, a, b, sigma, eta, rho, r0, times, ee, hazard_rate, recovery_rate):
+        times, ee = _f64(times), _f64(ee)
+        out = ctypes.c_double()
+        _check(lib().eqhip_unilateral_cva_from_exposure_2f(self.handle, a, b, sigma, eta, rho, r0, _dp(times),
+                                                           _dp(ee), len(times), hazard_rate, recovery_rate,
+                                                           ctypes.byref(out)))
+        return out.value
+
+    def unilateral_cva_2f_batch(self, a, b, sigma, eta, rho, r0, times, ee, hazard_rate, recovery_rate):
+        times, ee = _f64(times), np.atleast_2d(_f64(ee))
+        out = np.zeros(ee.shape[0])
+        _check(lib().eqhip_unilateral_cva_from_exposure_2f_batch(self.handle, a, b, sigma, eta, rho, r0,
+                                                                 _dp(times), _dp(ee), ee.shape[0], len(times),
+                                                                 hazard_rate, recovery_rate, _dp(out)))
         return out
 
     def pv_dv01(self, pillars, zero_rates, portfolio: Portfolio, bump=1e-4, bucketed=True):

@@ -43,6 +43,13 @@ typedef struct {
     double a, b, sigma, r0;
 } eqhip_model_hw1f;
 
+/* HullWhite2F (G2++) model parameters, cpp/engine/src/model.cpp:54-60: r = x + y + r0 with
+ * dx = -a x dt + sigma dW1, dy = -b y dt + eta dW2, d<W1,W2> = rho dt, x0 = y0 = 0
+ * (rust/crates/engine-core/src/models/hull_white_2f.rs). */
+typedef struct {
+    double a, b, sigma, eta, rho, r0;
+} eqhip_model_hw2f;
+
 /* Portfolio of vanilla IRS in structure-of-arrays form. Trade j belongs to netting set
  * set_id[j]; its schedule is payment_times/accruals[flow_offset[j] .. flow_offset[j+1]) with
  * floating-leg start start[j]. Payer swaps have notional > 0, receivers notional < 0 (the
@@ -154,6 +161,48 @@ int eqhip_unilateral_cva_from_exposure_batch(eqhip_context* ctx, double a, doubl
                                              int32_t n_dates, double hazard_rate,
                                              double recovery_rate, double* out_cva);
 
+/* Replaces engine::ffi::irs_hull_white_2f_exposure_profile (engine-ffi/src/lib.rs:547-565;
+ * api.rs:587-665; exposure.rs:305-370): the G2++ profile of one payer IRS. use_par_rate != 0 sets
+ * the fixed rate to the par rate at the initial state (0, 0). host_draws is [2][n_steps][n_paths]
+ * (every x shock, then every independent y shock, as exposure.rs:339-344 draws them) and is read
+ * only when rng == EQHIP_RNG_HOST_DRAWS. A date on grid index 0 averages n_paths equal values, as
+ * the reference's path[0] holds n_paths zero states. Other arguments as in
+ * eqhip_irs_hull_white_exposure_profile. */
+int eqhip_irs_hull_white_2f_exposure_profile(
+    eqhip_context* ctx, double a, double b, double sigma, double eta, double rho, double r0,
+    double notional, double fixed_rate, int use_par_rate, double start, const double* payment_times,
+    const double* accruals, int32_t n_flows, const double* monitoring_times, int32_t n_dates,
+    uint32_t n_steps, uint64_t n_paths, uint64_t seed, int32_t rng, int32_t disc,
+    const double* host_draws, int32_t allow_any_date, double* out_times, double* out_ee,
+    double* out_pfe, int32_t* n_out);
+
+/* Replaces engine::ffi::irs_hull_white_2f_exposure_profile_batch (engine-ffi/src/lib.rs:677-694;
+ * exposure.rs:378-447): homogeneous batch, one G2++ simulation, out_ee / out_pfe
+ * [n_trades][n_dates]. */
+int eqhip_irs_hull_white_2f_exposure_profile_batch(
+    eqhip_context* ctx, double a, double b, double sigma, double eta, double rho, double r0,
+    const double* notionals, const double* fixed_rates, int32_t n_trades, double start,
+    const double* payment_times, const double* accruals, int32_t n_flows,
+    const double* monitoring_times, int32_t n_dates, uint32_t n_steps, uint64_t n_paths,
+    uint64_t seed, int32_t rng, int32_t disc, const double* host_draws, int32_t allow_any_date,
+    double* out_times, double* out_ee, double* out_pfe, int32_t* n_out);
+
+/* Replaces engine::ffi::unilateral_cva_from_exposure_2f (engine-ffi/src/lib.rs:567-579;
+ * exposure.rs:452-466): the discount is the G2++ bond at the initial state (0, 0). */
+int eqhip_unilateral_cva_from_exposure_2f(eqhip_context* ctx, double a, double b, double sigma,
+                                          double eta, double rho, double r0, const double* times,
+                                          const double* ee, int32_t n_dates, double hazard_rate,
+                                          double recovery_rate, double* out_cva);
+
+/* Replaces engine::ffi::unilateral_cva_from_exposure_2f_batch (engine-ffi/src/lib.rs:696-708;
+ * exposure.rs:468-480): ee is [n_profiles][n_dates] over shared times. */
+int eqhip_unilateral_cva_from_exposure_2f_batch(eqhip_context* ctx, double a, double b,
+                                                double sigma, double eta, double rho, double r0,
+                                                const double* times, const double* ee,
+                                                int32_t n_profiles, int32_t n_dates,
+                                                double hazard_rate, double recovery_rate,
+                                                double* out_cva);
+
 /* PV and DV01 of every trade from the market zero curve, cpp/engine/src/measure.cpp:124-222
  * (npv_from_market, compute_dv01, dv01_bucketed_from_market) with market.cpp:26-41,95-111.
  * fixed_rate in the portfolio is the effective rate (par under the market for par swaps).
@@ -183,8 +232,14 @@ int eqhip_plan_create(eqhip_context* ctx, const eqhip_model_hw1f* model,
                       const eqhip_portfolio* portfolio, const double* dates, int32_t n_dates,
                       const eqhip_mc* mc, int32_t want_trade_ee, eqhip_plan** out);
 
-/* Runs the job on resident data and copies the results into out. host_draws ([n_steps][n_paths])
- * is read only in EQHIP_RNG_HOST_DRAWS mode. */
+/* The same plan under G2++ (the portfolio form of exposure.rs:305-370 and measure.cpp:53-64,
+ * 257-268): every trade's value is the reference's swap valuation with the two-factor bond. */
+int eqhip_plan_create_2f(eqhip_context* ctx, const eqhip_model_hw2f* model,
+                         const eqhip_portfolio* portfolio, const double* dates, int32_t n_dates,
+                         const eqhip_mc* mc, int32_t want_trade_ee, eqhip_plan** out);
+
+/* Runs the job on resident data and copies the results into out. host_draws ([n_steps][n_paths],
+ * [2][n_steps][n_paths] for a G2++ plan) is read only in EQHIP_RNG_HOST_DRAWS mode. */
 int eqhip_plan_execute(eqhip_plan* plan, const double* host_draws, const eqhip_credit* credit,
                        eqhip_outputs* out);
 
@@ -207,7 +262,8 @@ int eqhip_exposure_run(eqhip_context* ctx, const eqhip_model_hw1f* model,
  * to rounding). Curve-pillar sensitivities of this CVA are identically zero: it discounts with the
  * model bond P(0,t|r0), not the market curve (exposure.rs:264-275). Requires one grid step per
  * monitoring date. The reference computes CVA Greeks by bump-and-revalue with common random
- * numbers (cpp/engine/src/greeks.cpp:160-169). */
+ * numbers (cpp/engine/src/greeks.cpp:160-169). HullWhite1F plans only (EQHIP_E_INVALID on a G2++
+ * plan). */
 int eqhip_cva_adjoint(eqhip_plan* plan, const double* host_draws, int32_t set, const eqhip_credit* credit,
                       double* out_cva, double* out_grad, double* out_check);
 

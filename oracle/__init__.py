@@ -81,6 +81,28 @@ def lib():
             _D, _D, _D, _D, _I, _I, _PI32, _PD, _PD, _PD, _PI32, _PD, _PD, _PD, _I, _U32, _U64, _I,
             _PD, _D, _D, _PD, _PD, _PD, _PD, _PD, _PD, _PD, _PD, _I]
         L.eqr_max_threads.restype = _I
+        L.eqr_hw2_var.restype = _D
+        L.eqr_hw2_var.argtypes = [_D] * 3
+        L.eqr_hw2_cross.restype = _D
+        L.eqr_hw2_cross.argtypes = [_D] * 6
+        L.eqr_zcb2.restype = _D
+        L.eqr_zcb2.argtypes = [_D] * 10
+        L.eqr_philox_normals2.argtypes = [_U64, _U64, _U32, _PD]
+        L.eqr_simulate2.argtypes = [_D] * 6 + [_U32, _U64, _I, _PD, _PD, _PD]
+        L.eqr_par_rate2.restype = _D
+        L.eqr_par_rate2.argtypes = [_D] * 7 + [_PD, _PD, _I]
+        L.eqr_irs_npv_reset2.restype = _D
+        L.eqr_irs_npv_reset2.argtypes = [_D] * 12 + [_PD, _PD, _I]
+        L.eqr_irs_value2.restype = _D
+        L.eqr_irs_value2.argtypes = [_D] * 14 + [_PD, _PD, _I]
+        L.eqr_unilateral_cva2.restype = _D
+        L.eqr_unilateral_cva2.argtypes = [_D] * 6 + [_PD, _PD, _I, _D, _D]
+        L.eqr_exposure_profile2.restype = _I
+        L.eqr_exposure_profile2.argtypes = [_D] * 8 + [_I, _D, _PD, _PD, _I, _PD, _I, _U32, _U64, _PD, _PD, _PD]
+        L.eqr_portfolio_exposure2.restype = _I
+        L.eqr_portfolio_exposure2.argtypes = [
+            _D, _D, _D, _D, _D, _D, _I, _I, _PI32, _PD, _PD, _PD, _PI32, _PD, _PD, _PD, _I, _U32, _U64, _I,
+            _PD, _D, _D, _PD, _PD, _PD, _PD, _PD, _PD, _PD, _PD, _I]
         L.eqr_berm_simulate.argtypes = [_D, _D, _D, _D, _D, _U32, _U64, _PD, _PD, _PD]
         L.eqr_berm_x.restype = _D
         L.eqr_berm_x.argtypes = [_D] * 6
@@ -215,6 +237,75 @@ def exposure_profile(a, b, sigma, r0, notional, fixed_rate, use_par_rate, start,
     return t[:n].copy(), ee[:n], pfe[:n]
 
 
+# --------------------------------------------------------------------------- G2++ (hull_white_2f.rs)
+# model2 = (a, b, sigma, eta, rho, r0); state (x, y) starts at (0, 0)
+def hw2_var(z, vol, tau):
+    """Own-factor variance term, hull_white_2f.rs:66-75."""
+    return lib().eqr_hw2_var(z, vol, tau)
+
+
+def hw2_cross(a, b, sigma, eta, rho, tau):
+    """Covariance term, hull_white_2f.rs:79-91."""
+    return lib().eqr_hw2_cross(a, b, sigma, eta, rho, tau)
+
+
+def zcb2(model2, x, y, t, maturity):
+    """P(t,T|x,y), hull_white_2f.rs:96-106."""
+    return lib().eqr_zcb2(*model2, x, y, t, maturity)
+
+
+def philox_normals2(seed, n_paths, n_steps):
+    """Z[2][step][path]: x shocks, then independent y shocks (Philox counter stream 1)."""
+    z = np.empty((2, n_steps, n_paths), dtype=np.float64)
+    lib().eqr_philox_normals2(seed, n_paths, n_steps, _dp(z))
+    return z
+
+
+def simulate2(model2, dt, z, scheme=0):
+    """(x, y)[step][path], hull_white_2f.rs:116-140 (scheme 0 Euler, 1 exact)."""
+    z = _f64(z)
+    _, n_steps, n_paths = z.shape
+    x = np.empty((n_steps + 1, n_paths))
+    y = np.empty((n_steps + 1, n_paths))
+    a, b, sigma, eta, rho, _r0 = model2
+    lib().eqr_simulate2(a, b, sigma, eta, rho, dt, n_steps, n_paths, scheme, _dp(z), _dp(x), _dp(y))
+    return x, y
+
+
+def par_rate2(model2, start, pay, acc):
+    """irs.rs:126-145 at state (0, 0)."""
+    p, q = _f64(pay), _f64(acc)
+    return lib().eqr_par_rate2(*model2, start, _dp(p), _dp(q), len(p))
+
+
+def irs_npv_reset2(model2, x, y, t, notional, fixed_rate, start, pay, acc):
+    p, q = _f64(pay), _f64(acc)
+    return lib().eqr_irs_npv_reset2(*model2, x, y, t, notional, fixed_rate, start, _dp(p), _dp(q), len(p))
+
+
+def irs_value2(model2, x, y, t, xf, yf, notional, fixed_rate, start, pay, acc):
+    p, q = _f64(pay), _f64(acc)
+    return lib().eqr_irs_value2(*model2, x, y, t, xf, yf, notional, fixed_rate, start, _dp(p), _dp(q), len(p))
+
+
+def unilateral_cva2(model2, times, ee, hazard_rate, recovery_rate):
+    """exposure.rs:452-466 (discount at state (0, 0))."""
+    t, e = _f64(times), _f64(ee)
+    return lib().eqr_unilateral_cva2(*model2, _dp(t), _dp(e), len(t), hazard_rate, recovery_rate)
+
+
+def exposure_profile2(model2, notional, fixed_rate, use_par_rate, start, pay, acc, times, n_steps, z):
+    """exposure.rs:305-370 on explicit draws z[2][step][path]; returns (times, ee, pfe)."""
+    p, q, t = _f64(pay), _f64(acc), _f64(times)
+    z = _f64(z)
+    n_paths = z.shape[2]
+    ee = np.zeros(len(t))
+    pfe = np.zeros(len(t))
+    n = lib().eqr_exposure_profile2(*model2, notional, fixed_rate, int(use_par_rate), start, _dp(p), _dp(q),
+                                    len(p), _dp(t), len(t), n_steps, n_paths, _dp(z), _dp(ee), _dp(pfe))
+    return t[:n].copy(), ee[:n], pfe[:n]
+
+
 # --------------------------------------------------------------------------- market curve
 def curve_zero_rate(pillars, zeros, t):
     p, z = _f64(pillars), _f64(zeros)
@@ -256,10 +347,11 @@ def portfolio_exposure(model, port, dates, n_steps, z, scheme=0, hazard_rate=0.0
                        recovery_rate=0.0, want_trade=False, want_path=False, n_threads=0):
     """Per-trade restatement of the netting-set exposure run.
 
-    ``model`` = (a, b, sigma, r0); ``port`` = dict with set_id, notional, fixed_rate, start,
-    flow_off, pay, acc (numpy arrays); ``z`` = normals [n_steps][n_paths].
+    ``model`` = (a, b, sigma, r0) for HW1F or (a, b, sigma, eta, rho, r0) for G2++; ``port`` = dict
+    with set_id, notional, fixed_rate, start, flow_off, pay, acc (numpy arrays); ``z`` = normals
+    [n_steps][n_paths] (HW1F) or [2][n_steps][n_paths] (G2++).
     """
-    a, b, sigma, r0 = model
+    two = len(model) == 6
     set_id = np.ascontiguousarray(port["set_id"], dtype=np.int32)
     flow_off = np.ascontiguousarray(port["flow_off"], dtype=np.int32)
     notional, fixed, start = _f64(port["notional"]), _f64(port["fixed_rate"]), _f64(port["start"])
@@ -268,7 +360,7 @@ def portfolio_exposure(model, port, dates, n_steps, z, scheme=0, hazard_rate=0.0
     n_sets = int(set_id.max()) + 1 if n_trades else 0
     dates = _f64(dates)
     z = _f64(z)
-    n_paths = z.shape[1]
+    n_paths = z.shape[-1]
     nd = len(dates)
     out = {
         "set_ee": np.zeros((n_sets, nd)), "set_pfe": np.zeros((n_sets, nd)),
@@ -279,8 +371,9 @@ def portfolio_exposure(model, port, dates, n_steps, z, scheme=0, hazard_rate=0.0
                    trade_cva=np.zeros(n_trades))
     if want_path:
         out.update(path_mtm=np.zeros((nd, n_trades, n_paths)), set_mtm=np.zeros((nd, n_sets, n_paths)))
-    lib().eqr_portfolio_exposure(
-        a, b, sigma, r0, n_trades, n_sets, set_id.ctypes.data_as(_PI32), _dp(notional), _dp(fixed),
+    fn = lib().eqr_portfolio_exposure2 if two else lib().eqr_portfolio_exposure
+    fn(
+        *model, n_trades, n_sets, set_id.ctypes.data_as(_PI32), _dp(notional), _dp(fixed),
         _dp(start), flow_off.ctypes.data_as(_PI32), _dp(pay), _dp(acc), _dp(dates), nd, n_steps,
         n_paths, scheme, _dp(z), hazard_rate, recovery_rate, _dp(out["set_ee"]),
         _dp(out["set_pfe"]), _dp(out["set_cva"]), _dp(out.get("trade_ee")),

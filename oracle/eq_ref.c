@@ -50,6 +50,53 @@ double eqr_zcb(double a, double b, double sigma, double r, double t, double matu
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/* Hull-White 2F (G2++): r = x + y + r0, dx = -a x dt + sigma dW1, dy = -b y dt + eta dW2,     */
+/* d<W1,W2> = rho dt, x0 = y0 = 0 (models/hull_white_2f.rs)                                    */
+/* ------------------------------------------------------------------------------------------ */
+
+/* vol^2/z^2 * [tau + (2/z)e^{-z tau} - (1/(2z))e^{-2 z tau} - 3/(2z)], hull_white_2f.rs:66-75 */
+double eqr_hw2_var(double z, double vol, double tau) {
+    double inv_z = pow(z, -1.0);
+    double e1 = exp(z * (-tau));
+    double e2 = exp(z * (-2.0 * tau));
+    double bracket = (((inv_z * 2.0) * e1 - (inv_z * 0.5) * e2) - inv_z * 1.5) + tau;
+    return ((vol * vol) / (z * z)) * bracket;
+}
+
+/* 2 rho sigma eta/(a b) * [tau + (e^{-a tau}-1)/a + (e^{-b tau}-1)/b - (e^{-(a+b)tau}-1)/(a+b)],
+ * hull_white_2f.rs:79-91 */
+double eqr_hw2_cross(double a, double b, double sigma, double eta, double rho, double tau) {
+    double apb = a + b;
+    double ta = (exp(a * (-tau)) + (-1.0)) / a;
+    double tb = (exp(b * (-tau)) + (-1.0)) / b;
+    double tab = (exp(apb * (-tau)) + (-1.0)) / apb;
+    double bracket = ((ta + tau) + tb) - tab;
+    double coef = ((sigma * eta) * (2.0 * rho)) / (a * b);
+    return coef * bracket;
+}
+
+/* P(t,T | x, y) = exp(((r0 (-tau) + V/2) - B(a,tau) x) - B(b,tau) y), hull_white_2f.rs:96-106 */
+double eqr_zcb2(double a, double b, double sigma, double eta, double rho, double r0, double x, double y,
+                double t, double maturity) {
+    double tau = maturity - t;
+    double v = (eqr_hw2_var(a, sigma, tau) + eqr_hw2_var(b, eta, tau)) + eqr_hw2_cross(a, b, sigma, eta, rho, tau);
+    double e = ((r0 * (-tau) + v * 0.5) - eqr_hw_b(a, tau) * x) - eqr_hw_b(b, tau) * y;
+    return exp(e);
+}
+
+/* The swap valuations and the portfolio run below are written once over this descriptor:
+ * nf = 1 is HW1F (b the long-run mean, state x = r), nf = 2 is G2++ (state (x, y)). */
+typedef struct {
+    int nf;
+    double a, b, sigma, eta, rho, r0;
+} eqr_model;
+
+static double zcb_m(const eqr_model* M, double x, double y, double t, double maturity) {
+    if (M->nf == 2) return eqr_zcb2(M->a, M->b, M->sigma, M->eta, M->rho, M->r0, x, y, t, maturity);
+    return eqr_zcb(M->a, M->b, M->sigma, x, t, maturity);
+}
+
+/* ------------------------------------------------------------------------------------------ */
 /* Philox4x32-10 + Box-Muller                                                                 */
 /* (no reference function: the reference's exposure path draws from burn-ndarray's global RNG,  */
 /*  exposure.rs:113-116; this counter-based stream replaces it in both the oracle and the GPU) */
@@ -80,8 +127,11 @@ void eqr_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
     out[3] = c3;
 }
 
-void eqr_philox_uniforms(uint64_t seed, uint64_t path, uint32_t pair, double* u1, double* u2) {
-    uint32_t ctr[4] = {pair, (uint32_t)path, (uint32_t)(path >> 32), 0u};
+/* Counter (pair, path lo, path hi, stream): stream 0 feeds HW1F (two consecutive steps per pair),
+ * stream 1 feeds G2++ (the x and y shocks of one step per pair). */
+static void philox_uniforms_s(uint64_t seed, uint64_t path, uint32_t pair, uint32_t stream, double* u1,
+                              double* u2) {
+    uint32_t ctr[4] = {pair, (uint32_t)path, (uint32_t)(path >> 32), stream};
     uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
     uint32_t out[4];
     eqr_philox4x32_10(ctr, key, out);
@@ -90,6 +140,10 @@ void eqr_philox_uniforms(uint64_t seed, uint64_t path, uint32_t pair, double* u1
     const double two_m53 = 1.0 / 9007199254740992.0;
     *u1 = (double)((wa >> 11) + 1u) * two_m53; /* (0, 1] */
     *u2 = (double)(wb >> 11) * two_m53;        /* [0, 1) */
+}
+
+void eqr_philox_uniforms(uint64_t seed, uint64_t path, uint32_t pair, double* u1, double* u2) {
+    philox_uniforms_s(seed, path, pair, 0u, u1, u2);
 }
 
 void eqr_philox_normals(uint64_t seed, uint64_t n_paths, uint32_t n_steps, double* z) {
@@ -102,6 +156,22 @@ void eqr_philox_normals(uint64_t seed, uint64_t n_paths, uint32_t n_steps, doubl
             double th = two_pi * u2;
             z[(uint64_t)s * n_paths + p] = rad * cos(th);
             if (s + 1 < n_steps) z[(uint64_t)(s + 1) * n_paths + p] = rad * sin(th);
+        }
+    }
+}
+
+/* z[2][n_steps][n_paths]: every x shock (cos branch of step s's pair), then every independent
+ * y shock (sin branch); the layout of the reference's two shock lists, exposure.rs:339-344. */
+void eqr_philox_normals2(uint64_t seed, uint64_t n_paths, uint32_t n_steps, double* z) {
+    const double two_pi = 6.283185307179586476925286766559;
+    for (uint64_t p = 0; p < n_paths; ++p) {
+        for (uint32_t s = 0; s < n_steps; ++s) {
+            double u1, u2;
+            philox_uniforms_s(seed, p, s, 1u, &u1, &u2);
+            double rad = sqrt(-2.0 * log(u1));
+            double th = two_pi * u2;
+            z[(uint64_t)s * n_paths + p] = rad * cos(th);
+            z[((uint64_t)n_steps + s) * n_paths + p] = rad * sin(th);
         }
     }
 }
@@ -159,24 +229,99 @@ int eqr_is_reset_date(double start, const double* pay, int m, double t) {
 }
 
 /* products/irs.rs:126-145: (P(s,s) - P(s,T_n)) / sum_i P(s,T_i)*tau_i, reduce in order. */
-double eqr_par_rate(double a, double b, double sigma, double r, double start,
-                    const double* pay, const double* acc, int m) {
-    double p_start = eqr_zcb(a, b, sigma, r, start, start);
-    double p_end = eqr_zcb(a, b, sigma, r, start, pay[m - 1]);
+static double par_rate_m(const eqr_model* M, double x, double y, double start, const double* pay,
+                         const double* acc, int m) {
+    double p_start = zcb_m(M, x, y, start, start);
+    double p_end = zcb_m(M, x, y, start, pay[m - 1]);
     double numerator = p_start - p_end;
     double den = 0.0;
     for (int i = 0; i < m; ++i) {
-        double leg = eqr_zcb(a, b, sigma, r, start, pay[i]) * acc[i];
+        double leg = zcb_m(M, x, y, start, pay[i]) * acc[i];
         den = (i == 0) ? leg : den + leg;
     }
     return numerator / den;
 }
 
+double eqr_par_rate(double a, double b, double sigma, double r, double start,
+                    const double* pay, const double* acc, int m) {
+    eqr_model M = {1, a, b, sigma, 0.0, 0.0, 0.0};
+    return par_rate_m(&M, r, 0.0, start, pay, acc, m);
+}
+
+/* irs.rs:126-145 at the initial G2++ state (0, 0) (api.rs build_irs_swap_2f). */
+double eqr_par_rate2(double a, double b, double sigma, double eta, double rho, double r0, double start,
+                     const double* pay, const double* acc, int m) {
+    eqr_model M = {2, a, b, sigma, eta, rho, r0};
+    return par_rate_m(&M, 0.0, 0.0, start, pay, acc, m);
+}
+
+/* G2++ paths of one path p, x_path/y_path [n_steps+1].
+ * Euler (hull_white_2f.rs:116-140 with kernel.rs:41-50): zc = zy*sqrt(1-rho^2) + zx*rho,
+ *   x' = (x + (-a x) dt) + sigma (zx sqrt(dt)), y' = (y + (-b y) dt) + eta (zc sqrt(dt)).
+ * Exact (no reference function): the Gaussian transition of the OU pair,
+ *   x' = x e^{-a dt} + sx zx, y' = y e^{-b dt} + sy (c zx + sqrt(1-c^2) zy),
+ *   sx^2 = sigma^2 (1-e^{-2a dt})/(2a), sy^2 = eta^2 (1-e^{-2b dt})/(2b),
+ *   c = rho sigma eta (1-e^{-(a+b)dt})/(a+b) / (sx sy). */
+static void simulate2_one(const eqr_model* M, double dt, uint32_t n_steps, uint64_t n_paths, uint64_t p,
+                          int scheme, const double* z, double* xp, double* yp) {
+    double x = 0.0, y = 0.0;
+    xp[0] = x;
+    yp[0] = y;
+    const double a = M->a, b = M->b, sigma = M->sigma, eta = M->eta, rho = M->rho;
+    if (scheme == 0) {
+        double sq = sqrt(dt), s1 = sqrt(1.0 - rho * rho);
+        for (uint32_t s = 0; s < n_steps; ++s) {
+            double zx = z[(uint64_t)s * n_paths + p];
+            double zi = z[((uint64_t)n_steps + s) * n_paths + p];
+            double zc = zi * s1 + zx * rho;
+            double drift_x = (-a) * x;
+            double drift_y = (-b) * y;
+            x = (x + drift_x * dt) + sigma * (zx * sq);
+            y = (y + drift_y * dt) + eta * (zc * sq);
+            xp[s + 1] = x;
+            yp[s + 1] = y;
+        }
+    } else {
+        double ex = exp(-(a * dt)), ey = exp(-(b * dt));
+        double sx = sigma * sqrt((1.0 - exp(-(2.0 * a * dt))) / (2.0 * a));
+        double sy = eta * sqrt((1.0 - exp(-(2.0 * b * dt))) / (2.0 * b));
+        double cov = rho * sigma * eta * (1.0 - exp(-((a + b) * dt))) / (a + b);
+        double c = (sx > 0.0 && sy > 0.0) ? cov / (sx * sy) : 0.0;
+        if (c > 1.0) c = 1.0;
+        if (c < -1.0) c = -1.0;
+        double s1 = sqrt(1.0 - c * c);
+        for (uint32_t s = 0; s < n_steps; ++s) {
+            double zx = z[(uint64_t)s * n_paths + p];
+            double zi = z[((uint64_t)n_steps + s) * n_paths + p];
+            double zc = c * zx + s1 * zi;
+            x = x * ex + sx * zx;
+            y = y * ey + sy * zc;
+            xp[s + 1] = x;
+            yp[s + 1] = y;
+        }
+    }
+}
+
+void eqr_simulate2(double a, double b, double sigma, double eta, double rho, double dt, uint32_t n_steps,
+                   uint64_t n_paths, int scheme, const double* z, double* x_out, double* y_out) {
+    eqr_model M = {2, a, b, sigma, eta, rho, 0.0};
+    double* xp = (double*)malloc(sizeof(double) * (n_steps + 1));
+    double* yp = (double*)malloc(sizeof(double) * (n_steps + 1));
+    for (uint64_t p = 0; p < n_paths; ++p) {
+        simulate2_one(&M, dt, n_steps, n_paths, p, scheme, z, xp, yp);
+        for (uint32_t s = 0; s <= n_steps; ++s) {
+            x_out[(uint64_t)s * n_paths + p] = xp[s];
+            y_out[(uint64_t)s * n_paths + p] = yp[s];
+        }
+    }
+    free(xp);
+    free(yp);
+}
+
 /* remaining_from(t) (irs.rs:163-187) followed by npv (irs.rs:46-68):
  * N*(P(t,t) - P(t,T_n)) - sum_{period start >= t-1e-9} (N*K)*(P(t,T_i)*tau_i). */
-double eqr_irs_npv_reset(double a, double b, double sigma, double r, double t, double notional,
-                         double fixed_rate, double start, const double* pay, const double* acc,
-                         int m) {
+static double npv_reset_m(const eqr_model* M, double x, double y, double t, double notional,
+                          double fixed_rate, double start, const double* pay, const double* acc, int m) {
     /* remaining_from keeps the periods whose start >= t - 1e-9 and sets start := t */
     int first = -1;
     double period_start = start;
@@ -188,16 +333,30 @@ double eqr_irs_npv_reset(double a, double b, double sigma, double r, double t, d
         period_start = pay[i];
     }
     if (first < 0) return 0.0; /* no period left: the reference would reduce an empty leg */
-    double p_start = eqr_zcb(a, b, sigma, r, t, t);
-    double p_end = eqr_zcb(a, b, sigma, r, t, pay[m - 1]);
+    double p_start = zcb_m(M, x, y, t, t);
+    double p_end = zcb_m(M, x, y, t, pay[m - 1]);
     double floating_leg = notional * (p_start - p_end);
     double fixed_leg = 0.0;
     for (int i = first; i < m; ++i) {
-        double p_i = eqr_zcb(a, b, sigma, r, t, pay[i]);
+        double p_i = zcb_m(M, x, y, t, pay[i]);
         double leg = (notional * fixed_rate) * (p_i * acc[i]);
         fixed_leg = (i == first) ? leg : fixed_leg + leg;
     }
     return floating_leg - fixed_leg;
+}
+
+double eqr_irs_npv_reset(double a, double b, double sigma, double r, double t, double notional,
+                         double fixed_rate, double start, const double* pay, const double* acc,
+                         int m) {
+    eqr_model M = {1, a, b, sigma, 0.0, 0.0, 0.0};
+    return npv_reset_m(&M, r, 0.0, t, notional, fixed_rate, start, pay, acc, m);
+}
+
+double eqr_irs_npv_reset2(double a, double b, double sigma, double eta, double rho, double r0, double x,
+                          double y, double t, double notional, double fixed_rate, double start,
+                          const double* pay, const double* acc, int m) {
+    eqr_model M = {2, a, b, sigma, eta, rho, r0};
+    return npv_reset_m(&M, x, y, t, notional, fixed_rate, start, pay, acc, m);
 }
 
 double eqr_fixing_date(double t, double start, const double* pay, int m) {
@@ -221,39 +380,54 @@ double eqr_fixing_date(double t, double start, const double* pay, int m) {
  *   T_{k-1} < t < T_k    : float = P(t,T_k)/P(T_{k-1},T_k | r_fix) - P(t,T_n)
  *   t >= T_n             : value 0
  *   value = N*float - sum_{T_i > t} (N*K)*(P(t,T_i)*tau_i). */
-double eqr_irs_value(double a, double b, double sigma, double r, double t, double r_fix,
-                     double notional, double fixed_rate, double start, const double* pay,
-                     const double* acc, int m) {
+static double value_m(const eqr_model* M, double x, double y, double t, double xf, double yf,
+                      double notional, double fixed_rate, double start, const double* pay, const double* acc,
+                      int m) {
     if (t >= pay[m - 1] - EQR_TOL) return 0.0;
     int first = 0;
     while (first < m && !(pay[first] > t + EQR_TOL)) ++first;
-    double p_end = eqr_zcb(a, b, sigma, r, t, pay[m - 1]);
+    double p_end = zcb_m(M, x, y, t, pay[m - 1]);
     double fl;
     if (fabs(t - start) < EQR_TOL) {
-        double p_start = eqr_zcb(a, b, sigma, r, t, t);
+        double p_start = zcb_m(M, x, y, t, t);
         fl = p_start - p_end;
     } else if (t < start) {
-        double p_start = eqr_zcb(a, b, sigma, r, t, start);
+        double p_start = zcb_m(M, x, y, t, start);
         fl = p_start - p_end;
     } else {
         double prev = (first == 0) ? start : pay[first - 1];
         if (fabs(t - prev) < EQR_TOL) {
-            double p_start = eqr_zcb(a, b, sigma, r, t, t);
+            double p_start = zcb_m(M, x, y, t, t);
             fl = p_start - p_end;
         } else {
-            double p_next = eqr_zcb(a, b, sigma, r, t, pay[first]);
-            double p_fix = eqr_zcb(a, b, sigma, r_fix, prev, pay[first]);
+            double p_next = zcb_m(M, x, y, t, pay[first]);
+            double p_fix = zcb_m(M, xf, yf, prev, pay[first]);
             fl = p_next / p_fix - p_end;
         }
     }
     double floating_leg = notional * fl;
     double fixed_leg = 0.0;
     for (int i = first; i < m; ++i) {
-        double p_i = eqr_zcb(a, b, sigma, r, t, pay[i]);
+        double p_i = zcb_m(M, x, y, t, pay[i]);
         double leg = (notional * fixed_rate) * (p_i * acc[i]);
         fixed_leg = (i == first) ? leg : fixed_leg + leg;
     }
     return floating_leg - fixed_leg;
+}
+
+double eqr_irs_value(double a, double b, double sigma, double r, double t, double r_fix,
+                     double notional, double fixed_rate, double start, const double* pay,
+                     const double* acc, int m) {
+    eqr_model M = {1, a, b, sigma, 0.0, 0.0, 0.0};
+    return value_m(&M, r, 0.0, t, r_fix, 0.0, notional, fixed_rate, start, pay, acc, m);
+}
+
+/* The same under G2++: state (x, y) at t, (xf, yf) at the fixing date of the period in progress. */
+double eqr_irs_value2(double a, double b, double sigma, double eta, double rho, double r0, double x, double y,
+                      double t, double xf, double yf, double notional, double fixed_rate, double start,
+                      const double* pay, const double* acc, int m) {
+    eqr_model M = {2, a, b, sigma, eta, rho, r0};
+    return value_m(&M, x, y, t, xf, yf, notional, fixed_rate, start, pay, acc, m);
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -281,19 +455,34 @@ void eqr_ee_pfe(double* exposures, uint64_t n, double* ee, double* pfe) {
 /* CVA, exposure.rs:242-275                                                                   */
 /* ------------------------------------------------------------------------------------------ */
 
-double eqr_unilateral_cva(double a, double b, double sigma, double r0, const double* times,
-                          const double* ee, int n, double hazard_rate, double recovery_rate) {
+/* The discount is the model's bond from the initial state: r0 under HW1F, (0, 0) under G2++
+ * (exposure.rs:452-466). */
+static double cva_m(const eqr_model* M, const double* times, const double* ee, int n, double hazard_rate,
+                    double recovery_rate) {
     double cva = 0.0;
     double prev_survival = 1.0;
     for (int i = 0; i < n; ++i) {
         double t = times[i];
         double survival = exp(-hazard_rate * t);
         double default_prob = prev_survival - survival;
-        double disc = eqr_zcb(a, b, sigma, r0, 0.0, t);
+        double disc = (M->nf == 2) ? zcb_m(M, 0.0, 0.0, 0.0, t) : zcb_m(M, M->r0, 0.0, 0.0, t);
         cva += (1.0 - recovery_rate) * ee[i] * default_prob * disc;
         prev_survival = survival;
     }
     return cva;
+}
+
+double eqr_unilateral_cva(double a, double b, double sigma, double r0, const double* times,
+                          const double* ee, int n, double hazard_rate, double recovery_rate) {
+    eqr_model M = {1, a, b, sigma, 0.0, 0.0, r0};
+    return cva_m(&M, times, ee, n, hazard_rate, recovery_rate);
+}
+
+double eqr_unilateral_cva2(double a, double b, double sigma, double eta, double rho, double r0,
+                           const double* times, const double* ee, int n, double hazard_rate,
+                           double recovery_rate) {
+    eqr_model M = {2, a, b, sigma, eta, rho, r0};
+    return cva_m(&M, times, ee, n, hazard_rate, recovery_rate);
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -337,6 +526,55 @@ int eqr_exposure_profile(double a, double b, double sigma, double r0, double not
     free(x);
     free(r_all);
     free(rp);
+    return n_dates;
+}
+
+/* G2++ profile of one swap, exposure.rs:305-370 with api.rs:587-611 (build_irs_swap_2f): Euler
+ * paths from (0, 0); z is [2][n_steps][n_paths]. path[0] holds n_paths zero states, so a date on
+ * grid index 0 averages n_paths equal values (unlike HW1F, whose path[0] is a single r0). */
+int eqr_exposure_profile2(double a, double b, double sigma, double eta, double rho, double r0,
+                          double notional, double fixed_rate, int use_par_rate, double start,
+                          const double* pay, const double* acc, int m, const double* times, int n_dates,
+                          uint32_t n_steps, uint64_t n_paths, const double* z, double* ee, double* pfe) {
+    eqr_model M = {2, a, b, sigma, eta, rho, r0};
+    double k = use_par_rate ? par_rate_m(&M, 0.0, 0.0, start, pay, acc, m) : fixed_rate;
+    double t_max = -INFINITY;
+    for (int i = 0; i < n_dates; ++i) t_max = fmax(t_max, times[i]);
+    if (t_max <= 0.0) {
+        double v = npv_reset_m(&M, 0.0, 0.0, times[0], notional, k, start, pay, acc, m);
+        v = fmax(v, 0.0);
+        ee[0] = v;
+        pfe[0] = v;
+        return 1;
+    }
+    double dt = t_max / (double)n_steps;
+    double* xp = (double*)malloc(sizeof(double) * (n_steps + 1));
+    double* yp = (double*)malloc(sizeof(double) * (n_steps + 1));
+    double* x_all = (double*)malloc(sizeof(double) * (n_steps + 1) * n_paths);
+    double* y_all = (double*)malloc(sizeof(double) * (n_steps + 1) * n_paths);
+    for (uint64_t p = 0; p < n_paths; ++p) {
+        simulate2_one(&M, dt, n_steps, n_paths, p, 0, z, xp, yp);
+        for (uint32_t s = 0; s <= n_steps; ++s) {
+            x_all[(uint64_t)s * n_paths + p] = xp[s];
+            y_all[(uint64_t)s * n_paths + p] = yp[s];
+        }
+    }
+    double* x = (double*)malloc(sizeof(double) * n_paths);
+    for (int d = 0; d < n_dates; ++d) {
+        double t = times[d];
+        uint64_t idx = (uint64_t)round(t / dt);
+        for (uint64_t p = 0; p < n_paths; ++p) {
+            double v = npv_reset_m(&M, x_all[idx * n_paths + p], y_all[idx * n_paths + p], t, notional, k, start,
+                                   pay, acc, m);
+            x[p] = fmax(v, 0.0);
+        }
+        eqr_ee_pfe(x, n_paths, &ee[d], &pfe[d]);
+    }
+    free(x);
+    free(y_all);
+    free(x_all);
+    free(yp);
+    free(xp);
     return n_dates;
 }
 
@@ -420,15 +658,16 @@ int eqr_max_threads(void) {
 #endif
 }
 
-int eqr_portfolio_exposure(double a, double b, double sigma, double r0, int n_trades, int n_sets,
-                           const int32_t* set_id, const double* notional,
-                           const double* fixed_rate, const double* start,
-                           const int32_t* flow_off, const double* pay, const double* acc,
-                           const double* dates, int n_dates, uint32_t n_steps, uint64_t n_paths,
-                           int scheme, const double* z, double hazard_rate, double recovery_rate,
-                           double* set_ee, double* set_pfe, double* set_cva, double* trade_ee,
-                           double* trade_pfe, double* trade_cva, double* path_mtm,
-                           double* set_mtm, int n_threads) {
+/* z is [n_steps][n_paths] under HW1F and [2][n_steps][n_paths] under G2++. A date on grid index 0
+ * has one row under HW1F (path[0] is r0 alone) and n_paths rows under G2++ (path[0] is n_paths
+ * zero states), one row when every date is <= 0 (exposure.rs:98-108, 325-337). */
+static int portfolio_m(const eqr_model* M, int n_trades, int n_sets, const int32_t* set_id,
+                       const double* notional, const double* fixed_rate, const double* start,
+                       const int32_t* flow_off, const double* pay, const double* acc, const double* dates,
+                       int n_dates, uint32_t n_steps, uint64_t n_paths, int scheme, const double* z,
+                       double hazard_rate, double recovery_rate, double* set_ee, double* set_pfe,
+                       double* set_cva, double* trade_ee, double* trade_pfe, double* trade_cva,
+                       double* path_mtm, double* set_mtm, int n_threads) {
     double t_max = -INFINITY;
     for (int k = 0; k < n_dates; ++k) t_max = fmax(t_max, dates[k]);
     int deterministic = !(t_max > 0.0);
@@ -449,26 +688,35 @@ int eqr_portfolio_exposure(double a, double b, double sigma, double r0, int n_tr
 #endif
     {
         double* rp = (double*)malloc(sizeof(double) * (steps + 1));
+        double* yp = (double*)malloc(sizeof(double) * (steps + 1));
         double* vset = (double*)malloc(sizeof(double) * (n_sets > 0 ? n_sets : 1));
 #ifdef _OPENMP
 #pragma omp for schedule(static)
 #endif
         for (int64_t pp = 0; pp < (int64_t)n_paths; ++pp) {
             uint64_t p = (uint64_t)pp;
-            simulate_one(a, b, sigma, r0, dt, steps, n_paths, p, scheme, z, rp);
+            if (M->nf == 2) {
+                simulate2_one(M, dt, steps, n_paths, p, scheme, z, rp, yp);
+            } else {
+                simulate_one(M->a, M->b, M->sigma, M->r0, dt, steps, n_paths, p, scheme, z, rp);
+                for (uint32_t s = 0; s <= steps; ++s) yp[s] = 0.0;
+            }
             for (int k = 0; k < n_dates; ++k) {
                 double t = dates[k];
-                double r = rp[idx[k]];
+                double r = rp[idx[k]], y = yp[idx[k]];
                 for (int s = 0; s < n_sets; ++s) vset[s] = 0.0;
                 for (int j = 0; j < n_trades; ++j) {
                     const double* pj = pay + flow_off[j];
                     const double* aj = acc + flow_off[j];
                     int m = flow_off[j + 1] - flow_off[j];
                     double t_fix = eqr_fixing_date(t, start[j], pj, m);
-                    double r_fix = 0.0;
-                    if (t_fix >= 0.0) r_fix = rp[deterministic ? 0 : (uint64_t)round(t_fix / dt)];
-                    double v = eqr_irs_value(a, b, sigma, r, t, r_fix, notional[j], fixed_rate[j],
-                                             start[j], pj, aj, m);
+                    double r_fix = 0.0, y_fix = 0.0;
+                    if (t_fix >= 0.0) {
+                        uint64_t gf = deterministic ? 0 : (uint64_t)round(t_fix / dt);
+                        r_fix = rp[gf];
+                        y_fix = yp[gf];
+                    }
+                    double v = value_m(M, r, y, t, r_fix, y_fix, notional[j], fixed_rate[j], start[j], pj, aj, m);
                     vset[set_id[j]] += v;
                     if (xt) xt[((size_t)j * n_dates + k) * n_paths + p] = fmax(v, 0.0);
                     if (path_mtm) path_mtm[((size_t)k * n_trades + j) * n_paths + p] = v;
@@ -480,25 +728,25 @@ int eqr_portfolio_exposure(double a, double b, double sigma, double r0, int n_tr
             }
         }
         free(vset);
+        free(yp);
         free(rp);
     }
 
     double* scratch = (double*)malloc(sizeof(double) * n_paths);
     for (int s = 0; s < n_sets; ++s) {
         for (int k = 0; k < n_dates; ++k) {
-            uint64_t n = (idx[k] == 0) ? 1 : n_paths;
+            uint64_t n = (idx[k] == 0 && (M->nf == 1 || deterministic)) ? 1 : n_paths;
             memcpy(scratch, xs + ((size_t)s * n_dates + k) * n_paths, sizeof(double) * n);
             eqr_ee_pfe(scratch, n, &set_ee[(size_t)s * n_dates + k], &set_pfe[(size_t)s * n_dates + k]);
         }
         if (set_cva)
-            set_cva[s] = eqr_unilateral_cva(a, b, sigma, r0, dates, set_ee + (size_t)s * n_dates,
-                                            n_dates, hazard_rate, recovery_rate);
+            set_cva[s] = cva_m(M, dates, set_ee + (size_t)s * n_dates, n_dates, hazard_rate, recovery_rate);
     }
     if (want_trade) {
         double* tee = (double*)malloc(sizeof(double) * (size_t)n_trades * n_dates);
         for (int j = 0; j < n_trades; ++j) {
             for (int k = 0; k < n_dates; ++k) {
-                uint64_t n = (idx[k] == 0) ? 1 : n_paths;
+                uint64_t n = (idx[k] == 0 && (M->nf == 1 || deterministic)) ? 1 : n_paths;
                 double e, q;
                 memcpy(scratch, xt + ((size_t)j * n_dates + k) * n_paths, sizeof(double) * n);
                 eqr_ee_pfe(scratch, n, &e, &q);
@@ -507,8 +755,7 @@ int eqr_portfolio_exposure(double a, double b, double sigma, double r0, int n_tr
                 if (trade_pfe) trade_pfe[(size_t)j * n_dates + k] = q;
             }
             if (trade_cva)
-                trade_cva[j] = eqr_unilateral_cva(a, b, sigma, r0, dates, tee + (size_t)j * n_dates,
-                                                  n_dates, hazard_rate, recovery_rate);
+                trade_cva[j] = cva_m(M, dates, tee + (size_t)j * n_dates, n_dates, hazard_rate, recovery_rate);
         }
         free(tee);
     }
@@ -517,6 +764,34 @@ int eqr_portfolio_exposure(double a, double b, double sigma, double r0, int n_tr
     free(xs);
     free(idx);
     return 0;
+}
+
+int eqr_portfolio_exposure(double a, double b, double sigma, double r0, int n_trades, int n_sets,
+                           const int32_t* set_id, const double* notional,
+                           const double* fixed_rate, const double* start,
+                           const int32_t* flow_off, const double* pay, const double* acc,
+                           const double* dates, int n_dates, uint32_t n_steps, uint64_t n_paths,
+                           int scheme, const double* z, double hazard_rate, double recovery_rate,
+                           double* set_ee, double* set_pfe, double* set_cva, double* trade_ee,
+                           double* trade_pfe, double* trade_cva, double* path_mtm,
+                           double* set_mtm, int n_threads) {
+    eqr_model M = {1, a, b, sigma, 0.0, 0.0, r0};
+    return portfolio_m(&M, n_trades, n_sets, set_id, notional, fixed_rate, start, flow_off, pay, acc, dates,
+                       n_dates, n_steps, n_paths, scheme, z, hazard_rate, recovery_rate, set_ee, set_pfe,
+                       set_cva, trade_ee, trade_pfe, trade_cva, path_mtm, set_mtm, n_threads);
+}
+
+int eqr_portfolio_exposure2(double a, double b, double sigma, double eta, double rho, double r0, int n_trades,
+                            int n_sets, const int32_t* set_id, const double* notional, const double* fixed_rate,
+                            const double* start, const int32_t* flow_off, const double* pay, const double* acc,
+                            const double* dates, int n_dates, uint32_t n_steps, uint64_t n_paths, int scheme,
+                            const double* z, double hazard_rate, double recovery_rate, double* set_ee,
+                            double* set_pfe, double* set_cva, double* trade_ee, double* trade_pfe,
+                            double* trade_cva, double* path_mtm, double* set_mtm, int n_threads) {
+    eqr_model M = {2, a, b, sigma, eta, rho, r0};
+    return portfolio_m(&M, n_trades, n_sets, set_id, notional, fixed_rate, start, flow_off, pay, acc, dates,
+                       n_dates, n_steps, n_paths, scheme, z, hazard_rate, recovery_rate, set_ee, set_pfe,
+                       set_cva, trade_ee, trade_pfe, trade_cva, path_mtm, set_mtm, n_threads);
 }
 
 /* ------------------------------------------------------------------------------------------ */

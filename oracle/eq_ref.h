@@ -109,6 +109,40 @@ int eqr_portfolio_exposure(double a, double b, double sigma, double r0, int n_tr
                            double* trade_pfe, double* trade_cva, double* path_mtm,
                            double* set_mtm, int n_threads);
 
+/* ---- Hull-White 2F (G2++), models/hull_white_2f.rs; state (x, y) from (0, 0) ---- */
+double eqr_hw2_var(double z, double vol, double tau);
+double eqr_hw2_cross(double a, double b, double sigma, double eta, double rho, double tau);
+double eqr_zcb2(double a, double b, double sigma, double eta, double rho, double r0, double x, double y,
+                double t, double maturity);
+/* z[2][n_steps][n_paths] from Philox counter stream 1 (x shocks, then independent y shocks). */
+void eqr_philox_normals2(uint64_t seed, uint64_t n_paths, uint32_t n_steps, double* z);
+/* scheme 0 Euler (reference), 1 exact; x_out, y_out [n_steps+1][n_paths]. */
+void eqr_simulate2(double a, double b, double sigma, double eta, double rho, double dt, uint32_t n_steps,
+                   uint64_t n_paths, int scheme, const double* z, double* x_out, double* y_out);
+double eqr_par_rate2(double a, double b, double sigma, double eta, double rho, double r0, double start,
+                     const double* pay, const double* acc, int m);
+double eqr_irs_npv_reset2(double a, double b, double sigma, double eta, double rho, double r0, double x,
+                          double y, double t, double notional, double fixed_rate, double start,
+                          const double* pay, const double* acc, int m);
+double eqr_irs_value2(double a, double b, double sigma, double eta, double rho, double r0, double x, double y,
+                      double t, double xf, double yf, double notional, double fixed_rate, double start,
+                      const double* pay, const double* acc, int m);
+double eqr_unilateral_cva2(double a, double b, double sigma, double eta, double rho, double r0,
+                           const double* times, const double* ee, int n, double hazard_rate,
+                           double recovery_rate);
+int eqr_exposure_profile2(double a, double b, double sigma, double eta, double rho, double r0,
+                          double notional, double fixed_rate, int use_par_rate, double start,
+                          const double* pay, const double* acc, int m, const double* times, int n_dates,
+                          uint32_t n_steps, uint64_t n_paths, const double* z, double* ee, double* pfe);
+/* eqr_portfolio_exposure under G2++; z is [2][n_steps][n_paths]. */
+int eqr_portfolio_exposure2(double a, double b, double sigma, double eta, double rho, double r0, int n_trades,
+                            int n_sets, const int32_t* set_id, const double* notional, const double* fixed_rate,
+                            const double* start, const int32_t* flow_off, const double* pay, const double* acc,
+                            const double* dates, int n_dates, uint32_t n_steps, uint64_t n_paths, int scheme,
+                            const double* z, double hazard_rate, double recovery_rate, double* set_ee,
+                            double* set_pfe, double* set_cva, double* trade_ee, double* trade_pfe,
+                            double* trade_cva, double* path_mtm, double* set_mtm, int n_threads);
+
 /* ---- Bermudan payer swaptions, Longstaff-Schwartz with a degree-3 basis ---- */
 void eqr_berm_simulate(double a, double b, double sigma, double r0, double dt, uint32_t n_steps,
                        uint64_t n_paths, const double* z, double* r_out, double* lb_out);

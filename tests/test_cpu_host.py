@@ -47,6 +47,12 @@ def test_host_contexts_validate_like_the_reference():
         eng.create_product("IRSwap", {"notional": 1.0, "payment_times": [1.0, 2.0], "accruals": [1.0]})
     assert sorted(eng.list_measures()) == ["DV01", "ExpectedExposure", "ExposureProfile", "Greek", "PFE95", "PV",
                                            "UnilateralCVA"]
+    # model.cpp:39-71: both short-rate models, every parameter required
+    assert sorted(eng.list_models()) == ["HullWhite1F", "HullWhite2F"]
+    m2 = eng.create_model("HullWhite2F", {"a": 0.1, "b": 0.2, "sigma": 0.01, "eta": 0.012, "rho": -0.7, "r0": 0.03})
+    assert m2.type_name == "HullWhite2F"
+    with pytest.raises((KeyError, ValueError)):
+        eng.create_model("HullWhite2F", {"a": 0.1, "b": 0.2, "sigma": 0.01, "rho": -0.7, "r0": 0.03})
 
 
 def test_cpu_backend_is_the_reference_engine_not_this_library():
