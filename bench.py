@@ -26,7 +26,7 @@ HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 EXP_FLOPS = 20            # FP64 exp as range reduction + degree-~10 FMA polynomial + scaling
 
 
-def algorithmic_work(port, dates, n_paths, n_sets):
+def algorithmic_work(port, dates, n_paths, n_sets, factors=1):
     """Algorithmic FP64 FLOPs and bytes of one value_kernel launch (the dominant kernel).
 
     Units and per-unit figures:
@@ -35,7 +35,8 @@ def algorithmic_work(port, dates, n_paths, n_sets):
                                                    floating leg (sub, + div mid-period)
       * live (set, class, path, date):   4 FLOPs   V_s += alpha F - beta A (2 FMAs)
       * live (cash-flow date, path, date): exp + 2 FLOPs   P = A exp(-B r)
-    Bytes: r read (8 B per path-date) + exposure write (8 B per set-path-date).
+                                           (G2++: exp + 4, P = exp((c - B1 x) - B2 y))
+    Bytes: state read (8 B per factor and path-date) + exposure write (8 B per set-path-date).
     """
     tol = 1e-9
     off = port.flow_off
@@ -58,22 +59,23 @@ def algorithmic_work(port, dates, n_paths, n_sets):
         rem = np.array([np.sum(f > t + tol) for f in cflows])
         u_live += float(np.sum(n_cls_trades[live]))
         per_path = (5.0 * np.sum(n_cls_trades[live]) + np.sum(2.0 * rem[live] + 2.0)
-                    + 4.0 * set_cls * np.mean(live) + (EXP_FLOPS + 2.0) * np.sum(cf_dates > t + tol))
+                    + 4.0 * set_cls * np.mean(live) + (EXP_FLOPS + 2.0 * factors) * np.sum(cf_dates > t + tol))
         flops += per_path
     flops *= n_paths
     u_live *= n_paths
-    bytes_ = 8.0 * n_paths * len(dates) * (1 + n_sets)
+    bytes_ = 8.0 * n_paths * len(dates) * (factors + n_sets)
     return flops, bytes_, u_live
 
 
 PROFILE = os.path.join(ROOT, "profiles", "value_kernel_hw1f_1000swaps_100kpaths_360dates.md")
+PROFILE2 = os.path.join(ROOT, "profiles", "value2_kernel_hw2f_1000swaps_100kpaths_360dates.md")
 
 
-def measured_traffic(kernel="value_kernel"):
+def measured_traffic(kernel="value_kernel", profile=PROFILE):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
     of the default workload (each counter scaled by what it reads for a plain 1 GiB copy)."""
     try:
-        with open(PROFILE) as f:
+        with open(profile) as f:
             in_traffic = False
             for line in f:
                 if line.startswith("## "):
@@ -94,15 +96,16 @@ def cpu_baseline(port, dates, n_steps, model, seed, disc, target_s=15.0):
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
     threads = min(threads, 16)
     # calibrate on a tiny sample, then size the timed sample to ~target_s seconds
+    normals = oracle.philox_normals2 if len(model) == 6 else oracle.philox_normals
     probe = 8
-    z = oracle.philox_normals(seed, probe, n_steps)
+    z = normals(seed, probe, n_steps)
     t0 = time.perf_counter()
     oracle.portfolio_exposure(model, port.as_dict(), dates, n_steps, z, scheme=disc, want_trade=True,
                               n_threads=threads)
     dt = time.perf_counter() - t0
     n_sample = int(max(threads, min(4096, probe * target_s / max(dt, 1e-6))))
     n_sample = max(threads, (n_sample // threads) * threads)
-    z = oracle.philox_normals(seed, n_sample, n_steps)
+    z = normals(seed, n_sample, n_steps)
     t0 = time.perf_counter()
     ref = oracle.portfolio_exposure(model, port.as_dict(), dates, n_steps, z, scheme=disc, hazard_rate=0.02,
                                     recovery_rate=0.4, want_trade=True, n_threads=threads)
@@ -318,6 +321,8 @@ def main():
     ap.add_argument("--years", type=int, default=30)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--disc", choices=["exact", "euler"], default="exact")
+    ap.add_argument("--model", choices=["hw1f", "hw2f"], default="hw1f",
+                    help="short-rate model of the swaps workload: HullWhite1F (default) or HullWhite2F (G2++)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", choices=["swaps", "batch", "adjoint", "bermudan"], default="swaps",
@@ -360,7 +365,7 @@ def main():
         return
 
     from engine_quant_amd import eqhip
-    from engine_quant_amd.portfolios import MODEL, monthly_dates, swap_portfolio, zero_curve
+    from engine_quant_amd.portfolios import MODEL, MODEL2, monthly_dates, swap_portfolio, zero_curve
 
     pillars, zeros = zero_curve(args.seed)
     port = swap_portfolio(args.trades, seed=args.seed, pillars=pillars, zeros=zeros)
@@ -368,9 +373,11 @@ def main():
     n_steps = len(dates)
     disc = eqhip.DISC_EXACT if args.disc == "exact" else eqhip.DISC_EULER
     hazard, recovery = 0.02, 0.4
+    model = MODEL2 if args.model == "hw2f" else MODEL
+    factors = 2 if args.model == "hw2f" else 1
 
     ctx = eqhip.Context(0)
-    plan = ctx.plan(MODEL, port, dates, args.paths, n_steps, seed=args.seed, rng=eqhip.RNG_PHILOX, disc=disc,
+    plan = ctx.plan(model, port, dates, args.paths, n_steps, seed=args.seed, rng=eqhip.RNG_PHILOX, disc=disc,
                     want_trade_ee=True)
 
     def step():
@@ -394,12 +401,14 @@ def main():
     units = port.n_trades * args.paths * len(dates)
     value = units / (ms_per_step / 1000.0)
 
-    flops, bytes_, u_live = algorithmic_work(port, dates, args.paths, port.n_sets)
+    flops, bytes_, u_live = algorithmic_work(port, dates, args.paths, port.n_sets, factors)
     v_ms = float(np.mean(value_ms))
     achieved = flops / (v_ms / 1000.0) / 1e12
     tm = plan.timing()
     default_workload = (args.trades, args.paths, args.years, args.seed, args.disc) == (1000, 100_000, 30, 7, "exact")
-    traffic = measured_traffic() if default_workload else None
+    kernel = "value2_kernel" if factors == 2 else "value_kernel"
+    profile = PROFILE2 if factors == 2 else PROFILE
+    traffic = measured_traffic(kernel, profile) if default_workload else None
     out = {
         "metric": METRIC,
         "value": value,
@@ -413,16 +422,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "hw1f-irs-one-netting-set", "n_trades": args.trades, "n_paths": args.paths,
+        "config": {"workload": f"{args.model}-irs-one-netting-set", "n_trades": args.trades, "n_paths": args.paths,
                    "n_dates": len(dates), "n_steps": n_steps, "dates": "monthly", "rng": "philox4x32-10",
                    "discretisation": args.disc, "seed": args.seed, "n_sets": port.n_sets,
                    "outputs": "set EE/PFE95/CVA, trade EE/CVA, PV/DV01/bucketed DV01 (20 pillars)",
                    "live_units": u_live},
-        "roofline": {"kernel": "value_kernel", "bound": "mfma", "achieved": achieved,
+        "roofline": {"kernel": kernel, "bound": "mfma", "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": traffic,
                      "compute": "fp64 on the vector ALUs (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
-                     "traffic_source": os.path.relpath(PROFILE, ROOT) if traffic else None,
+                     "traffic_source": os.path.relpath(profile, ROOT) if traffic else None,
                      "algorithmic_flops": flops, "algorithmic_bytes": bytes_,
                      "value_kernel_ms": v_ms, "exposure_kernels_ms": float(np.mean(total_kernel_ms)),
                      "paths_per_tile": tm["paths_per_tile"], "tiles_per_block": tm["tiles_per_block"],
@@ -430,10 +439,10 @@ def main():
         "set_cva": float(res["set_cva"][0]),
     }
     if not args.no_cpu:
-        rate, threads, n_sample, cpu_s, ref = cpu_baseline(port, dates, n_steps, MODEL, args.seed, disc,
+        rate, threads, n_sample, cpu_s, ref = cpu_baseline(port, dates, n_steps, model, args.seed, disc,
                                                            args.cpu_seconds)
         # the same paths on the GPU (counter-based stream: path p's draws do not depend on n_paths)
-        small = ctx.exposure_run(MODEL, port, dates, n_sample, n_steps, seed=args.seed, rng=eqhip.RNG_PHILOX,
+        small = ctx.exposure_run(model, port, dates, n_sample, n_steps, seed=args.seed, rng=eqhip.RNG_PHILOX,
                                  disc=disc, hazard_rate=hazard, recovery_rate=recovery, want_trade_ee=True)
         cva_err = abs(small["set_cva"][0] - ref["set_cva"][0]) / max(abs(ref["set_cva"][0]), 1e-300)
         out["cpu_baseline"] = {"value": rate, "unit": "trade-paths-dates/s", "cores": threads, "kind": "port",

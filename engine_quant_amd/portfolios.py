@@ -15,6 +15,7 @@ import numpy as np
 from .eqhip import Portfolio
 
 MODEL = (0.10, 0.03, 0.01, 0.02)  # HullWhite1F a, b, sigma, r0
+MODEL2 = (0.10, 0.20, 0.01, 0.012, -0.7, 0.03)  # HullWhite2F a, b, sigma, eta, rho, r0 (G2++)
 
 
 def zero_curve(seed=7, n_pillars=20):
