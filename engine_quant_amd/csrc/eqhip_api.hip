@@ -107,6 +107,8 @@ struct eqhip_plan {
     int tiles_per_block = 1, n_chunks = 1;
     uint64_t n_tiles = 0;
     int want_trade = 0;
+    bool keep_cube = false;    // the signed netting-set values stay resident in `cube` after execute
+    bool executed = false;
     int n_items = 0;
     int set_mode = 0;
     uint64_t target_units = 2048;   // work units the launch aims for (chunking of the path tiles)
@@ -119,7 +121,7 @@ struct eqhip_plan {
     DevBuf slot_of_step, R, Z, AB, date_slot, date_dfirst, date_det, fdesc, fam_off, flow_d, flow_tau,
         flow_cls, fix_off, fix_slot, fix_AB, set_off, set_nlive, set_cls, set_alpha, set_beta, X, items, cls_mat, tr_K,
         tr_absN, tr_N, tr_NK, tr_orig, item_first, partial, trade_ee, trade_cva, set_ee, set_pfe, set_cva, neff, qrank,
-        d_dates, vplan, units, rows, R2, AB2, fix_B2;
+        d_dates, vplan, units, rows, R2, AB2, fix_B2, cube, xva_pos;
     ValuePlan vp{};  // host copy of the device descriptor (per-run pointers patched at launch)
     PinnedBuf h_out; // page-locked staging of the outputs
     DevBuf cva_w;    // per-date CVA factors
@@ -222,7 +224,8 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
     P.dates.assign(dates, dates + n_dates);
     P.n_trades = pf->n_trades;
     P.S = pf->n_sets;
-    P.want_trade = want_trade ? 1 : 0;
+    P.want_trade = (want_trade & EQHIP_PLAN_TRADE_EE) ? 1 : 0;
+    P.keep_cube = (want_trade & EQHIP_PLAN_KEEP_CUBE) != 0;
     hipStream_t st = P.ctx->stream;
 
     // time grid (exposure.rs:91-116): dt = t_max / n_steps, idx = round(t / dt)
@@ -530,7 +533,7 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
     size_t free_b = 0, total_b = 0;
     HIPCHK(hipMemGetInfo(&free_b, &total_b));
     double need = 8.0 * ((all_steps ? 0.0 : (double)P.factors * P.n_slots * mc->n_paths) +
-                         (double)P.S * n_dates * mc->n_paths +
+                         (double)(P.keep_cube ? 2 : 1) * P.S * n_dates * mc->n_paths +
                          (P.want_trade ? (double)P.n_trades * n_dates * P.n_chunks : 0.0) +
                          (mc->rng == EQHIP_RNG_HOST_DRAWS ? (double)P.factors * mc->n_steps * mc->n_paths : 0.0));
     if (need > 0.9 * (double)free_b)
@@ -612,6 +615,7 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
     if (!all_steps) P.R.alloc((size_t)P.n_slots * mc->n_paths * 8);
     if (P.factors == 2) P.R2.alloc((size_t)P.n_slots * mc->n_paths * 8);
     P.X.alloc((size_t)P.S * n_dates * mc->n_paths * 8);
+    if (P.keep_cube) P.cube.alloc((size_t)n_dates * P.S * mc->n_paths * 8);
     P.set_ee.alloc((size_t)P.S * n_dates * 8);
     P.set_pfe.alloc((size_t)P.S * n_dates * 8);
     P.set_cva.alloc((size_t)P.S * 8);
@@ -719,8 +723,7 @@ void build_value_plan(eqhip_plan& P) {
 
 void launch_value(eqhip_plan& P, double* path_mtm, double* set_mtm) {
     hipStream_t st = P.ctx->stream;
-    const bool patched = path_mtm || set_mtm || P.vp.path_mtm || P.vp.set_mtm;
-    if (patched) {  // per-run debug outputs live in the descriptor
+    if (path_mtm != P.vp.path_mtm || set_mtm != P.vp.set_mtm) {  // per-run outputs live in the descriptor
         P.vp.path_mtm = path_mtm;
         P.vp.set_mtm = set_mtm;
         HIPCHK(hipMemcpy(P.vplan.p, &P.vp, sizeof(ValuePlan), hipMemcpyHostToDevice));
@@ -815,7 +818,8 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
         if (!P.want_trade) raise(EQHIP_E_INVALID, "path_mtm needs a plan created with want_trade_ee");
         dpath.alloc((size_t)P.n_dates * P.n_trades * np * 8);
     }
-    if (out->set_mtm) dset.alloc((size_t)P.n_dates * P.S * np * 8);
+    if (out->set_mtm && !P.keep_cube) dset.alloc((size_t)P.n_dates * P.S * np * 8);
+    double* set_values = P.keep_cube ? P.cube.as<double>() : dset.as<double>();
 
     HIPCHK(hipEventRecord(P.ev[0], st));
     if (P.factors == 2) {
@@ -848,7 +852,7 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
     HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(P.ev[1], st));
-    launch_value(P, dpath.as<double>(), dset.as<double>());
+    launch_value(P, dpath.as<double>(), set_values);
     HIPCHK(hipEventRecord(P.ev[2], st));
     hipLaunchKernelGGL(select_kernel, dim3((unsigned)(P.S * P.n_dates)), dim3(kSelectThreads), 0, st,
                        P.rows.as<const SelectRow>());
@@ -899,8 +903,9 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
     if (out->path_mtm)
         HIPCHK(hipMemcpyAsync(out->path_mtm, dpath.p, (size_t)P.n_dates * P.n_trades * np * 8, hipMemcpyDeviceToHost, st));
     if (out->set_mtm)
-        HIPCHK(hipMemcpyAsync(out->set_mtm, dset.p, (size_t)P.n_dates * P.S * np * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(out->set_mtm, set_values, (size_t)P.n_dates * P.S * np * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    P.executed = true;
     for (const Out& o : outs)
         if (o.dst) std::memcpy(o.dst, P.h_out.as<char>() + o.off, o.bytes);
 
@@ -914,6 +919,113 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
     P.timing.n_value_blocks = (uint64_t)P.n_dates * P.n_chunks;
     P.timing.paths_per_tile = (uint32_t)kPathsPerTile;
     P.timing.tiles_per_block = (uint32_t)P.tiles_per_block;
+}
+
+// ---------------------------------------------------------------------------------------------
+// XVA reducer over an exposure cube (quant-engine/src/xva.rs:97-246): the row work (CSA adjustment,
+// positive / negative parts, EE sums and the row quantile) on the device, then the accumulation
+// over time rows on the host in the reference's loop order.
+// ---------------------------------------------------------------------------------------------
+struct XvaRows {
+    const double* q;   // device rows of the Q cube
+    uint64_t q_stride;
+    const double* p;   // device rows of the P cube, or null
+    uint64_t p_stride;
+    uint64_t n;        // paths per row
+    int n_times;
+};
+
+void validate_xva_params(const eqhip_xva_params* x) {
+    if (!x) raise(EQHIP_E_INVALID, "xva params are null");
+    if (!(x->confidence_level >= 0.0 && x->confidence_level <= 1.0))
+        raise(EQHIP_E_INVALID, "confidence_level must be in [0,1]");
+    for (const eqhip_default_spec* d : {&x->counterparty_default, &x->own_default})
+        if (d->n_survival < 0 || (d->n_survival > 0 && !d->survival_probabilities))
+            raise(EQHIP_E_INVALID, "survival_probabilities: null array with a positive length");
+}
+
+// xva.rs:17-23
+double xva_dt(const double* times, int i) {
+    return (i == 0) ? std::max(times[i], 0.0) : std::max(times[i] - times[i - 1], 0.0);
+}
+// xva.rs:31-46
+double xva_default_increment(const eqhip_default_spec& spec, int i, const double* times) {
+    if (i < spec.n_survival) {
+        const double survival = spec.survival_probabilities[i];
+        const double previous = (i == 0) ? 1.0 : spec.survival_probabilities[i - 1];
+        return std::max(previous - survival, 0.0);
+    }
+    const double h = std::max(spec.hazard_rate, 0.0);
+    const double start = (i == 0) ? 0.0 : times[i - 1];
+    return std::exp(-(h * start)) - std::exp(-(h * std::max(times[i], start)));
+}
+double xva_clamp_recovery(double r) { return std::min(std::max(r, 0.0), 1.0); }
+
+void xva_reduce(eqhip_context* ctx, const XvaRows& rows, DevBuf& pos, const eqhip_xva_params& x, const double* times,
+                const double* discount, double* out_ee, double* out_pfe, double* out_measures) {
+#pragma clang fp contract(off)
+    hipStream_t st = ctx->stream;
+    const int nt = rows.n_times;
+    const uint64_t n = rows.n;
+    reserve(pos, (size_t)nt * n * 8);
+    DevBuf sums, dev_rows;
+    sums.alloc((size_t)nt * 4 * 8);  // ee, pfe, ene_q, ene_p
+    double* d_sums = sums.as<double>();
+    XvaRowArgs g{};
+    g.q = rows.q;
+    g.q_stride = rows.q_stride;
+    g.p = rows.p;
+    g.p_stride = rows.p_stride;
+    g.n = n;
+    g.pos = pos.as<double>();
+    g.ene_q = d_sums + 2 * nt;
+    g.ene_p = d_sums + 3 * nt;
+    const eqhip_csa* csa = x.csa;
+    g.csa_vm = (csa && csa->variation_margin) ? 1 : 0;
+    if (csa) {
+        g.threshold = std::max(csa->threshold, 0.0);
+        g.mta = std::max(csa->minimum_transfer_amount, 0.0);
+        g.ia = std::max(csa->independent_amount, 0.0);
+    }
+    hipLaunchKernelGGL(xva_rows_kernel, dim3((unsigned)nt), dim3(kSelectThreads), 0, st, g);
+    HIPCHK(hipGetLastError());
+    // the quantile of xva.rs:66-75: sorted[round((n - 1) p)] of the positive parts
+    const uint64_t rank = (uint64_t)std::llround((double)(n - 1) * x.confidence_level);
+    std::vector<SelectRow> sel(nt);
+    for (int k = 0; k < nt; ++k)
+        sel[k] = SelectRow{g.pos + (size_t)k * n, n, std::min(rank, n - 1), d_sums + k, d_sums + nt + k};
+    upload(dev_rows, sel, st);
+    hipLaunchKernelGGL(select_kernel, dim3((unsigned)nt), dim3(kSelectThreads), 0, st,
+                       dev_rows.as<const SelectRow>());
+    HIPCHK(hipGetLastError());
+    std::vector<double> h((size_t)nt * 4);
+    HIPCHK(hipMemcpyAsync(h.data(), d_sums, (size_t)nt * 4 * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const double* ee = h.data();
+    const double* pfe = h.data() + nt;
+    const double* ene_q = h.data() + 2 * nt;
+    const double* ene_p = h.data() + 3 * nt;
+    const double inv_rc = 1.0 - xva_clamp_recovery(x.counterparty_default.recovery_rate);
+    const double inv_ro = 1.0 - xva_clamp_recovery(x.own_default.recovery_rate);
+    double cva = 0.0, dva = 0.0, fva = 0.0, mva = 0.0, kva = 0.0;
+    for (int i = 0; i < nt; ++i) {
+        const double expected_positive = ee[i];
+        const double expected_negative = (rows.p ? ene_p[i] : ene_q[i]) / (double)n;
+        const double d = discount ? discount[i] : std::exp(-0.0 * times[i]);
+        const double yf = xva_dt(times, i);
+        out_ee[i] = expected_positive;
+        out_pfe[i] = pfe[i];
+        cva += d * expected_positive * xva_default_increment(x.counterparty_default, i, times) * inv_rc;
+        fva += d * expected_positive * x.funding_spread * yf;
+        mva += d * x.initial_margin_total * x.borrowing_spread * yf;
+        kva += d * expected_positive * std::max(x.capital_factor, 0.0) * yf;
+        dva += d * expected_negative * xva_default_increment(x.own_default, i, times) * inv_ro;
+    }
+    out_measures[0] = cva;
+    out_measures[1] = dva;
+    out_measures[2] = fva;
+    out_measures[3] = mva;
+    out_measures[4] = kva;
 }
 
 eqhip_context* check_ctx(eqhip_context* ctx) {
@@ -1718,6 +1830,104 @@ int eqhip_unilateral_cva_from_exposure_2f(eqhip_context* ctx, double a, double b
                                           double* out_cva) {
     return eqhip_unilateral_cva_from_exposure_2f_batch(ctx, a, b, sigma, eta, rho, r0, times, ee, 1, n_dates,
                                                        hazard_rate, recovery_rate, out_cva);
+}
+
+int eqhip_xva_calculate(eqhip_context* ctx, const eqhip_xva_params* params, const double* times, int32_t n_times,
+                        uint64_t n_paths, const double* q_paths, const double* discount_factors,
+                        const double* p_paths, double* out_ee, double* out_pfe, double* out_measures) {
+    return guarded([&] {
+        check_ctx(ctx);
+        validate_xva_params(params);
+        if (!out_ee || !out_pfe || !out_measures) raise(EQHIP_E_INVALID, "out_ee / out_pfe / out_measures are required");
+        // ExposureCubeInput::validate, quant-domain/src/xva.rs:187-220
+        if (n_times < 1 || !times || !q_paths)
+            raise(EQHIP_E_INVALID, "exposure times and rows must be non-empty and have equal length");
+        for (int i = 0; i < n_times; ++i)
+            if (!std::isfinite(times[i]) || (i > 0 && !(times[i] > times[i - 1])) || (i == 0 && times[0] < 0.0))
+                raise(EQHIP_E_INVALID, "exposure times must be finite and strictly increasing");
+        if (n_paths < 1) raise(EQHIP_E_INVALID, "exposure paths must be rectangular finite rows");
+        const size_t cells = (size_t)n_times * n_paths;
+        for (const double* c : {q_paths, p_paths})
+            if (c)
+                for (size_t i = 0; i < cells; ++i)
+                    if (!std::isfinite(c[i])) raise(EQHIP_E_INVALID, "exposure paths must be rectangular finite rows");
+        if (discount_factors)
+            for (int i = 0; i < n_times; ++i)
+                if (!std::isfinite(discount_factors[i]) || discount_factors[i] < 0.0)
+                    raise(EQHIP_E_INVALID, "discount_factors must match times and be non-negative");
+        hipStream_t st = ctx->stream;
+        DevBuf dq, dp, pos;
+        dq.alloc(cells * 8);
+        HIPCHK(hipMemcpyAsync(dq.p, q_paths, cells * 8, hipMemcpyHostToDevice, st));
+        if (p_paths) {
+            dp.alloc(cells * 8);
+            HIPCHK(hipMemcpyAsync(dp.p, p_paths, cells * 8, hipMemcpyHostToDevice, st));
+        }
+        XvaRows rows{dq.as<double>(), n_paths, dp.as<double>(), n_paths, n_paths, n_times};
+        xva_reduce(ctx, rows, pos, *params, times, discount_factors, out_ee, out_pfe, out_measures);
+    });
+}
+
+int eqhip_plan_exposure_cube(eqhip_plan* plan, int32_t set, double* out_paths, double* out_discount) {
+    return guarded([&] {
+        if (!plan) raise(EQHIP_E_INVALID, "plan is null");
+        eqhip_plan& P = *plan;
+        if (!P.keep_cube) raise(EQHIP_E_INVALID, "the plan was created without EQHIP_PLAN_KEEP_CUBE");
+        if (!P.executed) raise(EQHIP_E_INVALID, "the plan has not been executed");
+        if (set < 0 || set >= P.S) raise(EQHIP_E_INVALID, "set index out of range");
+        if (!out_paths) raise(EQHIP_E_INVALID, "out_paths is null");
+        hipStream_t st = P.ctx->stream;
+        HIPCHK(hipSetDevice(P.ctx->device));
+        const uint64_t np = P.mc.n_paths;
+        // rows (date k, set) of the [date][set][path] cube, one strided copy
+        HIPCHK(hipMemcpy2DAsync(out_paths, np * 8, P.cube.as<double>() + (size_t)set * np, (size_t)P.S * np * 8, np * 8,
+                                P.n_dates, hipMemcpyDeviceToHost, st));
+        if (out_discount) {
+            DevBuf w;
+            w.alloc((size_t)P.n_dates * 16);
+            if (P.factors == 2)
+                launch_cva_weights2(st, P.d_dates.as<double>(), P.n_dates, P.model2, 0.0, w.as<double2>());
+            else
+                launch_cva_weights(st, P.d_dates.as<double>(), P.n_dates, P.model, 0.0, w.as<double2>());
+            std::vector<double2> hw(P.n_dates);
+            HIPCHK(hipMemcpyAsync(hw.data(), w.p, (size_t)P.n_dates * 16, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            for (int k = 0; k < P.n_dates; ++k) out_discount[k] = hw[k].y;
+        }
+        HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
+int eqhip_plan_xva(eqhip_plan* plan, int32_t set, const eqhip_xva_params* params, double* out_ee, double* out_pfe,
+                   double* out_measures) {
+    return guarded([&] {
+        if (!plan) raise(EQHIP_E_INVALID, "plan is null");
+        eqhip_plan& P = *plan;
+        if (!P.keep_cube) raise(EQHIP_E_INVALID, "the plan was created without EQHIP_PLAN_KEEP_CUBE");
+        if (!P.executed) raise(EQHIP_E_INVALID, "the plan has not been executed");
+        if (set < 0 || set >= P.S) raise(EQHIP_E_INVALID, "set index out of range");
+        validate_xva_params(params);
+        if (!out_ee || !out_pfe || !out_measures) raise(EQHIP_E_INVALID, "out_ee / out_pfe / out_measures are required");
+        for (int k = 0; k < P.n_dates; ++k)
+            if (!(P.dates[k] >= 0.0) || (k > 0 && !(P.dates[k] > P.dates[k - 1])))
+                raise(EQHIP_E_INVALID, "exposure times must be finite and strictly increasing");
+        hipStream_t st = P.ctx->stream;
+        HIPCHK(hipSetDevice(P.ctx->device));
+        const uint64_t np = P.mc.n_paths;
+        // the model discount P(0, t_k) of the plan's own CVA
+        reserve(P.cva_w, (size_t)P.n_dates * 16);
+        if (P.factors == 2)
+            launch_cva_weights2(st, P.d_dates.as<double>(), P.n_dates, P.model2, 0.0, P.cva_w.as<double2>());
+        else
+            launch_cva_weights(st, P.d_dates.as<double>(), P.n_dates, P.model, 0.0, P.cva_w.as<double2>());
+        std::vector<double2> hw(P.n_dates);
+        HIPCHK(hipMemcpyAsync(hw.data(), P.cva_w.p, (size_t)P.n_dates * 16, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        std::vector<double> disc(P.n_dates);
+        for (int k = 0; k < P.n_dates; ++k) disc[k] = hw[k].y;
+        XvaRows rows{P.cube.as<double>() + (size_t)set * np, (uint64_t)P.S * np, nullptr, 0, np, P.n_dates};
+        xva_reduce(P.ctx, rows, P.xva_pos, *params, P.dates.data(), disc.data(), out_ee, out_pfe, out_measures);
+    });
 }
 
 int eqhip_pv_dv01(eqhip_context* ctx, const double* pillars, const double* zero_rates, int32_t n_pillars,

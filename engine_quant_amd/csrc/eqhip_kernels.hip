@@ -934,6 +934,86 @@ __global__ void cva_weights_kernel(const double* times, int n_dates, double a, d
     w[k] = make_double2(default_prob, disc);
 }
 
+// ---------------------------------------------------------------------------------------------
+// XVA reducer rows (quant-engine/src/xva.rs:48-64, 150-183). One block per time row; the values are
+// read once (eight loads in flight per thread), the positive parts of the adjusted Q values are
+// written for the row's quantile, and the negative parts are summed per thread in index order, then
+// across threads by a fixed tree.
+// ---------------------------------------------------------------------------------------------
+namespace {
+__device__ __forceinline__ double xva_adjust(double v, const XvaRowArgs& g) {
+    if (!g.csa_vm) return v;
+    const double excess = fabs(v) - g.threshold;
+    const double collateral = (excess > 0.0) ? excess : 0.0;
+    if (collateral < g.mta) return v;
+    return (v >= 0.0) ? (v - collateral) - g.ia : (v + collateral) + g.ia;
+}
+__device__ __forceinline__ double block_sum(double x, double* red) {
+    const int tid = threadIdx.x;
+    red[tid] = x;
+    __syncthreads();
+    for (int w = kSelectThreads / 2; w > 0; w >>= 1) {
+        if (tid < w) red[tid] = red[tid] + red[tid + w];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+}  // namespace
+
+__global__ void __launch_bounds__(kSelectThreads) xva_rows_kernel(XvaRowArgs g) {
+#pragma clang fp contract(off)
+    __shared__ double red[kSelectThreads];
+    const int tid = threadIdx.x;
+    const uint64_t k = blockIdx.x;
+    const uint64_t n = g.n;
+    const gptr<double> q = glb(g.q + k * g.q_stride);
+    const gmut<double> pos = glb(g.pos + k * n);
+    constexpr int U = 8;
+    double neg_q = 0.0;
+    uint64_t i = tid;
+    for (; i + (uint64_t)(U - 1) * kSelectThreads < n; i += (uint64_t)U * kSelectThreads) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = q[i + (uint64_t)u * kSelectThreads];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const double a = xva_adjust(v[u], g);
+            pos[i + (uint64_t)u * kSelectThreads] = (a > 0.0) ? a : 0.0;
+            neg_q += (a < 0.0) ? -a : 0.0;
+        }
+    }
+    for (; i < n; i += kSelectThreads) {
+        const double a = xva_adjust(q[i], g);
+        pos[i] = (a > 0.0) ? a : 0.0;
+        neg_q += (a < 0.0) ? -a : 0.0;
+    }
+    const double tq = block_sum(neg_q, red);
+    if (tid == 0) g.ene_q[k] = tq;
+    if (g.p) {
+        const gptr<double> pr = glb(g.p + k * g.p_stride);
+        double neg_p = 0.0;
+        uint64_t j = tid;
+        for (; j + (uint64_t)(U - 1) * kSelectThreads < n; j += (uint64_t)U * kSelectThreads) {
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = pr[j + (uint64_t)u * kSelectThreads];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const double a = xva_adjust(v[u], g);
+                neg_p += (a < 0.0) ? -a : 0.0;
+            }
+        }
+        for (; j < n; j += kSelectThreads) {
+            const double a = xva_adjust(pr[j], g);
+            neg_p += (a < 0.0) ? -a : 0.0;
+        }
+        const double tp = block_sum(neg_p, red);
+        if (tid == 0) g.ene_p[k] = tp;
+    }
+}
+
 // G2++ CVA factors: the discount is the model bond at the initial state (0, 0),
 // exposure.rs:470-479 (unilateral_cva_2f), i.e. exp(c(t)).
 __global__ void cva_weights2_kernel(const double* times, int n_dates, double a, double b, double sigma, double eta,

@@ -196,6 +196,23 @@ struct SelectRow {
     double* pfe;
 };
 __global__ void select_kernel(const SelectRow* rows);
+
+// One time row of an exposure cube for the XVA reducer (quant-engine/src/xva.rs:150-183): the row's
+// values after the first CSA's variation margin, their positive parts (written out for the quantile)
+// and the fixed-order sums of their negative parts, for the Q row and, when given, the P row.
+struct XvaRowArgs {
+    const double* q;     // row k at q + k * q_stride, n values
+    uint64_t q_stride;
+    const double* p;     // the P cube's rows (nullable), same shape
+    uint64_t p_stride;
+    uint64_t n;
+    double* pos;         // [rows][n] positive parts of the adjusted Q rows
+    double* ene_q;       // [rows] sum of negative parts of the adjusted Q row
+    double* ene_p;       // [rows] the same for the P row (when p)
+    int csa_vm;          // 1: a CSA with variation margin adjusts the values
+    double threshold, mta, ia;  // max(x, 0) of the CSA's threshold, minimum transfer, independent amount
+};
+__global__ void xva_rows_kernel(XvaRowArgs g);
 __global__ void trade_ee_kernel(const double* partial, int n_trades, int n_dates, int n_chunks,
                                 const uint64_t* n_eff, double* trade_ee);
 __global__ void cva_weights_kernel(const double* times, int n_dates, double a, double b, double sigma,

@@ -37,7 +37,10 @@ EXPORTED = [
     "eqhip_berm_adam_step", "eqhip_berm_get_timing", "eqhip_berm_destroy",
     "eqhip_irs_hull_white_2f_exposure_profile", "eqhip_irs_hull_white_2f_exposure_profile_batch",
     "eqhip_unilateral_cva_from_exposure_2f", "eqhip_unilateral_cva_from_exposure_2f_batch", "eqhip_plan_create_2f",
+    "eqhip_xva_calculate", "eqhip_plan_exposure_cube", "eqhip_plan_xva",
 ]
+
+PLAN_TRADE_EE, PLAN_KEEP_CUBE = 1, 2  # flags of eqhip_plan_create's want_trade_ee argument
 
 
 class EqhipError(RuntimeError):
@@ -71,6 +74,21 @@ class CreditC(ctypes.Structure):
 class OutputsC(ctypes.Structure):
     _fields_ = [("set_ee", _PD), ("set_pfe", _PD), ("set_cva", _PD), ("trade_ee", _PD),
                 ("trade_cva", _PD), ("path_mtm", _PD), ("set_mtm", _PD)]
+
+
+class CsaC(ctypes.Structure):
+    _fields_ = [("threshold", _D), ("minimum_transfer_amount", _D), ("independent_amount", _D),
+                ("variation_margin", _I32)]
+
+
+class DefaultSpecC(ctypes.Structure):
+    _fields_ = [("hazard_rate", _D), ("recovery_rate", _D), ("survival_probabilities", _PD), ("n_survival", _I32)]
+
+
+class XvaParamsC(ctypes.Structure):
+    _fields_ = [("counterparty_default", DefaultSpecC), ("own_default", DefaultSpecC), ("funding_spread", _D),
+                ("borrowing_spread", _D), ("capital_factor", _D), ("initial_margin_total", _D),
+                ("csa", ctypes.POINTER(CsaC)), ("confidence_level", _D)]
 
 
 class TimingC(ctypes.Structure):
@@ -107,6 +125,10 @@ def lib():
         L.eqhip_unilateral_cva_from_exposure_2f_batch.argtypes = prof2 + [_PD, _PD, _I32, _I32, _D, _D, _PD]
         L.eqhip_plan_create_2f.argtypes = [ctypes.c_void_p, ctypes.POINTER(ModelHW2F), ctypes.POINTER(PortfolioC),
                                            _PD, _I32, ctypes.POINTER(McC), _I32, ctypes.POINTER(ctypes.c_void_p)]
+        L.eqhip_xva_calculate.argtypes = [ctypes.c_void_p, ctypes.POINTER(XvaParamsC), _PD, _I32, _U64, _PD, _PD,
+                                          _PD, _PD, _PD, _PD]
+        L.eqhip_plan_exposure_cube.argtypes = [ctypes.c_void_p, _I32, _PD, _PD]
+        L.eqhip_plan_xva.argtypes = [ctypes.c_void_p, _I32, ctypes.POINTER(XvaParamsC), _PD, _PD, _PD]
         L.eqhip_pv_dv01.argtypes = [ctypes.c_void_p, _PD, _PD, _I32, ctypes.POINTER(PortfolioC), _D, _PD,
                                     _PD, _PD]
         L.eqhip_plan_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(ModelHW1F),
@@ -197,21 +219,23 @@ class Plan:
     """
 
     def __init__(self, ctx, model, portfolio: Portfolio, dates, n_paths, n_steps, seed=0,
-                 rng=RNG_PHILOX, disc=DISC_EULER, want_trade_ee=False):
+                 rng=RNG_PHILOX, disc=DISC_EULER, want_trade_ee=False, keep_cube=False):
         self.ctx = ctx
         self.portfolio = portfolio
         self.dates = _f64(dates)
         self.n_paths, self.n_steps = int(n_paths), int(n_steps)
         self.rng = rng
         self.want_trade_ee = bool(want_trade_ee)
+        self.keep_cube = bool(keep_cube)
         self._pc = portfolio._c()
         self.factors = 2 if len(model) == 6 else 1
         m = ModelHW2F(*model) if self.factors == 2 else ModelHW1F(*model)
         create = lib().eqhip_plan_create_2f if self.factors == 2 else lib().eqhip_plan_create
         mc = McC(self.n_paths, self.n_steps, int(seed), rng, disc)
         h = ctypes.c_void_p()
+        flags = (PLAN_TRADE_EE if want_trade_ee else 0) | (PLAN_KEEP_CUBE if keep_cube else 0)
         _check(create(ctx.handle, ctypes.byref(m), ctypes.byref(self._pc), _dp(self.dates), len(self.dates),
-                      ctypes.byref(mc), int(want_trade_ee), ctypes.byref(h)))
+                      ctypes.byref(mc), flags, ctypes.byref(h)))
         self.handle = h
 
     def execute(self, host_draws=None, hazard_rate=0.0, recovery_rate=0.0, want_cva=True,
@@ -251,6 +275,21 @@ class Plan:
         _check(lib().eqhip_cva_adjoint(self.handle, _dp(z), set_index, ctypes.byref(cr), ctypes.byref(cva),
                                        _dp(grad), ctypes.byref(check)))
         return cva.value, dict(zip(("a", "b", "sigma", "r0", "hazard_rate", "recovery_rate"), grad)), check.value
+
+    def exposure_cube(self, set_index=0):
+        """Signed netting-set values [n_dates][n_paths] of the last execute and the model discount
+        factors P(0, t_k) (eqhip_plan_exposure_cube; the plan needs keep_cube=True)."""
+        values = np.zeros((len(self.dates), self.n_paths))
+        df = np.zeros(len(self.dates))
+        _check(lib().eqhip_plan_exposure_cube(self.handle, set_index, _dp(values), _dp(df)))
+        return values, df
+
+    def xva(self, params, set_index=0):
+        """XVA reductions of the resident cube of one netting set (eqhip_plan_xva): (ee, pfe, measures[5])."""
+        nd = len(self.dates)
+        ee, pfe, m = np.zeros(nd), np.zeros(nd), np.zeros(5)
+        _check(lib().eqhip_plan_xva(self.handle, set_index, ctypes.byref(params), _dp(ee), _dp(pfe), _dp(m)))
+        return ee, pfe, m
 
     def timing(self):
         t = TimingC()
@@ -382,6 +421,23 @@ class Context:
                                                                  _dp(times), _dp(ee), ee.shape[0], len(times),
                                                                  hazard_rate, recovery_rate, _dp(out)))
         return out
+
+    def xva_calculate(self, params, times, q_paths, discount_factors=None, p_paths=None):
+        """eqhip_xva_calculate on host cubes [n_times][n_paths]: (ee, pfe, measures[5])."""
+        times, q = _f64(times), np.ascontiguousarray(_f64(q_paths))
+        if q.ndim != 2 or q.shape[0] != len(times):
+            raise ValueError("exposure times and rows must be non-empty and have equal length")
+        p = None if p_paths is None else np.ascontiguousarray(_f64(p_paths))
+        if p is not None and p.shape != q.shape:
+            raise ValueError("Q and P exposure grids must match")
+        df = None if discount_factors is None or len(discount_factors) == 0 else _f64(discount_factors)
+        if df is not None and len(df) != len(times):
+            raise ValueError("discount_factors must match times and be non-negative")
+        nt = len(times)
+        ee, pfe, m = np.zeros(nt), np.zeros(nt), np.zeros(5)
+        _check(lib().eqhip_xva_calculate(self.handle, ctypes.byref(params), _dp(times), nt, q.shape[1], _dp(q),
+                                         _dp(df), _dp(p), _dp(ee), _dp(pfe), _dp(m)))
+        return ee, pfe, m
 
     def pv_dv01(self, pillars, zero_rates, portfolio: Portfolio, bump=1e-4, bucketed=True):
         pillars, zero_rates = _f64(pillars), _f64(zero_rates)

@@ -35,6 +35,11 @@ extern "C" {
 #define EQHIP_DISC_EULER 0 /* Euler-Maruyama, as kernel.rs:41-50 (reference semantics) */
 #define EQHIP_DISC_EXACT 1 /* exact Gaussian transition of the Vasicek/HW1F short rate */
 
+/* Flags of eqhip_plan_create's want_trade_ee argument (a plain 0 / 1 keeps its old meaning). */
+#define EQHIP_PLAN_TRADE_EE 1  /* per-trade EE / CVA outputs */
+#define EQHIP_PLAN_KEEP_CUBE 2 /* keep the signed netting-set values [date][set][path] resident
+                                  after each execute (eqhip_plan_exposure_cube, eqhip_plan_xva) */
+
 typedef struct eqhip_context eqhip_context;
 typedef struct eqhip_plan eqhip_plan;
 
@@ -251,6 +256,55 @@ int eqhip_exposure_run(eqhip_context* ctx, const eqhip_model_hw1f* model,
                        const eqhip_portfolio* portfolio, const double* dates, int32_t n_dates,
                        const eqhip_mc* mc, const double* host_draws, const eqhip_credit* credit,
                        eqhip_outputs* out);
+
+/* ---- exposure cube and the XVA reducer -------------------------------------------------------- */
+
+/* The first collateral agreement of an XVA request (quant-domain/src/xva.rs:55-75); the reducer
+ * adjusts values by its variation margin (quant-engine/src/xva.rs:48-64). */
+typedef struct {
+    double threshold, minimum_transfer_amount, independent_amount;
+    int32_t variation_margin;
+} eqhip_csa;
+
+/* DefaultSpec (quant-domain/src/xva.rs:97-121): survival_probabilities, when given, override the
+ * flat hazard rate at the dates they cover. */
+typedef struct {
+    double hazard_rate, recovery_rate;
+    const double* survival_probabilities;
+    int32_t n_survival;
+} eqhip_default_spec;
+
+/* The numeric part of an XvaRequest (quant-domain/src/xva.rs:234-262). initial_margin_total is the
+ * sum over the request's CSAs of max(initial_margin, 0); csa is the first CSA or NULL. */
+typedef struct {
+    eqhip_default_spec counterparty_default, own_default;
+    double funding_spread, borrowing_spread, capital_factor;
+    double initial_margin_total;
+    const eqhip_csa* csa;
+    double confidence_level;
+} eqhip_xva_params;
+
+/* Replaces the reductions of quant-engine xva::calculate (quant-engine/src/xva.rs:97-246) over an
+ * ExposureCubeInput (quant-domain/src/xva.rs:176-220): q_paths (and p_paths, nullable, the P cube for
+ * DVA) are row-major [n_times][n_paths]; discount_factors (nullable) [n_times]. Writes EE and the
+ * confidence quantile of the positive exposure per time (out_ee, out_pfe [n_times]) and
+ * out_measures[5] = {cva, dva, fva, mva, kva}. The Q/P tags, source hashes and the artifact record
+ * of the request stay with the caller. */
+int eqhip_xva_calculate(eqhip_context* ctx, const eqhip_xva_params* params, const double* times,
+                        int32_t n_times, uint64_t n_paths, const double* q_paths,
+                        const double* discount_factors, const double* p_paths, double* out_ee,
+                        double* out_pfe, double* out_measures);
+
+/* The exposure cube of netting set `set` from the plan's last execute, in the reducer's layout: the
+ * signed netting-set values [n_dates][n_paths] and the model discount factors P(0, t_k) (nullable).
+ * This is the Monte Carlo cube that replaces the flat one-path PV cube of
+ * quant-engine/src/lib.rs:751-779. Needs a plan created with EQHIP_PLAN_KEEP_CUBE. */
+int eqhip_plan_exposure_cube(eqhip_plan* plan, int32_t set, double* out_paths, double* out_discount);
+
+/* eqhip_xva_calculate on the plan's resident cube of `set` (Q measure, DVA from the same cube,
+ * discount factors P(0, t_k) of the model): the cube never leaves the device. */
+int eqhip_plan_xva(eqhip_plan* plan, int32_t set, const eqhip_xva_params* params, double* out_ee,
+                   double* out_pfe, double* out_measures);
 
 /* ---- sensitivities ---------------------------------------------------------------------------- */
 

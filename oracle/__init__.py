@@ -81,6 +81,10 @@ def lib():
             _D, _D, _D, _D, _I, _I, _PI32, _PD, _PD, _PD, _PI32, _PD, _PD, _PD, _I, _U32, _U64, _I,
             _PD, _D, _D, _PD, _PD, _PD, _PD, _PD, _PD, _PD, _PD, _I]
         L.eqr_max_threads.restype = _I
+        L.eqr_collateral_adjust.restype = _D
+        L.eqr_collateral_adjust.argtypes = [_D, _I, _I, _D, _D, _D]
+        L.eqr_xva_calculate.argtypes = [_PD, _I, _U64, _PD, _PD, _PD, _D, _D, _PD, _I, _D, _D, _PD, _I, _D, _D, _D,
+                                        _D, _I, _I, _D, _D, _D, _D, _PD, _PD, _PD]
         L.eqr_hw2_var.restype = _D
         L.eqr_hw2_var.argtypes = [_D] * 3
         L.eqr_hw2_cross.restype = _D
@@ -304,6 +308,36 @@ def exposure_profile2(model2, notional, fixed_rate, use_par_rate, start, pay, ac
     n = lib().eqr_exposure_profile2(*model2, notional, fixed_rate, int(use_par_rate), start, _dp(p), _dp(q),
                                     len(p), _dp(t), len(t), n_steps, n_paths, _dp(z), _dp(ee), _dp(pfe))
     return t[:n].copy(), ee[:n], pfe[:n]
+
+
+# --------------------------------------------------------------------------- XVA reducer
+def collateral_adjust(value, csa=None):
+    """quant-engine/src/xva.rs:48-64; csa = dict(threshold, minimum_transfer_amount, independent_amount,
+    variation_margin) or None."""
+    c = csa or {}
+    return lib().eqr_collateral_adjust(value, int(csa is not None), int(c.get("variation_margin", True)),
+                                       c.get("threshold", 0.0), c.get("minimum_transfer_amount", 0.0),
+                                       c.get("independent_amount", 0.0))
+
+
+def xva_calculate(times, q, df=None, p=None, counterparty=(0.0, 0.4, ()), own=(0.0, 0.4, ()), funding_spread=0.0,
+                  borrowing_spread=0.0, capital_factor=0.08, initial_margin_total=0.0, csa=None, confidence=0.95):
+    """quant-engine/src/xva.rs:97-183 on q (and p) [time][path]; counterparty / own = (hazard, recovery,
+    survival probabilities). Returns (ee, pfe, {cva, dva, fva, mva, kva})."""
+    t, qq = _f64(times), _f64(q)
+    n_times, n_paths = qq.shape
+    d = None if df is None else _f64(df)
+    pp = None if p is None else _f64(p)
+    csp, osp = _f64(counterparty[2]), _f64(own[2])
+    c = csa or {}
+    ee, pfe, m = np.zeros(n_times), np.zeros(n_times), np.zeros(5)
+    lib().eqr_xva_calculate(_dp(t), n_times, n_paths, _dp(qq), _dp(d), _dp(pp), counterparty[0], counterparty[1],
+                            _dp(csp) if len(csp) else None, len(csp), own[0], own[1], _dp(osp) if len(osp) else None,
+                            len(osp), funding_spread, borrowing_spread, capital_factor, initial_margin_total,
+                            int(csa is not None), int(c.get("variation_margin", True)), c.get("threshold", 0.0),
+                            c.get("minimum_transfer_amount", 0.0), c.get("independent_amount", 0.0), confidence,
+                            _dp(ee), _dp(pfe), _dp(m))
+    return ee, pfe, dict(zip(("cva", "dva", "fva", "mva", "kva"), m))
 
 
 # --------------------------------------------------------------------------- market curve

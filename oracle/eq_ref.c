@@ -795,6 +795,91 @@ int eqr_portfolio_exposure2(double a, double b, double sigma, double eta, double
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/* XVA reducer over an exposure cube, quant-engine/src/xva.rs:14-246                           */
+/* ------------------------------------------------------------------------------------------ */
+
+/* xva.rs:48-64 */
+double eqr_collateral_adjust(double value, int has_csa, int variation_margin, double threshold, double mta,
+                             double ia) {
+    if (!has_csa || !variation_margin) return value;
+    double th = fmax(threshold, 0.0);
+    double collateral = fmax(fabs(value) - th, 0.0);
+    if (collateral < fmax(mta, 0.0)) return value;
+    if (value >= 0.0) return value - collateral - fmax(ia, 0.0);
+    return value + collateral + fmax(ia, 0.0);
+}
+
+/* xva.rs:31-46 */
+static double xva_default_increment(double hazard, const double* sp, int n_sp, int i, const double* times) {
+    if (i < n_sp) {
+        double previous = (i == 0) ? 1.0 : sp[i - 1];
+        return fmax(previous - sp[i], 0.0);
+    }
+    double h = fmax(hazard, 0.0);
+    double start = (i == 0) ? 0.0 : times[i - 1];
+    return exp(-(h * start)) - exp(-(h * fmax(times[i], start)));
+}
+
+static double clamp01(double r) { return fmin(fmax(r, 0.0), 1.0); }
+
+/* xva.rs:97-183: per time row, the CSA-adjusted values' positive and negative parts, their means
+ * (sums in path order), the quantile sorted[round((n-1) p)] of the positive parts (xva.rs:66-75),
+ * and the running CVA / DVA / FVA / MVA / KVA sums in row order. q, p: [n_times][n_paths], p nullable;
+ * df nullable (then exp(-0 t) = 1); out_measures[5] = cva, dva, fva, mva, kva. */
+void eqr_xva_calculate(const double* times, int n_times, uint64_t n_paths, const double* q, const double* df,
+                       const double* p, double cp_hazard, double cp_recovery, const double* cp_sp, int cp_nsp,
+                       double own_hazard, double own_recovery, const double* own_sp, int own_nsp,
+                       double funding_spread, double borrowing_spread, double capital_factor,
+                       double initial_margin_total, int has_csa, int variation_margin, double threshold,
+                       double mta, double ia, double confidence, double* out_ee, double* out_pfe,
+                       double* out_measures) {
+    double* pos = (double*)malloc(sizeof(double) * n_paths);
+    double cva = 0.0, dva = 0.0, fva = 0.0, mva = 0.0, kva = 0.0;
+    double pc = fmin(fmax(confidence, 0.0), 1.0);
+    uint64_t index = (uint64_t)round((double)(n_paths - 1) * pc);
+    for (int i = 0; i < n_times; ++i) {
+        const double* row = q + (size_t)i * n_paths;
+        double sp = 0.0, sn = 0.0;
+        for (uint64_t j = 0; j < n_paths; ++j) {
+            double a = eqr_collateral_adjust(row[j], has_csa, variation_margin, threshold, mta, ia);
+            pos[j] = fmax(a, 0.0);
+            sp += pos[j];
+            sn += fmax(-a, 0.0);
+        }
+        double expected_positive = sp / (double)n_paths;
+        double expected_negative = sn / (double)n_paths;
+        qsort(pos, (size_t)n_paths, sizeof(double), cmp_double);
+        double discount = df ? df[i] : exp(-0.0 * times[i]);
+        out_ee[i] = expected_positive;
+        out_pfe[i] = pos[index];
+        double yf = (i == 0) ? fmax(times[i], 0.0) : fmax(times[i] - times[i - 1], 0.0);
+        cva += discount * expected_positive * xva_default_increment(cp_hazard, cp_sp, cp_nsp, i, times) *
+               (1.0 - clamp01(cp_recovery));
+        fva += discount * expected_positive * funding_spread * yf;
+        mva += discount * initial_margin_total * borrowing_spread * yf;
+        kva += discount * expected_positive * fmax(capital_factor, 0.0) * yf;
+        if (p) {
+            const double* prow = p + (size_t)i * n_paths;
+            double snp = 0.0;
+            for (uint64_t j = 0; j < n_paths; ++j)
+                snp += fmax(-eqr_collateral_adjust(prow[j], has_csa, variation_margin, threshold, mta, ia), 0.0);
+            double expected_negative_p = snp / (double)n_paths;
+            dva += discount * expected_negative_p * xva_default_increment(own_hazard, own_sp, own_nsp, i, times) *
+                   (1.0 - clamp01(own_recovery));
+        } else {
+            dva += discount * expected_negative * xva_default_increment(own_hazard, own_sp, own_nsp, i, times) *
+                   (1.0 - clamp01(own_recovery));
+        }
+    }
+    out_measures[0] = cva;
+    out_measures[1] = dva;
+    out_measures[2] = fva;
+    out_measures[3] = mva;
+    out_measures[4] = kva;
+    free(pos);
+}
+
+/* ------------------------------------------------------------------------------------------ */
 /* Bermudan payer swaptions, Longstaff-Schwartz (no reference function: the reference's LSM is  */
 /* the quadratic GBM one of payoff/lsm.rs:73-107; this restates the same regression method for   */
 /* HW1F with a degree-3 basis in the standardized short rate)                                   */

@@ -143,6 +143,17 @@ int eqr_portfolio_exposure2(double a, double b, double sigma, double eta, double
                             double* set_pfe, double* set_cva, double* trade_ee, double* trade_pfe,
                             double* trade_cva, double* path_mtm, double* set_mtm, int n_threads);
 
+/* ---- XVA reducer over an exposure cube, quant-engine/src/xva.rs:14-246 ---- */
+double eqr_collateral_adjust(double value, int has_csa, int variation_margin, double threshold, double mta,
+                             double ia);
+void eqr_xva_calculate(const double* times, int n_times, uint64_t n_paths, const double* q, const double* df,
+                       const double* p, double cp_hazard, double cp_recovery, const double* cp_sp, int cp_nsp,
+                       double own_hazard, double own_recovery, const double* own_sp, int own_nsp,
+                       double funding_spread, double borrowing_spread, double capital_factor,
+                       double initial_margin_total, int has_csa, int variation_margin, double threshold,
+                       double mta, double ia, double confidence, double* out_ee, double* out_pfe,
+                       double* out_measures);
+
 /* ---- Bermudan payer swaptions, Longstaff-Schwartz with a degree-3 basis ---- */
 void eqr_berm_simulate(double a, double b, double sigma, double r0, double dt, uint32_t n_steps,
                        uint64_t n_paths, const double* z, double* r_out, double* lb_out);

@@ -1,0 +1,125 @@
+"""The XVA reducer on the GPU (quant-engine/src/xva.rs:97-246) against the oracle, and the Monte Carlo
+exposure cube of a plan fed to it (in place of the flat PV cube of quant-engine/src/lib.rs:751-779).
+
+Tolerances: the quantile is an exact order statistic of values computed by the same operations on both
+sides, so PFE is bit-identical; EE / ENE are sums in another order, within 1e-12 relative, and the
+measures accumulate them in the reference's order, within 1e-12 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from engine_quant_amd import eqhip, xva
+from helpers import MODEL, assert_rel, small_portfolio
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle(req, q, p=None):
+    c = req.collateral_agreements[0] if req.collateral_agreements else None
+    csa = None if c is None else {"threshold": c.threshold, "minimum_transfer_amount": c.minimum_transfer_amount,
+                                  "independent_amount": c.independent_amount,
+                                  "variation_margin": c.variation_margin}
+    cd, od, f = req.counterparty_default, req.own_default, req.funding
+    return oracle.xva_calculate(
+        q.times, q.paths, df=q.discount_factors if len(q.discount_factors) else None,
+        p=None if p is None else p.paths,
+        counterparty=(cd.hazard_rate, cd.recovery_rate, cd.survival_probabilities),
+        own=(od.hazard_rate, od.recovery_rate, od.survival_probabilities), funding_spread=f.funding_spread,
+        borrowing_spread=f.borrowing_spread, capital_factor=f.capital_factor,
+        initial_margin_total=sum(max(a.initial_margin, 0.0) for a in req.collateral_agreements), csa=csa,
+        confidence=req.confidence_level)
+
+
+def _compare(res, ref, what):
+    ee, pfe, m = ref
+    assert_rel(res.ee, ee, 1e-12, scale=np.maximum(np.abs(ee), 1e-300), what=f"{what} EE")
+    assert np.array_equal(res.pfe, pfe), f"{what} PFE"
+    for k in ("cva", "dva", "fva", "mva", "kva"):
+        assert abs(res.measures[k] - m[k]) <= 1e-12 * max(abs(m[k]), 1e-300), (what, k, res.measures[k], m[k])
+
+
+def test_reducer_matches_oracle_plain_and_with_csa_and_p_cube(ctx):
+    rng = np.random.default_rng(11)
+    times = np.array([0.25, 0.5, 1.0, 2.0, 3.5])
+    q = xva.ExposureCube("Q", times, rng.normal(5e4, 1e5, (5, 20000)), np.exp(-0.03 * times))
+    plain = xva.XvaRequest(operation_id="x1", counterparty_default=xva.DefaultSpec(0.02, 0.4),
+                           own_default=xva.DefaultSpec(0.01, 0.4))
+    _compare(xva.calculate(ctx, plain, q), _oracle(plain, q), "plain")
+    p = xva.ExposureCube("P", times, rng.normal(-2e4, 8e4, (5, 20000)))
+    full = xva.XvaRequest(
+        operation_id="x2", netting_sets=["ns-1"],
+        collateral_agreements=[xva.CollateralAgreement("csa-1", threshold=5e4, minimum_transfer_amount=1e4,
+                                                       independent_amount=2e3, initial_margin=1e5),
+                               xva.CollateralAgreement("csa-2", initial_margin=3e4)],
+        counterparty_default=xva.DefaultSpec(0.03, 0.35, survival_probabilities=[0.995, 0.99, 0.98]),
+        own_default=xva.DefaultSpec(0.015, 1.2), funding=xva.FundingSpec(0.004, 0.006, 0.0, 0.1),
+        confidence_level=0.99)
+    res = xva.calculate(ctx, full, q, p)
+    _compare(res, _oracle(full, q, p), "csa+p")
+    assert res.provenance["p_measure"] and res.exposure_cube["shape"] == [5, 20000]
+    assert "collateral is represented as thresholded variation margin from the first CSA" in \
+        res.provenance["approximations"]
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (3, 1), (1, 1025), (2, 8193)])
+def test_reducer_edge_shapes(ctx, shape):
+    rng = np.random.default_rng(shape[1])
+    q = xva.ExposureCube("Q", np.arange(1, shape[0] + 1, dtype=float), rng.normal(0, 1, shape))
+    req = xva.XvaRequest(counterparty_default=xva.DefaultSpec(0.05, 0.4), confidence_level=0.5)
+    _compare(xva.calculate(ctx, req, q), _oracle(req, q), f"shape {shape}")
+
+
+def test_reducer_rejects_bad_cubes_through_the_abi(ctx):
+    params = xva._params(xva.XvaRequest())
+    with pytest.raises(eqhip.EqhipError, match="strictly increasing"):
+        ctx.xva_calculate(params, [1.0, 1.0], np.zeros((2, 3)))
+    with pytest.raises(eqhip.EqhipError, match="finite rows"):
+        ctx.xva_calculate(params, [1.0, 2.0], np.array([[0.0, np.inf, 1.0], [0.0, 0.0, 0.0]]))
+    bad = xva._params(xva.XvaRequest(confidence_level=1.5))
+    with pytest.raises(eqhip.EqhipError, match="confidence_level"):
+        ctx.xva_calculate(bad, [1.0], np.zeros((1, 3)))
+
+
+@pytest.mark.parametrize("model", [MODEL, (0.1, 0.2, 0.01, 0.012, -0.7, 0.03)])
+def test_plan_cube_feeds_the_reducer(ctx, model):
+    port = small_portfolio(n_trades=12, n_sets=3, seed=9)
+    dates = np.arange(1, 12 * 5 + 1) / 12.0
+    n_paths, n_steps = 3000, 60
+    plan = ctx.plan(model, port, dates, n_paths, n_steps, seed=5, keep_cube=True)
+    try:
+        with pytest.raises(eqhip.EqhipError, match="not been executed"):
+            plan.exposure_cube(0)
+        out = plan.execute(None, 0.02, 0.4, want_set_mtm=True)
+        req = xva.XvaRequest(operation_id="mc", counterparty_default=xva.DefaultSpec(0.02, 0.4),
+                             own_default=xva.DefaultSpec(0.01, 0.4))
+        for s in range(port.n_sets):
+            cube = xva.plan_cube(plan, s)
+            cube.validate()
+            assert np.array_equal(cube.paths, out["set_mtm"][:, s, :])  # the signed netting-set values
+            if len(model) == 4:
+                disc = [oracle.zcb(model[0], model[1], model[2], model[3], 0.0, t) for t in dates]
+            else:
+                disc = [oracle.zcb2(model, 0.0, 0.0, 0.0, t) for t in dates]
+            assert_rel(cube.discount_factors, disc, 1e-14, what="model discount")
+            resident = xva.plan_xva(plan, req, s)
+            copied = xva.calculate(ctx, req, cube)
+            assert np.array_equal(resident.ee, copied.ee) and np.array_equal(resident.pfe, copied.pfe)
+            assert resident.measures == copied.measures
+            _compare(copied, _oracle(req, cube), f"set {s}")
+            # no CSA: EE is the exposure path's EE row for row, CVA its CVA up to the term order
+            assert np.array_equal(resident.ee, out["set_ee"][s])
+            assert abs(resident.cva - out["set_cva"][s]) <= 1e-13 * abs(out["set_cva"][s])
+    finally:
+        plan.close()
+
+
+def test_plan_without_cube_refuses(ctx):
+    port = small_portfolio(n_trades=3, n_sets=1, seed=2)
+    plan = ctx.plan(MODEL, port, [0.5, 1.0], 256, 8)
+    try:
+        plan.execute()
+        with pytest.raises(eqhip.EqhipError, match="KEEP_CUBE"):
+            plan.exposure_cube(0)
+    finally:
+        plan.close()
