@@ -962,7 +962,8 @@ double xva_default_increment(const eqhip_default_spec& spec, int i, const double
 double xva_clamp_recovery(double r) { return std::min(std::max(r, 0.0), 1.0); }
 
 void xva_reduce(eqhip_context* ctx, const XvaRows& rows, DevBuf& pos, const eqhip_xva_params& x, const double* times,
-                const double* discount, double* out_ee, double* out_pfe, double* out_measures) {
+                const double* discount, double* out_ee, double* out_pfe, double* out_measures,
+                hipEvent_t* ev = nullptr, double* ms_rows = nullptr, double* ms_select = nullptr) {
 #pragma clang fp contract(off)
     hipStream_t st = ctx->stream;
     const int nt = rows.n_times;
@@ -987,20 +988,30 @@ void xva_reduce(eqhip_context* ctx, const XvaRows& rows, DevBuf& pos, const eqhi
         g.mta = std::max(csa->minimum_transfer_amount, 0.0);
         g.ia = std::max(csa->independent_amount, 0.0);
     }
-    hipLaunchKernelGGL(xva_rows_kernel, dim3((unsigned)nt), dim3(kSelectThreads), 0, st, g);
-    HIPCHK(hipGetLastError());
     // the quantile of xva.rs:66-75: sorted[round((n - 1) p)] of the positive parts
     const uint64_t rank = (uint64_t)std::llround((double)(n - 1) * x.confidence_level);
     std::vector<SelectRow> sel(nt);
     for (int k = 0; k < nt; ++k)
         sel[k] = SelectRow{g.pos + (size_t)k * n, n, std::min(rank, n - 1), d_sums + k, d_sums + nt + k};
     upload(dev_rows, sel, st);
+    if (ev) HIPCHK(hipEventRecord(ev[0], st));
+    hipLaunchKernelGGL(xva_rows_kernel, dim3((unsigned)nt), dim3(kSelectThreads), 0, st, g);
+    HIPCHK(hipGetLastError());
+    if (ev) HIPCHK(hipEventRecord(ev[1], st));
     hipLaunchKernelGGL(select_kernel, dim3((unsigned)nt), dim3(kSelectThreads), 0, st,
                        dev_rows.as<const SelectRow>());
     HIPCHK(hipGetLastError());
+    if (ev) HIPCHK(hipEventRecord(ev[2], st));
     std::vector<double> h((size_t)nt * 4);
     HIPCHK(hipMemcpyAsync(h.data(), d_sums, (size_t)nt * 4 * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    if (ev) {
+        float a = 0.f, b = 0.f;
+        HIPCHK(hipEventElapsedTime(&a, ev[0], ev[1]));
+        HIPCHK(hipEventElapsedTime(&b, ev[1], ev[2]));
+        *ms_rows = a;
+        *ms_select = b;
+    }
     const double* ee = h.data();
     const double* pfe = h.data() + nt;
     const double* ene_q = h.data() + 2 * nt;
@@ -1926,7 +1937,8 @@ int eqhip_plan_xva(eqhip_plan* plan, int32_t set, const eqhip_xva_params* params
         std::vector<double> disc(P.n_dates);
         for (int k = 0; k < P.n_dates; ++k) disc[k] = hw[k].y;
         XvaRows rows{P.cube.as<double>() + (size_t)set * np, (uint64_t)P.S * np, nullptr, 0, np, P.n_dates};
-        xva_reduce(P.ctx, rows, P.xva_pos, *params, P.dates.data(), disc.data(), out_ee, out_pfe, out_measures);
+        xva_reduce(P.ctx, rows, P.xva_pos, *params, P.dates.data(), disc.data(), out_ee, out_pfe, out_measures, P.ev,
+                   &P.timing.xva_rows_ms, &P.timing.xva_select_ms);
     });
 }
 

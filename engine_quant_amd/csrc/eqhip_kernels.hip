@@ -779,6 +779,21 @@ __device__ void locate_rank(const unsigned int* hist, unsigned int* scan, unsign
 
 }  // namespace
 
+// Histogram increment of one key by the lanes active in the calling branch: the lanes whose bin equals
+// the first active lane's add once for all of them, the others one by one. A row whose keys pile into
+// one bin (ties, narrow distributions) would otherwise serialise on one LDS address.
+__device__ __forceinline__ void hist_add(unsigned int* hist, unsigned int b) {
+    const unsigned int b0 = __builtin_amdgcn_readfirstlane(b);
+    const uint64_t same = __ballot(b == b0);
+    if (b == b0) {
+        const unsigned int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(same >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)same, 0u));
+        if (below == 0u) atomicAdd(&hist[b0], (unsigned int)__popcll(same));
+    } else {
+        atomicAdd(&hist[b], 1u);
+    }
+}
+
 // Visits x[i] for i = tid, tid + T, tid + 2T, ... in that order, with eight loads in flight per
 // thread (a pass over a row is latency-bound with one).
 template <class F>
@@ -801,7 +816,7 @@ __global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow*
     __shared__ unsigned int scan[kSelectThreads];
     __shared__ double red[kSelectThreads];
     __shared__ unsigned long long cand[kSelectCand];
-    __shared__ unsigned long long s_prefix, s_rank, s_before;
+    __shared__ unsigned long long s_prefix, s_rank, s_before, s_min, s_max;
     __shared__ unsigned int s_bin, s_count, s_ncand;
 
     const int tid = threadIdx.x;
@@ -820,7 +835,7 @@ __global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow*
         sum += __longlong_as_double(key);
         const unsigned int b = (unsigned int)(key >> 52);
         if (b == 0u) ++c0;
-        else atomicAdd(&hist[b], 1u);
+        else hist_add(hist, b);
     });
     if (c0) atomicAdd(&hist[0], c0);
     red[tid] = sum;
@@ -846,17 +861,34 @@ __global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow*
         const int wd = hi >= 12 ? 12 : hi;
         const int sh = hi - wd;
         for (int b = tid; b < 4096; b += kSelectThreads) hist[b] = 0u;
+        if (tid == 0) {
+            s_min = ~0ull;
+            s_max = 0ull;
+        }
         __syncthreads();
+        // the smallest and largest key under the prefix as well: when they are equal, every candidate
+        // is the same value (ties, e.g. rows of zero exposure) and the remaining passes are skipped
         unsigned int c0 = 0u;
+        unsigned long long kmin = ~0ull, kmax = 0ull;
         select_pass(x, n, tid, [&](unsigned long long key) {
             if ((key >> hi) == (prefix >> hi)) {
                 const unsigned int b = (unsigned int)((key >> sh) & ((1u << wd) - 1u));
                 if (b == 0u) ++c0;
-                else atomicAdd(&hist[b], 1u);
+                else hist_add(hist, b);
+                kmin = key < kmin ? key : kmin;
+                kmax = key > kmax ? key : kmax;
             }
         });
         if (c0) atomicAdd(&hist[0], c0);
+        if (kmin <= kmax) {
+            atomicMin(&s_min, kmin);
+            atomicMax(&s_max, kmax);
+        }
         __syncthreads();
+        if (s_min == s_max) {  // uniform across the block
+            if (tid == 0) *row.pfe = __longlong_as_double(s_min);
+            return;
+        }
         locate_rank(hist, scan, s_rank, &s_bin, &s_before);
         if (tid == 0) {
             s_prefix = prefix | ((unsigned long long)s_bin << sh);

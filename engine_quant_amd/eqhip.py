@@ -94,7 +94,7 @@ class XvaParamsC(ctypes.Structure):
 class TimingC(ctypes.Structure):
     _fields_ = [("simulate_ms", _D), ("value_ms", _D), ("reduce_ms", _D), ("cva_ms", _D),
                 ("total_ms", _D), ("n_value_blocks", _U64), ("paths_per_tile", _U32),
-                ("tiles_per_block", _U32)]
+                ("tiles_per_block", _U32), ("xva_rows_ms", _D), ("xva_select_ms", _D)]
 
 
 _lib = None

@@ -109,6 +109,8 @@ typedef struct {
     uint64_t n_value_blocks;
     uint32_t paths_per_tile;
     uint32_t tiles_per_block;
+    double xva_rows_ms;   /* eqhip_plan_xva: CSA adjustment, positive parts and ENE sums of the cube rows */
+    double xva_select_ms; /* eqhip_plan_xva: EE sums and the quantile of every row */
 } eqhip_timing;
 
 /* ---- library / context -------------------------------------------------------------------- */
