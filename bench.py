@@ -27,9 +27,14 @@ EXP_FLOPS = 20            # FP64 exp as range reduction + degree-~10 FMA polynom
 
 
 def algorithmic_work(port, dates, n_paths, n_sets, factors=1):
-    """Algorithmic FP64 FLOPs and bytes of one value_kernel launch (the dominant kernel).
+    """Algorithmic work of one value_kernel launch (the dominant kernel).
 
-    Units and per-unit figures:
+    The roofline figure is the per-unit work of the swap valuation itself: per live (trade, path,
+    date) 2 m + 6 FP64 FLOPs (m fixed-leg FMAs over the trade's m remaining flows, the two floating-leg
+    terms, max, accumulate), plus one exp per live cash-flow date per (path, date), counted apart.
+
+    Returned beside it, the work this kernel executes after sharing every discount factor and annuity
+    across the trades of a schedule class:
       * live (trade, path, date):        5 FLOPs   V = N*F - (N K)*A (mul + FMA), max, add
       * live (class, path, date):        2 m + 2   annuity FMAs over the m remaining flows,
                                                    floating leg (sub, + div mid-period)
@@ -37,6 +42,8 @@ def algorithmic_work(port, dates, n_paths, n_sets, factors=1):
       * live (cash-flow date, path, date): exp + 2 FLOPs   P = A exp(-B r)
                                            (G2++: exp + 4, P = exp((c - B1 x) - B2 y))
     Bytes: state read (8 B per factor and path-date) + exposure write (8 B per set-path-date).
+
+    Returns (valuation FLOPs, executed FLOPs, bytes, live trade-path-dates, exps).
     """
     tol = 1e-9
     off = port.flow_off
@@ -54,17 +61,21 @@ def algorithmic_work(port, dates, n_paths, n_sets, factors=1):
     cf_dates = np.unique(np.concatenate([port.pay, port.start]))
     flops = 0.0
     u_live = 0.0
+    val_flops = 0.0
+    n_exp = 0.0
     for t in dates:
         live = mats > t + tol
         rem = np.array([np.sum(f > t + tol) for f in cflows])
         u_live += float(np.sum(n_cls_trades[live]))
+        val_flops += float(np.sum(n_cls_trades[live] * (2.0 * rem[live] + 6.0)))
+        n_exp += float(np.sum(cf_dates > t + tol))
         per_path = (5.0 * np.sum(n_cls_trades[live]) + np.sum(2.0 * rem[live] + 2.0)
                     + 4.0 * set_cls * np.mean(live) + (EXP_FLOPS + 2.0 * factors) * np.sum(cf_dates > t + tol))
         flops += per_path
     flops *= n_paths
     u_live *= n_paths
     bytes_ = 8.0 * n_paths * len(dates) * (factors + n_sets)
-    return flops, bytes_, u_live
+    return val_flops * n_paths, flops, bytes_, u_live, n_exp * n_paths
 
 
 PROFILE = os.path.join(ROOT, "profiles", "value_kernel_hw1f_1000swaps_100kpaths_360dates.md")
@@ -90,29 +101,54 @@ def measured_traffic(kernel="value_kernel", profile=PROFILE):
 
 
 def cpu_baseline(port, dates, n_steps, model, seed, disc, target_s=15.0):
-    """The oracle (CPU FP64 restatement of the reference, OpenMP over paths) on a bounded path sample
-    of the same workload; returns (units/s, threads, sample description, oracle CVA)."""
+    """The oracle (CPU FP64 restatement of the reference) on bounded path samples of the same workload.
+
+    fused: one simulation for the whole portfolio, OpenMP over paths (the conservative baseline);
+    median of five timed runs after one warm-up run.
+    reference structure: every trade simulated on its own (as per-calendar-group re-simulation does),
+    full path storage and a sort per date, one thread, on the first trades of the portfolio.
+    Returns (fused units/s, threads, paths per run, median seconds, oracle results of the fused sample,
+    reference-structure units/s and its sample description)."""
     import oracle
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
     threads = min(threads, 16)
-    # calibrate on a tiny sample, then size the timed sample to ~target_s seconds
     normals = oracle.philox_normals2 if len(model) == 6 else oracle.philox_normals
+    pd_ = port.as_dict()
+    # calibrate on a tiny sample, then size each of the six runs to ~target_s / 6 seconds
     probe = 8
     z = normals(seed, probe, n_steps)
     t0 = time.perf_counter()
-    oracle.portfolio_exposure(model, port.as_dict(), dates, n_steps, z, scheme=disc, want_trade=True,
-                              n_threads=threads)
+    oracle.portfolio_exposure(model, pd_, dates, n_steps, z, scheme=disc, want_trade=True, n_threads=threads)
     dt = time.perf_counter() - t0
-    n_sample = int(max(threads, min(4096, probe * target_s / max(dt, 1e-6))))
+    n_sample = int(max(threads, min(4096, probe * (target_s / 6.0) / max(dt, 1e-6))))
     n_sample = max(threads, (n_sample // threads) * threads)
     z = normals(seed, n_sample, n_steps)
-    t0 = time.perf_counter()
-    ref = oracle.portfolio_exposure(model, port.as_dict(), dates, n_steps, z, scheme=disc, hazard_rate=0.02,
-                                    recovery_rate=0.4, want_trade=True, n_threads=threads)
-    dt = time.perf_counter() - t0
+    runs = []
+    for _ in range(6):
+        t0 = time.perf_counter()
+        ref = oracle.portfolio_exposure(model, pd_, dates, n_steps, z, scheme=disc, hazard_rate=0.02,
+                                        recovery_rate=0.4, want_trade=True, n_threads=threads)
+        runs.append(time.perf_counter() - t0)
+    med = float(np.median(runs[1:]))
     units = port.n_trades * n_sample * len(dates)
-    return units / dt, threads, n_sample, dt, ref
+    # reference structure on the first trades: a singleton portfolio each, one thread
+    n_ref_trades, n_ref_paths = 16, 512
+    zr = normals(seed, n_ref_paths, n_steps)
+    off = port.flow_off
+    t0 = time.perf_counter()
+    for j in range(n_ref_trades):
+        f0, f1 = off[j], off[j + 1]
+        single = {"set_id": np.zeros(1, np.int32), "notional": port.notional[j:j + 1],
+                  "fixed_rate": port.fixed_rate[j:j + 1], "start": port.start[j:j + 1],
+                  "flow_off": np.array([0, f1 - f0], np.int32), "pay": port.pay[f0:f1], "acc": port.acc[f0:f1]}
+        oracle.portfolio_exposure(model, single, dates, n_steps, zr, scheme=disc, hazard_rate=0.02,
+                                  recovery_rate=0.4, n_threads=1)
+    ref_s = time.perf_counter() - t0
+    ref_rate = n_ref_trades * n_ref_paths * len(dates) / ref_s
+    ref_sample = (f"first {n_ref_trades} trades x {n_ref_paths} paths, each trade simulated on its own, "
+                  f"one thread ({ref_s:.1f} s)")
+    return units / med, threads, n_sample, med, ref, ref_rate, ref_sample
 
 
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix
@@ -478,9 +514,10 @@ def main():
     units = port.n_trades * args.paths * len(dates)
     value = units / (ms_per_step / 1000.0)
 
-    flops, bytes_, u_live = algorithmic_work(port, dates, args.paths, port.n_sets, factors)
+    flops, exec_flops, bytes_, u_live, n_exp = algorithmic_work(port, dates, args.paths, port.n_sets, factors)
     v_ms = float(np.mean(value_ms))
     achieved = flops / (v_ms / 1000.0) / 1e12
+    executed = exec_flops / (v_ms / 1000.0) / 1e12
     tm = plan.timing()
     default_workload = (args.trades, args.paths, args.years, args.seed, args.disc) == (1000, 100_000, 30, 7, "exact")
     kernel = "value2_kernel" if factors == 2 else "value_kernel"
@@ -509,22 +546,27 @@ def main():
                      "traffic": traffic,
                      "compute": "fp64 on the vector ALUs (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
                      "traffic_source": os.path.relpath(profile, ROOT) if traffic else None,
-                     "algorithmic_flops": flops, "algorithmic_bytes": bytes_,
+                     "algorithmic_flops": flops, "algorithmic_exps": n_exp,
+                     "algorithmic_unit": "2m+6 FLOPs per live (trade, path, date), m = remaining fixed flows",
+                     "executed_flops": exec_flops, "executed_tflops": executed,
+                     "executed_frac": executed / FP64_PEAK_TFLOPS, "algorithmic_bytes": bytes_,
                      "value_kernel_ms": v_ms, "exposure_kernels_ms": float(np.mean(total_kernel_ms)),
                      "paths_per_tile": tm["paths_per_tile"], "tiles_per_block": tm["tiles_per_block"],
                      "blocks": tm["n_value_blocks"]},
         "set_cva": float(res["set_cva"][0]),
     }
     if not args.no_cpu:
-        rate, threads, n_sample, cpu_s, ref = cpu_baseline(port, dates, n_steps, model, args.seed, disc,
-                                                           args.cpu_seconds)
+        rate, threads, n_sample, cpu_s, ref, ref_rate, ref_sample = cpu_baseline(
+            port, dates, n_steps, model, args.seed, disc, args.cpu_seconds)
         # the same paths on the GPU (counter-based stream: path p's draws do not depend on n_paths)
         small = ctx.exposure_run(model, port, dates, n_sample, n_steps, seed=args.seed, rng=eqhip.RNG_PHILOX,
                                  disc=disc, hazard_rate=hazard, recovery_rate=recovery, want_trade_ee=True)
         cva_err = abs(small["set_cva"][0] - ref["set_cva"][0]) / max(abs(ref["set_cva"][0]), 1e-300)
         out["cpu_baseline"] = {"value": rate, "unit": "trade-paths-dates/s", "cores": threads, "kind": "port",
-                               "sample": f"same portfolio and dates, first {n_sample} Philox paths "
-                                         f"({cpu_s:.1f} s of oracle time, OpenMP over paths)"}
+                               "sample": f"same portfolio and dates, first {n_sample} Philox paths, one "
+                                         f"simulation for the portfolio, OpenMP over paths; median of 5 runs "
+                                         f"after a warm-up ({cpu_s:.2f} s each)",
+                               "reference_structure": {"value": ref_rate, "cores": 1, "sample": ref_sample}}
         out["speedup_vs_cpu"] = value / rate
         out["cva_rel_err_vs_fp64_oracle"] = cva_err
         out["trade_ee_max_rel_err_vs_oracle"] = float(
