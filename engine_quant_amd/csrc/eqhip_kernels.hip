@@ -3,14 +3,18 @@
 //   simulate_kernel    one thread per path; the short rate lives in a register, shocks come from
 //                      a coalesced [step][path] buffer or from in-register Philox4x32-10; r is
 //                      stored only at the grid steps the valuation needs ([slot][path]).
-//   ab_kernel          A(tau), B(tau) of the affine ZCB for every (date, cash-flow date) pair.
-//   value_kernel<PT>   one block per (monitoring date, path chunk); per tile of PT paths it
-//                      stages P(t,T_d) for every live cash-flow date in LDS, builds each schedule
-//                      class's floating-leg and annuity factors in LDS, writes the netting-set
-//                      exposure max(V,0) for every set, and accumulates per-trade EE partial sums
-//                      in registers (trades sorted by class, several trades per thread).
-//   select_kernel      one block per (set, date) row: fixed-order EE sum + exact radix select of
-//                      the PFE95 order statistic on the bit patterns of the non-negative doubles.
+//   simulate2_kernel   the same for G2++: x and y in registers, stored as two [slot][path] buffers.
+//   ab_kernel          A(tau), B(tau) of the affine ZCB for every (date, cash-flow date) pair;
+//   ab2_kernel         (c, B(a,tau)) and B(b,tau) of the G2++ bond likewise.
+//   value_kernel       one 1024-thread block per work unit (plan, monitoring date, chunk of 64-path
+//   value2_kernel      tiles); per tile it stages P(t,T_d) for every live cash-flow date and the
+//                      fixing factors in LDS, builds each schedule class's floating-leg and annuity
+//                      factors, writes the netting-set values (and their positive parts) for every
+//                      set, and accumulates per-trade exposure sums over path slices. value2_kernel
+//                      is the G2++ instance (two-factor bonds in stage A).
+//   select_kernel      one block per row: fixed-order sum + exact radix select of an order
+//                      statistic on the bit patterns of the non-negative doubles.
+//   xva_rows_kernel    one block per exposure-cube row: CSA adjustment, positive parts and ENE.
 //   trade_ee_kernel    fixed-order sum of per-trade EE partials.
 //   cva_kernel         unilateral CVA of each profile, exposure.rs:242-275.
 //   pv_dv01_kernel     market-curve PV / DV01 / bucketed DV01 per trade, measure.cpp:124-222.
