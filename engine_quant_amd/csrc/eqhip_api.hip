@@ -111,7 +111,7 @@ struct eqhip_plan {
     bool executed = false;
     int n_items = 0;
     int set_mode = 0;
-    uint64_t target_units = 2048;   // work units the launch aims for (chunking of the path tiles)
+    uint64_t target_units = 4096;   // work units the launch aims for (chunking of the path tiles)
     const double* r_shared = nullptr;  // batch-owned short-rate buffer (all grid steps), or null
     uint64_t r_stride = 0;
     std::vector<double> dates;
@@ -189,8 +189,12 @@ void choose_launch(eqhip_plan& P) {
     if (P.want_trade) lds = std::max(lds, value_trade_scratch_bytes());
     P.lds_bytes = lds;
     P.n_tiles = (P.mc.n_paths + kPathsPerTile - 1) / kPathsPerTile;
-    // enough units for ~8 blocks per CU (one 1024-thread block per CU at this LDS footprint)
+    // enough units for ~16 blocks per CU (one 1024-thread block per CU at this LDS footprint). Units run
+    // date-major and blocks go to the 8 XCDs round-robin, so with a multiple of 8 chunks every unit of
+    // one path chunk lands on the same XCD, whose L2 then serves the rows later dates read again (the
+    // fixings of periods in progress).
     uint64_t target_chunks = std::max<uint64_t>(1, (P.target_units + P.n_dates - 1) / P.n_dates);
+    target_chunks = (target_chunks + 7) / 8 * 8;
     uint64_t tpb = std::max<uint64_t>(1, (P.n_tiles + target_chunks - 1) / target_chunks);
     P.tiles_per_block = (int)tpb;
     P.n_chunks = (int)((P.n_tiles + tpb - 1) / tpb);
