@@ -757,7 +757,7 @@ void launch_cva_weights2(hipStream_t st, const double* d_times, int n_dates, con
 }
 void launch_cva_sum(hipStream_t st, const double* ee, int n_profiles, int n_dates, const double2* w, double recovery,
                     double* out) {
-    hipLaunchKernelGGL(cva_kernel, dim3((n_profiles + 63) / 64), dim3(64), 0, st, ee, n_profiles, n_dates, w, recovery,
+    hipLaunchKernelGGL(cva_kernel, dim3((n_profiles + 3) / 4), dim3(256), 0, st, ee, n_profiles, n_dates, w, recovery,
                        out);
     HIPCHK(hipGetLastError());
 }

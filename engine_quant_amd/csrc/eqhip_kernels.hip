@@ -16,7 +16,7 @@
 //                      statistic on the bit patterns of the non-negative doubles.
 //   xva_rows_kernel    one block per exposure-cube row: CSA adjustment, positive parts and ENE.
 //   trade_ee_kernel    fixed-order sum of per-trade EE partials.
-//   cva_kernel         unilateral CVA of each profile, exposure.rs:242-275.
+//   cva_kernel         unilateral CVA of each profile (one wave per profile), exposure.rs:242-275.
 //   pv_dv01_kernel     market-curve PV / DV01 / bucketed DV01 per trade, measure.cpp:124-222.
 #include "eqhip_device.hpp"
 #include "eqhip_kernels.hpp"
@@ -1066,22 +1066,43 @@ __global__ void cva_weights2_kernel(const double* times, int n_dates, double a, 
 }
 
 // One profile per thread, the dates in order, each term as (1-R) * EE_k * dS_k * P_k.
-__global__ void cva_kernel(const double* ee, int n_profiles, int n_dates, const double2* w, double recovery,
-                           double* cva) {
+__global__ void __launch_bounds__(256) cva_kernel(const double* ee, int n_profiles, int n_dates, const double2* w,
+                                                  double recovery, double* cva) {
 #pragma clang fp contract(off)
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n_profiles) return;
+    // one wave per profile: the lanes load and form 64 terms at a time, then every lane adds them in
+    // date order (the reference's loop order, so the sum rounds exactly as one sequential loop)
+    const int lane = threadIdx.x & 63;
+    const int j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (j >= n_profiles) return;  // uniform across the wave
     const gptr<double> e = glb(ee + (uint64_t)j * n_dates);
     const gptr<double> wd = glb((const double*)w);
+    const double q = 1.0 - recovery;
     double acc = 0.0;
-    for (int k = 0; k < n_dates; ++k) acc += (1.0 - recovery) * e[k] * wd[2 * k] * wd[2 * k + 1];
-    cva[j] = acc;
+    for (int base = 0; base < n_dates; base += 64) {
+        const int k = base + lane;
+        const double term = (k < n_dates) ? q * e[k] * wd[2 * k] * wd[2 * k + 1] : 0.0;
+        const int cnt = min(64, n_dates - base);
+        for (int i = 0; i < cnt; ++i) acc += __shfl(term, i);
+    }
+    if (lane == 0) cva[j] = acc;
 }
 
 // ---------------------------------------------------------------------------------------------
 // Market-curve PV / DV01 (measure.cpp:160-222, market.cpp:26-41). Thread (trade, variant):
 // variant 0 = PV, 1 = parallel bump, 2+i = pillar i bumped.
 // ---------------------------------------------------------------------------------------------
+// The first pillar at or after t, for pillars[0] < t < pillars[np - 1]: the index market.cpp:26-41's
+// linear scan stops at, found by bisection.
+__device__ __forceinline__ int pillar_at_or_after(const double* pillars, int np, double t) {
+    int lo = 1, hi = np - 1;  // pillars[lo - 1] < t <= pillars[hi]
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (pillars[mid] < t) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 __device__ double curve_df(const double* pillars, const double* zeros, int np, int bump_pillar,
                            double bump, double t) {
 #pragma clang fp contract(off)
@@ -1096,9 +1117,7 @@ __device__ double curve_df(const double* pillars, const double* zeros, int np, i
     } else if (t >= pillars[np - 1]) {
         z = zr(np - 1);
     } else {
-        int i = 0;
-        while (i < np && pillars[i] < t) ++i;
-        if (i == 0) i = 1;
+        const int i = pillar_at_or_after(pillars, np, t);
         double t0 = pillars[i - 1], t1 = pillars[i];
         double z0 = zr(i - 1), z1 = zr(i);
         double w = (t - t0) / (t1 - t0);
@@ -1124,6 +1143,17 @@ __device__ double npv_from_curve(const double* pillars, const double* zeros, int
 
 __global__ void pv_dv01_kernel(PvArgs g) {
 #pragma clang fp contract(off)
+    // the curve in LDS: every discount factor bisects it
+    __shared__ double s_pillars[kPvLdsPillars], s_zeros[kPvLdsPillars];
+    const bool in_lds = g.np <= kPvLdsPillars;
+    if (in_lds)
+        for (int p = threadIdx.x; p < g.np; p += blockDim.x) {
+            s_pillars[p] = g.pillars[p];
+            s_zeros[p] = g.zeros[p];
+        }
+    __syncthreads();
+    const double* pillars = in_lds ? s_pillars : g.pillars;
+    const double* zeros = in_lds ? s_zeros : g.zeros;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int nv = 2 + g.np;
     if (i >= g.n_trades * nv) return;
@@ -1131,14 +1161,14 @@ __global__ void pv_dv01_kernel(PvArgs g) {
     int f0 = g.flow_off[j], m = g.flow_off[j + 1] - f0;
     const double* pay = g.pay + f0;
     const double* acc = g.acc + f0;
-    double base = npv_from_curve(g.pillars, g.zeros, g.np, -1, 0.0, g.notional[j], g.fixed[j],
+    double base = npv_from_curve(pillars, zeros, g.np, -1, 0.0, g.notional[j], g.fixed[j],
                                  g.start[j], pay, acc, m);
     if (v == 0) {
         g.pv[j] = base;
         return;
     }
     int bp = (v == 1) ? -2 : v - 2;
-    double bumped = npv_from_curve(g.pillars, g.zeros, g.np, bp, g.bump, g.notional[j], g.fixed[j],
+    double bumped = npv_from_curve(pillars, zeros, g.np, bp, g.bump, g.notional[j], g.fixed[j],
                                    g.start[j], pay, acc, m);
     if (v == 1) g.dv01[j] = bumped - base;
     else if (g.bucketed) g.bucketed[(uint64_t)j * g.np + (v - 2)] = bumped - base;
@@ -1155,9 +1185,7 @@ __device__ void curve_weights(const double* pillars, int np, double t, int& i0, 
         i0 = i1 = np - 1;
         w1 = 1.0;
     } else {
-        int i = 0;
-        while (i < np && pillars[i] < t) ++i;
-        if (i == 0) i = 1;
+        const int i = pillar_at_or_after(pillars, np, t);
         i0 = i - 1;
         i1 = i;
         w1 = (t - pillars[i0]) / (pillars[i1] - pillars[i0]);

@@ -14,6 +14,7 @@ constexpr int kTradesPerItem = 4;    // trades of one schedule class and sign pe
 constexpr int kTradeSlots = kValueThreads * kItemsPerThread;  // (item, path slice) work slots per block
 constexpr int kMaxSplitSets = 8;     // netting sets whose class sums are split across the waves
 constexpr int kSelectThreads = 1024;  // EE / PFE select block
+constexpr int kPvLdsPillars = 256;    // curves up to this many pillars are staged in LDS by pv_dv01_kernel
 constexpr int kSelectCand = 4096;     // candidates ranked in LDS by the select kernel
 constexpr double kDateTolHost = 1e-9; // date matching tolerance, products/irs.rs:151-154,173
 
