@@ -430,6 +430,8 @@ def main():
     ap.add_argument("--years", type=int, default=30)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--disc", choices=["exact", "euler"], default="exact")
+    ap.add_argument("--serial-pv", action="store_true",
+                    help="run the PV/DV01 ladder after the exposure job on the same stream instead of beside it")
     ap.add_argument("--model", choices=["hw1f", "hw2f"], default="hw1f",
                     help="short-rate model of the swaps workload: HullWhite1F (default) or HullWhite2F (G2++)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -492,10 +494,20 @@ def main():
     ctx = eqhip.Context(0)
     plan = ctx.plan(model, port, dates, args.paths, n_steps, seed=args.seed, rng=eqhip.RNG_PHILOX, disc=disc,
                     want_trade_ee=True)
+    # the PV / DV01 ladder does not depend on the exposure job: a second context (its own stream) runs
+    # it from a worker thread while the exposure kernels run (ctypes releases the GIL in foreign calls)
+    from concurrent.futures import ThreadPoolExecutor
+    ctx_pv = eqhip.Context(0)
+    pool = ThreadPoolExecutor(max_workers=1)
 
     def step():
+        if args.serial_pv:
+            res = plan.execute(None, hazard, recovery, want_cva=True)
+            pv, dv, bk = ctx.pv_dv01(pillars, zeros, port, bump=1e-4, bucketed=True)
+            return res, pv, dv
+        ladder = pool.submit(ctx_pv.pv_dv01, pillars, zeros, port, 1e-4, True)
         res = plan.execute(None, hazard, recovery, want_cva=True)
-        pv, dv, bk = ctx.pv_dv01(pillars, zeros, port, bump=1e-4, bucketed=True)
+        pv, dv, bk = ladder.result()
         return res, pv, dv
 
     for _ in range(args.warmup):
@@ -540,6 +552,8 @@ def main():
                    "n_dates": len(dates), "n_steps": n_steps, "dates": "monthly", "rng": "philox4x32-10",
                    "discretisation": args.disc, "seed": args.seed, "n_sets": port.n_sets,
                    "outputs": "set EE/PFE95/CVA, trade EE/CVA, PV/DV01/bucketed DV01 (20 pillars)",
+                   "pv_ladder": "serial on the job's stream" if args.serial_pv else
+                                "second context and stream, concurrent with the exposure job",
                    "live_units": u_live},
         "roofline": {"kernel": kernel, "bound": "mfma", "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
@@ -571,7 +585,9 @@ def main():
         out["cva_rel_err_vs_fp64_oracle"] = cva_err
         out["trade_ee_max_rel_err_vs_oracle"] = float(
             np.max(np.abs(small["trade_ee"] - ref["trade_ee"]) / np.maximum(np.abs(ref["trade_ee"]).max(), 1e-300)))
+    pool.shutdown()
     plan.close()
+    ctx_pv.close()
     ctx.close()
     print(json.dumps(out), flush=True)
 
