@@ -793,12 +793,13 @@ void launch_simulate2(eqhip_plan& P) {
     s.X = P.R.as<double>();
     s.Y = P.R2.as<double>();
     dim3 grid((unsigned)((P.mc.n_paths + 255) / 256));
+    dim3 pgrid((unsigned)((P.mc.n_paths + kSimPaths - 1) / kSimPaths));
     hipStream_t st = P.ctx->stream;
     switch (P.mc.disc * 2 + P.mc.rng) {
         case 0: hipLaunchKernelGGL((simulate2_kernel<0, 0>), grid, dim3(256), 0, st, s); break;
-        case 1: hipLaunchKernelGGL((simulate2_kernel<0, 1>), grid, dim3(256), 0, st, s); break;
+        case 1: hipLaunchKernelGGL((simulate2_philox_kernel<0>), pgrid, dim3(256), 0, st, s); break;
         case 2: hipLaunchKernelGGL((simulate2_kernel<1, 0>), grid, dim3(256), 0, st, s); break;
-        default: hipLaunchKernelGGL((simulate2_kernel<1, 1>), grid, dim3(256), 0, st, s); break;
+        default: hipLaunchKernelGGL((simulate2_philox_kernel<1>), pgrid, dim3(256), 0, st, s); break;
     }
     HIPCHK(hipGetLastError());
 }
@@ -846,12 +847,13 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
     s.Z = P.Z.as<double>();
     s.R = P.R.as<double>();
     dim3 sgrid((unsigned)((np + 255) / 256));
+    dim3 pgrid((unsigned)((np + kSimPaths - 1) / kSimPaths));
     int key = P.mc.disc * 2 + P.mc.rng;
     switch (key) {
         case 0: hipLaunchKernelGGL((simulate_kernel<0, 0>), sgrid, dim3(256), 0, st, s); break;
-        case 1: hipLaunchKernelGGL((simulate_kernel<0, 1>), sgrid, dim3(256), 0, st, s); break;
+        case 1: hipLaunchKernelGGL((simulate_philox_kernel<0>), pgrid, dim3(256), 0, st, s); break;
         case 2: hipLaunchKernelGGL((simulate_kernel<1, 0>), sgrid, dim3(256), 0, st, s); break;
-        default: hipLaunchKernelGGL((simulate_kernel<1, 1>), sgrid, dim3(256), 0, st, s); break;
+        default: hipLaunchKernelGGL((simulate_philox_kernel<1>), pgrid, dim3(256), 0, st, s); break;
     }
     HIPCHK(hipGetLastError());
     }
@@ -1471,10 +1473,10 @@ void simulate_batch(eqhip_batch& B) {
     s.slot_of_step = B.slot_of_step.as<int>();
     s.Z = nullptr;
     s.R = B.R.as<double>();
-    dim3 grid((unsigned)((B.max_paths + 255) / 256));
     hipStream_t st = B.ctx->stream;
-    if (B.disc == EQHIP_DISC_EULER) hipLaunchKernelGGL((simulate_kernel<0, 1>), grid, dim3(256), 0, st, s);
-    else hipLaunchKernelGGL((simulate_kernel<1, 1>), grid, dim3(256), 0, st, s);
+    const dim3 pgrid((unsigned)((s.n_paths + kSimPaths - 1) / kSimPaths));
+    if (B.disc == EQHIP_DISC_EULER) hipLaunchKernelGGL((simulate_philox_kernel<0>), pgrid, dim3(256), 0, st, s);
+    else hipLaunchKernelGGL((simulate_philox_kernel<1>), pgrid, dim3(256), 0, st, s);
     HIPCHK(hipGetLastError());
 }
 

@@ -59,6 +59,53 @@ __global__ void __launch_bounds__(256) simulate_kernel(SimArgs g) {
     }
 }
 
+// Philox paths with four threads per path drawing the shocks: the 4 waves of a block share 64 paths.
+// Per batch of kSimBatch steps each wave draws a quarter of the batch's Philox pairs for every path into
+// LDS, then wave 0 runs the paths' recursions over the batch. With four times the waves of one thread
+// per path, the Box-Muller transcendentals (the bulk of the work) have the occupancy to hide their
+// latency; the shocks and every step's rounding are those of simulate_kernel<DISC, 1>.
+template <int DISC>
+__global__ void __launch_bounds__(256) simulate_philox_kernel(SimArgs g) {
+#pragma clang fp contract(off)
+    __shared__ double sz[kSimBatch][kSimPaths];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t p = (uint64_t)blockIdx.x * kSimPaths + lane;
+    const bool valid = p < g.n_paths;
+    double r = g.r0;
+    for (uint32_t s0 = 0; s0 < g.n_steps; s0 += kSimBatch) {
+        for (int q = wave; q < kSimBatch / 2; q += 4) {  // pair (s0 + 2q) / 2: steps s0 + 2q, s0 + 2q + 1
+            double z0 = 0.0, z1 = 0.0;
+            if (valid) philox_normal_pair(g.seed, p, (s0 >> 1) + (uint32_t)q, z0, z1);
+            sz[2 * q][lane] = z0;
+            sz[2 * q + 1][lane] = z1;
+        }
+        __syncthreads();
+        if (wave == 0 && valid) {
+            const uint32_t cnt = min((uint32_t)kSimBatch, g.n_steps - s0);
+            for (uint32_t u = 0; u < cnt; ++u) {
+                const int slot = g.slot_of_step[s0 + u];
+                if (slot >= 0) g.R[(uint64_t)slot * g.n_paths + p] = r;
+                const double z = sz[u][lane];
+                if (DISC == 0) {
+                    double drift = g.a * (g.b - r);
+                    double dw = z * g.sqrt_dt;
+                    r = (r + drift * g.dt) + g.sigma * dw;
+                } else {
+                    r = (r * g.decay + g.mean_c) + g.sd * z;
+                }
+            }
+        }
+        __syncthreads();  // the next batch overwrites sz
+    }
+    if (wave == 0 && valid) {
+        const int slot = g.slot_of_step[g.n_steps];
+        if (slot >= 0) g.R[(uint64_t)slot * g.n_paths + p] = r;
+    }
+}
+
+template __global__ void simulate_philox_kernel<0>(SimArgs);
+template __global__ void simulate_philox_kernel<1>(SimArgs);
+
 template __global__ void simulate_kernel<0, 0>(SimArgs);
 template __global__ void simulate_kernel<0, 1>(SimArgs);
 template __global__ void simulate_kernel<1, 0>(SimArgs);
@@ -99,6 +146,61 @@ __global__ void __launch_bounds__(256) simulate2_kernel(Sim2Args g) {
         }
     }
 }
+
+// simulate_philox_kernel for G2++: one Philox pair per step gives both factors' shocks.
+template <int DISC>
+__global__ void __launch_bounds__(256) simulate2_philox_kernel(Sim2Args g) {
+#pragma clang fp contract(off)
+    constexpr int kB = kSimBatch / 2;
+    __shared__ double szx[kB][kSimPaths], szy[kB][kSimPaths];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t p = (uint64_t)blockIdx.x * kSimPaths + lane;
+    const uint64_t n = g.n_paths;
+    const bool valid = p < n;
+    double x = 0.0, y = 0.0;
+    for (uint32_t s0 = 0; s0 < g.n_steps; s0 += kB) {
+        for (int q = wave; q < kB; q += 4) {
+            double zx = 0.0, zy = 0.0;
+            if (valid) philox_normal_pair(g.seed, p, s0 + (uint32_t)q, zx, zy, 1u);
+            szx[q][lane] = zx;
+            szy[q][lane] = zy;
+        }
+        __syncthreads();
+        if (wave == 0 && valid) {
+            const uint32_t cnt = min((uint32_t)kB, g.n_steps - s0);
+            for (uint32_t u = 0; u < cnt; ++u) {
+                const int slot = g.slot_of_step[s0 + u];
+                if (slot >= 0) {
+                    g.X[(uint64_t)slot * n + p] = x;
+                    g.Y[(uint64_t)slot * n + p] = y;
+                }
+                const double zx = szx[u][lane], zy = szy[u][lane];
+                if (DISC == 0) {
+                    const double zc = zy * g.sqrt_1mrho2 + zx * g.rho;
+                    const double drift_x = (-g.a) * x;
+                    const double drift_y = (-g.b) * y;
+                    x = (x + drift_x * g.dt) + g.sigma * (zx * g.sqrt_dt);
+                    y = (y + drift_y * g.dt) + g.eta * (zc * g.sqrt_dt);
+                } else {
+                    const double zc = g.c_xy * zx + g.sqrt_1mc2 * zy;
+                    x = x * g.ex + g.sdx * zx;
+                    y = y * g.ey + g.sdy * zc;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (wave == 0 && valid) {
+        const int slot = g.slot_of_step[g.n_steps];
+        if (slot >= 0) {
+            g.X[(uint64_t)slot * n + p] = x;
+            g.Y[(uint64_t)slot * n + p] = y;
+        }
+    }
+}
+
+template __global__ void simulate2_philox_kernel<0>(Sim2Args);
+template __global__ void simulate2_philox_kernel<1>(Sim2Args);
 
 template __global__ void simulate2_kernel<0, 0>(Sim2Args);
 template __global__ void simulate2_kernel<0, 1>(Sim2Args);

@@ -183,6 +183,10 @@ __global__ void ab2_kernel(const double* tau, int n, double a, double b, double 
                            double r0, double2* c_b1, double* b2);
 __global__ void cva_weights2_kernel(const double* times, int n_dates, double a, double b, double sigma, double eta,
                                     double rho, double r0, double hazard, double2* w);
+constexpr int kSimPaths = 64;  // paths per block of the Philox simulation kernels (4 threads per path)
+constexpr int kSimBatch = 32;  // steps per batch of shocks in LDS (one factor; 16 for two factors)
+template <int DISC> __global__ void simulate_philox_kernel(SimArgs g);
+template <int DISC> __global__ void simulate2_philox_kernel(Sim2Args g);
 __global__ void value_kernel(const ValuePlan* plans, const int4* units);
 __global__ void value2_kernel(const ValuePlan* plans, const int4* units);
 __global__ void value_kernel_persistent(const ValuePlan* plans, const int4* units, int n_units,
