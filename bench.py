@@ -420,6 +420,83 @@ def run_xva(args):
     print(json.dumps(out), flush=True)
 
 
+# The reference's own CPU-vs-GPU benchmark (rust/crates/engine-core/examples/gpu_vs_cpu_bench.rs:29-87):
+# wall time of expected_exposure_profile for the 5y annual par payer IRS, 260 Euler steps, monitoring dates
+# {0,1,2,3,4}, seed 42, after one warm-up call; published GPU (burn-wgpu) times, PLAN.md:1066-1069.
+REFERENCE_GPU_MS = {1_000: 53.8, 10_000: 76.9, 100_000: 90.7, 1_000_000: 2170.0}
+
+
+def run_reference_case(args):
+    """The reference's published benchmark case through the reference-shaped entry point
+    (eqhip_irs_hull_white_exposure_profile): each measurement is one whole call (plan set-up, Philox
+    draws, Euler simulation, valuation, EE/PFE, copies back), as the reference timed one call."""
+    from engine_quant_amd import eqhip
+    import oracle
+
+    a, b, sigma, r0 = 0.1, 0.03, 0.01, 0.02
+    pay, acc, times = [1.0, 2.0, 3.0, 4.0, 5.0], [1.0] * 5, [0.0, 1.0, 2.0, 3.0, 4.0]
+    n_steps, seed = 260, 42
+    ctx = eqhip.Context(0)
+
+    def call(n):
+        return ctx.exposure_profile(a, b, sigma, r0, 1e6, 0.0, True, 0.0, pay, acc, times, n_steps, n, seed,
+                                    rng=eqhip.RNG_PHILOX, disc=eqhip.DISC_EULER)
+
+    sizes = {}
+    for n in sorted(REFERENCE_GPU_MS):
+        call(min(n, 1_000))  # warm-up, as the reference discards its first call
+        call(n)
+        runs = []
+        for _ in range(max(5, args.steps // 4)):
+            t0 = time.perf_counter()
+            call(n)
+            runs.append(1000.0 * (time.perf_counter() - t0))
+        ms = float(np.median(runs))
+        sizes[n] = {"ms_per_call": ms, "reference_gpu_ms": REFERENCE_GPU_MS[n],
+                    "speedup_vs_reference_gpu": REFERENCE_GPU_MS[n] / ms}
+        print(f"# {n} paths: {ms:.3f} ms per call (reference GPU {REFERENCE_GPU_MS[n]} ms)", file=sys.stderr,
+              flush=True)
+    n = 100_000
+    units = 1 * n * len(times)
+    value = units / (sizes[n]["ms_per_call"] / 1000.0)
+    ref_value = units / (REFERENCE_GPU_MS[n] / 1000.0)
+    # kernel times of the same case from a plan's HIP events
+    bf = lambda t: (1 - np.exp(-a * t)) / a  # noqa: E731  (the model bond, hull_white.rs:50-80)
+    zcb = lambda t: np.exp((bf(t) - t) * (a * a * b - sigma * sigma / 2) / (a * a) - sigma * sigma * bf(t) ** 2
+                           / (4 * a)) * np.exp(-bf(t) * r0)  # noqa: E731
+    k_par = (1.0 - zcb(5.0)) / sum(zcb(t) for t in pay)
+    port = eqhip.singleton_portfolio([1e6], [k_par], 0.0, pay, acc)
+    plan = ctx.plan((a, b, sigma, r0), port, times, n, n_steps, seed=seed, rng=eqhip.RNG_PHILOX,
+                    disc=eqhip.DISC_EULER)
+    plan.execute()
+    tm = plan.timing()
+    plan.close()
+    out = {
+        "metric": METRIC, "value": value, "unit": "trade-paths-dates/s", "n_gpus": 1,
+        "steps": max(5, args.steps // 4), "warmup": 1, "ms_per_step": sizes[n]["ms_per_call"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": value / ref_value, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "reference-gpu-vs-cpu-bench-case", "n_trades": 1, "n_paths": n, "n_dates": 5,
+                   "n_steps": n_steps, "rng": "philox4x32-10", "discretisation": "euler", "seed": seed,
+                   "swap": "5y annual par payer, N 1e6", "baseline": "reference burn-wgpu GPU time at 100k paths, "
+                   "PLAN.md:1068 (unspecified development machine)", "sizes": sizes},
+        "kernels_ms": {"simulate": tm["simulate_ms"], "value": tm["value_ms"], "reduce": tm["reduce_ms"]},
+        "roofline": None,
+    }
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        z = oracle.philox_normals(seed, n, n_steps)
+        _, ee_o, pfe_o = oracle.exposure_profile(a, b, sigma, r0, 1e6, 0.0, True, 0.0, pay, acc, times, n_steps, z)
+        cpu_s = time.perf_counter() - t0
+        _, ee, pfe = call(n)
+        out["cpu_baseline"] = {"value": units / cpu_s, "unit": "trade-paths-dates/s", "cores": 1, "kind": "port",
+                               "sample": f"the whole case ({n} paths), Philox draws and the reference-structured "
+                                         f"profile, one thread ({cpu_s:.2f} s)"}
+        out["ee_max_rel_err_vs_oracle"] = float(np.max(np.abs(ee - ee_o)) / max(np.max(np.abs(ee_o)), 1e-300))
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -436,11 +513,13 @@ def main():
                     help="short-rate model of the swaps workload: HullWhite1F (default) or HullWhite2F (G2++)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", choices=["swaps", "batch", "adjoint", "bermudan", "xva"], default="swaps",
+    ap.add_argument("--workload", choices=["swaps", "batch", "adjoint", "bermudan", "xva", "reference-case"],
+                    default="swaps",
                     help="swaps: the one-netting-set IRS exposure job (default); batch: 64 netting sets in one "
                          "launch; adjoint: pathwise CVA Greeks; bermudan: the learned continuation-value "
-                         "exposure of the Bermudan book; xva: the XVA reducer on the resident Monte Carlo cube "
-                         "(each prints its own JSON line)")
+                         "exposure of the Bermudan book; xva: the XVA reducer on the resident Monte Carlo cube; "
+                         "reference-case: the reference's own published GPU benchmark case (each prints its own "
+                         "JSON line)")
     ap.add_argument("--adj-paths", type=int, default=200_000)
     ap.add_argument("--berm-trades", type=int, default=200)
     ap.add_argument("--berm-train", type=int, default=1 << 18)
@@ -477,6 +556,9 @@ def main():
         return
     if args.workload == "xva":
         run_xva(args)
+        return
+    if args.workload == "reference-case":
+        run_reference_case(args)
         return
 
     from engine_quant_amd import eqhip
