@@ -141,3 +141,43 @@ def test_adjoint_and_batch_validation(ctx):
     code = eqhip.lib().eqhip_batch_create(ctx.handle, ctypes.byref(m), 1, pcs, npaths, eqhip._dp(d), 2, 2, 0,
                                           eqhip.RNG_HOST_DRAWS, eqhip.DISC_EULER, 0, ctypes.byref(h))
     assert code == eqhip.E_INVALID and "Philox" in eqhip.last_error()
+
+
+def test_two_contexts_from_two_threads_match_serial_runs(ctx):
+    """One context per host thread (its own stream): a plan executing on one while the PV/DV01 ladder and
+    a second plan run on another gives the results of running them one after the other."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    port = small_portfolio(n_trades=20, n_sets=3, seed=11)
+    pillars, zeros = np.array([0.5, 1.0, 2.0, 5.0, 10.0]), np.array([0.02, 0.021, 0.023, 0.026, 0.028])
+    dates = np.arange(1, 12 * 4 + 1) / 12.0
+    ctx2 = eqhip.Context(0)
+    plan_a = ctx.plan(MODEL, port, dates, 20000, 48, seed=3, want_trade_ee=True)
+    plan_b = ctx2.plan((0.1, 0.2, 0.01, 0.012, -0.7, 0.03), port, dates, 20000, 48, seed=4)
+    try:
+        serial_a = plan_a.execute(None, 0.02, 0.4)
+        serial_b = plan_b.execute(None, 0.02, 0.4)
+        serial_pv = ctx2.pv_dv01(pillars, zeros, port)
+
+        def other():
+            out = []
+            for _ in range(3):
+                out.append((plan_b.execute(None, 0.02, 0.4), ctx2.pv_dv01(pillars, zeros, port)))
+            return out
+
+        with ThreadPoolExecutor(max_workers=1) as pool:
+            fut = pool.submit(other)
+            mine = [plan_a.execute(None, 0.02, 0.4) for _ in range(3)]
+            theirs = fut.result()
+        for res in mine:
+            for k in serial_a:
+                assert np.array_equal(res[k], serial_a[k]), k
+        for res_b, pv in theirs:
+            for k in serial_b:
+                assert np.array_equal(res_b[k], serial_b[k]), k
+            for x, y in zip(pv, serial_pv):
+                assert np.array_equal(x, y)
+    finally:
+        plan_a.close()
+        plan_b.close()
+        ctx2.close()
