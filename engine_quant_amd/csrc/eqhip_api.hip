@@ -122,6 +122,9 @@ struct eqhip_plan {
         flow_cls, fix_off, fix_slot, fix_AB, set_off, set_nlive, set_cls, set_alpha, set_beta, X, items, cls_mat, tr_K,
         tr_absN, tr_N, tr_NK, tr_orig, item_first, partial, trade_ee, trade_cva, set_ee, set_pfe, set_cva, neff, qrank,
         d_dates, vplan, units, rows, R2, AB2, fix_B2, cube, xva_pos;
+    DevBuf tables;   // storage of the host-built tables above (they are views into it)
+    std::vector<uint64_t> h_neff, h_qrank;
+    bool direct_out = false;  // one-shot plans: outputs go straight to the caller's arrays
     ValuePlan vp{};  // host copy of the device descriptor (per-run pointers patched at launch)
     PinnedBuf h_out; // page-locked staging of the outputs
     DevBuf cva_w;    // per-date CVA factors
@@ -532,6 +535,8 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
         neff[k] = n;
         qrank[k] = q;
     }
+    P.h_neff = neff;
+    P.h_qrank = qrank;
 
     // memory check for the large buffers
     size_t free_b = 0, total_b = 0;
@@ -544,49 +549,52 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
         raise(EQHIP_E_CAPACITY, "exposure plan needs " + std::to_string(need / 1e9) + " GB of device memory, " +
                                     std::to_string(free_b / 1e9) + " GB free");
 
-    // uploads
-    upload(P.slot_of_step, slot_of_step, st);
-    upload(P.date_slot, date_slot, st);
-    upload(P.date_dfirst, date_dfirst, st);
-    upload(P.date_det, date_det, st);
-    upload(P.fdesc, fdesc, st);
-    upload(P.fam_off, fam_off, st);
-    upload(P.flow_d, flow_d, st);
-    upload(P.flow_tau, flow_tau, st);
-    upload(P.flow_cls, flow_cls, st);
-    upload(P.lflow_off, lflow_off, st);
-    upload(P.lflow, lflow, st);
-    upload(P.ltau, ltau, st);
-    upload(P.lseg, lseg, st);
-    upload(P.zcls_off, zcls_off, st);
-    upload(P.zcls, zcls, st);
-    upload(P.cls_mat, cls_mat, st);
-    upload(P.fix_off, fix_off, st);
-    upload(P.fix_slot, fix_slot, st);
-    upload(P.set_off, set_off, st);
-    upload(P.set_nlive, set_nlive, st);
-    upload(P.set_cls, set_cls, st);
-    upload(P.set_alpha, set_alpha, st);
-    upload(P.set_beta, set_beta, st);
-    upload(P.neff, neff, st);
-    upload(P.qrank, qrank, st);
-    upload(P.d_dates, P.dates, st);
+    // uploads: every host-built table in one allocation and one copy
+    TablePack pack;
+    pack.add(P.slot_of_step, slot_of_step);
+    pack.add(P.date_slot, date_slot);
+    pack.add(P.date_dfirst, date_dfirst);
+    pack.add(P.date_det, date_det);
+    pack.add(P.fdesc, fdesc);
+    pack.add(P.fam_off, fam_off);
+    pack.add(P.flow_d, flow_d);
+    pack.add(P.flow_tau, flow_tau);
+    pack.add(P.flow_cls, flow_cls);
+    pack.add(P.lflow_off, lflow_off);
+    pack.add(P.lflow, lflow);
+    pack.add(P.ltau, ltau);
+    pack.add(P.lseg, lseg);
+    pack.add(P.zcls_off, zcls_off);
+    pack.add(P.zcls, zcls);
+    pack.add(P.cls_mat, cls_mat);
+    pack.add(P.fix_off, fix_off);
+    pack.add(P.fix_slot, fix_slot);
+    pack.add(P.set_off, set_off);
+    pack.add(P.set_nlive, set_nlive);
+    pack.add(P.set_cls, set_cls);
+    pack.add(P.set_alpha, set_alpha);
+    pack.add(P.set_beta, set_beta);
+    pack.add(P.neff, neff);
+    pack.add(P.qrank, qrank);
+    pack.add(P.d_dates, P.dates);
     if (P.want_trade) {
-        upload(P.items, items, st);
-        upload(P.tr_K, tr_K, st);
-        upload(P.tr_absN, tr_absN, st);
-        upload(P.tr_N, tr_N, st);
-        upload(P.tr_NK, tr_NK, st);
-        upload(P.tr_orig, tr_orig, st);
-        upload(P.item_first, item_first, st);
+        pack.add(P.items, items);
+        pack.add(P.tr_K, tr_K);
+        pack.add(P.tr_absN, tr_absN);
+        pack.add(P.tr_N, tr_N);
+        pack.add(P.tr_NK, tr_NK);
+        pack.add(P.tr_orig, tr_orig);
+        pack.add(P.item_first, item_first);
         P.partial.alloc((size_t)P.n_trades * n_dates * P.n_chunks * 8);
         P.trade_ee.alloc((size_t)P.n_trades * n_dates * 8);
         P.trade_cva.alloc((size_t)P.n_trades * 8);
     }
+    DevBuf tau, ftau;  // views into the plan's tables
+    pack.add(tau, ab_tau);
+    pack.add(ftau, fix_tau);
+    pack.flush(P.tables);
     // ZCB factor tables, computed on the device
     {
-        DevBuf tau;
-        upload(tau, ab_tau, st);
         P.AB.alloc(ab_tau.size() * sizeof(double2));
         int n = (int)ab_tau.size();
         const eqhip_model_hw2f& m2 = P.model2;
@@ -599,8 +607,6 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
                                P.model.a, P.model.b, P.model.sigma, P.AB.as<double2>());
         }
         HIPCHK(hipGetLastError());
-        DevBuf ftau;
-        upload(ftau, fix_tau, st);
         P.fix_AB.alloc(std::max<size_t>(1, fix_tau.size()) * sizeof(double2));
         if (P.factors == 2) P.fix_B2.alloc(std::max<size_t>(1, fix_tau.size()) * sizeof(double));
         int nf = (int)fix_tau.size();
@@ -614,7 +620,6 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
                                    P.model.a, P.model.b, P.model.sigma, P.fix_AB.as<double2>());
             HIPCHK(hipGetLastError());
         }
-        HIPCHK(hipStreamSynchronize(st));  // tau buffers die here
     }
     if (!all_steps) P.R.alloc((size_t)P.n_slots * mc->n_paths * 8);
     if (P.factors == 2) P.R2.alloc((size_t)P.n_slots * mc->n_paths * 8);
@@ -712,9 +717,8 @@ void build_value_plan(eqhip_plan& P) {
     for (int k = 0; k < P.n_dates; ++k)
         for (int c = 0; c < P.n_chunks; ++c) P.unit_list.push_back(make_int4(0, k, c, 0));
     upload(P.units, P.unit_list, P.ctx->stream);
-    std::vector<uint64_t> neff(P.n_dates), qrank(P.n_dates);
-    HIPCHK(hipMemcpy(neff.data(), P.neff.p, P.n_dates * 8, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(qrank.data(), P.qrank.p, P.n_dates * 8, hipMemcpyDeviceToHost));
+    const std::vector<uint64_t>& neff = P.h_neff;
+    const std::vector<uint64_t>& qrank = P.h_qrank;
     std::vector<SelectRow> rows;
     for (int s = 0; s < P.S; ++s)
         for (int k = 0; k < P.n_dates; ++k) {
@@ -903,17 +907,24 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
             o.off = staged;
             staged += (o.bytes + 255) & ~(size_t)255;
         }
-    reserve(P.h_out, staged);
-    for (const Out& o : outs)
-        if (o.dst) HIPCHK(hipMemcpyAsync(P.h_out.as<char>() + o.off, o.src->p, o.bytes, hipMemcpyDeviceToHost, st));
+    if (P.direct_out) {  // a one-shot plan: no page-locked staging to set up for a single use
+        for (const Out& o : outs)
+            if (o.dst) HIPCHK(hipMemcpyAsync(o.dst, o.src->p, o.bytes, hipMemcpyDeviceToHost, st));
+    } else {
+        reserve(P.h_out, staged);
+        for (const Out& o : outs)
+            if (o.dst)
+                HIPCHK(hipMemcpyAsync(P.h_out.as<char>() + o.off, o.src->p, o.bytes, hipMemcpyDeviceToHost, st));
+    }
     if (out->path_mtm)
         HIPCHK(hipMemcpyAsync(out->path_mtm, dpath.p, (size_t)P.n_dates * P.n_trades * np * 8, hipMemcpyDeviceToHost, st));
     if (out->set_mtm)
         HIPCHK(hipMemcpyAsync(out->set_mtm, set_values, (size_t)P.n_dates * P.S * np * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     P.executed = true;
-    for (const Out& o : outs)
-        if (o.dst) std::memcpy(o.dst, P.h_out.as<char>() + o.off, o.bytes);
+    if (!P.direct_out)
+        for (const Out& o : outs)
+            if (o.dst) std::memcpy(o.dst, P.h_out.as<char>() + o.off, o.bytes);
 
     float ms[4];
     for (int i = 0; i < 4; ++i) HIPCHK(hipEventElapsedTime(&ms[i], P.ev[i], P.ev[i + 1]));
@@ -1086,6 +1097,7 @@ void profile_batch(eqhip_context* ctx, double a, double b, double sigma, double 
     eqhip_plan P;
     P.ctx = ctx;
     P.device = ctx->device;
+    P.direct_out = true;
     if (model2) {
         P.factors = 2;
         P.model2 = *model2;
@@ -1717,7 +1729,8 @@ int eqhip_exposure_run(eqhip_context* ctx, const eqhip_model_hw1f* model, const 
         check_ctx(ctx);
         eqhip_plan P;
         P.ctx = ctx;
-    P.device = ctx->device;
+        P.device = ctx->device;
+        P.direct_out = true;
         int want_trade = out && (out->trade_ee || out->trade_cva || out->path_mtm);
         build_plan(P, model, portfolio, dates, n_dates, mc, want_trade);
         execute_plan(P, host_draws, credit, out);

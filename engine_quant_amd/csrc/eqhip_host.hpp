@@ -47,23 +47,30 @@ int guarded(F&& f) {
     }
 }
 
-// Device allocation owned by its holder; freed when the holder dies.
+// Device allocation owned by its holder and freed when the holder dies, or a view into another
+// holder's allocation (set by TablePack), which the other holder outlives.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    bool owned = true;
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
     }
     void alloc(size_t n) {
-        if (p) {
-            (void)hipFree(p);
-            p = nullptr;
-        }
+        if (p && owned) (void)hipFree(p);
+        p = nullptr;
+        owned = true;
         bytes = n;
         if (n) HIPCHK(hipMalloc(&p, n));
+    }
+    void view(void* base, size_t n) {
+        if (p && owned) (void)hipFree(p);
+        p = base;
+        bytes = n;
+        owned = false;
     }
     template <class T>
     T* as() const {
@@ -101,6 +108,33 @@ template <class B>
 void reserve(B& b, size_t n) {
     if (b.bytes < n) b.alloc(std::max(n, b.bytes + b.bytes / 2));
 }
+
+// Many small host tables uploaded as one: add() copies each table into a host staging area at a
+// 256-byte aligned offset and remembers its destination; flush() makes one device allocation, one
+// copy, and points every destination at its slice (the storage buffer must outlive the views).
+struct TablePack {
+    struct Entry {
+        DevBuf* dst;
+        size_t off, bytes;
+    };
+    std::vector<char> host;
+    std::vector<Entry> entries;
+    template <class T>
+    void add(DevBuf& dst, const std::vector<T>& v) {
+        const size_t bytes = std::max<size_t>(1, v.size()) * sizeof(T);
+        const size_t off = (host.size() + 255) & ~(size_t)255;
+        host.resize(off + bytes, 0);
+        if (!v.empty()) std::copy((const char*)v.data(), (const char*)v.data() + v.size() * sizeof(T), host.data() + off);
+        entries.push_back(Entry{&dst, off, bytes});
+    }
+    void flush(DevBuf& storage) {
+        storage.alloc(std::max<size_t>(1, host.size()));
+        if (!host.empty()) HIPCHK(hipMemcpy(storage.p, host.data(), host.size(), hipMemcpyHostToDevice));
+        for (const Entry& e : entries) e.dst->view(static_cast<char*>(storage.p) + e.off, e.bytes);
+        host.clear();
+        entries.clear();
+    }
+};
 
 // Synchronous copy: the source vector may be a temporary of the caller.
 template <class T>
