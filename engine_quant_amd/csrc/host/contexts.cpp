@@ -48,6 +48,10 @@ ExecutionContext::ExecutionContext(const Params& params) {
     if (discretisation_ != "euler" && discretisation_ != "exact")
         throw std::invalid_argument("ExecutionContext: discretisation desconocida: '" + discretisation_ +
                                     "' (valores validos: \"euler\", \"exact\")");
+    simulation_ = to_lower(get_string(params, "simulation", "per_group"));
+    if (simulation_ != "per_group" && simulation_ != "shared")
+        throw std::invalid_argument("ExecutionContext: simulation desconocida: '" + simulation_ +
+                                    "' (valores validos: \"per_group\", \"shared\")");
 }
 
 }  // namespace engine

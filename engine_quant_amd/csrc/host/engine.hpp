@@ -188,9 +188,12 @@ public:
     const std::string& precision() const { return precision_; }
     const std::string& rng() const { return rng_; }
     const std::string& discretisation() const { return discretisation_; }
+    // "per_group" (the reference's price_many: one simulation per calendar group, each on its own grid)
+    // or "shared" (one simulation for every group, on the grid of the latest date)
+    const std::string& simulation() const { return simulation_; }
 
 private:
-    std::string backend_, precision_, rng_, discretisation_;
+    std::string backend_, precision_, rng_, discretisation_, simulation_;
 };
 
 bool is_gpu_backend_available();
@@ -209,6 +212,13 @@ ExposureProfile irs_hull_white_exposure_profile(const ExecutionContext& executio
                                                 const std::vector<double>& accruals,
                                                 const std::vector<double>& monitoring_times, std::size_t n_steps,
                                                 std::uint64_t n_paths, std::uint64_t seed);
+// Every trade's profile at its own reset dates from one simulation (one plan, each trade its own
+// netting set, monitoring dates the union of the trades' reset dates): the shared-simulation form of
+// price_many.
+std::vector<ExposureProfile> shared_exposure_profiles(const IModel& model,
+                                                      const std::vector<const IrSwapProduct*>& products,
+                                                      const PricingContext& pricing,
+                                                      const ExecutionContext& execution);
 std::vector<ExposureProfile> irs_hull_white_exposure_profile_batch(
     const ExecutionContext& execution, double a, double b, double sigma, double r0,
     const std::vector<double>& notionals, const std::vector<double>& fixed_rates, double start,

@@ -262,6 +262,68 @@ std::vector<double> unilateral_cva_from_exposure_2f_batch(const ExecutionContext
     return out;
 }
 
+std::vector<ExposureProfile> shared_exposure_profiles(const IModel& model,
+                                                      const std::vector<const IrSwapProduct*>& products,
+                                                      const PricingContext& pricing,
+                                                      const ExecutionContext& execution) {
+    require_gpu(execution);
+    const std::size_t n = products.size();
+    std::vector<int32_t> set_id(n), off(n + 1, 0);
+    std::vector<double> notional(n), fixed(n), start(n), pay, acc, dates;
+    std::vector<std::vector<double>> own(n);
+    for (std::size_t j = 0; j < n; ++j) {
+        const IrSwapProduct& p = *products[j];
+        set_id[j] = (int32_t)j;
+        notional[j] = p.notional();
+        fixed[j] = p.fixed_rate();
+        start[j] = p.start();
+        pay.insert(pay.end(), p.payment_times().begin(), p.payment_times().end());
+        acc.insert(acc.end(), p.accruals().begin(), p.accruals().end());
+        off[j + 1] = (int32_t)pay.size();
+        own[j] = reset_dates(p);
+        dates.insert(dates.end(), own[j].begin(), own[j].end());
+    }
+    std::sort(dates.begin(), dates.end());
+    dates.erase(std::unique(dates.begin(), dates.end()), dates.end());
+    const eqhip_portfolio pf{(int32_t)n, (int32_t)n, set_id.data(), notional.data(), fixed.data(), start.data(),
+                             off.data(), pay.data(), acc.data()};
+    const eqhip_mc mc{pricing.n_paths(), (uint32_t)pricing.n_steps(), pricing.seed(), EQHIP_RNG_PHILOX,
+                      disc_of(execution)};
+    eqhip_plan* plan = nullptr;
+    if (const auto* hw2 = dynamic_cast<const HullWhite2FModel*>(&model)) {
+        const eqhip_model_hw2f m{hw2->a(), hw2->b(), hw2->sigma(), hw2->eta(), hw2->rho(), hw2->r0()};
+        check(eqhip_plan_create_2f(gpu_context(), &m, &pf, dates.data(), (int32_t)dates.size(), &mc, 0, &plan));
+    } else {
+        const auto& hw1 = require_hw1f(model, "ExposureProfileMeasure");
+        const eqhip_model_hw1f m{hw1.a(), hw1.b(), hw1.sigma(), hw1.r0()};
+        check(eqhip_plan_create(gpu_context(), &m, &pf, dates.data(), (int32_t)dates.size(), &mc, 0, &plan));
+    }
+    struct Guard {
+        eqhip_plan* p;
+        ~Guard() { eqhip_plan_destroy(p); }
+    } guard{plan};
+    const std::size_t nd = dates.size();
+    std::vector<double> ee(n * nd), pfe(n * nd);
+    eqhip_outputs out{};
+    out.set_ee = ee.data();
+    out.set_pfe = pfe.data();
+    check(eqhip_plan_execute(plan, nullptr, nullptr, &out));
+    // every date <= 0: the plan returns one deterministic row (exposure.rs:98-108)
+    const bool det = !(dates.back() > 0.0);
+    std::vector<ExposureProfile> profiles(n);
+    for (std::size_t j = 0; j < n; ++j) {
+        ExposureProfile& pr = profiles[j];
+        for (double t : own[j]) {
+            const std::size_t k = (std::size_t)(std::lower_bound(dates.begin(), dates.end(), t) - dates.begin());
+            pr.times.push_back(t);
+            pr.ee.push_back(ee[j * nd + k]);
+            pr.pfe_95.push_back(pfe[j * nd + k]);
+            if (det) break;
+        }
+    }
+    return profiles;
+}
+
 // ---- batched helpers (measure.cpp:238-325) ----------------------------------------------------
 
 std::vector<ExposureProfile> compute_exposure_profile_batch(const IModel& model,

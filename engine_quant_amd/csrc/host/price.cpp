@@ -177,10 +177,27 @@ PriceResult price(const Registries& registries, const IProduct& product, const s
     return result;
 }
 
+static PriceBatchResult price_batch_with_profiles(const Registries& registries,
+                                                  const std::vector<const IProduct*>& products,
+                                                  const std::vector<MeasureSpec>& measures, const IModel& model,
+                                                  const MarketSnapshot& market, const PricingContext& pricing,
+                                                  const ExecutionContext& execution,
+                                                  const std::vector<ExposureProfile>* given_profiles);
+
 PriceBatchResult price_batch(const Registries& registries, const std::vector<const IProduct*>& products,
                              const std::vector<MeasureSpec>& measures, const IModel& model,
                              const MarketSnapshot& market, const PricingContext& pricing,
                              const ExecutionContext& execution) {
+    return price_batch_with_profiles(registries, products, measures, model, market, pricing, execution, nullptr);
+}
+
+// price_batch; profiles (nullable) are the batch's exposure profiles when the caller already has them.
+static PriceBatchResult price_batch_with_profiles(const Registries& registries,
+                                                  const std::vector<const IProduct*>& products,
+                                                  const std::vector<MeasureSpec>& measures, const IModel& model,
+                                                  const MarketSnapshot& market, const PricingContext& pricing,
+                                                  const ExecutionContext& execution,
+                                                  const std::vector<ExposureProfile>* given_profiles) {
     if (products.empty()) throw std::invalid_argument("engine::price_batch: el lote no puede estar vacío");
     std::vector<ResolvedMeasure> resolved;
     for (const MeasureSpec& spec : measures) {
@@ -193,6 +210,7 @@ PriceBatchResult price_batch(const Registries& registries, const std::vector<con
 
     std::unordered_map<std::string, std::vector<MeasureResult>> computed;
     std::optional<std::vector<ExposureProfile>> profiles;  // one simulation for the whole batch
+    if (given_profiles) profiles = *given_profiles;
     auto get_profiles = [&]() -> const std::vector<ExposureProfile>& {
         if (!profiles) profiles = compute_exposure_profile_batch(model, irs, pricing, execution);
         return *profiles;
@@ -255,11 +273,63 @@ PriceBatchResult price_batch(const Registries& registries, const std::vector<con
     return result;
 }
 
+// price_many with one simulation for every calendar group: the profiles of all trades from one plan
+// (each trade its own netting set), then per group the CVA on the group's own dates and the measures
+// that need no simulation, in price_batch's order and aliases.
+static PriceBatchResult price_many_shared(const Registries& registries, const std::vector<const IProduct*>& products,
+                                   const std::vector<MeasureSpec>& measures, const IModel& model,
+                                   const MarketSnapshot& market, const PricingContext& pricing,
+                                   const ExecutionContext& execution) {
+    std::vector<const IrSwapProduct*> irs;
+    for (const IProduct* p : products) {
+        const auto* s = dynamic_cast<const IrSwapProduct*>(p);
+        if (!s)
+            throw std::invalid_argument("engine::price_many: simulation \"shared\" serves IRSwap trades only, got '" +
+                                        p->type_name() + "'");
+        if (s->use_par_rate())
+            throw std::invalid_argument(
+                "engine::price_many: los trades deben traer fixed_rate explícito (use_par_rate no soportado en lote)");
+        irs.push_back(s);
+    }
+    bool needs_profiles = false;
+    for (const MeasureSpec& spec : measures) {
+        const std::string t = resolve_measure_name(spec.name).registered_type;
+        if (t == "ExposureProfile" || t == "UnilateralCVA") needs_profiles = true;
+    }
+    std::vector<ExposureProfile> all;
+    if (needs_profiles) all = shared_exposure_profiles(model, irs, pricing, execution);
+    // the trades' calendar groups (price_many's grouping), each priced by price_batch's measure logic
+    // on profiles taken from the shared run
+    std::unordered_map<std::string, std::vector<std::size_t>> groups;
+    std::vector<std::string> group_order;
+    for (std::size_t i = 0; i < irs.size(); ++i) {
+        auto ins = groups.try_emplace(calendar_group_key(*irs[i]));
+        if (ins.second) group_order.push_back(ins.first->first);
+        ins.first->second.push_back(i);
+    }
+    PriceBatchResult result(products.size());
+    for (const std::string& key : group_order) {
+        const std::vector<std::size_t>& idx = groups.at(key);
+        std::vector<const IProduct*> group;
+        std::vector<ExposureProfile> group_profiles;
+        for (std::size_t i : idx) {
+            group.push_back(products[i]);
+            if (needs_profiles) group_profiles.push_back(all[i]);
+        }
+        PriceBatchResult g = price_batch_with_profiles(registries, group, measures, model, market, pricing, execution,
+                                                       needs_profiles ? &group_profiles : nullptr);
+        for (std::size_t j = 0; j < idx.size(); ++j) result[idx[j]] = PriceBatchResultEntry{idx[j], std::move(g[j].measures)};
+    }
+    return result;
+}
+
 PriceBatchResult price_many(const Registries& registries, const std::vector<const IProduct*>& products,
                             const std::vector<MeasureSpec>& measures, const IModel& model,
                             const MarketSnapshot& market, const PricingContext& pricing,
                             const ExecutionContext& execution) {
     if (products.empty()) throw std::invalid_argument("engine::price_many: la lista de trades no puede estar vacía");
+    if (execution.simulation() == "shared") return price_many_shared(registries, products, measures, model, market,
+                                                                      pricing, execution);
     std::unordered_map<std::string, std::vector<std::size_t>> groups;
     std::vector<std::string> group_order;
     for (std::size_t i = 0; i < products.size(); ++i) {

@@ -149,3 +149,50 @@ def test_exact_discretisation_is_selectable():
                 acc=np.ones(5))
     ref = oracle.portfolio_exposure((0.1, 0.03, 0.01, 0.02), port, [0, 1, 2, 3, 4.0], 208, z, scheme=1)
     assert np.allclose(res["ExpectedExposure"].primary, ref["set_ee"][0], rtol=1e-10, atol=1e-6)
+
+
+def test_price_many_on_one_shared_simulation():
+    """ExecutionContext simulation="shared": every calendar group from one simulation. With one grid for
+    all groups (same latest date) it equals the per-group run up to the rounding of the annuity sums (the
+    plan's flow lists differ); otherwise each trade's profile is the oracle's on the shared grid, at the
+    trade's own reset dates."""
+    eng = engine.Engine()
+    model = eng.create_model("HullWhite1F", _hull_white_params())
+    market, pr = _market_with_credit(0.02, 0.4), _golden_pricing(3000, 5)
+    measures = ["PV", "DV01", "ExpectedExposure", "PFE95", "UnilateralCVA"]
+    shared = _gpu_execution(simulation="shared")
+    # three calendars ending on the same reset date 4.0: one grid either way
+    same_end = [eng.create_product("IRSwap", {"notional": 1e6, "fixed_rate": 0.02, "payment_times": [1, 2, 3, 4, 5.0],
+                                              "accruals": [1.0] * 5}),
+                eng.create_product("IRSwap", {"notional": -2e6, "fixed_rate": 0.021, "start": 1.0,
+                                              "payment_times": [2, 3, 4, 5.0], "accruals": [1.0] * 4}),
+                eng.create_product("IRSwap", {"notional": 5e5, "fixed_rate": 0.019,
+                                              "payment_times": [2, 4, 6.0], "accruals": [2.0] * 3})]
+    a = eng.price_many(same_end, measures, model, market, pr, _gpu_execution())
+    b = eng.price_many(same_end, measures, model, market, pr, shared)
+    for x, y in zip(a, b):
+        for m in measures:
+            assert np.allclose(x.measures[m].primary, y.measures[m].primary, rtol=1e-12, atol=1e-8), m
+            if x.measures[m].has_scalar:
+                assert abs(x.measures[m].scalar - y.measures[m].scalar) <= 1e-12 * max(1.0, abs(x.measures[m].scalar)), m
+    # different latest dates: the shared grid is that of the latest date of all
+    trades = same_end + [eng.create_product("IRSwap", {"notional": 1e6, "fixed_rate": 0.02,
+                                                       "payment_times": [1, 2, 3.0], "accruals": [1.0] * 3})]
+    c = eng.price_many(trades, ["ExpectedExposure", "PFE95"], model, market, pr, shared)
+    specs = [([1, 2, 3, 4, 5.0], [1.0] * 5, 0.0, 1e6, 0.02), ([2, 3, 4, 5.0], [1.0] * 4, 1.0, -2e6, 0.021),
+             ([2, 4, 6.0], [2.0] * 3, 0.0, 5e5, 0.019), ([1, 2, 3.0], [1.0] * 3, 0.0, 1e6, 0.02)]
+    own = [[s] + p[:-1] for p, _, s, _, _ in specs]
+    union = sorted({t for d in own for t in d})
+    port = {"set_id": np.arange(4, dtype=np.int32), "notional": np.array([x[3] for x in specs]),
+            "fixed_rate": np.array([x[4] for x in specs]), "start": np.array([x[2] for x in specs]),
+            "flow_off": np.cumsum([0] + [len(x[0]) for x in specs]).astype(np.int32),
+            "pay": np.concatenate([x[0] for x in specs]), "acc": np.concatenate([x[1] for x in specs])}
+    n_steps = 208
+    ref = oracle.portfolio_exposure((0.1, 0.03, 0.01, 0.02), port, union, n_steps,
+                                    oracle.philox_normals(5, 3000, n_steps))
+    for j, res in enumerate(c):
+        k = [union.index(t) for t in own[j]]
+        assert np.allclose(res.measures["ExpectedExposure"].primary, ref["set_ee"][j, k], rtol=1e-10, atol=1e-6)
+        assert np.allclose(res.measures["PFE95"].primary, ref["set_pfe"][j, k], rtol=1e-10, atol=1e-6)
+    with pytest.raises(ValueError, match="simulation"):
+        engine.ExecutionContext({"backend": "gpu", "simulation": "sometimes"})
