@@ -497,6 +497,59 @@ def run_reference_case(args):
     print(json.dumps(out), flush=True)
 
 
+def run_caller(args):
+    """The portfolio caller path: Engine.price_many over the headline portfolio through the C++ host (all
+    five measures), one simulation per calendar group (the reference's price_many, price.cpp:534-578)
+    against one simulation for every group (ExecutionContext simulation="shared")."""
+    import engine_quant_amd as engine
+    from engine_quant_amd.portfolios import MODEL, swap_portfolio, zero_curve
+
+    pillars, zeros = zero_curve(args.seed)
+    port = swap_portfolio(args.trades, seed=args.seed, pillars=pillars, zeros=zeros)
+    eng = engine.Engine()
+    a, b, sigma, r0 = MODEL
+    model = eng.create_model("HullWhite1F", {"a": a, "b": b, "sigma": sigma, "r0": r0})
+    off = port.flow_off
+    trades = [eng.create_product("IRSwap", {"notional": float(port.notional[j]), "fixed_rate": float(port.fixed_rate[j]),
+                                            "start": float(port.start[j]), "payment_times": list(port.pay[off[j]:off[j + 1]]),
+                                            "accruals": list(port.acc[off[j]:off[j + 1]])})
+              for j in range(port.n_trades)]
+    market = engine.MarketSnapshot(pillars=list(pillars), zero_rates=list(zeros), hazard_rate=0.02, recovery_rate=0.4)
+    n_paths, n_steps = args.paths, 360
+    pricing = engine.PricingContext({"n_paths": float(n_paths), "n_steps": float(n_steps), "seed": float(args.seed)})
+    measures = ["PV", "DV01", "ExpectedExposure", "PFE95", "UnilateralCVA"]
+    own_dates = sum(1 + int(off[j + 1] - off[j]) - 1 for j in range(port.n_trades))  # start + all but the last payment
+    units = n_paths * own_dates
+    groups = len({(float(port.start[j]), tuple(port.pay[off[j]:off[j + 1]])) for j in range(port.n_trades)})
+    res = {}
+    for mode in ("per_group", "shared"):
+        ex = engine.ExecutionContext({"backend": "gpu", "discretisation": "exact", "simulation": mode})
+        eng.price_many(trades, measures, model, market, pricing, ex)  # warm-up
+        runs = []
+        for _ in range(max(3, args.steps // 5)):
+            t0 = time.perf_counter()
+            out = eng.price_many(trades, measures, model, market, pricing, ex)
+            runs.append(time.perf_counter() - t0)
+            print(f"# {mode}: {1000 * runs[-1]:.1f} ms", file=sys.stderr, flush=True)
+        cva = float(sum(r.measures["UnilateralCVA"].scalar for r in out))
+        res[mode] = {"ms": 1000.0 * float(np.median(runs)), "cva_sum": cva}
+    value = units / (res["shared"]["ms"] / 1000.0)
+    out = {
+        "metric": METRIC, "value": value, "unit": "trade-paths-dates/s", "n_gpus": 1,
+        "steps": max(3, args.steps // 5), "warmup": 1, "ms_per_step": res["shared"]["ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "price-many-caller-path", "n_trades": port.n_trades, "calendar_groups": groups,
+                   "n_paths": n_paths, "n_steps": n_steps, "monitoring": "each trade's reset dates",
+                   "measures": measures, "discretisation": "exact"},
+        "per_group": {"ms_per_call": res["per_group"]["ms"],
+                      "value": units / (res["per_group"]["ms"] / 1000.0), "cva_sum": res["per_group"]["cva_sum"]},
+        "shared": {"ms_per_call": res["shared"]["ms"], "cva_sum": res["shared"]["cva_sum"]},
+        "speedup_shared_vs_per_group": res["per_group"]["ms"] / res["shared"]["ms"],
+        "roofline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -513,13 +566,14 @@ def main():
                     help="short-rate model of the swaps workload: HullWhite1F (default) or HullWhite2F (G2++)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", choices=["swaps", "batch", "adjoint", "bermudan", "xva", "reference-case"],
+    ap.add_argument("--workload", choices=["swaps", "batch", "adjoint", "bermudan", "xva", "reference-case",
+                                           "caller"],
                     default="swaps",
                     help="swaps: the one-netting-set IRS exposure job (default); batch: 64 netting sets in one "
                          "launch; adjoint: pathwise CVA Greeks; bermudan: the learned continuation-value "
                          "exposure of the Bermudan book; xva: the XVA reducer on the resident Monte Carlo cube; "
-                         "reference-case: the reference's own published GPU benchmark case (each prints its own "
-                         "JSON line)")
+                         "reference-case: the reference's own published GPU benchmark case; caller: price_many "
+                         "through the host engine, per calendar group and shared (each prints its own JSON line)")
     ap.add_argument("--adj-paths", type=int, default=200_000)
     ap.add_argument("--berm-trades", type=int, default=200)
     ap.add_argument("--berm-train", type=int, default=1 << 18)
@@ -559,6 +613,9 @@ def main():
         return
     if args.workload == "reference-case":
         run_reference_case(args)
+        return
+    if args.workload == "caller":
+        run_caller(args)
         return
 
     from engine_quant_amd import eqhip

@@ -124,7 +124,8 @@ struct eqhip_plan {
         d_dates, vplan, units, rows, R2, AB2, fix_B2, cube, xva_pos;
     DevBuf tables;   // storage of the host-built tables above (they are views into it)
     std::vector<uint64_t> h_neff, h_qrank;
-    bool direct_out = false;  // one-shot plans: outputs go straight to the caller's arrays
+    bool direct_out = false;  // one-shot plans: outputs go straight to the caller's arrays, and the large
+                              // work buffers are views into the context's pool (under its one-shot lock)
     ValuePlan vp{};  // host copy of the device descriptor (per-run pointers patched at launch)
     PinnedBuf h_out; // page-locked staging of the outputs
     DevBuf cva_w;    // per-date CVA factors
@@ -204,6 +205,17 @@ void choose_launch(eqhip_plan& P) {
 }
 
 void build_value_plan(eqhip_plan& P);
+
+// A plan's large work buffer: its own allocation, or for a one-shot plan a view into the context's pool
+// buffer of that role (grown to the largest request and kept).
+void work_buffer(eqhip_plan& P, DevBuf& b, DevBuf& pool, size_t bytes) {
+    if (!P.direct_out) {
+        b.alloc(bytes);
+        return;
+    }
+    reserve(pool, bytes);
+    b.view(pool.p, bytes);
+}
 
 // all_steps: r is stored at every grid step (slot = step), so plans of a batch can share one
 // simulation; the batch then owns the short-rate buffer.
@@ -585,7 +597,7 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
         pack.add(P.tr_NK, tr_NK);
         pack.add(P.tr_orig, tr_orig);
         pack.add(P.item_first, item_first);
-        P.partial.alloc((size_t)P.n_trades * n_dates * P.n_chunks * 8);
+        work_buffer(P, P.partial, P.ctx->pool_partial, (size_t)P.n_trades * n_dates * P.n_chunks * 8);
         P.trade_ee.alloc((size_t)P.n_trades * n_dates * 8);
         P.trade_cva.alloc((size_t)P.n_trades * 8);
     }
@@ -621,14 +633,15 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
             HIPCHK(hipGetLastError());
         }
     }
-    if (!all_steps) P.R.alloc((size_t)P.n_slots * mc->n_paths * 8);
-    if (P.factors == 2) P.R2.alloc((size_t)P.n_slots * mc->n_paths * 8);
-    P.X.alloc((size_t)P.S * n_dates * mc->n_paths * 8);
+    if (!all_steps) work_buffer(P, P.R, P.ctx->pool_R, (size_t)P.n_slots * mc->n_paths * 8);
+    if (P.factors == 2) work_buffer(P, P.R2, P.ctx->pool_R2, (size_t)P.n_slots * mc->n_paths * 8);
+    work_buffer(P, P.X, P.ctx->pool_X, (size_t)P.S * n_dates * mc->n_paths * 8);
     if (P.keep_cube) P.cube.alloc((size_t)n_dates * P.S * mc->n_paths * 8);
     P.set_ee.alloc((size_t)P.S * n_dates * 8);
     P.set_pfe.alloc((size_t)P.S * n_dates * 8);
     P.set_cva.alloc((size_t)P.S * 8);
-    if (mc->rng == EQHIP_RNG_HOST_DRAWS) P.Z.alloc((size_t)P.factors * mc->n_steps * mc->n_paths * 8);
+    if (mc->rng == EQHIP_RNG_HOST_DRAWS)
+        work_buffer(P, P.Z, P.ctx->pool_Z, (size_t)P.factors * mc->n_steps * mc->n_paths * 8);
     P.h_T = T;
     P.h_fdesc = fdesc;
     P.h_fam_off = fam_off;
@@ -1094,6 +1107,7 @@ void profile_batch(eqhip_context* ctx, double a, double b, double sigma, double 
                        off.data(), P_.data(), A_.data()};
     eqhip_model_hw1f model{a, b, sigma, r0};
     eqhip_mc mc{n_paths, n_steps, seed, rng, disc};
+    std::lock_guard<std::mutex> lock(ctx->oneshot_lock);
     eqhip_plan P;
     P.ctx = ctx;
     P.device = ctx->device;
@@ -1727,6 +1741,7 @@ int eqhip_exposure_run(eqhip_context* ctx, const eqhip_model_hw1f* model, const 
                        const eqhip_credit* credit, eqhip_outputs* out) {
     return guarded([&] {
         check_ctx(ctx);
+        std::lock_guard<std::mutex> lock(ctx->oneshot_lock);
         eqhip_plan P;
         P.ctx = ctx;
         P.device = ctx->device;

@@ -154,4 +154,9 @@ struct eqhip_context {
     std::mutex staging_lock;
     eqhip_host::PinnedBuf host_stage;
     eqhip_host::DevBuf dev_stage;
+    // work buffers lent to the plans of the one-shot exposure entry points (paths, exposure rows, host
+    // draws, per-trade partial sums): they grow to the largest call and stay, so repeated calls do not
+    // allocate and free hundreds of MB each time; the lock makes one-shot calls on one context take turns
+    std::mutex oneshot_lock;
+    eqhip_host::DevBuf pool_R, pool_R2, pool_X, pool_Z, pool_partial;
 };

@@ -188,8 +188,8 @@ public:
     const std::string& precision() const { return precision_; }
     const std::string& rng() const { return rng_; }
     const std::string& discretisation() const { return discretisation_; }
-    // "per_group" (the reference's price_many: one simulation per calendar group, each on its own grid)
-    // or "shared" (one simulation for every group, on the grid of the latest date)
+    // "per_group" (the reference's price_many: each calendar group simulated on its own grid) or
+    // "shared" (every group on one scenario set: the grid of the latest date and the same paths)
     const std::string& simulation() const { return simulation_; }
 
 private:
@@ -212,9 +212,9 @@ ExposureProfile irs_hull_white_exposure_profile(const ExecutionContext& executio
                                                 const std::vector<double>& accruals,
                                                 const std::vector<double>& monitoring_times, std::size_t n_steps,
                                                 std::uint64_t n_paths, std::uint64_t seed);
-// Every trade's profile at its own reset dates from one simulation (one plan, each trade its own
-// netting set, monitoring dates the union of the trades' reset dates): the shared-simulation form of
-// price_many.
+// Every trade's profile at its own reset dates, all on one scenario set: the grid of the latest reset
+// date of all trades and the same Philox paths for every calendar group (the shared-simulation form of
+// price_many).
 std::vector<ExposureProfile> shared_exposure_profiles(const IModel& model,
                                                       const std::vector<const IrSwapProduct*>& products,
                                                       const PricingContext& pricing,

@@ -267,58 +267,59 @@ std::vector<ExposureProfile> shared_exposure_profiles(const IModel& model,
                                                       const PricingContext& pricing,
                                                       const ExecutionContext& execution) {
     require_gpu(execution);
-    const std::size_t n = products.size();
-    std::vector<int32_t> set_id(n), off(n + 1, 0);
-    std::vector<double> notional(n), fixed(n), start(n), pay, acc, dates;
-    std::vector<std::vector<double>> own(n);
-    for (std::size_t j = 0; j < n; ++j) {
+    // one scenario set for every calendar: each group is valued in a plan of its own (the per-trade
+    // PFE rows of one group at a time, not of the whole book at once) whose dates end at the latest
+    // reset date of all, so every plan has the same grid; the counter-based Philox stream then gives
+    // every group the same paths
+    double t_last = -INFINITY;
+    for (const IrSwapProduct* p : products)
+        for (double t : reset_dates(*p)) t_last = std::max(t_last, t);
+    std::map<std::string, std::vector<std::size_t>> groups;
+    for (std::size_t j = 0; j < products.size(); ++j) {
         const IrSwapProduct& p = *products[j];
-        set_id[j] = (int32_t)j;
-        notional[j] = p.notional();
-        fixed[j] = p.fixed_rate();
-        start[j] = p.start();
-        pay.insert(pay.end(), p.payment_times().begin(), p.payment_times().end());
-        acc.insert(acc.end(), p.accruals().begin(), p.accruals().end());
-        off[j + 1] = (int32_t)pay.size();
-        own[j] = reset_dates(p);
-        dates.insert(dates.end(), own[j].begin(), own[j].end());
+        std::string key;
+        for (double t : p.payment_times()) key += std::to_string(t) + ",";
+        key += "|" + std::to_string(p.start());
+        for (double a : p.accruals()) key += "," + std::to_string(a);
+        groups[key].push_back(j);
     }
-    std::sort(dates.begin(), dates.end());
-    dates.erase(std::unique(dates.begin(), dates.end()), dates.end());
-    const eqhip_portfolio pf{(int32_t)n, (int32_t)n, set_id.data(), notional.data(), fixed.data(), start.data(),
-                             off.data(), pay.data(), acc.data()};
-    const eqhip_mc mc{pricing.n_paths(), (uint32_t)pricing.n_steps(), pricing.seed(), EQHIP_RNG_PHILOX,
-                      disc_of(execution)};
-    eqhip_plan* plan = nullptr;
-    if (const auto* hw2 = dynamic_cast<const HullWhite2FModel*>(&model)) {
-        const eqhip_model_hw2f m{hw2->a(), hw2->b(), hw2->sigma(), hw2->eta(), hw2->rho(), hw2->r0()};
-        check(eqhip_plan_create_2f(gpu_context(), &m, &pf, dates.data(), (int32_t)dates.size(), &mc, 0, &plan));
-    } else {
-        const auto& hw1 = require_hw1f(model, "ExposureProfileMeasure");
-        const eqhip_model_hw1f m{hw1.a(), hw1.b(), hw1.sigma(), hw1.r0()};
-        check(eqhip_plan_create(gpu_context(), &m, &pf, dates.data(), (int32_t)dates.size(), &mc, 0, &plan));
-    }
-    struct Guard {
-        eqhip_plan* p;
-        ~Guard() { eqhip_plan_destroy(p); }
-    } guard{plan};
-    const std::size_t nd = dates.size();
-    std::vector<double> ee(n * nd), pfe(n * nd);
-    eqhip_outputs out{};
-    out.set_ee = ee.data();
-    out.set_pfe = pfe.data();
-    check(eqhip_plan_execute(plan, nullptr, nullptr, &out));
-    // every date <= 0: the plan returns one deterministic row (exposure.rs:98-108)
-    const bool det = !(dates.back() > 0.0);
-    std::vector<ExposureProfile> profiles(n);
-    for (std::size_t j = 0; j < n; ++j) {
-        ExposureProfile& pr = profiles[j];
-        for (double t : own[j]) {
-            const std::size_t k = (std::size_t)(std::lower_bound(dates.begin(), dates.end(), t) - dates.begin());
-            pr.times.push_back(t);
-            pr.ee.push_back(ee[j * nd + k]);
-            pr.pfe_95.push_back(pfe[j * nd + k]);
-            if (det) break;
+    std::vector<ExposureProfile> profiles(products.size());
+    for (const auto& kv : groups) {
+        const IrSwapProduct& first = *products[kv.second.front()];
+        const std::vector<double> own = reset_dates(first);
+        std::vector<double> dates = own;
+        const bool extra = t_last > own.back();
+        if (extra) dates.push_back(t_last);
+        std::vector<double> notionals, fixed_rates;
+        for (std::size_t j : kv.second) {
+            notionals.push_back(products[j]->notional());
+            fixed_rates.push_back(products[j]->fixed_rate());
+        }
+        const std::size_t nt = notionals.size(), nd = dates.size();
+        std::vector<double> times(nd), ee(nt * nd), pfe(nt * nd);
+        int32_t n_out = 0;
+        if (const auto* hw2 = dynamic_cast<const HullWhite2FModel*>(&model)) {
+            check(eqhip_irs_hull_white_2f_exposure_profile_batch(
+                gpu_context(), hw2->a(), hw2->b(), hw2->sigma(), hw2->eta(), hw2->rho(), hw2->r0(), notionals.data(),
+                fixed_rates.data(), (int32_t)nt, first.start(), first.payment_times().data(), first.accruals().data(),
+                (int32_t)first.payment_times().size(), dates.data(), (int32_t)nd, (uint32_t)pricing.n_steps(),
+                pricing.n_paths(), pricing.seed(), EQHIP_RNG_PHILOX, disc_of(execution), nullptr, 1, times.data(),
+                ee.data(), pfe.data(), &n_out));
+        } else {
+            const auto& hw1 = require_hw1f(model, "ExposureProfileMeasure");
+            check(eqhip_irs_hull_white_exposure_profile_batch(
+                gpu_context(), hw1.a(), hw1.b(), hw1.sigma(), hw1.r0(), notionals.data(), fixed_rates.data(),
+                (int32_t)nt, first.start(), first.payment_times().data(), first.accruals().data(),
+                (int32_t)first.payment_times().size(), dates.data(), (int32_t)nd, (uint32_t)pricing.n_steps(),
+                pricing.n_paths(), pricing.seed(), EQHIP_RNG_PHILOX, disc_of(execution), nullptr, 1, times.data(),
+                ee.data(), pfe.data(), &n_out));
+        }
+        const std::size_t keep = std::min<std::size_t>((std::size_t)n_out, own.size());
+        for (std::size_t i = 0; i < nt; ++i) {
+            ExposureProfile& pr = profiles[kv.second[i]];
+            pr.times.assign(times.begin(), times.begin() + keep);
+            pr.ee.assign(ee.begin() + i * nd, ee.begin() + i * nd + keep);
+            pr.pfe_95.assign(pfe.begin() + i * nd, pfe.begin() + i * nd + keep);
         }
     }
     return profiles;

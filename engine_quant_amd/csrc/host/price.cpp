@@ -273,9 +273,9 @@ static PriceBatchResult price_batch_with_profiles(const Registries& registries,
     return result;
 }
 
-// price_many with one simulation for every calendar group: the profiles of all trades from one plan
-// (each trade its own netting set), then per group the CVA on the group's own dates and the measures
-// that need no simulation, in price_batch's order and aliases.
+// price_many with one scenario set for every calendar group: every trade's profile on the common grid
+// and paths, then per group the CVA on the group's own dates and the measures that need no simulation,
+// in price_batch's order and aliases.
 static PriceBatchResult price_many_shared(const Registries& registries, const std::vector<const IProduct*>& products,
                                    const std::vector<MeasureSpec>& measures, const IModel& model,
                                    const MarketSnapshot& market, const PricingContext& pricing,

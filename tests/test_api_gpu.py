@@ -152,10 +152,9 @@ def test_exact_discretisation_is_selectable():
 
 
 def test_price_many_on_one_shared_simulation():
-    """ExecutionContext simulation="shared": every calendar group from one simulation. With one grid for
-    all groups (same latest date) it equals the per-group run up to the rounding of the annuity sums (the
-    plan's flow lists differ); otherwise each trade's profile is the oracle's on the shared grid, at the
-    trade's own reset dates."""
+    """ExecutionContext simulation="shared": every calendar group on one scenario set. With one grid for
+    all groups (same latest date) it equals the per-group run; otherwise each trade's profile is the
+    oracle's on the common grid and paths, at the trade's own reset dates."""
     eng = engine.Engine()
     model = eng.create_model("HullWhite1F", _hull_white_params())
     market, pr = _market_with_credit(0.02, 0.4), _golden_pricing(3000, 5)
