@@ -550,6 +550,53 @@ def run_caller(args):
     print(json.dumps(out), flush=True)
 
 
+def run_greeks(args):
+    """The Greek measure on the GPU measures (cpp/engine/src/greeks.cpp): the CVA Greek sweep of one swap
+    through the host engine (Engine.all_greeks: model, curve-pillar and credit factors by central
+    bump-and-revalue with common random numbers), and the pathwise route (the CVA adjoint) for the four
+    model parameters beside the bumped ones."""
+    import engine_quant_amd as engine
+
+    eng = engine.Engine()
+    a, b, sigma, r0 = 0.1, 0.03, 0.01, 0.02
+    model = eng.create_model("HullWhite1F", {"a": a, "b": b, "sigma": sigma, "r0": r0})
+    swap = eng.create_product("IRSwap", {"notional": 1e6, "fixed_rate": 0.021, "payment_times": [1, 2, 3, 4, 5.0],
+                                         "accruals": [1.0] * 5})
+    market = engine.MarketSnapshot(pillars=[1.0, 2.0, 5.0], zero_rates=[0.02, 0.022, 0.025], hazard_rate=0.02,
+                                   recovery_rate=0.4)
+    n_paths, n_steps = args.paths, 208
+    pricing = engine.PricingContext({"n_paths": float(n_paths), "n_steps": float(n_steps), "seed": float(args.seed)})
+    gpu = engine.ExecutionContext({"backend": "gpu", "discretisation": "exact"})
+    eng.all_greeks(swap, "UnilateralCVA", model, market, pricing, gpu, include_curve_buckets=True)  # warm-up
+    runs = []
+    for _ in range(max(3, args.steps // 5)):
+        t0 = time.perf_counter()
+        rep = eng.all_greeks(swap, "UnilateralCVA", model, market, pricing, gpu, include_curve_buckets=True)
+        runs.append(1000.0 * (time.perf_counter() - t0))
+    sweep_ms = float(np.median(runs))
+    greeks = {g.risk_factor: g.value for g in rep.greeks}
+    route = {}
+    for name in ("a", "b", "sigma", "r0"):
+        for method in ("bump_and_reval", "pathwise"):
+            spec = [("Greek", {"metric": "UnilateralCVA", "risk_factor": f"model.{name}", "method": method})]
+            eng.price(swap, spec, model, market, pricing, gpu)
+            t0 = time.perf_counter()
+            g = eng.price(swap, spec, model, market, pricing, gpu)["Greek"]
+            route[f"{name}/{method}"] = {"ms": 1000.0 * (time.perf_counter() - t0), "value": g.scalar}
+    n_revals = sum(2 if not k.startswith("time.") else 1 for k in greeks)  # central stencils
+    units = n_revals * n_paths * 5  # revaluations x paths x monitoring dates
+    out = {
+        "metric": "Greek sweep of UnilateralCVA through the host engine (bump-and-revalue with CRN on the GPU)",
+        "value": units / (sweep_ms / 1000.0), "unit": "revalued trade-paths-dates/s", "n_gpus": 1,
+        "steps": len(runs), "warmup": 1, "ms_per_step": sweep_ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "greek-sweep-unilateral-cva", "swap": "5y annual payer, N 1e6, K 2.1%",
+                   "n_paths": n_paths, "n_steps": n_steps, "risk_factors": len(greeks), "revaluations": n_revals},
+        "greeks": greeks, "skipped": list(rep.skipped), "routes": route, "roofline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -567,13 +614,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", choices=["swaps", "batch", "adjoint", "bermudan", "xva", "reference-case",
-                                           "caller"],
+                                           "caller", "greeks"],
                     default="swaps",
                     help="swaps: the one-netting-set IRS exposure job (default); batch: 64 netting sets in one "
                          "launch; adjoint: pathwise CVA Greeks; bermudan: the learned continuation-value "
                          "exposure of the Bermudan book; xva: the XVA reducer on the resident Monte Carlo cube; "
                          "reference-case: the reference's own published GPU benchmark case; caller: price_many "
-                         "through the host engine, per calendar group and shared (each prints its own JSON line)")
+                         "through the host engine, per calendar group and shared; greeks: the CVA Greek sweep "
+                         "(each prints its own JSON line)")
     ap.add_argument("--adj-paths", type=int, default=200_000)
     ap.add_argument("--berm-trades", type=int, default=200)
     ap.add_argument("--berm-train", type=int, default=1 << 18)
@@ -616,6 +664,9 @@ def main():
         return
     if args.workload == "caller":
         run_caller(args)
+        return
+    if args.workload == "greeks":
+        run_greeks(args)
         return
 
     from engine_quant_amd import eqhip
