@@ -5,7 +5,7 @@ paths, per-path mark-to-market of every swap at every monitoring date, netting-s
 CVA, per-trade EE / CVA, plus the PV / DV01 / bucketed-DV01 ladder of every trade. The workload is
 the one-netting-set portfolio: 1000 seeded synthetic swaps, 100k paths x 360 monthly dates, FP64.
 
-Prints one JSON line (rank 0). ``--gpus`` above 1 is refused: this build measures one GPU.
+Prints one JSON line (rank 0). ``--gpus`` above 1 is refused: the bench measures one GPU.
 """
 from __future__ import annotations
 
@@ -643,7 +643,7 @@ def main():
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
+        print("bench.py measures one GPU only: the library runs on one GPU "
               "(paths would shard by global path index across GPUs)", file=sys.stderr)
         sys.exit(2)
 

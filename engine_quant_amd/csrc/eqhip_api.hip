@@ -1654,7 +1654,7 @@ int eqhip_context_create(int device, eqhip_context** out) {
         hipDeviceProp_t prop;
         HIPCHK(hipGetDeviceProperties(&prop, device));
         if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
-            raise(EQHIP_E_NO_DEVICE, std::string("this build targets gfx950, device is ") + prop.gcnArchName);
+            raise(EQHIP_E_NO_DEVICE, std::string("the library is compiled for gfx950, device is ") + prop.gcnArchName);
         auto ctx = std::make_unique<eqhip_context>();
         ctx->device = device;
         HIPCHK(hipSetDevice(device));

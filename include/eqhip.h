@@ -25,7 +25,7 @@ extern "C" {
 #define EQHIP_E_INVALID 1   /* invalid argument (the reference's std::invalid_argument / assert!) */
 #define EQHIP_E_NO_DEVICE 2 /* backend "gpu" requested with no usable device: never a CPU run */
 #define EQHIP_E_HIP 3       /* HIP runtime error */
-#define EQHIP_E_CAPACITY 4  /* problem exceeds a documented capacity limit of this build */
+#define EQHIP_E_CAPACITY 4  /* problem exceeds a documented capacity limit of the library */
 
 /* Random-number source of the short-rate shocks. */
 #define EQHIP_RNG_HOST_DRAWS 0 /* caller supplies Z[step][path] (shared-draw parity mode) */
