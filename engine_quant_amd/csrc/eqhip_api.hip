@@ -126,6 +126,9 @@ struct eqhip_plan {
     std::vector<uint64_t> h_neff, h_qrank;
     bool direct_out = false;  // one-shot plans: outputs go straight to the caller's arrays, and the large
                               // work buffers are views into the context's pool (under its one-shot lock)
+    bool flat = false;        // homogeneous batch valued by value_flat_kernel (build_flat)
+    int flat_m = 0;           // its schedule's flow count
+    DevBuf flat_tables, flat_first, flat_AB, flat_acc, flat_N, flat_NK;
     ValuePlan vp{};  // host copy of the device descriptor (per-run pointers patched at launch)
     PinnedBuf h_out; // page-locked staging of the outputs
     DevBuf cva_w;    // per-date CVA factors
@@ -144,6 +147,17 @@ struct eqhip_plan {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
     }
+};
+
+// A scenario set: the short rate at every grid step of t_max / n_steps on every path, read by the
+// plans of many batch calls (their r_shared) in place of a simulation of their own.
+struct eqhip_scenarios {
+    int device = 0;
+    eqhip_model_hw1f model{};
+    eqhip_mc mc{};
+    double t_max = 0.0;
+    DevBuf slot_of_step;  // identity: step g is row g
+    DevBuf R;             // [step][path]
 };
 
 namespace {
@@ -758,6 +772,28 @@ void launch_value(eqhip_plan& P, double* path_mtm, double* set_mtm) {
     HIPCHK(hipGetLastError());
 }
 
+void launch_value_flat(eqhip_plan& P) {
+    FlatArgs g{};
+    g.R = P.r_shared ? P.r_shared : P.R.as<double>();
+    g.r_stride = P.r_shared ? P.r_stride : P.mc.n_paths;
+    g.n_paths = P.mc.n_paths;
+    g.n_dates = P.n_dates;
+    g.m = P.flat_m;
+    g.n_trades = P.n_trades;
+    g.r0 = P.model.r0;
+    g.date_slot = P.date_slot.as<int>();
+    g.date_det = P.date_det.as<int>();
+    g.first = P.flat_first.as<int>();
+    g.AB = P.flat_AB.as<double2>();
+    g.acc = P.flat_acc.as<double>();
+    g.N = P.flat_N.as<double>();
+    g.NK = P.flat_NK.as<double>();
+    g.X = P.X.as<double>();
+    dim3 grid((unsigned)((P.mc.n_paths + kFlatThreads - 1) / kFlatThreads), (unsigned)P.n_dates);
+    hipLaunchKernelGGL(value_flat_kernel, grid, dim3(kFlatThreads), 0, P.ctx->stream, g);
+    HIPCHK(hipGetLastError());
+}
+
 // Unilateral CVA of n_profiles EE rows ([profile][date]) on the stream: the per-date factors once
 // into w (n_dates double2), then one fixed-order sum per profile.
 void launch_cva_weights(hipStream_t st, const double* d_times, int n_dates, const eqhip_model_hw1f& m, double hazard,
@@ -776,6 +812,38 @@ void launch_cva_sum(hipStream_t st, const double* ee, int n_profiles, int n_date
                     double* out) {
     hipLaunchKernelGGL(cva_kernel, dim3((n_profiles + 3) / 4), dim3(256), 0, st, ee, n_profiles, n_dates, w, recovery,
                        out);
+    HIPCHK(hipGetLastError());
+}
+
+// HW1F paths into R[slot_of_step[step]][path] for steps 0..n_steps (Euler, kernel.rs:41-50; or the
+// exact transition over dt), from host draws Z or the Philox stream of mc.seed.
+void launch_simulate1(hipStream_t st, const eqhip_model_hw1f& m, const eqhip_mc& mc, double dt, uint32_t n_steps,
+                      const int* slot_of_step, const double* Z, double* R) {
+    const uint64_t np = mc.n_paths;
+    SimArgs s{};
+    s.a = m.a;
+    s.b = m.b;
+    s.sigma = m.sigma;
+    s.r0 = m.r0;
+    s.dt = dt;
+    s.sqrt_dt = std::sqrt(dt);
+    s.decay = std::exp(-(m.a * dt));
+    s.mean_c = m.b * (1.0 - s.decay);
+    s.sd = m.sigma * std::sqrt((1.0 - std::exp(-(2.0 * m.a * dt))) / (2.0 * m.a));
+    s.n_paths = np;
+    s.n_steps = n_steps;
+    s.seed = mc.seed;
+    s.slot_of_step = slot_of_step;
+    s.Z = Z;
+    s.R = R;
+    dim3 sgrid((unsigned)((np + 255) / 256));
+    dim3 pgrid((unsigned)((np + kSimPaths - 1) / kSimPaths));
+    switch (mc.disc * 2 + mc.rng) {
+        case 0: hipLaunchKernelGGL((simulate_kernel<0, 0>), sgrid, dim3(256), 0, st, s); break;
+        case 1: hipLaunchKernelGGL((simulate_philox_kernel<0>), pgrid, dim3(256), 0, st, s); break;
+        case 2: hipLaunchKernelGGL((simulate_kernel<1, 0>), sgrid, dim3(256), 0, st, s); break;
+        default: hipLaunchKernelGGL((simulate_philox_kernel<1>), pgrid, dim3(256), 0, st, s); break;
+    }
     HIPCHK(hipGetLastError());
 }
 
@@ -844,38 +912,19 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
     double* set_values = P.keep_cube ? P.cube.as<double>() : dset.as<double>();
 
     HIPCHK(hipEventRecord(P.ev[0], st));
-    if (P.factors == 2) {
+    if (P.r_shared) {
+        // the short rates come from a scenario set simulated beforehand
+    } else if (P.factors == 2) {
         launch_simulate2(P);
     } else {
-    SimArgs s{};
-    s.a = P.model.a;
-    s.b = P.model.b;
-    s.sigma = P.model.sigma;
-    s.r0 = P.model.r0;
-    s.dt = P.dt;
-    s.sqrt_dt = std::sqrt(P.dt);
-    s.decay = std::exp(-(P.model.a * P.dt));
-    s.mean_c = P.model.b * (1.0 - s.decay);
-    s.sd = P.model.sigma * std::sqrt((1.0 - std::exp(-(2.0 * P.model.a * P.dt))) / (2.0 * P.model.a));
-    s.n_paths = np;
-    s.n_steps = P.det ? 0u : P.mc.n_steps;
-    s.seed = P.mc.seed;
-    s.slot_of_step = P.slot_of_step.as<int>();
-    s.Z = P.Z.as<double>();
-    s.R = P.R.as<double>();
-    dim3 sgrid((unsigned)((np + 255) / 256));
-    dim3 pgrid((unsigned)((np + kSimPaths - 1) / kSimPaths));
-    int key = P.mc.disc * 2 + P.mc.rng;
-    switch (key) {
-        case 0: hipLaunchKernelGGL((simulate_kernel<0, 0>), sgrid, dim3(256), 0, st, s); break;
-        case 1: hipLaunchKernelGGL((simulate_philox_kernel<0>), pgrid, dim3(256), 0, st, s); break;
-        case 2: hipLaunchKernelGGL((simulate_kernel<1, 0>), sgrid, dim3(256), 0, st, s); break;
-        default: hipLaunchKernelGGL((simulate_philox_kernel<1>), pgrid, dim3(256), 0, st, s); break;
-    }
-    HIPCHK(hipGetLastError());
+        launch_simulate1(st, P.model, P.mc, P.dt, P.det ? 0u : P.mc.n_steps, P.slot_of_step.as<int>(),
+                         P.Z.as<double>(), P.R.as<double>());
     }
     HIPCHK(hipEventRecord(P.ev[1], st));
-    launch_value(P, dpath.as<double>(), set_values);
+    if (P.flat && !out->path_mtm && !out->set_mtm)
+        launch_value_flat(P);
+    else
+        launch_value(P, dpath.as<double>(), set_values);
     HIPCHK(hipEventRecord(P.ev[2], st));
     hipLaunchKernelGGL(select_kernel, dim3((unsigned)(P.S * P.n_dates)), dim3(kSelectThreads), 0, st,
                        P.rows.as<const SelectRow>());
@@ -1069,6 +1118,59 @@ void xva_reduce(eqhip_context* ctx, const XvaRows& rows, DevBuf& pos, const eqhi
     out_measures[4] = kva;
 }
 
+// Homogeneous one-shot batches (one schedule, trade j alone in netting set j, HW1F): when every date
+// is a reset date of the schedule, precedes its start or follows its maturity, the netting-set values
+// come from value_flat_kernel instead of the general valuation. The date rules are those of the
+// general valuation (and of irs.rs:163-187 at reset dates): flows alive at t are those paid after
+// t + 1e-9; the floating leg starts at t on a reset date, at the start before it. A date inside a
+// period (a fixing in progress) leaves the plan on the general kernel.
+void build_flat(eqhip_plan& P, const eqhip_portfolio& pf) {
+    if (P.factors != 1 || P.want_trade || P.keep_cube || P.det || P.n_dates > 65535) return;
+    const int m = pf.flow_offset[1] - pf.flow_offset[0];
+    const double start = pf.start[0];
+    const double* pay = pf.payment_times;
+    const int nd = P.n_dates;
+    std::vector<int> first(nd, m);
+    std::vector<double> tau((size_t)nd * (m + 1), 0.0);
+    for (int k = 0; k < nd; ++k) {
+        const double t = P.dates[k];
+        if (t >= pay[m - 1] - kDateTolHost) continue;  // matured
+        int f = 0;
+        while (f < m && !(pay[f] > t + kDateTolHost)) ++f;
+        double s = t;  // the floating leg's first bond date
+        if (std::fabs(t - start) >= kDateTolHost) {
+            if (t < start) {
+                s = start;
+            } else if (std::fabs(t - (f == 0 ? start : pay[f - 1])) >= kDateTolHost) {
+                return;  // inside a period
+            }
+        }
+        first[k] = f;
+        for (int i = f; i < m; ++i) tau[(size_t)k * (m + 1) + i] = pay[i] - t;
+        tau[(size_t)k * (m + 1) + m] = s - t;
+    }
+    P.flat_m = m;
+    std::vector<double> acc(pf.accruals, pf.accruals + m), N(pf.n_trades), NK(pf.n_trades);
+    for (int j = 0; j < pf.n_trades; ++j) {
+        N[j] = pf.notional[j];
+        NK[j] = pf.notional[j] * pf.fixed_rate[j];
+    }
+    TablePack pack;
+    DevBuf d_tau;
+    pack.add(P.flat_first, first);
+    pack.add(d_tau, tau);
+    pack.add(P.flat_acc, acc);
+    pack.add(P.flat_N, N);
+    pack.add(P.flat_NK, NK);
+    pack.flush(P.flat_tables);
+    const int n = (int)tau.size();
+    P.flat_AB.alloc((size_t)n * sizeof(double2));
+    hipLaunchKernelGGL(ab_kernel, dim3((n + 255) / 256), dim3(256), 0, P.ctx->stream, d_tau.as<double>(), n, P.model.a,
+                       P.model.b, P.model.sigma, P.flat_AB.as<double2>());
+    HIPCHK(hipGetLastError());
+    P.flat = true;
+}
+
 eqhip_context* check_ctx(eqhip_context* ctx) {
     if (!ctx) raise(EQHIP_E_INVALID, "context is null");
     HIPCHK(hipSetDevice(ctx->device));
@@ -1082,8 +1184,21 @@ void profile_batch(eqhip_context* ctx, double a, double b, double sigma, double 
                    const double* pay, const double* acc, int m, const double* times, int n_dates,
                    uint32_t n_steps, uint64_t n_paths, uint64_t seed, int rng, int disc,
                    const double* host_draws, int allow_any_date, double* out_times, double* out_ee,
-                   double* out_pfe, int32_t* n_out, const eqhip_model_hw2f* model2 = nullptr) {
+                   double* out_pfe, int32_t* n_out, const eqhip_model_hw2f* model2 = nullptr,
+                   const eqhip_scenarios* scen = nullptr) {
     check_ctx(ctx);
+    if (scen) {
+        if (scen->device != ctx->device) raise(EQHIP_E_INVALID, "the scenario set lives on another device");
+        if (n_dates < 1 || !times) raise(EQHIP_E_INVALID, "monitoring_times vacío");
+        double last = -INFINITY;
+        for (int k = 0; k < n_dates; ++k) {
+            if (!(times[k] >= 0.0)) raise(EQHIP_E_INVALID, "monitoring dates on a scenario set must be >= 0");
+            last = std::max(last, times[k]);
+        }
+        if (last != scen->t_max)
+            raise(EQHIP_E_INVALID, "the last monitoring date (" + std::to_string(last) +
+                                       ") must be the scenario set's t_max (" + std::to_string(scen->t_max) + ")");
+    }
     if (n_dates < 1 || !times) raise(EQHIP_E_INVALID, "monitoring_times vacío");
     if (n_steps < 1) raise(EQHIP_E_INVALID, "n_steps debe ser al menos 1");
     if (m < 1 || !pay || !acc) raise(EQHIP_E_INVALID, "un swap necesita al menos un periodo");
@@ -1116,7 +1231,12 @@ void profile_batch(eqhip_context* ctx, double a, double b, double sigma, double 
         P.factors = 2;
         P.model2 = *model2;
     }
-    build_plan(P, model2 ? nullptr : &model, &pf, times, nd, &mc, 0);
+    if (scen) {  // every grid step of the set's paths, nothing to simulate
+        P.r_shared = scen->R.as<double>();
+        P.r_stride = n_paths;
+    }
+    build_plan(P, model2 ? nullptr : &model, &pf, times, nd, &mc, 0, scen != nullptr);
+    build_flat(P, pf);
     std::vector<double> ee((size_t)n_trades * nd), pfe((size_t)n_trades * nd);
     eqhip_outputs out{};
     out.set_ee = ee.data();
@@ -1764,6 +1884,70 @@ int eqhip_irs_hull_white_exposure_profile(
         profile_batch(ctx, a, b, sigma, r0, &notional, &k, 1, start, payment_times, accruals, n_flows,
                       monitoring_times, n_dates, n_steps, n_paths, seed, rng, disc, host_draws, allow_any_date,
                       out_times, out_ee, out_pfe, n_out);
+    });
+}
+
+int eqhip_scenarios_create(eqhip_context* ctx, const eqhip_model_hw1f* model, double t_max, const eqhip_mc* mc,
+                           eqhip_scenarios** out) {
+    return guarded([&] {
+        if (!out) raise(EQHIP_E_INVALID, "out is null");
+        *out = nullptr;
+        check_ctx(ctx);
+        if (!model || !mc) raise(EQHIP_E_INVALID, "model / mc is null");
+        if (!(model->a != 0.0)) raise(EQHIP_E_INVALID, "HullWhite1F: a must be non-zero");
+        if (!(t_max > 0.0) || !std::isfinite(t_max)) raise(EQHIP_E_INVALID, "t_max must be positive and finite");
+        if (mc->n_paths < 1) raise(EQHIP_E_INVALID, "PricingContext: n_paths must be > 0");
+        if (mc->n_steps < 1) raise(EQHIP_E_INVALID, "PricingContext: n_steps must be > 0");
+        if (mc->rng != EQHIP_RNG_PHILOX) raise(EQHIP_E_INVALID, "a scenario set draws from the Philox stream");
+        if (mc->disc != EQHIP_DISC_EULER && mc->disc != EQHIP_DISC_EXACT)
+            raise(EQHIP_E_INVALID, "unknown discretisation");
+        if (mc->n_paths > (uint64_t)1 << 40) raise(EQHIP_E_CAPACITY, "n_paths too large");
+        const uint32_t steps = mc->n_steps;
+        const double bytes = 8.0 * (double)(steps + 1) * (double)mc->n_paths;
+        size_t free_b = 0, total_b = 0;
+        HIPCHK(hipMemGetInfo(&free_b, &total_b));
+        if (bytes > 0.9 * (double)free_b)
+            raise(EQHIP_E_CAPACITY, "scenario set needs " + std::to_string(bytes / 1e9) + " GB of device memory, " +
+                                        std::to_string(free_b / 1e9) + " GB free");
+        auto S = std::make_unique<eqhip_scenarios>();
+        S->device = ctx->device;
+        S->model = *model;
+        S->mc = *mc;
+        S->t_max = t_max;
+        std::vector<int> slot(steps + 1);
+        for (uint32_t g = 0; g <= steps; ++g) slot[g] = (int)g;
+        upload(S->slot_of_step, slot, ctx->stream);
+        S->R.alloc((size_t)bytes);
+        // the grid of a plan whose last date is t_max: dt = t_max / n_steps (exposure.rs:91-116)
+        launch_simulate1(ctx->stream, S->model, S->mc, t_max / (double)steps, steps, S->slot_of_step.as<int>(),
+                         nullptr, S->R.as<double>());
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        *out = S.release();
+    });
+}
+
+int eqhip_scenarios_destroy(eqhip_scenarios* scenarios) {
+    return guarded([&] {
+        if (!scenarios) return;
+        (void)hipSetDevice(scenarios->device);
+        delete scenarios;
+    });
+}
+
+int eqhip_irs_hull_white_exposure_profile_batch_on(
+    eqhip_context* ctx, const eqhip_scenarios* scenarios, const double* notionals, const double* fixed_rates,
+    int32_t n_trades, double start, const double* payment_times, const double* accruals, int32_t n_flows,
+    const double* monitoring_times, int32_t n_dates, double* out_times, double* out_ee, double* out_pfe,
+    int32_t* n_out) {
+    return guarded([&] {
+        if (!scenarios) raise(EQHIP_E_INVALID, "scenario set is null");
+        if (n_trades < 1 || !notionals || !fixed_rates)
+            raise(EQHIP_E_INVALID, "el lote requiere al menos un trade con notional y fixed_rate");
+        const eqhip_model_hw1f& m = scenarios->model;
+        const eqhip_mc& mc = scenarios->mc;
+        profile_batch(ctx, m.a, m.b, m.sigma, m.r0, notionals, fixed_rates, n_trades, start, payment_times, accruals,
+                      n_flows, monitoring_times, n_dates, mc.n_steps, mc.n_paths, mc.seed, mc.rng, mc.disc, nullptr,
+                      1, out_times, out_ee, out_pfe, n_out, nullptr, scenarios);
     });
 }
 

@@ -12,6 +12,8 @@
 //                      factors, writes the netting-set values (and their positive parts) for every
 //                      set, and accumulates per-trade exposure sums over path slices. value2_kernel
 //                      is the G2++ instance (two-factor bonds in stage A).
+//   value_flat_kernel  homogeneous batches (one schedule, one trade per set): one thread per path
+//                      and date, one floating leg and annuity shared by the trades.
 //   select_kernel      one block per row: fixed-order sum + exact radix select of an order
 //                      statistic on the bit patterns of the non-negative doubles.
 //   xva_rows_kernel    one block per exposure-cube row: CSA adjustment, positive parts and ENE.
@@ -822,6 +824,37 @@ __global__ void __launch_bounds__(kValueThreads) value2_kernel(const ValuePlan* 
     extern __shared__ double smem[];
     const int4 u = units[blockIdx.x];
     value_unit<2>(plans[u.x], u.y, u.z, smem);
+}
+
+// One thread per path, blockIdx.y the date. The bonds of the date's remaining flows are summed in
+// schedule order (the last one is also the floating leg's end bond); the trades' stores are
+// coalesced across the wave (consecutive paths).
+__global__ void __launch_bounds__(kFlatThreads) value_flat_kernel(FlatArgs g) {
+#pragma clang fp contract(off)
+    const uint64_t p = (uint64_t)blockIdx.x * kFlatThreads + threadIdx.x;
+    if (p >= g.n_paths) return;
+    const int k = blockIdx.y, m = g.m, first = g.first[k];
+    const size_t stride = (size_t)g.n_dates * g.n_paths;
+    double* x = g.X + (size_t)k * g.n_paths + p;
+    if (first >= m) {  // matured: every trade is worth zero
+        for (int j = 0; j < g.n_trades; ++j) x[j * stride] = 0.0;
+        return;
+    }
+    const double r = g.date_det[k] ? g.r0 : g.R[(uint64_t)g.date_slot[k] * g.r_stride + p];
+    const double2* ab = g.AB + (size_t)k * (m + 1);
+    const double p_start = ab[m].x * exp(-(ab[m].y * r));
+    double annuity = 0.0, p_end = 0.0;
+    for (int i = first; i < m; ++i) {
+        const double P = ab[i].x * exp(-(ab[i].y * r));
+        const double leg = P * g.acc[i];
+        annuity = (i == first) ? leg : annuity + leg;
+        p_end = P;
+    }
+    const double fl = p_start - p_end;
+    for (int j = 0; j < g.n_trades; ++j) {
+        const double v = g.N[j] * fl - g.NK[j] * annuity;
+        x[j * stride] = v > 0.0 ? v : 0.0;
+    }
 }
 
 // Persistent variant: a fixed grid of blocks pulls work units from a device counter until the

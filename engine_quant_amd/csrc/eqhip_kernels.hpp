@@ -202,6 +202,30 @@ struct SelectRow {
 };
 __global__ void select_kernel(const SelectRow* rows);
 
+// Homogeneous batch: one schedule, each trade its own netting set (the reference's
+// npv_batch_over_paths, irs.rs:87-114), at dates that are reset dates, precede the start or follow
+// maturity. Per (path, date) one floating leg P(t,S) - P(t,T_n) and one annuity sum_i P(t,T_i) tau_i
+// over the remaining flows; per trade N * float - (N K) * annuity, positive part into X.
+constexpr int kFlatThreads = 256;
+struct FlatArgs {
+    const double* R;       // short rates [slot][path], row stride r_stride
+    uint64_t r_stride;
+    uint64_t n_paths;
+    int n_dates;
+    int m;                 // flows of the schedule
+    int n_trades;
+    double r0;
+    const int* date_slot;  // R row of each date
+    const int* date_det;   // 1: the date sits on grid step 0 (r = r0)
+    const int* first;      // first remaining flow of each date; m when matured
+    const double2* AB;     // [date][m + 1]: (A, B) of P(t, T_i) for i < m, of P(t, S) at i = m
+    const double* acc;     // [m] accruals
+    const double* N;       // [n_trades] signed notionals
+    const double* NK;      // [n_trades] notional * fixed rate
+    double* X;             // [trade][date][path] positive parts
+};
+__global__ void value_flat_kernel(FlatArgs g);
+
 // One time row of an exposure cube for the XVA reducer (quant-engine/src/xva.rs:150-183): the row's
 // values after the first CSA's variation margin, their positive parts (written out for the quantile)
 // and the fixed-order sums of their negative parts, for the Q row and, when given, the P row.

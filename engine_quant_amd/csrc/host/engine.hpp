@@ -212,6 +212,15 @@ ExposureProfile irs_hull_white_exposure_profile(const ExecutionContext& executio
                                                 const std::vector<double>& accruals,
                                                 const std::vector<double>& monitoring_times, std::size_t n_steps,
                                                 std::uint64_t n_paths, std::uint64_t seed);
+// Runs task(0) .. task(n - 1) on the calling thread and up to kGpuStreams - 1 persistent worker threads,
+// each with its own eqhip context and HIP stream, so that the GPU work and host set-up of independent
+// tasks overlap. Tasks are claimed in the order claim_order lists them (index order when it is empty;
+// largest first balances the streams). Once every task has ended, the exception of the lowest failed
+// index is rethrown (the one a serial loop would have thrown). While another call is using the
+// workers, or for n < 2, the tasks run serially on the calling thread.
+constexpr int kGpuStreams = 4;
+void run_on_gpu_streams(std::size_t n, const std::function<void(std::size_t)>& task,
+                        const std::vector<std::size_t>& claim_order = {});
 // Every trade's profile at its own reset dates, all on one scenario set: the grid of the latest reset
 // date of all trades and the same Philox paths for every calendar group (the shared-simulation form of
 // price_many).

@@ -1,7 +1,13 @@
 // Measures of the exposure path and their GPU arm (cpp/engine/src/measure.cpp:20-462,
 // cpp/engine/src/engine.cpp:73-113). Every numeric result comes from libeqhip (include/eqhip.h).
+#include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <condition_variable>
+#include <exception>
 #include <map>
+#include <mutex>
+#include <thread>
 
 #include "eqhip.h"
 #include "engine.hpp"
@@ -34,6 +40,55 @@ eqhip_context* gpu_context() {
     if (!holder.ctx) check(eqhip_context_create(0, &holder.ctx));
     return holder.ctx;
 }
+
+// Worker threads for run_on_gpu_streams. They live until the process ends (the pool is never
+// destroyed, so no thread is joined at exit), and each keeps its gpu_context() and the context's
+// pooled buffers between calls.
+class StreamWorkers {
+  public:
+    explicit StreamWorkers(int n) {
+        for (int i = 0; i < n; ++i) std::thread([this] { loop(); }).detach();
+        n_workers_ = n;
+    }
+
+    void run(const std::function<void()>& drain) {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            job_ = &drain;
+            running_ = n_workers_;
+            ++generation_;
+        }
+        wake_.notify_all();
+        drain();
+        std::unique_lock<std::mutex> g(m_);
+        done_.wait(g, [&] { return running_ == 0; });
+        job_ = nullptr;
+    }
+
+  private:
+    void loop() {
+        std::uint64_t seen = 0;
+        for (;;) {
+            const std::function<void()>* job;
+            {
+                std::unique_lock<std::mutex> g(m_);
+                wake_.wait(g, [&] { return generation_ != seen; });
+                seen = generation_;
+                job = job_;
+            }
+            (*job)();
+            std::lock_guard<std::mutex> g(m_);
+            if (--running_ == 0) done_.notify_all();
+        }
+    }
+
+    std::mutex m_;
+    std::condition_variable wake_, done_;
+    const std::function<void()>* job_ = nullptr;
+    std::uint64_t generation_ = 0;
+    int running_ = 0;
+    int n_workers_ = 0;
+};
 
 // backend "cpu" is the reference's engine-core; this library is its GPU arm only (api.rs:151-159).
 void require_gpu(const ExecutionContext& execution) {
@@ -262,6 +317,34 @@ std::vector<double> unilateral_cva_from_exposure_2f_batch(const ExecutionContext
     return out;
 }
 
+void run_on_gpu_streams(std::size_t n, const std::function<void(std::size_t)>& task,
+                        const std::vector<std::size_t>& claim_order) {
+    if (!claim_order.empty() && claim_order.size() != n)
+        throw std::logic_error("run_on_gpu_streams: claim_order must list every task once");
+    static std::mutex busy;
+    std::unique_lock<std::mutex> lock(busy, std::try_to_lock);
+    if (n < 2 || !lock.owns_lock()) {
+        for (std::size_t i = 0; i < n; ++i) task(i);
+        return;
+    }
+    static StreamWorkers* workers = new StreamWorkers(kGpuStreams - 1);
+    std::vector<std::exception_ptr> errors(n);
+    std::atomic<std::size_t> next{0};
+    const std::function<void()> drain = [&] {
+        for (std::size_t c; (c = next.fetch_add(1)) < n;) {
+            const std::size_t i = claim_order.empty() ? c : claim_order[c];
+            try {
+                task(i);
+            } catch (...) {
+                errors[i] = std::current_exception();
+            }
+        }
+    };
+    workers->run(drain);
+    for (const std::exception_ptr& e : errors)
+        if (e) std::rethrow_exception(e);
+}
+
 std::vector<ExposureProfile> shared_exposure_profiles(const IModel& model,
                                                       const std::vector<const IrSwapProduct*>& products,
                                                       const PricingContext& pricing,
@@ -269,8 +352,8 @@ std::vector<ExposureProfile> shared_exposure_profiles(const IModel& model,
     require_gpu(execution);
     // one scenario set for every calendar: each group is valued in a plan of its own (the per-trade
     // PFE rows of one group at a time, not of the whole book at once) whose dates end at the latest
-    // reset date of all, so every plan has the same grid; the counter-based Philox stream then gives
-    // every group the same paths
+    // reset date of all, so every plan has the same grid. Under HW1F the paths are simulated once and
+    // every plan reads them; under G2++ each plan simulates the same counter-based Philox paths again
     double t_last = -INFINITY;
     for (const IrSwapProduct* p : products)
         for (double t : reset_dates(*p)) t_last = std::max(t_last, t);
@@ -284,7 +367,31 @@ std::vector<ExposureProfile> shared_exposure_profiles(const IModel& model,
         groups[key].push_back(j);
     }
     std::vector<ExposureProfile> profiles(products.size());
-    for (const auto& kv : groups) {
+    // HW1F: the paths are simulated once, into a scenario set every group values on
+    eqhip_scenarios* scen = nullptr;
+    const auto* hw1_model = dynamic_cast<const HullWhite1FModel*>(&model);
+    if (hw1_model && t_last > 0.0) {
+        eqhip_model_hw1f m{hw1_model->a(), hw1_model->b(), hw1_model->sigma(), hw1_model->r0()};
+        eqhip_mc mc{pricing.n_paths(), (uint32_t)pricing.n_steps(), pricing.seed(), EQHIP_RNG_PHILOX,
+                    disc_of(execution)};
+        check(eqhip_scenarios_create(gpu_context(), &m, t_last, &mc, &scen));
+    }
+    struct ScenariosGuard {
+        eqhip_scenarios* s;
+        ~ScenariosGuard() {
+            if (s) eqhip_scenarios_destroy(s);
+        }
+    } scen_guard{scen};
+    // the groups spread over the GPU streams, largest first; each writes only its own trades' profiles
+    std::vector<const std::pair<const std::string, std::vector<std::size_t>>*> group_list;
+    for (const auto& kv : groups) group_list.push_back(&kv);
+    std::vector<std::size_t> largest_first(group_list.size());
+    for (std::size_t g = 0; g < group_list.size(); ++g) largest_first[g] = g;
+    std::stable_sort(largest_first.begin(), largest_first.end(), [&](std::size_t x, std::size_t y) {
+        return group_list[x]->second.size() > group_list[y]->second.size();
+    });
+    run_on_gpu_streams(group_list.size(), [&](std::size_t g) {
+        const auto& kv = *group_list[g];
         const IrSwapProduct& first = *products[kv.second.front()];
         const std::vector<double> own = reset_dates(first);
         std::vector<double> dates = own;
@@ -298,7 +405,12 @@ std::vector<ExposureProfile> shared_exposure_profiles(const IModel& model,
         const std::size_t nt = notionals.size(), nd = dates.size();
         std::vector<double> times(nd), ee(nt * nd), pfe(nt * nd);
         int32_t n_out = 0;
-        if (const auto* hw2 = dynamic_cast<const HullWhite2FModel*>(&model)) {
+        if (scen) {
+            check(eqhip_irs_hull_white_exposure_profile_batch_on(
+                gpu_context(), scen, notionals.data(), fixed_rates.data(), (int32_t)nt, first.start(),
+                first.payment_times().data(), first.accruals().data(), (int32_t)first.payment_times().size(),
+                dates.data(), (int32_t)nd, times.data(), ee.data(), pfe.data(), &n_out));
+        } else if (const auto* hw2 = dynamic_cast<const HullWhite2FModel*>(&model)) {
             check(eqhip_irs_hull_white_2f_exposure_profile_batch(
                 gpu_context(), hw2->a(), hw2->b(), hw2->sigma(), hw2->eta(), hw2->rho(), hw2->r0(), notionals.data(),
                 fixed_rates.data(), (int32_t)nt, first.start(), first.payment_times().data(), first.accruals().data(),
@@ -321,7 +433,7 @@ std::vector<ExposureProfile> shared_exposure_profiles(const IModel& model,
             pr.ee.assign(ee.begin() + i * nd, ee.begin() + i * nd + keep);
             pr.pfe_95.assign(pfe.begin() + i * nd, pfe.begin() + i * nd + keep);
         }
-    }
+    }, largest_first);
     return profiles;
 }
 

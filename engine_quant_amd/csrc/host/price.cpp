@@ -215,6 +215,15 @@ static PriceBatchResult price_batch_with_profiles(const Registries& registries,
         if (!profiles) profiles = compute_exposure_profile_batch(model, irs, pricing, execution);
         return *profiles;
     };
+    // one PV / DV01 ladder per (bump, bucketed); PV is the same in every ladder (the unbumped curve)
+    std::map<std::pair<double, bool>, PvDv01> ladders;
+    auto get_ladder = [&](double bump, bool bucketed) -> const PvDv01& {
+        auto it = ladders.find({bump, bucketed});
+        if (it == ladders.end())
+            it = ladders.emplace(std::make_pair(bump, bucketed),
+                                 compute_pv_dv01_batch(market, irs, bump, bucketed, execution)).first;
+        return it->second;
+    };
     for (std::size_t i = 0; i < measures.size(); ++i) {
         const std::string& type = resolved[i].registered_type;
         const std::string key = type + "#" + params_cache_key(measures[i].params);
@@ -240,7 +249,7 @@ static PriceBatchResult price_batch_with_profiles(const Registries& registries,
         } else if (type == "PV" || type == "DV01") {
             double bump = get_double(measures[i].params, "bump", 0.0001);
             bool bucketed = type == "DV01" && get_bool(measures[i].params, "bucketed", false);
-            PvDv01 v = compute_pv_dv01_batch(market, irs, bump, bucketed, execution);
+            const PvDv01& v = (type == "PV" && !ladders.empty()) ? ladders.begin()->second : get_ladder(bump, bucketed);
             for (std::size_t j = 0; j < irs.size(); ++j) {
                 MeasureResult r;
                 if (type == "PV") {
@@ -308,8 +317,8 @@ static PriceBatchResult price_many_shared(const Registries& registries, const st
         ins.first->second.push_back(i);
     }
     PriceBatchResult result(products.size());
-    for (const std::string& key : group_order) {
-        const std::vector<std::size_t>& idx = groups.at(key);
+    run_on_gpu_streams(group_order.size(), [&](std::size_t k) {
+        const std::vector<std::size_t>& idx = groups.at(group_order[k]);
         std::vector<const IProduct*> group;
         std::vector<ExposureProfile> group_profiles;
         for (std::size_t i : idx) {
@@ -319,7 +328,7 @@ static PriceBatchResult price_many_shared(const Registries& registries, const st
         PriceBatchResult g = price_batch_with_profiles(registries, group, measures, model, market, pricing, execution,
                                                        needs_profiles ? &group_profiles : nullptr);
         for (std::size_t j = 0; j < idx.size(); ++j) result[idx[j]] = PriceBatchResultEntry{idx[j], std::move(g[j].measures)};
-    }
+    });
     return result;
 }
 
@@ -340,14 +349,32 @@ PriceBatchResult price_many(const Registries& registries, const std::vector<cons
         if (ins.second) group_order.push_back(key);
         ins.first->second.push_back(i);
     }
+    // Each calendar group is an independent price_batch. On the GPU backend the groups run over
+    // several streams at once, the ones with the most trade-dates first; each writes only its own rows
+    // of the result, and an error is reported as the serial loop would report it.
     PriceBatchResult result(products.size());
-    for (const std::string& key : group_order) {
-        const std::vector<std::size_t>& idx = groups.at(key);
+    auto price_group = [&](std::size_t k) {
+        const std::vector<std::size_t>& idx = groups.at(group_order[k]);
         std::vector<const IProduct*> group;
         for (std::size_t i : idx) group.push_back(products[i]);
         PriceBatchResult g = price_batch(registries, group, measures, model, market, pricing, execution);
         for (std::size_t j = 0; j < idx.size(); ++j) result[idx[j]] = PriceBatchResultEntry{idx[j], std::move(g[j].measures)};
+    };
+    if (execution.backend() != "gpu") {
+        for (std::size_t k = 0; k < group_order.size(); ++k) price_group(k);
+        return result;
     }
+    std::vector<double> cost(group_order.size());
+    for (std::size_t k = 0; k < group_order.size(); ++k) {
+        const std::vector<std::size_t>& idx = groups.at(group_order[k]);
+        const auto* irs = dynamic_cast<const IrSwapProduct*>(products[idx.front()]);
+        cost[k] = (double)idx.size() * (irs ? (double)irs->payment_times().size() : 1.0);
+    }
+    std::vector<std::size_t> largest_first(group_order.size());
+    for (std::size_t k = 0; k < largest_first.size(); ++k) largest_first[k] = k;
+    std::stable_sort(largest_first.begin(), largest_first.end(),
+                     [&](std::size_t x, std::size_t y) { return cost[x] > cost[y]; });
+    run_on_gpu_streams(group_order.size(), price_group, largest_first);
     return result;
 }
 

@@ -37,7 +37,8 @@ EXPORTED = [
     "eqhip_berm_adam_step", "eqhip_berm_get_timing", "eqhip_berm_destroy",
     "eqhip_irs_hull_white_2f_exposure_profile", "eqhip_irs_hull_white_2f_exposure_profile_batch",
     "eqhip_unilateral_cva_from_exposure_2f", "eqhip_unilateral_cva_from_exposure_2f_batch", "eqhip_plan_create_2f",
-    "eqhip_xva_calculate", "eqhip_plan_exposure_cube", "eqhip_plan_xva",
+    "eqhip_xva_calculate", "eqhip_plan_exposure_cube", "eqhip_plan_xva", "eqhip_scenarios_create",
+    "eqhip_scenarios_destroy", "eqhip_irs_hull_white_exposure_profile_batch_on",
 ]
 
 PLAN_TRADE_EE, PLAN_KEEP_CUBE = 1, 2  # flags of eqhip_plan_create's want_trade_ee argument
@@ -128,6 +129,11 @@ def lib():
         L.eqhip_xva_calculate.argtypes = [ctypes.c_void_p, ctypes.POINTER(XvaParamsC), _PD, _I32, _U64, _PD, _PD,
                                           _PD, _PD, _PD, _PD]
         L.eqhip_plan_exposure_cube.argtypes = [ctypes.c_void_p, _I32, _PD, _PD]
+        L.eqhip_scenarios_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(ModelHW1F), _D, ctypes.POINTER(McC),
+                                             ctypes.POINTER(ctypes.c_void_p)]
+        L.eqhip_scenarios_destroy.argtypes = [ctypes.c_void_p]
+        L.eqhip_irs_hull_white_exposure_profile_batch_on.argtypes = [
+            ctypes.c_void_p, ctypes.c_void_p, _PD, _PD, _I32, _D, _PD, _PD, _I32, _PD, _I32, _PD, _PD, _PD, _PI32]
         L.eqhip_plan_xva.argtypes = [ctypes.c_void_p, _I32, ctypes.POINTER(XvaParamsC), _PD, _PD, _PD]
         L.eqhip_pv_dv01.argtypes = [ctypes.c_void_p, _PD, _PD, _I32, ctypes.POINTER(PortfolioC), _D, _PD,
                                     _PD, _PD]
@@ -209,6 +215,30 @@ class Portfolio:
         return PortfolioC(self.n_trades, self.n_sets, self.set_id.ctypes.data_as(_PI32),
                           _dp(self.notional), _dp(self.fixed_rate), _dp(self.start),
                           self.flow_off.ctypes.data_as(_PI32), _dp(self.pay), _dp(self.acc))
+
+
+class Scenarios:
+    """Device-resident HW1F scenario set (eqhip_scenarios_create / destroy): the short rate at every grid
+    step of t_max / n_steps on n_paths Philox paths, valued on by Context.exposure_profile_batch_on."""
+
+    def __init__(self, ctx, model, t_max, n_paths, n_steps, seed=0, disc=DISC_EULER):
+        self.ctx = ctx
+        self.handle = ctypes.c_void_p()
+        m = ModelHW1F(*model)
+        mc = McC(n_paths, n_steps, seed, RNG_PHILOX, disc)
+        _check(lib().eqhip_scenarios_create(ctx.handle, ctypes.byref(m), float(t_max), ctypes.byref(mc),
+                                            ctypes.byref(self.handle)))
+
+    def close(self):
+        if self.handle:
+            lib().eqhip_scenarios_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Plan:
@@ -356,6 +386,22 @@ class Context:
             self.handle, a, b, sigma, r0, _dp(notionals), _dp(fixed_rates), nt, start, _dp(pay), _dp(acc),
             len(pay), _dp(times), nd, n_steps, n_paths, seed, rng, disc, _dp(z), int(allow_any_date), _dp(ot),
             _dp(oe), _dp(op), ctypes.byref(n_out)))
+        n = n_out.value
+        return ot[:n], oe[:, :n], op[:, :n]
+
+    def scenarios(self, model, t_max, n_paths, n_steps, seed=0, disc=DISC_EULER):
+        """A scenario set (HW1F paths on the grid of t_max / n_steps, every step kept on the device)."""
+        return Scenarios(self, model, t_max, n_paths, n_steps, seed, disc)
+
+    def exposure_profile_batch_on(self, scenarios, notionals, fixed_rates, start, pay, acc, times):
+        notionals, fixed_rates = _f64(notionals), _f64(fixed_rates)
+        pay, acc, times = _f64(pay), _f64(acc), _f64(times)
+        nt, nd = len(notionals), len(times)
+        ot, oe, op = np.zeros(nd), np.zeros((nt, nd)), np.zeros((nt, nd))
+        n_out = ctypes.c_int32()
+        _check(lib().eqhip_irs_hull_white_exposure_profile_batch_on(
+            self.handle, scenarios.handle, _dp(notionals), _dp(fixed_rates), nt, start, _dp(pay), _dp(acc), len(pay),
+            _dp(times), nd, _dp(ot), _dp(oe), _dp(op), ctypes.byref(n_out)))
         n = n_out.value
         return ot[:n], oe[:, :n], op[:, :n]
 

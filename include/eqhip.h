@@ -153,6 +153,27 @@ int eqhip_irs_hull_white_exposure_profile_batch(
     const double* host_draws, int32_t allow_any_date, double* out_times, double* out_ee,
     double* out_pfe, int32_t* n_out);
 
+/* A scenario set: the HW1F short-rate paths of one model on the grid of t_max / n_steps (every grid
+ * step kept on the device), simulated once from the Philox stream of mc->seed. Replaces the
+ * per-calendar-group re-simulation of price_many (cpp/engine/src/price.cpp:534-578; measure.cpp:348-365)
+ * when every group is valued on one scenario set. The set is read-only once created: any context of
+ * the same device may value on it, from any thread, at the same time. */
+typedef struct eqhip_scenarios eqhip_scenarios;
+int eqhip_scenarios_create(eqhip_context* ctx, const eqhip_model_hw1f* model, double t_max, const eqhip_mc* mc,
+                           eqhip_scenarios** out);
+int eqhip_scenarios_destroy(eqhip_scenarios* scenarios);
+
+/* eqhip_irs_hull_white_exposure_profile_batch on a scenario set instead of a simulation of its own:
+ * the same homogeneous batch and outputs, with the set's model, grid, paths and seed. The dates may be
+ * any dates in [0, t_max] (as with allow_any_date) and the last of them must be the set's t_max (so
+ * the batch's grid is the set's). Equals the batch call with the same arguments and allow_any_date,
+ * bit for bit. */
+int eqhip_irs_hull_white_exposure_profile_batch_on(
+    eqhip_context* ctx, const eqhip_scenarios* scenarios, const double* notionals, const double* fixed_rates,
+    int32_t n_trades, double start, const double* payment_times, const double* accruals, int32_t n_flows,
+    const double* monitoring_times, int32_t n_dates, double* out_times, double* out_ee, double* out_pfe,
+    int32_t* n_out);
+
 /* Replaces engine::ffi::unilateral_cva_from_exposure (engine-ffi/src/lib.rs:344-354;
  * api.rs:273-304; exposure.rs:242-275). */
 int eqhip_unilateral_cva_from_exposure(eqhip_context* ctx, double a, double b, double sigma,

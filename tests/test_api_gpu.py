@@ -136,6 +136,42 @@ def test_price_many_groups_calendars_like_price_batch():
         assert np.allclose(many[i].measures["ExpectedExposure"].primary, ee, rtol=1e-10, atol=1e-6)
 
 
+def test_price_many_over_gpu_streams_equals_serial_price_batch():
+    """price_many runs its calendar groups over several GPU streams at once: every result equals, bit for
+    bit, price_batch of that group alone, for groups of very different sizes, called twice in a row."""
+    eng = engine.Engine()
+    model = eng.create_model("HullWhite1F", _hull_white_params())
+    market = engine.MarketSnapshot(pillars=[1.0, 3.0, 10.0], zero_rates=[0.02, 0.023, 0.027], hazard_rate=0.02,
+                                   recovery_rate=0.4)
+    pr, ex = _golden_pricing(4000, 11), _gpu_execution(discretisation="exact")
+    rng = np.random.default_rng(5)
+    trades, calendars = [], []
+    for g in range(11):
+        n_pay, freq = 2 + 3 * g, 1.0 if g % 2 else 0.5
+        pay = [freq * (i + 1) for i in range(n_pay)]
+        for _ in range(1 + (g * 7) % 9):
+            trades.append(eng.create_product("IRSwap", {"notional": float(rng.choice([-1, 1]) * rng.uniform(5e5, 5e6)),
+                                                        "fixed_rate": float(rng.uniform(0.015, 0.03)),
+                                                        "payment_times": pay, "accruals": [freq] * n_pay}))
+            calendars.append(g)
+    order = rng.permutation(len(trades))
+    trades = [trades[i] for i in order]
+    calendars = [calendars[i] for i in order]
+    names = ["PV", "DV01", "ExpectedExposure", "PFE95", "UnilateralCVA"]
+    first = eng.price_many(trades, names, model, market, pr, ex)
+    again = eng.price_many(trades, names, model, market, pr, ex)
+    for g in range(11):
+        idx = [i for i, c in enumerate(calendars) if c == g]
+        alone = eng.price_batch([trades[i] for i in idx], names, model, market, pr, ex)
+        for j, i in enumerate(idx):
+            for res in (first[i], again[i]):
+                assert res.trade_index == i
+                for n in names:
+                    assert np.array_equal(res.measures[n].primary, alone[j].measures[n].primary), (g, n)
+                    assert np.array_equal(res.measures[n].times, alone[j].measures[n].times), (g, n)
+                    assert res.measures[n].scalar == alone[j].measures[n].scalar, (g, n)
+
+
 def test_exact_discretisation_is_selectable():
     eng = engine.Engine()
     model = eng.create_model("HullWhite1F", _hull_white_params())

@@ -67,6 +67,24 @@ def test_cpu_backend_is_the_reference_engine_not_this_library():
         eng.price(product, ["NoSuchMeasure"], model, market, pricing, engine.ExecutionContext({"backend": "gpu"}))
 
 
+def test_price_many_reports_the_error_a_serial_loop_would():
+    """price_many runs its calendar groups over several GPU streams, largest first, and still reports
+    the error of the first failing group in calendar order: here the first group's (a trade without
+    fixed_rate, refused before any device work), though that group is claimed last and the others
+    fail on the missing device (or succeed on a GPU) first. Twice: the workers outlive a failed call."""
+    eng = engine.Engine()
+    model = eng.create_model("HullWhite1F", {"a": 0.1, "b": 0.03, "sigma": 0.01, "r0": 0.02})
+    par = eng.create_product("IRSwap", {"notional": 1e6, "payment_times": [1.0, 2.0], "accruals": [1.0, 1.0]})
+    longer = [eng.create_product("IRSwap", {"notional": 1e6, "fixed_rate": 0.02,
+                                            "payment_times": [float(t) for t in range(1, n + 1)],
+                                            "accruals": [1.0] * n}) for n in (10, 20, 30)]
+    market = engine.MarketSnapshot(pillars=[1.0], zero_rates=[0.02])
+    pricing = engine.PricingContext({"n_paths": 64.0, "n_steps": 10.0, "seed": 1.0})
+    for _ in range(2):
+        with pytest.raises(ValueError, match="fixed_rate"):
+            eng.price_many([par] + longer, ["PV"], model, market, pricing, engine.ExecutionContext({"backend": "gpu"}))
+
+
 def test_synthetic_portfolio_generator_shape():
     pillars, zeros = zero_curve(7)
     assert len(pillars) == 20 and pillars[0] == 0.25 and pillars[-1] == 30.0
