@@ -880,17 +880,22 @@ __global__ void reset_counter_kernel(unsigned int* counter) { *counter = 0u; }
 // ---------------------------------------------------------------------------------------------
 // EE + PFE95 of one (set, date) row. EE = (fixed-order sum) / n. PFE = the element of rank
 // q = min(ceil(0.95 n) - 1, n - 1) of the ascending order (exposure.rs:42-49), found exactly on
-// the IEEE-754 bit patterns (monotone for x >= +0.0): 12-bit MSB radix passes narrow the key
-// prefix (the first pass is fused with the EE sum); once at most kSelectCand candidates share the
-// prefix they are compacted into LDS and ranked by a bitonic sort.
+// the IEEE-754 bit patterns (monotone for x >= +0.0). The first pass is fused with the EE sum and
+// builds two histograms: the top 12 bits (sign and exponent: one bin per binade) and, for the
+// kSelectWin binades around the order statistic of the row's first 8192 keys, the binade's top 10
+// mantissa bits. When the target binade lies in that window, one pass fixes 22 bits of the key;
+// otherwise 12-bit MSB radix passes narrow the prefix. Once at most kSelectCand candidates share
+// the prefix they are compacted into LDS and ranked by a bitonic sort.
 // ---------------------------------------------------------------------------------------------
 namespace {
 
-// Block-wide: the bin holding `rank` among hist[0..4096) (counts), as (bin, count before it).
+// Block-wide: the bin holding `rank` among hist[0..NB) (counts), as (bin, count before it).
+template <int NB>
 __device__ void locate_rank(const unsigned int* hist, unsigned int* scan, unsigned long long rank,
                             unsigned int* s_bin, unsigned long long* s_before) {
+    static_assert(NB % kSelectThreads == 0, "bins per thread");
     const int tid = threadIdx.x;
-    const int per = 4096 / kSelectThreads;
+    const int per = NB / kSelectThreads;
     unsigned int local = 0;
     for (int b = 0; b < per; ++b) local += hist[tid * per + b];
     scan[tid] = local;
@@ -933,13 +938,14 @@ __device__ __forceinline__ void hist_add(unsigned int* hist, unsigned int b) {
     }
 }
 
-// Visits x[i] for i = tid, tid + T, tid + 2T, ... in that order, with eight loads in flight per
-// thread (a pass over a row is latency-bound with one).
+// Visits x[i] for i = start, start + T, start + 2T, ... in that order, with eight loads in flight
+// per thread (a pass over a row is latency-bound with one).
+constexpr int kSelectU = 8;
 template <class F>
-__device__ __forceinline__ void select_pass(const unsigned long long* x, uint64_t n, int tid, F f) {
-    constexpr int U = 8;
+__device__ __forceinline__ void select_pass(const unsigned long long* x, uint64_t n, uint64_t start, F f) {
+    constexpr int U = kSelectU;
     const gptr<unsigned long long> gx = glb(x);
-    uint64_t i = tid;
+    uint64_t i = start;
     for (; i + (uint64_t)(U - 1) * kSelectThreads < n; i += (uint64_t)U * kSelectThreads) {
         unsigned long long k[U];
 #pragma unroll
@@ -954,7 +960,10 @@ __global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow*
     __shared__ unsigned int hist[4096];
     __shared__ unsigned int scan[kSelectThreads];
     __shared__ double red[kSelectThreads];
+    // the window histogram of pass 1 ([kSelectWin][1024]); the same LDS later holds the candidates
     __shared__ unsigned long long cand[kSelectCand];
+    unsigned int* hist2 = (unsigned int*)cand;
+    static_assert(kSelectWin * 1024 * sizeof(unsigned int) <= sizeof(cand), "window histogram fits");
     __shared__ unsigned long long s_prefix, s_rank, s_before, s_min, s_max;
     __shared__ unsigned int s_bin, s_count, s_ncand;
 
@@ -964,27 +973,71 @@ __global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow*
     const unsigned long long* x = (const unsigned long long*)row.x;
 
     for (int b = tid; b < 4096; b += kSelectThreads) hist[b] = 0u;
+    for (int b = tid; b < kSelectWin * 1024; b += kSelectThreads) hist2[b] = 0u;
     __syncthreads();
-    // pass 1: EE sum and histogram of bits 63..52
-    // keys in bin 0 (zero exposures, the common case) are counted per thread, not by LDS atomics
-    // that would all hit one address
+    // pass 1: EE sum, histogram of bits 63..52 and the window histogram. The first kSelectU keys of
+    // every thread (x[tid + u T], the row's first U T keys) are binned first; the binade of their
+    // rank-scaled order statistic centres the window.
+    constexpr uint64_t kSample = (uint64_t)kSelectU * kSelectThreads;
+    unsigned long long k0[kSelectU];
+#pragma unroll
+    for (int u = 0; u < kSelectU; ++u) {
+        const uint64_t i = (uint64_t)tid + (uint64_t)u * kSelectThreads;
+        k0[u] = (i < n) ? glb(x)[i] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < kSelectU; ++u)
+        if ((uint64_t)tid + (uint64_t)u * kSelectThreads < n) hist_add(hist, (unsigned int)(k0[u] >> 52));
+    __syncthreads();
+    const uint64_t ns = n < kSample ? n : kSample;
+    locate_rank<4096>(hist, scan, (row.q * ns) / n, &s_bin, &s_before);
+    const unsigned int est = s_bin;
+    // window [wb, wb + kSelectWin) of binades, inside bins 1..4095 (bin 0 holds zeros and subnormals)
+    const unsigned int wb = est > (unsigned)(kSelectWin / 2) ? min(est - kSelectWin / 2, 4096u - kSelectWin) : 1u;
+    // A key inside the window is counted in the window histogram only (its binade's bin receives the
+    // window's per-binade totals after the pass), so every key costs one LDS atomic; the sample keys
+    // inside the window move there from the binade histogram.
     double sum = 0.0;
     unsigned int c0 = 0u;
-    select_pass(x, n, tid, [&](unsigned long long key) {
+#pragma unroll
+    for (int u = 0; u < kSelectU; ++u) {
+        if ((uint64_t)tid + (uint64_t)u * kSelectThreads < n) {
+            sum += __longlong_as_double(k0[u]);
+            const unsigned int b = (unsigned int)(k0[u] >> 52);
+            const unsigned int w = b - wb;  // wraps for b < wb
+            if (w < (unsigned)kSelectWin) {
+                atomicSub(&hist[b], 1u);
+                hist_add(hist2, (w << 10) | (unsigned int)((k0[u] >> 42) & 1023u));
+            }
+        }
+    }
+    // keys in bin 0 (zero exposures, the common case) are counted per thread, not by LDS atomics
+    // that would all hit one address
+    select_pass(x, n, (uint64_t)tid + kSample, [&](unsigned long long key) {
         sum += __longlong_as_double(key);
         const unsigned int b = (unsigned int)(key >> 52);
+        const unsigned int w = b - wb;
         if (b == 0u) ++c0;
+        else if (w < (unsigned)kSelectWin) hist_add(hist2, (w << 10) | (unsigned int)((key >> 42) & 1023u));
         else hist_add(hist, b);
     });
     if (c0) atomicAdd(&hist[0], c0);
     red[tid] = sum;
     __syncthreads();
+    {  // per-binade totals of the window into the binade histogram (a thread's bins share one binade)
+        constexpr int per = kSelectWin * 1024 / kSelectThreads;
+        static_assert(1024 % per == 0, "a thread's window bins lie in one binade");
+        unsigned int c = 0u;
+#pragma unroll
+        for (int j = 0; j < per; ++j) c += hist2[tid * per + j];
+        if (c) atomicAdd(&hist[wb + (unsigned)(tid * per) / 1024u], c);
+    }
     for (int w = kSelectThreads / 2; w > 0; w >>= 1) {
         if (tid < w) red[tid] = red[tid] + red[tid + w];
         __syncthreads();
     }
     if (tid == 0) *row.ee = red[0] / (double)n;
-    locate_rank(hist, scan, row.q, &s_bin, &s_before);
+    locate_rank<4096>(hist, scan, row.q, &s_bin, &s_before);
     if (tid == 0) {
         s_prefix = (unsigned long long)s_bin << 52;
         s_rank = row.q - s_before;
@@ -993,6 +1046,22 @@ __global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow*
     __syncthreads();
 
     int hi = 52;  // bits >= hi are fixed
+    {  // the target binade inside the window: its mantissa histogram fixes ten more bits
+        const unsigned int e = s_bin;
+        const unsigned int w = e - wb;
+        if (s_count > (unsigned int)kSelectCand && e != 0u && w < (unsigned)kSelectWin) {
+            const unsigned long long rank = s_rank;
+            const unsigned int* h = hist2 + (w << 10);
+            locate_rank<1024>(h, scan, rank, &s_bin, &s_before);
+            if (tid == 0) {
+                s_prefix |= (unsigned long long)s_bin << 42;
+                s_rank = rank - s_before;
+                s_count = h[s_bin];
+            }
+            __syncthreads();
+            hi = 42;
+        }
+    }
     while (true) {
         const unsigned long long prefix = s_prefix;
         const unsigned int count = s_count;
@@ -1028,7 +1097,7 @@ __global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow*
             if (tid == 0) *row.pfe = __longlong_as_double(s_min);
             return;
         }
-        locate_rank(hist, scan, s_rank, &s_bin, &s_before);
+        locate_rank<4096>(hist, scan, s_rank, &s_bin, &s_before);
         if (tid == 0) {
             s_prefix = prefix | ((unsigned long long)s_bin << sh);
             s_rank = s_rank - s_before;

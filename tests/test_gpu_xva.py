@@ -123,3 +123,31 @@ def test_plan_without_cube_refuses(ctx):
             plan.exposure_cube(0)
     finally:
         plan.close()
+
+
+def test_exact_select_on_adversarial_rows(ctx):
+    """Rows of 200,003 paths that drive each branch of the select: the target binade inside the window
+    centred by the row's first 8192 keys, a window misplaced by those keys, a target prefix still
+    holding more than 4096 keys after the window (further 12-bit passes), ties at the target, a zero
+    target, and keys spread over 60 binades. The quantiles must be the oracle's, bit for bit."""
+    rng = np.random.default_rng(29)
+    n = 200_003
+    rows = [rng.normal(5e4, 1e5, n),                          # the common case
+            np.exp2(rng.uniform(-30.0, 30.0, n))]              # few keys per binade
+    r = rng.normal(1e6, 2e5, n)
+    r[:8192] = rng.uniform(1e-3, 2e-3, 8192)                  # the first keys place the window far below
+    rows.append(r)
+    rows.append(1.0 + rng.uniform(0.0, 2.0 ** -20, n))        # one 22-bit prefix for every key
+    r = np.full(n, 1e5)
+    m = rng.random(n) < 0.08
+    r[m] = rng.uniform(0.0, 2e5, int(m.sum()))                # 92% ties at the target
+    rows.append(r)
+    r = np.zeros(n)
+    m = rng.random(n) < 0.03
+    r[m] = rng.uniform(1.0, 2.0, int(m.sum()))                # the target is zero
+    rows.append(r)
+    paths = np.stack(rows)
+    q = xva.ExposureCube("Q", np.arange(1, len(rows) + 1, dtype=float), paths)
+    req = xva.XvaRequest(counterparty_default=xva.DefaultSpec(0.02, 0.4))
+    res = xva.calculate(ctx, req, q)
+    _compare(res, _oracle(req, q), "adversarial rows")
