@@ -612,6 +612,30 @@ __global__ void __launch_bounds__(kThreads) mlp_train_kernel(Book bk, PathSet ps
     const int batch = tiles_per_wave * 16 * kWaves;
     const float inv_b2 = 2.0f / (float)batch;
     const uint64_t n = ps.n;
+    // SRC 0: a sample is a Philox-drawn (path, date) of the training set, gathered from HBM one tile
+    // ahead, so that the gathers of tile tt + 1 are in flight during tile tt's forward and backward pass
+    // (a block has one wave per SIMD, with no other wave to hide their latency)
+    struct Gather {
+        double r, lb, y0, bn;
+        int tv, k;
+    };
+    auto gather = [&](int tt) {
+        Gather s;
+        const int i = (w * tiles_per_wave + tt) * 16 + n16;
+        uint32_t o[4];
+        philox4x32_10((uint32_t)i, step, (uint32_t)j, 0x7A11u, (uint32_t)seed, (uint32_t)(seed >> 32), o);
+        const uint64_t p = __umul64hi(((uint64_t)o[0] << 32) | o[1], n);
+        s.k = 1 + (int)__umulhi(o[2], (uint32_t)bk.kmax[j]);
+        const uint64_t ro = (uint64_t)s.k * n + p;
+        s.r = ps.R[ro];
+        s.lb = ps.LB[ro];
+        s.bn = ps.BN[ro];
+        s.tv = ps.tau[(uint64_t)j * n + p];
+        s.y0 = ps.y0[(uint64_t)j * n + p];
+        return s;
+    };
+    Gather next{};
+    if (SRC == 0 && tiles_per_wave > 0) next = gather(0);
     __syncthreads();
     for (int tt = 0; tt < tiles_per_wave; ++tt) {
         const int i = (w * tiles_per_wave + tt) * 16 + n16;
@@ -625,18 +649,13 @@ __global__ void __launch_bounds__(kThreads) mlp_train_kernel(Book bk, PathSet ps
             msk = xm[(size_t)j * batch + i];
         } else {
 #pragma clang fp contract(off)
-            uint32_t o[4];
-            philox4x32_10((uint32_t)i, step, (uint32_t)j, 0x7A11u, (uint32_t)seed, (uint32_t)(seed >> 32), o);
-            const uint64_t p = __umul64hi(((uint64_t)o[0] << 32) | o[1], n);
-            const int k = 1 + (int)__umulhi(o[2], (uint32_t)bk.kmax[j]);
-            const uint64_t ro = (uint64_t)k * n + p;
-            const double r = ps.R[ro], lb = ps.LB[ro];
-            const int tv = ps.tau[(uint64_t)j * n + p];
-            u0 = (float)((r - nm.r_shift) * nm.r_scale);
-            u1 = (float)(lb * nm.lb_scale + nm.lb_shift);
-            u2 = (float)((double)k / bk.spy * nm.t_scale + nm.t_shift);
-            tgt = (float)(ps.y0[(uint64_t)j * n + p] * ps.BN[ro] / bk.notional[j] * nm.y_scale);
-            msk = (tv < 0 || tv * bk.spy >= k) ? 1.0f : 0.0f;
+            const Gather c = next;
+            if (tt + 1 < tiles_per_wave) next = gather(tt + 1);
+            u0 = (float)((c.r - nm.r_shift) * nm.r_scale);
+            u1 = (float)(c.lb * nm.lb_scale + nm.lb_shift);
+            u2 = (float)((double)c.k / bk.spy * nm.t_scale + nm.t_shift);
+            tgt = (float)(c.y0 * c.bn / bk.notional[j] * nm.y_scale);
+            msk = (c.tv < 0 || c.tv * bk.spy >= c.k) ? 1.0f : 0.0f;
         }
         if (g == 0) {
             sU[w][0 * 16 + n16] = (__bf16)u0;
