@@ -81,6 +81,10 @@ def lib():
             _D, _D, _D, _D, _I, _I, _PI32, _PD, _PD, _PD, _PI32, _PD, _PD, _PD, _I, _U32, _U64, _I,
             _PD, _D, _D, _PD, _PD, _PD, _PD, _PD, _PD, _PD, _PD, _I]
         L.eqr_max_threads.restype = _I
+        L.eqr_portfolio_exposure_fused.restype = _I
+        L.eqr_portfolio_exposure_fused.argtypes = [
+            _D, _D, _D, _D, _I, _I, _PI32, _PD, _PD, _PD, _PI32, _PD, _PD, _PD, _I, _U32, _U64, _I, _PD, _U64,
+            _D, _D, _I, _PD, _PD, _PD, _PD, _PD, _PD, _I]
         L.eqr_collateral_adjust.restype = _D
         L.eqr_collateral_adjust.argtypes = [_D, _I, _I, _D, _D, _D]
         L.eqr_xva_calculate.argtypes = [_PD, _I, _U64, _PD, _PD, _PD, _D, _D, _PD, _I, _D, _D, _PD, _I, _D, _D, _D,
@@ -413,6 +417,41 @@ def portfolio_exposure(model, port, dates, n_steps, z, scheme=0, hazard_rate=0.0
         _dp(out["set_pfe"]), _dp(out["set_cva"]), _dp(out.get("trade_ee")),
         _dp(out.get("trade_pfe")), _dp(out.get("trade_cva")), _dp(out.get("path_mtm")),
         _dp(out.get("set_mtm")), n_threads)
+    return out
+
+
+def portfolio_exposure_fused(model, port, dates, n_steps, z=None, n_paths=None, seed=0, scheme=0, hazard_rate=0.0,
+                             recovery_rate=0.0, want_trade=False, share_classes=False, want_set_mtm=False,
+                             n_threads=0):
+    """The netting-set exposure run (HW1F) as a fast CPU engine organises it (eq_ref_fused.c): one
+    simulation, each distinct zero-coupon bond once per (path, date), OpenMP over paths, PFE95 by
+    selection. ``z`` = normals [n_steps][n_paths], or None to draw the Philox stream of ``seed`` for
+    ``n_paths`` paths inside the run. ``share_classes``: trades of one schedule share their legs (values
+    then agree with portfolio_exposure to rounding; without it set values are bit-identical)."""
+    set_id = np.ascontiguousarray(port["set_id"], dtype=np.int32)
+    flow_off = np.ascontiguousarray(port["flow_off"], dtype=np.int32)
+    notional, fixed, start = _f64(port["notional"]), _f64(port["fixed_rate"]), _f64(port["start"])
+    pay, acc = _f64(port["pay"]), _f64(port["acc"])
+    n_trades = len(notional)
+    n_sets = int(set_id.max()) + 1
+    dates = _f64(dates)
+    if z is not None:
+        z = _f64(z)
+        n_paths = z.shape[1]
+    nd = len(dates)
+    out = {"set_ee": np.zeros((n_sets, nd)), "set_pfe": np.zeros((n_sets, nd)), "set_cva": np.zeros(n_sets)}
+    if want_trade:
+        out.update(trade_ee=np.zeros((n_trades, nd)), trade_cva=np.zeros(n_trades))
+    if want_set_mtm:
+        out["set_mtm"] = np.zeros((nd, n_sets, int(n_paths)))
+    a, b, sigma, r0 = model
+    rc = lib().eqr_portfolio_exposure_fused(
+        a, b, sigma, r0, n_trades, n_sets, set_id.ctypes.data_as(_PI32), _dp(notional), _dp(fixed), _dp(start),
+        flow_off.ctypes.data_as(_PI32), _dp(pay), _dp(acc), _dp(dates), nd, n_steps, int(n_paths), scheme, _dp(z),
+        int(seed), hazard_rate, recovery_rate, int(bool(share_classes)), _dp(out["set_ee"]), _dp(out["set_pfe"]),
+        _dp(out["set_cva"]), _dp(out.get("trade_ee")), _dp(out.get("trade_cva")), _dp(out.get("set_mtm")), n_threads)
+    if rc != 0:
+        raise ValueError("portfolio_exposure_fused: empty portfolio, dates or paths")
     return out
 
 

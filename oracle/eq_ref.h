@@ -109,6 +109,22 @@ int eqr_portfolio_exposure(double a, double b, double sigma, double r0, int n_tr
                            double* trade_pfe, double* trade_cva, double* path_mtm,
                            double* set_mtm, int n_threads);
 
+/* ---- the same run as a fast CPU engine organises it (eq_ref_fused.c) ----
+ * One simulation per path, every zero-coupon bond of a distinct live cash-flow time once per
+ * (path, date), OpenMP over paths, PFE95 by selection. z = NULL draws the Philox stream of `seed`
+ * per path (the normals eqr_philox_normals would produce). share_classes = 1: trades with identical
+ * schedules share one floating leg and one annuity per (path, date). Outputs: set_ee, set_pfe
+ * [n_sets][n_dates] (required); set_cva [n_sets], trade_ee [n_trades][n_dates], trade_cva, set_mtm
+ * [n_dates][n_sets][n_paths] (nullable; a date on grid index 0 holds path 0's value and zeros). */
+int eqr_portfolio_exposure_fused(double a, double b, double sigma, double r0, int n_trades, int n_sets,
+                                 const int32_t* set_id, const double* notional, const double* fixed_rate,
+                                 const double* start, const int32_t* flow_off, const double* pay,
+                                 const double* acc, const double* dates, int n_dates, uint32_t n_steps,
+                                 uint64_t n_paths, int scheme, const double* z, uint64_t seed,
+                                 double hazard_rate, double recovery_rate, int share_classes, double* set_ee,
+                                 double* set_pfe, double* set_cva, double* trade_ee, double* trade_cva,
+                                 double* set_mtm, int n_threads);
+
 /* ---- Hull-White 2F (G2++), models/hull_white_2f.rs; state (x, y) from (0, 0) ---- */
 double eqr_hw2_var(double z, double vol, double tau);
 double eqr_hw2_cross(double a, double b, double sigma, double eta, double rho, double tau);
