@@ -35,6 +35,29 @@ def leg_scale(port, model, dates):
     return scale
 
 
+def assert_value_rel(actual, expected, rel=1e-10, leg=None, what=""):
+    """EE / PFE / CVA against the oracle relative to the quantity itself: |a - e| <= rel * max(|e|, m, f),
+    with m the largest |e| of the row (the last axis: a profile over dates; a scalar or a vector of
+    CVAs is its own row) and f = 1e-6 * leg, the legs' magnitude of the row (the floor at which the
+    rounding of a value that is a difference of its legs is the whole error, for trades whose
+    exposure is almost nil). Rows the oracle leaves at exactly zero must be exactly zero."""
+    a, e = np.asarray(actual, dtype=np.float64), np.asarray(expected, dtype=np.float64)
+    assert a.shape == e.shape, f"{what}: shape {a.shape} != {e.shape}"
+    if e.size == 0:
+        return
+    row = np.abs(e).max(axis=-1, keepdims=True) if e.ndim >= 2 else np.abs(e)
+    s = np.maximum(np.abs(e), row)
+    if leg is not None:
+        lg = np.asarray(leg, dtype=np.float64)
+        s = np.maximum(s, 1e-6 * (lg.reshape(lg.shape + (1,) * (e.ndim - lg.ndim)) if e.ndim > lg.ndim else lg))
+    err = np.abs(a - e)
+    bad = err > rel * s
+    if bad.any():
+        i = np.unravel_index(int(np.argmax(np.where(bad, err / np.where(s > 0, s, 1e-300), 0.0))), e.shape)
+        raise AssertionError(f"{what}: {int(bad.sum())} of {e.size} values off; worst at {i}: {a[i]!r} vs {e[i]!r} "
+                             f"(scale {s[i]:.3e}, bound {rel:.0e})")
+
+
 def assert_rel(actual, expected, rel, scale=None, what=""):
     actual, expected = np.asarray(actual), np.asarray(expected)
     s = np.maximum(np.abs(expected), 1.0) if scale is None else scale

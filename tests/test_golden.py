@@ -62,7 +62,7 @@ def test_reference_values_against_oracle():
 def test_gpu_matches_netting_fixture(ctx):
     from engine_quant_amd import eqhip
     from engine_quant_amd.eqhip import Portfolio
-    from helpers import assert_rel, leg_scale
+    from helpers import assert_rel, assert_value_rel, leg_scale
     fx = _load("netting_tiny.json")
     p = _arrays(fx["portfolio"])
     port = Portfolio(set_id=p["set_id"], notional=p["notional"], fixed_rate=p["fixed_rate"], start=p["start"],
@@ -77,10 +77,10 @@ def test_gpu_matches_netting_fixture(ctx):
         scale = leg_scale(port, model, fx["dates"])
         set_scale = np.array([scale[port.set_id == s].sum() for s in range(port.n_sets)])
         assert_rel(got["path_mtm"], ref["path_mtm"], 1e-12, scale=scale[None, :, None], what=f"{name} path MtM")
-        assert_rel(got["set_ee"], ref["set_ee"], 1e-10, scale=set_scale[:, None], what=f"{name} set EE")
-        assert_rel(got["set_pfe"], ref["set_pfe"], 1e-10, scale=set_scale[:, None], what=f"{name} set PFE")
-        assert_rel(got["set_cva"], ref["set_cva"], 1e-10, scale=set_scale, what=f"{name} set CVA")
-        assert_rel(got["trade_ee"], ref["trade_ee"], 1e-10, scale=scale[:, None], what=f"{name} trade EE")
+        assert_value_rel(got["set_ee"], ref["set_ee"], leg=set_scale, what=f"{name} set EE")
+        assert_value_rel(got["set_pfe"], ref["set_pfe"], leg=set_scale, what=f"{name} set PFE")
+        assert_value_rel(got["set_cva"], ref["set_cva"], leg=set_scale, what=f"{name} set CVA")
+        assert_value_rel(got["trade_ee"], ref["trade_ee"], leg=scale, what=f"{name} trade EE")
 
 
 @pytest.mark.gpu

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import engine_quant_amd as engine
+import oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -70,6 +71,37 @@ def test_recovery_and_hazard_derivatives_match_the_cva_formula():
     dS = -tp * np.exp(-h * tp) + t * np.exp(-h * t)
     analytic = np.sum((1 - 0.4) * ee * dS * p0)
     assert abs(g_h.scalar - analytic) <= 1e-6 * abs(analytic)
+
+
+def test_model_greeks_match_oracle_differences_on_the_same_paths():
+    """The bump-and-reval model Greeks (default bumps, greeks.cpp:160-164) of UnilateralCVA and of the EE /
+    PFE95 profile against central differences of the oracle's runs on the same Philox paths, exact scheme:
+    the GPU's bumped runs and the oracle's see identical shocks (the reference's test of the same
+    stencil, test_greeks.cpp:923-983, compares with its own bumped runs)."""
+    n_paths, n_steps = 4000, 104
+    eng, model, product, market, pricing, execution = setup(n_paths=n_paths, n_steps=n_steps)
+    z = oracle.philox_normals(7, n_paths, n_steps)
+    port = dict(set_id=[0], notional=[1e6], fixed_rate=[0.021], start=[0.0], flow_off=[0, 5], pay=PAY,
+                acc=[1.0] * 5)
+    times = [0.0, 1.0, 2.0, 3.0, 4.0]
+    base = dict(a=A, b=B, sigma=SIG, r0=R0)
+    for name in ("a", "b", "sigma", "r0"):
+        g = greek(eng, product, model, market, pricing, execution, metric="UnilateralCVA",
+                  risk_factor=f"model.{name}")
+        h = g.bump_used
+        assert h == max(1e-2 * abs(base[name]), 1e-4)
+        runs = []
+        for sgn in (1.0, -1.0):
+            m = dict(base)
+            m[name] += sgn * h
+            runs.append(oracle.portfolio_exposure((m["a"], m["b"], m["sigma"], m["r0"]), port, times, n_steps, z,
+                                                  scheme=1, hazard_rate=0.02, recovery_rate=0.4))
+        fd = (runs[0]["set_cva"][0] - runs[1]["set_cva"][0]) / (2 * h)
+        assert abs(g.scalar - fd) <= 1e-8 * abs(fd), (name, g.scalar, fd)
+        ee_fd = (runs[0]["set_ee"][0] - runs[1]["set_ee"][0]) / (2 * h)
+        pfe_fd = (runs[0]["set_pfe"][0] - runs[1]["set_pfe"][0]) / (2 * h)
+        assert np.all(np.abs(np.array(g.primary) - ee_fd) <= 1e-8 * np.abs(ee_fd).max()), name
+        assert np.all(np.abs(np.array(g.secondary) - pfe_fd) <= 1e-8 * np.abs(pfe_fd).max()), name
 
 
 def test_exposure_does_not_depend_on_the_market_curve():

@@ -9,7 +9,7 @@ import pytest
 import engine_quant_amd as engine
 import oracle
 from engine_quant_amd import eqhip
-from helpers import assert_rel, leg_scale, small_portfolio
+from helpers import assert_rel, assert_value_rel, leg_scale, small_portfolio
 
 pytestmark = pytest.mark.gpu
 
@@ -86,11 +86,11 @@ def test_portfolio_netting_shared_draws(ctx, disc):
     assert_rel(got["path_mtm"], ref["path_mtm"], 1e-12, scale=scale[None, :, None], what="per-path MtM")
     set_scale = np.array([scale[port.set_id == s].sum() for s in range(port.n_sets)])
     assert_rel(got["set_mtm"], ref["set_mtm"], 1e-12, scale=set_scale[None, :, None], what="set MtM")
-    assert_rel(got["set_ee"], ref["set_ee"], 1e-10, scale=set_scale[:, None], what="set EE")
-    assert_rel(got["set_pfe"], ref["set_pfe"], 1e-10, scale=set_scale[:, None], what="set PFE")
-    assert_rel(got["set_cva"], ref["set_cva"], 1e-10, scale=set_scale, what="set CVA")
-    assert_rel(got["trade_ee"], ref["trade_ee"], 1e-10, scale=scale[:, None], what="trade EE")
-    assert_rel(got["trade_cva"], ref["trade_cva"], 1e-10, scale=scale, what="trade CVA")
+    assert_value_rel(got["set_ee"], ref["set_ee"], leg=set_scale, what="set EE")
+    assert_value_rel(got["set_pfe"], ref["set_pfe"], leg=set_scale, what="set PFE")
+    assert_value_rel(got["set_cva"], ref["set_cva"], leg=set_scale, what="set CVA")
+    assert_value_rel(got["trade_ee"], ref["trade_ee"], leg=scale, what="trade EE")
+    assert_value_rel(got["trade_cva"], ref["trade_cva"], leg=scale, what="trade CVA")
 
 
 def test_portfolio_philox_more_fixings_than_waves(ctx):
@@ -115,9 +115,9 @@ def test_portfolio_philox_more_fixings_than_waves(ctx):
     z = oracle.philox_normals2(21, n_paths, n_steps)
     ref = oracle.portfolio_exposure(M2, port.as_dict(), dates, n_steps, z, hazard_rate=0.02, recovery_rate=0.4)
     set_scale = np.array([np.abs(port.notional[port.set_id == s]).sum() * 1.2 for s in range(5)])
-    assert_rel(got["set_ee"], ref["set_ee"], 1e-10, scale=set_scale[:, None], what="set EE")
-    assert_rel(got["set_pfe"], ref["set_pfe"], 1e-10, scale=set_scale[:, None], what="set PFE")
-    assert_rel(got["set_cva"], ref["set_cva"], 1e-10, scale=set_scale, what="set CVA")
+    assert_value_rel(got["set_ee"], ref["set_ee"], leg=set_scale, what="set EE")
+    assert_value_rel(got["set_pfe"], ref["set_pfe"], leg=set_scale, what="set PFE")
+    assert_value_rel(got["set_cva"], ref["set_cva"], leg=set_scale, what="set CVA")
 
 
 def test_g2pp_refusals(ctx):

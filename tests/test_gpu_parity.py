@@ -9,7 +9,7 @@ import pytest
 
 import oracle
 from engine_quant_amd import eqhip
-from helpers import MODEL, assert_rel, leg_scale, small_portfolio
+from helpers import MODEL, assert_rel, assert_value_rel, leg_scale, small_portfolio
 
 pytestmark = pytest.mark.gpu
 
@@ -73,11 +73,11 @@ def test_portfolio_netting_shared_draws(ctx, disc):
     assert_rel(got["path_mtm"], ref["path_mtm"], 1e-12, scale=scale[None, :, None], what="per-path MtM")
     set_scale = np.array([scale[port.set_id == s].sum() for s in range(port.n_sets)])
     assert_rel(got["set_mtm"], ref["set_mtm"], 1e-12, scale=set_scale[None, :, None], what="set MtM")
-    assert_rel(got["set_ee"], ref["set_ee"], 1e-10, scale=set_scale[:, None], what="set EE")
-    assert_rel(got["set_pfe"], ref["set_pfe"], 1e-10, scale=set_scale[:, None], what="set PFE")
-    assert_rel(got["set_cva"], ref["set_cva"], 1e-10, scale=set_scale, what="set CVA")
-    assert_rel(got["trade_ee"], ref["trade_ee"], 1e-10, scale=scale[:, None], what="trade EE")
-    assert_rel(got["trade_cva"], ref["trade_cva"], 1e-10, scale=scale, what="trade CVA")
+    assert_value_rel(got["set_ee"], ref["set_ee"], leg=set_scale, what="set EE")
+    assert_value_rel(got["set_pfe"], ref["set_pfe"], leg=set_scale, what="set PFE")
+    assert_value_rel(got["set_cva"], ref["set_cva"], leg=set_scale, what="set CVA")
+    assert_value_rel(got["trade_ee"], ref["trade_ee"], leg=scale, what="trade EE")
+    assert_value_rel(got["trade_cva"], ref["trade_cva"], leg=scale, what="trade CVA")
 
 
 def test_pfe_is_an_exact_order_statistic(ctx):
@@ -142,9 +142,9 @@ def test_many_tiles_per_unit_shared_draws(ctx):
                                     recovery_rate=0.4, want_trade=True)
     scale = leg_scale(port, MODEL, dates)
     set_scale = np.array([scale[port.set_id == s].sum() for s in range(port.n_sets)])
-    assert_rel(got["set_ee"], ref["set_ee"], 1e-10, scale=set_scale[:, None], what="set EE")
-    assert_rel(got["set_pfe"], ref["set_pfe"], 1e-10, scale=set_scale[:, None], what="set PFE")
-    assert_rel(got["trade_ee"], ref["trade_ee"], 1e-10, scale=scale[:, None], what="trade EE")
+    assert_value_rel(got["set_ee"], ref["set_ee"], leg=set_scale, what="set EE")
+    assert_value_rel(got["set_pfe"], ref["set_pfe"], leg=set_scale, what="set PFE")
+    assert_value_rel(got["trade_ee"], ref["trade_ee"], leg=scale, what="trade EE")
 
 
 def staggered_annual_portfolio(n_sets=12, seed=6):
@@ -187,7 +187,7 @@ def test_many_sets_and_more_fixings_than_waves(ctx):
     set_scale = np.array([scale[port.set_id == s].sum() for s in range(port.n_sets)])
     assert_rel(got["path_mtm"], ref["path_mtm"], 1e-12, scale=scale[None, :, None], what="per-path MtM")
     assert_rel(got["set_mtm"], ref["set_mtm"], 1e-12, scale=set_scale[None, :, None], what="set MtM")
-    assert_rel(got["set_ee"], ref["set_ee"], 1e-10, scale=set_scale[:, None], what="set EE")
-    assert_rel(got["set_pfe"], ref["set_pfe"], 1e-10, scale=set_scale[:, None], what="set PFE")
-    assert_rel(got["set_cva"], ref["set_cva"], 1e-10, scale=set_scale, what="set CVA")
-    assert_rel(got["trade_ee"], ref["trade_ee"], 1e-10, scale=scale[:, None], what="trade EE")
+    assert_value_rel(got["set_ee"], ref["set_ee"], leg=set_scale, what="set EE")
+    assert_value_rel(got["set_pfe"], ref["set_pfe"], leg=set_scale, what="set PFE")
+    assert_value_rel(got["set_cva"], ref["set_cva"], leg=set_scale, what="set CVA")
+    assert_value_rel(got["trade_ee"], ref["trade_ee"], leg=scale, what="trade EE")
