@@ -2162,9 +2162,22 @@ int eqhip_plan_xva(eqhip_plan* plan, int32_t set, const eqhip_xva_params* params
 
 int eqhip_pv_dv01(eqhip_context* ctx, const double* pillars, const double* zero_rates, int32_t n_pillars,
                   const eqhip_portfolio* pf, double bump, double* out_pv, double* out_dv01, double* out_bucketed) {
+    return eqhip_pv_dv01_at(ctx, pillars, zero_rates, n_pillars, pf, bump, 0.0, out_pv, out_dv01, out_bucketed);
+}
+
+int eqhip_pv_dv01_at(eqhip_context* ctx, const double* pillars, const double* zero_rates, int32_t n_pillars,
+                     const eqhip_portfolio* pf, double bump, double valuation_time, double* out_pv, double* out_dv01,
+                     double* out_bucketed) {
     return guarded([&] {
         check_ctx(ctx);
         validate_portfolio(pf);
+        if (!std::isfinite(valuation_time)) raise(EQHIP_E_INVALID, "valuation_time must be finite");
+        for (int j = 0; j < pf->n_trades; ++j)
+            if (valuation_time > pf->start[j])  // measure.cpp:166-174
+                raise(EQHIP_E_INVALID, "npv_from_market: valuation_time (" + std::to_string(valuation_time) +
+                                           ") posterior al start (" + std::to_string(pf->start[j]) +
+                                           ") del swap -- el primer reset de la pata flotante ya habria ocurrido y "
+                                           "este motor no modela fixings historicos de esa pata");
         if (n_pillars < 1 || !pillars || !zero_rates) raise(EQHIP_E_INVALID, "Curve: necesita al menos un pillar");
         for (int i = 1; i < n_pillars; ++i)
             if (!(pillars[i] > pillars[i - 1])) raise(EQHIP_E_INVALID, "Curve: pillars debe ser estrictamente creciente");
@@ -2212,6 +2225,7 @@ int eqhip_pv_dv01(eqhip_context* ctx, const double* pillars, const double* zero_
         g.fixed = (const double*)(d + o_k);
         g.start = (const double*)(d + o_s);
         g.bump = bump;
+        g.valuation_time = valuation_time;
         g.pv = (double*)(d + o_pv);
         g.dv01 = (double*)(d + o_dv);
         g.bucketed = out_bucketed ? (double*)(d + o_bk) : nullptr;

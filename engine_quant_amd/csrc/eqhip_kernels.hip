@@ -1330,11 +1330,13 @@ __device__ double curve_df(const double* pillars, const double* zeros, int np, i
     return exp(-z * t);
 }
 
+// measure.cpp:146-184 at valuation time v: each discount factor over DF(v) (DF(0) = 1 exactly, so v = 0
+// is the t = 0 form bit for bit); the caller has refused v > start.
 __device__ double npv_from_curve(const double* pillars, const double* zeros, int np,
                                  int bump_pillar, double bump, double notional, double fixed_rate,
-                                 double start, const double* pay, const double* acc, int m) {
+                                 double start, const double* pay, const double* acc, int m, double valuation_time) {
 #pragma clang fp contract(off)
-    double dv = curve_df(pillars, zeros, np, bump_pillar, bump, 0.0);
+    double dv = curve_df(pillars, zeros, np, bump_pillar, bump, valuation_time);
     double p_start = curve_df(pillars, zeros, np, bump_pillar, bump, start) / dv;
     double p_end = curve_df(pillars, zeros, np, bump_pillar, bump, pay[m - 1]) / dv;
     double floating_leg = notional * (p_start - p_end);
@@ -1365,15 +1367,17 @@ __global__ void pv_dv01_kernel(PvArgs g) {
     int f0 = g.flow_off[j], m = g.flow_off[j + 1] - f0;
     const double* pay = g.pay + f0;
     const double* acc = g.acc + f0;
+    // PV and the parallel DV01 at the valuation time; the bucketed ladder at t = 0 (measure.cpp:211-222)
+    const double vt = (v < 2) ? g.valuation_time : 0.0;
     double base = npv_from_curve(pillars, zeros, g.np, -1, 0.0, g.notional[j], g.fixed[j],
-                                 g.start[j], pay, acc, m);
+                                 g.start[j], pay, acc, m, vt);
     if (v == 0) {
         g.pv[j] = base;
         return;
     }
     int bp = (v == 1) ? -2 : v - 2;
     double bumped = npv_from_curve(pillars, zeros, g.np, bp, g.bump, g.notional[j], g.fixed[j],
-                                   g.start[j], pay, acc, m);
+                                   g.start[j], pay, acc, m, vt);
     if (v == 1) g.dv01[j] = bumped - base;
     else if (g.bucketed) g.bucketed[(uint64_t)j * g.np + (v - 2)] = bumped - base;
 }

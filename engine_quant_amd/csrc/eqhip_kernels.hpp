@@ -129,6 +129,7 @@ struct PvArgs {
     const double* fixed;
     const double* start;
     double bump;
+    double valuation_time;  // PV and parallel DV01 at this time (every DF divided by DF(v)); bucketed at 0
     double* pv;
     double* dv01;
     double* bucketed;

@@ -336,7 +336,8 @@ struct PvDv01 {
     std::vector<std::vector<double>> bucketed;
 };
 PvDv01 compute_pv_dv01_batch(const MarketSnapshot& market, const std::vector<const IrSwapProduct*>& products,
-                             double bump, bool bucketed, const ExecutionContext& execution);
+                             double bump, bool bucketed, const ExecutionContext& execution,
+                             double valuation_time = 0.0);
 
 // ---- bootstrap.hpp ----------------------------------------------------------------------------
 struct Registries {
@@ -370,7 +371,9 @@ struct GreekOrder {
     std::optional<RiskFactor> cross_factor;
 };
 
-enum class GreekMethod { Auto, BumpAndReval, Pathwise, AadReverse, LikelihoodRatioHessian, AadForwardOverForward };
+// The methods this engine serves. "auto" is bump-and-reval; the reference's other method names
+// ("aad", "likelihood_ratio_hessian", "aad_forward_over_forward") are refused by parse_greek_method.
+enum class GreekMethod { Auto, BumpAndReval, Pathwise };
 
 GreekMethod parse_greek_method(const std::string& text);
 std::string to_string(GreekMethod method);

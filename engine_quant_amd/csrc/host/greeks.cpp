@@ -1,11 +1,21 @@
-// Greeks of the registered measures (cpp/engine/src/greeks.cpp, cpp/engine/include/engine/greeks.hpp):
-// risk-factor parsing, the default bump policy, bump-and-reval stencils of first and second order and
-// the cross (4-point) stencil, the automatic sweep, and the "Greek" measure. Every evaluation runs the
-// measure on the GPU; the Philox stream is counter-based, so the bumped runs draw the same shocks
-// (common random numbers) without any extra bookkeeping. UnilateralCVA of a fixed-rate swap under
-// HullWhite1F also has a pathwise route (method "pathwise"), served by the GPU CVA adjoint.
+// Greeks of the registered measures, by finite differences over GPU evaluations, plus a pathwise route
+// for UnilateralCVA served by the GPU CVA adjoint.
+//
+// Design. A request is turned into a stencil: a list of evaluation points, each a weight and a signed
+// multiple of the step along one or two factor axes, and a divisor. An axis is a direction in the space of
+// (model, market, pricing date): a model parameter, the whole zero curve, one pillar, a credit parameter or
+// the clock. Every point is one evaluation of the metric at the shifted inputs; the Greek is the weighted
+// sum of the points' results over the divisor, element by element over the scalar and the two profiles.
+// The Philox stream is counter-based, so every shifted evaluation draws the same shocks: common random
+// numbers without bookkeeping.
+//
+// What callers rely on is kept as the reference states it (cpp/engine/src/greeks.cpp, greeks.hpp): the
+// risk-factor strings, the default steps (greeks.cpp:160-171), the stencils (central first order,
+// second order about the base, the four-point cross, one-sided theta) and the error texts.
 #include <algorithm>
+#include <cctype>
 #include <cmath>
+#include <memory>
 
 #include "engine.hpp"
 
@@ -14,401 +24,405 @@ namespace greeks {
 
 namespace {
 
-// Default bump policy, greeks.cpp:160-171: relative with an absolute floor for model parameters,
-// one basis point for curve and credit, one day for theta.
-double default_model_parameter_bump(double base_value) { return std::max(1e-2 * std::abs(base_value), 1e-4); }
-double default_curve_bump() { return 0.0001; }
-double default_time_shift_bump() { return 1.0 / 365.0; }
+// ---- the risk-factor language -----------------------------------------------------------------
+// "<scope>.<rest>" with scopes model, curve, credit and time.
 
-bool is_second_order_capable_kind(RiskFactorKind kind) {
-    return kind == RiskFactorKind::ModelParameter || kind == RiskFactorKind::CurveParallel ||
-           kind == RiskFactorKind::CurvePillar || kind == RiskFactorKind::CreditParameter;
+[[noreturn]] void bad_factor(const std::string& msg) { throw std::invalid_argument("RiskFactor: " + msg); }
+
+RiskFactor model_factor(const std::string& rest) {
+    if (rest.empty()) bad_factor("'model.' requiere un nombre de parametro, p.ej. 'model.spot'");
+    return RiskFactor{RiskFactorKind::ModelParameter, "model", rest, std::nullopt};
 }
 
-// Measures that honour PricingContext::pricing_date (greeks.cpp:1081-1089 names PV and PayoffPriceQ;
-// PayoffPriceQ is not part of this path).
-bool metric_supports_time_shift(const std::string& metric_name) { return metric_name == "PV"; }
-
-// greeks.cpp:205-259 (model parameters of HullWhite1F are their to_params keys, no aliases)
-double resolve_bump(const IModel& model, const MarketSnapshot& market, const RiskFactor& risk_factor,
-                    std::optional<double> override_h) {
-    if (risk_factor.kind == RiskFactorKind::ModelParameter) {
-        Params params = model.to_params();
-        auto it = params.find(risk_factor.name);
-        if (it == params.end() || !std::holds_alternative<double>(it->second))
-            throw std::invalid_argument("compute_greek: el modelo '" + model.type_name() +
-                                        "' no tiene el parametro de riesgo '" + to_string(risk_factor) + "'");
-        return override_h.value_or(default_model_parameter_bump(std::get<double>(it->second)));
+RiskFactor curve_factor(const std::string& rest) {
+    static const std::string pillar_tag = "pillar:";
+    if (rest == "parallel") return RiskFactor{RiskFactorKind::CurveParallel, "curve", "", std::nullopt};
+    if (rest.compare(0, pillar_tag.size(), pillar_tag) == 0) {
+        const std::string digits = rest.substr(pillar_tag.size());
+        const bool ok = !digits.empty() && digits.size() <= 9 &&
+                        std::all_of(digits.begin(), digits.end(), [](unsigned char c) { return std::isdigit(c); });
+        if (!ok)
+            bad_factor("'curve.pillar:<i>' requiere un indice entero no negativo, recibido 'curve." + rest + "'");
+        return RiskFactor{RiskFactorKind::CurvePillar, "curve", "", static_cast<std::size_t>(std::stoul(digits))};
     }
-    if (risk_factor.kind == RiskFactorKind::CurvePillar) {
-        const std::size_t pillar_index = risk_factor.pillar_index.value_or(0);
-        if (pillar_index >= market.pillars().size())
-            throw std::invalid_argument("compute_greek: 'curve.pillar:" + std::to_string(pillar_index) +
-                                        "' fuera de rango (la curva tiene " +
-                                        std::to_string(market.pillars().size()) + " pillars)");
-    }
-    if (risk_factor.kind == RiskFactorKind::CurveParallel || risk_factor.kind == RiskFactorKind::CurvePillar ||
-        risk_factor.kind == RiskFactorKind::CreditParameter)
-        return override_h.value_or(default_curve_bump());
-    throw std::invalid_argument("compute_greek: RiskFactorKind de '" + to_string(risk_factor) +
-                                "' no soportado para bump-and-reval");
+    bad_factor("'curve." + rest + "' desconocido (validos: 'curve.parallel', 'curve.pillar:<i>')");
 }
 
-// (model, market) with one risk factor moved by h (greeks.cpp:262-330); a second call on a bumped
-// state moves a second factor on top of the first (the cross stencil).
-struct BumpedState {
-    std::shared_ptr<IModel> owned_model;
-    const IModel* model;
+RiskFactor credit_factor(const std::string& rest) {
+    if (rest != "hazard_rate" && rest != "recovery_rate")
+        bad_factor("'credit." + rest + "' desconocido (validos: 'credit.hazard_rate', 'credit.recovery_rate')");
+    return RiskFactor{RiskFactorKind::CreditParameter, "credit", rest, std::nullopt};
+}
+
+RiskFactor time_factor(const std::string& rest) {
+    if (rest != "theta") bad_factor("'time." + rest + "' desconocido (valido: 'time.theta')");
+    return RiskFactor{RiskFactorKind::TimeShift, "time", "", std::nullopt};
+}
+
+struct ScopeRule {
+    const char* scope;
+    RiskFactor (*make)(const std::string&);
+};
+constexpr ScopeRule kScopes[] = {
+    {"model", model_factor}, {"curve", curve_factor}, {"credit", credit_factor}, {"time", time_factor}};
+
+// ---- axes -------------------------------------------------------------------------------------
+// The inputs one evaluation sees. A shifted model is owned by the point that built it.
+struct Inputs {
+    std::shared_ptr<const IModel> owned;
+    const IModel* model = nullptr;
     MarketSnapshot market;
-    BumpedState(const IModel& base_model, MarketSnapshot base_market)
-        : model(&base_model), market(std::move(base_market)) {}
+    PricingContext pricing;
 };
 
-BumpedState bump_state(const Registries& registries, const BumpedState& base, const RiskFactor& risk_factor,
-                       double h) {
-    BumpedState state = base;
-    switch (risk_factor.kind) {
-        case RiskFactorKind::ModelParameter: {
-            Params params = base.model->to_params();
-            auto it = params.find(risk_factor.name);
-            if (it == params.end() || !std::holds_alternative<double>(it->second))
-                throw std::invalid_argument("compute_greek: el modelo '" + base.model->type_name() +
-                                            "' no tiene el parametro de riesgo '" + to_string(risk_factor) + "'");
-            params[risk_factor.name] = std::get<double>(it->second) + h;
-            state.owned_model = registries.models.create(base.model->type_name(), params);
-            state.model = state.owned_model.get();
-            break;
-        }
-        case RiskFactorKind::CurveParallel:
-            state.market = bump_market_parallel(base.market, h);
-            break;
-        case RiskFactorKind::CurvePillar:
-            state.market = bump_market_pillar(base.market, *risk_factor.pillar_index, h);
-            break;
-        case RiskFactorKind::CreditParameter:
-            state.market = bump_market_credit(base.market, risk_factor.name, h);
-            break;
-        default:
-            throw std::invalid_argument("compute_greek: RiskFactorKind de '" + to_string(risk_factor) +
-                                        "' no soportado para bump-and-reval");
+std::string factor_error(const RiskFactor& rf) {
+    return "compute_greek: RiskFactorKind de '" + to_string(rf) + "' no soportado para bump-and-reval";
+}
+
+// A direction in input space: its default step at the base inputs, and how it moves a set of inputs.
+class Axis {
+public:
+    virtual ~Axis() = default;
+    virtual double default_step() const = 0;
+    virtual void move(Inputs& in, double delta) const = 0;
+};
+
+class ModelParameterAxis final : public Axis {
+public:
+    ModelParameterAxis(const Registries& registries, const IModel& base, const RiskFactor& rf)
+        : registries_(registries), key_(rf.name) {
+        base_value_ = read(base, rf);
     }
-    return state;
+    double default_step() const override { return std::max(1e-2 * std::abs(base_value_), 1e-4); }
+    void move(Inputs& in, double delta) const override {
+        Params p = in.model->to_params();
+        p[key_] = std::get<double>(p.at(key_)) + delta;
+        in.owned = registries_.models.create(in.model->type_name(), p);
+        in.model = in.owned.get();
+    }
+
+private:
+    static double read(const IModel& model, const RiskFactor& rf) {
+        const Params p = model.to_params();
+        const auto it = p.find(rf.name);
+        if (it == p.end() || !std::holds_alternative<double>(it->second))
+            throw std::invalid_argument("compute_greek: el modelo '" + model.type_name() +
+                                        "' no tiene el parametro de riesgo '" + to_string(rf) + "'");
+        return std::get<double>(it->second);
+    }
+    const Registries& registries_;
+    std::string key_;
+    double base_value_ = 0.0;
+};
+
+class CurveAxis final : public Axis {
+public:
+    explicit CurveAxis(std::optional<std::size_t> pillar) : pillar_(pillar) {}
+    double default_step() const override { return 1e-4; }
+    void move(Inputs& in, double delta) const override {
+        in.market = pillar_ ? bump_market_pillar(in.market, *pillar_, delta) : bump_market_parallel(in.market, delta);
+    }
+
+private:
+    std::optional<std::size_t> pillar_;
+};
+
+class CreditAxis final : public Axis {
+public:
+    explicit CreditAxis(std::string name) : name_(std::move(name)) {}
+    double default_step() const override { return 1e-4; }
+    void move(Inputs& in, double delta) const override { in.market = bump_market_credit(in.market, name_, delta); }
+
+private:
+    std::string name_;
+};
+
+// The clock: the pricing date moves, the curve observed today stays (one day by default).
+class ClockAxis final : public Axis {
+public:
+    double default_step() const override { return 1.0 / 365.0; }
+    void move(Inputs& in, double delta) const override {
+        const PricingContext& p = in.pricing;
+        in.pricing = PricingContext(Params{{"pricing_date", p.pricing_date() + delta},
+                                           {"n_paths", static_cast<double>(p.n_paths())},
+                                           {"n_steps", static_cast<double>(p.n_steps())},
+                                           {"seed", static_cast<double>(p.seed())}});
+    }
+};
+
+std::unique_ptr<Axis> axis_for(const Registries& registries, const RiskFactor& rf, const IModel& model,
+                               const MarketSnapshot& market) {
+    switch (rf.kind) {
+        case RiskFactorKind::ModelParameter:
+            return std::make_unique<ModelParameterAxis>(registries, model, rf);
+        case RiskFactorKind::CurveParallel:
+            return std::make_unique<CurveAxis>(std::nullopt);
+        case RiskFactorKind::CurvePillar: {
+            const std::size_t i = rf.pillar_index.value_or(0);
+            if (i >= market.pillars().size())
+                throw std::invalid_argument("compute_greek: 'curve.pillar:" + std::to_string(i) +
+                                            "' fuera de rango (la curva tiene " +
+                                            std::to_string(market.pillars().size()) + " pillars)");
+            return std::make_unique<CurveAxis>(i);
+        }
+        case RiskFactorKind::CreditParameter:
+            return std::make_unique<CreditAxis>(rf.name);
+        case RiskFactorKind::TimeShift:
+            return std::make_unique<ClockAxis>();
+    }
+    throw std::invalid_argument(factor_error(rf));
 }
 
-void require_same_shape(const MeasureResult& a, const MeasureResult& b, const std::string& metric_name) {
-    if (a.has_scalar != b.has_scalar || a.times.size() != b.times.size() || a.primary.size() != b.primary.size() ||
-        a.secondary.size() != b.secondary.size())
+// ---- stencils ---------------------------------------------------------------------------------
+struct Point {
+    double weight;
+    int m1, m2;  // signed multiples of the first and second axis step
+};
+
+struct Stencil {
+    std::vector<Point> points;
+    double divisor;
+};
+
+Stencil first_order(double h) { return {{{1.0, 1, 0}, {-1.0, -1, 0}}, 2.0 * h}; }
+Stencil second_order(double h) { return {{{1.0, 1, 0}, {-2.0, 0, 0}, {1.0, -1, 0}}, h * h}; }
+Stencil cross_order(double h1, double h2) {
+    return {{{1.0, 1, 1}, {-1.0, 1, -1}, {-1.0, -1, 1}, {1.0, -1, -1}}, 4.0 * h1 * h2};
+}
+Stencil one_sided(double) { return {{{1.0, 1, 0}, {-1.0, 0, 0}}, 1.0}; }  // M(x + h) - M(x), not divided
+
+bool same_shape(const MeasureResult& a, const MeasureResult& b) {
+    return a.has_scalar == b.has_scalar && a.times.size() == b.times.size() &&
+           a.primary.size() == b.primary.size() && a.secondary.size() == b.secondary.size();
+}
+
+// Evaluates the metric at every point of the stencil and combines the results.
+GreekResult apply_stencil(const IMeasure& metric, const std::string& metric_name, const Stencil& stencil,
+                          const Axis& a1, double h1, const Axis* a2, double h2, const IModel& model,
+                          const IProduct& product, const MarketSnapshot& market, const PricingContext& pricing,
+                          const ExecutionContext& execution) {
+    std::vector<MeasureResult> at;
+    at.reserve(stencil.points.size());
+    for (const Point& pt : stencil.points) {
+        Inputs in{nullptr, &model, market, pricing};
+        if (pt.m1 != 0) a1.move(in, pt.m1 * h1);
+        if (pt.m2 != 0) a2->move(in, pt.m2 * h2);
+        at.push_back(metric.evaluate(*in.model, product, in.market, in.pricing, execution));
+        if (!same_shape(at.front(), at.back()))
+            throw std::invalid_argument("compute_greek: la medida '" + metric_name +
+                                        "' devolvio formas incompatibles entre evaluaciones bumpeadas");
+    }
+    const MeasureResult& shape = at.front();
+    if (!shape.has_scalar && shape.times.empty() && shape.primary.empty() && shape.secondary.empty())
         throw std::invalid_argument("compute_greek: la medida '" + metric_name +
-                                    "' devolvio formas incompatibles entre evaluaciones bumpeadas");
+                                    "' no produce ningun resultado (ni escalar ni perfil)");
+    auto combine = [&](auto field) {
+        double acc = 0.0;
+        for (std::size_t p = 0; p < at.size(); ++p) acc += stencil.points[p].weight * field(at[p]);
+        return acc / stencil.divisor;
+    };
+    GreekResult out;
+    out.has_scalar = shape.has_scalar;
+    if (shape.has_scalar) out.value = combine([](const MeasureResult& r) { return r.scalar; });
+    out.times = shape.times;
+    for (std::size_t i = 0; i < shape.primary.size(); ++i)
+        out.primary.push_back(combine([i](const MeasureResult& r) { return r.primary[i]; }));
+    for (std::size_t i = 0; i < shape.secondary.size(); ++i)
+        out.secondary.push_back(combine([i](const MeasureResult& r) { return r.secondary[i]; }));
+    out.method_used = GreekMethod::BumpAndReval;
+    out.bump_used = h1;
+    return out;
 }
 
-// The pathwise route of this engine: UnilateralCVA x HullWhite1F x {a, b, sigma, r0}, first order.
-std::optional<GreekResult> try_pathwise(const GreekRequest& request, const IModel& model, const IProduct& product,
-                                        const MarketSnapshot& market, const PricingContext& pricing,
-                                        const ExecutionContext& execution) {
-    if (request.metric_name != "UnilateralCVA" || model.type_name() != "HullWhite1F" ||
-        product.type_name() != "IRSwap" || execution.backend() != "gpu")
-        return std::nullopt;
-    static const char* names[4] = {"a", "b", "sigma", "r0"};
-    int idx = -1;
-    for (int i = 0; i < 4; ++i)
-        if (request.risk_factor.name == names[i]) idx = i;
-    if (idx < 0) return std::nullopt;
-    CvaAdjoint adj = unilateral_cva_adjoint(model, product, market, pricing, execution);
-    GreekResult result;
-    result.has_scalar = true;
-    result.value = adj.grad[idx];
-    result.order = request.order;
-    result.risk_factor = request.risk_factor;
-    result.method_used = GreekMethod::Pathwise;
-    return result;
+// ---- the pathwise route: UnilateralCVA of a swap under HullWhite1F in a, b, sigma, r0 -----------
+std::optional<GreekResult> pathwise(const GreekRequest& request, const IModel& model, const IProduct& product,
+                                    const MarketSnapshot& market, const PricingContext& pricing,
+                                    const ExecutionContext& execution) {
+    static const std::string adjoint_order[4] = {"a", "b", "sigma", "r0"};
+    const bool served = request.metric_name == "UnilateralCVA" && model.type_name() == "HullWhite1F" &&
+                        product.type_name() == "IRSwap" && execution.backend() == "gpu" &&
+                        request.order.order == 1 && !request.order.cross_factor;
+    if (!served) return std::nullopt;
+    const auto at = std::find(std::begin(adjoint_order), std::end(adjoint_order), request.risk_factor.name);
+    if (at == std::end(adjoint_order)) return std::nullopt;
+    const CvaAdjoint adj = unilateral_cva_adjoint(model, product, market, pricing, execution);
+    GreekResult out;
+    out.has_scalar = true;
+    out.value = adj.grad[at - std::begin(adjoint_order)];
+    out.method_used = GreekMethod::Pathwise;
+    return out;
 }
 
-GreekResult difference(const MeasureResult& up, const MeasureResult& down, const MeasureResult* base, double h,
-                       const GreekRequest& request) {
-    const bool gamma = base != nullptr;
-    const double den = gamma ? h * h : 2.0 * h;
-    auto d = [&](double u, double dn, double b) { return gamma ? (u - 2.0 * b + dn) / den : (u - dn) / den; };
-    GreekResult result;
-    result.has_scalar = up.has_scalar;
-    if (up.has_scalar) result.value = d(up.scalar, down.scalar, gamma ? base->scalar : 0.0);
-    result.times = up.times;
-    for (std::size_t i = 0; i < up.primary.size(); ++i)
-        result.primary.push_back(d(up.primary[i], down.primary[i], gamma ? base->primary[i] : 0.0));
-    for (std::size_t i = 0; i < up.secondary.size(); ++i)
-        result.secondary.push_back(d(up.secondary[i], down.secondary[i], gamma ? base->secondary[i] : 0.0));
-    result.order = request.order;
-    result.risk_factor = request.risk_factor;
-    result.method_used = GreekMethod::BumpAndReval;
-    result.bump_used = h;
-    return result;
+// Checks of the request that do not need the inputs; the messages are the reference's.
+void validate(const GreekRequest& request) {
+    const int order = request.order.order;
+    const bool cross = request.order.cross_factor.has_value();
+    if (order != 1 && order != 2)
+        throw std::invalid_argument("compute_greek: order debe ser 1 o 2 (pedido: order=" + std::to_string(order) + ")");
+    if (order == 2 && cross)
+        throw std::invalid_argument(
+            "compute_greek: order=2 con cross_factor no soportado (tercera derivada, fuera de alcance)");
+    if (request.risk_factor.kind == RiskFactorKind::TimeShift) {
+        if (order != 1 || cross)
+            throw std::invalid_argument("compute_greek: 'time.theta' solo soporta order=1 sin cross_factor");
+        // the metrics that honour PricingContext::pricing_date (greeks.cpp:1081-1089: PV on this path)
+        if (request.metric_name != "PV")
+            throw std::invalid_argument("compute_greek: 'time.theta' todavia no esta cableado para la metrica '" +
+                                        request.metric_name +
+                                        "' (pedirlo daria un Theta silenciosamente nulo, asi que se rechaza)");
+    }
+    if (cross && request.order.cross_factor->kind == RiskFactorKind::TimeShift)
+        throw std::invalid_argument("compute_greek: cross_factor '" + to_string(*request.order.cross_factor) +
+                                    "' tiene un RiskFactorKind no soportado para derivadas cruzadas");
+    if (request.method == GreekMethod::Pathwise && request.risk_factor.kind != RiskFactorKind::ModelParameter)
+        throw std::invalid_argument("compute_greek: metodo 'pathwise' solo soporta RiskFactorKind::ModelParameter "
+                                    "(pedido: '" + to_string(request.risk_factor) + "')");
 }
 
 }  // namespace
 
-// greeks.cpp:954-1010
 RiskFactor parse_risk_factor(const std::string& text) {
-    auto dot = text.find('.');
+    const std::size_t dot = text.find('.');
     if (dot == std::string::npos)
-        throw std::invalid_argument("RiskFactor: '" + text +
-                                    "' no tiene el formato '<scope>.<nombre>' esperado "
-                                    "(scopes validos: model, curve, credit, time)");
-    const std::string scope = text.substr(0, dot), rest = text.substr(dot + 1);
-    if (scope == "model") {
-        if (rest.empty())
-            throw std::invalid_argument("RiskFactor: 'model.' requiere un nombre de parametro, p.ej. 'model.spot'");
-        return RiskFactor{RiskFactorKind::ModelParameter, "model", rest, std::nullopt};
-    }
-    if (scope == "curve") {
-        if (rest == "parallel") return RiskFactor{RiskFactorKind::CurveParallel, "curve", "", std::nullopt};
-        const std::string prefix = "pillar:";
-        if (rest.rfind(prefix, 0) == 0) {
-            const std::string idx_str = rest.substr(prefix.size());
-            try {
-                std::size_t consumed = 0;
-                long idx = std::stol(idx_str, &consumed);
-                if (idx < 0 || consumed != idx_str.size()) throw std::invalid_argument("");
-                return RiskFactor{RiskFactorKind::CurvePillar, "curve", "", static_cast<std::size_t>(idx)};
-            } catch (...) {
-                throw std::invalid_argument(
-                    "RiskFactor: 'curve.pillar:<i>' requiere un indice entero no negativo, recibido 'curve." + rest +
-                    "'");
-            }
-        }
-        throw std::invalid_argument("RiskFactor: 'curve." + rest +
-                                    "' desconocido (validos: 'curve.parallel', 'curve.pillar:<i>')");
-    }
-    if (scope == "credit") {
-        if (rest == "hazard_rate" || rest == "recovery_rate")
-            return RiskFactor{RiskFactorKind::CreditParameter, "credit", rest, std::nullopt};
-        throw std::invalid_argument("RiskFactor: 'credit." + rest +
-                                    "' desconocido (validos: 'credit.hazard_rate', 'credit.recovery_rate')");
-    }
-    if (scope == "time") {
-        if (rest == "theta") return RiskFactor{RiskFactorKind::TimeShift, "time", "", std::nullopt};
-        throw std::invalid_argument("RiskFactor: 'time." + rest + "' desconocido (valido: 'time.theta')");
-    }
-    throw std::invalid_argument("RiskFactor: scope '" + scope +
-                                "' desconocido (validos: 'model', 'curve', 'credit', 'time')");
+        bad_factor("'" + text + "' no tiene el formato '<scope>.<nombre>' esperado "
+                   "(scopes validos: model, curve, credit, time)");
+    const std::string scope = text.substr(0, dot);
+    for (const ScopeRule& rule : kScopes)
+        if (scope == rule.scope) return rule.make(text.substr(dot + 1));
+    bad_factor("scope '" + scope + "' desconocido (validos: 'model', 'curve', 'credit', 'time')");
 }
 
-std::string to_string(const RiskFactor& risk_factor) {
-    switch (risk_factor.kind) {
-        case RiskFactorKind::ModelParameter: return "model." + risk_factor.name;
-        case RiskFactorKind::CurveParallel: return "curve.parallel";
-        case RiskFactorKind::CurvePillar: return "curve.pillar:" + std::to_string(risk_factor.pillar_index.value_or(0));
-        case RiskFactorKind::CreditParameter: return "credit." + risk_factor.name;
-        case RiskFactorKind::TimeShift: return "time.theta";
+std::string to_string(const RiskFactor& rf) {
+    switch (rf.kind) {
+        case RiskFactorKind::CurvePillar:
+            return "curve.pillar:" + std::to_string(rf.pillar_index.value_or(0));
+        case RiskFactorKind::CurveParallel:
+            return "curve.parallel";
+        case RiskFactorKind::TimeShift:
+            return "time.theta";
+        case RiskFactorKind::ModelParameter:
+        case RiskFactorKind::CreditParameter:
+            return rf.scope + "." + rf.name;
     }
     throw std::logic_error("to_string(RiskFactor): RiskFactorKind desconocido");
 }
 
-// greeks.cpp:1028-1051
+namespace {
+struct MethodName {
+    const char* text;
+    GreekMethod method;
+};
+constexpr MethodName kMethods[] = {
+    {"auto", GreekMethod::Auto}, {"bump_and_reval", GreekMethod::BumpAndReval}, {"pathwise", GreekMethod::Pathwise}};
+// names of the reference's other methods (greeks.hpp): known, never served here
+constexpr const char* kUnserved[] = {"aad", "likelihood_ratio_hessian", "aad_forward_over_forward"};
+}  // namespace
+
 GreekMethod parse_greek_method(const std::string& text) {
-    if (text == "auto") return GreekMethod::Auto;
-    if (text == "bump_and_reval") return GreekMethod::BumpAndReval;
-    if (text == "pathwise") return GreekMethod::Pathwise;
-    if (text == "aad") return GreekMethod::AadReverse;
-    if (text == "likelihood_ratio_hessian") return GreekMethod::LikelihoodRatioHessian;
-    if (text == "aad_forward_over_forward") return GreekMethod::AadForwardOverForward;
-    throw std::invalid_argument("GreekMethod: '" + text +
-                                "' desconocido (validos: 'auto', 'bump_and_reval', 'pathwise', 'aad', "
-                                "'likelihood_ratio_hessian', 'aad_forward_over_forward')");
+    for (const MethodName& m : kMethods)
+        if (text == m.text) return m.method;
+    for (const char* name : kUnserved)
+        if (text == name)
+            throw std::invalid_argument("compute_greek: metodo '" + text +
+                                        "' no disponible en este motor (use 'auto', 'bump_and_reval' o 'pathwise')");
+    throw std::invalid_argument("GreekMethod: '" + text + "' desconocido (validos: 'auto', 'bump_and_reval', 'pathwise')");
 }
 
 std::string to_string(GreekMethod method) {
-    switch (method) {
-        case GreekMethod::Auto: return "auto";
-        case GreekMethod::BumpAndReval: return "bump_and_reval";
-        case GreekMethod::Pathwise: return "pathwise";
-        case GreekMethod::AadReverse: return "aad";
-        case GreekMethod::LikelihoodRatioHessian: return "likelihood_ratio_hessian";
-        case GreekMethod::AadForwardOverForward: return "aad_forward_over_forward";
-    }
+    for (const MethodName& m : kMethods)
+        if (m.method == method) return m.text;
     throw std::logic_error("to_string(GreekMethod): GreekMethod desconocido");
 }
 
-// greeks.cpp:1062-1343
 GreekResult compute_greek(const Registries& registries, const GreekRequest& request, const IModel& model,
                           const IProduct& product, const MarketSnapshot& market, const PricingContext& pricing,
                           const ExecutionContext& execution) {
-    if (request.order.order < 1 || request.order.order > 2)
-        throw std::invalid_argument("compute_greek: order debe ser 1 o 2 (pedido: order=" +
-                                    std::to_string(request.order.order) + ")");
-    if (request.order.order == 2 && request.order.cross_factor.has_value())
-        throw std::invalid_argument("compute_greek: order=2 con cross_factor no soportado (tercera derivada, fuera "
-                                    "de alcance)");
-    const bool is_time_factor = request.risk_factor.kind == RiskFactorKind::TimeShift;
-    if (is_time_factor) {
-        if (request.order.order != 1 || request.order.cross_factor.has_value())
-            throw std::invalid_argument("compute_greek: 'time.theta' solo soporta order=1 sin cross_factor");
-        if (!metric_supports_time_shift(request.metric_name))
-            throw std::invalid_argument("compute_greek: 'time.theta' todavia no esta cableado para la metrica '" +
-                                        request.metric_name +
-                                        "' (pedirlo daria un Theta silenciosamente nulo, asi que se rechaza)");
-    } else {
-        if (!is_second_order_capable_kind(request.risk_factor.kind))
-            throw std::invalid_argument("compute_greek: RiskFactorKind de '" + to_string(request.risk_factor) +
-                                        "' no soportado todavia");
-        if (request.order.cross_factor.has_value() &&
-            !is_second_order_capable_kind(request.order.cross_factor->kind))
-            throw std::invalid_argument("compute_greek: cross_factor '" + to_string(*request.order.cross_factor) +
-                                        "' tiene un RiskFactorKind no soportado para derivadas cruzadas");
-    }
-    if (request.method == GreekMethod::Pathwise || request.method == GreekMethod::AadReverse) {
-        if (request.risk_factor.kind != RiskFactorKind::ModelParameter)
-            throw std::invalid_argument("compute_greek: metodo '" + to_string(request.method) +
-                                        "' solo soporta RiskFactorKind::ModelParameter (pedido: '" +
-                                        to_string(request.risk_factor) + "')");
-    }
-    if (request.method == GreekMethod::AadReverse || request.method == GreekMethod::LikelihoodRatioHessian ||
-        request.method == GreekMethod::AadForwardOverForward)
-        throw std::invalid_argument("compute_greek: metodo '" + to_string(request.method) +
-                                    "' no disponible para la metrica '" + request.metric_name + "' del modelo '" +
-                                    model.type_name() + "' (use 'auto', 'bump_and_reval' o 'pathwise')");
+    validate(request);
+    const auto metric = registries.measures.create(request.metric_name, request.metric_params);
+    const auto a1 = axis_for(registries, request.risk_factor, model, market);
+    const double h1 = request.bump_override.value_or(a1->default_step());
+    const bool clock = request.risk_factor.kind == RiskFactorKind::TimeShift;
 
-    auto metric = registries.measures.create(request.metric_name, request.metric_params);
-
-    if (is_time_factor) {  // one-sided: Metric(t + dt) - Metric(t), not divided (greeks.cpp:1142-1171)
-        const double h = request.bump_override.value_or(default_time_shift_bump());
-        PricingContext shifted(Params{{"pricing_date", pricing.pricing_date() + h},
-                                      {"n_paths", static_cast<double>(pricing.n_paths())},
-                                      {"n_steps", static_cast<double>(pricing.n_steps())},
-                                      {"seed", static_cast<double>(pricing.seed())}});
-        MeasureResult up = metric->evaluate(model, product, market, shifted, execution);
-        MeasureResult down = metric->evaluate(model, product, market, pricing, execution);
-        require_same_shape(up, down, request.metric_name);
-        GreekResult result;
-        result.has_scalar = up.has_scalar;
-        if (up.has_scalar) result.value = up.scalar - down.scalar;
-        result.times = up.times;
-        for (std::size_t i = 0; i < up.primary.size(); ++i) result.primary.push_back(up.primary[i] - down.primary[i]);
-        for (std::size_t i = 0; i < up.secondary.size(); ++i)
-            result.secondary.push_back(up.secondary[i] - down.secondary[i]);
-        result.order = request.order;
-        result.risk_factor = request.risk_factor;
-        result.method_used = GreekMethod::BumpAndReval;
-        result.bump_used = h;
-        return result;
-    }
-
-    const double h1 = resolve_bump(model, market, request.risk_factor, request.bump_override);
-
+    GreekResult out;
     if (request.method == GreekMethod::Pathwise) {
-        if (request.order.order == 1 && !request.order.cross_factor.has_value()) {
-            std::optional<GreekResult> r = try_pathwise(request, model, product, market, pricing, execution);
-            if (r.has_value()) return *r;
-        }
-        throw std::invalid_argument(
-            "compute_greek: metodo 'pathwise' no disponible para la metrica '" + request.metric_name +
-            "' del modelo '" + model.type_name() + "' con order=" + std::to_string(request.order.order) +
-            (request.order.cross_factor.has_value() ? " y cross_factor" : "") +
-            " (este motor lo ofrece para UnilateralCVA de HullWhite1F, model.{a,b,sigma,r0}, order=1, backend gpu)");
+        std::optional<GreekResult> pw = pathwise(request, model, product, market, pricing, execution);
+        if (!pw)
+            throw std::invalid_argument(
+                "compute_greek: metodo 'pathwise' no disponible para la metrica '" + request.metric_name +
+                "' del modelo '" + model.type_name() + "' con order=" + std::to_string(request.order.order) +
+                (request.order.cross_factor ? " y cross_factor" : "") +
+                " (este motor lo ofrece para UnilateralCVA de HullWhite1F, model.{a,b,sigma,r0}, order=1, backend gpu)");
+        out = std::move(*pw);
+    } else if (request.order.cross_factor) {
+        const RiskFactor& second = *request.order.cross_factor;
+        const auto a2 = axis_for(registries, second, model, market);
+        const double h2 = a2->default_step();
+        out = apply_stencil(*metric, request.metric_name, cross_order(h1, h2), *a1, h1, a2.get(), h2, model, product,
+                            market, pricing, execution);
+        out.warnings.push_back("cross_factor '" + to_string(second) + "' bumpeado en +-" + std::to_string(h2));
+    } else {
+        const Stencil st = clock ? one_sided(h1) : (request.order.order == 2 ? second_order(h1) : first_order(h1));
+        out = apply_stencil(*metric, request.metric_name, st, *a1, h1, nullptr, 0.0, model, product, market, pricing,
+                            execution);
     }
-
-    const BumpedState base_state(model, market);
-    if (request.order.cross_factor.has_value()) {  // 4-point stencil (greeks.cpp:1236-1291)
-        const RiskFactor& cross = *request.order.cross_factor;
-        const double h2 = resolve_bump(model, market, cross, std::nullopt);
-        auto eval2 = [&](double s1, double s2) {
-            BumpedState a = bump_state(registries, base_state, request.risk_factor, s1 * h1);
-            BumpedState b = bump_state(registries, a, cross, s2 * h2);
-            return metric->evaluate(*b.model, product, b.market, pricing, execution);
-        };
-        MeasureResult uu = eval2(1, 1), ud = eval2(1, -1), du = eval2(-1, 1), dd = eval2(-1, -1);
-        require_same_shape(uu, ud, request.metric_name);
-        require_same_shape(uu, du, request.metric_name);
-        require_same_shape(uu, dd, request.metric_name);
-        const double den = 4.0 * h1 * h2;
-        GreekResult result;
-        result.has_scalar = uu.has_scalar;
-        if (uu.has_scalar) result.value = (uu.scalar - ud.scalar - du.scalar + dd.scalar) / den;
-        result.times = uu.times;
-        for (std::size_t i = 0; i < uu.primary.size(); ++i)
-            result.primary.push_back((uu.primary[i] - ud.primary[i] - du.primary[i] + dd.primary[i]) / den);
-        for (std::size_t i = 0; i < uu.secondary.size(); ++i)
-            result.secondary.push_back((uu.secondary[i] - ud.secondary[i] - du.secondary[i] + dd.secondary[i]) / den);
-        result.order = request.order;
-        result.risk_factor = request.risk_factor;
-        result.method_used = GreekMethod::BumpAndReval;
-        result.bump_used = h1;
-        result.warnings.push_back("cross_factor '" + to_string(cross) + "' bumpeado en +-" + std::to_string(h2));
-        return result;
-    }
-
-    const BumpedState s_up = bump_state(registries, base_state, request.risk_factor, h1);
-    const BumpedState s_down = bump_state(registries, base_state, request.risk_factor, -h1);
-    MeasureResult up = metric->evaluate(*s_up.model, product, s_up.market, pricing, execution);
-    MeasureResult down = metric->evaluate(*s_down.model, product, s_down.market, pricing, execution);
-    require_same_shape(up, down, request.metric_name);
-    if (!up.has_scalar && up.times.empty() && up.primary.empty() && up.secondary.empty())
-        throw std::invalid_argument("compute_greek: la medida '" + request.metric_name +
-                                    "' no produce ningun resultado (ni escalar ni perfil)");
-    if (request.order.order == 2) {
-        MeasureResult base = metric->evaluate(model, product, market, pricing, execution);
-        require_same_shape(up, base, request.metric_name);
-        return difference(up, down, &base, h1, request);
-    }
-    return difference(up, down, nullptr, h1, request);
+    out.order = request.order;
+    out.risk_factor = request.risk_factor;
+    return out;
 }
 
-// greeks.cpp:1345-1430: every double model parameter (sorted), curve.parallel (+ pillars on request),
-// both credit parameters and theta; a candidate that throws goes to `skipped` with its reason.
+// The sweep: every double parameter of the model in key order, the parallel curve (and each pillar on
+// request), both credit parameters and theta, each at first order (and second order on request for
+// the model parameters); a factor whose Greek raises is listed in `skipped` with the reason.
 GreeksReport compute_all_greeks(const Registries& registries, const std::string& metric_name,
                                 const Params& metric_params, const IModel& model, const IProduct& product,
                                 const MarketSnapshot& market, const PricingContext& pricing,
                                 const ExecutionContext& execution, bool include_curve_buckets,
                                 bool include_second_order) {
+    std::vector<std::pair<RiskFactor, int>> plan;
+    std::vector<std::string> keys;
+    for (const auto& kv : model.to_params())
+        if (std::holds_alternative<double>(kv.second)) keys.push_back(kv.first);
+    std::sort(keys.begin(), keys.end());
+    for (const std::string& key : keys) {
+        const RiskFactor rf = model_factor(key);
+        plan.emplace_back(rf, 1);
+        if (include_second_order) plan.emplace_back(rf, 2);
+    }
+    plan.emplace_back(curve_factor("parallel"), 1);
+    for (std::size_t i = 0; include_curve_buckets && i < market.pillars().size(); ++i)
+        plan.emplace_back(curve_factor("pillar:" + std::to_string(i)), 1);
+    plan.emplace_back(credit_factor("hazard_rate"), 1);
+    plan.emplace_back(credit_factor("recovery_rate"), 1);
+    plan.emplace_back(time_factor("theta"), 1);
+
     GreeksReport report;
-    auto try_request = [&](const GreekRequest& request) {
-        try {
-            report.greeks.push_back(compute_greek(registries, request, model, product, market, pricing, execution));
-        } catch (const std::exception& e) {
-            report.skipped.push_back(to_string(request.risk_factor) + " (order=" +
-                                     std::to_string(request.order.order) + "): " + e.what());
-        }
-    };
-    auto candidate = [&](const RiskFactor& rf, int order) {
+    for (const auto& [rf, order] : plan) {
         GreekRequest request;
         request.metric_name = metric_name;
         request.metric_params = metric_params;
         request.risk_factor = rf;
-        request.order = GreekOrder{order, std::nullopt};
-        request.method = GreekMethod::Auto;
-        try_request(request);
-    };
-    const Params model_params = model.to_params();
-    std::vector<std::string> keys;
-    for (const auto& kv : model_params)
-        if (std::holds_alternative<double>(kv.second)) keys.push_back(kv.first);
-    std::sort(keys.begin(), keys.end());
-    for (const std::string& key : keys) {
-        const RiskFactor rf{RiskFactorKind::ModelParameter, "model", key, std::nullopt};
-        candidate(rf, 1);
-        if (include_second_order) candidate(rf, 2);
+        request.order.order = order;
+        try {
+            report.greeks.push_back(compute_greek(registries, request, model, product, market, pricing, execution));
+        } catch (const std::exception& e) {
+            report.skipped.push_back(to_string(rf) + " (order=" + std::to_string(order) + "): " + e.what());
+        }
     }
-    candidate(RiskFactor{RiskFactorKind::CurveParallel, "curve", "", std::nullopt}, 1);
-    if (include_curve_buckets)
-        for (std::size_t i = 0; i < market.pillars().size(); ++i)
-            candidate(RiskFactor{RiskFactorKind::CurvePillar, "curve", "", i}, 1);
-    candidate(RiskFactor{RiskFactorKind::CreditParameter, "credit", "hazard_rate", std::nullopt}, 1);
-    candidate(RiskFactor{RiskFactorKind::CreditParameter, "credit", "recovery_rate", std::nullopt}, 1);
-    candidate(RiskFactor{RiskFactorKind::TimeShift, "time", "", std::nullopt}, 1);
     return report;
 }
 
 }  // namespace greeks
 
-// greeks.cpp:1670-1714
+// The "Greek" measure (greeks.hpp:304-318): its parameters are metric, risk_factor, order, cross_factor,
+// method and bump; the inner metric's own parameters arrive prefixed "metric.".
 GreekMeasure::GreekMeasure(const Params& params, const Registries& registries) : registries_(registries) {
+    static const std::string inner = "metric.";
+    for (const auto& [key, value] : params)
+        if (key.size() > inner.size() && key.compare(0, inner.size(), inner) == 0)
+            request_.metric_params.emplace(key.substr(inner.size()), value);
     request_.metric_name = get_string(params, "metric");
-    Params inner;
-    const std::string prefix = "metric.";
-    for (const auto& kv : params)
-        if (kv.first.rfind(prefix, 0) == 0) inner[kv.first.substr(prefix.size())] = kv.second;
-    request_.metric_params = std::move(inner);
     request_.risk_factor = greeks::parse_risk_factor(get_string(params, "risk_factor"));
-    request_.order = greeks::GreekOrder{static_cast<int>(get_double(params, "order", 1.0)), std::nullopt};
+    request_.order.order = static_cast<int>(get_double(params, "order", 1.0));
     if (contains(params, "cross_factor"))
         request_.order.cross_factor = greeks::parse_risk_factor(get_string(params, "cross_factor"));
     request_.method = greeks::parse_greek_method(get_string(params, "method", "auto"));
@@ -417,15 +431,16 @@ GreekMeasure::GreekMeasure(const Params& params, const Registries& registries) :
 
 MeasureResult GreekMeasure::evaluate(const IModel& model, const IProduct& product, const MarketSnapshot& market,
                                      const PricingContext& pricing, const ExecutionContext& execution) const {
-    greeks::GreekResult out = greeks::compute_greek(registries_, request_, model, product, market, pricing, execution);
-    MeasureResult result;
-    result.has_scalar = out.has_scalar;
-    result.scalar = out.value;
-    result.times = out.times;
-    result.primary = out.primary;
-    result.secondary = out.secondary;
-    result.bump_used = out.bump_used;
-    return result;
+    const greeks::GreekResult g =
+        greeks::compute_greek(registries_, request_, model, product, market, pricing, execution);
+    MeasureResult r;
+    r.has_scalar = g.has_scalar;
+    r.scalar = g.value;
+    r.times = g.times;
+    r.primary = g.primary;
+    r.secondary = g.secondary;
+    r.bump_used = g.bump_used;
+    return r;
 }
 
 }  // namespace engine

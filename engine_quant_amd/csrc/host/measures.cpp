@@ -473,7 +473,7 @@ std::vector<double> compute_cva_from_exposure_batch(const IModel& model, const E
 }
 
 PvDv01 compute_pv_dv01_batch(const MarketSnapshot& market, const std::vector<const IrSwapProduct*>& products,
-                             double bump, bool bucketed, const ExecutionContext& execution) {
+                             double bump, bool bucketed, const ExecutionContext& execution, double valuation_time) {
     require_gpu(execution);
     std::size_t n = products.size(), np = market.pillars().size();
     std::vector<int32_t> set_id(n), off(n + 1, 0);
@@ -494,8 +494,8 @@ PvDv01 compute_pv_dv01_batch(const MarketSnapshot& market, const std::vector<con
     out.pv.resize(n);
     out.dv01.resize(n);
     std::vector<double> bk(bucketed ? n * np : 0);
-    check(eqhip_pv_dv01(gpu_context(), market.pillars().data(), market.zero_rates().data(), (int32_t)np, &pf, bump,
-                        out.pv.data(), out.dv01.data(), bucketed ? bk.data() : nullptr));
+    check(eqhip_pv_dv01_at(gpu_context(), market.pillars().data(), market.zero_rates().data(), (int32_t)np, &pf, bump,
+                           valuation_time, out.pv.data(), out.dv01.data(), bucketed ? bk.data() : nullptr));
     if (bucketed)
         for (std::size_t j = 0; j < n; ++j) out.bucketed.emplace_back(bk.begin() + j * np, bk.begin() + (j + 1) * np);
     return out;
@@ -536,9 +536,8 @@ MeasureResult UnilateralCvaMeasure::evaluate(const IModel& model, const IProduct
 MeasureResult PresentValueMeasure::evaluate(const IModel&, const IProduct& product, const MarketSnapshot& market,
                                             const PricingContext& pricing, const ExecutionContext& execution) const {
     const auto& irs = require_irs(product, "PresentValueMeasure");
-    if (pricing.pricing_date() != 0.0)
-        throw std::invalid_argument("PresentValueMeasure: pricing_date != 0 is not supported by the GPU PV kernel");
-    PvDv01 v = compute_pv_dv01_batch(market, {&irs}, 0.0001, false, execution);
+    // valued at the pricing date on today's curve (measure.cpp:366-380)
+    PvDv01 v = compute_pv_dv01_batch(market, {&irs}, 0.0001, false, execution, pricing.pricing_date());
     MeasureResult r;
     r.has_scalar = true;
     r.scalar = v.pv[0];
@@ -548,9 +547,9 @@ MeasureResult PresentValueMeasure::evaluate(const IModel&, const IProduct& produ
 MeasureResult Dv01Measure::evaluate(const IModel&, const IProduct& product, const MarketSnapshot& market,
                                     const PricingContext& pricing, const ExecutionContext& execution) const {
     const auto& irs = require_irs(product, "Dv01Measure");
-    if (pricing.pricing_date() != 0.0)
-        throw std::invalid_argument("Dv01Measure: pricing_date != 0 is not supported by the GPU PV kernel");
-    PvDv01 v = compute_pv_dv01_batch(market, {&irs}, bump_, bucketed_, execution);
+    // the parallel DV01 at the pricing date, the bucketed ladder at t = 0 (measure.cpp:425-445)
+    PvDv01 v = compute_pv_dv01_batch(market, {&irs}, bump_, bucketed_, execution,
+                                     bucketed_ ? 0.0 : pricing.pricing_date());
     MeasureResult r;
     if (bucketed_) {
         r.times = market.pillars();

@@ -1,12 +1,14 @@
-// price / price_batch / price_many, cpp/engine/src/price.cpp:29-578 (aliases, per-call cache,
-// homogeneous batches, calendar grouping). On the GPU path one exposure simulation serves every
-// exposure-derived measure of a call: ExpectedExposure / PFE95 / ExposureProfile / UnilateralCVA
-// share it (the reference re-simulates for UnilateralCVA, measure.cpp:348-365; with a seeded
-// counter-based stream the shared result is identical).
+// price / price_batch / price_many (cpp/engine/src/price.cpp:301-578): the caller's list of measure
+// names becomes a list of requests, each served by one evaluation of a registered measure (requests of
+// the same type with equal parameters share it) and shown whole or as the EE / PFE95 part of an exposure
+// profile. On the GPU one exposure simulation serves every exposure-derived request of a call:
+// ExpectedExposure, PFE95, ExposureProfile and UnilateralCVA share it (the reference simulates again
+// for UnilateralCVA, measure.cpp:348-365; with a seeded counter-based stream the shared run gives the
+// same numbers). A batch is one schedule; price_many splits a portfolio into such batches.
 #include <algorithm>
 #include <map>
-#include <unordered_map>
-#include <unordered_set>
+#include <numeric>
+#include <tuple>
 
 #include "engine.hpp"
 
@@ -14,282 +16,341 @@ namespace engine {
 
 namespace {
 
-enum class Field { Scalar, Primary, Secondary };
+// ---- requests and evaluations -------------------------------------------------------------------
+enum class View { Whole, Expected, Potential };
 
-struct ResolvedMeasure {
-    std::string registered_type;
-    Field field;
+// Names that show a part of another measure's result (price.cpp:29-61: the legacy aliases of
+// ExposureProfile's primary and secondary profiles).
+struct PartName {
+    const char* name;
+    const char* type;
+    View view;
+};
+constexpr PartName kPartNames[] = {{"ExpectedExposure", "ExposureProfile", View::Expected},
+                                   {"PFE95", "ExposureProfile", View::Potential}};
+
+struct Request {
+    std::string shown_as;
+    View view = View::Whole;
+    std::size_t eval = 0;  // the evaluation that serves it
 };
 
-const std::unordered_map<std::string, ResolvedMeasure>& legacy_field_aliases() {
-    static const std::unordered_map<std::string, ResolvedMeasure> aliases = {
-        {"ExpectedExposure", {"ExposureProfile", Field::Primary}},
-        {"PFE95", {"ExposureProfile", Field::Secondary}},
-    };
-    return aliases;
-}
+struct Evaluation {
+    std::string type;
+    Params params;
+};
 
-MeasureResult extract_field(const MeasureResult& source, Field field) {
-    switch (field) {
-        case Field::Scalar:
-            return source;
-        case Field::Primary: {
-            MeasureResult out;
-            out.times = source.times;
-            out.primary = source.primary;
-            return out;
-        }
-        case Field::Secondary: {
-            MeasureResult out;
-            out.times = source.times;
-            out.primary = source.secondary;
-            return out;
-        }
+struct Plan {
+    std::vector<Request> requests;
+    std::vector<Evaluation> evals;
+};
+
+// Resolves every spec; `accepts(type)` says whether a registered type can be served here.
+template <class Accepts>
+Plan plan_requests(const std::vector<MeasureSpec>& specs, Accepts accepts, const std::string& unknown_prefix) {
+    Plan plan;
+    for (const MeasureSpec& spec : specs) {
+        std::string type = spec.name;
+        View view = View::Whole;
+        for (const PartName& part : kPartNames)
+            if (spec.name == part.name) {
+                type = part.type;
+                view = part.view;
+            }
+        if (!accepts(type)) throw std::invalid_argument(unknown_prefix + spec.name + "'");
+        std::size_t e = 0;
+        while (e < plan.evals.size() && !(plan.evals[e].type == type && plan.evals[e].params == spec.params)) ++e;
+        if (e == plan.evals.size()) plan.evals.push_back(Evaluation{type, spec.params});
+        plan.requests.push_back(Request{spec.alias.value_or(spec.name), view, e});
     }
-    throw std::logic_error("engine::price: Field desconocido");
+    return plan;
 }
 
-ResolvedMeasure resolve_measure_name(const std::string& name) {
-    const auto& aliases = legacy_field_aliases();
-    auto it = aliases.find(name);
-    if (it != aliases.end()) return it->second;
-    return ResolvedMeasure{name, Field::Scalar};
+MeasureResult shown(const MeasureResult& r, View view) {
+    if (view == View::Whole) return r;
+    MeasureResult part;
+    part.times = r.times;
+    part.primary = (view == View::Expected) ? r.primary : r.secondary;
+    return part;
 }
 
-// Deterministic serialisation of a Params bag for the per-call cache key (price.cpp:77-108).
-std::string params_cache_key(const Params& params) {
-    std::vector<std::string> parts;
-    for (const auto& kv : params) {
-        std::string part = kv.first + "=";
-        std::visit(
-            [&](const auto& v) {
-                using T = std::decay_t<decltype(v)>;
-                if constexpr (std::is_same_v<T, double>) {
-                    part += std::to_string(v);
-                } else if constexpr (std::is_same_v<T, bool>) {
-                    part += v ? "true" : "false";
-                } else if constexpr (std::is_same_v<T, std::string>) {
-                    part += v;
-                } else {
-                    for (double d : v) part += std::to_string(d) + ",";
-                }
-            },
-            kv.second);
-        parts.push_back(std::move(part));
+bool from_exposure_run(const std::string& type) { return type == "ExposureProfile" || type == "UnilateralCVA"; }
+
+// ---- batches of one schedule --------------------------------------------------------------------
+// Everything a calendar group shares: the product type and, for swaps, start, payment times, accruals.
+struct CalendarKey {
+    std::string type;
+    double start = 0.0;
+    const std::vector<double>* pay = nullptr;
+    const std::vector<double>* acc = nullptr;
+};
+
+CalendarKey calendar_of(const IProduct& p) {
+    CalendarKey k{p.type_name()};
+    if (const auto* s = dynamic_cast<const IrSwapProduct*>(&p)) {
+        k.start = s->start();
+        k.pay = &s->payment_times();
+        k.acc = &s->accruals();
     }
-    std::sort(parts.begin(), parts.end());
-    std::string key;
-    for (const auto& p : parts) key += p + "|";
-    return key;
+    return k;
 }
 
-const std::string& display_name(const MeasureSpec& spec) { return spec.alias ? *spec.alias : spec.name; }
+// A strict weak order on calendars (exact comparison of every time and accrual).
+bool calendar_less(const CalendarKey& x, const CalendarKey& y) {
+    static const std::vector<double> none;
+    const auto& xp = x.pay ? *x.pay : none;
+    const auto& yp = y.pay ? *y.pay : none;
+    const auto& xa = x.acc ? *x.acc : none;
+    const auto& ya = y.acc ? *y.acc : none;
+    return std::tie(x.type, x.start, xp, xa) < std::tie(y.type, y.start, yp, ya);
+}
 
-bool exposure_derived(const std::string& type) { return type == "ExposureProfile" || type == "UnilateralCVA"; }
-
-std::vector<const IrSwapProduct*> require_homogeneous_irs_batch(const std::vector<const IProduct*>& products) {
+// The swaps of a batch, which must share one product type, one schedule and explicit fixed rates; the
+// messages are the reference's (price.cpp:110-150).
+std::vector<const IrSwapProduct*> batch_swaps(const std::vector<const IProduct*>& products) {
     if (products.empty()) throw std::invalid_argument("engine::price_batch: el lote no puede estar vacío");
-    std::vector<const IrSwapProduct*> irs_products;
-    const std::string& type_name = products.front()->type_name();
+    const std::string& type = products.front()->type_name();
+    std::vector<const IrSwapProduct*> swaps;
+    swaps.reserve(products.size());
     for (const IProduct* p : products) {
-        if (p->type_name() != type_name)
-            throw std::invalid_argument("engine::price_batch: todos los trades del lote deben ser del mismo tipo de producto ('" +
-                                        type_name + "' vs '" + p->type_name() + "')");
-        const auto* irs = dynamic_cast<const IrSwapProduct*>(p);
-        if (!irs) throw std::invalid_argument("engine::price_batch: tipo de producto no soportado para lote: " + type_name);
-        if (irs->use_par_rate())
-            throw std::invalid_argument(
-                "engine::price_batch: los trades del lote deben traer fixed_rate explícito (use_par_rate no soportado en lote)");
-        irs_products.push_back(irs);
+        if (p->type_name() != type)
+            throw std::invalid_argument("engine::price_batch: todos los trades del lote deben ser del mismo tipo de "
+                                        "producto ('" + type + "' vs '" + p->type_name() + "')");
+        const auto* s = dynamic_cast<const IrSwapProduct*>(p);
+        if (s == nullptr)
+            throw std::invalid_argument("engine::price_batch: tipo de producto no soportado para lote: " + type);
+        if (s->use_par_rate())
+            throw std::invalid_argument("engine::price_batch: los trades del lote deben traer fixed_rate explícito "
+                                        "(use_par_rate no soportado en lote)");
+        swaps.push_back(s);
     }
-    const IrSwapProduct& first = *irs_products.front();
-    for (const auto* irs : irs_products)
-        if (irs->start() != first.start() || irs->payment_times() != first.payment_times() ||
-            irs->accruals() != first.accruals())
-            throw std::invalid_argument(
-                "engine::price_batch: todos los trades del lote deben compartir calendario (start/payment_times/accruals)");
-    return irs_products;
+    const CalendarKey first = calendar_of(*swaps.front());
+    for (const IrSwapProduct* s : swaps) {
+        const CalendarKey k = calendar_of(*s);
+        if (calendar_less(k, first) || calendar_less(first, k))
+            throw std::invalid_argument("engine::price_batch: todos los trades del lote deben compartir calendario "
+                                        "(start/payment_times/accruals)");
+    }
+    return swaps;
 }
 
-const std::unordered_set<std::string>& known_batch_registered_types() {
-    static const std::unordered_set<std::string> known = {"PV", "DV01", "UnilateralCVA", "ExposureProfile"};
-    return known;
+// The portfolio's calendar groups, each listing its trades in input order; groups are ordered by their
+// first trade (the order in which a serial loop over the portfolio meets them).
+std::vector<std::vector<std::size_t>> calendar_groups(const std::vector<const IProduct*>& products) {
+    std::vector<CalendarKey> keys;
+    keys.reserve(products.size());
+    for (const IProduct* p : products) keys.push_back(calendar_of(*p));
+    std::vector<std::size_t> by_key(products.size());
+    std::iota(by_key.begin(), by_key.end(), std::size_t{0});
+    std::stable_sort(by_key.begin(), by_key.end(),
+                     [&](std::size_t x, std::size_t y) { return calendar_less(keys[x], keys[y]); });
+    std::vector<std::vector<std::size_t>> groups;
+    for (std::size_t i = 0; i < by_key.size(); ++i) {
+        const bool new_group = i == 0 || calendar_less(keys[by_key[i - 1]], keys[by_key[i]]);
+        if (new_group) groups.emplace_back();
+        groups.back().push_back(by_key[i]);
+    }
+    std::sort(groups.begin(), groups.end(), [](const auto& x, const auto& y) { return x.front() < y.front(); });
+    return groups;
 }
 
-std::string calendar_group_key(const IrSwapProduct& irs) {
-    std::string key = std::to_string(irs.start()) + "|";
-    for (double t : irs.payment_times()) key += std::to_string(t) + ",";
-    key += "|";
-    for (double t : irs.accruals()) key += std::to_string(t) + ",";
-    return key;
+const char* const kBatchTypes[] = {"PV", "DV01", "UnilateralCVA", "ExposureProfile"};
+
+// One batch of one schedule. `profiles` (nullable) are the batch's exposure profiles when the caller
+// already has them (a shared scenario set).
+PriceBatchResult value_batch(const Registries& registries, const std::vector<const IProduct*>& products,
+                             const std::vector<MeasureSpec>& specs, const IModel& model, const MarketSnapshot& market,
+                             const PricingContext& pricing, const ExecutionContext& execution,
+                             const std::vector<ExposureProfile>* profiles) {
+    if (products.empty()) throw std::invalid_argument("engine::price_batch: el lote no puede estar vacío");
+    const Plan plan = plan_requests(
+        specs,
+        [&](const std::string& type) {
+            return registries.measures.contains(type) ||
+                   std::find(std::begin(kBatchTypes), std::end(kBatchTypes), type) != std::end(kBatchTypes);
+        },
+        "engine::price_batch: medida desconocida o no soportada en lote: '");
+    const std::vector<const IrSwapProduct*> swaps = batch_swaps(products);
+    const std::size_t n = swaps.size();
+
+    std::optional<std::vector<ExposureProfile>> run;  // one simulation for the whole batch
+    if (profiles) run = *profiles;
+    auto exposure = [&]() -> const std::vector<ExposureProfile>& {
+        if (!run) run = compute_exposure_profile_batch(model, swaps, pricing, execution);
+        return *run;
+    };
+    // PV / DV01 ladders: PV and the parallel DV01 at the pricing date, the bucketed ladder at t = 0, so a
+    // ladder is keyed by (bump, bucketed, valuation time); PV is the same in every ladder of its time
+    std::map<std::tuple<double, bool, double>, PvDv01> ladders;
+    auto ladder = [&](double bump, bool bucketed, double vt) -> const PvDv01& {
+        const auto key = std::make_tuple(bump, bucketed, vt);
+        auto it = ladders.find(key);
+        if (it == ladders.end())
+            it = ladders.emplace(key, compute_pv_dv01_batch(market, swaps, bump, bucketed, execution, vt)).first;
+        return it->second;
+    };
+    const double pd = pricing.pricing_date();
+
+    std::vector<std::vector<MeasureResult>> per_eval(plan.evals.size(), std::vector<MeasureResult>(n));
+    for (std::size_t e = 0; e < plan.evals.size(); ++e) {
+        const Evaluation& ev = plan.evals[e];
+        std::vector<MeasureResult>& out = per_eval[e];
+        if (ev.type == "ExposureProfile") {
+            const auto& p = exposure();
+            for (std::size_t j = 0; j < n; ++j) {
+                out[j].times = p[j].times;
+                out[j].primary = p[j].ee;
+                out[j].secondary = p[j].pfe_95;
+            }
+        } else if (ev.type == "UnilateralCVA") {
+            const std::vector<double> cva = compute_cva_from_exposure_batch(model, execution, exposure(),
+                                                                            market.hazard_rate(), market.recovery_rate());
+            for (std::size_t j = 0; j < n; ++j) {
+                out[j].has_scalar = true;
+                out[j].scalar = cva[j];
+            }
+        } else if (ev.type == "PV") {
+            const PvDv01* any = nullptr;
+            for (const auto& [key, l] : ladders)
+                if (std::get<2>(key) == pd) any = &l;
+            const PvDv01& l = any ? *any : ladder(0.0001, false, pd);
+            for (std::size_t j = 0; j < n; ++j) {
+                out[j].has_scalar = true;
+                out[j].scalar = l.pv[j];
+            }
+        } else if (ev.type == "DV01") {
+            const double bump = get_double(ev.params, "bump", 0.0001);
+            const bool bucketed = get_bool(ev.params, "bucketed", false);
+            const PvDv01& l = ladder(bump, bucketed, bucketed ? 0.0 : pd);
+            for (std::size_t j = 0; j < n; ++j) {
+                if (bucketed) {
+                    out[j].times = market.pillars();
+                    out[j].primary = l.bucketed[j];
+                } else {
+                    out[j].has_scalar = true;
+                    out[j].scalar = l.dv01[j];
+                }
+            }
+        } else {
+            const auto measure = registries.measures.create(ev.type, ev.params);
+            for (std::size_t j = 0; j < n; ++j) out[j] = measure->evaluate(model, *swaps[j], market, pricing, execution);
+        }
+    }
+    PriceBatchResult result(n);
+    for (std::size_t j = 0; j < n; ++j) {
+        result[j].trade_index = j;
+        for (const Request& r : plan.requests)
+            result[j].measures.push_back(PriceResultEntry{r.shown_as, shown(per_eval[r.eval][j], r.view)});
+    }
+    return result;
+}
+
+// Prices each calendar group as one batch (the groups' `batch(k)` results land in their trades' rows).
+template <class Batch>
+PriceBatchResult over_groups(const std::vector<std::vector<std::size_t>>& groups, std::size_t n_trades,
+                             const ExecutionContext& execution, const std::vector<double>& cost, Batch batch) {
+    PriceBatchResult result(n_trades);
+    auto price_group = [&](std::size_t k) {
+        PriceBatchResult g = batch(k);
+        for (std::size_t j = 0; j < groups[k].size(); ++j)
+            result[groups[k][j]] = PriceBatchResultEntry{groups[k][j], std::move(g[j].measures)};
+    };
+    if (execution.backend() != "gpu") {
+        for (std::size_t k = 0; k < groups.size(); ++k) price_group(k);
+        return result;
+    }
+    // GPU: the groups run over several streams at once, the most trade-flows first; an error is the one
+    // a serial loop over the groups would report
+    std::vector<std::size_t> heavy_first(groups.size());
+    std::iota(heavy_first.begin(), heavy_first.end(), std::size_t{0});
+    if (!cost.empty())
+        std::stable_sort(heavy_first.begin(), heavy_first.end(),
+                         [&](std::size_t x, std::size_t y) { return cost[x] > cost[y]; });
+    run_on_gpu_streams(groups.size(), price_group, heavy_first);
+    return result;
 }
 
 }  // namespace
 
 std::vector<std::string> price_measure_names(const Registries& registries) {
     std::vector<std::string> names = registries.measures.list();
-    for (const auto& kv : legacy_field_aliases()) names.push_back(kv.first);
+    for (const PartName& part : kPartNames) names.push_back(part.name);
     return names;
 }
 
-PriceResult price(const Registries& registries, const IProduct& product, const std::vector<MeasureSpec>& measures,
+PriceResult price(const Registries& registries, const IProduct& product, const std::vector<MeasureSpec>& specs,
                   const IModel& model, const MarketSnapshot& market, const PricingContext& pricing,
                   const ExecutionContext& execution) {
-    std::vector<ResolvedMeasure> resolved;
-    for (const MeasureSpec& spec : measures) {
-        ResolvedMeasure r = resolve_measure_name(spec.name);
-        if (!registries.measures.contains(r.registered_type))
-            throw std::invalid_argument("engine::price: medida desconocida: '" + spec.name + "'");
-        resolved.push_back(std::move(r));
+    const Plan plan = plan_requests(
+        specs, [&](const std::string& type) { return registries.measures.contains(type); },
+        "engine::price: medida desconocida: '");
+    // on the GPU, one exposure run (with the CVA when any is asked) serves every parameterless
+    // exposure-derived evaluation
+    std::optional<MeasureResult> run;
+    if (execution.backend() == "gpu") {
+        bool exposure = false, cva = false;
+        for (const Evaluation& ev : plan.evals) {
+            exposure |= from_exposure_run(ev.type);
+            cva |= ev.type == "UnilateralCVA";
+        }
+        if (exposure)
+            run = registries.measures.create(cva ? "UnilateralCVA" : "ExposureProfile", Params{})
+                      ->evaluate(model, product, market, pricing, execution);
     }
-    std::unordered_map<std::string, MeasureResult> computed;
-    // one exposure simulation shared by every exposure-derived measure of the call
-    bool want_cva = false, want_exposure = false;
-    for (const auto& r : resolved) {
-        want_cva |= r.registered_type == "UnilateralCVA";
-        want_exposure |= exposure_derived(r.registered_type);
-    }
-    std::optional<MeasureResult> shared;
-    if (want_exposure && execution.backend() == "gpu") {
-        auto m = registries.measures.create(want_cva ? "UnilateralCVA" : "ExposureProfile", Params{});
-        shared = m->evaluate(model, product, market, pricing, execution);
-    }
-    for (std::size_t i = 0; i < measures.size(); ++i) {
-        const std::string key = resolved[i].registered_type + "#" + params_cache_key(measures[i].params);
-        if (computed.count(key)) continue;
-        if (shared && exposure_derived(resolved[i].registered_type) && measures[i].params.empty()) {
-            MeasureResult r = *shared;
-            if (resolved[i].registered_type == "ExposureProfile") {
+    std::vector<MeasureResult> results;
+    results.reserve(plan.evals.size());
+    for (const Evaluation& ev : plan.evals) {
+        if (run && from_exposure_run(ev.type) && ev.params.empty()) {
+            MeasureResult r = *run;
+            if (ev.type == "ExposureProfile") {
                 r.has_scalar = false;
                 r.scalar = 0.0;
             }
-            computed.emplace(key, std::move(r));
-            continue;
+            results.push_back(std::move(r));
+        } else {
+            results.push_back(registries.measures.create(ev.type, ev.params)
+                                  ->evaluate(model, product, market, pricing, execution));
         }
-        auto measure = registries.measures.create(resolved[i].registered_type, measures[i].params);
-        computed.emplace(key, measure->evaluate(model, product, market, pricing, execution));
     }
-    PriceResult result;
-    for (std::size_t i = 0; i < measures.size(); ++i) {
-        const std::string key = resolved[i].registered_type + "#" + params_cache_key(measures[i].params);
-        result.push_back(PriceResultEntry{display_name(measures[i]), extract_field(computed.at(key), resolved[i].field)});
-    }
-    return result;
+    PriceResult out;
+    for (const Request& r : plan.requests) out.push_back(PriceResultEntry{r.shown_as, shown(results[r.eval], r.view)});
+    return out;
 }
-
-static PriceBatchResult price_batch_with_profiles(const Registries& registries,
-                                                  const std::vector<const IProduct*>& products,
-                                                  const std::vector<MeasureSpec>& measures, const IModel& model,
-                                                  const MarketSnapshot& market, const PricingContext& pricing,
-                                                  const ExecutionContext& execution,
-                                                  const std::vector<ExposureProfile>* given_profiles);
 
 PriceBatchResult price_batch(const Registries& registries, const std::vector<const IProduct*>& products,
                              const std::vector<MeasureSpec>& measures, const IModel& model,
                              const MarketSnapshot& market, const PricingContext& pricing,
                              const ExecutionContext& execution) {
-    return price_batch_with_profiles(registries, products, measures, model, market, pricing, execution, nullptr);
+    return value_batch(registries, products, measures, model, market, pricing, execution, nullptr);
 }
 
-// price_batch; profiles (nullable) are the batch's exposure profiles when the caller already has them.
-static PriceBatchResult price_batch_with_profiles(const Registries& registries,
-                                                  const std::vector<const IProduct*>& products,
-                                                  const std::vector<MeasureSpec>& measures, const IModel& model,
-                                                  const MarketSnapshot& market, const PricingContext& pricing,
-                                                  const ExecutionContext& execution,
-                                                  const std::vector<ExposureProfile>* given_profiles) {
-    if (products.empty()) throw std::invalid_argument("engine::price_batch: el lote no puede estar vacío");
-    std::vector<ResolvedMeasure> resolved;
-    for (const MeasureSpec& spec : measures) {
-        ResolvedMeasure r = resolve_measure_name(spec.name);
-        if (!known_batch_registered_types().count(r.registered_type) && !registries.measures.contains(r.registered_type))
-            throw std::invalid_argument("engine::price_batch: medida desconocida o no soportada en lote: '" + spec.name + "'");
-        resolved.push_back(std::move(r));
-    }
-    std::vector<const IrSwapProduct*> irs = require_homogeneous_irs_batch(products);
-
-    std::unordered_map<std::string, std::vector<MeasureResult>> computed;
-    std::optional<std::vector<ExposureProfile>> profiles;  // one simulation for the whole batch
-    if (given_profiles) profiles = *given_profiles;
-    auto get_profiles = [&]() -> const std::vector<ExposureProfile>& {
-        if (!profiles) profiles = compute_exposure_profile_batch(model, irs, pricing, execution);
-        return *profiles;
+PriceBatchResult price_many(const Registries& registries, const std::vector<const IProduct*>& products,
+                            const std::vector<MeasureSpec>& measures, const IModel& model,
+                            const MarketSnapshot& market, const PricingContext& pricing,
+                            const ExecutionContext& execution) {
+    if (products.empty()) throw std::invalid_argument("engine::price_many: la lista de trades no puede estar vacía");
+    const std::vector<std::vector<std::size_t>> groups = calendar_groups(products);
+    auto members = [&](std::size_t k) {
+        std::vector<const IProduct*> g;
+        for (std::size_t i : groups[k]) g.push_back(products[i]);
+        return g;
     };
-    // one PV / DV01 ladder per (bump, bucketed); PV is the same in every ladder (the unbumped curve)
-    std::map<std::pair<double, bool>, PvDv01> ladders;
-    auto get_ladder = [&](double bump, bool bucketed) -> const PvDv01& {
-        auto it = ladders.find({bump, bucketed});
-        if (it == ladders.end())
-            it = ladders.emplace(std::make_pair(bump, bucketed),
-                                 compute_pv_dv01_batch(market, irs, bump, bucketed, execution)).first;
-        return it->second;
-    };
-    for (std::size_t i = 0; i < measures.size(); ++i) {
-        const std::string& type = resolved[i].registered_type;
-        const std::string key = type + "#" + params_cache_key(measures[i].params);
-        if (computed.count(key)) continue;
-        std::vector<MeasureResult> results;
-        if (type == "ExposureProfile") {
-            for (const auto& p : get_profiles()) {
-                MeasureResult r;
-                r.times = p.times;
-                r.primary = p.ee;
-                r.secondary = p.pfe_95;
-                results.push_back(std::move(r));
-            }
-        } else if (type == "UnilateralCVA") {
-            std::vector<double> cvas = compute_cva_from_exposure_batch(model, execution, get_profiles(),
-                                                                       market.hazard_rate(), market.recovery_rate());
-            for (double c : cvas) {
-                MeasureResult r;
-                r.has_scalar = true;
-                r.scalar = c;
-                results.push_back(r);
-            }
-        } else if (type == "PV" || type == "DV01") {
-            double bump = get_double(measures[i].params, "bump", 0.0001);
-            bool bucketed = type == "DV01" && get_bool(measures[i].params, "bucketed", false);
-            const PvDv01& v = (type == "PV" && !ladders.empty()) ? ladders.begin()->second : get_ladder(bump, bucketed);
-            for (std::size_t j = 0; j < irs.size(); ++j) {
-                MeasureResult r;
-                if (type == "PV") {
-                    r.has_scalar = true;
-                    r.scalar = v.pv[j];
-                } else if (bucketed) {
-                    r.times = market.pillars();
-                    r.primary = v.bucketed[j];
-                } else {
-                    r.has_scalar = true;
-                    r.scalar = v.dv01[j];
-                }
-                results.push_back(std::move(r));
-            }
-        } else {
-            auto measure = registries.measures.create(type, measures[i].params);
-            for (const IrSwapProduct* p : irs) results.push_back(measure->evaluate(model, *p, market, pricing, execution));
-        }
-        computed.emplace(key, std::move(results));
-    }
-    PriceBatchResult result;
-    for (std::size_t j = 0; j < irs.size(); ++j) {
-        PriceResult row;
-        for (std::size_t i = 0; i < measures.size(); ++i) {
-            const std::string key = resolved[i].registered_type + "#" + params_cache_key(measures[i].params);
-            row.push_back(PriceResultEntry{display_name(measures[i]), extract_field(computed.at(key)[j], resolved[i].field)});
-        }
-        result.push_back(PriceBatchResultEntry{j, std::move(row)});
-    }
-    return result;
-}
 
-// price_many with one scenario set for every calendar group: every trade's profile on the common grid
-// and paths, then per group the CVA on the group's own dates and the measures that need no simulation,
-// in price_batch's order and aliases.
-static PriceBatchResult price_many_shared(const Registries& registries, const std::vector<const IProduct*>& products,
-                                   const std::vector<MeasureSpec>& measures, const IModel& model,
-                                   const MarketSnapshot& market, const PricingContext& pricing,
-                                   const ExecutionContext& execution) {
-    std::vector<const IrSwapProduct*> irs;
+    if (execution.simulation() != "shared") {
+        // each group simulates on its own grid (the reference's price_many)
+        std::vector<double> cost(groups.size());
+        for (std::size_t k = 0; k < groups.size(); ++k) {
+            const auto* s = dynamic_cast<const IrSwapProduct*>(products[groups[k].front()]);
+            cost[k] = (double)groups[k].size() * (s ? (double)s->payment_times().size() : 1.0);
+        }
+        return over_groups(groups, products.size(), execution, cost, [&](std::size_t k) {
+            return value_batch(registries, members(k), measures, model, market, pricing, execution, nullptr);
+        });
+    }
+
+    // one scenario set for every group: every trade's profile on the common grid and paths, then each
+    // group's measures from its trades' profiles
+    std::vector<const IrSwapProduct*> swaps;
     for (const IProduct* p : products) {
         const auto* s = dynamic_cast<const IrSwapProduct*>(p);
         if (!s)
@@ -298,84 +359,24 @@ static PriceBatchResult price_many_shared(const Registries& registries, const st
         if (s->use_par_rate())
             throw std::invalid_argument(
                 "engine::price_many: los trades deben traer fixed_rate explícito (use_par_rate no soportado en lote)");
-        irs.push_back(s);
+        swaps.push_back(s);
     }
-    bool needs_profiles = false;
+    bool needs_run = false;
     for (const MeasureSpec& spec : measures) {
-        const std::string t = resolve_measure_name(spec.name).registered_type;
-        if (t == "ExposureProfile" || t == "UnilateralCVA") needs_profiles = true;
+        std::string type = spec.name;
+        for (const PartName& part : kPartNames)
+            if (spec.name == part.name) type = part.type;
+        needs_run |= from_exposure_run(type);
     }
-    std::vector<ExposureProfile> all;
-    if (needs_profiles) all = shared_exposure_profiles(model, irs, pricing, execution);
-    // the trades' calendar groups (price_many's grouping), each priced by price_batch's measure logic
-    // on profiles taken from the shared run
-    std::unordered_map<std::string, std::vector<std::size_t>> groups;
-    std::vector<std::string> group_order;
-    for (std::size_t i = 0; i < irs.size(); ++i) {
-        auto ins = groups.try_emplace(calendar_group_key(*irs[i]));
-        if (ins.second) group_order.push_back(ins.first->first);
-        ins.first->second.push_back(i);
-    }
-    PriceBatchResult result(products.size());
-    run_on_gpu_streams(group_order.size(), [&](std::size_t k) {
-        const std::vector<std::size_t>& idx = groups.at(group_order[k]);
-        std::vector<const IProduct*> group;
-        std::vector<ExposureProfile> group_profiles;
-        for (std::size_t i : idx) {
-            group.push_back(products[i]);
-            if (needs_profiles) group_profiles.push_back(all[i]);
-        }
-        PriceBatchResult g = price_batch_with_profiles(registries, group, measures, model, market, pricing, execution,
-                                                       needs_profiles ? &group_profiles : nullptr);
-        for (std::size_t j = 0; j < idx.size(); ++j) result[idx[j]] = PriceBatchResultEntry{idx[j], std::move(g[j].measures)};
+    const std::vector<ExposureProfile> all =
+        needs_run ? shared_exposure_profiles(model, swaps, pricing, execution) : std::vector<ExposureProfile>{};
+    return over_groups(groups, products.size(), execution, {}, [&](std::size_t k) {
+        std::vector<ExposureProfile> mine;
+        if (needs_run)
+            for (std::size_t i : groups[k]) mine.push_back(all[i]);
+        return value_batch(registries, members(k), measures, model, market, pricing, execution,
+                           needs_run ? &mine : nullptr);
     });
-    return result;
-}
-
-PriceBatchResult price_many(const Registries& registries, const std::vector<const IProduct*>& products,
-                            const std::vector<MeasureSpec>& measures, const IModel& model,
-                            const MarketSnapshot& market, const PricingContext& pricing,
-                            const ExecutionContext& execution) {
-    if (products.empty()) throw std::invalid_argument("engine::price_many: la lista de trades no puede estar vacía");
-    if (execution.simulation() == "shared") return price_many_shared(registries, products, measures, model, market,
-                                                                      pricing, execution);
-    std::unordered_map<std::string, std::vector<std::size_t>> groups;
-    std::vector<std::string> group_order;
-    for (std::size_t i = 0; i < products.size(); ++i) {
-        const IProduct& p = *products[i];
-        std::string key = p.type_name();
-        if (const auto* irs = dynamic_cast<const IrSwapProduct*>(&p)) key += "#" + calendar_group_key(*irs);
-        auto ins = groups.try_emplace(key);
-        if (ins.second) group_order.push_back(key);
-        ins.first->second.push_back(i);
-    }
-    // Each calendar group is an independent price_batch. On the GPU backend the groups run over
-    // several streams at once, the ones with the most trade-dates first; each writes only its own rows
-    // of the result, and an error is reported as the serial loop would report it.
-    PriceBatchResult result(products.size());
-    auto price_group = [&](std::size_t k) {
-        const std::vector<std::size_t>& idx = groups.at(group_order[k]);
-        std::vector<const IProduct*> group;
-        for (std::size_t i : idx) group.push_back(products[i]);
-        PriceBatchResult g = price_batch(registries, group, measures, model, market, pricing, execution);
-        for (std::size_t j = 0; j < idx.size(); ++j) result[idx[j]] = PriceBatchResultEntry{idx[j], std::move(g[j].measures)};
-    };
-    if (execution.backend() != "gpu") {
-        for (std::size_t k = 0; k < group_order.size(); ++k) price_group(k);
-        return result;
-    }
-    std::vector<double> cost(group_order.size());
-    for (std::size_t k = 0; k < group_order.size(); ++k) {
-        const std::vector<std::size_t>& idx = groups.at(group_order[k]);
-        const auto* irs = dynamic_cast<const IrSwapProduct*>(products[idx.front()]);
-        cost[k] = (double)idx.size() * (irs ? (double)irs->payment_times().size() : 1.0);
-    }
-    std::vector<std::size_t> largest_first(group_order.size());
-    for (std::size_t k = 0; k < largest_first.size(); ++k) largest_first[k] = k;
-    std::stable_sort(largest_first.begin(), largest_first.end(),
-                     [&](std::size_t x, std::size_t y) { return cost[x] > cost[y]; });
-    run_on_gpu_streams(group_order.size(), price_group, largest_first);
-    return result;
 }
 
 }  // namespace engine

@@ -38,7 +38,7 @@ EXPORTED = [
     "eqhip_irs_hull_white_2f_exposure_profile", "eqhip_irs_hull_white_2f_exposure_profile_batch",
     "eqhip_unilateral_cva_from_exposure_2f", "eqhip_unilateral_cva_from_exposure_2f_batch", "eqhip_plan_create_2f",
     "eqhip_xva_calculate", "eqhip_plan_exposure_cube", "eqhip_plan_xva", "eqhip_scenarios_create",
-    "eqhip_scenarios_destroy", "eqhip_irs_hull_white_exposure_profile_batch_on",
+    "eqhip_scenarios_destroy", "eqhip_irs_hull_white_exposure_profile_batch_on", "eqhip_pv_dv01_at",
 ]
 
 PLAN_TRADE_EE, PLAN_KEEP_CUBE = 1, 2  # flags of eqhip_plan_create's want_trade_ee argument
@@ -137,6 +137,8 @@ def lib():
         L.eqhip_plan_xva.argtypes = [ctypes.c_void_p, _I32, ctypes.POINTER(XvaParamsC), _PD, _PD, _PD]
         L.eqhip_pv_dv01.argtypes = [ctypes.c_void_p, _PD, _PD, _I32, ctypes.POINTER(PortfolioC), _D, _PD,
                                     _PD, _PD]
+        L.eqhip_pv_dv01_at.argtypes = [ctypes.c_void_p, _PD, _PD, _I32, ctypes.POINTER(PortfolioC), _D, _D, _PD,
+                                       _PD, _PD]
         L.eqhip_plan_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(ModelHW1F),
                                         ctypes.POINTER(PortfolioC), _PD, _I32, ctypes.POINTER(McC),
                                         _I32, ctypes.POINTER(ctypes.c_void_p)]
@@ -485,14 +487,20 @@ class Context:
                                          _dp(df), _dp(p), _dp(ee), _dp(pfe), _dp(m)))
         return ee, pfe, m
 
-    def pv_dv01(self, pillars, zero_rates, portfolio: Portfolio, bump=1e-4, bucketed=True):
+    def pv_dv01(self, pillars, zero_rates, portfolio: Portfolio, bump=1e-4, bucketed=True, valuation_time=None):
+        """PV, parallel DV01 and (optionally) the bucketed ladder of every trade from the market curve;
+        with a valuation time, PV and DV01 are valued then (eqhip_pv_dv01_at), the ladder at t = 0."""
         pillars, zero_rates = _f64(pillars), _f64(zero_rates)
         pc = portfolio._c()
         nt = portfolio.n_trades
         pv, dv = np.zeros(nt), np.zeros(nt)
         bk = np.zeros((nt, len(pillars))) if bucketed else None
-        _check(lib().eqhip_pv_dv01(self.handle, _dp(pillars), _dp(zero_rates), len(pillars), ctypes.byref(pc),
-                                   bump, _dp(pv), _dp(dv), _dp(bk)))
+        if valuation_time is None:
+            _check(lib().eqhip_pv_dv01(self.handle, _dp(pillars), _dp(zero_rates), len(pillars), ctypes.byref(pc),
+                                       bump, _dp(pv), _dp(dv), _dp(bk)))
+        else:
+            _check(lib().eqhip_pv_dv01_at(self.handle, _dp(pillars), _dp(zero_rates), len(pillars), ctypes.byref(pc),
+                                          bump, float(valuation_time), _dp(pv), _dp(dv), _dp(bk)))
         return pv, dv, bk
 
     def pv_sensitivities(self, pillars, zero_rates, portfolio: Portfolio, model=None):

@@ -239,6 +239,17 @@ int eqhip_pv_dv01(eqhip_context* ctx, const double* pillars, const double* zero_
                   int32_t n_pillars, const eqhip_portfolio* portfolio, double bump, double* out_pv,
                   double* out_dv01, double* out_bucketed);
 
+/* eqhip_pv_dv01 with PV and the parallel DV01 valued at valuation_time v on the curve observed today:
+ * every discount factor is divided by DF(v), the valuation-time argument of npv_from_market /
+ * compute_dv01 (cpp/engine/src/measure.cpp:146-200, reached from PresentValueMeasure and Dv01Measure
+ * through PricingContext::pricing_date, measure.cpp:366-380,425-445). v = 0 is eqhip_pv_dv01 bit for
+ * bit. A trade with v > start is refused with EQHIP_E_INVALID and the reference's message (its first
+ * floating coupon would already be fixed). The bucketed ladder stays at t = 0, as the reference's
+ * dv01_bucketed_from_market does. */
+int eqhip_pv_dv01_at(eqhip_context* ctx, const double* pillars, const double* zero_rates,
+                     int32_t n_pillars, const eqhip_portfolio* portfolio, double bump, double valuation_time,
+                     double* out_pv, double* out_dv01, double* out_bucketed);
+
 /* Analytic first-order sensitivities of every trade's PV:
  * out_pillar_delta [n_trades][n_pillars] = dPV/dz_p, the derivative of npv_from_market
  * (cpp/engine/src/measure.cpp:160-184) with respect to each pillar zero rate, which the one-sided

@@ -75,6 +75,10 @@ def lib():
         L.eqr_npv_from_market.argtypes = [_PD, _PD, _I, _D, _D, _D, _PD, _PD, _I]
         L.eqr_dv01.restype = _D
         L.eqr_dv01.argtypes = [_PD, _PD, _I, _D, _D, _D, _PD, _PD, _I, _D]
+        L.eqr_npv_from_market_at.restype = _D
+        L.eqr_npv_from_market_at.argtypes = [_PD, _PD, _I, _D, _D, _D, _PD, _PD, _I, _D]
+        L.eqr_dv01_at.restype = _D
+        L.eqr_dv01_at.argtypes = [_PD, _PD, _I, _D, _D, _D, _PD, _PD, _I, _D, _D]
         L.eqr_dv01_bucketed.argtypes = [_PD, _PD, _I, _D, _D, _D, _PD, _PD, _I, _D, _PD]
         L.eqr_portfolio_exposure.restype = _I
         L.eqr_portfolio_exposure.argtypes = [
@@ -370,6 +374,19 @@ def dv01(pillars, zeros, notional, fixed_rate, start, pay, acc, bump=1e-4):
     p, z, a, c = _f64(pillars), _f64(zeros), _f64(pay), _f64(acc)
     return lib().eqr_dv01(_dp(p), _dp(z), len(p), notional, fixed_rate, start, _dp(a), _dp(c),
                           len(a), bump)
+
+
+def npv_from_market_at(pillars, zeros, notional, fixed_rate, start, pay, acc, valuation_time):
+    """npv_from_market at a valuation time (measure.cpp:146-184); NaN when it is after the start."""
+    p, z, a, c = _f64(pillars), _f64(zeros), _f64(pay), _f64(acc)
+    return lib().eqr_npv_from_market_at(_dp(p), _dp(z), len(p), notional, fixed_rate, start, _dp(a), _dp(c),
+                                        len(a), valuation_time)
+
+
+def dv01_at(pillars, zeros, notional, fixed_rate, start, pay, acc, valuation_time, bump=1e-4):
+    p, z, a, c = _f64(pillars), _f64(zeros), _f64(pay), _f64(acc)
+    return lib().eqr_dv01_at(_dp(p), _dp(z), len(p), notional, fixed_rate, start, _dp(a), _dp(c), len(a), bump,
+                             valuation_time)
 
 
 def dv01_bucketed(pillars, zeros, notional, fixed_rate, start, pay, acc, bump=1e-4):

@@ -607,11 +607,16 @@ double eqr_par_rate_from_market(const double* pillars, const double* zeros, int 
     return numerator / denominator;
 }
 
-/* valuation_time = 0: discount_at_valuation = DF(0) = 1 (measure.cpp:160-184). */
-double eqr_npv_from_market(const double* pillars, const double* zeros, int np, double notional,
-                           double fixed_rate, double start, const double* pay, const double* acc,
-                           int m) {
-    double dv = eqr_curve_df(pillars, zeros, np, 0.0);
+/* Value at valuation_time v of the swap's cash flows on the curve observed today: every discount
+ * factor DF(t) is divided by DF(v) (measure.cpp:146-184); v = 0 divides by DF(0) = 1, the t = 0 form
+ * bit for bit. A swap already started (v > start) has fixed its first floating coupon, which the
+ * two-bond floating leg cannot represent: the reference refuses it (measure.cpp:166-174), the caller
+ * checks it here (the function returns NaN). */
+double eqr_npv_from_market_at(const double* pillars, const double* zeros, int np, double notional,
+                              double fixed_rate, double start, const double* pay, const double* acc, int m,
+                              double valuation_time) {
+    if (valuation_time > start) return NAN;
+    double dv = eqr_curve_df(pillars, zeros, np, valuation_time);
     double p_start = eqr_curve_df(pillars, zeros, np, start) / dv;
     double p_end = eqr_curve_df(pillars, zeros, np, pay[m - 1]) / dv;
     double floating_leg = notional * (p_start - p_end);
@@ -621,15 +626,27 @@ double eqr_npv_from_market(const double* pillars, const double* zeros, int np, d
     return floating_leg - fixed_leg;
 }
 
+double eqr_npv_from_market(const double* pillars, const double* zeros, int np, double notional,
+                           double fixed_rate, double start, const double* pay, const double* acc,
+                           int m) {
+    return eqr_npv_from_market_at(pillars, zeros, np, notional, fixed_rate, start, pay, acc, m, 0.0);
+}
+
+/* DV01 at valuation_time: NPV(z + bump) - NPV(z), the same fixed rate in both (measure.cpp:195-200). */
+double eqr_dv01_at(const double* pillars, const double* zeros, int np, double notional, double fixed_rate,
+                   double start, const double* pay, const double* acc, int m, double bump, double valuation_time) {
+    double* bumped = (double*)malloc(sizeof(double) * np);
+    for (int i = 0; i < np; ++i) bumped[i] = zeros[i] + bump;
+    double base = eqr_npv_from_market_at(pillars, zeros, np, notional, fixed_rate, start, pay, acc, m, valuation_time);
+    double up = eqr_npv_from_market_at(pillars, bumped, np, notional, fixed_rate, start, pay, acc, m, valuation_time);
+    free(bumped);
+    return up - base;
+}
+
 double eqr_dv01(const double* pillars, const double* zeros, int np, double notional,
                 double fixed_rate, double start, const double* pay, const double* acc, int m,
                 double bump) {
-    double* bumped = (double*)malloc(sizeof(double) * np);
-    for (int i = 0; i < np; ++i) bumped[i] = zeros[i] + bump;
-    double base = eqr_npv_from_market(pillars, zeros, np, notional, fixed_rate, start, pay, acc, m);
-    double up = eqr_npv_from_market(pillars, bumped, np, notional, fixed_rate, start, pay, acc, m);
-    free(bumped);
-    return up - base;
+    return eqr_dv01_at(pillars, zeros, np, notional, fixed_rate, start, pay, acc, m, bump, 0.0);
 }
 
 void eqr_dv01_bucketed(const double* pillars, const double* zeros, int np, double notional,

@@ -82,6 +82,13 @@ double eqr_par_rate_from_market(const double* pillars, const double* zeros, int 
 double eqr_npv_from_market(const double* pillars, const double* zeros, int np, double notional,
                            double fixed_rate, double start, const double* pay, const double* acc,
                            int m);
+/* The same at valuation time v (every DF divided by DF(v)); NaN when v > start (refused by the
+ * reference, measure.cpp:166-174). */
+double eqr_npv_from_market_at(const double* pillars, const double* zeros, int np, double notional,
+                              double fixed_rate, double start, const double* pay, const double* acc, int m,
+                              double valuation_time);
+double eqr_dv01_at(const double* pillars, const double* zeros, int np, double notional, double fixed_rate,
+                   double start, const double* pay, const double* acc, int m, double bump, double valuation_time);
 /* DV01 = NPV(z+bump) - NPV(z) at the same effective fixed rate; bucketed: one pillar at a time. */
 double eqr_dv01(const double* pillars, const double* zeros, int np, double notional,
                 double fixed_rate, double start, const double* pay, const double* acc, int m,

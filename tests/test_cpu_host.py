@@ -113,6 +113,8 @@ def test_greek_measure_validates_like_the_reference():
         return eng.price(product, [("Greek", params)], model, market, pricing, gpu)
 
     for rf, msg in (("a", "formato"), ("model.", "requiere"), ("curve.pillar:-1", "entero"),
+                    ("curve.pillar:+1", "entero"), ("curve.pillar: 1", "entero"), ("curve.pillar:", "entero"),
+                    ("curve.pillar:1x", "entero"), ("curve.flat", "desconocido"),
                     ("credit.spread", "desconocido"), ("time.rho", "desconocido"), ("fx.spot", "scope")):
         with pytest.raises(ValueError, match=msg):
             price(metric="UnilateralCVA", risk_factor=rf)
@@ -132,3 +134,8 @@ def test_greek_measure_validates_like_the_reference():
         price(metric="UnilateralCVA", risk_factor="model.kappa")
     with pytest.raises(ValueError, match="fuera de rango"):
         price(metric="UnilateralCVA", risk_factor="curve.pillar:3")
+    with pytest.raises(ValueError, match="derivadas cruzadas"):
+        price(metric="UnilateralCVA", risk_factor="model.a", cross_factor="time.theta")
+    for unserved in ("likelihood_ratio_hessian", "aad_forward_over_forward"):
+        with pytest.raises(ValueError, match="no disponible en este motor"):
+            price(metric="UnilateralCVA", risk_factor="model.a", method=unserved)

@@ -104,6 +104,40 @@ def test_model_greeks_match_oracle_differences_on_the_same_paths():
         assert np.all(np.abs(np.array(g.secondary) - pfe_fd) <= 1e-8 * np.abs(pfe_fd).max()), name
 
 
+def test_pv_at_the_pricing_date_and_its_theta():
+    """PV and DV01 honour PricingContext::pricing_date (measure.cpp:146-200,366-380,425-445): every
+    discount factor over DF(pricing_date), the bucketed ladder at t = 0; a pricing date after the start is
+    refused with the reference's message; time.theta of PV is PV(t + 1/365) - PV(t), not divided."""
+    eng = engine.Engine()
+    model = eng.create_model("HullWhite1F", {"a": A, "b": B, "sigma": SIG, "r0": R0})
+    pay = [2.0, 3.0, 4.0, 5.0, 6.0]
+    product = eng.create_product("IRSwap", {"notional": 1e6, "fixed_rate": 0.021, "start": 1.0, "payment_times": pay,
+                                            "accruals": [1.0] * 5})
+    pillars, zeros = [1.0, 2.0, 5.0], [0.02, 0.022, 0.025]
+    market = engine.MarketSnapshot(pillars=pillars, zero_rates=zeros, hazard_rate=0.02, recovery_rate=0.4)
+    execution = engine.ExecutionContext({"backend": "gpu"})
+    args = (1e6, 0.021, 1.0, pay, [1.0] * 5)
+    for t in (0.0, 0.4, 1.0):
+        pricing = engine.PricingContext({"pricing_date": t, "n_paths": 100.0, "n_steps": 10.0, "seed": 7.0})
+        out = eng.price(product, ["PV", "DV01"], model, market, pricing, execution)
+        assert abs(out["PV"].scalar - oracle.npv_from_market_at(pillars, zeros, *args, t)) <= 1e-9
+        assert abs(out["DV01"].scalar - oracle.dv01_at(pillars, zeros, *args, t)) <= 1e-9
+    pricing = engine.PricingContext({"pricing_date": 0.4, "n_paths": 100.0, "n_steps": 10.0, "seed": 7.0})
+    g = greek(eng, product, model, market, pricing, execution, metric="PV", risk_factor="time.theta")
+    ref = (oracle.npv_from_market_at(pillars, zeros, *args, 0.4 + 1.0 / 365.0)
+           - oracle.npv_from_market_at(pillars, zeros, *args, 0.4))
+    assert g.bump_used == 1.0 / 365.0 and abs(g.scalar - ref) <= 1e-9
+    late = engine.PricingContext({"pricing_date": 1.5, "n_paths": 100.0, "n_steps": 10.0, "seed": 7.0})
+    with pytest.raises(ValueError, match="posterior al start"):
+        eng.price(product, ["PV"], model, market, late, execution)
+    # the bucketed ladder stays at t = 0, as the reference's does, so it is served after the start too
+    bk = eng.price(product, [("DV01", {"bucketed": True})], model, market, late, execution)["DV01"]
+    ref_bk = oracle.dv01_bucketed(pillars, zeros, *args)
+    assert np.all(np.abs(np.array(bk.primary) - ref_bk) <= 1e-9)
+    rep = eng.all_greeks(product, "PV", model, market, pricing, execution)
+    assert "time.theta" in [x.risk_factor for x in rep.greeks]
+
+
 def test_exposure_does_not_depend_on_the_market_curve():
     eng, model, product, market, pricing, execution = setup()
     for rf in ("curve.parallel", "curve.pillar:1"):

@@ -113,6 +113,28 @@ def test_pv_dv01_matches_oracle(ctx):
         assert_rel(bk[j], oracle.dv01_bucketed(p20, z20, *args), 1e-12, scale=abs(port2.notional[j]))
 
 
+def test_pv_dv01_at_a_valuation_time(ctx):
+    """eqhip_pv_dv01_at: PV and parallel DV01 at valuation time v against the oracle, v = 0 equal to
+    eqhip_pv_dv01 bit for bit, the bucketed ladder at t = 0, a trade started before v refused."""
+    port = small_portfolio(n_trades=9, n_sets=1, seed=4, start=1.0)
+    p20, z20 = np.linspace(0.25, 30, 20), np.linspace(0.015, 0.035, 20)
+    base = ctx.pv_dv01(p20, z20, port)
+    at0 = ctx.pv_dv01(p20, z20, port, valuation_time=0.0)
+    for a, b in zip(base, at0):
+        assert np.array_equal(a, b)
+    v = 0.6
+    pv, dv, bk = ctx.pv_dv01(p20, z20, port, valuation_time=v)
+    assert np.array_equal(bk, base[2])
+    for j in range(port.n_trades):
+        f0, f1 = port.flow_off[j], port.flow_off[j + 1]
+        args = (port.notional[j], port.fixed_rate[j], port.start[j], port.pay[f0:f1], port.acc[f0:f1])
+        assert_rel(pv[j], oracle.npv_from_market_at(p20, z20, *args, v), 1e-12, scale=abs(port.notional[j]))
+        assert_rel(dv[j], oracle.dv01_at(p20, z20, *args, v), 1e-12, scale=abs(port.notional[j]))
+    with pytest.raises(eqhip.EqhipError) as e:
+        ctx.pv_dv01(p20, z20, port, valuation_time=float(port.start.min()) + 0.1)
+    assert e.value.code == eqhip.E_INVALID and "posterior al start" in str(e.value)
+
+
 def test_deterministic_branch_returns_one_date(ctx):
     """exposure.rs:98-108: t_max <= 0 returns a single deterministic point."""
     t, ee, pfe = ctx.exposure_profile(A, B, SIG, R0, 1e6, 0.025, False, 0.0, PAY5, ACC5, [0.0], 10, 100, 1)

@@ -38,6 +38,27 @@ def test_bucketed_dv01_sums_to_parallel_up_to_convexity():
     assert bk[-1] > 0 and all(bk > -1e-9)
 
 
+def test_pv_at_a_valuation_time():
+    """measure.cpp:146-184: every discount factor over DF(v). v = 0 is the t = 0 form bit for bit; on a
+    flat curve DF(t)/DF(v) = exp(-z (t - v)), so the value at v of a swap starting at s equals the t = 0
+    value of the same swap moved v earlier; a swap already started (v > start) is refused (NaN)."""
+    p, z = [1.0, 2.0, 3.0, 5.0], [0.018, 0.019, 0.02, 0.021]
+    pay = [2.0, 3.0, 4.0, 5.0, 6.0]
+    assert o.npv_from_market_at(p, z, 1e6, 0.02, 1.0, pay, ACC5, 0.0) == o.npv_from_market(p, z, 1e6, 0.02, 1.0, pay, ACC5)
+    assert o.dv01_at(p, z, 1e6, 0.02, 1.0, pay, ACC5, 0.0) == o.dv01(p, z, 1e6, 0.02, 1.0, pay, ACC5)
+    flat = [0.02] * 4
+    for v in (0.25, 0.7, 1.0):
+        moved = [t - v for t in pay]
+        got = o.npv_from_market_at(p, flat, 1e6, 0.021, 1.0, pay, ACC5, v)
+        ref = o.npv_from_market(p, flat, 1e6, 0.021, 1.0 - v, moved, ACC5)
+        assert abs(got - ref) <= 1e-9 * 1e6, (v, got, ref)
+        d_got = o.dv01_at(p, flat, 1e6, 0.021, 1.0, pay, ACC5, v)
+        # the parallel bump moves DF(v) too: on a flat curve the DV01 at v is the moved swap's DV01
+        d_ref = o.dv01(p, flat, 1e6, 0.021, 1.0 - v, moved, ACC5)
+        assert abs(d_got - d_ref) <= 1e-9 * 1e6
+    assert math.isnan(o.npv_from_market_at(p, z, 1e6, 0.02, 1.0, pay, ACC5, 1.5))
+
+
 def test_curve_interpolation_and_flat_extrapolation():
     p, z = [1.0, 2.0, 4.0], [0.01, 0.02, 0.03]
     assert o.curve_zero_rate(p, z, 0.5) == 0.01
