@@ -1761,6 +1761,47 @@ size_t eqhip_last_error(char* buf, size_t len) {
     return last_error().size();
 }
 
+// ---------------------------------------------------------------------------------------------
+// Device memory of the allocation-free CVA entry point: the per-date weights (S_{k-1} - S_k, P(0,t_k))
+// of up to kCvaScratchDates dates, in a file-scope device array instead of an allocation. Every GPU the
+// library runs on holds its own copy (256 KiB) until the process ends. Host code never passes the
+// array's name as an address (it is a host-side stand-in): each device's address of it is resolved
+// once, with that device current, when the first context on the device is created, and kept here.
+// ---------------------------------------------------------------------------------------------
+constexpr int kCvaScratchDates = 16384;
+static __device__ __align__(256) double2 scratch[kCvaScratchDates];
+
+namespace {
+std::mutex g_scratch_lock;
+std::map<int, double2*> g_scratch_addr;  // device -> its copy of `scratch`
+
+double2* device_scratch(int device) {  // the caller has made `device` current
+    std::lock_guard<std::mutex> lock(g_scratch_lock);
+    const auto it = g_scratch_addr.find(device);
+    if (it != g_scratch_addr.end()) return it->second;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+        (void)hipGetLastError();
+        raise(EQHIP_E_HIP, std::string("hipGetSymbolAddress of the CVA scratch array failed: ") + hipGetErrorString(e));
+    }
+    g_scratch_addr.emplace(device, static_cast<double2*>(p));
+    return static_cast<double2*>(p);
+}
+
+// The caller's pointer is memory the device can read and write (device or managed memory, or
+// page-locked host memory mapped into the device).
+bool device_accessible(const void* p) {
+    hipPointerAttribute_t attr{};
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged ||
+           (attr.type == hipMemoryTypeHost && attr.devicePointer != nullptr);
+}
+}  // namespace
+
 int eqhip_context_create(int device, eqhip_context** out) {
     return guarded([&] {
         if (!out) raise(EQHIP_E_INVALID, "out is null");
@@ -1779,6 +1820,7 @@ int eqhip_context_create(int device, eqhip_context** out) {
         ctx->device = device;
         HIPCHK(hipSetDevice(device));
         HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        ctx->cva_scratch = device_scratch(device);
         ctx->lds_limit = prop.sharedMemPerBlock;  // 160 KB per workgroup on gfx950
         if (hipFuncSetAttribute((const void*)value_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)ctx->lds_limit) != hipSuccess ||
@@ -1986,6 +2028,32 @@ int eqhip_unilateral_cva_from_exposure_batch(eqhip_context* ctx, double a, doubl
         launch_cva_sum(st, de.as<double>(), n_profiles, n_dates, dw.as<double2>(), recovery_rate, dc.as<double>());
         HIPCHK(hipMemcpyAsync(out_cva, dc.p, (size_t)n_profiles * 8, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
+int eqhip_unilateral_cva_from_exposure_batch_noalloc(eqhip_context* ctx, double a, double b, double sigma, double r0,
+                                                     const double* d_times, const double* d_ee, int32_t n_profiles,
+                                                     int32_t n_dates, double hazard_rate, double recovery_rate,
+                                                     double* d_out_cva, void* stream) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (n_profiles < 0 || n_dates < 0) raise(EQHIP_E_INVALID, "invalid CVA arguments");
+        if (n_dates > kCvaScratchDates)
+            raise(EQHIP_E_CAPACITY, "the allocation-free CVA entry point serves at most " +
+                                        std::to_string(kCvaScratchDates) + " dates (got " + std::to_string(n_dates) +
+                                        "); use eqhip_unilateral_cva_from_exposure_batch");
+        if (n_profiles == 0) return;
+        if ((n_dates > 0 && (!device_accessible(d_times) || !device_accessible(d_ee))) || !device_accessible(d_out_cva))
+            raise(EQHIP_E_INVALID, "times, ee and out_cva must be device memory of the context's GPU");
+        int current = -1;
+        HIPCHK(hipGetDevice(&current));
+        if (current != ctx->device) HIPCHK(hipSetDevice(ctx->device));
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+        const eqhip_model_hw1f m{a, b, sigma, r0};
+        // every weight the sum reads is written first: no clearing needed
+        if (n_dates > 0) launch_cva_weights(st, d_times, n_dates, m, hazard_rate, ctx->cva_scratch);
+        launch_cva_sum(st, d_ee, n_profiles, n_dates, ctx->cva_scratch, recovery_rate, d_out_cva);
+        if (!stream) HIPCHK(hipStreamSynchronize(st));
     });
 }
 

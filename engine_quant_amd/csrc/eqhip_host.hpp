@@ -148,6 +148,7 @@ void upload(DevBuf& b, const std::vector<T>& v, hipStream_t) {
 struct eqhip_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    double2* cva_scratch = nullptr;  // this device's copy of the allocation-free CVA entry point's weights
     size_t lds_limit = 65536;
     // staging for the stateless entry points (PV / DV01 and its sensitivities): one packed upload,
     // one kernel, one packed download per call; the lock makes calls on one context take turns

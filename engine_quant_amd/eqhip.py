@@ -39,6 +39,7 @@ EXPORTED = [
     "eqhip_unilateral_cva_from_exposure_2f", "eqhip_unilateral_cva_from_exposure_2f_batch", "eqhip_plan_create_2f",
     "eqhip_xva_calculate", "eqhip_plan_exposure_cube", "eqhip_plan_xva", "eqhip_scenarios_create",
     "eqhip_scenarios_destroy", "eqhip_irs_hull_white_exposure_profile_batch_on", "eqhip_pv_dv01_at",
+    "eqhip_unilateral_cva_from_exposure_batch_noalloc",
 ]
 
 PLAN_TRADE_EE, PLAN_KEEP_CUBE = 1, 2  # flags of eqhip_plan_create's want_trade_ee argument
@@ -119,6 +120,8 @@ def lib():
         L.eqhip_irs_hull_white_exposure_profile_batch.argtypes = prof + [_PD, _PD, _I32] + tail
         L.eqhip_unilateral_cva_from_exposure.argtypes = prof + [_PD, _PD, _I32, _D, _D, _PD]
         L.eqhip_unilateral_cva_from_exposure_batch.argtypes = prof + [_PD, _PD, _I32, _I32, _D, _D, _PD]
+        L.eqhip_unilateral_cva_from_exposure_batch_noalloc.argtypes = prof + [
+            ctypes.c_void_p, ctypes.c_void_p, _I32, _I32, _D, _D, ctypes.c_void_p, ctypes.c_void_p]
         prof2 = [ctypes.c_void_p, _D, _D, _D, _D, _D, _D]
         L.eqhip_irs_hull_white_2f_exposure_profile.argtypes = prof2 + [_D, _D, ctypes.c_int] + tail
         L.eqhip_irs_hull_white_2f_exposure_profile_batch.argtypes = prof2 + [_PD, _PD, _I32] + tail
@@ -422,6 +425,16 @@ class Context:
                                                               _dp(ee), ee.shape[0], len(times), hazard_rate,
                                                               recovery_rate, _dp(out)))
         return out
+
+    def unilateral_cva_batch_device(self, a, b, sigma, r0, d_times, d_ee, n_profiles, n_dates, hazard_rate,
+                                    recovery_rate, d_out_cva, stream=None):
+        """eqhip_unilateral_cva_from_exposure_batch_noalloc on device memory (integer addresses, e.g. a
+        torch tensor's data_ptr()): no allocation; with a stream (an integer hipStream_t) the call only
+        enqueues, without one it waits for the result."""
+        _check(lib().eqhip_unilateral_cva_from_exposure_batch_noalloc(
+            self.handle, a, b, sigma, r0, ctypes.c_void_p(d_times), ctypes.c_void_p(d_ee), int(n_profiles),
+            int(n_dates), hazard_rate, recovery_rate, ctypes.c_void_p(d_out_cva),
+            ctypes.c_void_p(stream) if stream else None))
 
     def exposure_profile_2f(self, a, b, sigma, eta, rho, r0, notional, fixed_rate, use_par_rate, start, pay, acc,
                             times, n_steps, n_paths, seed, rng=RNG_PHILOX, disc=DISC_EULER, host_draws=None,

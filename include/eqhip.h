@@ -189,6 +189,19 @@ int eqhip_unilateral_cva_from_exposure_batch(eqhip_context* ctx, double a, doubl
                                              int32_t n_dates, double hazard_rate,
                                              double recovery_rate, double* out_cva);
 
+/* Allocation-free form of eqhip_unilateral_cva_from_exposure_batch. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc), whose EE profiles are already on the device. d_times [n_dates],
+ * d_ee [n_profiles][n_dates] and d_out_cva [n_profiles] are device memory of the context's GPU. The
+ * per-date weights go to a fixed device array of the library (at most 16384 dates: more is refused with
+ * EQHIP_E_CAPACITY); the call makes no allocation and no free. stream: the HIP stream (hipStream_t) to
+ * enqueue on, and then the call returns without synchronising; NULL uses the context's stream and
+ * waits for the result. Calls that overlap on one GPU share that array and race: order them. */
+int eqhip_unilateral_cva_from_exposure_batch_noalloc(eqhip_context* ctx, double a, double b, double sigma,
+                                                     double r0, const double* d_times, const double* d_ee,
+                                                     int32_t n_profiles, int32_t n_dates, double hazard_rate,
+                                                     double recovery_rate, double* d_out_cva, void* stream);
+
 /* Replaces engine::ffi::irs_hull_white_2f_exposure_profile (engine-ffi/src/lib.rs:547-565;
  * api.rs:587-665; exposure.rs:305-370): the G2++ profile of one payer IRS. use_par_rate != 0 sets
  * the fixed rate to the par rate at the initial state (0, 0). host_draws is [2][n_steps][n_paths]
