@@ -81,6 +81,35 @@ def algorithmic_work(port, dates, n_paths, n_sets, factors=1):
 PROFILE = os.path.join(ROOT, "profiles", "value_kernel_hw1f_1000swaps_100kpaths_360dates.md")
 PROFILE2 = os.path.join(ROOT, "profiles", "value2_kernel_hw2f_1000swaps_100kpaths_360dates.md")
 PROFILEX = os.path.join(ROOT, "profiles", "xva_reducer_1000swaps_100kpaths_360dates.md")
+PROFILEB = os.path.join(ROOT, "profiles", "value_kernel_batch_64sets_4Mpaths_120dates.md")
+
+
+def profile_counters(profile=PROFILE, section="## Issue counters"):
+    """The per-launch counter table of a committed rocprofv3 summary (name -> value)."""
+    out = {}
+    try:
+        with open(profile) as f:
+            inside = False
+            for line in f:
+                if line.startswith("## "):
+                    inside = line.startswith(section)
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if inside and len(cells) == 2 and cells[0].isupper():
+                    try:
+                        out[cells[0]] = float(cells[1])
+                    except ValueError:
+                        pass
+    except OSError:
+        pass
+    return out
+
+
+def counter_fp64_flops(counters):
+    """FP64 FLOPs one launch executed, from the SQ counters (wave instructions x 64 lanes; an FMA is two)."""
+    keys = ("SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64")
+    if not all(k in counters for k in keys):
+        return None
+    return 64.0 * (2.0 * counters[keys[0]] + counters[keys[1]] + counters[keys[2]])
 
 
 def measured_traffic(kernel="value_kernel", profile=PROFILE):
@@ -100,39 +129,83 @@ def measured_traffic(kernel="value_kernel", profile=PROFILE):
     return None
 
 
-def cpu_baseline(port, dates, n_steps, model, seed, disc, target_s=15.0):
-    """The oracle (CPU FP64 restatement of the reference) on bounded path samples of the same workload.
+ORACLE_FLAGS = "gcc -O2 -fPIC -std=c99 -ffp-contract=off -fopenmp (oracle/Makefile), scalar FP64, no -march"
 
-    fused: one simulation for the whole portfolio, OpenMP over paths (the conservative baseline);
-    median of five timed runs after one warm-up run.
-    reference structure: every trade simulated on its own (as per-calendar-group re-simulation does),
-    full path storage and a sort per date, one thread, on the first trades of the portfolio.
-    Returns (fused units/s, threads, paths per run, median seconds, oracle results of the fused sample,
-    reference-structure units/s and its sample description)."""
+
+def host_cpu():
+    """The host the CPU leg runs on: logical CPUs, the CPUs this process may use (affinity and cgroup
+    quota) and the model name."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0))
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()
+            if q != "max":
+                usable = min(usable, max(1, int(int(q) // int(period))))
+    except (OSError, ValueError):
+        pass
+    return {"nproc": os.cpu_count(), "usable_cpus": usable, "model": model, "compiler": ORACLE_FLAGS}
+
+
+def cpu_threads():
+    return min(16, host_cpu()["usable_cpus"])
+
+
+def _median_rate(run, units, repeats=5):
+    run()  # warm-up
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        run()
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return units / med, med
+
+
+def cpu_baseline(port, dates, n_steps, model, seed, disc, target_s=15.0):
+    """The CPU FP64 restatement timed on this host's cores on bounded samples of the same job (the same
+    portfolio, dates and Philox paths, drawn inside the run):
+
+    * fused, schedule classes shared (the baseline): one simulation, every distinct zero-coupon bond once
+      per (path, date), trades of one schedule sharing their floating leg and annuity, OpenMP over paths,
+      PFE95 by selection; median of five runs after a warm-up;
+    * fused, bonds shared only: the same without class sharing (its set values are the per-trade
+      restatement's bit for bit);
+    * reference structure: every trade simulated on its own with a sort per date, one thread (as the
+      reference's per-calendar-group re-simulation and per-trade profiles do).
+    Returns (baseline dict, oracle results of the bonds-shared sample and its path count)."""
     import oracle
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-    threads = min(threads, 16)
-    normals = oracle.philox_normals2 if len(model) == 6 else oracle.philox_normals
+    threads = cpu_threads()
     pd_ = port.as_dict()
-    # calibrate on a tiny sample, then size each of the six runs to ~target_s / 6 seconds
-    probe = 8
-    z = normals(seed, probe, n_steps)
-    t0 = time.perf_counter()
-    oracle.portfolio_exposure(model, pd_, dates, n_steps, z, scheme=disc, want_trade=True, n_threads=threads)
-    dt = time.perf_counter() - t0
-    n_sample = int(max(threads, min(4096, probe * (target_s / 6.0) / max(dt, 1e-6))))
-    n_sample = max(threads, (n_sample // threads) * threads)
-    z = normals(seed, n_sample, n_steps)
-    runs = []
-    for _ in range(6):
+    units_per_path = port.n_trades * len(dates)
+    kw = dict(scheme=disc, hazard_rate=0.02, recovery_rate=0.4, want_trade=True, n_threads=threads, seed=seed)
+    # size the six runs of the baseline to about target_s in all, from two probes (the difference leaves
+    # out the run's fixed set-up)
+    times = []
+    for probe in (32 * threads, 256 * threads):
         t0 = time.perf_counter()
-        ref = oracle.portfolio_exposure(model, pd_, dates, n_steps, z, scheme=disc, hazard_rate=0.02,
-                                        recovery_rate=0.4, want_trade=True, n_threads=threads)
-        runs.append(time.perf_counter() - t0)
-    med = float(np.median(runs[1:]))
-    units = port.n_trades * n_sample * len(dates)
-    # reference structure on the first trades: a singleton portfolio each, one thread
+        oracle.portfolio_exposure_fused(model, pd_, dates, n_steps, n_paths=probe, share_classes=True, **kw)
+        times.append(time.perf_counter() - t0)
+    per_path = max(times[1] - times[0], 1e-6) / (224 * threads)
+    n_cls = int(min(1 << 20, max(probe, (target_s / 6.0) / per_path)))
+    rate_cls, s_cls = _median_rate(lambda: oracle.portfolio_exposure_fused(
+        model, pd_, dates, n_steps, n_paths=n_cls, share_classes=True, **kw), units_per_path * n_cls)
+    n_zcb = max(threads, n_cls // 4)
+    t0 = time.perf_counter()
+    ref = oracle.portfolio_exposure_fused(model, pd_, dates, n_steps, n_paths=n_zcb, **kw)
+    s_zcb = time.perf_counter() - t0
+    rate_zcb = units_per_path * n_zcb / s_zcb
+    # reference structure on the first trades
+    normals = oracle.philox_normals
     n_ref_trades, n_ref_paths = 16, 512
     zr = normals(seed, n_ref_paths, n_steps)
     off = port.flow_off
@@ -144,11 +217,111 @@ def cpu_baseline(port, dates, n_steps, model, seed, disc, target_s=15.0):
                   "flow_off": np.array([0, f1 - f0], np.int32), "pay": port.pay[f0:f1], "acc": port.acc[f0:f1]}
         oracle.portfolio_exposure(model, single, dates, n_steps, zr, scheme=disc, hazard_rate=0.02,
                                   recovery_rate=0.4, n_threads=1)
-    ref_s = time.perf_counter() - t0
-    ref_rate = n_ref_trades * n_ref_paths * len(dates) / ref_s
-    ref_sample = (f"first {n_ref_trades} trades x {n_ref_paths} paths, each trade simulated on its own, "
-                  f"one thread ({ref_s:.1f} s)")
-    return units / med, threads, n_sample, med, ref, ref_rate, ref_sample
+    s_ref = time.perf_counter() - t0
+    baseline = {
+        "value": rate_cls, "unit": "trade-paths-dates/s", "cores": threads, "kind": "port",
+        "sample": f"same portfolio and dates, first {n_cls} Philox paths (drawn in the run), one simulation, "
+                  f"zero-coupon bonds once per (path, date), schedule classes sharing legs, OpenMP over paths, "
+                  f"PFE95 by selection; median of 5 runs after a warm-up ({s_cls:.2f} s each)",
+        "host": host_cpu(),
+        "bonds_shared_only": {"value": rate_zcb, "cores": threads,
+                              "sample": f"the same without class sharing, first {n_zcb} paths, one run "
+                                        f"({s_zcb:.2f} s)"},
+        "reference_structure": {"value": n_ref_trades * n_ref_paths * len(dates) / s_ref, "cores": 1,
+                                "sample": f"first {n_ref_trades} trades x {n_ref_paths} paths, each trade simulated "
+                                          f"on its own with a sort per date, one thread ({s_ref:.1f} s)"},
+    }
+    return baseline, ref, n_zcb
+
+
+def cpu_baseline_sets(jobs, dates, n_steps, model, seed, disc, target_s=15.0):
+    """Batched sets on the CPU: the fused restatement with schedule classes shared (one simulation per
+    set, OpenMP over paths), every set on the same fraction of its paths, sized to about target_s."""
+    import oracle
+
+    threads = cpu_threads()
+    probe_port, _ = max(jobs, key=lambda j: j[0].n_trades)
+    kw = dict(scheme=disc, hazard_rate=0.02, recovery_rate=0.4, want_trade=False, share_classes=True,
+              n_threads=threads, seed=seed)
+    times = []
+    for probe in (32 * threads, 256 * threads):
+        t0 = time.perf_counter()
+        oracle.portfolio_exposure_fused(model, probe_port.as_dict(), dates, n_steps, n_paths=probe, **kw)
+        times.append(time.perf_counter() - t0)
+    per_unit = max(times[1] - times[0], 1e-9) / (224 * threads * probe_port.n_trades * len(dates))
+    total_units = sum(p.n_trades * n for p, n in jobs) * len(dates)
+    frac = min(1.0, target_s / (per_unit * total_units))
+    units, elapsed = 0, 0.0
+    for p, n in jobs:
+        m = max(threads, int(frac * n))
+        t0 = time.perf_counter()
+        oracle.portfolio_exposure_fused(model, p.as_dict(), dates, n_steps, n_paths=m, **kw)
+        elapsed += time.perf_counter() - t0
+        units += p.n_trades * m * len(dates)
+    return {"value": units / elapsed, "unit": "trade-paths-dates/s", "cores": threads, "kind": "port",
+            "sample": f"every set on its first {100 * frac:.2f}% of paths (at least {threads}), the fused "
+                      f"restatement with schedule classes shared, one simulation per set, OpenMP over paths "
+                      f"({elapsed:.1f} s in all)", "host": host_cpu()}
+
+
+def cpu_greeks_by_differences(port, dates, n_steps, model, seed, bumps, target_s=15.0):
+    """CVA and its model sensitivities on the CPU as a fast CPU engine gets them: the fused restatement
+    (schedule classes shared, OpenMP over paths) at the base and at the eight central bumps with common
+    random numbers (the same Philox paths); credit sensitivities follow from the base EE in closed form.
+    The rate counts the job's trade-paths-dates once, as the GPU line does."""
+    import oracle
+
+    threads = cpu_threads()
+    pd_ = port.as_dict()
+    kw = dict(scheme=1, hazard_rate=0.02, recovery_rate=0.4, share_classes=True, n_threads=threads, seed=seed)
+    times = []
+    for probe in (32 * threads, 256 * threads):
+        t0 = time.perf_counter()
+        oracle.portfolio_exposure_fused(model, pd_, dates, n_steps, n_paths=probe, **kw)
+        times.append(time.perf_counter() - t0)
+    per_path_run = max(times[1] - times[0], 1e-9) / (224 * threads)
+    n = max(threads, int(target_s / (9 * per_path_run)))
+    names = ("a", "b", "sigma", "r0")
+    t0 = time.perf_counter()
+    oracle.portfolio_exposure_fused(model, pd_, dates, n_steps, n_paths=n, **kw)
+    for i, nm in enumerate(names):
+        for sgn in (1, -1):
+            m = list(model)
+            m[i] += sgn * bumps[nm]
+            oracle.portfolio_exposure_fused(tuple(m), pd_, dates, n_steps, n_paths=n, **kw)
+    elapsed = time.perf_counter() - t0
+    return {"value": port.n_trades * n * len(dates) / elapsed, "unit": "trade-paths-dates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"CVA + a, b, sigma, r0 by central differences (9 fused runs, schedule classes shared, "
+                      f"common Philox paths) on the first {n} paths, credit sensitivities in closed form "
+                      f"({elapsed:.1f} s)", "host": host_cpu()}
+
+
+def cpu_baseline_2f(port, dates, n_steps, model, seed, disc, target_s=15.0):
+    """G2++: the per-trade restatement (one simulation for the portfolio, OpenMP over paths) on the first
+    Philox paths; median of five runs after a warm-up."""
+    import oracle
+
+    threads = cpu_threads()
+    pd_ = port.as_dict()
+    probe = 2 * threads
+    z = oracle.philox_normals2(seed, probe, n_steps)
+    t0 = time.perf_counter()
+    oracle.portfolio_exposure(model, pd_, dates, n_steps, z, scheme=disc, want_trade=True, n_threads=threads)
+    per_path = max(time.perf_counter() - t0, 1e-6) / probe
+    n = max(threads, int((target_s / 6.0) / per_path) // threads * threads)
+    z = oracle.philox_normals2(seed, n, n_steps)
+    holder = {}
+
+    def run():
+        holder["ref"] = oracle.portfolio_exposure(model, pd_, dates, n_steps, z, scheme=disc, hazard_rate=0.02,
+                                                  recovery_rate=0.4, want_trade=True, n_threads=threads)
+
+    rate, med = _median_rate(run, port.n_trades * n * len(dates))
+    return ({"value": rate, "unit": "trade-paths-dates/s", "cores": threads, "kind": "port",
+             "sample": f"G2++ restatement, first {n} Philox paths, one simulation for the portfolio, OpenMP over "
+                       f"paths; median of 5 runs after a warm-up ({med:.2f} s each)", "host": host_cpu()},
+            holder["ref"], n)
 
 
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix
@@ -272,6 +445,10 @@ def run_batch(args):
     ms = 1000.0 * (time.perf_counter() - t0) / args.steps
     tm = batch.timing()
     units = sum(p.n_trades * n for p, n in jobs) * len(dates)
+    v_ms = tm["value_ms"]
+    flops = sum(algorithmic_work(p, dates, n, p.n_sets)[0] for p, n in jobs)
+    achieved = flops / (v_ms / 1000.0) / 1e12
+    traffic = measured_traffic("value_kernel", PROFILEB)
     out = {"metric": METRIC, "value": units / (ms / 1000.0), "unit": "trade-paths-dates/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
@@ -282,7 +459,19 @@ def run_batch(args):
            "kernel_ms": {"simulate": tm["simulate_ms"], "value": tm["value_ms"], "reduce": tm["reduce_ms"],
                          "cva": tm["cva_ms"], "total": tm["total_ms"]},
            "autotuner": tm0["tune_log"].strip().splitlines()[-1] if tm0["tune_log"].strip() else "",
+           "tune_log": tm0["tune_log"].strip().splitlines(),
+           "roofline": {"kernel": "value_kernel (one launch for all 64 sets)", "bound": "fp64-valu",
+                        "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                        "traffic_source": os.path.relpath(PROFILEB, ROOT) if traffic else None,
+                        "algorithmic_flops": flops,
+                        "algorithmic_unit": "2m+6 FLOPs per live (trade, path, date), m = remaining fixed flows",
+                        "value_kernel_ms": v_ms},
            "cva_total": float(sum(r["set_cva"].sum() for r in res))}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_sets(jobs, dates, len(dates), MODEL, args.seed, eqhip.DISC_EXACT,
+                                                args.cpu_seconds)
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     batch.close()
     ctx.close()
     print(json.dumps(out), flush=True)
@@ -311,6 +500,9 @@ def run_adjoint(args):
     t0 = time.perf_counter()
     plan.execute(None, 0.02, 0.4)
     fwd_ms = 1000.0 * (time.perf_counter() - t0)
+    v_ms = plan.timing()["value_ms"]
+    flops = algorithmic_work(port, dates, n_paths, port.n_sets)[0]
+    achieved = flops / (v_ms / 1000.0) / 1e12
     bumps = {"a": 1e-4, "b": 1e-4, "sigma": 1e-4, "r0": 1e-4}
     names = ("a", "b", "sigma", "r0")
     fd = {}
@@ -342,7 +534,17 @@ def run_adjoint(args):
            "pv_pillar_sensitivities": {"ms": sens_ms, "n_pillars": len(pillars),
                                        "max_rel_err_vs_bucketed_dv01_over_bump": float(
                                            np.max(np.abs(bk - 1e-4 * dz) / np.maximum(np.abs(bk), 1.0)))},
-           "cva_pillar_sensitivities": "identically zero: the CVA discounts with the model bond P(0,t|r0)"}
+           "cva_pillar_sensitivities": "identically zero: the CVA discounts with the model bond P(0,t|r0)",
+           "roofline": {"kernel": "value_kernel (the adjoint's forward run)", "bound": "fp64-valu",
+                        "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "algorithmic_flops": flops,
+                        "algorithmic_unit": "2m+6 FLOPs per live (trade, path, date), m = remaining fixed flows",
+                        "value_kernel_ms": v_ms,
+                        "note": "the reverse sweep (adjoint_kernel) is timed in the committed adjoint profile"}}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_greeks_by_differences(port, dates, n_steps, MODEL, args.seed, bumps,
+                                                        args.cpu_seconds)
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     plan.close()
     ctx.close()
     print(json.dumps(out), flush=True)
@@ -725,6 +927,8 @@ def main():
     kernel = "value2_kernel" if factors == 2 else "value_kernel"
     profile = PROFILE2 if factors == 2 else PROFILE
     traffic = measured_traffic(kernel, profile) if default_workload else None
+    counted_flops = counter_fp64_flops(profile_counters(profile)) if default_workload else None
+    counted_tflops = counted_flops / (v_ms / 1000.0) / 1e12 if counted_flops else None
     out = {
         "metric": METRIC,
         "value": value,
@@ -745,34 +949,38 @@ def main():
                    "pv_ladder": "serial on the job's stream" if args.serial_pv else
                                 "second context and stream, concurrent with the exposure job",
                    "live_units": u_live},
-        "roofline": {"kernel": kernel, "bound": "mfma", "achieved": achieved,
+        "roofline": {"kernel": kernel, "bound": "fp64-valu", "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": traffic,
                      "compute": "fp64 on the vector ALUs (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
                      "traffic_source": os.path.relpath(profile, ROOT) if traffic else None,
                      "algorithmic_flops": flops, "algorithmic_exps": n_exp,
                      "algorithmic_unit": "2m+6 FLOPs per live (trade, path, date), m = remaining fixed flows",
-                     "executed_flops": exec_flops, "executed_tflops": executed,
-                     "executed_frac": executed / FP64_PEAK_TFLOPS, "algorithmic_bytes": bytes_,
+                     "executed_flops_model": exec_flops, "executed_tflops_model": executed,
+                     "executed_flops_counters": counted_flops,
+                     "executed_tflops_counters": counted_tflops,
+                     "executed_frac_counters": (counted_tflops / FP64_PEAK_TFLOPS) if counted_tflops else None,
+                     "counters_source": os.path.relpath(profile, ROOT) if counted_flops else None,
+                     "algorithmic_bytes": bytes_,
                      "value_kernel_ms": v_ms, "exposure_kernels_ms": float(np.mean(total_kernel_ms)),
                      "paths_per_tile": tm["paths_per_tile"], "tiles_per_block": tm["tiles_per_block"],
                      "blocks": tm["n_value_blocks"]},
         "set_cva": float(res["set_cva"][0]),
     }
     if not args.no_cpu:
-        rate, threads, n_sample, cpu_s, ref, ref_rate, ref_sample = cpu_baseline(
-            port, dates, n_steps, model, args.seed, disc, args.cpu_seconds)
+        if factors == 2:
+            out["cpu_baseline"], ref, n_ref = cpu_baseline_2f(port, dates, n_steps, model, args.seed, disc,
+                                                              args.cpu_seconds)
+        else:
+            out["cpu_baseline"], ref, n_ref = cpu_baseline(port, dates, n_steps, model, args.seed, disc,
+                                                           args.cpu_seconds)
         # the same paths on the GPU (counter-based stream: path p's draws do not depend on n_paths)
-        small = ctx.exposure_run(model, port, dates, n_sample, n_steps, seed=args.seed, rng=eqhip.RNG_PHILOX,
+        small = ctx.exposure_run(model, port, dates, n_ref, n_steps, seed=args.seed, rng=eqhip.RNG_PHILOX,
                                  disc=disc, hazard_rate=hazard, recovery_rate=recovery, want_trade_ee=True)
-        cva_err = abs(small["set_cva"][0] - ref["set_cva"][0]) / max(abs(ref["set_cva"][0]), 1e-300)
-        out["cpu_baseline"] = {"value": rate, "unit": "trade-paths-dates/s", "cores": threads, "kind": "port",
-                               "sample": f"same portfolio and dates, first {n_sample} Philox paths, one "
-                                         f"simulation for the portfolio, OpenMP over paths; median of 5 runs "
-                                         f"after a warm-up ({cpu_s:.2f} s each)",
-                               "reference_structure": {"value": ref_rate, "cores": 1, "sample": ref_sample}}
-        out["speedup_vs_cpu"] = value / rate
-        out["cva_rel_err_vs_fp64_oracle"] = cva_err
+        out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+        out["cva_rel_err_vs_fp64_oracle"] = abs(small["set_cva"][0] - ref["set_cva"][0]) / max(
+            abs(ref["set_cva"][0]), 1e-300)
+        out["accuracy_sample_paths"] = n_ref
         out["trade_ee_max_rel_err_vs_oracle"] = float(
             np.max(np.abs(small["trade_ee"] - ref["trade_ee"]) / np.maximum(np.abs(ref["trade_ee"]).max(), 1e-300)))
     pool.shutdown()
