@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <set>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -111,6 +112,21 @@ struct eqhip_plan {
     bool executed = false;
     int n_items = 0;
     int set_mode = 0;
+    int ppl = kDefaultPpl, tpi = kDefaultTpi;  // valuation variant (paths per lane, trades per item)
+    bool autotune = false;     // the first execute times the valid variants and keeps the fastest
+    bool tuned = false;
+    std::vector<std::string> tune_log;
+    int n_items_t[3] = {0, 0, 0};             // per-trade work items of TPI 2, 4, 8
+    int max_chunks = 0;                       // per-trade partial sums are sized for this many chunks
+    // A portfolio too diverse for one block's LDS (or with more work items than a block has slots) is
+    // valued in parts: groups of its schedule classes, each a value plan of its own over the same
+    // paths (every grid step simulated into this plan's R), launched one after another into this
+    // plan's netting-set rows and per-trade sums (x_mode: the first part stores its signed set
+    // values, the next ones add theirs, the last one adds, takes the positive part and writes the
+    // signed values out). A part points back to the plan it belongs to.
+    std::vector<std::unique_ptr<eqhip_plan>> parts;
+    eqhip_plan* whole = nullptr;
+    int x_mode = 0;
     uint64_t target_units = 4096;   // work units the launch aims for (chunking of the path tiles)
     const double* r_shared = nullptr;  // batch-owned short-rate buffer (all grid steps), or null
     uint64_t r_stride = 0;
@@ -119,8 +135,9 @@ struct eqhip_plan {
 
     DevBuf lflow_off, lflow, ltau, lseg, zcls_off, zcls;
     DevBuf slot_of_step, R, Z, AB, date_slot, date_dfirst, date_det, fdesc, fam_off, flow_d, flow_tau,
-        flow_cls, fix_off, fix_slot, fix_AB, set_off, set_nlive, set_cls, set_alpha, set_beta, X, items, cls_mat, tr_K,
-        tr_absN, tr_N, tr_NK, tr_orig, item_first, partial, trade_ee, trade_cva, set_ee, set_pfe, set_cva, neff, qrank,
+        flow_cls, fix_off, fix_slot, fix_AB, set_off, set_nlive, set_cls, set_alpha, set_beta, X, cls_mat, tr_K,
+        tr_absN, tr_N, tr_NK, tr_orig, partial, trade_ee, trade_cva, set_ee, set_pfe, set_cva, neff, qrank,
+        items_t[3], item_first_t[3],
         d_dates, vplan, units, rows, R2, AB2, fix_B2, cube, xva_pos;
     DevBuf tables;   // storage of the host-built tables above (they are views into it)
     std::vector<uint64_t> h_neff, h_qrank;
@@ -194,19 +211,47 @@ void validate_portfolio(const eqhip_portfolio* pf) {
     }
 }
 
+int tpi_index(int tpi) { return tpi == 2 ? 0 : (tpi == 4 ? 1 : 2); }
+
+// Whether the plan can run variant (ppl, tpi): its LDS fits, its per-trade work items fit the block's
+// work slots, and two-factor plans take one path per lane.
+bool variant_fits(const eqhip_plan& P, int ppl, int tpi) {
+    if (P.factors == 2 && ppl != 1) return false;
+    if (ppl == 4 && tpi != 2) return false;  // the kernel's register budget
+    const bool split = P.S <= kMaxSplitSets;
+    size_t lds = value_lds_bytes(P.D, P.Fmax, P.C, split ? P.S : 0, ppl);
+    if (split && lds > P.ctx->lds_limit) lds = value_lds_bytes(P.D, P.Fmax, P.C, 0, ppl);
+    if (P.want_trade) {
+        if (P.n_items_t[tpi_index(tpi)] > kTradeSlots) return false;
+        lds = std::max(lds, value_trade_scratch_bytes(tpi));
+    }
+    return lds <= P.ctx->lds_limit;
+}
+
+void tile_launch(eqhip_plan& P);
+
 void choose_launch(eqhip_plan& P) {
+    const int ppl = P.ppl;
     // split mode keeps one partial sum per (set, wave) in LDS; when those do not fit, one wave per set
-    P.set_mode = (P.S <= kMaxSplitSets && value_lds_bytes(P.D, P.Fmax, P.C, P.S) <= P.ctx->lds_limit) ? 0 : 1;
-    size_t lds = value_lds_bytes(P.D, P.Fmax, P.C, P.set_mode == 0 ? P.S : 0);
+    P.set_mode =
+        (P.S <= kMaxSplitSets && value_lds_bytes(P.D, P.Fmax, P.C, P.S, ppl) <= P.ctx->lds_limit) ? 0 : 1;
+    size_t lds = value_lds_bytes(P.D, P.Fmax, P.C, P.set_mode == 0 ? P.S : 0, ppl);
     if (lds > P.ctx->lds_limit)
         raise(EQHIP_E_CAPACITY, "portfolio too diverse for one plan: " + std::to_string(P.D) +
                                     " cash-flow dates and " + std::to_string(P.C) + " schedule classes need " +
                                     std::to_string(lds) + " B of LDS (limit " + std::to_string(P.ctx->lds_limit) +
                                     "); split it into several plans");
     // per-trade sums of a unit's work slots are reduced through LDS at the end of the unit
-    if (P.want_trade) lds = std::max(lds, value_trade_scratch_bytes());
+    if (P.want_trade) lds = std::max(lds, value_trade_scratch_bytes(P.tpi));
     P.lds_bytes = lds;
-    P.n_tiles = (P.mc.n_paths + kPathsPerTile - 1) / kPathsPerTile;
+    tile_launch(P);
+}
+
+// The path tiling of the plan's variant: tiles of 64 PPL paths, grouped into chunks, one work unit
+// per (date, chunk).
+void tile_launch(eqhip_plan& P) {
+    const uint64_t tw = (uint64_t)kPathsPerTile * P.ppl;
+    P.n_tiles = (P.mc.n_paths + tw - 1) / tw;
     // enough units for ~16 blocks per CU (one 1024-thread block per CU at this LDS footprint). Units run
     // date-major and blocks go to the 8 XCDs round-robin, so with a multiple of 8 chunks every unit of
     // one path chunk lands on the same XCD, whose L2 then serves the rows later dates read again (the
@@ -231,10 +276,18 @@ void work_buffer(eqhip_plan& P, DevBuf& b, DevBuf& pool, size_t bytes) {
     b.view(pool.p, bytes);
 }
 
+void build_parts(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfolio* pf, const double* dates,
+                 int n_dates, const eqhip_mc* mc, const std::vector<std::vector<double>>& cls_dates,
+                 const std::vector<int>& trade_cls);
+void build_select_rows(eqhip_plan& P);
+
 // all_steps: r is stored at every grid step (slot = step), so plans of a batch can share one
-// simulation; the batch then owns the short-rate buffer.
+// simulation; the batch then owns the short-rate buffer. A part of a partitioned plan (P.whole set)
+// reads the whole plan's rates and writes its rows; whole_index maps the part's trades to the whole's.
 void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfolio* pf,
-                const double* dates, int n_dates, const eqhip_mc* mc, int want_trade, bool all_steps = false) {
+                const double* dates, int n_dates, const eqhip_mc* mc, int want_trade, bool all_steps = false,
+                const std::vector<int>* whole_index = nullptr) {
+    const bool is_part = P.whole != nullptr;
     // a two-factor plan arrives with factors = 2 and model2 set, and no one-factor model
     if (!mc) raise(EQHIP_E_INVALID, "mc is null");
     if (P.factors == 2) {
@@ -259,6 +312,7 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
     P.S = pf->n_sets;
     P.want_trade = (want_trade & EQHIP_PLAN_TRADE_EE) ? 1 : 0;
     P.keep_cube = (want_trade & EQHIP_PLAN_KEEP_CUBE) != 0;
+    P.autotune = (want_trade & EQHIP_PLAN_AUTOTUNE) != 0;
     hipStream_t st = P.ctx->stream;
 
     // time grid (exposure.rs:91-116): dt = t_max / n_steps, idx = round(t / dt)
@@ -503,11 +557,12 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
             set_nlive[(size_t)k * P.S + s] = n;
         }
 
-    // per-trade work items: trades sorted by (last date of the class, class, sign), up to
-    // kTradesPerItem per item. Ordering by maturity keeps the items still alive at a date a
-    // contiguous suffix of the list, so the waves whose items have all matured skip the date.
-    std::vector<int4> items;
-    std::vector<int> tr_orig, item_first(n_dates, 0);
+    // per-trade work items: trades sorted by (last date of the class, class, sign), up to TPI trades
+    // per item, for each TPI the kernel offers (2, 4, 8). Ordering by maturity keeps the items still
+    // alive at a date a contiguous suffix of the list, so the waves whose items have all matured skip
+    // the date.
+    std::vector<int4> items_t[3];
+    std::vector<int> tr_orig, item_first_t[3];
     std::vector<double> tr_K, tr_absN, tr_N, tr_NK;
     if (P.want_trade) {
         auto sgn = [&](int j) { return pf->notional[j] < 0.0 ? -1 : 1; };
@@ -521,35 +576,56 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
             return std::make_tuple(last_t(x), trade_cls[x], sgn(x)) <
                    std::make_tuple(last_t(y), trade_cls[y], sgn(y));
         });
-        for (int i = 0; i < pf->n_trades;) {
-            const int c = trade_cls[order[i]], sg = sgn(order[i]);
-            int4 item = make_int4(c, i, 0, sg);
-            while (i < pf->n_trades && trade_cls[order[i]] == c && sgn(order[i]) == sg && item.z < kTradesPerItem) {
-                const int j = order[i];
-                tr_orig.push_back(j);
-                tr_K.push_back(pf->fixed_rate[j]);
-                tr_absN.push_back(std::fabs(pf->notional[j]));
-                tr_N.push_back(pf->notional[j]);
-                tr_NK.push_back(pf->notional[j] * pf->fixed_rate[j]);
-                ++i;
-                ++item.z;
+        for (int j : order) {
+            tr_orig.push_back(whole_index ? (*whole_index)[j] : j);
+            tr_K.push_back(pf->fixed_rate[j]);
+            tr_absN.push_back(std::fabs(pf->notional[j]));
+            tr_N.push_back(pf->notional[j]);
+            tr_NK.push_back(pf->notional[j] * pf->fixed_rate[j]);
+        }
+        const int tpis[3] = {2, 4, 8};
+        for (int v = 0; v < 3; ++v) {
+            std::vector<int4>& items = items_t[v];
+            for (int i = 0; i < pf->n_trades;) {
+                const int c = trade_cls[order[i]], sg = sgn(order[i]);
+                int4 item = make_int4(c, i, 0, sg);
+                while (i < pf->n_trades && trade_cls[order[i]] == c && sgn(order[i]) == sg && item.z < tpis[v]) {
+                    ++i;
+                    ++item.z;
+                }
+                items.push_back(item);
             }
-            items.push_back(item);
+            P.n_items_t[v] = (int)items.size();
+            // first item alive at each date: the items alive at a date are a suffix of the list
+            item_first_t[v].assign(n_dates, 0);
+            for (int k = 0; k < n_dates; ++k) {
+                int f = 0;
+                while (f < (int)items.size() && !(cls_mat[items[f].x] > P.dates[k] + kDateTolHost)) ++f;
+                item_first_t[v][k] = f;
+            }
         }
-        P.n_items = (int)items.size();
-        // first item alive at each date: the items alive at a date are a suffix of the list
-        for (int k = 0; k < n_dates; ++k) {
-            int f = 0;
-            while (f < P.n_items && !(cls_mat[items[f].x] > P.dates[k] + kDateTolHost)) ++f;
-            item_first[k] = f;
-        }
-        if (P.n_items > kValueThreads * kItemsPerThread)
-            raise(EQHIP_E_CAPACITY, "per-trade EE supports at most " +
-                                        std::to_string(kValueThreads * kItemsPerThread) +
-                                        " work items per plan (got " + std::to_string(P.n_items) + ")");
+    }
+    // default variant: one path per lane, four trades per item (eight when four leave too many items)
+    P.ppl = 1;
+    P.tpi = (!P.want_trade || P.n_items_t[1] <= kTradeSlots) ? 4 : 8;
+    P.n_items = P.want_trade ? P.n_items_t[tpi_index(P.tpi)] : 0;
+    // A one-factor plan that no variant fits is valued in parts (a batch job or a part must fit)
+    const bool fits = variant_fits(P, 1, 2) || variant_fits(P, 1, 4) || variant_fits(P, 1, 8);
+    const bool partitioned = !fits && !is_part && !all_steps && P.factors == 1;
+    if (!fits && !partitioned && P.want_trade && P.n_items_t[2] > kTradeSlots)
+        raise(EQHIP_E_CAPACITY, "per-trade EE supports at most " + std::to_string(kTradeSlots) +
+                                    " work items per plan (got " + std::to_string(P.n_items_t[2]) + ")");
+    if (partitioned) {
+        // the parts read r at their own dates and fixings: every grid step is simulated
+        P.n_slots = (int)steps + 1;
+        for (uint32_t g2 = 0; g2 <= steps; ++g2) slot_of_step[g2] = (int)g2;
     }
 
-    choose_launch(P);
+    if (partitioned)
+        tile_launch(P);  // the parts' tiling (one path per lane), for the per-trade sums they share
+    else
+        choose_launch(P);
+    P.max_chunks = P.n_chunks;  // one path per lane has the most chunks of any variant
 
     // effective path counts and PFE ranks (exposure.rs:42-49; grid index 0 has one row)
     std::vector<uint64_t> neff(n_dates), qrank(n_dates);
@@ -564,10 +640,10 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
     P.h_neff = neff;
     P.h_qrank = qrank;
 
-    // memory check for the large buffers
+    // memory check for the large buffers (a part has none: it writes the whole plan's)
     size_t free_b = 0, total_b = 0;
     HIPCHK(hipMemGetInfo(&free_b, &total_b));
-    double need = 8.0 * ((all_steps ? 0.0 : (double)P.factors * P.n_slots * mc->n_paths) +
+    double need = is_part ? 0.0 : 8.0 * ((all_steps ? 0.0 : (double)P.factors * P.n_slots * mc->n_paths) +
                          (double)(P.keep_cube ? 2 : 1) * P.S * n_dates * mc->n_paths +
                          (P.want_trade ? (double)P.n_trades * n_dates * P.n_chunks : 0.0) +
                          (mc->rng == EQHIP_RNG_HOST_DRAWS ? (double)P.factors * mc->n_steps * mc->n_paths : 0.0));
@@ -604,16 +680,20 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
     pack.add(P.qrank, qrank);
     pack.add(P.d_dates, P.dates);
     if (P.want_trade) {
-        pack.add(P.items, items);
+        for (int v = 0; v < 3; ++v) {
+            pack.add(P.items_t[v], items_t[v]);
+            pack.add(P.item_first_t[v], item_first_t[v]);
+        }
         pack.add(P.tr_K, tr_K);
         pack.add(P.tr_absN, tr_absN);
         pack.add(P.tr_N, tr_N);
         pack.add(P.tr_NK, tr_NK);
         pack.add(P.tr_orig, tr_orig);
-        pack.add(P.item_first, item_first);
-        work_buffer(P, P.partial, P.ctx->pool_partial, (size_t)P.n_trades * n_dates * P.n_chunks * 8);
-        P.trade_ee.alloc((size_t)P.n_trades * n_dates * 8);
-        P.trade_cva.alloc((size_t)P.n_trades * 8);
+        if (!is_part) {
+            work_buffer(P, P.partial, P.ctx->pool_partial, (size_t)P.n_trades * n_dates * P.max_chunks * 8);
+            P.trade_ee.alloc((size_t)P.n_trades * n_dates * 8);
+            P.trade_cva.alloc((size_t)P.n_trades * 8);
+        }
     }
     DevBuf tau, ftau;  // views into the plan's tables
     pack.add(tau, ab_tau);
@@ -647,15 +727,17 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
             HIPCHK(hipGetLastError());
         }
     }
-    if (!all_steps) work_buffer(P, P.R, P.ctx->pool_R, (size_t)P.n_slots * mc->n_paths * 8);
-    if (P.factors == 2) work_buffer(P, P.R2, P.ctx->pool_R2, (size_t)P.n_slots * mc->n_paths * 8);
-    work_buffer(P, P.X, P.ctx->pool_X, (size_t)P.S * n_dates * mc->n_paths * 8);
-    if (P.keep_cube) P.cube.alloc((size_t)n_dates * P.S * mc->n_paths * 8);
-    P.set_ee.alloc((size_t)P.S * n_dates * 8);
-    P.set_pfe.alloc((size_t)P.S * n_dates * 8);
-    P.set_cva.alloc((size_t)P.S * 8);
-    if (mc->rng == EQHIP_RNG_HOST_DRAWS)
-        work_buffer(P, P.Z, P.ctx->pool_Z, (size_t)P.factors * mc->n_steps * mc->n_paths * 8);
+    if (!is_part) {
+        if (!all_steps) work_buffer(P, P.R, P.ctx->pool_R, (size_t)P.n_slots * mc->n_paths * 8);
+        if (P.factors == 2) work_buffer(P, P.R2, P.ctx->pool_R2, (size_t)P.n_slots * mc->n_paths * 8);
+        work_buffer(P, P.X, P.ctx->pool_X, (size_t)P.S * n_dates * mc->n_paths * 8);
+        if (P.keep_cube) P.cube.alloc((size_t)n_dates * P.S * mc->n_paths * 8);
+        P.set_ee.alloc((size_t)P.S * n_dates * 8);
+        P.set_pfe.alloc((size_t)P.S * n_dates * 8);
+        P.set_cva.alloc((size_t)P.S * 8);
+        if (mc->rng == EQHIP_RNG_HOST_DRAWS)
+            work_buffer(P, P.Z, P.ctx->pool_Z, (size_t)P.factors * mc->n_steps * mc->n_paths * 8);
+    }
     P.h_T = T;
     P.h_fdesc = fdesc;
     P.h_fam_off = fam_off;
@@ -676,9 +758,105 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
     for (int k = 0; k < n_dates; ++k) P.h_date_grid[k] = (int)grid_of(P.dates[k]);
     P.h_slot_grid.assign(P.n_slots, 0);
     for (auto& kv : slot_of) P.h_slot_grid[kv.second] = (int)kv.first;
-    build_value_plan(P);
-    for (auto& e : P.ev) HIPCHK(hipEventCreate(&e));
+    if (partitioned) {
+        std::vector<std::vector<double>> cls_dates(P.C);
+        for (int c = 0; c < P.C; ++c) {
+            cls_dates[c].push_back(classes[c].start);
+            cls_dates[c].insert(cls_dates[c].end(), classes[c].pay.begin(), classes[c].pay.end());
+        }
+        build_parts(P, model, pf, dates, n_dates, mc, cls_dates, trade_cls);
+        build_select_rows(P);
+    } else {
+        build_value_plan(P);
+    }
+    if (!is_part)
+        for (auto& e : P.ev) HIPCHK(hipEventCreate(&e));
 }
+
+// The parts of a partitioned plan: the schedule classes in maturity order (a class with many trades
+// in chunks of its trades) gathered greedily into groups whose cash-flow dates, classes, fixings and
+// per-trade work items fit one block, each group a part over the same paths and dates. The parts run
+// in order: first, middle and last differ only in how they store the netting-set values (x_mode).
+void build_parts(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfolio* pf, const double* dates,
+                 int n_dates, const eqhip_mc* mc, const std::vector<std::vector<double>>& cls_dates,
+                 const std::vector<int>& trade_cls) {
+    const size_t budget = P.ctx->lds_limit;
+    const int C = (int)cls_dates.size();
+    std::vector<std::vector<int>> cls_trades(C);
+    for (int j = 0; j < pf->n_trades; ++j) cls_trades[trade_cls[j]].push_back(j);
+    constexpr int kChunkTrades = 2048;  // at most 512 + 2 work items of four trades per class chunk
+    // LDS of a group with D dates and C classes, fixings bounded by the classes (split-free set sums)
+    auto group_lds = [&](size_t D, size_t nc) {
+        size_t lds = value_lds_bytes((int)D, (int)nc, (int)nc, 0, 1);
+        if (P.want_trade) lds = std::max(lds, value_trade_scratch_bytes(4));
+        return lds;
+    };
+    std::vector<std::vector<int>> groups;  // trades of each group
+    std::set<double> g_dates;
+    size_t g_classes = 0, g_items = 0;
+    for (int c = 0; c < C; ++c) {
+        const auto& tr = cls_trades[c];
+        for (size_t t0 = 0; t0 < tr.size(); t0 += kChunkTrades) {
+            const size_t t1 = std::min(tr.size(), t0 + (size_t)kChunkTrades);
+            size_t pos = 0;
+            for (size_t t = t0; t < t1; ++t) pos += pf->notional[tr[t]] >= 0.0 ? 1 : 0;
+            const size_t items = (pos + 3) / 4 + ((t1 - t0 - pos) + 3) / 4;
+            std::set<double> nd = g_dates;
+            nd.insert(cls_dates[c].begin(), cls_dates[c].end());
+            const bool fits = group_lds(nd.size(), g_classes + 1) <= budget &&
+                              (!P.want_trade || g_items + items <= (size_t)kTradeSlots);
+            if (groups.empty() || !fits) {  // a new group, starting with this chunk
+                groups.emplace_back();
+                g_dates.clear();
+                g_classes = g_items = 0;
+                nd = std::set<double>(cls_dates[c].begin(), cls_dates[c].end());
+            }
+            if (group_lds(nd.size(), 1) > budget)
+                raise(EQHIP_E_CAPACITY, "a schedule with " + std::to_string(cls_dates[c].size()) +
+                                            " cash-flow dates needs more LDS than a block has (" +
+                                            std::to_string(budget) + " B)");
+            groups.back().insert(groups.back().end(), tr.begin() + (long)t0, tr.begin() + (long)t1);
+            g_dates = std::move(nd);
+            ++g_classes;
+            g_items += items;
+        }
+    }
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+        std::vector<int>& idx = groups[gi];
+        std::sort(idx.begin(), idx.end());  // the whole portfolio's order within a part
+        std::vector<int32_t> set_id, off{0};
+        std::vector<double> notional, fixed, start, pay, acc;
+        for (int j : idx) {
+            set_id.push_back(pf->set_id[j]);
+            notional.push_back(pf->notional[j]);
+            fixed.push_back(pf->fixed_rate[j]);
+            start.push_back(pf->start[j]);
+            for (int f = pf->flow_offset[j]; f < pf->flow_offset[j + 1]; ++f) {
+                pay.push_back(pf->payment_times[f]);
+                acc.push_back(pf->accruals[f]);
+            }
+            off.push_back((int32_t)pay.size());
+        }
+        const eqhip_portfolio sub{(int32_t)idx.size(), P.S, set_id.data(), notional.data(), fixed.data(),
+                                  start.data(), off.data(), pay.data(), acc.data()};
+        auto part = std::make_unique<eqhip_plan>();
+        part->ctx = P.ctx;
+        part->device = P.device;
+        part->whole = &P;
+        part->target_units = P.target_units;
+        part->x_mode = (gi == 0) ? 1 : (gi + 1 == groups.size() ? 3 : 2);
+        part->r_shared = P.R.as<double>();
+        part->r_stride = mc->n_paths;
+        build_plan(*part, model, &sub, dates, n_dates, mc, P.want_trade ? EQHIP_PLAN_TRADE_EE : 0, true, &idx);
+        if (part->n_chunks != P.n_chunks)
+            raise(EQHIP_E_HIP, "internal: the parts of a plan must share its path tiling");
+        P.parts.push_back(std::move(part));
+    }
+    P.lds_bytes = 0;
+    for (auto& q : P.parts) P.lds_bytes = std::max(P.lds_bytes, q->lds_bytes);
+}
+
+void build_select_rows(eqhip_plan& P);
 
 // Device descriptor of a plan (pointers into the plan's buffers) and its work units.
 void build_value_plan(eqhip_plan& P) {
@@ -697,6 +875,7 @@ void build_value_plan(eqhip_plan& P) {
     g.n_chunks = P.n_chunks;
     g.tiles_per_block = P.tiles_per_block;
     g.set_mode = P.set_mode;
+    g.variant = kVariant(P.ppl, P.tpi);
     g.two_factor = (P.factors == 2) ? 1 : 0;
     g.R2 = P.R2.as<double>();
     g.AB2 = P.AB2.as<double>();
@@ -725,25 +904,34 @@ void build_value_plan(eqhip_plan& P) {
     g.set_cls = P.set_cls.as<int>();
     g.set_alpha = P.set_alpha.as<double>();
     g.set_beta = P.set_beta.as<double>();
-    g.X = P.X.as<double>();
+    // a part writes the whole plan's netting-set rows and per-trade sums, indexed by its trades
+    const eqhip_plan& W = P.whole ? *P.whole : P;
+    g.X = W.X.as<double>();
+    g.x_mode = P.x_mode;
     g.want_trade = P.want_trade;
+    g.n_trades = W.n_trades;
+    P.n_items = P.want_trade ? P.n_items_t[tpi_index(P.tpi)] : 0;
     g.n_items = P.n_items;
-    g.n_trades = P.n_trades;
-    g.item_first = P.item_first.as<int>();
-    g.items = P.items.as<int4>();
+    g.item_first = P.item_first_t[tpi_index(P.tpi)].as<int>();
+    g.items = P.items_t[tpi_index(P.tpi)].as<int4>();
     g.cls_mat = P.cls_mat.as<double>();
     g.tr_K = P.tr_K.as<double>();
     g.tr_absN = P.tr_absN.as<double>();
     g.tr_N = P.tr_N.as<double>();
     g.tr_NK = P.tr_NK.as<double>();
     g.tr_orig = P.tr_orig.as<int>();
-    g.partial = P.partial.as<double>();
+    g.partial = W.partial.as<double>();
     P.vplan.alloc(sizeof(ValuePlan));
     HIPCHK(hipMemcpy(P.vplan.p, &g, sizeof(ValuePlan), hipMemcpyHostToDevice));
     P.unit_list.clear();
     for (int k = 0; k < P.n_dates; ++k)
         for (int c = 0; c < P.n_chunks; ++c) P.unit_list.push_back(make_int4(0, k, c, 0));
     upload(P.units, P.unit_list, P.ctx->stream);
+    if (!P.whole) build_select_rows(P);
+}
+
+// The EE / PFE95 select rows of a plan's netting sets ([set][date] rows of X).
+void build_select_rows(eqhip_plan& P) {
     const std::vector<uint64_t>& neff = P.h_neff;
     const std::vector<uint64_t>& qrank = P.h_qrank;
     std::vector<SelectRow> rows;
@@ -758,6 +946,11 @@ void build_value_plan(eqhip_plan& P) {
 
 void launch_value(eqhip_plan& P, double* path_mtm, double* set_mtm) {
     hipStream_t st = P.ctx->stream;
+    if (!P.parts.empty()) {  // the parts in order; the last one writes the signed set values
+        for (size_t i = 0; i < P.parts.size(); ++i)
+            launch_value(*P.parts[i], path_mtm, i + 1 == P.parts.size() ? set_mtm : nullptr);
+        return;
+    }
     if (path_mtm != P.vp.path_mtm || set_mtm != P.vp.set_mtm) {  // per-run outputs live in the descriptor
         P.vp.path_mtm = path_mtm;
         P.vp.set_mtm = set_mtm;
@@ -889,6 +1082,56 @@ void launch_simulate2(eqhip_plan& P) {
     HIPCHK(hipGetLastError());
 }
 
+// Times one valuation launch of the plan as it is laid out now (after a warm-up launch).
+float time_value_launch(eqhip_plan& P) {
+    hipStream_t st = P.ctx->stream;
+    launch_value(P, nullptr, nullptr);
+    HIPCHK(hipEventRecord(P.ev[0], st));
+    launch_value(P, nullptr, nullptr);
+    HIPCHK(hipEventRecord(P.ev[1], st));
+    HIPCHK(hipEventSynchronize(P.ev[1]));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, P.ev[0], P.ev[1]));
+    return ms;
+}
+
+// Autotuner of a plan: on its own simulated paths, times every (paths per lane, trades per item)
+// variant that fits and keeps the fastest. The trial launches write the outputs the real launch then
+// overwrites, and every variant writes the same netting-set values.
+void tune_plan(eqhip_plan& P) {
+    if (!P.parts.empty()) {  // the parts keep their default variant
+        P.tuned = true;
+        return;
+    }
+    const int ppls[3] = {1, 2, 4}, tpis[3] = {2, 4, 8};
+    float best = 1e30f;
+    int best_ppl = P.ppl, best_tpi = P.tpi;
+    for (int ppl : ppls)
+        for (int tpi : tpis) {
+            // without a per-trade step TPI does not matter: one variant per PPL
+            if (!P.want_trade && tpi != (ppl == 4 ? 2 : 4)) continue;
+            if (!variant_fits(P, ppl, tpi)) continue;
+            P.ppl = ppl;
+            P.tpi = tpi;
+            choose_launch(P);
+            build_value_plan(P);
+            const float ms = time_value_launch(P);
+            P.tune_log.push_back("paths/lane=" + std::to_string(ppl) + " trades/item=" + std::to_string(tpi) + ": " +
+                                 std::to_string(ms) + " ms");
+            if (ms < best) {
+                best = ms;
+                best_ppl = ppl;
+                best_tpi = tpi;
+            }
+        }
+    P.ppl = best_ppl;
+    P.tpi = best_tpi;
+    choose_launch(P);
+    build_value_plan(P);
+    P.tune_log.push_back("chosen: paths/lane=" + std::to_string(best_ppl) + " trades/item=" + std::to_string(best_tpi));
+    P.tuned = true;
+}
+
 void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* credit,
                   eqhip_outputs* out) {
     if (!out || !out->set_ee || !out->set_pfe) raise(EQHIP_E_INVALID, "outputs set_ee / set_pfe are required");
@@ -921,6 +1164,10 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
                          P.Z.as<double>(), P.R.as<double>());
     }
     HIPCHK(hipEventRecord(P.ev[1], st));
+    if (P.autotune && !P.tuned && !P.flat) {
+        tune_plan(P);  // the paths are simulated: time the variants on them, then value with the fastest
+        HIPCHK(hipEventRecord(P.ev[1], st));
+    }
     if (P.flat && !out->path_mtm && !out->set_mtm)
         launch_value_flat(P);
     else
@@ -995,9 +1242,11 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
     P.timing.reduce_ms = ms[2];
     P.timing.cva_ms = ms[3];
     P.timing.total_ms = (double)ms[0] + ms[1] + ms[2] + ms[3];
-    P.timing.n_value_blocks = (uint64_t)P.n_dates * P.n_chunks;
-    P.timing.paths_per_tile = (uint32_t)kPathsPerTile;
+    P.timing.n_value_blocks = (uint64_t)P.n_dates * P.n_chunks * std::max<size_t>(1, P.parts.size());
+    P.timing.n_parts = (uint32_t)std::max<size_t>(1, P.parts.size());
+    P.timing.paths_per_tile = (uint32_t)(kPathsPerTile * P.ppl);
     P.timing.tiles_per_block = (uint32_t)P.tiles_per_block;
+    P.timing.trades_per_item = (uint32_t)P.tpi;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1318,7 +1567,7 @@ void cva_adjoint(eqhip_plan& P, const double* host_draws, int set, const eqhip_c
         cls_beta[P.h_set_cls[e]] = P.h_set_beta[e];
     }
     for (int k = 0; k < nd; ++k) {
-        std::map<std::pair<int, int>, int> mid_here;  // (family) -> index
+        std::map<std::pair<int, int>, int> mid_here;  // (next payment, fixing entry) -> mid entry
         for (int F = 0; F < P.NF; ++F) {
             const int4 fd = P.h_fdesc[(size_t)k * P.NF + F];
             if (fd.x == 0) continue;
@@ -1337,10 +1586,17 @@ void cva_adjoint(eqhip_plan& P, const double* host_draws, int set, const eqhip_c
                 else alpha_mid += al;
             }
             if (fd.x == 3 && alpha_mid != 0.0) {
+                // families with the same next payment and fixing share one entry
+                auto same = mid_here.find(std::make_pair(fd.z, fd.w));
+                if (same != mid_here.end()) {
+                    mid_alpha[(size_t)same->second] += alpha_mid;
+                    continue;
+                }
                 const int fe = P.h_fix_off[k] + fd.w;              // fixing entry of this date
                 const int64_t fstep = P.h_slot_grid[P.h_fix_slot[fe]];
                 auto it = gslot.find(fstep);
                 if (it == gslot.end()) it = gslot.emplace(fstep, (int)gslot.size()).first;
+                mid_here.emplace(std::make_pair(fd.z, fd.w), (int)mid.size());
                 mid.push_back(make_int4(fd.z, rrow(fstep), (int)fix_tau_adj.size(), it->second));
                 fix_tau_adj.push_back(P.h_fix_tau[fe]);
                 mid_alpha.push_back(alpha_mid);
@@ -1518,6 +1774,7 @@ struct eqhip_batch {
     std::vector<int4> unit_list;
     size_t lds_bytes = 0;
     int persistent = 1;
+    int best_ppl = kDefaultPpl, best_tpi = kDefaultTpi;
     uint64_t target_units_per_cu = 8;
     bool tuned = false;
     std::vector<std::string> tune_log;
@@ -1626,33 +1883,57 @@ void simulate_batch(eqhip_batch& B) {
     HIPCHK(hipGetLastError());
 }
 
-// Autotuner: times the valuation launch for each (units per CU, persistent) candidate on the
-// batch's own data and keeps the fastest.
-void autotune_batch(eqhip_batch& B) {
+// Times one valuation launch of the batch as laid out now (after a warm-up launch).
+float time_batch_launch(eqhip_batch& B, hipEvent_t e0, hipEvent_t e1) {
     hipStream_t st = B.ctx->stream;
+    launch_batch_value(B);
+    HIPCHK(hipEventRecord(e0, st));
+    launch_batch_value(B);
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    return ms;
+}
+
+// Every job's variant: the most paths per lane up to `ppl` and `tpi` trades per item where they fit,
+// else the job's default.
+void set_batch_variants(eqhip_batch& B, int ppl, int tpi) {
+    for (auto& J : B.jobs) {
+        // without a per-trade step TPI does not matter: take the one each PPL allows
+        auto tpi_for = [&](int p) { return J->want_trade ? tpi : (p == 4 ? 2 : 4); };
+        int p = ppl;
+        while (p > 1 && !variant_fits(*J, p, tpi_for(p))) p /= 2;
+        if (variant_fits(*J, p, tpi_for(p))) {
+            J->ppl = p;
+            J->tpi = tpi_for(p);
+        } else {  // the job's default
+            J->ppl = kDefaultPpl;
+            J->tpi = (!J->want_trade || J->n_items_t[1] <= kTradeSlots) ? 4 : 8;
+        }
+    }
+}
+
+// Autotuner of a batch, on the batch's own simulated paths, in two stages: the launch shape (units per
+// CU, persistent grid or one block per unit) with the default variant, then at that shape the
+// variant (paths per lane, trades per item) every job takes where it fits. Every trial is logged.
+void autotune_batch(eqhip_batch& B) {
     simulate_batch(B);
-    const uint64_t unit_targets[] = {4, 8, 16};
-    const int modes[] = {1, 0};
-    double best = 1e300;
-    uint64_t best_t = B.target_units_per_cu;
-    int best_m = B.persistent;
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    for (uint64_t t : unit_targets) {
+    float best = 1e30f;
+    uint64_t best_t = B.target_units_per_cu;
+    int best_m = B.persistent, best_ppl = kDefaultPpl, best_tpi = kDefaultTpi;
+    set_batch_variants(B, kDefaultPpl, kDefaultTpi);
+    for (uint64_t t : {4, 8, 16}) {
         B.target_units_per_cu = t;
         layout_batch(B);
-        for (int m : modes) {
+        for (int m : {1, 0}) {
             B.persistent = m;
-            launch_batch_value(B);  // warm
-            HIPCHK(hipEventRecord(e0, st));
-            launch_batch_value(B);
-            HIPCHK(hipEventRecord(e1, st));
-            HIPCHK(hipEventSynchronize(e1));
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-            B.tune_log.push_back("units/CU=" + std::to_string(t) + (m ? " persistent" : " per-unit") + ": " +
-                                 std::to_string(ms) + " ms");
+            const float ms = time_batch_launch(B, e0, e1);
+            B.tune_log.push_back("units/CU=" + std::to_string(t) + (m ? " persistent" : " per-unit") +
+                                 " paths/lane=1 trades/item=4: " + std::to_string(ms) + " ms");
             if (ms < best) {
                 best = ms;
                 best_t = t;
@@ -1660,11 +1941,35 @@ void autotune_batch(eqhip_batch& B) {
             }
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     B.target_units_per_cu = best_t;
     B.persistent = best_m;
+    bool want_trade = false;
+    for (auto& J : B.jobs) want_trade |= J->want_trade != 0;
+    for (int ppl : {1, 2, 4})
+        for (int tpi : {2, 4, 8}) {
+            if (ppl == 4 && tpi != 2) continue;
+            if ((ppl == kDefaultPpl && tpi == kDefaultTpi) || (!want_trade && tpi != (ppl == 4 ? 2 : 4))) continue;
+            set_batch_variants(B, ppl, tpi);
+            layout_batch(B);
+            const float ms = time_batch_launch(B, e0, e1);
+            B.tune_log.push_back("units/CU=" + std::to_string(best_t) + (best_m ? " persistent" : " per-unit") +
+                                 " paths/lane<=" + std::to_string(ppl) + " trades/item=" + std::to_string(tpi) + ": " +
+                                 std::to_string(ms) + " ms");
+            if (ms < best) {
+                best = ms;
+                best_ppl = ppl;
+                best_tpi = tpi;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    set_batch_variants(B, best_ppl, best_tpi);
     layout_batch(B);
+    B.best_ppl = best_ppl;
+    B.best_tpi = best_tpi;
+    B.tune_log.push_back("chosen: " + std::string(best_m ? "persistent" : "per-unit") + ", units/CU=" +
+                         std::to_string(best_t) + ", paths/lane<=" + std::to_string(best_ppl) + ", trades/item=" +
+                         std::to_string(best_tpi));
     B.tuned = true;
 }
 
@@ -1730,8 +2035,9 @@ void execute_batch(eqhip_batch& B, const eqhip_credit* credit, eqhip_outputs* ou
     B.timing.cva_ms = ms[3];
     B.timing.total_ms = (double)ms[0] + ms[1] + ms[2] + ms[3];
     B.timing.n_value_blocks = B.unit_list.size();
-    B.timing.paths_per_tile = (uint32_t)kPathsPerTile;
+    B.timing.paths_per_tile = (uint32_t)(kPathsPerTile * B.best_ppl);
     B.timing.tiles_per_block = (uint32_t)B.target_units_per_cu;
+    B.timing.trades_per_item = (uint32_t)B.best_tpi;
 }
 
 }  // namespace
@@ -1880,6 +2186,25 @@ int eqhip_plan_execute(eqhip_plan* plan, const double* host_draws, const eqhip_c
     return guarded([&] {
         if (!plan) raise(EQHIP_E_INVALID, "plan is null");
         execute_plan(*plan, host_draws, credit, out);
+    });
+}
+
+int eqhip_plan_set_variant(eqhip_plan* plan, int32_t paths_per_lane, int32_t trades_per_item) {
+    return guarded([&] {
+        if (!plan) raise(EQHIP_E_INVALID, "plan is null");
+        const int p = paths_per_lane, t = trades_per_item;
+        if ((p != 1 && p != 2 && p != 4) || (t != 2 && t != 4 && t != 8))
+            raise(EQHIP_E_INVALID, "paths_per_lane must be 1, 2 or 4 and trades_per_item 2, 4 or 8");
+        if (!plan->parts.empty()) raise(EQHIP_E_INVALID, "a plan valued in parts keeps its parts' variant");
+        if (!variant_fits(*plan, p, t))
+            raise(EQHIP_E_CAPACITY, "variant paths_per_lane=" + std::to_string(p) + " trades_per_item=" +
+                                        std::to_string(t) + " does not fit this plan (LDS, work items or registers)");
+        HIPCHK(hipSetDevice(plan->ctx->device));
+        plan->ppl = p;
+        plan->tpi = t;
+        choose_launch(*plan);
+        build_value_plan(*plan);
+        plan->tuned = true;  // an explicit choice is not tuned over
     });
 }
 

@@ -302,10 +302,11 @@ __device__ void adjoint_path(const AdjArgs& g, uint64_t p, double acc[5]) {
             gb += nP * z.lnA_b;
             gs += nP * z.lnA_s;
         }
-        double grf[4];
-        int fslot[4];
-        int nm = 0;
-        for (int m = g.mid_off[k]; m < g.mid_off[k + 1] && nm < 4; ++m, ++nm) {
+        // mid-period legs N P(t, T_next) / P(t_fix, T_next): every entry of the date in the value, then,
+        // where the exposure is positive, their adjoints into the fixing rows (recomputed, so a date
+        // may hold any number of entries)
+        const int m0 = g.mid_off[k], m1 = g.mid_off[k + 1];
+        for (int m = m0; m < m1; ++m) {
             const int4 md = g.mid[m];
             const ZcbSens zn = zk[md.x];
             const ZcbSens zf = g.fix_zs[md.z];
@@ -318,8 +319,6 @@ __device__ void adjoint_path(const AdjArgs& g, uint64_t p, double acc[5]) {
             ga += term * ((zn.lnA_a - r * zn.B_a) - (zf.lnA_a - rf * zf.B_a));
             gb += term * (zn.lnA_b - zf.lnA_b);
             gs += term * (zn.lnA_s - zf.lnA_s);
-            grf[nm] = term * zf.B;
-            fslot[nm] = md.w;
         }
         if (V > 0.0) {
             vsum += w * V;
@@ -327,9 +326,14 @@ __device__ void adjoint_path(const AdjArgs& g, uint64_t p, double acc[5]) {
             abar += w * ga;
             bbar += w * gb;
             sbar += w * gs;
-#pragma unroll
-            for (int m = 0; m < 4; ++m)
-                if (m < nm) g.G[(uint64_t)fslot[m] * n + p] += w * grf[m];
+            for (int m = m0; m < m1; ++m) {
+                const int4 md = g.mid[m];
+                const ZcbSens zn = zk[md.x];
+                const ZcbSens zf = g.fix_zs[md.z];
+                const double rf = g.RF[(uint64_t)md.y * n + p];
+                const double term = g.mid_alpha[m] * ((zn.A * exp(-(zn.B * r))) / (zf.A * exp(-(zf.B * rf))));
+                g.G[(uint64_t)md.w * n + p] += w * (term * zf.B);
+            }
         }
     };
     auto step_inputs = [&](int j) {  // adjoints arriving at grid step j
@@ -457,8 +461,15 @@ template <class T> __device__ __forceinline__ cptr<T> cst(const T* p) { return (
 
 // NFAC = 2: G2++ plans; the state is (x, y) in (R, R2) and the bond factors are
 // P = exp((c - B(a,tau) x) - B(b,tau) y), with (c, B(a,tau)) in AB / fix_AB and B(b,tau) in AB2 / fix_B2.
-template <int NFAC>
+// PPL: paths per lane, so that a tile is TW = 64 PPL paths and every phase between two barriers
+// carries PPL independent chains per lane (the latency of each phase is paid once per TW paths).
+// TPI: trades per work item of the per-trade step (one LDS read of a class's legs per TPI trades).
+// The class legs and set values are formed with explicit fused multiply-adds in a fixed order, so
+// every (PPL, TPI) variant writes the same netting-set values bit for bit.
+template <int NFAC, int PPL, int TPI>
 __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, const int chunk, double* smem) {
+    constexpr int TW = kPathsPerTile * PPL;
+    constexpr int FU = (PPL >= 4) ? 1 : 4 / PPL;  // flows (classes) per step of the unrolled loops
     // the descriptor read through the constant address space: no store of the kernel can alias it,
     // so a field is a scalar load wherever it is used instead of a register held for the whole unit
     const __attribute__((address_space(4))) ValuePlan& g = *cst(&plan);
@@ -466,14 +477,13 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
-    double* sP = smem;                                     // [D][64]
-    double* sF = sP + (size_t)g.D * kPathsPerTile;         // [Fmax][64] reciprocal fixing factors
-    double2* sC = (double2*)(sF + (size_t)g.Fmax * kPathsPerTile);  // [64][C] path-major
+    double* sP = smem;                                     // [D][TW]
+    double* sF = sP + (size_t)g.D * TW;                    // [Fmax][TW] reciprocal fixing factors
+    double2* sC = (double2*)(sF + (size_t)g.Fmax * TW);    // [TW][C] path-major
     const int CS = g.C;
-    double* sPart = (double*)(sC + (size_t)g.C * kPathsPerTile);    // [S][waves][64] (split mode)
-    double* sSeg = sPart + (g.set_mode == 0 ? (size_t)g.S * kValueWaves * kPathsPerTile : 0);  // [waves][64]
+    double* sPart = (double*)(sC + (size_t)g.C * TW);      // [S][waves][TW] (split mode)
+    double* sSeg = sPart + (g.set_mode == 0 ? (size_t)g.S * kValueWaves * TW : 0);  // [waves][TW]
 
-    const double t = g.dates[k];
     const int dfirst = g.date_dfirst[k];
     const int n_live = g.D - dfirst;
     const int fbeg = g.fix_off[k];
@@ -487,40 +497,33 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     // Per-trade work. The items alive at t_k are a suffix of the maturity-sorted list (every item
     // when per-path values are requested); each gets ns equal slices of a tile's paths, ns as large
     // as the block's kTradeSlots work slots allow, so that the date's work is spread over every wave.
-    // Slot u = (slice u / n_li, live item u % n_li): the lanes of a wave read one path row of sC at
-    // consecutive items, whose class ids are consecutive (classes are numbered by maturity), so their
-    // reads fall in consecutive banks or share an address. Thread tid holds slots tid + i * kValueThreads.
+    // Slot tid = (slice tid / n_li, live item tid % n_li): the lanes of a wave read one path row of sC
+    // at consecutive items, whose class ids are consecutive (classes are numbered by maturity), so
+    // their reads fall in consecutive banks or share an address.
     const int item0 = !g.want_trade ? g.n_items : (g.path_mtm ? 0 : g.item_first[k]);
     const int n_li = g.n_items - item0;
-    const int ns = (n_li > 0) ? max(1, min(kPathsPerTile, kTradeSlots / n_li)) : 1;
+    const int ns = (n_li > 0) ? max(1, min(TW, kTradeSlots / n_li)) : 1;
     const int n_slots = n_li * ns;
-    double acc[kItemsPerThread][kTradesPerItem];
-    double tK[kItemsPerThread][kTradesPerItem];
-    int icls[kItemsPerThread], icnt[kItemsPerThread], ifirst[kItemsPerThread];
-    double isgn[kItemsPerThread];
-    int islice_b[kItemsPerThread], islice_e[kItemsPerThread];
-#pragma unroll
-    for (int i = 0; i < kItemsPerThread; ++i) {
-        const int u = tid + i * kValueThreads;
-        const int sl = (n_li > 0) ? u / n_li : 0;
-        const int li = u - sl * n_li;
-        islice_b[i] = (sl * kPathsPerTile) / ns;
-        islice_e[i] = ((sl + 1) * kPathsPerTile) / ns;
-        icnt[i] = 0;
-        icls[i] = 0;
-        ifirst[i] = 0;
-        isgn[i] = 1.0;
-        if (u < n_slots) {
+    double acc[TPI], tK[TPI];
+    int icls = 0, icnt = 0, ifirst = 0;
+    double isgn = 1.0;
+    int islice_b, islice_e;
+    {
+        const int sl = (n_li > 0) ? tid / n_li : 0;
+        const int li = tid - sl * n_li;
+        islice_b = (sl * TW) / ns;
+        islice_e = ((sl + 1) * TW) / ns;
+        if (tid < n_slots) {
             const int4 item = g.items[item0 + li];
-            icls[i] = item.x;
-            ifirst[i] = item.y;
-            isgn[i] = (double)item.w;
-            icnt[i] = item.z;
+            icls = item.x;
+            ifirst = item.y;
+            isgn = (double)item.w;
+            icnt = item.z;
         }
 #pragma unroll
-        for (int q = 0; q < kTradesPerItem; ++q) {
-            acc[i][q] = 0.0;
-            tK[i][q] = (q < icnt[i]) ? isgn[i] * g.tr_K[ifirst[i] + q] : 0.0;  // sign * K
+        for (int q = 0; q < TPI; ++q) {
+            acc[q] = 0.0;
+            tK[q] = (q < icnt) ? isgn * g.tr_K[ifirst + q] : 0.0;  // sign * K
         }
     }
 
@@ -533,7 +536,9 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     const int seg_b = uni(segd.x), seg_e = uni(segd.y), carry_lo = uni(segd.z), n_carry = uni(segd.w);
     __syncthreads();  // the previous unit of a persistent block may still read this LDS
     // classes already matured at t_k: zeroed once, no live flow writes them during this unit
-    for (int i = wave; i < n_zc; i += kValueWaves) sC[lane * CS + cst(g.zcls)[zb + i]] = make_double2(0.0, 0.0);
+    for (int i = wave; i < n_zc; i += kValueWaves)
+#pragma unroll
+        for (int pp = 0; pp < PPL; ++pp) sC[(lane + 64 * pp) * CS + cst(g.zcls)[zb + i]] = make_double2(0.0, 0.0);
 
     const uint64_t tile0 = (uint64_t)chunk * g.tiles_per_block;
     uint64_t tile_end = tile0 + g.tiles_per_block;
@@ -549,153 +554,202 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     const gptr<double> Yfix =
         (NFAC == 2 && wave < nfix) ? glb(g.R2 + (uint64_t)g.fix_slot[fbeg + wave] * g.r_stride) : nullptr;
     const double fb2_w = (NFAC == 2 && wave < nfix) ? g.fix_B2[fbeg + wave] : 0.0;
-    double r_next = 0.0, rf_next = 0.0, y_next = 0.0, yf_next = 0.0;
-    if (tile0 < tile_end) {
-        const uint64_t gq = tile0 * kPathsPerTile + lane;
-        if (gq < n_paths) {
-            r_next = Rk[gq];
-            if (Rfix) rf_next = Rfix[gq];
+    double r_next[PPL], rf_next[PPL], y_next[PPL], yf_next[PPL];
+    auto prefetch = [&](uint64_t tile, bool live) {  // r (and the fixing rates) of `tile` into the _next registers
+#pragma unroll
+        for (int pp = 0; pp < PPL; ++pp) {
+            const uint64_t gq = tile * TW + lane + 64 * pp;
+            const bool vq = live && gq < n_paths;
+            r_next[pp] = vq ? Rk[gq] : 0.0;
+            rf_next[pp] = (Rfix && vq) ? Rfix[gq] : 0.0;
             if (NFAC == 2) {
-                y_next = Yk[gq];
-                if (Yfix) yf_next = Yfix[gq];
+                y_next[pp] = vq ? Yk[gq] : 0.0;
+                yf_next[pp] = (Yfix && vq) ? Yfix[gq] : 0.0;
+            } else {
+                y_next[pp] = yf_next[pp] = 0.0;
             }
         }
-    }
+    };
+    prefetch(tile0, tile0 < tile_end);
     // A: one list over the waves, the fixings of periods in progress (entry f < nfix, taken by wave f;
     // sF holds 1 / P(T_fix, T_next | r_fix) = exp(B r_fix) / A, so that a family start multiplies) and
-    // then the zero-coupon bonds of the live cash-flow dates, for the tile starting at path pa.
-    auto stage_a = [&](uint64_t pa, double r, double rf_w, double y, double yf_w) {
-        const uint64_t gpa = pa + lane;
+    // then the zero-coupon bonds of the live cash-flow dates, for the tile starting at path pa, from
+    // the rates in the _next registers.
+    auto stage_a = [&](uint64_t pa) {
         if (NFAC == 2) {  // sF = 1 / P(T_fix, T_next | x_f, y_f), sP = P(t_k, T_d | x, y)
             const cptr<double> b2d = cst(g.AB2 + (size_t)k * g.D);
             for (int e = wave; e < nfix + n_live; e += kValueWaves) {
                 if (e < nfix) {
-                    double xf = rf_w, yf = yf_w, c = fab_w.x, b1 = fab_w.y, b2 = fb2_w;
+                    double c = fab_w.x, b1 = fab_w.y, b2 = fb2_w;
+                    uint64_t row = 0;
                     if (e != wave) {  // more fixings than waves
-                        const uint64_t row = (uint64_t)glb(g.fix_slot)[fbeg + e] * g.r_stride;
-                        xf = (gpa < n_paths) ? glb(g.R)[row + gpa] : 0.0;
-                        yf = (gpa < n_paths) ? glb(g.R2)[row + gpa] : 0.0;
+                        row = (uint64_t)glb(g.fix_slot)[fbeg + e] * g.r_stride;
                         const gptr<double> fp = glb((const double*)(g.fix_AB + fbeg + e));
                         c = fp[0];
                         b1 = fp[1];
                         b2 = glb(g.fix_B2)[fbeg + e];
                     }
-                    sF[e * kPathsPerTile + lane] = exp(-((c - b1 * xf) - b2 * yf));
+#pragma unroll
+                    for (int pp = 0; pp < PPL; ++pp) {
+                        const uint64_t gpa = pa + lane + 64 * pp;
+                        double xf = rf_next[pp], yf = yf_next[pp];
+                        if (e != wave) {
+                            xf = (gpa < n_paths) ? glb(g.R)[row + gpa] : 0.0;
+                            yf = (gpa < n_paths) ? glb(g.R2)[row + gpa] : 0.0;
+                        }
+                        sF[e * TW + lane + 64 * pp] = exp(-((c - b1 * xf) - b2 * yf));
+                    }
                 } else {
                     const int d = dfirst + (e - nfix);
-                    sP[d * kPathsPerTile + lane] = exp((abd[2 * d] - abd[2 * d + 1] * r) - b2d[d] * y);
+#pragma unroll
+                    for (int pp = 0; pp < PPL; ++pp)
+                        sP[d * TW + lane + 64 * pp] = exp((abd[2 * d] - abd[2 * d + 1] * r_next[pp]) - b2d[d] * y_next[pp]);
                 }
             }
             return;
         }
         for (int e = wave; e < nfix + n_live; e += kValueWaves) {
             if (e < nfix) {
-                double rf = rf_w, inv_a = inv_a_w, b = fab_w.y;
+                double inv_a = inv_a_w, b = fab_w.y;
+                uint64_t row = 0;
                 if (e != wave) {  // more fixings than waves
-                    rf = (gpa < n_paths) ? glb(g.R)[(uint64_t)glb(g.fix_slot)[fbeg + e] * g.r_stride + gpa] : 0.0;
+                    row = (uint64_t)glb(g.fix_slot)[fbeg + e] * g.r_stride;
                     const gptr<double> fp = glb((const double*)(g.fix_AB + fbeg + e));
                     inv_a = 1.0 / fp[0];
                     b = fp[1];
                 }
-                sF[e * kPathsPerTile + lane] = inv_a * exp(b * rf);
+#pragma unroll
+                for (int pp = 0; pp < PPL; ++pp) {
+                    const uint64_t gpa = pa + lane + 64 * pp;
+                    double rf = rf_next[pp];
+                    if (e != wave) rf = (gpa < n_paths) ? glb(g.R)[row + gpa] : 0.0;
+                    sF[e * TW + lane + 64 * pp] = inv_a * exp(b * rf);
+                }
             } else {
                 const int d = dfirst + (e - nfix);
-                sP[d * kPathsPerTile + lane] = abd[2 * d] * exp(-(abd[2 * d + 1] * r));
+#pragma unroll
+                for (int pp = 0; pp < PPL; ++pp) sP[d * TW + lane + 64 * pp] = abd[2 * d] * exp(-(abd[2 * d + 1] * r_next[pp]));
             }
         }
     };
-    if (tile0 < tile_end) stage_a(tile0 * kPathsPerTile, r_next, rf_next, y_next, yf_next);
+    // a netting-set value of this unit: stored as its positive part (and the signed value when asked),
+    // or for a part of a plan valued in parts, added to the earlier parts' values at the same place
+    const int x_mode = g.x_mode;
+    auto store_set = [&](int s, uint64_t gp, double v) {
+        const uint64_t xi = ((uint64_t)s * g.n_dates + k) * n_paths + gp;
+        if (x_mode >= 2) v = glb(g.X)[xi] + v;
+        glb(g.X)[xi] = (x_mode == 1 || x_mode == 2) ? v : ((v > 0.0) ? v : 0.0);
+        if (g.set_mtm && (x_mode == 0 || x_mode == 3)) glb(g.set_mtm)[((uint64_t)k * g.S + s) * n_paths + gp] = v;
+    };
+    if (tile0 < tile_end) stage_a(tile0 * TW);
     __syncthreads();
     // Per tile: B pass 1 | B pass 2 | C, D and A of the next tile | set reduction, with a barrier after
     // each of the first three. A of the next tile can run beside C and D: it writes only sP and sF,
     // which this tile last read in pass 1.
     for (uint64_t tile = tile0; tile < tile_end; ++tile) {
-        const uint64_t p0 = tile * kPathsPerTile;
-        const uint64_t gp = p0 + lane;
-        const bool valid = gp < n_paths;
+        const uint64_t p0 = tile * TW;
+        // next tile's short rates: in flight during passes 1 and 2, used by the next tile's A
+        prefetch(tile + 1, tile + 1 < tile_end);
         // B: schedule families. Every class's (floating leg, annuity) is a running sum over its
         // family's live flows; the date's flows are split into one contiguous segment per wave, each
         // summed from zero, and a second pass adds the totals of the earlier segments of the same family.
-        {  // next tile's short rates: in flight during passes 1 and 2, used by the next tile's A
-            const uint64_t gq = gp + kPathsPerTile;
-            const bool vq = (tile + 1 < tile_end) && gq < n_paths;
-            r_next = vq ? Rk[gq] : 0.0;
-            if (Rfix) rf_next = vq ? Rfix[gq] : 0.0;
-            if (NFAC == 2) {
-                y_next = vq ? Yk[gq] : 0.0;
-                if (Yfix) yf_next = vq ? Yfix[gq] : 0.0;
-            }
-        }
         {
-            double run = 0.0, st = 0.0;
+            double run[PPL], st[PPL];
+#pragma unroll
+            for (int pp = 0; pp < PPL; ++pp) run[pp] = st[pp] = 0.0;
             int fam = -1;
             auto family_start = [&](int f) {  // P(t, start) or the carried fixing of the period in progress
-                const int4 fd = make_int4(fdc[4 * f], 0, fdc[4 * f + 2], fdc[4 * f + 3]);
-                if (fd.x == 1) return 1.0;
-                if (fd.x == 2) return sP[fd.z * kPathsPerTile + lane];
-                return sP[fd.z * kPathsPerTile + lane] * sF[fd.w * kPathsPerTile + lane];
+                const int mode = fdc[4 * f], dz = fdc[4 * f + 2], fw = fdc[4 * f + 3];
+#pragma unroll
+                for (int pp = 0; pp < PPL; ++pp) {
+                    const int lp = lane + 64 * pp;
+                    st[pp] = (mode == 1) ? 1.0
+                                         : (mode == 2 ? sP[dz * TW + lp] : sP[dz * TW + lp] * sF[fw * TW + lp]);
+                    run[pp] = 0.0;
+                }
             };
             int i = seg_b;
-            for (; i + 4 <= seg_e; i += 4) {  // all LDS reads of four flows first, then the chain
-                int4 e[4];
-                double tau[4], P[4];
+            for (; i + FU <= seg_e; i += FU) {  // all LDS reads of FU flows first, then the chain
+                int4 e[FU];
+                double tau[FU], P[FU][PPL];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < FU; ++u) {
                     e[u] = make_int4(lf[4 * (i + u)], lf[4 * (i + u) + 1], lf[4 * (i + u) + 2], 0);
                     tau[u] = lt[i + u];
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) P[u] = sP[e[u].x * kPathsPerTile + lane];
+                for (int u = 0; u < FU; ++u)
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                    for (int pp = 0; pp < PPL; ++pp) P[u][pp] = sP[e[u].x * TW + lane + 64 * pp];
+#pragma unroll
+                for (int u = 0; u < FU; ++u) {
                     if (e[u].z != fam) {
                         fam = e[u].z;
-                        run = 0.0;
-                        st = family_start(fam);
+                        family_start(fam);
                     }
-                    run += P[u] * tau[u];
-                    if (e[u].y >= 0) sC[lane * CS + e[u].y] = make_double2(st - P[u], run);
+#pragma unroll
+                    for (int pp = 0; pp < PPL; ++pp) run[pp] = __builtin_fma(P[u][pp], tau[u], run[pp]);
+                    if (e[u].y >= 0)
+#pragma unroll
+                        for (int pp = 0; pp < PPL; ++pp)
+                            sC[(lane + 64 * pp) * CS + e[u].y] = make_double2(st[pp] - P[u][pp], run[pp]);
                 }
             }
             for (; i < seg_e; ++i) {
                 const int4 e = make_int4(lf[4 * i], lf[4 * i + 1], lf[4 * i + 2], 0);
                 if (e.z != fam) {
                     fam = e.z;
-                    run = 0.0;
-                    st = family_start(fam);
+                    family_start(fam);
                 }
-                const double P = sP[e.x * kPathsPerTile + lane];
-                run += P * lt[i];
-                if (e.y >= 0) sC[lane * CS + e.y] = make_double2(st - P, run);
+                const double tau = lt[i];
+#pragma unroll
+                for (int pp = 0; pp < PPL; ++pp) {
+                    const double P = sP[e.x * TW + lane + 64 * pp];
+                    run[pp] = __builtin_fma(P, tau, run[pp]);
+                    if (e.y >= 0) sC[(lane + 64 * pp) * CS + e.y] = make_double2(st[pp] - P, run[pp]);
+                }
             }
-            sSeg[wave * kPathsPerTile + lane] = run;
+#pragma unroll
+            for (int pp = 0; pp < PPL; ++pp) sSeg[wave * TW + lane + 64 * pp] = run[pp];
         }
         __syncthreads();
         if (n_carry > 0) {  // the totals of the earlier segments of this segment's first family
-            double carry = 0.0;
+            double carry[PPL];
+#pragma unroll
+            for (int pp = 0; pp < PPL; ++pp) carry[pp] = 0.0;
             int w2 = carry_lo;
-            for (; w2 + 4 <= wave; w2 += 4) {  // four reads in flight, added in order
-                double t4[4];
+            for (; w2 + FU <= wave; w2 += FU) {  // FU reads per path in flight, added in order
+                double t4[FU][PPL];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) t4[u] = sSeg[(w2 + u) * kPathsPerTile + lane];
+                for (int u = 0; u < FU; ++u)
 #pragma unroll
-                for (int u = 0; u < 4; ++u) carry += t4[u];
+                    for (int pp = 0; pp < PPL; ++pp) t4[u][pp] = sSeg[(w2 + u) * TW + lane + 64 * pp];
+#pragma unroll
+                for (int u = 0; u < FU; ++u)
+#pragma unroll
+                    for (int pp = 0; pp < PPL; ++pp) carry[pp] += t4[u][pp];
             }
-            for (; w2 < wave; ++w2) carry += sSeg[w2 * kPathsPerTile + lane];
-            for (int i = seg_b; i < seg_b + n_carry; i += 4) {  // reads of four classes, then the writes
-                int cls[4];
-                double a[4];
+            for (; w2 < wave; ++w2)
 #pragma unroll
-                for (int u = 0; u < 4; ++u) cls[u] = (i + u < seg_b + n_carry) ? lf[4 * (i + u) + 1] : -1;
+                for (int pp = 0; pp < PPL; ++pp) carry[pp] += sSeg[w2 * TW + lane + 64 * pp];
+            for (int i = seg_b; i < seg_b + n_carry; i += FU) {  // reads of FU classes, then the writes
+                int cls[FU];
+                double a[FU][PPL];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) a[u] = (cls[u] >= 0) ? sC[lane * CS + cls[u]].y : 0.0;
+                for (int u = 0; u < FU; ++u) cls[u] = (i + u < seg_b + n_carry) ? lf[4 * (i + u) + 1] : -1;
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    if (cls[u] >= 0) sC[lane * CS + cls[u]].y = carry + a[u];
+                for (int u = 0; u < FU; ++u)
+#pragma unroll
+                    for (int pp = 0; pp < PPL; ++pp) a[u][pp] = (cls[u] >= 0) ? sC[(lane + 64 * pp) * CS + cls[u]].y : 0.0;
+#pragma unroll
+                for (int u = 0; u < FU; ++u)
+                    if (cls[u] >= 0)
+#pragma unroll
+                        for (int pp = 0; pp < PPL; ++pp) sC[(lane + 64 * pp) * CS + cls[u]].y = carry[pp] + a[u][pp];
             }
         }
         __syncthreads();
-        // C: netting sets
+        // C: netting sets, V_s = sum over the set's live classes of alpha F - beta A, in list order
         if (g.set_mode == 0) {
             for (int s = 0; s < g.S; ++s) {
                 // the set's classes alive at t (a prefix of its entries), split evenly over the waves
@@ -706,77 +760,94 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                 const int ce = min(end, cb + per);
                 const cptr<int> scl = cst(g.set_cls);
                 const cptr<double> sal = cst(g.set_alpha), sbe = cst(g.set_beta);
-                double v = 0.0;
+                double v[PPL];
+#pragma unroll
+                for (int pp = 0; pp < PPL; ++pp) v[pp] = 0.0;
                 int e = cb;
-                for (; e + 4 <= ce; e += 4) {
-                    double2 c2[4];
+                for (; e + FU <= ce; e += FU) {
+                    double2 c2[FU][PPL];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) c2[u] = sC[lane * CS + scl[e + u]];
+                    for (int u = 0; u < FU; ++u)
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) v += sal[e + u] * c2[u].x - sbe[e + u] * c2[u].y;
+                        for (int pp = 0; pp < PPL; ++pp) c2[u][pp] = sC[(lane + 64 * pp) * CS + scl[e + u]];
+#pragma unroll
+                    for (int u = 0; u < FU; ++u)
+#pragma unroll
+                        for (int pp = 0; pp < PPL; ++pp)
+                            v[pp] = __builtin_fma(-sbe[e + u], c2[u][pp].y, __builtin_fma(sal[e + u], c2[u][pp].x, v[pp]));
                 }
-                for (; e < ce; ++e) {
-                    const double2 c2 = sC[lane * CS + scl[e]];
-                    v += sal[e] * c2.x - sbe[e] * c2.y;
-                }
-                sPart[(s * kValueWaves + wave) * kPathsPerTile + lane] = v;
+                for (; e < ce; ++e)
+#pragma unroll
+                    for (int pp = 0; pp < PPL; ++pp) {
+                        const double2 c2 = sC[(lane + 64 * pp) * CS + scl[e]];
+                        v[pp] = __builtin_fma(-sbe[e], c2.y, __builtin_fma(sal[e], c2.x, v[pp]));
+                    }
+#pragma unroll
+                for (int pp = 0; pp < PPL; ++pp) sPart[(s * kValueWaves + wave) * TW + lane + 64 * pp] = v[pp];
             }
         } else {
+            // one wave per set, summing the same kValueWaves chunks in the same order as the split
+            // path (each chunk from zero, then the chunk sums in chunk order): the same values bit for bit
             for (int s = wave; s < g.S; s += kValueWaves) {
-                double v = 0.0;
                 const int beg = cst(g.set_off)[s], end = beg + cst(g.set_nlive)[(size_t)k * g.S + s];
-                for (int e = beg; e < end; ++e) {
-                    const double2 c2 = sC[lane * CS + cst(g.set_cls)[e]];
-                    v += cst(g.set_alpha)[e] * c2.x - cst(g.set_beta)[e] * c2.y;
-                }
-                if (valid) {
-                    glb(g.X)[((uint64_t)s * g.n_dates + k) * n_paths + gp] = (v > 0.0) ? v : 0.0;
-                    if (g.set_mtm) glb(g.set_mtm)[((uint64_t)k * g.S + s) * n_paths + gp] = v;
+                const int per = (end - beg + kValueWaves - 1) / kValueWaves;
+#pragma unroll
+                for (int pp = 0; pp < PPL; ++pp) {
+                    double v = 0.0;
+                    for (int w = 0; w < kValueWaves; ++w) {
+                        const int cb = beg + w * per, ce = min(end, cb + per);
+                        double vw = 0.0;
+                        for (int e = cb; e < ce; ++e) {
+                            const double2 c2 = sC[(lane + 64 * pp) * CS + cst(g.set_cls)[e]];
+                            vw = __builtin_fma(-cst(g.set_beta)[e], c2.y, __builtin_fma(cst(g.set_alpha)[e], c2.x, vw));
+                        }
+                        v += vw;
+                    }
+                    const uint64_t gp = p0 + lane + 64 * pp;
+                    if (gp < n_paths) store_set(s, gp, v);
                 }
             }
         }
         // D: per-trade exposure sums (reads sC only); work unit = (item, slice of the tile's paths)
-        if (g.want_trade) {
+        if (g.want_trade && icnt > 0) {
             const uint64_t left = n_paths - p0;
-            const int pvalid = (left < (uint64_t)kPathsPerTile) ? (int)left : kPathsPerTile;
-#pragma unroll
-            for (int i = 0; i < kItemsPerThread; ++i) {
-                if (icnt[i] == 0) continue;
-                const double2* col = sC + icls[i];
-                const double sg = isgn[i];
-                int pb = islice_b[i], pe = min(islice_e[i], pvalid);
-                if (det) pe = (p0 == 0 && pb == 0) ? 1 : pb;  // grid index 0: path 0 only
+            const int pvalid = (left < (uint64_t)TW) ? (int)left : TW;
+            const double2* col = sC + icls;
+            const double sg = isgn;
+            const int pb = islice_b;
+            int pe = min(islice_e, pvalid);
+            if (det) pe = (p0 == 0 && pb == 0) ? 1 : pb;  // grid index 0: path 0 only
 #pragma unroll 4
-                for (int p = pb; p < pe; ++p) {
-                    const double2 c2 = col[p * CS];
-                    const double sfl = sg * c2.x;  // sign * F - (sign * K) A, exact sign flips
+            for (int p = pb; p < pe; ++p) {
+                const double2 c2 = col[p * CS];
+                const double sfl = sg * c2.x;  // sign * F - (sign * K) A, exact sign flips
 #pragma unroll
-                    for (int q = 0; q < kTradesPerItem; ++q) {
-                        const double x = fma(-tK[i][q], c2.y, sfl);
-                        acc[i][q] += (x > 0.0) ? x : 0.0;
-                    }
+                for (int q = 0; q < TPI; ++q) {
+                    const double x = __builtin_fma(-tK[q], c2.y, sfl);
+                    acc[q] += (x > 0.0) ? x : 0.0;
                 }
-                if (g.path_mtm) {
-                    for (int q = 0; q < icnt[i]; ++q) {
-                        const int j = ifirst[i] + q;
-                        const double N = g.tr_N[j], NK = g.tr_NK[j];
-                        const gmut<double> dst = glb(g.path_mtm + ((uint64_t)k * g.n_trades + g.tr_orig[j]) * n_paths + p0);
-                        for (int p = islice_b[i]; p < min(islice_e[i], pvalid); ++p)
-                            dst[p] = N * col[p * CS].x - NK * col[p * CS].y;
-                    }
+            }
+            if (g.path_mtm) {
+                for (int q = 0; q < icnt; ++q) {
+                    const int j = ifirst + q;
+                    const double N = g.tr_N[j], NK = g.tr_NK[j];
+                    const gmut<double> dst = glb(g.path_mtm + ((uint64_t)k * g.n_trades + g.tr_orig[j]) * n_paths + p0);
+                    for (int p = islice_b; p < min(islice_e, pvalid); ++p)
+                        dst[p] = N * col[p * CS].x - NK * col[p * CS].y;
                 }
             }
         }
-        if (tile + 1 < tile_end) stage_a(p0 + kPathsPerTile, r_next, rf_next, y_next, yf_next);
+        if (tile + 1 < tile_end) stage_a(p0 + TW);
         __syncthreads();
         if (g.set_mode == 0) {
             if (tid < g.S * kPathsPerTile) {
                 const int s = tid >> 6;
-                double v = 0.0;
-                for (int w = 0; w < kValueWaves; ++w) v += sPart[(s * kValueWaves + w) * kPathsPerTile + lane];
-                if (valid) {
-                    glb(g.X)[((uint64_t)s * g.n_dates + k) * n_paths + gp] = (v > 0.0) ? v : 0.0;
-                    if (g.set_mtm) glb(g.set_mtm)[((uint64_t)k * g.S + s) * n_paths + gp] = v;
+#pragma unroll
+                for (int pp = 0; pp < PPL; ++pp) {
+                    double v = 0.0;
+                    for (int w = 0; w < kValueWaves; ++w) v += sPart[(s * kValueWaves + w) * TW + lane + 64 * pp];
+                    const uint64_t gp = p0 + lane + 64 * pp;
+                    if (gp < n_paths) store_set(s, gp, v);
                 }
             }
         }
@@ -784,27 +855,23 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     if (g.want_trade) {
         // the slots' sums through LDS, then one fixed-order sum over the slices of each trade
         __syncthreads();
-        double* sAcc = smem;  // [kTradeSlots][kTradesPerItem]
+        double* sAcc = smem;  // [kTradeSlots][TPI]
+        if (tid < n_slots)
 #pragma unroll
-        for (int i = 0; i < kItemsPerThread; ++i) {
-            const int u = tid + i * kValueThreads;
-            if (u < n_slots)
-#pragma unroll
-                for (int q = 0; q < kTradesPerItem; ++q) sAcc[u * kTradesPerItem + q] = acc[i][q];
-        }
+            for (int q = 0; q < TPI; ++q) sAcc[tid * TPI + q] = acc[q];
         __syncthreads();
         const gmut<double> part = glb(g.partial + ((uint64_t)k * g.n_chunks + chunk) * g.n_trades);
-        for (int w = tid; w < g.n_items * kTradesPerItem; w += kValueThreads) {
-            const int it = w / kTradesPerItem;
-            const int q = w - it * kTradesPerItem;
+        for (int w = tid; w < g.n_items * TPI; w += kValueThreads) {
+            const int it = w / TPI;
+            const int q = w - it * TPI;
             const int4 item = g.items[it];
             if (q >= item.z) continue;
             const int j = item.y + q;
             double e = 0.0;  // a matured class contributes exact zeros
             if (it >= item0) {
-                const double* a = sAcc + (it - item0) * kTradesPerItem + q;
+                const double* a = sAcc + (it - item0) * TPI + q;
                 double sum = 0.0;
-                for (int sl = 0; sl < ns; ++sl) sum += a[sl * n_li * kTradesPerItem];
+                for (int sl = 0; sl < ns; ++sl) sum += a[sl * n_li * TPI];
                 e = g.tr_absN[j] * sum;
             }
             part[g.tr_orig[j]] = e;
@@ -812,18 +879,40 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     }
 }
 
+// The plan's (PPL, TPI) variant, a uniform switch: one launch can hold plans of several variants.
+// Four paths per lane come with two trades per item (more would not fit the registers of a
+// 1024-thread block); two-factor plans take one path per lane.
+template <int NFAC>
+__device__ __forceinline__ void value_dispatch(const ValuePlan& plan, int k, int chunk, double* smem) {
+    switch (__builtin_amdgcn_readfirstlane(cst(&plan)->variant)) {
+        case kVariant(1, 2): value_unit<NFAC, 1, 2>(plan, k, chunk, smem); break;
+        case kVariant(1, 8): value_unit<NFAC, 1, 8>(plan, k, chunk, smem); break;
+        case kVariant(2, 2): if constexpr (NFAC == 1) value_unit<1, 2, 2>(plan, k, chunk, smem); break;
+        case kVariant(2, 4): if constexpr (NFAC == 1) value_unit<1, 2, 4>(plan, k, chunk, smem); break;
+        case kVariant(2, 8): if constexpr (NFAC == 1) value_unit<1, 2, 8>(plan, k, chunk, smem); break;
+        case kVariant(4, 2): if constexpr (NFAC == 1) value_unit<1, 4, 2>(plan, k, chunk, smem); break;
+        default: value_unit<NFAC, 1, 4>(plan, k, chunk, smem); break;
+    }
+}
+
+// The same dispatch as a call of its own, so that the persistent loop's state does not add to the
+// registers of the largest variant.
+__device__ __attribute__((noinline)) void value_dispatch_call(const ValuePlan* plan, int k, int chunk, double* smem) {
+    value_dispatch<1>(*plan, k, chunk, smem);
+}
+
 // One block per work unit; the hardware dispatcher hands out units in list order.
 __global__ void __launch_bounds__(kValueThreads) value_kernel(const ValuePlan* plans, const int4* units) {
     extern __shared__ double smem[];
     const int4 u = units[blockIdx.x];
-    value_unit<1>(plans[u.x], u.y, u.z, smem);
+    value_dispatch<1>(plans[u.x], u.y, u.z, smem);
 }
 
 // The same for two-factor (G2++) plans.
 __global__ void __launch_bounds__(kValueThreads) value2_kernel(const ValuePlan* plans, const int4* units) {
     extern __shared__ double smem[];
     const int4 u = units[blockIdx.x];
-    value_unit<2>(plans[u.x], u.y, u.z, smem);
+    value_dispatch<2>(plans[u.x], u.y, u.z, smem);
 }
 
 // One thread per path, blockIdx.y the date. The bonds of the date's remaining flows are summed in
@@ -870,7 +959,7 @@ __global__ void __launch_bounds__(kValueThreads) value_kernel_persistent(const V
         const int idx = s_unit;
         if (idx >= n_units) break;
         const int4 u = units[idx];
-        value_unit<1>(plans[u.x], u.y, u.z, smem);
+        value_dispatch_call(plans + u.x, u.y, u.z, smem);
     }
 }
 

@@ -8,10 +8,12 @@ namespace eqhip {
 
 constexpr int kValueThreads = 1024;  // 16 wave64s per valuation block
 constexpr int kValueWaves = kValueThreads / 64;
-constexpr int kPathsPerTile = 64;    // one path per lane
-constexpr int kItemsPerThread = 1;   // trade work items per thread in the valuation block
-constexpr int kTradesPerItem = 4;    // trades of one schedule class and sign per work item
-constexpr int kTradeSlots = kValueThreads * kItemsPerThread;  // (item, path slice) work slots per block
+constexpr int kPathsPerTile = 64;    // paths of a tile per path-per-lane: a tile is 64 * PPL paths
+constexpr int kTradeSlots = kValueThreads;  // (item, path slice) work slots per block, one per thread
+// Valuation variants: PPL paths per lane (1, 2, 4) and TPI trades of one class and sign per work
+// item (2, 4, 8); four paths per lane take two trades per item. Two-factor plans use PPL = 1.
+__host__ __device__ constexpr int kVariant(int ppl, int tpi) { return ppl * 16 + tpi; }
+constexpr int kDefaultPpl = 1, kDefaultTpi = 4;
 constexpr int kMaxSplitSets = 8;     // netting sets whose class sums are split across the waves
 constexpr int kSelectThreads = 1024;  // EE / PFE select block
 constexpr int kPvLdsPillars = 256;    // curves up to this many pillars are staged in LDS by pv_dv01_kernel
@@ -58,6 +60,9 @@ struct ValuePlan {
     int n_dates, D, C, NF, S, Fmax;
     int n_chunks, tiles_per_block;
     int set_mode;             // 0: S <= kMaxSplitSets, class sums split across waves; 1: one wave per set
+    int variant;              // kVariant(PPL, TPI) of the valuation unit
+    int x_mode;               // 0: X = max(V, 0) and set_mtm = V; a part of a plan valued in parts: 1 stores V
+                              //    in X, 2 adds V to X, 3 adds, stores max(total, 0) and writes the total
     int two_factor;           // 1: G2++ (R holds x, R2 holds y; P = exp((c - B1 x) - B2 y) with AB = (c, B1),
                               //    AB2 = B2; fixings likewise with fix_AB, fix_B2)
     const double* R2;         // [n_slots][r_stride] second factor (two-factor plans)
@@ -106,16 +111,16 @@ struct ValuePlan {
     double* path_mtm;         // optional [n_dates][n_trades][n_paths]
 };
 
-// Dynamic LDS of one value_kernel block: sP [D][64], sF [Fmax][64], sC [64][C] (double2),
-// set partials [S_split][waves][64] and segment totals [waves][64] (the per-date descriptor tables
-// are read from global memory through the scalar cache).
-__host__ __device__ inline size_t value_lds_bytes(int D, int Fmax, int C, int S_split) {
-    return 8 * ((size_t)kPathsPerTile * (D + Fmax) + 2 * (size_t)kPathsPerTile * C +
-                (size_t)kPathsPerTile * kValueWaves * S_split + (size_t)kPathsPerTile * kValueWaves);
+// Dynamic LDS of one value_kernel block for a tile of TW = 64 PPL paths: sP [D][TW], sF [Fmax][TW],
+// sC [TW][C] (double2), set partials [S_split][waves][TW] and segment totals [waves][TW] (the per-date
+// descriptor tables are read from global memory through the scalar cache).
+__host__ __device__ inline size_t value_lds_bytes(int D, int Fmax, int C, int S_split, int ppl = 1) {
+    const size_t tw = (size_t)kPathsPerTile * ppl;
+    return 8 * (tw * (D + Fmax) + 2 * tw * C + tw * kValueWaves * S_split + tw * kValueWaves);
 }
 
 // LDS that holds the per-slot trade sums of a unit for their reduction (floor of the allocation)
-__host__ __device__ inline size_t value_trade_scratch_bytes() { return 8 * (size_t)kTradeSlots * kTradesPerItem; }
+__host__ __device__ inline size_t value_trade_scratch_bytes(int tpi) { return 8 * (size_t)kTradeSlots * tpi; }
 
 struct PvArgs {
     const double* pillars;

@@ -39,10 +39,10 @@ EXPORTED = [
     "eqhip_unilateral_cva_from_exposure_2f", "eqhip_unilateral_cva_from_exposure_2f_batch", "eqhip_plan_create_2f",
     "eqhip_xva_calculate", "eqhip_plan_exposure_cube", "eqhip_plan_xva", "eqhip_scenarios_create",
     "eqhip_scenarios_destroy", "eqhip_irs_hull_white_exposure_profile_batch_on", "eqhip_pv_dv01_at",
-    "eqhip_unilateral_cva_from_exposure_batch_noalloc",
+    "eqhip_unilateral_cva_from_exposure_batch_noalloc", "eqhip_plan_set_variant",
 ]
 
-PLAN_TRADE_EE, PLAN_KEEP_CUBE = 1, 2  # flags of eqhip_plan_create's want_trade_ee argument
+PLAN_TRADE_EE, PLAN_KEEP_CUBE, PLAN_AUTOTUNE = 1, 2, 4  # flags of eqhip_plan_create's want_trade_ee argument
 
 
 class EqhipError(RuntimeError):
@@ -96,7 +96,8 @@ class XvaParamsC(ctypes.Structure):
 class TimingC(ctypes.Structure):
     _fields_ = [("simulate_ms", _D), ("value_ms", _D), ("reduce_ms", _D), ("cva_ms", _D),
                 ("total_ms", _D), ("n_value_blocks", _U64), ("paths_per_tile", _U32),
-                ("tiles_per_block", _U32), ("xva_rows_ms", _D), ("xva_select_ms", _D)]
+                ("tiles_per_block", _U32), ("xva_rows_ms", _D), ("xva_select_ms", _D), ("trades_per_item", _U32),
+                ("n_parts", _U32)]
 
 
 _lib = None
@@ -149,6 +150,7 @@ def lib():
                                          ctypes.POINTER(OutputsC)]
         L.eqhip_plan_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(TimingC)]
         L.eqhip_plan_destroy.argtypes = [ctypes.c_void_p]
+        L.eqhip_plan_set_variant.argtypes = [ctypes.c_void_p, _I32, _I32]
         L.eqhip_exposure_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(ModelHW1F),
                                          ctypes.POINTER(PortfolioC), _PD, _I32, ctypes.POINTER(McC), _PD,
                                          ctypes.POINTER(CreditC), ctypes.POINTER(OutputsC)]
@@ -254,7 +256,7 @@ class Plan:
     """
 
     def __init__(self, ctx, model, portfolio: Portfolio, dates, n_paths, n_steps, seed=0,
-                 rng=RNG_PHILOX, disc=DISC_EULER, want_trade_ee=False, keep_cube=False):
+                 rng=RNG_PHILOX, disc=DISC_EULER, want_trade_ee=False, keep_cube=False, autotune=False):
         self.ctx = ctx
         self.portfolio = portfolio
         self.dates = _f64(dates)
@@ -268,7 +270,8 @@ class Plan:
         create = lib().eqhip_plan_create_2f if self.factors == 2 else lib().eqhip_plan_create
         mc = McC(self.n_paths, self.n_steps, int(seed), rng, disc)
         h = ctypes.c_void_p()
-        flags = (PLAN_TRADE_EE if want_trade_ee else 0) | (PLAN_KEEP_CUBE if keep_cube else 0)
+        flags = ((PLAN_TRADE_EE if want_trade_ee else 0) | (PLAN_KEEP_CUBE if keep_cube else 0) |
+                 (PLAN_AUTOTUNE if autotune else 0))
         _check(create(ctx.handle, ctypes.byref(m), ctypes.byref(self._pc), _dp(self.dates), len(self.dates),
                       ctypes.byref(mc), flags, ctypes.byref(h)))
         self.handle = h
@@ -330,6 +333,10 @@ class Plan:
         t = TimingC()
         _check(lib().eqhip_plan_timing(self.handle, ctypes.byref(t)))
         return {f: getattr(t, f) for f, _ in TimingC._fields_}
+
+    def set_variant(self, paths_per_lane, trades_per_item):
+        """Valuation variant (eqhip_plan_set_variant): paths per lane 1 / 2 / 4, trades per item 2 / 4 / 8."""
+        _check(lib().eqhip_plan_set_variant(self.handle, int(paths_per_lane), int(trades_per_item)))
 
     def close(self):
         if getattr(self, "handle", None):

@@ -39,6 +39,9 @@ extern "C" {
 #define EQHIP_PLAN_TRADE_EE 1  /* per-trade EE / CVA outputs */
 #define EQHIP_PLAN_KEEP_CUBE 2 /* keep the signed netting-set values [date][set][path] resident
                                   after each execute (eqhip_plan_exposure_cube, eqhip_plan_xva) */
+#define EQHIP_PLAN_AUTOTUNE 4  /* the first execute times the valuation's variants (paths per lane
+                                  1 / 2 / 4, trades per work item 2 / 4 / 8) that fit the plan on its
+                                  own paths and keeps the fastest; results do not depend on it */
 
 typedef struct eqhip_context eqhip_context;
 typedef struct eqhip_plan eqhip_plan;
@@ -111,6 +114,9 @@ typedef struct {
     uint32_t tiles_per_block;
     double xva_rows_ms;   /* eqhip_plan_xva: CSA adjustment, positive parts and ENE sums of the cube rows */
     double xva_select_ms; /* eqhip_plan_xva: EE sums and the quantile of every row */
+    uint32_t trades_per_item; /* trades of one class and sign per work item of the per-trade step */
+    uint32_t n_parts;         /* value plans the valuation runs in (more than one for a portfolio whose
+                                 schedules do not fit one block: groups of classes, one after another) */
 } eqhip_timing;
 
 /* ---- library / context -------------------------------------------------------------------- */
@@ -296,6 +302,14 @@ int eqhip_plan_execute(eqhip_plan* plan, const double* host_draws, const eqhip_c
                        eqhip_outputs* out);
 
 int eqhip_plan_timing(const eqhip_plan* plan, eqhip_timing* out);
+
+/* Sets the plan's valuation variant: paths_per_lane (1, 2, 4: a tile of 64 x that many paths) and
+ * trades_per_item (2, 4, 8). Refused with EQHIP_E_CAPACITY when it does not fit the plan (LDS, work
+ * items; four paths per lane take two trades per item; two-factor plans one path per lane). Results
+ * do not depend on the variant: netting-set values bit for bit, per-trade EE to rounding. No
+ * counterpart in the reference (it has no GPU valuation kernel); it fixes what EQHIP_PLAN_AUTOTUNE
+ * would otherwise choose. */
+int eqhip_plan_set_variant(eqhip_plan* plan, int32_t paths_per_lane, int32_t trades_per_item);
 int eqhip_plan_destroy(eqhip_plan* plan);
 
 /* One-shot convenience: plan_create + plan_execute + plan_destroy. */

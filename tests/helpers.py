@@ -25,6 +25,38 @@ def small_portfolio(n_trades=12, n_sets=3, seed=3, max_years=6, start=0.0):
                      pay=np.concatenate(pays), acc=np.concatenate(accs))
 
 
+def diverse_book(n_trades=600, n_sets=2, seed=11, max_years=10):
+    """A book of mostly distinct schedules on the monthly grid: forward starts up to 3y ahead, trades
+    already running whose first period is a short stub (the rest of a period that began before today,
+    paid at a random month), and spot starts; frequencies 1, 2, 4; tenors up to max_years."""
+    rng = np.random.default_rng(seed)
+    pays, accs, off, starts = [], [], [0], []
+    for _ in range(n_trades):
+        f = int(rng.choice([1, 2, 4]))
+        n_per = int(rng.integers(1, max_years * f + 1))
+        kind = rng.uniform()
+        if kind < 0.4:
+            s0 = int(rng.integers(1, 37)) / 12.0
+            pay = s0 + np.arange(1, n_per + 1) / f
+        elif kind < 0.8:
+            s0 = 0.0
+            stub = int(rng.integers(1, 12 // f)) / 12.0 if f < 12 else 1.0 / 12.0
+            pay = stub + np.arange(0, n_per) / f
+        else:
+            s0 = 0.0
+            pay = np.arange(1, n_per + 1) / f
+        pay = np.round(pay * 12.0) / 12.0
+        acc = np.diff(np.concatenate([[s0], pay]))
+        pays.append(pay)
+        accs.append(acc)
+        starts.append(s0)
+        off.append(off[-1] + n_per)
+    notional = np.exp(rng.normal(np.log(1e6), 0.7, n_trades)) * np.where(rng.uniform(size=n_trades) < 0.5, 1, -1)
+    return Portfolio(set_id=rng.integers(0, n_sets, n_trades).astype(np.int32), notional=notional,
+                     fixed_rate=0.02 + rng.normal(0, 0.005, n_trades), start=np.array(starts),
+                     flow_off=np.array(off, dtype=np.int32), pay=np.concatenate(pays), acc=np.concatenate(accs))
+
+
 def leg_scale(port, model, dates):
     """|notional| * (1 + sum of accruals * |K|) per trade: the magnitude of the swap's legs, the
     scale of its mark-to-market rounding."""

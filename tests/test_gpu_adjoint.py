@@ -7,7 +7,7 @@ import pytest
 
 import oracle
 from engine_quant_amd import eqhip
-from helpers import MODEL, small_portfolio
+from helpers import MODEL, diverse_book, small_portfolio
 
 pytestmark = pytest.mark.gpu
 
@@ -74,6 +74,34 @@ def test_adjoint_matches_oracle_differences_on_shared_draws(ctx):
         fd = (up - dn) / (2 * h)
         assert abs(grad[name] - fd) <= 2e-3 * abs(fd) + 1e-9, (name, grad[name], fd)
     plan.close()
+
+
+def test_adjoint_with_many_periods_in_progress(ctx):
+    """Forward starts, running stubs (periods fixed at t = 0) and monthly dates: dozens of families in the
+    middle of a period at every date, each a term P(t, T_next) / P(t_fix, T_next) whose adjoint reaches
+    the short rate at its fixing step. The adjoint's own forward value and its gradient against oracle
+    central differences on shared draws."""
+    port = diverse_book(60, n_sets=1, seed=5, max_years=4)
+    dates = np.arange(1, 49) / 12.0
+    n_paths, n_steps = 2000, 48
+    z = oracle.philox_normals(8, n_paths, n_steps)
+    plan = ctx.plan(MODEL, port, dates, n_paths, n_steps, rng=eqhip.RNG_HOST_DRAWS, disc=eqhip.DISC_EXACT)
+    cva, grad, check = plan.cva_adjoint(0, H, RR, host_draws=z)
+    plan.close()
+
+    def ocva(model):
+        return oracle.portfolio_exposure_fused(model, port.as_dict(), dates, n_steps, z=z, scheme=1, hazard_rate=H,
+                                               recovery_rate=RR)["set_cva"][0]
+
+    base = ocva(MODEL)
+    assert abs(cva - base) <= 1e-10 * abs(base)
+    assert abs(check - base) <= 1e-10 * abs(base)
+    for i, name in enumerate(("a", "b", "sigma", "r0")):
+        m_up, m_dn = list(MODEL), list(MODEL)
+        m_up[i] += BUMPS[name]
+        m_dn[i] -= BUMPS[name]
+        fd = (ocva(tuple(m_up)) - ocva(tuple(m_dn))) / (2 * BUMPS[name])
+        assert abs(grad[name] - fd) <= 2e-3 * abs(fd) + 1e-9 * abs(cva), (name, grad[name], fd)
 
 
 def test_pv_pillar_and_model_sensitivities(ctx):

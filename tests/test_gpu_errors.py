@@ -91,22 +91,26 @@ def test_execute_validation(ctx):
 
 
 def test_capacity_limits(ctx):
-    # 200 distinct schedules (400 cash-flow dates): more LDS than a block has
+    # G2++: 200 distinct schedules (400 cash-flow dates) need more LDS than a block has, and two-factor
+    # plans are not valued in parts
+    from engine_quant_amd.portfolios import MODEL2
     n = 200
     start = np.arange(n) / 1000.0
     pay = np.stack([start + 1.0, start + 2.0], axis=1).ravel()
     port = eqhip.Portfolio(set_id=np.zeros(n, np.int32), notional=np.full(n, 1e6), fixed_rate=np.full(n, 0.02),
                            start=start, flow_off=np.arange(0, 2 * n + 1, 2, dtype=np.int32), pay=pay,
                            acc=np.ones(2 * n))
-    _raises(eqhip.E_CAPACITY, "too diverse", ctx.plan, MODEL, port, [0.5, 1.5], 64, 4)
-    # per-trade EE: more work items (4 same-signed trades of a class each) than the block has threads
-    n = 4 * 1024 + 1
+    _raises(eqhip.E_CAPACITY, "too diverse", ctx.plan, MODEL2, port, [0.5, 1.5], 64, 4)
+    # a batch job must fit one block as well
+    _raises(eqhip.E_CAPACITY, "too diverse", eqhip.Batch, ctx, MODEL, [(port, 64)], [0.5, 1.5], 4)
+    # per-trade EE of a G2++ plan: more work items (8 same-signed trades of a class each) than threads
+    n = 8 * 1024 + 1
     port = eqhip.Portfolio(set_id=np.zeros(n, np.int32), notional=np.full(n, 1e6), fixed_rate=np.full(n, 0.02),
                            start=np.zeros(n), flow_off=np.arange(n + 1, dtype=np.int32), pay=np.ones(n),
                            acc=np.ones(n))
-    _raises(eqhip.E_CAPACITY, "per-trade EE supports at most", ctx.plan, MODEL, port, [0.5], 64, 4,
+    _raises(eqhip.E_CAPACITY, "per-trade EE supports at most", ctx.plan, MODEL2, port, [0.5], 64, 4,
             want_trade_ee=True)
-    ctx.plan(MODEL, port, [0.5], 64, 4).close()  # the same portfolio without per-trade EE is fine
+    ctx.plan(MODEL2, port, [0.5], 64, 4).close()  # the same portfolio without per-trade EE is fine
 
 
 def test_reference_shaped_entry_point_validation(ctx):
