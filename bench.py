@@ -814,6 +814,7 @@ def main():
     ap.add_argument("--model", choices=["hw1f", "hw2f"], default="hw1f",
                     help="short-rate model of the swaps workload: HullWhite1F (default) or HullWhite2F (G2++)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-trade-pfe", action="store_true", help="skip the per-trade PFE95 measurement")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", choices=["swaps", "batch", "adjoint", "bermudan", "xva", "reference-case",
                                            "caller", "greeks"],
@@ -967,6 +968,32 @@ def main():
                      "blocks": tm["n_value_blocks"]},
         "set_cva": float(res["set_cva"][0]),
     }
+    if factors == 1 and not args.no_trade_pfe:
+        # per-trade PFE95 beside the headline outputs, outside its timed region: the same job with the
+        # class legs written by the valuation and one exact select per live (trade, date)
+        plan_pfe = ctx.plan(model, port, dates, args.paths, n_steps, seed=args.seed, rng=eqhip.RNG_PHILOX, disc=disc,
+                            want_trade_ee=True, want_trade_pfe=True)
+        runs = []
+        for i in range(4):
+            t1 = time.perf_counter()
+            r_pfe = plan_pfe.execute(None, hazard, recovery, want_cva=True)
+            wall = time.perf_counter() - t1
+            if i > 0:
+                runs.append((plan_pfe.timing(), wall))
+        plan_pfe.close()
+        assert np.array_equal(r_pfe["set_ee"], res["set_ee"]) and np.array_equal(r_pfe["set_pfe"], res["set_pfe"])
+        live_rows = int(u_live // args.paths)
+        pfe_ms = float(np.mean([t["trade_pfe_ms"] for t, _ in runs]))
+        out["trade_pfe"] = {
+            "trade_select_ms": pfe_ms,
+            "value_kernel_with_class_legs_ms": float(np.mean([t["value_ms"] for t, _ in runs])),
+            "exposure_kernels_ms": float(np.mean([t["total_ms"] + t["trade_pfe_ms"] for t, _ in runs])),
+            "wall_ms_per_execute": 1000.0 * float(np.mean([w for _, w in runs])),
+            "date_groups": int(runs[-1][0]["trade_pfe_groups"]),
+            "live_trade_date_rows": live_rows,
+            "note": "exact rank-ceil(0.95 n)-1 order statistic of every live (trade, date) over all paths; "
+                    "runs 1-3 after one warm-up execute",
+        }
     if not args.no_cpu:
         if factors == 2:
             out["cpu_baseline"], ref, n_ref = cpu_baseline_2f(port, dates, n_steps, model, args.seed, disc,

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <set>
@@ -159,9 +160,24 @@ struct eqhip_plan {
     std::vector<int4> unit_list;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     eqhip_timing timing{};
+    // per-trade PFE95 (EQHIP_PLAN_TRADE_PFE): the valuation of a group of dates writes the legs of the
+    // classes alive at those dates into ccube ([date of the group][class][path], the classes of the
+    // parts one after another), then one select row per (trade, date) with the trade alive
+    bool want_pfe = false;
+    std::vector<int> h_trade_cls;        // class of each trade of this plan's portfolio
+    std::vector<double> h_cls_mat;       // last payment of each class (classes are numbered by it)
+    std::vector<int> h_whole_index;      // a part: its trades' indices in the whole plan
+    double2* ccube_p = nullptr;          // the whole plan's class-leg buffer (a part points into it)
+    int cube_C = 0, cube_cls0 = 0;       // classes of the buffer; this value plan's first class in it
+    DevBuf ccube, cube_row, cls_live0, trade_rows, trade_pfe;
+    std::vector<int> grp_k;              // first date of each date group, then n_dates
+    std::vector<size_t> grp_row;         // first select row of each group, then the row count
+    std::vector<hipEvent_t> pfe_ev;      // before and after each group's select launch
 
     ~eqhip_plan() {
         for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto& e : pfe_ev)
             if (e) (void)hipEventDestroy(e);
     }
 };
@@ -280,6 +296,7 @@ void build_parts(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portf
                  int n_dates, const eqhip_mc* mc, const std::vector<std::vector<double>>& cls_dates,
                  const std::vector<int>& trade_cls);
 void build_select_rows(eqhip_plan& P);
+void build_trade_pfe(eqhip_plan& P, const eqhip_portfolio* pf);
 
 // all_steps: r is stored at every grid step (slot = step), so plans of a batch can share one
 // simulation; the batch then owns the short-rate buffer. A part of a partitioned plan (P.whole set)
@@ -313,6 +330,9 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
     P.want_trade = (want_trade & EQHIP_PLAN_TRADE_EE) ? 1 : 0;
     P.keep_cube = (want_trade & EQHIP_PLAN_KEEP_CUBE) != 0;
     P.autotune = (want_trade & EQHIP_PLAN_AUTOTUNE) != 0;
+    P.want_pfe = (want_trade & EQHIP_PLAN_TRADE_PFE) != 0;
+    if (P.want_pfe && P.factors == 2) raise(EQHIP_E_INVALID, "per-trade PFE serves HullWhite1F plans only");
+    if (whole_index) P.h_whole_index = *whole_index;
     hipStream_t st = P.ctx->stream;
 
     // time grid (exposure.rs:91-116): dt = t_max / n_steps, idx = round(t / dt)
@@ -412,6 +432,8 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
     for (int c = 0; c < P.C; ++c) flow_cls[fam_off[cls_fam[c]] + cls_len[c] - 1] = c;
     std::vector<double> cls_mat(P.C);
     for (int c = 0; c < P.C; ++c) cls_mat[c] = classes[c].pay.back();
+    P.h_trade_cls = trade_cls;
+    P.h_cls_mat = cls_mat;
 
     // slots: grid indices at which r is stored
     std::map<int64_t, int> slot_of;
@@ -769,6 +791,7 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
     } else {
         build_value_plan(P);
     }
+    if (P.want_pfe) build_trade_pfe(P, pf);
     if (!is_part)
         for (auto& e : P.ev) HIPCHK(hipEventCreate(&e));
 }
@@ -921,6 +944,11 @@ void build_value_plan(eqhip_plan& P) {
     g.tr_NK = P.tr_NK.as<double>();
     g.tr_orig = P.tr_orig.as<int>();
     g.partial = W.partial.as<double>();
+    g.ccube = P.ccube_p;
+    g.cube_row = P.cube_row.as<int>();
+    g.cls_live0 = P.cls_live0.as<int>();
+    g.cube_C = P.cube_C;
+    g.cube_cls0 = P.cube_cls0;
     P.vplan.alloc(sizeof(ValuePlan));
     HIPCHK(hipMemcpy(P.vplan.p, &g, sizeof(ValuePlan), hipMemcpyHostToDevice));
     P.unit_list.clear();
@@ -944,11 +972,98 @@ void build_select_rows(eqhip_plan& P) {
     upload(P.rows, rows, P.ctx->stream);
 }
 
-void launch_value(eqhip_plan& P, double* path_mtm, double* set_mtm) {
+// Per-trade PFE95 rows of a plan created with EQHIP_PLAN_TRADE_PFE. The trades of one schedule class
+// differ only in notional and fixed rate, so a trade's value on a path is N F - N K A with (F, A) the
+// legs of its class: the valuation writes the legs of the classes alive at each date of a group of
+// dates (16 B per class, path and date instead of 8 B per trade, path and date), and a select row per
+// live (trade, date) forms the trade's positive parts from them in each of its passes. Rows of one
+// date and class are adjacent, so the class's trades read its legs from the caches. A group holds as
+// many dates as a quarter of the free device memory allows (or EQHIP_TRADE_PFE_BYTES, when smaller).
+void build_trade_pfe(eqhip_plan& P, const eqhip_portfolio* pf) {
+    std::vector<eqhip_plan*> vps;  // the value plans: the parts, or the plan itself
+    if (P.parts.empty()) vps.push_back(&P);
+    for (auto& q : P.parts) vps.push_back(q.get());
+    std::vector<int> cube_cls(P.n_trades, 0);
+    std::vector<double> mat(P.n_trades, 0.0);
+    int C_total = 0;
+    for (eqhip_plan* Q : vps) {
+        for (size_t i = 0; i < Q->h_trade_cls.size(); ++i) {
+            const int j = Q->whole ? Q->h_whole_index[i] : (int)i;
+            cube_cls[j] = C_total + Q->h_trade_cls[i];
+            mat[j] = Q->h_cls_mat[Q->h_trade_cls[i]];
+        }
+        Q->cube_cls0 = C_total;
+        C_total += Q->C;
+    }
+    const uint64_t np = P.mc.n_paths;
+    const int nd = P.n_dates;
+    const double per_date = 16.0 * (double)C_total * (double)np;
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    double budget = 0.25 * (double)free_b;
+    if (const char* cap = std::getenv("EQHIP_TRADE_PFE_BYTES")) {  // a smaller buffer: more groups
+        const double c = std::atof(cap);
+        if (c > 0.0) budget = std::min(budget, c);
+    }
+    const double fit = std::floor(budget / per_date);
+    if (fit < 1.0)
+        raise(EQHIP_E_CAPACITY, "per-trade PFE needs " + std::to_string(per_date / 1e9) +
+                                    " GB of device memory per date for the legs of " + std::to_string(C_total) +
+                                    " schedule classes, " + std::to_string(free_b / 1e9) + " GB free");
+    const int G = (int)std::min<double>((double)nd, fit);
+    P.ccube.alloc((size_t)G * C_total * np * 16);
+    P.grp_k.clear();
+    for (int k0 = 0; k0 < nd; k0 += G) P.grp_k.push_back(k0);
+    P.grp_k.push_back(nd);
+    std::vector<int> cube_row(nd);
+    for (int k = 0; k < nd; ++k) cube_row[k] = k % G;
+    for (eqhip_plan* Q : vps) {
+        std::vector<int> live0(nd);
+        for (int k = 0; k < nd; ++k) {
+            int c = 0;
+            while (c < Q->C && !(Q->h_cls_mat[c] > P.dates[k] + kDateTolHost)) ++c;
+            live0[k] = c;
+        }
+        upload(Q->cls_live0, live0, P.ctx->stream);
+        upload(Q->cube_row, cube_row, P.ctx->stream);
+        Q->ccube_p = P.ccube.as<double2>();
+        Q->cube_C = C_total;
+        build_value_plan(*Q);  // the descriptor with the cube fields
+    }
+    P.trade_pfe.alloc((size_t)P.n_trades * nd * 8);
+    HIPCHK(hipMemsetAsync(P.trade_pfe.p, 0, (size_t)P.n_trades * nd * 8, P.ctx->stream));  // matured trades
+    std::vector<TradeSelectRow> rows;
+    std::vector<int> live;
+    P.grp_row.assign(1, 0);
+    for (size_t gi = 0; gi + 1 < P.grp_k.size(); ++gi) {
+        for (int k = P.grp_k[gi]; k < P.grp_k[gi + 1]; ++k) {
+            live.clear();
+            for (int j = 0; j < P.n_trades; ++j)
+                if (mat[j] > P.dates[k] + kDateTolHost) live.push_back(j);
+            std::stable_sort(live.begin(), live.end(), [&](int x, int y) { return cube_cls[x] < cube_cls[y]; });
+            for (int j : live) {
+                const double s = pf->notional[j] < 0.0 ? -1.0 : 1.0;
+                rows.push_back(TradeSelectRow{
+                    P.ccube.as<double2>() + ((uint64_t)cube_row[k] * C_total + cube_cls[j]) * np, P.h_neff[k],
+                    P.h_qrank[k], s, s * pf->fixed_rate[j], std::fabs(pf->notional[j]),
+                    P.trade_pfe.as<double>() + (size_t)j * nd + k});
+            }
+        }
+        P.grp_row.push_back(rows.size());
+    }
+    upload(P.trade_rows, rows, P.ctx->stream);
+    P.pfe_ev.assign(2 * (P.grp_k.size() - 1), nullptr);
+    for (auto& e : P.pfe_ev) HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipStreamSynchronize(P.ctx->stream));
+}
+
+// The valuation of dates [k0, k1) (every date when k1 < 0); cube: write the class legs for the
+// per-trade PFE rows.
+void launch_value(eqhip_plan& P, double* path_mtm, double* set_mtm, int k0 = 0, int k1 = -1, bool cube = false) {
     hipStream_t st = P.ctx->stream;
     if (!P.parts.empty()) {  // the parts in order; the last one writes the signed set values
         for (size_t i = 0; i < P.parts.size(); ++i)
-            launch_value(*P.parts[i], path_mtm, i + 1 == P.parts.size() ? set_mtm : nullptr);
+            launch_value(*P.parts[i], path_mtm, i + 1 == P.parts.size() ? set_mtm : nullptr, k0, k1, cube);
         return;
     }
     if (path_mtm != P.vp.path_mtm || set_mtm != P.vp.set_mtm) {  // per-run outputs live in the descriptor
@@ -956,12 +1071,15 @@ void launch_value(eqhip_plan& P, double* path_mtm, double* set_mtm) {
         P.vp.set_mtm = set_mtm;
         HIPCHK(hipMemcpy(P.vplan.p, &P.vp, sizeof(ValuePlan), hipMemcpyHostToDevice));
     }
-    if (P.factors == 2)
-        hipLaunchKernelGGL(value2_kernel, dim3((unsigned)P.unit_list.size()), dim3(kValueThreads), P.lds_bytes, st,
-                           P.vplan.as<const ValuePlan>(), P.units.as<const int4>());
-    else
-        hipLaunchKernelGGL(value_kernel, dim3((unsigned)P.unit_list.size()), dim3(kValueThreads), P.lds_bytes, st,
-                           P.vplan.as<const ValuePlan>(), P.units.as<const int4>());
+    if (k1 < 0) k1 = P.n_dates;
+    const size_t u0 = (size_t)k0 * P.n_chunks, nu = (size_t)(k1 - k0) * P.n_chunks;
+    if (nu == 0) return;
+    if (cube && (P.ppl != 1 || P.factors != 1 || !P.ccube_p))
+        raise(EQHIP_E_HIP, "internal: class legs are written by one-factor plans at one path per lane");
+    auto kernel = (P.factors == 2) ? value2_kernel
+                                   : (cube ? value_cube_kernel : (P.ppl == 1 ? value_p1_kernel : value_kernel));
+    hipLaunchKernelGGL(kernel, dim3((unsigned)nu), dim3(kValueThreads), P.lds_bytes, st, P.vplan.as<const ValuePlan>(),
+                       P.units.as<const int4>() + u0);
     HIPCHK(hipGetLastError());
 }
 
@@ -1110,6 +1228,7 @@ void tune_plan(eqhip_plan& P) {
         for (int tpi : tpis) {
             // without a per-trade step TPI does not matter: one variant per PPL
             if (!P.want_trade && tpi != (ppl == 4 ? 2 : 4)) continue;
+            if (P.want_pfe && ppl != 1) continue;  // the class legs are written at one path per lane
             if (!variant_fits(P, ppl, tpi)) continue;
             P.ppl = ppl;
             P.tpi = tpi;
@@ -1138,6 +1257,8 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
     if ((out->trade_ee || out->trade_cva) && !P.want_trade)
         raise(EQHIP_E_INVALID, "per-trade outputs need a plan created with want_trade_ee");
     if ((out->set_cva || out->trade_cva) && !credit) raise(EQHIP_E_INVALID, "CVA needs credit data");
+    if (out->trade_pfe && !P.want_pfe)
+        raise(EQHIP_E_INVALID, "trade_pfe needs a plan created with EQHIP_PLAN_TRADE_PFE");
     hipStream_t st = P.ctx->stream;
     HIPCHK(hipSetDevice(P.ctx->device));
     const uint64_t np = P.mc.n_paths;
@@ -1168,10 +1289,22 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
         tune_plan(P);  // the paths are simulated: time the variants on them, then value with the fastest
         HIPCHK(hipEventRecord(P.ev[1], st));
     }
-    if (P.flat && !out->path_mtm && !out->set_mtm)
+    if (out->trade_pfe) {  // per date group: the valuation with the class legs, then the trades' rows
+        for (size_t gi = 0; gi + 1 < P.grp_k.size(); ++gi) {
+            launch_value(P, dpath.as<double>(), set_values, P.grp_k[gi], P.grp_k[gi + 1], true);
+            HIPCHK(hipEventRecord(P.pfe_ev[2 * gi], st));
+            const size_t r0 = P.grp_row[gi], r1 = P.grp_row[gi + 1];
+            if (r1 > r0)
+                hipLaunchKernelGGL(trade_select_kernel, dim3((unsigned)(r1 - r0)), dim3(kSelectThreads), 0, st,
+                                   P.trade_rows.as<const TradeSelectRow>() + r0);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipEventRecord(P.pfe_ev[2 * gi + 1], st));
+        }
+    } else if (P.flat && !out->path_mtm && !out->set_mtm) {
         launch_value_flat(P);
-    else
+    } else {
         launch_value(P, dpath.as<double>(), set_values);
+    }
     HIPCHK(hipEventRecord(P.ev[2], st));
     hipLaunchKernelGGL(select_kernel, dim3((unsigned)(P.S * P.n_dates)), dim3(kSelectThreads), 0, st,
                        P.rows.as<const SelectRow>());
@@ -1205,11 +1338,12 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
         const DevBuf* src;
         size_t bytes, off;
     };
-    Out outs[5] = {{out->set_ee, &P.set_ee, (size_t)P.S * P.n_dates * 8, 0},
+    Out outs[6] = {{out->set_ee, &P.set_ee, (size_t)P.S * P.n_dates * 8, 0},
                    {out->set_pfe, &P.set_pfe, (size_t)P.S * P.n_dates * 8, 0},
                    {out->set_cva, &P.set_cva, (size_t)P.S * 8, 0},
                    {out->trade_ee, &P.trade_ee, (size_t)P.n_trades * P.n_dates * 8, 0},
-                   {out->trade_cva, &P.trade_cva, (size_t)P.n_trades * 8, 0}};
+                   {out->trade_cva, &P.trade_cva, (size_t)P.n_trades * 8, 0},
+                   {out->trade_pfe, &P.trade_pfe, (size_t)P.n_trades * P.n_dates * 8, 0}};
     size_t staged = 0;
     for (Out& o : outs)
         if (o.dst) {
@@ -1237,8 +1371,17 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
 
     float ms[4];
     for (int i = 0; i < 4; ++i) HIPCHK(hipEventElapsedTime(&ms[i], P.ev[i], P.ev[i + 1]));
+    double pfe_ms = 0.0;
+    if (out->trade_pfe)
+        for (size_t gi = 0; 2 * gi + 1 < P.pfe_ev.size(); ++gi) {
+            float m = 0.0f;
+            HIPCHK(hipEventElapsedTime(&m, P.pfe_ev[2 * gi], P.pfe_ev[2 * gi + 1]));
+            pfe_ms += m;
+        }
     P.timing.simulate_ms = ms[0];
-    P.timing.value_ms = ms[1];
+    P.timing.value_ms = ms[1] - pfe_ms;
+    P.timing.trade_pfe_ms = pfe_ms;
+    P.timing.trade_pfe_groups = out->trade_pfe ? (uint32_t)(P.grp_k.size() - 1) : 0u;
     P.timing.reduce_ms = ms[2];
     P.timing.cva_ms = ms[3];
     P.timing.total_ms = (double)ms[0] + ms[1] + ms[2] + ms[3];
@@ -1980,6 +2123,7 @@ void execute_batch(eqhip_batch& B, const eqhip_credit* credit, eqhip_outputs* ou
         if ((outs[j].trade_ee || outs[j].trade_cva) && !B.jobs[j]->want_trade)
             raise(EQHIP_E_INVALID, "per-trade outputs need a batch created with want_trade_ee");
         if (outs[j].path_mtm || outs[j].set_mtm) raise(EQHIP_E_INVALID, "per-path outputs are not produced by batches");
+        if (outs[j].trade_pfe) raise(EQHIP_E_INVALID, "per-trade PFE is not produced by batches");
     }
     if (!B.tuned) autotune_batch(B);
     hipStream_t st = B.ctx->stream;
@@ -2196,6 +2340,8 @@ int eqhip_plan_set_variant(eqhip_plan* plan, int32_t paths_per_lane, int32_t tra
         if ((p != 1 && p != 2 && p != 4) || (t != 2 && t != 4 && t != 8))
             raise(EQHIP_E_INVALID, "paths_per_lane must be 1, 2 or 4 and trades_per_item 2, 4 or 8");
         if (!plan->parts.empty()) raise(EQHIP_E_INVALID, "a plan valued in parts keeps its parts' variant");
+        if (plan->want_pfe && p != 1)
+            raise(EQHIP_E_INVALID, "a plan with per-trade PFE values one path per lane");
         if (!variant_fits(*plan, p, t))
             raise(EQHIP_E_CAPACITY, "variant paths_per_lane=" + std::to_string(p) + " trades_per_item=" +
                                         std::to_string(t) + " does not fit this plan (LDS, work items or registers)");
@@ -2233,8 +2379,9 @@ int eqhip_exposure_run(eqhip_context* ctx, const eqhip_model_hw1f* model, const 
         P.ctx = ctx;
         P.device = ctx->device;
         P.direct_out = true;
-        int want_trade = out && (out->trade_ee || out->trade_cva || out->path_mtm);
-        build_plan(P, model, portfolio, dates, n_dates, mc, want_trade);
+        int flags = (out && (out->trade_ee || out->trade_cva || out->path_mtm)) ? EQHIP_PLAN_TRADE_EE : 0;
+        if (out && out->trade_pfe) flags |= EQHIP_PLAN_TRADE_PFE;
+        build_plan(P, model, portfolio, dates, n_dates, mc, flags);
         execute_plan(P, host_draws, credit, out);
     });
 }

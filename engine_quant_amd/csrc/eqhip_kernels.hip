@@ -466,7 +466,7 @@ template <class T> __device__ __forceinline__ cptr<T> cst(const T* p) { return (
 // TPI: trades per work item of the per-trade step (one LDS read of a class's legs per TPI trades).
 // The class legs and set values are formed with explicit fused multiply-adds in a fixed order, so
 // every (PPL, TPI) variant writes the same netting-set values bit for bit.
-template <int NFAC, int PPL, int TPI>
+template <int NFAC, int PPL, int TPI, bool CUBE = false>
 __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, const int chunk, double* smem) {
     constexpr int TW = kPathsPerTile * PPL;
     constexpr int FU = (PPL >= 4) ? 1 : 4 / PPL;  // flows (classes) per step of the unrolled loops
@@ -808,6 +808,21 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                 }
             }
         }
+        // the legs of the classes alive at t_k, for the per-trade PFE rows (reads sC only)
+        if constexpr (CUBE) {
+            const gmut<double> cube =
+                glb((double*)(g.ccube + ((uint64_t)g.cube_row[k] * g.cube_C + g.cube_cls0) * n_paths));
+            for (int c = g.cls_live0[k] + wave; c < g.C; c += kValueWaves)
+#pragma unroll
+                for (int pp = 0; pp < PPL; ++pp) {
+                    const uint64_t gp = p0 + lane + 64 * pp;
+                    if (gp < n_paths) {
+                        const double2 v = sC[(lane + 64 * pp) * CS + c];
+                        cube[2 * ((uint64_t)c * n_paths + gp)] = v.x;
+                        cube[2 * ((uint64_t)c * n_paths + gp) + 1] = v.y;
+                    }
+                }
+        }
         // D: per-trade exposure sums (reads sC only); work unit = (item, slice of the tile's paths)
         if (g.want_trade && icnt > 0) {
             const uint64_t left = n_paths - p0;
@@ -906,6 +921,31 @@ __global__ void __launch_bounds__(kValueThreads) value_kernel(const ValuePlan* p
     extern __shared__ double smem[];
     const int4 u = units[blockIdx.x];
     value_dispatch<1>(plans[u.x], u.y, u.z, smem);
+}
+
+// A plan's own launch at one path per lane: only those variants, so that the registers of the wider
+// ones do not weigh on them.
+__global__ void __launch_bounds__(kValueThreads) value_p1_kernel(const ValuePlan* plans, const int4* units) {
+    extern __shared__ double smem[];
+    const int4 u = units[blockIdx.x];
+    const ValuePlan& plan = plans[u.x];
+    switch (__builtin_amdgcn_readfirstlane(cst(&plan)->variant)) {
+        case kVariant(1, 2): value_unit<1, 1, 2>(plan, u.y, u.z, smem); break;
+        case kVariant(1, 8): value_unit<1, 1, 8>(plan, u.y, u.z, smem); break;
+        default: value_unit<1, 1, 4>(plan, u.y, u.z, smem); break;
+    }
+}
+
+// The same, writing the legs of the live classes for the per-trade PFE rows (one path per lane).
+__global__ void __launch_bounds__(kValueThreads) value_cube_kernel(const ValuePlan* plans, const int4* units) {
+    extern __shared__ double smem[];
+    const int4 u = units[blockIdx.x];
+    const ValuePlan& plan = plans[u.x];
+    switch (__builtin_amdgcn_readfirstlane(cst(&plan)->variant)) {
+        case kVariant(1, 2): value_unit<1, 1, 2, true>(plan, u.y, u.z, smem); break;
+        case kVariant(1, 8): value_unit<1, 1, 8, true>(plan, u.y, u.z, smem); break;
+        default: value_unit<1, 1, 4, true>(plan, u.y, u.z, smem); break;
+    }
 }
 
 // The same for two-factor (G2++) plans.
@@ -1030,22 +1070,41 @@ __device__ __forceinline__ void hist_add(unsigned int* hist, unsigned int b) {
 // Visits x[i] for i = start, start + T, start + 2T, ... in that order, with eight loads in flight
 // per thread (a pass over a row is latency-bound with one).
 constexpr int kSelectU = 8;
-template <class F>
-__device__ __forceinline__ void select_pass(const unsigned long long* x, uint64_t n, uint64_t start, F f) {
+template <class Keys, class F>
+__device__ __forceinline__ void select_pass(const Keys& keys, uint64_t n, uint64_t start, F f) {
     constexpr int U = kSelectU;
-    const gptr<unsigned long long> gx = glb(x);
     uint64_t i = start;
     for (; i + (uint64_t)(U - 1) * kSelectThreads < n; i += (uint64_t)U * kSelectThreads) {
         unsigned long long k[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) k[u] = gx[i + (uint64_t)u * kSelectThreads];
+        for (int u = 0; u < U; ++u) k[u] = keys(i + (uint64_t)u * kSelectThreads);
 #pragma unroll
         for (int u = 0; u < U; ++u) f(k[u]);
     }
-    for (; i < n; i += kSelectThreads) f(gx[i]);
+    for (; i < n; i += kSelectThreads) f(keys(i));
 }
 
-__global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow* rows) {
+// Key sources of a select row: the bits of non-negative doubles, which order as the doubles do.
+struct RowKeys {  // a row of positive parts
+    const unsigned long long* x;
+    __device__ unsigned long long operator()(uint64_t i) const { return glb(x)[i]; }
+    __device__ double out(double v) const { return v; }
+};
+struct TradeKeys {  // a trade's positive parts, formed from its class's legs as the per-trade EE forms them
+    const double2* c;
+    double s, sK, absN;
+    __device__ unsigned long long operator()(uint64_t i) const {
+        const gptr<double> legs = glb((const double*)c);
+        const double F = legs[2 * i], A = legs[2 * i + 1];
+        const double x = __builtin_fma(-sK, A, s * F);
+        return (unsigned long long)__double_as_longlong(x > 0.0 ? x : 0.0);
+    }
+    __device__ double out(double v) const { return absN * v; }
+};
+
+// EE (the mean, when ee is not null) and the rank-q order statistic of one row of n keys.
+template <class Keys>
+__device__ void select_row(const Keys& keys, const uint64_t n, const uint64_t q, double* ee, double* pfe) {
     __shared__ unsigned int hist[4096];
     __shared__ unsigned int scan[kSelectThreads];
     __shared__ double red[kSelectThreads];
@@ -1057,9 +1116,6 @@ __global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow*
     __shared__ unsigned int s_bin, s_count, s_ncand;
 
     const int tid = threadIdx.x;
-    const SelectRow row = rows[blockIdx.x];
-    const uint64_t n = row.n;
-    const unsigned long long* x = (const unsigned long long*)row.x;
 
     for (int b = tid; b < 4096; b += kSelectThreads) hist[b] = 0u;
     for (int b = tid; b < kSelectWin * 1024; b += kSelectThreads) hist2[b] = 0u;
@@ -1072,14 +1128,14 @@ __global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow*
 #pragma unroll
     for (int u = 0; u < kSelectU; ++u) {
         const uint64_t i = (uint64_t)tid + (uint64_t)u * kSelectThreads;
-        k0[u] = (i < n) ? glb(x)[i] : 0ull;
+        k0[u] = (i < n) ? keys(i) : 0ull;
     }
 #pragma unroll
     for (int u = 0; u < kSelectU; ++u)
         if ((uint64_t)tid + (uint64_t)u * kSelectThreads < n) hist_add(hist, (unsigned int)(k0[u] >> 52));
     __syncthreads();
     const uint64_t ns = n < kSample ? n : kSample;
-    locate_rank<4096>(hist, scan, (row.q * ns) / n, &s_bin, &s_before);
+    locate_rank<4096>(hist, scan, (q * ns) / n, &s_bin, &s_before);
     const unsigned int est = s_bin;
     // window [wb, wb + kSelectWin) of binades, inside bins 1..4095 (bin 0 holds zeros and subnormals)
     const unsigned int wb = est > (unsigned)(kSelectWin / 2) ? min(est - kSelectWin / 2, 4096u - kSelectWin) : 1u;
@@ -1102,7 +1158,7 @@ __global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow*
     }
     // keys in bin 0 (zero exposures, the common case) are counted per thread, not by LDS atomics
     // that would all hit one address
-    select_pass(x, n, (uint64_t)tid + kSample, [&](unsigned long long key) {
+    select_pass(keys, n, (uint64_t)tid + kSample, [&](unsigned long long key) {
         sum += __longlong_as_double(key);
         const unsigned int b = (unsigned int)(key >> 52);
         const unsigned int w = b - wb;
@@ -1125,11 +1181,11 @@ __global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow*
         if (tid < w) red[tid] = red[tid] + red[tid + w];
         __syncthreads();
     }
-    if (tid == 0) *row.ee = red[0] / (double)n;
-    locate_rank<4096>(hist, scan, row.q, &s_bin, &s_before);
+    if (tid == 0 && ee) *ee = keys.out(red[0] / (double)n);
+    locate_rank<4096>(hist, scan, q, &s_bin, &s_before);
     if (tid == 0) {
         s_prefix = (unsigned long long)s_bin << 52;
-        s_rank = row.q - s_before;
+        s_rank = q - s_before;
         s_count = hist[s_bin];
     }
     __syncthreads();
@@ -1167,7 +1223,7 @@ __global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow*
         // is the same value (ties, e.g. rows of zero exposure) and the remaining passes are skipped
         unsigned int c0 = 0u;
         unsigned long long kmin = ~0ull, kmax = 0ull;
-        select_pass(x, n, tid, [&](unsigned long long key) {
+        select_pass(keys, n, tid, [&](unsigned long long key) {
             if ((key >> hi) == (prefix >> hi)) {
                 const unsigned int b = (unsigned int)((key >> sh) & ((1u << wd) - 1u));
                 if (b == 0u) ++c0;
@@ -1183,7 +1239,7 @@ __global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow*
         }
         __syncthreads();
         if (s_min == s_max) {  // uniform across the block
-            if (tid == 0) *row.pfe = __longlong_as_double(s_min);
+            if (tid == 0) *pfe = keys.out(__longlong_as_double(s_min));
             return;
         }
         locate_rank<4096>(hist, scan, s_rank, &s_bin, &s_before);
@@ -1196,14 +1252,14 @@ __global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow*
         hi = sh;
     }
     if (hi == 0) {  // every bit fixed: the prefix is the key
-        if (tid == 0) *row.pfe = __longlong_as_double(s_prefix);
+        if (tid == 0) *pfe = keys.out(__longlong_as_double(s_prefix));
         return;
     }
     // compact the candidates sharing the prefix and rank them exactly
     if (tid == 0) s_ncand = 0u;
     __syncthreads();
     const unsigned long long prefix = s_prefix;
-    select_pass(x, n, tid, [&](unsigned long long key) {
+    select_pass(keys, n, tid, [&](unsigned long long key) {
         if ((key >> hi) == (prefix >> hi)) cand[atomicAdd(&s_ncand, 1u)] = key;
     });
     __syncthreads();
@@ -1228,7 +1284,18 @@ __global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow*
             __syncthreads();
         }
     }
-    if (tid == 0) *row.pfe = __longlong_as_double(cand[s_rank]);
+    if (tid == 0) *pfe = keys.out(__longlong_as_double(cand[s_rank]));
+}
+
+__global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow* rows) {
+    const SelectRow row = rows[blockIdx.x];
+    select_row(RowKeys{(const unsigned long long*)row.x}, row.n, row.q, row.ee, row.pfe);
+}
+
+// Per-trade PFE95 rows of one date group (each row's class legs written by the valuation).
+__global__ void __launch_bounds__(kSelectThreads) trade_select_kernel(const TradeSelectRow* rows) {
+    const TradeSelectRow row = rows[blockIdx.x];
+    select_row(TradeKeys{row.c, row.s, row.sK, row.absN}, row.n, row.q, nullptr, row.pfe);
 }
 
 // Per-trade EE: partial sums over chunks in chunk order, divided by the effective path count.

@@ -109,6 +109,12 @@ struct ValuePlan {
     const int* tr_orig;       // sorted -> caller trade index
     double* partial;          // [n_dates][n_chunks][n_trades] per-trade EE partial sums
     double* path_mtm;         // optional [n_dates][n_trades][n_paths]
+    // per-trade PFE: the legs (F, A) of the classes alive at each date of a date group,
+    // [cube_row[k]][cube_C][n_paths] (a part's classes start at cube_cls0), or null
+    double2* ccube;
+    const int* cube_row;      // [n_dates] the date's row in its date group
+    const int* cls_live0;     // [n_dates] first class alive at the date (classes are numbered by maturity)
+    int cube_C, cube_cls0;
 };
 
 // Dynamic LDS of one value_kernel block for a tile of TW = 64 PPL paths: sP [D][TW], sF [Fmax][TW],
@@ -196,6 +202,8 @@ template <int DISC> __global__ void simulate_philox_kernel(SimArgs g);
 template <int DISC> __global__ void simulate2_philox_kernel(Sim2Args g);
 __global__ void value_kernel(const ValuePlan* plans, const int4* units);
 __global__ void value2_kernel(const ValuePlan* plans, const int4* units);
+__global__ void value_p1_kernel(const ValuePlan* plans, const int4* units);    // one path per lane
+__global__ void value_cube_kernel(const ValuePlan* plans, const int4* units);  // one path per lane
 __global__ void value_kernel_persistent(const ValuePlan* plans, const int4* units, int n_units,
                                         unsigned int* counter);
 __global__ void reset_counter_kernel(unsigned int* counter);
@@ -208,6 +216,16 @@ struct SelectRow {
     double* pfe;
 };
 __global__ void select_kernel(const SelectRow* rows);
+// One per-trade PFE95 row: the positive parts max(s F - (s K) A, 0) of a trade of sign s = sign(N) over
+// the n paths of its class's legs c ([n] (F, A) pairs), rank q; pfe = |N| times that order statistic.
+struct TradeSelectRow {
+    const double2* c;
+    uint64_t n;
+    uint64_t q;
+    double s, sK, absN;
+    double* pfe;
+};
+__global__ void trade_select_kernel(const TradeSelectRow* rows);
 
 // Homogeneous batch: one schedule, each trade its own netting set (the reference's
 // npv_batch_over_paths, irs.rs:87-114), at dates that are reset dates, precede the start or follow

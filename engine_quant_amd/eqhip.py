@@ -42,7 +42,8 @@ EXPORTED = [
     "eqhip_unilateral_cva_from_exposure_batch_noalloc", "eqhip_plan_set_variant",
 ]
 
-PLAN_TRADE_EE, PLAN_KEEP_CUBE, PLAN_AUTOTUNE = 1, 2, 4  # flags of eqhip_plan_create's want_trade_ee argument
+PLAN_TRADE_EE, PLAN_KEEP_CUBE, PLAN_AUTOTUNE, PLAN_TRADE_PFE = 1, 2, 4, 8  # flags of eqhip_plan_create's
+# want_trade_ee argument
 
 
 class EqhipError(RuntimeError):
@@ -75,7 +76,7 @@ class CreditC(ctypes.Structure):
 
 class OutputsC(ctypes.Structure):
     _fields_ = [("set_ee", _PD), ("set_pfe", _PD), ("set_cva", _PD), ("trade_ee", _PD),
-                ("trade_cva", _PD), ("path_mtm", _PD), ("set_mtm", _PD)]
+                ("trade_cva", _PD), ("path_mtm", _PD), ("set_mtm", _PD), ("trade_pfe", _PD)]
 
 
 class CsaC(ctypes.Structure):
@@ -97,7 +98,7 @@ class TimingC(ctypes.Structure):
     _fields_ = [("simulate_ms", _D), ("value_ms", _D), ("reduce_ms", _D), ("cva_ms", _D),
                 ("total_ms", _D), ("n_value_blocks", _U64), ("paths_per_tile", _U32),
                 ("tiles_per_block", _U32), ("xva_rows_ms", _D), ("xva_select_ms", _D), ("trades_per_item", _U32),
-                ("n_parts", _U32)]
+                ("n_parts", _U32), ("trade_pfe_ms", _D), ("trade_pfe_groups", _U32)]
 
 
 _lib = None
@@ -256,13 +257,15 @@ class Plan:
     """
 
     def __init__(self, ctx, model, portfolio: Portfolio, dates, n_paths, n_steps, seed=0,
-                 rng=RNG_PHILOX, disc=DISC_EULER, want_trade_ee=False, keep_cube=False, autotune=False):
+                 rng=RNG_PHILOX, disc=DISC_EULER, want_trade_ee=False, keep_cube=False, autotune=False,
+                 want_trade_pfe=False):
         self.ctx = ctx
         self.portfolio = portfolio
         self.dates = _f64(dates)
         self.n_paths, self.n_steps = int(n_paths), int(n_steps)
         self.rng = rng
         self.want_trade_ee = bool(want_trade_ee)
+        self.want_trade_pfe = bool(want_trade_pfe)
         self.keep_cube = bool(keep_cube)
         self._pc = portfolio._c()
         self.factors = 2 if len(model) == 6 else 1
@@ -271,7 +274,7 @@ class Plan:
         mc = McC(self.n_paths, self.n_steps, int(seed), rng, disc)
         h = ctypes.c_void_p()
         flags = ((PLAN_TRADE_EE if want_trade_ee else 0) | (PLAN_KEEP_CUBE if keep_cube else 0) |
-                 (PLAN_AUTOTUNE if autotune else 0))
+                 (PLAN_AUTOTUNE if autotune else 0) | (PLAN_TRADE_PFE if want_trade_pfe else 0))
         _check(create(ctx.handle, ctypes.byref(m), ctypes.byref(self._pc), _dp(self.dates), len(self.dates),
                       ctypes.byref(mc), flags, ctypes.byref(h)))
         self.handle = h
@@ -290,8 +293,10 @@ class Plan:
             res["path_mtm"] = np.zeros((nd, nt, self.n_paths))
         if want_set_mtm:
             res["set_mtm"] = np.zeros((nd, ns, self.n_paths))
+        if self.want_trade_pfe:
+            res["trade_pfe"] = np.zeros((nt, nd))
         out = OutputsC(*[_dp(res.get(k)) for k in ("set_ee", "set_pfe", "set_cva", "trade_ee", "trade_cva",
-                                                    "path_mtm", "set_mtm")])
+                                                    "path_mtm", "set_mtm", "trade_pfe")])
         z = None
         if self.rng == RNG_HOST_DRAWS:
             z = _f64(host_draws)
@@ -543,9 +548,9 @@ class Context:
 
     def exposure_run(self, model, portfolio, dates, n_paths, n_steps, seed=0, rng=RNG_PHILOX,
                      disc=DISC_EULER, host_draws=None, hazard_rate=0.0, recovery_rate=0.0,
-                     want_trade_ee=False, want_path=False, want_set_mtm=False):
+                     want_trade_ee=False, want_path=False, want_set_mtm=False, want_trade_pfe=False):
         plan = Plan(self, model, portfolio, dates, n_paths, n_steps, seed=seed, rng=rng, disc=disc,
-                    want_trade_ee=want_trade_ee or want_path)
+                    want_trade_ee=want_trade_ee or want_path, want_trade_pfe=want_trade_pfe)
         try:
             return plan.execute(host_draws, hazard_rate, recovery_rate, True, want_path, want_set_mtm)
         finally:

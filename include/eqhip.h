@@ -42,6 +42,10 @@ extern "C" {
 #define EQHIP_PLAN_AUTOTUNE 4  /* the first execute times the valuation's variants (paths per lane
                                   1 / 2 / 4, trades per work item 2 / 4 / 8) that fit the plan on its
                                   own paths and keeps the fastest; results do not depend on it */
+#define EQHIP_PLAN_TRADE_PFE 8 /* per-trade PFE95 output (HullWhite1F): the valuation also writes the
+                                  legs of every live schedule class (16 B per class, path and date, in
+                                  groups of dates sized to a quarter of the free device memory) and one
+                                  exact select per live (trade, date) forms the trade's values from them */
 
 typedef struct eqhip_context eqhip_context;
 typedef struct eqhip_plan eqhip_plan;
@@ -99,6 +103,8 @@ typedef struct {
     double* trade_cva;
     double* path_mtm; /* per-path, per-trade mark-to-market before max(.,0) (small runs only) */
     double* set_mtm;  /* per-path netting-set value before max(.,0) (small runs only) */
+    double* trade_pfe; /* per-trade PFE95 [n_trades][n_dates] (ee_pfe_per_trade, exposure.rs:151-159);
+                          needs a plan created with EQHIP_PLAN_TRADE_PFE */
 } eqhip_outputs;
 
 /* Per-kernel device times (milliseconds, HIP events on the context's stream) of the last
@@ -117,6 +123,10 @@ typedef struct {
     uint32_t trades_per_item; /* trades of one class and sign per work item of the per-trade step */
     uint32_t n_parts;         /* value plans the valuation runs in (more than one for a portfolio whose
                                  schedules do not fit one block: groups of classes, one after another) */
+    double trade_pfe_ms;      /* per-trade PFE95 selects (the class legs they read are written by the
+                                 valuation, whose time value_ms includes that) */
+    uint32_t trade_pfe_groups; /* date groups of the class-leg buffer (its size: a quarter of the free
+                                  device memory at plan creation, or EQHIP_TRADE_PFE_BYTES when smaller) */
 } eqhip_timing;
 
 /* ---- library / context -------------------------------------------------------------------- */
