@@ -38,7 +38,7 @@ class MlpCfgC(ctypes.Structure):
 
 class BermTimingC(ctypes.Structure):
     _fields_ = [("simulate_ms", _D), ("policy_ms", _D), ("apply_ms", _D), ("judge_ms", _D), ("train_ms", _D),
-                ("infer_ms", _D)]
+                ("infer_ms", _D), ("lattice_ms", _D), ("lattice_net_ms", _D), ("lattice_interp_ms", _D)]
 
 
 _ready = False
@@ -55,6 +55,7 @@ def _lib():
         L.eqhip_berm_exposure_lsm.argtypes = [_VP, _I32, _PD, _PD]
         L.eqhip_berm_train.argtypes = [_VP, ctypes.POINTER(MlpCfgC), _PF]
         L.eqhip_berm_exposure_mlp.argtypes = [_VP, _I32, _PD]
+        L.eqhip_berm_exposure_mlp_lattice.argtypes = [_VP, _I32, _I32, _I32, _PD]
         L.eqhip_berm_mse.argtypes = [_VP, _PD, _PD]
         L.eqhip_berm_get_params.argtypes = [_VP, _PF]
         L.eqhip_berm_set_params.argtypes = [_VP, ctypes.POINTER(MlpCfgC), _PF]
@@ -184,9 +185,15 @@ class BermudanBook:
         _check(_lib().eqhip_berm_train(self.handle, ctypes.byref(cfg.c()), _fp(curve)))
         return curve
 
-    def exposure_mlp(self, set_index=1):
+    def exposure_mlp(self, set_index=1, lattice=None):
+        """Network EE [n_trades][n_steps]: every alive path through the network, or with
+        lattice=(nr, nl) the network on a per-date node lattice and bilinear interpolation per path."""
         ee = np.zeros((self.n_trades, self.n_steps))
-        _check(_lib().eqhip_berm_exposure_mlp(self.handle, set_index, _dp(ee)))
+        if lattice is None:
+            _check(_lib().eqhip_berm_exposure_mlp(self.handle, set_index, _dp(ee)))
+        else:
+            nr, nl = lattice
+            _check(_lib().eqhip_berm_exposure_mlp_lattice(self.handle, set_index, int(nr), int(nl), _dp(ee)))
         return ee
 
     def mse(self):

@@ -24,7 +24,8 @@ struct HostSet {
     int n_chunks = 0;
     DevBuf R, LB, BN, X, tau, y0;
     DevBuf judge_coef;          // [n_trades][n_steps][4] of the last polynomial fit on this set
-    bool have_policy = false, have_judge = false;
+    DevBuf bounds;              // [n_steps][4] per-date (r_lo, r_hi, lb_lo, lb_hi) over the paths
+    bool have_policy = false, have_judge = false, have_bounds = false;
 
     PathSet view() const {
         PathSet ps;
@@ -61,6 +62,7 @@ struct eqhip_berm {
     bool have_net = false;
     uint32_t adam_t = 0;
     DevBuf master, m1, v1, frags, fwdvec, grad_part, loss_part, loss_step, grad_out, xs_u, xs_t, xs_m;
+    DevBuf lattice;  // node values [n_trades][n_steps][nr nl] of the last lattice evaluation
     eqhip_berm_timing timing{};
 
     Book book() const {
@@ -475,6 +477,64 @@ int eqhip_berm_exposure_mlp(eqhip_berm* B, int32_t set, double* ee) {
         HIPCHK(hipMemcpyAsync(ee, d.p, (size_t)rows * 8, hipMemcpyDeviceToHost, B->st()));
         B->sync();
         B->timing.infer_ms = ms_since(t0);
+    });
+}
+
+int eqhip_berm_exposure_mlp_lattice(eqhip_berm* B, int32_t set, int32_t nr, int32_t nl, double* ee) {
+    return guarded([&] {
+        if (!B || !ee) raise(EQHIP_E_INVALID, "null argument");
+        require_set(*B, set);
+        require_net(*B);
+        auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
+        if (nr < 2 || nl < 2 || !pow2(nr) || !pow2(nl) || nr * nl < 32 || nr * nl > kLatticeMaxNodes)
+            raise(EQHIP_E_INVALID, "lattice: nr, nl powers of two >= 2 with 32 <= nr nl <= " +
+                                       std::to_string(kLatticeMaxNodes));
+        HIPCHK(hipSetDevice(B->ctx->device));
+        const auto t0 = std::chrono::steady_clock::now();
+        HostSet& S = B->sets[set];
+        const int rows = B->n_trades * B->n_steps;
+        if (!S.have_bounds) {  // the paths of a set do not change: their per-date box once
+            S.bounds.alloc((size_t)B->n_steps * 4 * 8);
+            hipLaunchKernelGGL(berm_bounds_kernel, dim3(B->n_steps), dim3(kThreads), 0, B->st(), S.view(),
+                               S.bounds.as<double>());
+            HIPCHK(hipGetLastError());
+            S.have_bounds = true;
+        }
+        const size_t ns = (size_t)nr * nl;
+        if (B->lattice.bytes < (size_t)rows * ns * 4) B->lattice.alloc((size_t)rows * ns * 4);
+        hipEvent_t ev[3];
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipEventRecord(ev[0], B->st()));
+        hipLaunchKernelGGL(mlp_lattice_kernel, dim3(B->n_steps, B->n_trades), dim3(kThreads), 0, B->st(), B->book(),
+                           B->norm(), B->frags.as<const uint4>(), B->fwdvec.as<const float>(),
+                           S.bounds.as<const double>(), nr, nl, B->lattice.as<float>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(ev[1], B->st()));
+        ensure_part(*B, B->part_bytes(set, 1));
+        const size_t lds = (size_t)kLatticeGroup * ns * 4;
+        HIPCHK(hipFuncSetAttribute((const void*)mlp_lattice_ee_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+        hipLaunchKernelGGL(mlp_lattice_ee_kernel, dim3(S.n_chunks, (B->n_trades + kLatticeGroup - 1) / kLatticeGroup),
+                           dim3(kLatticeThreads), lds, B->st(), B->book(), S.view(), S.bounds.as<const double>(),
+                           B->lattice.as<const float>(), nr, nl, B->d_part.as<double>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(ev[2], B->st()));
+        std::vector<double> scale(B->n_trades);
+        for (int j = 0; j < B->n_trades; ++j) scale[j] = B->notional[j] / B->cfg.y_scale;
+        upload(B->d_inv_scale, scale, B->st());
+        DevBuf d;
+        d.alloc((size_t)rows * 8);
+        finalize(*B, rows, S.n_chunks, B->d_part.as<const double>(), 1, (double)S.n, B->d_inv_scale.as<const double>(),
+                 B->n_steps, B->d_kmax.as<const int>(), d.as<double>());
+        HIPCHK(hipMemcpyAsync(ee, d.p, (size_t)rows * 8, hipMemcpyDeviceToHost, B->st()));
+        B->sync();
+        float m0 = 0.f, m1 = 0.f;
+        HIPCHK(hipEventElapsedTime(&m0, ev[0], ev[1]));
+        HIPCHK(hipEventElapsedTime(&m1, ev[1], ev[2]));
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        B->timing.lattice_ms = ms_since(t0);
+        B->timing.lattice_net_ms = m0;
+        B->timing.lattice_interp_ms = m1;
     });
 }
 

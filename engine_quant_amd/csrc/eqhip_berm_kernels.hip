@@ -928,5 +928,243 @@ __global__ void __launch_bounds__(64) mlp_forward_kernel(int n, const uint4* fra
     if (p < n) y[(size_t)j * n + p] = my_y;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Lattice evaluation of the continuation networks. At date k every path's network inputs (r, LB)
+// lie in the box [r_lo, r_hi] x [lb_lo, lb_hi] of the set's paths at that date. Each network is
+// evaluated by MFMA at nr x nl nodes spanning the box (node (i, m) at r_lo + i dr, lb_lo + m dl),
+// and a path's output is the bilinear interpolation of the four nodes around its (r, LB): the
+// network's work per date is nr nl samples instead of one per alive path.
+// ---------------------------------------------------------------------------------------------
+
+// Per-date bounds of R and LB over the set's paths: (r_lo, r_hi, lb_lo, lb_hi) of date k = 1..n_steps.
+__global__ void __launch_bounds__(kThreads) berm_bounds_kernel(PathSet ps, double* bounds) {
+    const int k = blockIdx.x + 1;
+    const uint64_t row = (uint64_t)k * ps.n;
+    double v[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
+    for (uint64_t p = threadIdx.x; p < ps.n; p += kThreads) {
+        const double r = ps.R[row + p], lb = ps.LB[row + p];
+        v[0] = fmin(v[0], r);
+        v[1] = fmax(v[1], r);
+        v[2] = fmin(v[2], lb);
+        v[3] = fmax(v[3], lb);
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        v[0] = fmin(v[0], __shfl_xor(v[0], o));
+        v[1] = fmax(v[1], __shfl_xor(v[1], o));
+        v[2] = fmin(v[2], __shfl_xor(v[2], o));
+        v[3] = fmax(v[3], __shfl_xor(v[3], o));
+    }
+    __shared__ double red[kWaves][4];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) red[w][q] = v[q];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int q = threadIdx.x;
+        double m = red[0][q];
+        for (int i = 1; i < kWaves; ++i) m = (q & 1) ? fmax(m, red[i][q]) : fmin(m, red[i][q]);
+        bounds[(size_t)(k - 1) * 4 + q] = m;
+    }
+}
+
+namespace {
+
+// Lattice node spacing of one axis (zero for a degenerate box, whose nodes then coincide).
+__device__ __forceinline__ double node_step(double lo, double hi, int n) { return (hi - lo) / (double)(n - 1); }
+
+// Two 16-sample tiles through the network, layer by layer, so that one tile's matrix products and
+// the other's activations are independent instructions the scheduler can overlap.
+__device__ __forceinline__ void mlp_fwd2(const FwdW& W, bf16x8 bxa, bf16x8 bxb, float& ya, float& yb) {
+    f32x4 ra[4], rb[4];
+#pragma unroll
+    for (int R = 0; R < 4; ++R) {
+        f32x4 za = {0.f, 0.f, 0.f, 0.f}, zb = {0.f, 0.f, 0.f, 0.f};
+        za = mfma32(W.w1[R], bxa, za);
+        zb = mfma32(W.w1[R], bxb, zb);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            ra[R][i] = act_r(za[i]);
+            rb[R][i] = act_r(zb[i]);
+        }
+    }
+#pragma unroll
+    for (int layer = 0; layer < 2; ++layer) {
+        const bf16x8* Wl = layer == 0 ? W.w2 : W.w3;
+        const f32x4* bl = layer == 0 ? W.b2 : W.b3;
+        const bf16x8 a0 = pack_b(ra[0], ra[1]), a1 = pack_b(ra[2], ra[3]);
+        const bf16x8 b0 = pack_b(rb[0], rb[1]), b1 = pack_b(rb[2], rb[3]);
+#pragma unroll
+        for (int R = 0; R < 4; ++R) {
+            f32x4 za = bl[R], zb = bl[R];
+            za = mfma32(Wl[2 * R], a0, za);
+            zb = mfma32(Wl[2 * R], b0, zb);
+            za = mfma32(Wl[2 * R + 1], a1, za);
+            zb = mfma32(Wl[2 * R + 1], b1, zb);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                ra[R][i] = act_r(za[i]);
+                rb[R][i] = act_r(zb[i]);
+            }
+        }
+    }
+    float pa = 0.f, pb = 0.f;
+#pragma unroll
+    for (int R = 0; R < 4; ++R)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            pa += W.w4[R][i] * ra[R][i];
+            pb += W.w4[R][i] * rb[R][i];
+        }
+    pa += __shfl_xor(pa, 16);
+    pb += __shfl_xor(pb, 16);
+    pa += __shfl_xor(pa, 32);
+    pb += __shfl_xor(pb, 32);
+    ya = pa + W.b4;
+    yb = pb + W.b4;
+}
+
+}  // namespace
+
+// Node values of every (trade, date k <= kmax) network: table[trade][k-1][m nr + i] (grid n_steps x
+// n_trades, 4 waves, two tiles per wave at a time; nr nl a multiple of 32). The inputs are formed as
+// the per-path evaluation forms them (FP64 affine maps rounded to float32).
+__global__ void __launch_bounds__(kThreads) mlp_lattice_kernel(Book bk, Norm nm, const uint4* frags,
+                                                               const float* fwdvec, const double* bounds, int nr,
+                                                               int nl, float* table) {
+    const int k = blockIdx.x + 1, j = blockIdx.y;
+    if (k > bk.kmax[j]) return;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 4, n16 = l & 15;
+    FwdW W;
+    load_fwd(W, frags + (size_t)j * kNFrag * 64, fwdvec + (size_t)j * kFwdStride, l);
+    const double* bd = bounds + (size_t)(k - 1) * 4;
+    const double r_lo = bd[0], dr = node_step(bd[0], bd[1], nr), lb_lo = bd[2], dl = node_step(bd[2], bd[3], nl);
+    float u2;
+    {
+#pragma clang fp contract(off)
+        u2 = (float)((double)k / bk.spy * nm.t_scale + nm.t_shift);
+    }
+    const int ns = nr * nl, n_tiles = ns / 16;
+    float* out = table + ((size_t)j * bk.n_steps + (k - 1)) * ns;
+    auto inputs = [&](int s, float& a0, float& a1) {
+#pragma clang fp contract(off)
+        const int m = s / nr, i = s - m * nr;
+        const double r = r_lo + (double)i * dr, lb = lb_lo + (double)m * dl;
+        a0 = (float)((r - nm.r_shift) * nm.r_scale);
+        a1 = (float)(lb * nm.lb_scale + nm.lb_shift);
+    };
+    for (int t = 2 * w; t < n_tiles; t += 2 * kWaves) {
+        const int sa = 16 * t + n16, sb = sa + 16;
+        float a0, a1, b0, b1, ya, yb;
+        inputs(sa, a0, a1);
+        inputs(sb, b0, b1);
+        mlp_fwd2(W, input_operand(g, a0, a1, u2), input_operand(g, b0, b1, u2), ya, yb);
+        if (g == 0) {
+            out[sa] = ya;
+            out[sb] = yb;
+        }
+    }
+}
+
+// Exposure sums from the node tables: per (trade, date) the chunk sum of max(y, 0) over the paths
+// alive at the date, y interpolated at the path's (r, LB). Grid (chunks, trade groups of
+// kLatticeGroup), kLatticeThreads threads; the group's node tables of the date in LDS ([group][nr nl]
+// floats, dynamic; nr nl a power of two). A thread takes kLatticeUnroll paths at a time and forms their
+// lattice coordinates once for the group's trades. Each thread sums its paths in path order (FP32), then the
+// waves (FP64) and the block's waves in index order.
+__global__ void __launch_bounds__(kLatticeThreads) mlp_lattice_ee_kernel(Book bk, PathSet ps, const double* bounds,
+                                                                         const float* table, int nr, int nl,
+                                                                         double* part) {
+    constexpr int TG = kLatticeGroup, NW = kLatticeThreads / 64;
+    extern __shared__ float sT[];
+    __shared__ double red[NW][TG];
+    const int c = blockIdx.x, j0 = blockIdx.y * TG;
+    const int nt = min(TG, bk.n_trades - j0);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int ns = nr * nl;
+    int kend = 0;
+    for (int q = 0; q < nt; ++q) kend = max(kend, bk.kmax[j0 + q]);
+    const uint64_t n = ps.n;
+    const uint64_t p0 = (uint64_t)c * kChunk;
+    const uint64_t p1 = (p0 + kChunk < n) ? p0 + kChunk : n;
+    for (int k = 1; k <= kend; ++k) {
+        __syncthreads();  // the previous date's readers of sT and red are done
+        {  // float4 copies; ns is a power of two, so a trade's table is i >> sh
+            const int sh = __builtin_ctz((unsigned)ns) - 2;
+            float4* d4 = reinterpret_cast<float4*>(sT);
+            for (int i = threadIdx.x; i < (nt * ns) >> 2; i += kLatticeThreads) {
+                const int q = i >> sh;
+                const float4* src =
+                    reinterpret_cast<const float4*>(table + ((size_t)(j0 + q) * bk.n_steps + (k - 1)) * ns);
+                d4[i] = (k <= bk.kmax[j0 + q]) ? src[i - (q << sh)] : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+        __syncthreads();
+        const double* bd = bounds + (size_t)(k - 1) * 4;
+        const double r_lo = bd[0], dr = node_step(bd[0], bd[1], nr), lb_lo = bd[2], dl = node_step(bd[2], bd[3], nl);
+        const double ir_s = dr > 0.0 ? 1.0 / dr : 0.0, il_s = dl > 0.0 ? 1.0 / dl : 0.0;
+        float acc[TG];
+#pragma unroll
+        for (int q = 0; q < TG; ++q) acc[q] = 0.0f;
+        const uint64_t row = (uint64_t)k * n;
+        // kLatticeUnroll paths per thread at a time, their loads issued together (invalid ones clamp to
+        // the chunk's first path and add nothing)
+        constexpr int U = kLatticeUnroll;
+        for (uint64_t pb = p0 + threadIdx.x; pb < p1; pb += (uint64_t)U * kLatticeThreads) {
+            double rv[U], lv[U];
+            bool ok[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t p = pb + (uint64_t)u * kLatticeThreads;
+                ok[u] = p < p1;
+                const uint64_t pc = ok[u] ? p : p0;
+                rv[u] = ps.R[row + pc];
+                lv[u] = ps.LB[row + pc];
+            }
+            int base[U];
+            float wr[U], wl[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const double fr = fmin(fmax((rv[u] - r_lo) * ir_s, 0.0), (double)(nr - 1));
+                const double fl = fmin(fmax((lv[u] - lb_lo) * il_s, 0.0), (double)(nl - 1));
+                const int ir = min((int)fr, nr - 2), il = min((int)fl, nl - 2);
+                wr[u] = (float)(fr - (double)ir);
+                wl[u] = (float)(fl - (double)il);
+                base[u] = il * nr + ir;
+            }
+#pragma unroll
+            for (int q = 0; q < TG; ++q) {
+                if (q >= nt || k > bk.kmax[j0 + q]) continue;  // uniform
+                int tv[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    tv[u] = ok[u] ? ps.tau[(uint64_t)(j0 + q) * n + pb + (uint64_t)u * kLatticeThreads] : 0;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (!ok[u] || (tv[u] >= 0 && tv[u] * bk.spy < k)) continue;
+                    const float* T = sT + q * ns + base[u];
+                    const float y0 = T[0] + wr[u] * (T[1] - T[0]);
+                    const float y1 = T[nr] + wr[u] * (T[nr + 1] - T[nr]);
+                    const float y = y0 + wl[u] * (y1 - y0);
+                    acc[q] += fmaxf(y, 0.0f);
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < TG; ++q) {
+            const double s = wave_sum((double)acc[q]);
+            if (l == 0) red[w][q] = s;
+        }
+        __syncthreads();
+        if (threadIdx.x < nt && k <= bk.kmax[j0 + threadIdx.x]) {
+            const int q = threadIdx.x;
+            double s = red[0][q];
+            for (int i = 1; i < NW; ++i) s += red[i][q];
+            part[((uint64_t)(j0 + q) * bk.n_steps + (k - 1)) * gridDim.x + c] = s;
+        }
+    }
+}
+
 }  // namespace berm
 }  // namespace eqhip

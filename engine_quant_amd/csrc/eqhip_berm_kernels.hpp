@@ -104,6 +104,16 @@ template <int MODE>
 __global__ void mlp_eval_kernel(Book bk, PathSet ps, Norm nm, const uint4* frags, const float* fwdvec,
                                 const double* poly, double* part);
 __global__ void mlp_forward_kernel(int n, const uint4* frags, const float* fwdvec, const float* xu, float* y);
+// lattice evaluation of the networks (per-date node tables, bilinear interpolation per path)
+constexpr int kLatticeGroup = 16;      // trades per block of the interpolation kernel
+constexpr int kLatticeThreads = 512;   // threads per block of the interpolation kernel
+constexpr int kLatticeUnroll = 8;      // paths per thread in flight in the interpolation kernel
+constexpr int kLatticeMaxNodes = 1024; // nr nl: the group's tables fill at most 64 KB of LDS
+__global__ void berm_bounds_kernel(PathSet ps, double* bounds);
+__global__ void mlp_lattice_kernel(Book bk, Norm nm, const uint4* frags, const float* fwdvec, const double* bounds,
+                                   int nr, int nl, float* table);
+__global__ void mlp_lattice_ee_kernel(Book bk, PathSet ps, const double* bounds, const float* table, int nr, int nl,
+                                      double* part);
 
 }  // namespace berm
 }  // namespace eqhip

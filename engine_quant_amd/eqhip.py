@@ -32,7 +32,8 @@ EXPORTED = [
     "eqhip_plan_execute", "eqhip_plan_timing", "eqhip_plan_destroy", "eqhip_exposure_run",
     "eqhip_batch_create", "eqhip_batch_execute", "eqhip_batch_timing", "eqhip_batch_destroy",
     "eqhip_cva_adjoint", "eqhip_pv_sensitivities", "eqhip_berm_create", "eqhip_berm_fit_policy", "eqhip_berm_pathwise",
-    "eqhip_berm_exposure_lsm", "eqhip_berm_train", "eqhip_berm_exposure_mlp", "eqhip_berm_mse",
+    "eqhip_berm_exposure_lsm", "eqhip_berm_train", "eqhip_berm_exposure_mlp", "eqhip_berm_exposure_mlp_lattice",
+    "eqhip_berm_mse",
     "eqhip_berm_get_params", "eqhip_berm_set_params", "eqhip_berm_mlp_grad", "eqhip_berm_mlp_forward",
     "eqhip_berm_adam_step", "eqhip_berm_get_timing", "eqhip_berm_destroy",
     "eqhip_irs_hull_white_2f_exposure_profile", "eqhip_irs_hull_white_2f_exposure_profile_batch",

@@ -453,6 +453,9 @@ typedef struct {
 
 typedef struct {
     double simulate_ms, policy_ms, apply_ms, judge_ms, train_ms, infer_ms;
+    double lattice_ms;        /* eqhip_berm_exposure_mlp_lattice, wall time of the call */
+    double lattice_net_ms;    /* its network evaluation at the nodes (HIP events) */
+    double lattice_interp_ms; /* its per-path interpolation and exposure sums (HIP events) */
 } eqhip_berm_timing;
 
 /* Simulates both path sets (exact scheme, trapezoid log-numeraire) and keeps them resident.
@@ -475,6 +478,12 @@ int eqhip_berm_exposure_lsm(eqhip_berm* berm, int32_t set, double* ee, double* c
 int eqhip_berm_train(eqhip_berm* berm, const eqhip_berm_mlp_cfg* cfg, float* loss_curve);
 /* Network exposure on set 0 or 1: ee [n_trades][n_steps]. */
 int eqhip_berm_exposure_mlp(eqhip_berm* berm, int32_t set, double* ee);
+/* The same exposure with each network evaluated on a per-date lattice: nr x nl nodes spanning the
+ * (r, log-numeraire) box of the set's paths at that date (BF16 MFMA, two 16-sample tiles per wave at
+ * a time), each alive path's value the bilinear interpolation of its four surrounding nodes.
+ * nr, nl powers of two >= 2 with 32 <= nr nl <= 1024. No counterpart in the reference (its LSM
+ * has no network); it serves the network's exposure at a cost per path independent of its size. */
+int eqhip_berm_exposure_mlp_lattice(eqhip_berm* berm, int32_t set, int32_t nr, int32_t nl, double* ee);
 /* Out-of-sample accuracy on set 1 against the pathwise values: per-trade mean squared error per
  * unit notional of the network and of the polynomials last fitted by eqhip_berm_exposure_lsm. */
 int eqhip_berm_mse(eqhip_berm* berm, double* mse_mlp, double* mse_poly);

@@ -156,6 +156,53 @@ def test_training_reduces_loss_and_tracks_the_judge(ctx):
     B.close()
 
 
+def test_surrogate_meets_the_per_trade_gate(ctx):
+    """100 trades of the benchmark book's generator, 2^18 training paths, 2^18 fresh inference paths:
+    the network's EE against the judge's, summed over trades and dates, weighted by notional (every
+    EE carries its trade's notional), within 0.5%, both through the node lattice and path by path."""
+    bk = bermudan_book(100, seed=7)
+    B = BermudanBook(ctx, MODEL, bk["notional"], bk["strike"], bk["first_ex"], bk["n_ex"], 1 << 18, 1 << 18, 4, 30,
+                     seed_train=707, seed_infer=1477)
+    B.fit_policy()
+    ee_lsm, _ = B.exposure_lsm(1)
+    cfg = MlpConfig.for_model(MODEL, 30, steps=20000, batch=4096, lr=3e-3, lr_final=1e-5)
+    B.train(cfg)
+    ee_dir = B.exposure_mlp(1)
+    ee_lat = B.exposure_mlp(1, lattice=(128, 8))
+    B.close()
+    err_dir = np.abs(ee_dir - ee_lsm).sum() / ee_lsm.sum()
+    err_lat = np.abs(ee_lat - ee_lsm).sum() / ee_lsm.sum()
+    print(f"notional-weighted trade EE error: per path {err_dir:.4%}, lattice {err_lat:.4%}")
+    assert err_dir <= 0.005 and err_lat <= 0.005, (err_dir, err_lat)
+    assert np.abs(ee_lat - ee_dir).sum() / ee_dir.sum() <= 1e-3
+
+
+def test_lattice_exposure_against_the_per_path_network(ctx):
+    """The lattice pass on a small book with host draws: it converges to the per-path pass as the
+    lattice refines, rows after a trade's last date stay zero, and bad lattices are refused."""
+    bk = _book(n=5, seed=8)
+    n_tr, n_in = 4096, 3000  # a partial last chunk
+    zt = np.random.default_rng(31).standard_normal((SPY * HORIZON, n_tr))
+    zi = np.random.default_rng(32).standard_normal((SPY * HORIZON, n_in))
+    B = BermudanBook(ctx, MODEL, bk["notional"], bk["strike"], bk["first_ex"], bk["n_ex"], n_tr, n_in, SPY, HORIZON,
+                     z_train=zt, z_infer=zi)
+    B.fit_policy()
+    B.train(MlpConfig.for_model(MODEL, HORIZON, steps=300, batch=512))
+    ee_dir = B.exposure_mlp(1)
+    errs = []
+    for lat in ((16, 2), (32, 8), (128, 8)):
+        ee = B.exposure_mlp(1, lattice=lat)
+        kmax = (bk["first_ex"] + bk["n_ex"] - 1) * SPY
+        for j in range(B.n_trades):
+            assert np.all(ee[j, kmax[j]:] == 0.0)
+        errs.append(np.abs(ee - ee_dir).sum() / ee_dir.sum())
+    assert errs[2] < errs[1] < errs[0] and errs[2] < 5e-3, errs
+    for bad in ((3, 32), (1, 64), (64, 32)):
+        with pytest.raises(Exception, match="lattice"):
+            B.exposure_mlp(1, lattice=bad)
+    B.close()
+
+
 def test_network_beats_cubic_on_closed_form_contracts(ctx):
     """One-period payer swaptions (single exercise): the continuation value at t < E is the bond
     put of oracle.swaption. Both estimators are out of sample: the cubic per date fitted on the
