@@ -35,7 +35,8 @@ def algorithmic_work(port, dates, n_paths, n_sets, factors=1):
 
     Returned beside it, the work this kernel executes after sharing every discount factor and annuity
     across the trades of a schedule class:
-      * live (trade, path, date):        5 FLOPs   V = N*F - (N K)*A (mul + FMA), max, add
+      * live (trade, path, date):        3 FLOPs   x = s F - (s K) A (one FMA; s F once per work item),
+                                                   max (not a FLOP), add
       * live (class, path, date):        2 m + 2   annuity FMAs over the m remaining flows,
                                                    floating leg (sub, + div mid-period)
       * live (set, class, path, date):   4 FLOPs   V_s += alpha F - beta A (2 FMAs)
@@ -69,7 +70,7 @@ def algorithmic_work(port, dates, n_paths, n_sets, factors=1):
         u_live += float(np.sum(n_cls_trades[live]))
         val_flops += float(np.sum(n_cls_trades[live] * (2.0 * rem[live] + 6.0)))
         n_exp += float(np.sum(cf_dates > t + tol))
-        per_path = (5.0 * np.sum(n_cls_trades[live]) + np.sum(2.0 * rem[live] + 2.0)
+        per_path = (3.0 * np.sum(n_cls_trades[live]) + np.sum(2.0 * rem[live] + 2.0)
                     + 4.0 * set_cls * np.mean(live) + (EXP_FLOPS + 2.0 * factors) * np.sum(cf_dates > t + tol))
         flops += per_path
     flops *= n_paths
@@ -537,8 +538,10 @@ def run_adjoint(args):
     plan.execute(None, 0.02, 0.4)
     fwd_ms = 1000.0 * (time.perf_counter() - t0)
     v_ms = plan.timing()["value_ms"]
-    flops = algorithmic_work(port, dates, n_paths, port.n_sets)[0]
-    achieved = flops / (v_ms / 1000.0) / 1e12
+    # on quarterly dates the class-shared kernel executes less than the per-trade algorithmic count, so the
+    # roofline takes the executed work (model of the kernel's arithmetic) and reports the algorithmic beside it
+    flops, exec_flops = algorithmic_work(port, dates, n_paths, port.n_sets)[:2]
+    achieved = exec_flops / (v_ms / 1000.0) / 1e12
     bumps = {"a": 1e-4, "b": 1e-4, "sigma": 1e-4, "r0": 1e-4}
     names = ("a", "b", "sigma", "r0")
     fd = {}
@@ -573,7 +576,9 @@ def run_adjoint(args):
            "cva_pillar_sensitivities": "identically zero: the CVA discounts with the model bond P(0,t|r0)",
            "roofline": {"kernel": "value_kernel (the adjoint's forward run)", "bound": "fp64-valu",
                         "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                        "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "algorithmic_flops": flops,
+                        "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "executed_flops_model": exec_flops,
+                        "algorithmic_flops": flops,
+                        "algorithmic_tflops": flops / (v_ms / 1000.0) / 1e12,
                         "algorithmic_unit": "2m+6 FLOPs per live (trade, path, date), m = remaining fixed flows",
                         "value_kernel_ms": v_ms,
                         "note": "the reverse sweep (adjoint_kernel) is timed in the committed adjoint profile"}}
@@ -963,7 +968,8 @@ def main():
     executed = exec_flops / (v_ms / 1000.0) / 1e12
     tm = plan.timing()
     default_workload = (args.trades, args.paths, args.years, args.seed, args.disc) == (1000, 100_000, 30, 7, "exact")
-    kernel = "value2_kernel" if factors == 2 else "value_kernel"
+    # one path per lane (the default variant) runs the kernel of its own
+    kernel = "value2_kernel" if factors == 2 else ("value_p1_kernel" if tm["paths_per_tile"] == 64 else "value_kernel")
     profile = PROFILE2 if factors == 2 else PROFILE
     traffic = measured_traffic(kernel, profile) if default_workload else None
     counted_flops = counter_fp64_flops(profile_counters(profile)) if default_workload else None
