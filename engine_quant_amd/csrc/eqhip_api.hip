@@ -976,8 +976,7 @@ void build_select_rows(eqhip_plan& P) {
 // differ only in notional and fixed rate, so a trade's value on a path is N F - N K A with (F, A) the
 // legs of its class: the valuation writes the legs of the classes alive at each date of a group of
 // dates (16 B per class, path and date instead of 8 B per trade, path and date), and a select row per
-// live (trade, date) forms the trade's positive parts from them in each of its passes. Rows of one
-// date and class are adjacent, so the class's trades read its legs from the caches. A group holds as
+// live (trade, date) forms the trade's positive parts from them in each of its passes. A group holds as
 // many dates as a quarter of the free device memory allows (or EQHIP_TRADE_PFE_BYTES, when smaller).
 void build_trade_pfe(eqhip_plan& P, const eqhip_portfolio* pf) {
     std::vector<eqhip_plan*> vps;  // the value plans: the parts, or the plan itself
@@ -1032,23 +1031,43 @@ void build_trade_pfe(eqhip_plan& P, const eqhip_portfolio* pf) {
     }
     P.trade_pfe.alloc((size_t)P.n_trades * nd * 8);
     HIPCHK(hipMemsetAsync(P.trade_pfe.p, 0, (size_t)P.n_trades * nd * 8, P.ctx->stream));  // matured trades
+    // Blocks reach the 8 XCDs round-robin in index order, and each XCD has its own L2: the rows of one
+    // (date, class) go to one XCD's positions (x, x + 8, x + 16, ...), so that its selects, running there
+    // at about the same time, read the class's legs from that L2. Each (date, class) goes to the XCD with
+    // the fewest rows so far.
+    constexpr int kXcds = 8;
     std::vector<TradeSelectRow> rows;
     std::vector<int> live;
     P.grp_row.assign(1, 0);
     for (size_t gi = 0; gi + 1 < P.grp_k.size(); ++gi) {
+        std::vector<std::vector<TradeSelectRow>> xq(kXcds);
         for (int k = P.grp_k[gi]; k < P.grp_k[gi + 1]; ++k) {
             live.clear();
             for (int j = 0; j < P.n_trades; ++j)
                 if (mat[j] > P.dates[k] + kDateTolHost) live.push_back(j);
             std::stable_sort(live.begin(), live.end(), [&](int x, int y) { return cube_cls[x] < cube_cls[y]; });
-            for (int j : live) {
-                const double s = pf->notional[j] < 0.0 ? -1.0 : 1.0;
-                rows.push_back(TradeSelectRow{
-                    P.ccube.as<double2>() + ((uint64_t)cube_row[k] * C_total + cube_cls[j]) * np, P.h_neff[k],
-                    P.h_qrank[k], s, s * pf->fixed_rate[j], std::fabs(pf->notional[j]),
-                    P.trade_pfe.as<double>() + (size_t)j * nd + k});
+            for (size_t a = 0; a < live.size();) {
+                size_t b = a;
+                while (b < live.size() && cube_cls[live[b]] == cube_cls[live[a]]) ++b;
+                int x = 0;
+                for (int y = 1; y < kXcds; ++y)
+                    if (xq[y].size() < xq[x].size()) x = y;
+                for (size_t i = a; i < b; ++i) {
+                    const int j = live[i];
+                    const double s = pf->notional[j] < 0.0 ? -1.0 : 1.0;
+                    xq[x].push_back(TradeSelectRow{
+                        P.ccube.as<double2>() + ((uint64_t)cube_row[k] * C_total + cube_cls[j]) * np, P.h_neff[k],
+                        P.h_qrank[k], s, s * pf->fixed_rate[j], std::fabs(pf->notional[j]),
+                        P.trade_pfe.as<double>() + (size_t)j * nd + k});
+                }
+                a = b;
             }
         }
+        size_t longest = 0;
+        for (auto& q : xq) longest = std::max(longest, q.size());
+        for (size_t i = 0; i < longest; ++i)
+            for (int x = 0; x < kXcds; ++x)
+                if (i < xq[x].size()) rows.push_back(xq[x][i]);
         P.grp_row.push_back(rows.size());
     }
     upload(P.trade_rows, rows, P.ctx->stream);
