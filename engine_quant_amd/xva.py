@@ -128,8 +128,9 @@ class ExposureCube:
 
     # JSON, with the serde field names of ExposureCubeInput
     def to_json(self) -> str:
-        return json.dumps({"measure": self.measure, "times": self.times.tolist(), "paths": self.paths.tolist(),
-                           "discount_factors": self.discount_factors.tolist(), "source_hash": self.source_hash})
+        """The cube as the reference serialises it for its hash (quant-engine/src/xva.rs:77-91:
+        serde_json::to_value, whose map keeps keys sorted, then compact serde_json::to_vec)."""
+        return canonical_json(self).decode()
 
     @classmethod
     def from_json(cls, text: str) -> "ExposureCube":
@@ -217,9 +218,58 @@ def _check_inputs(request: XvaRequest, q: ExposureCube, p: Optional[ExposureCube
         raise ValueError("confidence_level must be in [0,1]")
 
 
+def json_float(v: float) -> str:
+    """A finite f64 as serde_json writes it: the shortest digits that read back to the same double, laid
+    out as 1234e7 -> "12340000000.0", 1234e-2 -> "12.34", 1234e-6 -> "0.001234" while 1e-5 <= |v| < 1e16,
+    otherwise "1e30" / "1.234e33" / "1e-7" (no "+", no leading zeros in the exponent). Python's repr has
+    the same shortest digits and the same decimal layout from 1e-4 up to 1e16; below and above, it writes
+    an exponent, which is re-laid out here."""
+    r = repr(float(v))
+    if "n" in r:
+        raise ValueError("exposure values must be finite")
+    if "e" not in r:
+        return r
+    mant, exp = r.split("e")
+    sign = "-" if mant.startswith("-") else ""
+    m = mant.lstrip("-")
+    ip, fp = m.split(".") if "." in m else (m, "")
+    digits, e10 = ip + fp, int(exp) - len(fp)  # |v| = digits x 10^e10
+    stripped = digits.rstrip("0")
+    e10 += len(digits) - len(stripped)
+    digits = stripped.lstrip("0")
+    length = len(digits)
+    kk = length + e10  # 10^(kk-1) <= |v| < 10^kk
+    if 0 <= e10 and kk <= 16:
+        return sign + digits + "0" * e10 + ".0"
+    if 0 < kk <= 16:
+        return sign + digits[:kk] + "." + digits[kk:]
+    if -5 < kk <= 0:
+        return sign + "0." + "0" * (-kk) + digits
+    tail = "" if length == 1 else "." + digits[1:]
+    return sign + digits[0] + tail + "e" + str(kk - 1)
+
+
+def _json_list(a) -> str:
+    return "[" + ",".join(json_float(x) for x in np.asarray(a, dtype=np.float64).ravel().tolist()) + "]"
+
+
+def canonical_json(cube: "ExposureCube") -> bytes:
+    """The exact bytes the reference hashes for an exposure cube: keys in sorted order, no spaces, f64 as
+    serde_json formats them, source_hash null or its string."""
+    sh = "null" if cube.source_hash is None else json.dumps(cube.source_hash)
+    rows = ",".join(_json_list(row) for row in cube.paths)
+    return (f'{{"discount_factors":{_json_list(cube.discount_factors)},"measure":{json.dumps(cube.measure)},'
+            f'"paths":[{rows}],"source_hash":{sh},"times":{_json_list(cube.times)}}}').encode()
+
+
+def cube_hash(cube: "ExposureCube") -> str:
+    """hash_value of the cube (quant-domain/src/lib.rs:308-313): "sha256:" + hex digest of its JSON."""
+    return "sha256:" + hashlib.sha256(canonical_json(cube)).hexdigest()
+
+
 def _result(request: XvaRequest, q: ExposureCube, p: Optional[ExposureCube], ee, pfe, m) -> XvaResult:
-    """The result record of xva.rs:184-245. The artifact id hashes the binary cube (the reference hashes
-    its JSON form), so ids identify cubes within this library only."""
+    """The result record of xva.rs:184-245; the artifact id is the reference's: the SHA-256 of the Q cube's
+    JSON (artifact, xva.rs:77-91)."""
     cva, dva, fva, mva, kva = (float(v) for v in m)
     approximations = ["piecewise-constant hazard and simple discounted exposure integration",
                       "all requested measures reuse one Q exposure reduction graph"]
@@ -230,7 +280,7 @@ def _result(request: XvaRequest, q: ExposureCube, p: Optional[ExposureCube], ee,
     lineage = [request.context_hash or "context:unhashed", f"operation:{request.operation_id}"]
     lineage += [f"netting-set:{s}" for s in request.netting_sets]
     lineage += [f"csa:{c.id}" for c in request.collateral_agreements]
-    digest = hashlib.sha256(q.to_bytes()).hexdigest()
+    digest = cube_hash(q)[len("sha256:"):]
     cube_ref = {"artifact_id": f"exposure-{digest}", "hash": f"sha256:{digest}", "measure": q.measure,
                 "shape": [len(q.times), q.paths.shape[1]], "materialized": bool(request.materialize_exposure),
                 "bytes": q.estimate_memory_bytes(), "lineage": list(lineage)}

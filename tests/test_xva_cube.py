@@ -126,3 +126,39 @@ def test_wire_forms_round_trip():
         xva.ExposureCube.from_bytes(b"X" * 64)
     nodf = xva.ExposureCube("Q", [1.0], [[2.0]])
     assert xva.ExposureCube.from_bytes(nodf.to_bytes()).discount_factors.size == 0
+
+
+# serde_json's f64 layout (the shortest round-trip digits; decimal from 1e-5 up to 1e16, else an
+# exponent without "+" or leading zeros), worked by hand from its rules for each range
+JSON_FLOATS = [(0.0, "0.0"), (-0.0, "-0.0"), (1.0, "1.0"), (100.0, "100.0"), (0.1, "0.1"), (1e-4, "0.0001"),
+               (1.5e-5, "0.000015"), (1e-5, "0.00001"), (1e-6, "1e-6"), (-1.5e-6, "-1.5e-6"),
+               (1e15, "1000000000000000.0"), (1e16, "1e16"), (1.5e16, "1.5e16"), (123456.789, "123456.789"),
+               (1.2345678901234568e17, "1.2345678901234568e17"), (5e-324, "5e-324"),
+               (1.7976931348623157e308, "1.7976931348623157e308"), (2.5e-10, "2.5e-10")]
+
+
+@pytest.mark.parametrize("v, text", JSON_FLOATS)
+def test_json_float_layout(v, text):
+    assert xva.json_float(v) == text
+    assert float(text) == v
+
+
+def test_artifact_id_hashes_the_reference_json():
+    """artifact (quant-engine/src/xva.rs:77-91) hashes serde_json::to_value(cube) written compactly: keys in
+    sorted order (serde_json's map without preserve_order), every field present (serde(default) only
+    affects reading), source_hash null when absent."""
+    import hashlib
+    import json
+    c = xva.ExposureCube("Q", [0.5, 1.0], [[1.0, -2.5], [0.0, 3e-7]], [0.99, 0.98])
+    text = ('{"discount_factors":[0.99,0.98],"measure":"Q","paths":[[1.0,-2.5],[0.0,3e-7]],'
+            '"source_hash":null,"times":[0.5,1.0]}')
+    assert xva.canonical_json(c).decode() == text
+    assert json.loads(text) == json.loads(c.to_json())
+    digest = hashlib.sha256(text.encode()).hexdigest()
+    assert xva.cube_hash(c) == "sha256:" + digest
+    c2 = xva.ExposureCube("P", [1.0], [[2.0]], source_hash="sha256:abc")
+    assert xva.canonical_json(c2) == b'{"discount_factors":[],"measure":"P","paths":[[2.0]],"source_hash":"sha256:abc","times":[1.0]}'
+    req = xva.XvaRequest(context_hash="sha256:ctx")
+    res = xva._result(req, c, None, np.zeros(2), np.zeros(2), np.zeros(5))
+    assert res.exposure_cube["artifact_id"] == "exposure-" + digest
+    assert res.exposure_cube["hash"] == "sha256:" + digest
