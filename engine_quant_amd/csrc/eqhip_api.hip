@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -114,6 +115,11 @@ struct eqhip_plan {
     int n_items = 0;
     int set_mode = 0;
     int ppl = kDefaultPpl, tpi = kDefaultTpi;  // valuation variant (paths per lane, trades per item)
+    // path mode (value_path_kernel): a plan without per-trade outputs whose families and netting sets
+    // fit the kernel's registers is valued one thread per path by default (path_ok: it can be)
+    bool path_ok = false, path_mode = false;
+    int path_ns = 0;  // the netting-set registers of the launch that serves it (a batch's largest; 0: its own)
+    DevBuf pe_off, pe, pc_set, pc_ab;
     bool autotune = false;     // the first execute times the valid variants and keeps the fastest
     bool tuned = false;
     std::vector<std::string> tune_log;
@@ -247,6 +253,12 @@ bool variant_fits(const eqhip_plan& P, int ppl, int tpi) {
 void tile_launch(eqhip_plan& P);
 
 void choose_launch(eqhip_plan& P) {
+    if (P.path_mode) {  // no LDS
+        P.set_mode = 0;
+        P.lds_bytes = 0;
+        tile_launch(P);
+        return;
+    }
     const int ppl = P.ppl;
     // split mode keeps one partial sum per (set, wave) in LDS; when those do not fit, one wave per set
     P.set_mode =
@@ -266,13 +278,16 @@ void choose_launch(eqhip_plan& P) {
 // The path tiling of the plan's variant: tiles of 64 PPL paths, grouped into chunks, one work unit
 // per (date, chunk).
 void tile_launch(eqhip_plan& P) {
-    const uint64_t tw = (uint64_t)kPathsPerTile * P.ppl;
+    const uint64_t tw = P.path_mode ? (uint64_t)kPathThreads * path_ppt(P.path_ns ? P.path_ns : P.S)
+                                    : (uint64_t)kPathsPerTile * P.ppl;
     P.n_tiles = (P.mc.n_paths + tw - 1) / tw;
     // enough units for ~16 blocks per CU (one 1024-thread block per CU at this LDS footprint). Units run
     // date-major and blocks go to the 8 XCDs round-robin, so with a multiple of 8 chunks every unit of
     // one path chunk lands on the same XCD, whose L2 then serves the rows later dates read again (the
     // fixings of periods in progress).
-    uint64_t target_chunks = std::max<uint64_t>(1, (P.target_units + P.n_dates - 1) / P.n_dates);
+    // path mode: 256-thread blocks with no LDS, several per CU: four times the units
+    const uint64_t target_units = P.path_mode ? 4 * P.target_units : P.target_units;
+    uint64_t target_chunks = std::max<uint64_t>(1, (target_units + P.n_dates - 1) / P.n_dates);
     target_chunks = (target_chunks + 7) / 8 * 8;
     uint64_t tpb = std::max<uint64_t>(1, (P.n_tiles + target_chunks - 1) / target_chunks);
     P.tiles_per_block = (int)tpb;
@@ -579,6 +594,57 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
             set_nlive[(size_t)k * P.S + s] = n;
         }
 
+    // Path-mode tables. Per date, its live cash-flow dates in date order, each with every family's
+    // accrual at it and the netting-set entries (by family, then by set) of the classes ending at it.
+    std::vector<int> pe_off(n_dates + 1, 0), pc_set;
+    std::vector<PathBond> pe;
+    std::vector<double2> pc_ab;
+    P.path_ok = !P.want_trade && !P.want_pfe && !is_part && P.NF <= kPathFamilies && P.S <= kPathSets;
+    if (P.path_ok) {
+        std::vector<std::vector<int>> ents(P.C);  // set-entry indices of each class, by set
+        for (int s = 0; s < P.S; ++s)
+            for (int e = set_off[s]; e < set_off[s + 1]; ++e) ents[set_cls[e]].push_back(e);
+        std::map<int, PathBond> at;  // cash-flow date -> its record
+        std::map<int, std::array<int, kPathFamilies>> ends;  // cash-flow date -> class ending there, per family
+        for (int k = 0; k < n_dates; ++k) {
+            at.clear();
+            ends.clear();
+            for (int F = 0; F < P.NF; ++F) {
+                const int4 fd = fdesc[(size_t)k * P.NF + F];
+                if (fd.x == 0) continue;
+                for (int i = fam_off[F] + fd.y; i < fam_off[F + 1]; ++i) {
+                    const int d = flow_d[i];
+                    auto it = at.find(d);
+                    if (it == at.end()) {
+                        PathBond b{};
+                        b.d = d;
+                        it = at.emplace(d, b).first;
+                        std::array<int, kPathFamilies> none;
+                        none.fill(-1);
+                        ends.emplace(d, none);
+                    }
+                    it->second.tau[F] = flow_tau[i];
+                    ends[d][F] = flow_cls[i];
+                }
+            }
+            for (auto& kv : at) {
+                PathBond b = kv.second;
+                b.e0 = (int)pc_set.size();
+                for (int F = 0; F < kPathFamilies; ++F) {
+                    const int c = ends[kv.first][F];
+                    if (c < 0) continue;
+                    for (int e : ents[c]) {
+                        pc_set.push_back((int)(std::upper_bound(set_off.begin(), set_off.end(), e) - set_off.begin()) - 1);
+                        pc_ab.push_back(make_double2(set_alpha[e], set_beta[e]));
+                    }
+                    b.n[F] = (int)ents[c].size();
+                }
+                pe.push_back(b);
+            }
+            pe_off[k + 1] = (int)pe.size();
+        }
+    }
+
     // per-trade work items: trades sorted by (last date of the class, class, sign), up to TPI trades
     // per item, for each TPI the kernel offers (2, 4, 8). Ordering by maturity keeps the items still
     // alive at a date a contiguous suffix of the list, so the waves whose items have all matured skip
@@ -631,8 +697,11 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
     P.ppl = 1;
     P.tpi = (!P.want_trade || P.n_items_t[1] <= kTradeSlots) ? 4 : 8;
     P.n_items = P.want_trade ? P.n_items_t[tpi_index(P.tpi)] : 0;
+    // path mode, whenever the plan allows it (it needs no LDS)
+    P.path_mode = P.path_ok;
     // A one-factor plan that no variant fits is valued in parts (a batch job or a part must fit)
-    const bool fits = variant_fits(P, 1, 2) || variant_fits(P, 1, 4) || variant_fits(P, 1, 8);
+    const bool fits =
+        P.path_mode || variant_fits(P, 1, 2) || variant_fits(P, 1, 4) || variant_fits(P, 1, 8);
     const bool partitioned = !fits && !is_part && !all_steps && P.factors == 1;
     if (!fits && !partitioned && P.want_trade && P.n_items_t[2] > kTradeSlots)
         raise(EQHIP_E_CAPACITY, "per-trade EE supports at most " + std::to_string(kTradeSlots) +
@@ -700,6 +769,10 @@ void build_plan(eqhip_plan& P, const eqhip_model_hw1f* model, const eqhip_portfo
     pack.add(P.set_beta, set_beta);
     pack.add(P.neff, neff);
     pack.add(P.qrank, qrank);
+    pack.add(P.pe_off, pe_off);
+    pack.add(P.pe, pe);
+    pack.add(P.pc_set, pc_set);
+    pack.add(P.pc_ab, pc_ab);
     pack.add(P.d_dates, P.dates);
     if (P.want_trade) {
         for (int v = 0; v < 3; ++v) {
@@ -898,7 +971,11 @@ void build_value_plan(eqhip_plan& P) {
     g.n_chunks = P.n_chunks;
     g.tiles_per_block = P.tiles_per_block;
     g.set_mode = P.set_mode;
-    g.variant = kVariant(P.ppl, P.tpi);
+    g.variant = P.path_mode ? kPathVariant : kVariant(P.ppl, P.tpi);
+    g.pe_off = P.pe_off.as<int>();
+    g.pe = P.pe.as<PathBond>();
+    g.pc_set = P.pc_set.as<int>();
+    g.pc_ab = P.pc_ab.as<double2>();
     g.two_factor = (P.factors == 2) ? 1 : 0;
     g.R2 = P.R2.as<double>();
     g.AB2 = P.AB2.as<double>();
@@ -1078,6 +1155,20 @@ void build_trade_pfe(eqhip_plan& P, const eqhip_portfolio* pf) {
 
 // The valuation of dates [k0, k1) (every date when k1 < 0); cube: write the class legs for the
 // per-trade PFE rows.
+// The path-mode kernel whose family and netting-set registers cover nf families and ns sets.
+using ValueKernel = void (*)(const ValuePlan*, const int4*);
+ValueKernel path_kernel(int factors, int nf, int ns) {
+    static const ValueKernel table[2][2][4] = {
+        {{value_path_kernel<1, 2, 1>, value_path_kernel<1, 2, 2>, value_path_kernel<1, 2, 4>, value_path_kernel<1, 2, 8>},
+         {value_path_kernel<1, 4, 1>, value_path_kernel<1, 4, 2>, value_path_kernel<1, 4, 4>, value_path_kernel<1, 4, 8>}},
+        {{value_path_kernel<2, 2, 1>, value_path_kernel<2, 2, 2>, value_path_kernel<2, 2, 4>, value_path_kernel<2, 2, 8>},
+         {value_path_kernel<2, 4, 1>, value_path_kernel<2, 4, 2>, value_path_kernel<2, 4, 4>, value_path_kernel<2, 4, 8>}}};
+    if (nf > kPathFamilies || ns > kPathSets) raise(EQHIP_E_HIP, "internal: plan outside path mode");
+    const int nsr = path_set_regs(ns);
+    const int si = nsr == 1 ? 0 : (nsr == 2 ? 1 : (nsr == 4 ? 2 : 3));
+    return table[factors == 2 ? 1 : 0][nf <= 2 ? 0 : 1][si];
+}
+
 void launch_value(eqhip_plan& P, double* path_mtm, double* set_mtm, int k0 = 0, int k1 = -1, bool cube = false) {
     hipStream_t st = P.ctx->stream;
     if (!P.parts.empty()) {  // the parts in order; the last one writes the signed set values
@@ -1093,8 +1184,15 @@ void launch_value(eqhip_plan& P, double* path_mtm, double* set_mtm, int k0 = 0, 
     if (k1 < 0) k1 = P.n_dates;
     const size_t u0 = (size_t)k0 * P.n_chunks, nu = (size_t)(k1 - k0) * P.n_chunks;
     if (nu == 0) return;
-    if (cube && (P.ppl != 1 || P.factors != 1 || !P.ccube_p))
+    if (cube && (P.ppl != 1 || P.factors != 1 || !P.ccube_p || P.path_mode))
         raise(EQHIP_E_HIP, "internal: class legs are written by one-factor plans at one path per lane");
+    if (P.path_mode) {
+        if (path_mtm) raise(EQHIP_E_HIP, "internal: path mode writes no per-trade values");
+        hipLaunchKernelGGL(path_kernel(P.factors, P.NF, P.S), dim3((unsigned)nu), dim3(kPathThreads), 0, st,
+                           P.vplan.as<const ValuePlan>(), P.units.as<const int4>() + u0);
+        HIPCHK(hipGetLastError());
+        return;
+    }
     auto kernel = (P.factors == 2) ? value2_kernel
                                    : (cube ? value_cube_kernel : (P.ppl == 1 ? value_p1_kernel : value_kernel));
     hipLaunchKernelGGL(kernel, dim3((unsigned)nu), dim3(kValueThreads), P.lds_bytes, st, P.vplan.as<const ValuePlan>(),
@@ -1236,7 +1334,7 @@ float time_value_launch(eqhip_plan& P) {
 // variant that fits and keeps the fastest. The trial launches write the outputs the real launch then
 // overwrites, and every variant writes the same netting-set values.
 void tune_plan(eqhip_plan& P) {
-    if (!P.parts.empty()) {  // the parts keep their default variant
+    if (!P.parts.empty() || P.path_mode) {  // the parts keep their default variant; path mode has none
         P.tuned = true;
         return;
     }
@@ -1406,9 +1504,10 @@ void execute_plan(eqhip_plan& P, const double* host_draws, const eqhip_credit* c
     P.timing.total_ms = (double)ms[0] + ms[1] + ms[2] + ms[3];
     P.timing.n_value_blocks = (uint64_t)P.n_dates * P.n_chunks * std::max<size_t>(1, P.parts.size());
     P.timing.n_parts = (uint32_t)std::max<size_t>(1, P.parts.size());
-    P.timing.paths_per_tile = (uint32_t)(kPathsPerTile * P.ppl);
+    P.timing.paths_per_tile =
+        P.path_mode ? (uint32_t)(kPathThreads * path_ppt(P.S)) : (uint32_t)(kPathsPerTile * P.ppl);
     P.timing.tiles_per_block = (uint32_t)P.tiles_per_block;
-    P.timing.trades_per_item = (uint32_t)P.tpi;
+    P.timing.trades_per_item = P.path_mode ? 0u : (uint32_t)P.tpi;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1932,8 +2031,10 @@ struct eqhip_batch {
     int rng = 0, disc = 0;
     double dt = 1.0;
     bool det = false;
-    DevBuf R, slot_of_step, plans, units, rows, counter;
+    DevBuf R, slot_of_step, plans, units, rows, counter, path_units;
     std::vector<int4> unit_list;
+    std::vector<int4> path_list;      // units of the jobs in path mode, launched on their own
+    int path_nf = 0, path_ns = 0;     // the most families and netting sets of those jobs
     size_t lds_bytes = 0;
     int persistent = 1;
     int best_ppl = kDefaultPpl, best_tpi = kDefaultTpi;
@@ -1962,8 +2063,15 @@ void layout_batch(eqhip_batch& B) {
     const int cus = device_cus(B.ctx);
     std::vector<ValuePlan> vps;
     std::vector<SelectRow> rows;
-    std::vector<std::pair<double, int4>> costed;
+    std::vector<std::pair<double, int4>> costed, path_costed;
     B.lds_bytes = 0;
+    B.path_nf = B.path_ns = 0;
+    for (auto& J : B.jobs)  // the path-mode launch covers every path-mode job; they are tiled for it
+        if (J->path_mode) {
+            B.path_nf = std::max(B.path_nf, J->NF);
+            B.path_ns = std::max(B.path_ns, J->S);
+        }
+    for (auto& J : B.jobs) J->path_ns = B.path_ns;
     uint64_t total_tiles = 0;
     for (auto& J : B.jobs) total_tiles += J->n_tiles * (uint64_t)J->n_dates;
     for (size_t j = 0; j < B.jobs.size(); ++j) {
@@ -1984,7 +2092,7 @@ void layout_batch(eqhip_batch& B) {
             uint64_t t0 = (uint64_t)u.z * P.tiles_per_block;
             uint64_t tiles = std::min<uint64_t>(P.tiles_per_block, P.n_tiles - t0);
             double live = (double)P.n_trades + 1.0;  // cost ~ tiles x (trades + classes)
-            costed.emplace_back((double)tiles * live, v);
+            (P.path_mode ? path_costed : costed).emplace_back((double)tiles * live, v);
         }
         std::vector<uint64_t> neff(P.n_dates), qrank(P.n_dates);
         HIPCHK(hipMemcpy(neff.data(), P.neff.p, P.n_dates * 8, hipMemcpyDeviceToHost));
@@ -1996,17 +2104,28 @@ void layout_batch(eqhip_batch& B) {
                                          P.set_ee.as<double>() + r, P.set_pfe.as<double>() + r});
             }
     }
-    std::stable_sort(costed.begin(), costed.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+    auto longest_first = [](const auto& a, const auto& b) { return a.first > b.first; };
+    std::stable_sort(costed.begin(), costed.end(), longest_first);
+    std::stable_sort(path_costed.begin(), path_costed.end(), longest_first);
     B.unit_list.clear();
     for (auto& c : costed) B.unit_list.push_back(c.second);
+    B.path_list.clear();
+    for (auto& c : path_costed) B.path_list.push_back(c.second);
     upload(B.plans, vps, B.ctx->stream);
     upload(B.units, B.unit_list, B.ctx->stream);
+    upload(B.path_units, B.path_list, B.ctx->stream);
     upload(B.rows, rows, B.ctx->stream);
 }
 
 void launch_batch_value(eqhip_batch& B) {
     hipStream_t st = B.ctx->stream;
+    if (!B.path_list.empty()) {  // the jobs in path mode: one launch whose registers cover all of them
+        hipLaunchKernelGGL(path_kernel(1, B.path_nf, B.path_ns), dim3((unsigned)B.path_list.size()),
+                           dim3(kPathThreads), 0, st, B.plans.as<const ValuePlan>(), B.path_units.as<const int4>());
+        HIPCHK(hipGetLastError());
+    }
     const int n = (int)B.unit_list.size();
+    if (n == 0) return;
     if (B.persistent) {
         hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, st, B.counter.as<unsigned int>());
         const int cus = device_cus(B.ctx);
@@ -2062,6 +2181,7 @@ float time_batch_launch(eqhip_batch& B, hipEvent_t e0, hipEvent_t e1) {
 // else the job's default.
 void set_batch_variants(eqhip_batch& B, int ppl, int tpi) {
     for (auto& J : B.jobs) {
+        if (J->path_mode) continue;  // no variant
         // without a per-trade step TPI does not matter: take the one each PPL allows
         auto tpi_for = [&](int p) { return J->want_trade ? tpi : (p == 4 ? 2 : 4); };
         int p = ppl;
@@ -2088,10 +2208,13 @@ void autotune_batch(eqhip_batch& B) {
     uint64_t best_t = B.target_units_per_cu;
     int best_m = B.persistent, best_ppl = kDefaultPpl, best_tpi = kDefaultTpi;
     set_batch_variants(B, kDefaultPpl, kDefaultTpi);
+    bool any_tile = false;  // a job valued by the tile kernels (the rest are in path mode)
+    for (auto& J : B.jobs) any_tile |= !J->path_mode;
     for (uint64_t t : {4, 8, 16}) {
         B.target_units_per_cu = t;
         layout_batch(B);
         for (int m : {1, 0}) {
+            if (m == 1 && !any_tile) continue;  // the persistent grid serves tile units only
             B.persistent = m;
             const float ms = time_batch_launch(B, e0, e1);
             B.tune_log.push_back("units/CU=" + std::to_string(t) + (m ? " persistent" : " per-unit") +
@@ -2109,6 +2232,7 @@ void autotune_batch(eqhip_batch& B) {
     for (auto& J : B.jobs) want_trade |= J->want_trade != 0;
     for (int ppl : {1, 2, 4})
         for (int tpi : {2, 4, 8}) {
+            if (!any_tile) break;
             if (ppl == 4 && tpi != 2) continue;
             if ((ppl == kDefaultPpl && tpi == kDefaultTpi) || (!want_trade && tpi != (ppl == 4 ? 2 : 4))) continue;
             set_batch_variants(B, ppl, tpi);
@@ -2129,9 +2253,12 @@ void autotune_batch(eqhip_batch& B) {
     layout_batch(B);
     B.best_ppl = best_ppl;
     B.best_tpi = best_tpi;
+    int n_path = 0;
+    for (auto& J : B.jobs) n_path += J->path_mode ? 1 : 0;
     B.tune_log.push_back("chosen: " + std::string(best_m ? "persistent" : "per-unit") + ", units/CU=" +
                          std::to_string(best_t) + ", paths/lane<=" + std::to_string(best_ppl) + ", trades/item=" +
-                         std::to_string(best_tpi));
+                         std::to_string(best_tpi) + ", path mode: " + std::to_string(n_path) + " of " +
+                         std::to_string(B.jobs.size()) + " jobs");
     B.tuned = true;
 }
 
@@ -2356,8 +2483,20 @@ int eqhip_plan_set_variant(eqhip_plan* plan, int32_t paths_per_lane, int32_t tra
     return guarded([&] {
         if (!plan) raise(EQHIP_E_INVALID, "plan is null");
         const int p = paths_per_lane, t = trades_per_item;
+        if (p == 0) {  // path mode
+            if (!plan->path_ok)
+                raise(EQHIP_E_CAPACITY, "path mode serves plans without per-trade outputs, with at most " +
+                                            std::to_string(kPathFamilies) + " schedule families and " +
+                                            std::to_string(kPathSets) + " netting sets");
+            HIPCHK(hipSetDevice(plan->ctx->device));
+            plan->path_mode = true;
+            choose_launch(*plan);
+            build_value_plan(*plan);
+            plan->tuned = true;
+            return;
+        }
         if ((p != 1 && p != 2 && p != 4) || (t != 2 && t != 4 && t != 8))
-            raise(EQHIP_E_INVALID, "paths_per_lane must be 1, 2 or 4 and trades_per_item 2, 4 or 8");
+            raise(EQHIP_E_INVALID, "paths_per_lane must be 0 (path mode), 1, 2 or 4 and trades_per_item 2, 4 or 8");
         if (!plan->parts.empty()) raise(EQHIP_E_INVALID, "a plan valued in parts keeps its parts' variant");
         if (plan->want_pfe && p != 1)
             raise(EQHIP_E_INVALID, "a plan with per-trade PFE values one path per lane");
@@ -2365,6 +2504,7 @@ int eqhip_plan_set_variant(eqhip_plan* plan, int32_t paths_per_lane, int32_t tra
             raise(EQHIP_E_CAPACITY, "variant paths_per_lane=" + std::to_string(p) + " trades_per_item=" +
                                         std::to_string(t) + " does not fit this plan (LDS, work items or registers)");
         HIPCHK(hipSetDevice(plan->ctx->device));
+        plan->path_mode = false;
         plan->ppl = p;
         plan->tpi = t;
         choose_launch(*plan);

@@ -955,6 +955,181 @@ __global__ void __launch_bounds__(kValueThreads) value2_kernel(const ValuePlan* 
     value_dispatch<2>(plans[u.x], u.y, u.z, smem);
 }
 
+// ---------------------------------------------------------------------------------------------
+// Path mode: plans without per-trade outputs whose schedule families and netting sets fit registers.
+// One thread carries path_ppt(NS) paths (kPathThreads apart, so loads and stores are coalesced): no
+// LDS, no barriers. Per tile the families' start values st_f (1, P(t, start), or P(t, T_next) times
+// the fixing factor of the period in progress); then per live cash-flow date d in date order the bond
+// P_d, every family's flow at d (annuity_f += P_d tau_f, tau_f = 0 where f has none), and the set
+// entries of the classes ending at d. The loop modifies no register under a branch on the family,
+// so it compiles to straight-line arithmetic. The bond, fixing-factor, leg and set expressions are
+// those of value_unit; the set sums run in cash-flow-date order rather than in set-entry order.
+// ---------------------------------------------------------------------------------------------
+// fn(v) on netting set s's registers (s < NS, wave-uniform). Each case ends in an empty asm statement
+// of its own text, so that the compiler cannot merge the cases into one access at a variable index
+// (which would move the set values to scratch memory).
+#define EQHIP_SET_CASE(i)                      \
+    if constexpr (NS > i)                       \
+        if (s == i) {                           \
+            fn(V[i]);                           \
+            asm volatile("; path-mode set " #i); \
+        }
+template <int NS, int U, class Fn>
+__device__ __forceinline__ void on_set(int s, double (&V)[NS][U], Fn fn) {
+    static_assert(NS <= 8, "one case per set register");
+    EQHIP_SET_CASE(0)
+    EQHIP_SET_CASE(1)
+    EQHIP_SET_CASE(2)
+    EQHIP_SET_CASE(3)
+    EQHIP_SET_CASE(4)
+    EQHIP_SET_CASE(5)
+    EQHIP_SET_CASE(6)
+    EQHIP_SET_CASE(7)
+}
+#undef EQHIP_SET_CASE
+
+// NFM: families held in registers (>= the plan's NF); NS: netting-set registers (>= the plan's S).
+// The arithmetic of a path does not depend on NFM and NS, only the register footprint does.
+template <int NFAC, int NFM, int NS>
+__device__ __forceinline__ void value_path_unit(const ValuePlan& plan, const int k, const int chunk) {
+    constexpr int U = path_ppt(NS);
+    constexpr int TWP = kPathThreads * U;
+    const __attribute__((address_space(4))) ValuePlan& g = *cst(&plan);
+    const uint64_t n_paths = g.n_paths;
+    const uint64_t span = (uint64_t)g.tiles_per_block * TWP;
+    const uint64_t p_beg = (uint64_t)chunk * span;
+    const uint64_t p_end = (p_beg + span < n_paths) ? p_beg + span : n_paths;
+    const int fbeg = g.fix_off[k];
+    const int e0 = g.pe_off[k], ne = g.pe_off[k + 1] - e0;
+    const cptr<PathBond> pb = cst(g.pe + e0);
+    const cptr<double> abd = cst((const double*)(g.AB + (size_t)k * g.D));
+    const cptr<double> b2d = cst(g.AB2 + (NFAC == 2 ? (size_t)k * g.D : 0));
+    const cptr<double> pab = cst((const double*)g.pc_ab);
+    const cptr<int> pset = cst(g.pc_set);
+    const cptr<int> fdc = cst((const int*)(g.fdesc + (size_t)k * g.NF));  // int4 per family
+    const gptr<double> Rk = glb(g.R + (uint64_t)g.date_slot[k] * g.r_stride);
+    const gptr<double> Yk = glb((NFAC == 2 ? g.R2 : g.R) + (uint64_t)g.date_slot[k] * g.r_stride);
+    const int x_store = g.set_mtm != nullptr;
+    for (uint64_t p0 = p_beg + threadIdx.x; p0 < p_end; p0 += TWP) {
+        double r[U], y[U], P[U], st[NFM][U], run[NFM][U], V[NS][U];
+        bool live[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t p = p0 + (uint64_t)u * kPathThreads;
+            live[u] = p < p_end;
+            r[u] = live[u] ? Rk[p] : 0.0;
+            y[u] = (NFAC == 2 && live[u]) ? Yk[p] : 0.0;
+        }
+        auto bond = [&](double A, double B, double B2, int u) {
+            return (NFAC == 2) ? exp((A - B * r[u]) - B2 * y[u]) : A * exp(-(B * r[u]));
+        };
+        // family starts: a period starting at t_k (1), a forward start (P(t, start)), a period in
+        // progress (P(t, T_next) / P(T_fix, T_next | r_fix))
+#pragma unroll
+        for (int f = 0; f < NFM; ++f) {
+            const int mode = (f < g.NF) ? fdc[4 * f] : 0;
+            const int dz = fdc[4 * f + 2];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                run[f][u] = 0.0;
+                st[f][u] = (mode == 1) ? 1.0 : 0.0;
+            }
+            if (mode >= 2) {
+                const double A = abd[2 * dz], B = abd[2 * dz + 1], B2 = (NFAC == 2) ? b2d[dz] : 0.0;
+#pragma unroll
+                for (int u = 0; u < U; ++u) st[f][u] = bond(A, B, B2, u);
+            }
+            if (mode == 3) {
+                const int w = fbeg + fdc[4 * f + 3];
+                const uint64_t row = (uint64_t)g.fix_slot[w] * g.r_stride;
+                const double c = g.fix_AB[w].x, b = g.fix_AB[w].y;
+                const double b2 = (NFAC == 2) ? g.fix_B2[w] : 0.0;
+                const double inv_a = 1.0 / c;  // one factor: 1 / A, as value_unit's inv_a
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint64_t p = p0 + (uint64_t)u * kPathThreads;
+                    double ff;
+                    if (NFAC == 2) {  // 1 / P(T_fix, T_next | x_f, y_f)
+                        const double xf = live[u] ? glb(g.R)[row + p] : 0.0;
+                        const double yf = live[u] ? glb(g.R2)[row + p] : 0.0;
+                        ff = exp(-((c - b * xf) - b2 * yf));
+                    } else {
+                        const double rf = live[u] ? glb(g.R)[row + p] : 0.0;
+                        ff = inv_a * exp(b * rf);
+                    }
+                    st[f][u] = st[f][u] * ff;
+                }
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+#pragma unroll
+            for (int u = 0; u < U; ++u) V[s][u] = 0.0;
+
+        for (int e = 0; e < ne; ++e) {
+            const int d = pb[e].d;
+            const double A = abd[2 * d], B = abd[2 * d + 1], B2 = (NFAC == 2) ? b2d[d] : 0.0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) P[u] = bond(A, B, B2, u);
+#pragma unroll
+            for (int f = 0; f < NFM; ++f) {
+                const double tau = pb[e].tau[f];
+#pragma unroll
+                for (int u = 0; u < U; ++u) run[f][u] = __builtin_fma(P[u], tau, run[f][u]);
+            }
+            int c = pb[e].e0;
+#pragma unroll
+            for (int f = 0; f < NFM; ++f) {
+                const int n = pb[e].n[f];
+                for (int i = 0; i < n; ++i, ++c) {  // the sets of family f's class ending at d
+                    const double al = pab[2 * c], be = pab[2 * c + 1];
+                    on_set<NS, U>((NS == 1) ? 0 : pset[c], V, [&](double (&v)[U]) {
+#pragma unroll
+                        for (int u = 0; u < U; ++u)
+                            v[u] = __builtin_fma(-be, run[f][u], __builtin_fma(al, st[f][u] - P[u], v[u]));
+                    });
+                }
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            if (s >= g.S) break;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t p = p0 + (uint64_t)u * kPathThreads;
+                if (!live[u]) continue;
+                const double v = V[s][u];
+                glb(g.X)[((uint64_t)s * g.n_dates + k) * n_paths + p] = (v > 0.0) ? v : 0.0;
+                if (x_store) glb(g.set_mtm)[((uint64_t)k * g.S + s) * n_paths + p] = v;
+            }
+        }
+    }
+}
+
+// One block per work unit (plan, date, chunk of tiles of kPathThreads x path_ppt(NS) paths); the
+// launch's NFM and NS cover every plan it serves, and the plans are tiled for its NS.
+template <int NFAC, int NFM, int NS>
+__global__ void __launch_bounds__(kPathThreads) value_path_kernel(const ValuePlan* plans, const int4* units) {
+    const int4 u = units[blockIdx.x];
+    value_path_unit<NFAC, NFM, NS>(plans[u.x], u.y, u.z);
+}
+template __global__ void value_path_kernel<1, 2, 1>(const ValuePlan*, const int4*);
+template __global__ void value_path_kernel<1, 2, 2>(const ValuePlan*, const int4*);
+template __global__ void value_path_kernel<1, 2, 4>(const ValuePlan*, const int4*);
+template __global__ void value_path_kernel<1, 2, 8>(const ValuePlan*, const int4*);
+template __global__ void value_path_kernel<1, 4, 1>(const ValuePlan*, const int4*);
+template __global__ void value_path_kernel<1, 4, 2>(const ValuePlan*, const int4*);
+template __global__ void value_path_kernel<1, 4, 4>(const ValuePlan*, const int4*);
+template __global__ void value_path_kernel<1, 4, 8>(const ValuePlan*, const int4*);
+template __global__ void value_path_kernel<2, 2, 1>(const ValuePlan*, const int4*);
+template __global__ void value_path_kernel<2, 2, 2>(const ValuePlan*, const int4*);
+template __global__ void value_path_kernel<2, 2, 4>(const ValuePlan*, const int4*);
+template __global__ void value_path_kernel<2, 2, 8>(const ValuePlan*, const int4*);
+template __global__ void value_path_kernel<2, 4, 1>(const ValuePlan*, const int4*);
+template __global__ void value_path_kernel<2, 4, 2>(const ValuePlan*, const int4*);
+template __global__ void value_path_kernel<2, 4, 4>(const ValuePlan*, const int4*);
+template __global__ void value_path_kernel<2, 4, 8>(const ValuePlan*, const int4*);
+
 // One thread per path, blockIdx.y the date. The bonds of the date's remaining flows are summed in
 // schedule order (the last one is also the floating leg's end bond); the trades' stores are
 // coalesced across the wave (consecutive paths).

@@ -13,6 +13,16 @@ constexpr int kTradeSlots = kValueThreads;  // (item, path slice) work slots per
 // Valuation variants: PPL paths per lane (1, 2, 4) and TPI trades of one class and sign per work
 // item (2, 4, 8); four paths per lane take two trades per item. Two-factor plans use PPL = 1.
 __host__ __device__ constexpr int kVariant(int ppl, int tpi) { return ppl * 16 + tpi; }
+// Path mode: plans without per-trade outputs, with at most kPathFamilies schedule families and
+// kPathSets netting sets, are valued by value_path_kernel: path_ppt(NS) paths per thread, the family
+// sums and set values in registers, no LDS and no barriers. A tile is then kPathThreads x
+// path_ppt(NS) paths, NS the set registers of the launch.
+constexpr int kPathVariant = 1;
+constexpr int kPathFamilies = 4;
+constexpr int kPathSets = 8;
+constexpr int kPathThreads = 256;
+__host__ __device__ constexpr int path_set_regs(int ns) { return ns <= 1 ? 1 : (ns <= 2 ? 2 : (ns <= 4 ? 4 : 8)); }
+__host__ __device__ constexpr int path_ppt(int ns) { return path_set_regs(ns) > 4 ? 2 : 4; }
 constexpr int kDefaultPpl = 1, kDefaultTpi = 4;
 constexpr int kMaxSplitSets = 8;     // netting sets whose class sums are split across the waves
 constexpr int kSelectThreads = 1024;  // EE / PFE select block
@@ -48,6 +58,18 @@ struct Sim2Args {
     const double* Z;  // host draws [2][n_steps][n_paths]: every x shock, then every y shock
     double* X;        // [n_slots][n_paths]
     double* Y;        // [n_slots][n_paths]
+};
+
+// One cash-flow date of a valuation date in path mode: the bond P = A exp(-B r) of date d, then for
+// every family f its flow at d (annuity_f += P tau[f]; tau[f] = 0 when f has no flow at d, an exact
+// no-op), then the netting-set entries of the classes ending at d, family by family: n[f] entries
+// from e0 on, V_s += alpha (st_f - P) - beta annuity_f.
+struct PathBond {
+    int d;
+    int e0;
+    int n[kPathFamilies];
+    int pad[2];
+    double tau[kPathFamilies];
 };
 
 // One exposure plan as the valuation kernel sees it. A launch takes an array of plans and a list
@@ -115,6 +137,12 @@ struct ValuePlan {
     const int* cube_row;      // [n_dates] the date's row in its date group
     const int* cls_live0;     // [n_dates] first class alive at the date (classes are numbered by maturity)
     int cube_C, cube_cls0;
+    // path mode (variant kPathVariant): each date's live cash-flow dates in date order (PathBond),
+    // and the netting-set entries of the classes ending at them (set, alpha, beta)
+    const int* pe_off;        // [n_dates + 1]
+    const PathBond* pe;
+    const int* pc_set;
+    const double2* pc_ab;     // (alpha, beta)
 };
 
 // Dynamic LDS of one value_kernel block for a tile of TW = 64 PPL paths: sP [D][TW], sF [Fmax][TW],
@@ -203,6 +231,8 @@ template <int DISC> __global__ void simulate2_philox_kernel(Sim2Args g);
 __global__ void value_kernel(const ValuePlan* plans, const int4* units);
 __global__ void value2_kernel(const ValuePlan* plans, const int4* units);
 __global__ void value_p1_kernel(const ValuePlan* plans, const int4* units);    // one path per lane
+template <int NFAC, int NFM, int NS>
+__global__ void value_path_kernel(const ValuePlan* plans, const int4* units);  // path mode
 __global__ void value_cube_kernel(const ValuePlan* plans, const int4* units);  // one path per lane
 __global__ void value_kernel_persistent(const ValuePlan* plans, const int4* units, int n_units,
                                         unsigned int* counter);

@@ -341,7 +341,8 @@ class Plan:
         return {f: getattr(t, f) for f, _ in TimingC._fields_}
 
     def set_variant(self, paths_per_lane, trades_per_item):
-        """Valuation variant (eqhip_plan_set_variant): paths per lane 1 / 2 / 4, trades per item 2 / 4 / 8."""
+        """Valuation variant (eqhip_plan_set_variant): paths per lane 1 / 2 / 4, trades per item 2 / 4 / 8;
+        paths_per_lane 0 selects path mode (one thread per path, for plans without per-trade outputs)."""
         _check(lib().eqhip_plan_set_variant(self.handle, int(paths_per_lane), int(trades_per_item)))
 
     def close(self):

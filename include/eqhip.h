@@ -316,7 +316,12 @@ int eqhip_plan_timing(const eqhip_plan* plan, eqhip_timing* out);
 /* Sets the plan's valuation variant: paths_per_lane (1, 2, 4: a tile of 64 x that many paths) and
  * trades_per_item (2, 4, 8). Refused with EQHIP_E_CAPACITY when it does not fit the plan (LDS, work
  * items; four paths per lane take two trades per item; two-factor plans one path per lane). Results
- * do not depend on the variant: netting-set values bit for bit, per-trade EE to rounding. No
+ * do not depend on the variant: netting-set values bit for bit, per-trade EE to rounding.
+ * paths_per_lane = 0 selects path mode (trades_per_item ignored): one thread per path with the
+ * schedule families and netting sets in registers, for plans without per-trade outputs, with at most
+ * 4 families and 8 netting sets, which take it by default. It needs no LDS, so it also serves such
+ * plans too diverse for a block's LDS. Its netting-set sums run in cash-flow-date order, so its values
+ * agree with the tiled variants' to rounding (bit for bit between its own runs, batched or not). No
  * counterpart in the reference (it has no GPU valuation kernel); it fixes what EQHIP_PLAN_AUTOTUNE
  * would otherwise choose. */
 int eqhip_plan_set_variant(eqhip_plan* plan, int32_t paths_per_lane, int32_t trades_per_item);

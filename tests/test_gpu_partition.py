@@ -124,14 +124,15 @@ def test_cube_and_adjoint_of_a_plan_in_parts(ctx):
 
 
 def test_what_parts_cannot_serve(ctx):
-    # one schedule with more cash-flow dates than a block's LDS holds
+    # one schedule with more cash-flow dates than a block's LDS holds, with per-trade EE (without it,
+    # path mode values the plan)
     pay = np.arange(1, 401) / 12.0
     port = eqhip.Portfolio(set_id=np.zeros(1, np.int32), notional=np.full(1, 1e6), fixed_rate=np.full(1, 0.02),
                            start=np.zeros(1), flow_off=np.array([0, 400], np.int32), pay=pay, acc=np.full(400, 1 / 12))
     with pytest.raises(eqhip.EqhipError, match="more LDS than a block has") as e:
-        ctx.plan(MODEL, port, [0.5, 1.0], 64, 4)
+        ctx.plan(MODEL, port, [0.5, 1.0], 64, 4, want_trade_ee=True)
     assert e.value.code == eqhip.E_CAPACITY
-    # G2++ plans are not valued in parts
+    # G2++ plans are not valued in parts (this book has more schedule families than path mode holds)
     port = diverse_book(600, n_sets=1, seed=1)
     with pytest.raises(eqhip.EqhipError, match="too diverse") as e:
         ctx.plan(MODEL2, port, np.arange(1, 13) / 12.0, 64, 12)
