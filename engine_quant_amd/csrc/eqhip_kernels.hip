@@ -12,6 +12,10 @@
 //                      factors, writes the netting-set values (and their positive parts) for every
 //                      set, and accumulates per-trade exposure sums over path slices. value2_kernel
 //                      is the G2++ instance (two-factor bonds in stage A).
+//   value_path_kernel  path mode (plans without per-trade outputs, <= 4 schedule families and <= 8
+//                      netting sets): each thread carries 4 paths (2 past 4 sets) through the date's
+//                      live cash-flow dates in date order, family annuities and set values in
+//                      registers, no LDS and no barriers.
 //   value_flat_kernel  homogeneous batches (one schedule, one trade per set): one thread per path
 //                      and date, one floating leg and annuity shared by the trades.
 //   select_kernel      one block per row: fixed-order sum + exact radix select of an order
