@@ -2859,6 +2859,25 @@ int eqhip_plan_xva(eqhip_plan* plan, int32_t set, const eqhip_xva_params* params
     });
 }
 
+int eqhip_bond_exp(eqhip_context* ctx, const double* x, double* y, int32_t n) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (n < 0 || (n > 0 && (!x || !y))) raise(EQHIP_E_INVALID, "x and y must hold n values");
+        if (n == 0) return;
+        HIPCHK(hipSetDevice(ctx->device));
+        DevBuf buf;
+        buf.alloc(2 * (size_t)n * sizeof(double));
+        double* dx = buf.as<double>();
+        double* dy = dx + n;
+        hipStream_t st = ctx->stream;
+        HIPCHK(hipMemcpyAsync(dx, x, (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(bond_exp_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dx, dy, (int)n);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(y, dy, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
 int eqhip_pv_dv01(eqhip_context* ctx, const double* pillars, const double* zero_rates, int32_t n_pillars,
                   const eqhip_portfolio* pf, double bump, double* out_pv, double* out_dv01, double* out_bucketed) {
     return eqhip_pv_dv01_at(ctx, pillars, zero_rates, n_pillars, pf, bump, 0.0, out_pv, out_dv01, out_bucketed);

@@ -298,7 +298,7 @@ __device__ void adjoint_path(const AdjArgs& g, uint64_t p, double acc[5]) {
             const double nu = nk[d];
             if (nu == 0.0) continue;
             const ZcbSens z = zk[d];
-            const double P = z.A * exp(-(z.B * r));
+            const double P = z.A * bond_exp(-(z.B * r));
             const double nP = nu * P;
             V += nP;
             gr -= nP * z.B;
@@ -315,8 +315,8 @@ __device__ void adjoint_path(const AdjArgs& g, uint64_t p, double acc[5]) {
             const ZcbSens zn = zk[md.x];
             const ZcbSens zf = g.fix_zs[md.z];
             const double rf = g.RF[(uint64_t)md.y * n + p];
-            const double Pn = zn.A * exp(-(zn.B * r));
-            const double Pf = zf.A * exp(-(zf.B * rf));
+            const double Pn = zn.A * bond_exp(-(zn.B * r));
+            const double Pf = zf.A * bond_exp(-(zf.B * rf));
             const double term = g.mid_alpha[m] * (Pn / Pf);
             V += term;
             gr -= term * zn.B;
@@ -335,7 +335,7 @@ __device__ void adjoint_path(const AdjArgs& g, uint64_t p, double acc[5]) {
                 const ZcbSens zn = zk[md.x];
                 const ZcbSens zf = g.fix_zs[md.z];
                 const double rf = g.RF[(uint64_t)md.y * n + p];
-                const double term = g.mid_alpha[m] * ((zn.A * exp(-(zn.B * r))) / (zf.A * exp(-(zf.B * rf))));
+                const double term = g.mid_alpha[m] * ((zn.A * bond_exp(-(zn.B * r))) / (zf.A * bond_exp(-(zf.B * rf))));
                 g.G[(uint64_t)md.w * n + p] += w * (term * zf.B);
             }
         }
@@ -601,13 +601,13 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                             xf = (gpa < n_paths) ? glb(g.R)[row + gpa] : 0.0;
                             yf = (gpa < n_paths) ? glb(g.R2)[row + gpa] : 0.0;
                         }
-                        sF[e * TW + lane + 64 * pp] = exp(-((c - b1 * xf) - b2 * yf));
+                        sF[e * TW + lane + 64 * pp] = bond_exp(-((c - b1 * xf) - b2 * yf));
                     }
                 } else {
                     const int d = dfirst + (e - nfix);
 #pragma unroll
                     for (int pp = 0; pp < PPL; ++pp)
-                        sP[d * TW + lane + 64 * pp] = exp((abd[2 * d] - abd[2 * d + 1] * r_next[pp]) - b2d[d] * y_next[pp]);
+                        sP[d * TW + lane + 64 * pp] = bond_exp((abd[2 * d] - abd[2 * d + 1] * r_next[pp]) - b2d[d] * y_next[pp]);
                 }
             }
             return;
@@ -627,12 +627,12 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                     const uint64_t gpa = pa + lane + 64 * pp;
                     double rf = rf_next[pp];
                     if (e != wave) rf = (gpa < n_paths) ? glb(g.R)[row + gpa] : 0.0;
-                    sF[e * TW + lane + 64 * pp] = inv_a * exp(b * rf);
+                    sF[e * TW + lane + 64 * pp] = inv_a * bond_exp(b * rf);
                 }
             } else {
                 const int d = dfirst + (e - nfix);
 #pragma unroll
-                for (int pp = 0; pp < PPL; ++pp) sP[d * TW + lane + 64 * pp] = abd[2 * d] * exp(-(abd[2 * d + 1] * r_next[pp]));
+                for (int pp = 0; pp < PPL; ++pp) sP[d * TW + lane + 64 * pp] = abd[2 * d] * bond_exp(-(abd[2 * d + 1] * r_next[pp]));
             }
         }
     };
@@ -645,10 +645,26 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
         glb(g.X)[xi] = (x_mode == 1 || x_mode == 2) ? v : ((v > 0.0) ? v : 0.0);
         if (g.set_mtm && (x_mode == 0 || x_mode == 3)) glb(g.set_mtm)[((uint64_t)k * g.S + s) * n_paths + gp] = v;
     };
+    // Split mode: the netting-set values of a tile are the fixed-order sums of the waves' partials in
+    // sPart, taken by waves 0 .. S-1 during the next tile's pass 2 (wave 0 starts the date's first
+    // family, so it carries nothing there; sPart is rewritten only by the next tile's C) and after
+    // the last tile.
+    auto reduce_sets = [&](uint64_t t) {
+        if (g.set_mode == 0 && tid < g.S * kPathsPerTile) {
+            const int s = tid >> 6;
+#pragma unroll
+            for (int pp = 0; pp < PPL; ++pp) {
+                double v = 0.0;
+                for (int w = 0; w < kValueWaves; ++w) v += sPart[(s * kValueWaves + w) * TW + lane + 64 * pp];
+                const uint64_t gp = t * TW + lane + 64 * pp;
+                if (gp < n_paths) store_set(s, gp, v);
+            }
+        }
+    };
     if (tile0 < tile_end) stage_a(tile0 * TW);
     __syncthreads();
-    // Per tile: B pass 1 | B pass 2 | C, D and A of the next tile | set reduction, with a barrier after
-    // each of the first three. A of the next tile can run beside C and D: it writes only sP and sF,
+    // Per tile: B pass 1 | B pass 2 and the previous tile's set reduction | C, D and A of the next tile,
+    // with a barrier after each. A of the next tile can run beside C and D: it writes only sP and sF,
     // which this tile last read in pass 1.
     for (uint64_t tile = tile0; tile < tile_end; ++tile) {
         const uint64_t p0 = tile * TW;
@@ -717,6 +733,7 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
             for (int pp = 0; pp < PPL; ++pp) sSeg[wave * TW + lane + 64 * pp] = run[pp];
         }
         __syncthreads();
+        if (tile > tile0) reduce_sets(tile - 1);
         if (n_carry > 0) {  // the totals of the earlier segments of this segment's first family
             double carry[PPL];
 #pragma unroll
@@ -858,19 +875,8 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
         }
         if (tile + 1 < tile_end) stage_a(p0 + TW);
         __syncthreads();
-        if (g.set_mode == 0) {
-            if (tid < g.S * kPathsPerTile) {
-                const int s = tid >> 6;
-#pragma unroll
-                for (int pp = 0; pp < PPL; ++pp) {
-                    double v = 0.0;
-                    for (int w = 0; w < kValueWaves; ++w) v += sPart[(s * kValueWaves + w) * TW + lane + 64 * pp];
-                    const uint64_t gp = p0 + lane + 64 * pp;
-                    if (gp < n_paths) store_set(s, gp, v);
-                }
-            }
-        }
     }
+    if (tile0 < tile_end) reduce_sets(tile_end - 1);  // the last tile's set partials
     if (g.want_trade) {
         // the slots' sums through LDS, then one fixed-order sum over the slices of each trade
         __syncthreads();
@@ -1025,7 +1031,7 @@ __device__ __forceinline__ void value_path_unit(const ValuePlan& plan, const int
             y[u] = (NFAC == 2 && live[u]) ? Yk[p] : 0.0;
         }
         auto bond = [&](double A, double B, double B2, int u) {
-            return (NFAC == 2) ? exp((A - B * r[u]) - B2 * y[u]) : A * exp(-(B * r[u]));
+            return (NFAC == 2) ? bond_exp((A - B * r[u]) - B2 * y[u]) : A * bond_exp(-(B * r[u]));
         };
         // family starts: a period starting at t_k (1), a forward start (P(t, start)), a period in
         // progress (P(t, T_next) / P(T_fix, T_next | r_fix))
@@ -1056,10 +1062,10 @@ __device__ __forceinline__ void value_path_unit(const ValuePlan& plan, const int
                     if (NFAC == 2) {  // 1 / P(T_fix, T_next | x_f, y_f)
                         const double xf = live[u] ? glb(g.R)[row + p] : 0.0;
                         const double yf = live[u] ? glb(g.R2)[row + p] : 0.0;
-                        ff = exp(-((c - b * xf) - b2 * yf));
+                        ff = bond_exp(-((c - b * xf) - b2 * yf));
                     } else {
                         const double rf = live[u] ? glb(g.R)[row + p] : 0.0;
-                        ff = inv_a * exp(b * rf);
+                        ff = inv_a * bond_exp(b * rf);
                     }
                     st[f][u] = st[f][u] * ff;
                 }
@@ -1134,6 +1140,12 @@ template __global__ void value_path_kernel<2, 4, 2>(const ValuePlan*, const int4
 template __global__ void value_path_kernel<2, 4, 4>(const ValuePlan*, const int4*);
 template __global__ void value_path_kernel<2, 4, 8>(const ValuePlan*, const int4*);
 
+// The valuation kernels' exp (bond_exp) on n values, for its accuracy check (eqhip_bond_exp).
+__global__ void bond_exp_kernel(const double* x, double* y, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = bond_exp(x[i]);
+}
+
 // One thread per path, blockIdx.y the date. The bonds of the date's remaining flows are summed in
 // schedule order (the last one is also the floating leg's end bond); the trades' stores are
 // coalesced across the wave (consecutive paths).
@@ -1150,10 +1162,10 @@ __global__ void __launch_bounds__(kFlatThreads) value_flat_kernel(FlatArgs g) {
     }
     const double r = g.date_det[k] ? g.r0 : g.R[(uint64_t)g.date_slot[k] * g.r_stride + p];
     const double2* ab = g.AB + (size_t)k * (m + 1);
-    const double p_start = ab[m].x * exp(-(ab[m].y * r));
+    const double p_start = ab[m].x * bond_exp(-(ab[m].y * r));
     double annuity = 0.0, p_end = 0.0;
     for (int i = first; i < m; ++i) {
-        const double P = ab[i].x * exp(-(ab[i].y * r));
+        const double P = ab[i].x * bond_exp(-(ab[i].y * r));
         const double leg = P * g.acc[i];
         annuity = (i == first) ? leg : annuity + leg;
         p_end = P;

@@ -280,6 +280,7 @@ struct FlatArgs {
     double* X;             // [trade][date][path] positive parts
 };
 __global__ void value_flat_kernel(FlatArgs g);
+__global__ void bond_exp_kernel(const double* x, double* y, int n);
 
 // One time row of an exposure cube for the XVA reducer (quant-engine/src/xva.rs:150-183): the row's
 // values after the first CSA's variation margin, their positive parts (written out for the quantile)

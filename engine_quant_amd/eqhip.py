@@ -40,7 +40,7 @@ EXPORTED = [
     "eqhip_unilateral_cva_from_exposure_2f", "eqhip_unilateral_cva_from_exposure_2f_batch", "eqhip_plan_create_2f",
     "eqhip_xva_calculate", "eqhip_plan_exposure_cube", "eqhip_plan_xva", "eqhip_scenarios_create",
     "eqhip_scenarios_destroy", "eqhip_irs_hull_white_exposure_profile_batch_on", "eqhip_pv_dv01_at",
-    "eqhip_unilateral_cva_from_exposure_batch_noalloc", "eqhip_plan_set_variant",
+    "eqhip_unilateral_cva_from_exposure_batch_noalloc", "eqhip_plan_set_variant", "eqhip_bond_exp",
 ]
 
 PLAN_TRADE_EE, PLAN_KEEP_CUBE, PLAN_AUTOTUNE, PLAN_TRADE_PFE = 1, 2, 4, 8  # flags of eqhip_plan_create's
@@ -145,6 +145,7 @@ def lib():
                                     _PD, _PD]
         L.eqhip_pv_dv01_at.argtypes = [ctypes.c_void_p, _PD, _PD, _I32, ctypes.POINTER(PortfolioC), _D, _D, _PD,
                                        _PD, _PD]
+        L.eqhip_bond_exp.argtypes = [ctypes.c_void_p, _PD, _PD, _I32]
         L.eqhip_plan_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(ModelHW1F),
                                         ctypes.POINTER(PortfolioC), _PD, _I32, ctypes.POINTER(McC),
                                         _I32, ctypes.POINTER(ctypes.c_void_p)]
@@ -529,6 +530,13 @@ class Context:
             _check(lib().eqhip_pv_dv01_at(self.handle, _dp(pillars), _dp(zero_rates), len(pillars), ctypes.byref(pc),
                                           bump, float(valuation_time), _dp(pv), _dp(dv), _dp(bk)))
         return pv, dv, bk
+
+    def bond_exp(self, x):
+        """The valuation kernels' FP64 exp of every value of x, computed on the device."""
+        x = _f64(x)
+        y = np.empty_like(x)
+        _check(lib().eqhip_bond_exp(self.handle, _dp(x), _dp(y), len(x)))
+        return y
 
     def pv_sensitivities(self, pillars, zero_rates, portfolio: Portfolio, model=None):
         """Analytic dPV/dz per pillar [n_trades][n_pillars]; with a model also the model NPV at t = 0

@@ -142,6 +142,11 @@ size_t eqhip_last_error(char* buf, size_t len);
 int eqhip_context_create(int device, eqhip_context** out);
 int eqhip_context_destroy(eqhip_context* ctx);
 
+/* No counterpart in the reference: a self-check for callers and tests. y[i] = the valuation kernels'
+ * FP64 exp of x[i] (the e^x of every per-path zero-coupon bond A exp(-B r)), computed on the device;
+ * within 0.81 ulp of e^x for |x| <= 708, inf / 0 beyond, NaN propagated. */
+int eqhip_bond_exp(eqhip_context* ctx, const double* x, double* y, int32_t n);
+
 /* ---- reference-shaped entry points -------------------------------------------------------- */
 
 /* Replaces engine::ffi::irs_hull_white_exposure_profile (engine-ffi/src/lib.rs:326-342;
