@@ -1193,8 +1193,11 @@ void launch_value(eqhip_plan& P, double* path_mtm, double* set_mtm, int k0 = 0, 
         HIPCHK(hipGetLastError());
         return;
     }
-    auto kernel = (P.factors == 2) ? value2_kernel
-                                   : (cube ? value_cube_kernel : (P.ppl == 1 ? value_p1_kernel : value_kernel));
+    // one path per lane: the kernel of the plan's trades-per-item variant
+    auto p1 = (P.tpi == 2) ? value_p1_kernel<2> : (P.tpi == 8 ? value_p1_kernel<8> : value_p1_kernel<4>);
+    auto cb = (P.tpi == 2) ? value_cube_kernel<2> : (P.tpi == 8 ? value_cube_kernel<8> : value_cube_kernel<4>);
+    auto v2 = (P.tpi == 2) ? value2_kernel<2> : (P.tpi == 8 ? value2_kernel<8> : value2_kernel<4>);
+    auto kernel = (P.factors == 2) ? v2 : (cube ? cb : (P.ppl == 1 ? p1 : value_kernel));
     hipLaunchKernelGGL(kernel, dim3((unsigned)nu), dim3(kValueThreads), P.lds_bytes, st, P.vplan.as<const ValuePlan>(),
                        P.units.as<const int4>() + u0);
     HIPCHK(hipGetLastError());
@@ -2420,7 +2423,7 @@ int eqhip_context_create(int device, eqhip_context** out) {
         ctx->lds_limit = prop.sharedMemPerBlock;  // 160 KB per workgroup on gfx950
         if (hipFuncSetAttribute((const void*)value_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)ctx->lds_limit) != hipSuccess ||
-            hipFuncSetAttribute((const void*)value2_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+            hipFuncSetAttribute((const void*)value2_kernel<4>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)ctx->lds_limit) != hipSuccess) {
             (void)hipGetLastError();
             ctx->lds_limit = std::min<size_t>(ctx->lds_limit, 65536);

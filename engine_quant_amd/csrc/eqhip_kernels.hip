@@ -11,7 +11,9 @@
 //                      fixing factors in LDS, builds each schedule class's floating-leg and annuity
 //                      factors, writes the netting-set values (and their positive parts) for every
 //                      set, and accumulates per-trade exposure sums over path slices. value2_kernel
-//                      is the G2++ instance (two-factor bonds in stage A).
+//                      is the G2++ instance (two-factor bonds in stage A). A plan at one path per
+//                      lane launches value_p1_kernel<TPI> (value2_kernel<TPI>), compiled for its
+//                      trades-per-item variant alone.
 //   value_path_kernel  path mode (plans without per-trade outputs, <= 4 schedule families and <= 8
 //                      netting sets): each thread carries 4 paths (2 past 4 sets) through the date's
 //                      live cash-flow dates in date order, family annuities and set values in
@@ -673,6 +675,11 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
         // B: schedule families. Every class's (floating leg, annuity) is a running sum over its
         // family's live flows; the date's flows are split into one contiguous segment per wave, each
         // summed from zero, and a second pass adds the totals of the earlier segments of the same family.
+        // the annuities of the segment's leading FU flows' classes, kept for pass 2 (class -1: none)
+        int lead_c[FU];
+        double lead_a[FU][PPL];
+#pragma unroll
+        for (int u = 0; u < FU; ++u) lead_c[u] = -1;
         {
             double run[PPL], st[PPL];
 #pragma unroll
@@ -690,6 +697,7 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
             };
             int i = seg_b;
             for (; i + FU <= seg_e; i += FU) {  // all LDS reads of FU flows first, then the chain
+                const bool lead = (i == seg_b);
                 int4 e[FU];
                 double tau[FU], P[FU][PPL];
 #pragma unroll
@@ -713,20 +721,36 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
 #pragma unroll
                         for (int pp = 0; pp < PPL; ++pp)
                             sC[(lane + 64 * pp) * CS + e[u].y] = make_double2(st[pp] - P[u][pp], run[pp]);
+                    if (lead) {
+                        lead_c[u] = e[u].y;
+#pragma unroll
+                        for (int pp = 0; pp < PPL; ++pp) lead_a[u][pp] = run[pp];
+                    }
                 }
             }
-            for (; i < seg_e; ++i) {
-                const int4 e = make_int4(lf[4 * i], lf[4 * i + 1], lf[4 * i + 2], 0);
-                if (e.z != fam) {
-                    fam = e.z;
-                    family_start(fam);
-                }
-                const double tau = lt[i];
+            if (i < seg_e) {  // fewer than FU flows left (the segment's leading flows when i == seg_b)
+                const bool lead = (i == seg_b);
 #pragma unroll
-                for (int pp = 0; pp < PPL; ++pp) {
-                    const double P = sP[e.x * TW + lane + 64 * pp];
-                    run[pp] = __builtin_fma(P, tau, run[pp]);
-                    if (e.y >= 0) sC[(lane + 64 * pp) * CS + e.y] = make_double2(st[pp] - P, run[pp]);
+                for (int u = 0; u < FU - 1; ++u) {
+                    if (i + u < seg_e) {
+                        const int4 e = make_int4(lf[4 * (i + u)], lf[4 * (i + u) + 1], lf[4 * (i + u) + 2], 0);
+                        if (e.z != fam) {
+                            fam = e.z;
+                            family_start(fam);
+                        }
+                        const double tau = lt[i + u];
+#pragma unroll
+                        for (int pp = 0; pp < PPL; ++pp) {
+                            const double P = sP[e.x * TW + lane + 64 * pp];
+                            run[pp] = __builtin_fma(P, tau, run[pp]);
+                            if (e.y >= 0) sC[(lane + 64 * pp) * CS + e.y] = make_double2(st[pp] - P, run[pp]);
+                        }
+                        if (lead) {
+                            lead_c[u] = e.y;
+#pragma unroll
+                            for (int pp = 0; pp < PPL; ++pp) lead_a[u][pp] = run[pp];
+                        }
+                    }
                 }
             }
 #pragma unroll
@@ -735,25 +759,33 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
         __syncthreads();
         if (tile > tile0) reduce_sets(tile - 1);
         if (n_carry > 0) {  // the totals of the earlier segments of this segment's first family
+            constexpr int CB = 8 / PPL;  // totals read per batch, then added in order
             double carry[PPL];
 #pragma unroll
             for (int pp = 0; pp < PPL; ++pp) carry[pp] = 0.0;
             int w2 = carry_lo;
-            for (; w2 + FU <= wave; w2 += FU) {  // FU reads per path in flight, added in order
-                double t4[FU][PPL];
+            for (; w2 + CB <= wave; w2 += CB) {
+                double t8[CB][PPL];
 #pragma unroll
-                for (int u = 0; u < FU; ++u)
+                for (int u = 0; u < CB; ++u)
 #pragma unroll
-                    for (int pp = 0; pp < PPL; ++pp) t4[u][pp] = sSeg[(w2 + u) * TW + lane + 64 * pp];
+                    for (int pp = 0; pp < PPL; ++pp) t8[u][pp] = sSeg[(w2 + u) * TW + lane + 64 * pp];
 #pragma unroll
-                for (int u = 0; u < FU; ++u)
+                for (int u = 0; u < CB; ++u)
 #pragma unroll
-                    for (int pp = 0; pp < PPL; ++pp) carry[pp] += t4[u][pp];
+                    for (int pp = 0; pp < PPL; ++pp) carry[pp] += t8[u][pp];
             }
             for (; w2 < wave; ++w2)
 #pragma unroll
                 for (int pp = 0; pp < PPL; ++pp) carry[pp] += sSeg[w2 * TW + lane + 64 * pp];
-            for (int i = seg_b; i < seg_b + n_carry; i += FU) {  // reads of FU classes, then the writes
+            // the leading classes' annuities from pass 1 (the same values sC holds), then any further
+            // carried flows by reading sC back
+#pragma unroll
+            for (int u = 0; u < FU; ++u)
+                if (u < n_carry && lead_c[u] >= 0)
+#pragma unroll
+                    for (int pp = 0; pp < PPL; ++pp) sC[(lane + 64 * pp) * CS + lead_c[u]].y = carry[pp] + lead_a[u][pp];
+            for (int i = seg_b + FU; i < seg_b + n_carry; i += FU) {  // reads of FU classes, then the writes
                 int cls[FU];
                 double a[FU][PPL];
 #pragma unroll
@@ -933,37 +965,39 @@ __global__ void __launch_bounds__(kValueThreads) value_kernel(const ValuePlan* p
     value_dispatch<1>(plans[u.x], u.y, u.z, smem);
 }
 
-// A plan's own launch at one path per lane: only those variants, so that the registers of the wider
-// ones do not weigh on them.
+// A plan's own launch at one path per lane, one kernel per trades-per-item variant (the plan's), so
+// that no variant's registers weigh on another's.
+template <int TPI>
 __global__ void __launch_bounds__(kValueThreads) value_p1_kernel(const ValuePlan* plans, const int4* units) {
     extern __shared__ double smem[];
     const int4 u = units[blockIdx.x];
-    const ValuePlan& plan = plans[u.x];
-    switch (__builtin_amdgcn_readfirstlane(cst(&plan)->variant)) {
-        case kVariant(1, 2): value_unit<1, 1, 2>(plan, u.y, u.z, smem); break;
-        case kVariant(1, 8): value_unit<1, 1, 8>(plan, u.y, u.z, smem); break;
-        default: value_unit<1, 1, 4>(plan, u.y, u.z, smem); break;
-    }
+    value_unit<1, 1, TPI>(plans[u.x], u.y, u.z, smem);
 }
+template __global__ void value_p1_kernel<2>(const ValuePlan*, const int4*);
+template __global__ void value_p1_kernel<4>(const ValuePlan*, const int4*);
+template __global__ void value_p1_kernel<8>(const ValuePlan*, const int4*);
 
 // The same, writing the legs of the live classes for the per-trade PFE rows (one path per lane).
+template <int TPI>
 __global__ void __launch_bounds__(kValueThreads) value_cube_kernel(const ValuePlan* plans, const int4* units) {
     extern __shared__ double smem[];
     const int4 u = units[blockIdx.x];
-    const ValuePlan& plan = plans[u.x];
-    switch (__builtin_amdgcn_readfirstlane(cst(&plan)->variant)) {
-        case kVariant(1, 2): value_unit<1, 1, 2, true>(plan, u.y, u.z, smem); break;
-        case kVariant(1, 8): value_unit<1, 1, 8, true>(plan, u.y, u.z, smem); break;
-        default: value_unit<1, 1, 4, true>(plan, u.y, u.z, smem); break;
-    }
+    value_unit<1, 1, TPI, true>(plans[u.x], u.y, u.z, smem);
 }
+template __global__ void value_cube_kernel<2>(const ValuePlan*, const int4*);
+template __global__ void value_cube_kernel<4>(const ValuePlan*, const int4*);
+template __global__ void value_cube_kernel<8>(const ValuePlan*, const int4*);
 
-// The same for two-factor (G2++) plans.
+// The same for two-factor (G2++) plans (one path per lane), one kernel per trades-per-item variant.
+template <int TPI>
 __global__ void __launch_bounds__(kValueThreads) value2_kernel(const ValuePlan* plans, const int4* units) {
     extern __shared__ double smem[];
     const int4 u = units[blockIdx.x];
-    value_dispatch<2>(plans[u.x], u.y, u.z, smem);
+    value_unit<2, 1, TPI>(plans[u.x], u.y, u.z, smem);
 }
+template __global__ void value2_kernel<2>(const ValuePlan*, const int4*);
+template __global__ void value2_kernel<4>(const ValuePlan*, const int4*);
+template __global__ void value2_kernel<8>(const ValuePlan*, const int4*);
 
 // ---------------------------------------------------------------------------------------------
 // Path mode: plans without per-trade outputs whose schedule families and netting sets fit registers.

@@ -229,10 +229,13 @@ constexpr int kSimBatch = 32;  // steps per batch of shocks in LDS (one factor; 
 template <int DISC> __global__ void simulate_philox_kernel(SimArgs g);
 template <int DISC> __global__ void simulate2_philox_kernel(Sim2Args g);
 __global__ void value_kernel(const ValuePlan* plans, const int4* units);
+template <int TPI>
 __global__ void value2_kernel(const ValuePlan* plans, const int4* units);
+template <int TPI>
 __global__ void value_p1_kernel(const ValuePlan* plans, const int4* units);    // one path per lane
 template <int NFAC, int NFM, int NS>
 __global__ void value_path_kernel(const ValuePlan* plans, const int4* units);  // path mode
+template <int TPI>
 __global__ void value_cube_kernel(const ValuePlan* plans, const int4* units);  // one path per lane
 __global__ void value_kernel_persistent(const ValuePlan* plans, const int4* units, int n_units,
                                         unsigned int* counter);
