@@ -665,9 +665,10 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     };
     if (tile0 < tile_end) stage_a(tile0 * TW);
     __syncthreads();
-    // Per tile: B pass 1 | B pass 2 and the previous tile's set reduction | C, D and A of the next tile,
+    // Per tile: B pass 1 | B pass 2 and the previous tile's set reduction | A of the next tile, C and D,
     // with a barrier after each. A of the next tile can run beside C and D: it writes only sP and sF,
-    // which this tile last read in pass 1.
+    // which this tile last read in pass 1; it goes first, so its exponentials' FMA chains overlap the
+    // waits of C's descriptor and LDS reads.
     for (uint64_t tile = tile0; tile < tile_end; ++tile) {
         const uint64_t p0 = tile * TW;
         // next tile's short rates: in flight during passes 1 and 2, used by the next tile's A
@@ -802,6 +803,7 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
             }
         }
         __syncthreads();
+        if (tile + 1 < tile_end) stage_a(p0 + TW);
         // C: netting sets, V_s = sum over the set's live classes of alpha F - beta A, in list order
         if (g.set_mode == 0) {
             for (int s = 0; s < g.S; ++s) {
@@ -885,7 +887,7 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
             const int pb = islice_b;
             int pe = min(islice_e, pvalid);
             if (det) pe = (p0 == 0 && pb == 0) ? 1 : pb;  // grid index 0: path 0 only
-#pragma unroll 4
+#pragma unroll 8  // eight class-leg reads in flight per slot
             for (int p = pb; p < pe; ++p) {
                 const double2 c2 = col[p * CS];
                 const double sfl = sg * c2.x;  // sign * F - (sign * K) A, exact sign flips
@@ -905,7 +907,6 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                 }
             }
         }
-        if (tile + 1 < tile_end) stage_a(p0 + TW);
         __syncthreads();
     }
     if (tile0 < tile_end) reduce_sets(tile_end - 1);  // the last tile's set partials
