@@ -464,6 +464,26 @@ template <class T> __device__ __forceinline__ gmut<T> glb(T* p) { return (gmut<T
 // space they are scalar loads into SGPRs, served by the scalar cache after a unit's first tile.
 template <class T> using cptr = const __attribute__((address_space(4))) T*;
 template <class T> __device__ __forceinline__ cptr<T> cst(const T* p) { return (cptr<T>)p; }
+// A wave-uniform pointer made opaque to the compiler at this point: loads through it after this point
+// are issued after it, so that none is hoisted out of an enclosing loop.
+template <class T> __device__ __forceinline__ void opaque(cptr<T>& p) {
+    uint64_t b = (uint64_t)p;
+    b = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b) |
+        ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32)) << 32);
+    asm volatile("" : "+s"(b));
+    p = (cptr<T>)b;
+}
+
+// Positive parts in one instruction. min(max(a b + c, 0), 1) compiles to one fused multiply-add with
+// the clamp modifier, which bounds the result to [0, 1] (NaN to 0). For operands scaled by 2^-512 it
+// is 2^-512 max(x, 0), x the unscaled a b + c: scaling by a power of two is exact and commutes with
+// the rounding of the fused multiply-add and of the sums that follow, for |x| in [2^-510, 2^512) per
+// unit notional (a smaller x loses bits to subnormal range, a larger one saturates). The sums are
+// scaled back by 2^512.
+constexpr double kPosScale = 0x1p-512, kPosUnscale = 0x1p512;
+__device__ __forceinline__ double fma_pos(double a, double b, double c) {
+    return __builtin_fmin(__builtin_fmax(__builtin_fma(a, b, c), 0.0), 1.0);
+}
 
 // NFAC = 2: G2++ plans; the state is (x, y) in (R, R2) and the bond factors are
 // P = exp((c - B(a,tau) x) - B(b,tau) y), with (c, B(a,tau)) in AB / fix_AB and B(b,tau) in AB2 / fix_B2.
@@ -478,27 +498,27 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     constexpr int FU = (PPL >= 4) ? 1 : 4 / PPL;  // flows (classes) per step of the unrolled loops
     // the descriptor read through the constant address space: no store of the kernel can alias it,
     // so a field is a scalar load wherever it is used instead of a register held for the whole unit
-    const __attribute__((address_space(4))) ValuePlan& g = *cst(&plan);
+    cptr<ValuePlan> pg = cst(&plan);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
     double* sP = smem;                                     // [D][TW]
-    double* sF = sP + (size_t)g.D * TW;                    // [Fmax][TW] reciprocal fixing factors
-    double2* sC = (double2*)(sF + (size_t)g.Fmax * TW);    // [TW][C] path-major
-    const int CS = g.C;
-    double* sPart = (double*)(sC + (size_t)g.C * TW);      // [S][waves][TW] (split mode)
-    double* sSeg = sPart + (g.set_mode == 0 ? (size_t)g.S * kValueWaves * TW : 0);  // [waves][TW]
+    double* sF = sP + (size_t)pg->D * TW;                    // [Fmax][TW] reciprocal fixing factors
+    double2* sC = (double2*)(sF + (size_t)pg->Fmax * TW);    // [TW][C] path-major
+    const int CS = pg->C;
+    double* sPart = (double*)(sC + (size_t)pg->C * TW);      // [S][waves][TW] (split mode)
+    double* sSeg = sPart + (pg->set_mode == 0 ? (size_t)pg->S * kValueWaves * TW : 0);  // [waves][TW]
 
-    const int dfirst = g.date_dfirst[k];
-    const int n_live = g.D - dfirst;
-    const int fbeg = g.fix_off[k];
-    const int nfix = g.fix_off[k + 1] - fbeg;
-    const int det = g.date_det[k];
-    const uint64_t n_paths = g.n_paths;
-    const gptr<double> Rk = glb(g.R + (uint64_t)g.date_slot[k] * g.r_stride);
-    const cptr<double> abd = cst((const double*)(g.AB + (size_t)k * g.D));  // (A, B) pairs
-    const cptr<int> fdc = cst((const int*)(g.fdesc + (size_t)k * g.NF));     // int4 per family
+    const int dfirst = pg->date_dfirst[k];
+    const int n_live = pg->D - dfirst;
+    const int fbeg = pg->fix_off[k];
+    const int nfix = pg->fix_off[k + 1] - fbeg;
+    const int det = pg->date_det[k];
+    const uint64_t n_paths = pg->n_paths;
+    const gptr<double> Rk = glb(pg->R + (uint64_t)pg->date_slot[k] * pg->r_stride);
+    const cptr<double> abd = cst((const double*)(pg->AB + (size_t)k * pg->D));  // (A, B) pairs
+    const cptr<int> fdc = cst((const int*)(pg->fdesc + (size_t)k * pg->NF));     // int4 per family
 
     // Per-trade work. The items alive at t_k are a suffix of the maturity-sorted list (every item
     // when per-path values are requested); each gets ns equal slices of a tile's paths, ns as large
@@ -506,8 +526,8 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     // Slot tid = (slice tid / n_li, live item tid % n_li): the lanes of a wave read one path row of sC
     // at consecutive items, whose class ids are consecutive (classes are numbered by maturity), so
     // their reads fall in consecutive banks or share an address.
-    const int item0 = !g.want_trade ? g.n_items : (g.path_mtm ? 0 : g.item_first[k]);
-    const int n_li = g.n_items - item0;
+    const int item0 = !pg->want_trade ? pg->n_items : (pg->path_mtm ? 0 : pg->item_first[k]);
+    const int n_li = pg->n_items - item0;
     const int ns = (n_li > 0) ? max(1, min(TW, kTradeSlots / n_li)) : 1;
     const int n_slots = n_li * ns;
     double acc[TPI], tK[TPI];
@@ -520,7 +540,7 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
         islice_b = (sl * TW) / ns;
         islice_e = ((sl + 1) * TW) / ns;
         if (tid < n_slots) {
-            const int4 item = g.items[item0 + li];
+            const int4 item = pg->items[item0 + li];
             icls = item.x;
             ifirst = item.y;
             isgn = (double)item.w;
@@ -529,37 +549,37 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
 #pragma unroll
         for (int q = 0; q < TPI; ++q) {
             acc[q] = 0.0;
-            tK[q] = (q < icnt) ? isgn * g.tr_K[ifirst + q] : 0.0;  // sign * K
+            tK[q] = (q < icnt) ? -(isgn * pg->tr_K[ifirst + q]) * kPosScale : 0.0;  // -sign * K, scaled
         }
     }
 
     // the date's live flows (cash-flow date, class or -1, family) and accruals, this wave's segment
-    const int lb = g.lflow_off[k];
-    const cptr<int> lf = cst((const int*)(g.lflow + lb));  // int4 per flow
-    const cptr<double> lt = cst(g.ltau + lb);
-    const int zb = g.zcls_off[k], n_zc = g.zcls_off[k + 1] - zb;
-    const int4 segd = g.lseg[k * kValueWaves + wave];
+    const int lb = pg->lflow_off[k];
+    const cptr<int> lf = cst((const int*)(pg->lflow + lb));  // int4 per flow
+    const cptr<double> lt = cst(pg->ltau + lb);
+    const int zb = pg->zcls_off[k], n_zc = pg->zcls_off[k + 1] - zb;
+    const int4 segd = pg->lseg[k * kValueWaves + wave];
     const int seg_b = uni(segd.x), seg_e = uni(segd.y), carry_lo = uni(segd.z), n_carry = uni(segd.w);
     __syncthreads();  // the previous unit of a persistent block may still read this LDS
     // classes already matured at t_k: zeroed once, no live flow writes them during this unit
     for (int i = wave; i < n_zc; i += kValueWaves)
 #pragma unroll
-        for (int pp = 0; pp < PPL; ++pp) sC[(lane + 64 * pp) * CS + cst(g.zcls)[zb + i]] = make_double2(0.0, 0.0);
+        for (int pp = 0; pp < PPL; ++pp) sC[(lane + 64 * pp) * CS + cst(pg->zcls)[zb + i]] = make_double2(0.0, 0.0);
 
-    const uint64_t tile0 = (uint64_t)chunk * g.tiles_per_block;
-    uint64_t tile_end = tile0 + g.tiles_per_block;
-    if (tile_end > g.n_tiles) tile_end = g.n_tiles;
+    const uint64_t tile0 = (uint64_t)chunk * pg->tiles_per_block;
+    uint64_t tile_end = tile0 + pg->tiles_per_block;
+    if (tile_end > pg->n_tiles) tile_end = pg->n_tiles;
 
     // The short rates of the next tile are loaded one tile ahead (every wave: r at t_k for its own
     // lanes; wave f < nfix: r at fixing f), so that no tile starts by waiting on HBM.
-    const gptr<double> Rfix = (wave < nfix) ? glb(g.R + (uint64_t)g.fix_slot[fbeg + wave] * g.r_stride) : nullptr;
-    const double2 fab_w = (wave < nfix) ? g.fix_AB[fbeg + wave] : make_double2(1.0, 0.0);
+    const gptr<double> Rfix = (wave < nfix) ? glb(pg->R + (uint64_t)pg->fix_slot[fbeg + wave] * pg->r_stride) : nullptr;
+    const double2 fab_w = (wave < nfix) ? pg->fix_AB[fbeg + wave] : make_double2(1.0, 0.0);
     const double inv_a_w = 1.0 / fab_w.x;  // once per unit
     // two factors: the second state at t_k and at fixing `wave`, and that fixing's B(b, tau)
-    const gptr<double> Yk = (NFAC == 2) ? glb(g.R2 + (uint64_t)g.date_slot[k] * g.r_stride) : nullptr;
+    const gptr<double> Yk = (NFAC == 2) ? glb(pg->R2 + (uint64_t)pg->date_slot[k] * pg->r_stride) : nullptr;
     const gptr<double> Yfix =
-        (NFAC == 2 && wave < nfix) ? glb(g.R2 + (uint64_t)g.fix_slot[fbeg + wave] * g.r_stride) : nullptr;
-    const double fb2_w = (NFAC == 2 && wave < nfix) ? g.fix_B2[fbeg + wave] : 0.0;
+        (NFAC == 2 && wave < nfix) ? glb(pg->R2 + (uint64_t)pg->fix_slot[fbeg + wave] * pg->r_stride) : nullptr;
+    const double fb2_w = (NFAC == 2 && wave < nfix) ? pg->fix_B2[fbeg + wave] : 0.0;
     double r_next[PPL], rf_next[PPL], y_next[PPL], yf_next[PPL];
     auto prefetch = [&](uint64_t tile, bool live) {  // r (and the fixing rates) of `tile` into the _next registers
 #pragma unroll
@@ -583,25 +603,25 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     // the rates in the _next registers.
     auto stage_a = [&](uint64_t pa) {
         if (NFAC == 2) {  // sF = 1 / P(T_fix, T_next | x_f, y_f), sP = P(t_k, T_d | x, y)
-            const cptr<double> b2d = cst(g.AB2 + (size_t)k * g.D);
+            const cptr<double> b2d = cst(pg->AB2 + (size_t)k * pg->D);
             for (int e = wave; e < nfix + n_live; e += kValueWaves) {
                 if (e < nfix) {
                     double c = fab_w.x, b1 = fab_w.y, b2 = fb2_w;
                     uint64_t row = 0;
                     if (e != wave) {  // more fixings than waves
-                        row = (uint64_t)glb(g.fix_slot)[fbeg + e] * g.r_stride;
-                        const gptr<double> fp = glb((const double*)(g.fix_AB + fbeg + e));
+                        row = (uint64_t)glb(pg->fix_slot)[fbeg + e] * pg->r_stride;
+                        const gptr<double> fp = glb((const double*)(pg->fix_AB + fbeg + e));
                         c = fp[0];
                         b1 = fp[1];
-                        b2 = glb(g.fix_B2)[fbeg + e];
+                        b2 = glb(pg->fix_B2)[fbeg + e];
                     }
 #pragma unroll
                     for (int pp = 0; pp < PPL; ++pp) {
                         const uint64_t gpa = pa + lane + 64 * pp;
                         double xf = rf_next[pp], yf = yf_next[pp];
                         if (e != wave) {
-                            xf = (gpa < n_paths) ? glb(g.R)[row + gpa] : 0.0;
-                            yf = (gpa < n_paths) ? glb(g.R2)[row + gpa] : 0.0;
+                            xf = (gpa < n_paths) ? glb(pg->R)[row + gpa] : 0.0;
+                            yf = (gpa < n_paths) ? glb(pg->R2)[row + gpa] : 0.0;
                         }
                         sF[e * TW + lane + 64 * pp] = bond_exp(-((c - b1 * xf) - b2 * yf));
                     }
@@ -619,8 +639,8 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                 double inv_a = inv_a_w, b = fab_w.y;
                 uint64_t row = 0;
                 if (e != wave) {  // more fixings than waves
-                    row = (uint64_t)glb(g.fix_slot)[fbeg + e] * g.r_stride;
-                    const gptr<double> fp = glb((const double*)(g.fix_AB + fbeg + e));
+                    row = (uint64_t)glb(pg->fix_slot)[fbeg + e] * pg->r_stride;
+                    const gptr<double> fp = glb((const double*)(pg->fix_AB + fbeg + e));
                     inv_a = 1.0 / fp[0];
                     b = fp[1];
                 }
@@ -628,7 +648,7 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                 for (int pp = 0; pp < PPL; ++pp) {
                     const uint64_t gpa = pa + lane + 64 * pp;
                     double rf = rf_next[pp];
-                    if (e != wave) rf = (gpa < n_paths) ? glb(g.R)[row + gpa] : 0.0;
+                    if (e != wave) rf = (gpa < n_paths) ? glb(pg->R)[row + gpa] : 0.0;
                     sF[e * TW + lane + 64 * pp] = inv_a * bond_exp(b * rf);
                 }
             } else {
@@ -640,19 +660,19 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     };
     // a netting-set value of this unit: stored as its positive part (and the signed value when asked),
     // or for a part of a plan valued in parts, added to the earlier parts' values at the same place
-    const int x_mode = g.x_mode;
+    const int x_mode = pg->x_mode;
     auto store_set = [&](int s, uint64_t gp, double v) {
-        const uint64_t xi = ((uint64_t)s * g.n_dates + k) * n_paths + gp;
-        if (x_mode >= 2) v = glb(g.X)[xi] + v;
-        glb(g.X)[xi] = (x_mode == 1 || x_mode == 2) ? v : ((v > 0.0) ? v : 0.0);
-        if (g.set_mtm && (x_mode == 0 || x_mode == 3)) glb(g.set_mtm)[((uint64_t)k * g.S + s) * n_paths + gp] = v;
+        const uint64_t xi = ((uint64_t)s * pg->n_dates + k) * n_paths + gp;
+        if (x_mode >= 2) v = glb(pg->X)[xi] + v;
+        glb(pg->X)[xi] = (x_mode == 1 || x_mode == 2) ? v : ((v > 0.0) ? v : 0.0);
+        if (pg->set_mtm && (x_mode == 0 || x_mode == 3)) glb(pg->set_mtm)[((uint64_t)k * pg->S + s) * n_paths + gp] = v;
     };
     // Split mode: the netting-set values of a tile are the fixed-order sums of the waves' partials in
     // sPart, taken by waves 0 .. S-1 during the next tile's pass 2 (wave 0 starts the date's first
     // family, so it carries nothing there; sPart is rewritten only by the next tile's C) and after
     // the last tile.
     auto reduce_sets = [&](uint64_t t) {
-        if (g.set_mode == 0 && tid < g.S * kPathsPerTile) {
+        if (pg->set_mode == 0 && tid < pg->S * kPathsPerTile) {
             const int s = tid >> 6;
 #pragma unroll
             for (int pp = 0; pp < PPL; ++pp) {
@@ -670,6 +690,10 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     // which this tile last read in pass 1; it goes first, so its exponentials' FMA chains overlap the
     // waits of C's descriptor and LDS reads.
     for (uint64_t tile = tile0; tile < tile_end; ++tile) {
+        // The unit holds more values across the tile loop than the scalar registers do; with the
+        // plan's address made opaque here, the fields a tile uses are scalar loads (scalar-cache hits)
+        // where the tile uses them instead of registers held, or spilled, across the loop.
+        opaque(pg);
         const uint64_t p0 = tile * TW;
         // next tile's short rates: in flight during passes 1 and 2, used by the next tile's A
         prefetch(tile + 1, tile + 1 < tile_end);
@@ -805,16 +829,16 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
         __syncthreads();
         if (tile + 1 < tile_end) stage_a(p0 + TW);
         // C: netting sets, V_s = sum over the set's live classes of alpha F - beta A, in list order
-        if (g.set_mode == 0) {
-            for (int s = 0; s < g.S; ++s) {
+        if (pg->set_mode == 0) {
+            for (int s = 0; s < pg->S; ++s) {
                 // the set's classes alive at t (a prefix of its entries), split evenly over the waves
-                const int beg = cst(g.set_off)[s];
-                const int end = beg + cst(g.set_nlive)[(size_t)k * g.S + s];
+                const int beg = cst(pg->set_off)[s];
+                const int end = beg + cst(pg->set_nlive)[(size_t)k * pg->S + s];
                 const int per = (end - beg + kValueWaves - 1) / kValueWaves;
                 const int cb = beg + wave * per;
                 const int ce = min(end, cb + per);
-                const cptr<int> scl = cst(g.set_cls);
-                const cptr<double> sal = cst(g.set_alpha), sbe = cst(g.set_beta);
+                const cptr<int> scl = cst(pg->set_cls);
+                const cptr<double> sal = cst(pg->set_alpha), sbe = cst(pg->set_beta);
                 double v[PPL];
 #pragma unroll
                 for (int pp = 0; pp < PPL; ++pp) v[pp] = 0.0;
@@ -843,8 +867,8 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
         } else {
             // one wave per set, summing the same kValueWaves chunks in the same order as the split
             // path (each chunk from zero, then the chunk sums in chunk order): the same values bit for bit
-            for (int s = wave; s < g.S; s += kValueWaves) {
-                const int beg = cst(g.set_off)[s], end = beg + cst(g.set_nlive)[(size_t)k * g.S + s];
+            for (int s = wave; s < pg->S; s += kValueWaves) {
+                const int beg = cst(pg->set_off)[s], end = beg + cst(pg->set_nlive)[(size_t)k * pg->S + s];
                 const int per = (end - beg + kValueWaves - 1) / kValueWaves;
 #pragma unroll
                 for (int pp = 0; pp < PPL; ++pp) {
@@ -853,8 +877,8 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                         const int cb = beg + w * per, ce = min(end, cb + per);
                         double vw = 0.0;
                         for (int e = cb; e < ce; ++e) {
-                            const double2 c2 = sC[(lane + 64 * pp) * CS + cst(g.set_cls)[e]];
-                            vw = __builtin_fma(-cst(g.set_beta)[e], c2.y, __builtin_fma(cst(g.set_alpha)[e], c2.x, vw));
+                            const double2 c2 = sC[(lane + 64 * pp) * CS + cst(pg->set_cls)[e]];
+                            vw = __builtin_fma(-cst(pg->set_beta)[e], c2.y, __builtin_fma(cst(pg->set_alpha)[e], c2.x, vw));
                         }
                         v += vw;
                     }
@@ -866,8 +890,8 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
         // the legs of the classes alive at t_k, for the per-trade PFE rows (reads sC only)
         if constexpr (CUBE) {
             const gmut<double> cube =
-                glb((double*)(g.ccube + ((uint64_t)g.cube_row[k] * g.cube_C + g.cube_cls0) * n_paths));
-            for (int c = g.cls_live0[k] + wave; c < g.C; c += kValueWaves)
+                glb((double*)(pg->ccube + ((uint64_t)pg->cube_row[k] * pg->cube_C + pg->cube_cls0) * n_paths));
+            for (int c = pg->cls_live0[k] + wave; c < pg->C; c += kValueWaves)
 #pragma unroll
                 for (int pp = 0; pp < PPL; ++pp) {
                     const uint64_t gp = p0 + lane + 64 * pp;
@@ -879,29 +903,26 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                 }
         }
         // D: per-trade exposure sums (reads sC only); work unit = (item, slice of the tile's paths)
-        if (g.want_trade && icnt > 0) {
+        if (pg->want_trade && icnt > 0) {
             const uint64_t left = n_paths - p0;
             const int pvalid = (left < (uint64_t)TW) ? (int)left : TW;
             const double2* col = sC + icls;
-            const double sg = isgn;
+            const double sg = isgn * kPosScale;  // sign, scaled
             const int pb = islice_b;
             int pe = min(islice_e, pvalid);
             if (det) pe = (p0 == 0 && pb == 0) ? 1 : pb;  // grid index 0: path 0 only
 #pragma unroll 8  // eight class-leg reads in flight per slot
             for (int p = pb; p < pe; ++p) {
                 const double2 c2 = col[p * CS];
-                const double sfl = sg * c2.x;  // sign * F - (sign * K) A, exact sign flips
+                const double sfl = sg * c2.x;  // (sign * F - (sign * K) A) 2^-512, exact sign flips and scaling
 #pragma unroll
-                for (int q = 0; q < TPI; ++q) {
-                    const double x = __builtin_fma(-tK[q], c2.y, sfl);
-                    acc[q] += (x > 0.0) ? x : 0.0;
-                }
+                for (int q = 0; q < TPI; ++q) acc[q] += fma_pos(tK[q], c2.y, sfl);
             }
-            if (g.path_mtm) {
+            if (pg->path_mtm) {
                 for (int q = 0; q < icnt; ++q) {
                     const int j = ifirst + q;
-                    const double N = g.tr_N[j], NK = g.tr_NK[j];
-                    const gmut<double> dst = glb(g.path_mtm + ((uint64_t)k * g.n_trades + g.tr_orig[j]) * n_paths + p0);
+                    const double N = pg->tr_N[j], NK = pg->tr_NK[j];
+                    const gmut<double> dst = glb(pg->path_mtm + ((uint64_t)k * pg->n_trades + pg->tr_orig[j]) * n_paths + p0);
                     for (int p = islice_b; p < min(islice_e, pvalid); ++p)
                         dst[p] = N * col[p * CS].x - NK * col[p * CS].y;
                 }
@@ -910,7 +931,7 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
         __syncthreads();
     }
     if (tile0 < tile_end) reduce_sets(tile_end - 1);  // the last tile's set partials
-    if (g.want_trade) {
+    if (pg->want_trade) {
         // the slots' sums through LDS, then one fixed-order sum over the slices of each trade
         __syncthreads();
         double* sAcc = smem;  // [kTradeSlots][TPI]
@@ -918,11 +939,11 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
 #pragma unroll
             for (int q = 0; q < TPI; ++q) sAcc[tid * TPI + q] = acc[q];
         __syncthreads();
-        const gmut<double> part = glb(g.partial + ((uint64_t)k * g.n_chunks + chunk) * g.n_trades);
-        for (int w = tid; w < g.n_items * TPI; w += kValueThreads) {
+        const gmut<double> part = glb(pg->partial + ((uint64_t)k * pg->n_chunks + chunk) * pg->n_trades);
+        for (int w = tid; w < pg->n_items * TPI; w += kValueThreads) {
             const int it = w / TPI;
             const int q = w - it * TPI;
-            const int4 item = g.items[it];
+            const int4 item = pg->items[it];
             if (q >= item.z) continue;
             const int j = item.y + q;
             double e = 0.0;  // a matured class contributes exact zeros
@@ -930,9 +951,9 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                 const double* a = sAcc + (it - item0) * TPI + q;
                 double sum = 0.0;
                 for (int sl = 0; sl < ns; ++sl) sum += a[sl * n_li * TPI];
-                e = g.tr_absN[j] * sum;
+                e = pg->tr_absN[j] * (sum * kPosUnscale);
             }
-            part[g.tr_orig[j]] = e;
+            part[pg->tr_orig[j]] = e;
         }
     }
 }
