@@ -51,6 +51,34 @@ __device__ __forceinline__ double bond_exp(double x) {
     return __builtin_amdgcn_ldexp(p, (int)n);  // the conversion saturates and maps NaN to 0
 }
 
+// bond_exp with the same operations in the same order, its polynomial steps written as three-operand
+// fused multiply-adds: where the coefficients sit in vector registers, the compiler otherwise forms
+// each step as a copy of the coefficient into the accumulator followed by a two-operand multiply-add.
+__device__ __forceinline__ double fma_v(double a, double b, double c) {  // a b + c
+    double d;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+__device__ __forceinline__ double bond_exp_v(double x) {
+    const double n = __builtin_rint(x * 0x1.71547652b82fep+0);
+    double r = __builtin_fma(-n, 0x1.62e42fee00000p-1, x);
+    r = __builtin_fma(-n, 0x1.a39ef35793c76p-33, r);
+    double q = fma_v(0x1.1f66f85dfa20ap-29, r, 0x1.af38d53857513p-26);
+    q = fma_v(q, r, 0x1.27e4e1ea0f53ap-22);
+    q = fma_v(q, r, 0x1.71de0d9c145d0p-19);
+    q = fma_v(q, r, 0x1.a01a01a47db6fp-16);
+    q = fma_v(q, r, 0x1.a01a01a7c8d47p-13);
+    q = fma_v(q, r, 0x1.6c16c16c167dep-10);
+    q = fma_v(q, r, 0x1.11111111109adp-7);
+    q = fma_v(q, r, 0x1.5555555555555p-5);
+    q = fma_v(q, r, 0x1.5555555555556p-3);
+    double p = __builtin_fma(q, r, 0.5);
+    p = __builtin_fma(p, r, 1.0);
+    p = __builtin_fma(p, r, 1.0);
+    p = (__builtin_fabs(x) > 1.0e9) ? 1.0 : p;
+    return __builtin_amdgcn_ldexp(p, (int)n);
+}
+
 // G2++ (models/hull_white_2f.rs:58-113, Brigo-Mercurio eq. 4.11), same expression order.
 // Own variance of a factor with reversion z and volatility vol.
 __device__ __forceinline__ double hw2_var(double z, double vol, double tau) {

@@ -623,13 +623,13 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                             xf = (gpa < n_paths) ? glb(pg->R)[row + gpa] : 0.0;
                             yf = (gpa < n_paths) ? glb(pg->R2)[row + gpa] : 0.0;
                         }
-                        sF[e * TW + lane + 64 * pp] = bond_exp(-((c - b1 * xf) - b2 * yf));
+                        sF[e * TW + lane + 64 * pp] = bond_exp_v(-((c - b1 * xf) - b2 * yf));
                     }
                 } else {
                     const int d = dfirst + (e - nfix);
 #pragma unroll
                     for (int pp = 0; pp < PPL; ++pp)
-                        sP[d * TW + lane + 64 * pp] = bond_exp((abd[2 * d] - abd[2 * d + 1] * r_next[pp]) - b2d[d] * y_next[pp]);
+                        sP[d * TW + lane + 64 * pp] = bond_exp_v((abd[2 * d] - abd[2 * d + 1] * r_next[pp]) - b2d[d] * y_next[pp]);
                 }
             }
             return;
@@ -649,12 +649,12 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                     const uint64_t gpa = pa + lane + 64 * pp;
                     double rf = rf_next[pp];
                     if (e != wave) rf = (gpa < n_paths) ? glb(pg->R)[row + gpa] : 0.0;
-                    sF[e * TW + lane + 64 * pp] = inv_a * bond_exp(b * rf);
+                    sF[e * TW + lane + 64 * pp] = inv_a * bond_exp_v(b * rf);
                 }
             } else {
                 const int d = dfirst + (e - nfix);
 #pragma unroll
-                for (int pp = 0; pp < PPL; ++pp) sP[d * TW + lane + 64 * pp] = abd[2 * d] * bond_exp(-(abd[2 * d + 1] * r_next[pp]));
+                for (int pp = 0; pp < PPL; ++pp) sP[d * TW + lane + 64 * pp] = abd[2 * d] * bond_exp_v(-(abd[2 * d + 1] * r_next[pp]));
             }
         }
     };
@@ -1196,10 +1196,14 @@ template __global__ void value_path_kernel<2, 4, 2>(const ValuePlan*, const int4
 template __global__ void value_path_kernel<2, 4, 4>(const ValuePlan*, const int4*);
 template __global__ void value_path_kernel<2, 4, 8>(const ValuePlan*, const int4*);
 
-// The valuation kernels' exp (bond_exp) on n values, for its accuracy check (eqhip_bond_exp).
+// The valuation kernels' exp on n values, for its accuracy check (eqhip_bond_exp): bond_exp_v's value
+// where bond_exp gives the same bits, NaN where the two forms differ.
 __global__ void bond_exp_kernel(const double* x, double* y, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) y[i] = bond_exp(x[i]);
+    if (i < n) {
+        const double a = bond_exp(x[i]), b = bond_exp_v(x[i]);
+        y[i] = (__builtin_bit_cast(uint64_t, a) == __builtin_bit_cast(uint64_t, b)) ? b : __builtin_nan("");
+    }
 }
 
 // One thread per path, blockIdx.y the date. The bonds of the date's remaining flows are summed in

@@ -144,7 +144,8 @@ int eqhip_context_destroy(eqhip_context* ctx);
 
 /* No counterpart in the reference: a self-check for callers and tests. y[i] = the valuation kernels'
  * FP64 exp of x[i] (the e^x of every per-path zero-coupon bond A exp(-B r)), computed on the device;
- * within 0.81 ulp of e^x for |x| <= 708, inf / 0 beyond, NaN propagated. */
+ * within 0.81 ulp of e^x for |x| <= 708, inf / 0 beyond, NaN propagated. The kernels compile the exp in
+ * two instruction forms with the same arithmetic; y[i] is NaN wherever the two differ in any bit. */
 int eqhip_bond_exp(eqhip_context* ctx, const double* x, double* y, int32_t n);
 
 /* ---- reference-shaped entry points -------------------------------------------------------- */

@@ -1,6 +1,8 @@
-"""The FP64 exp of the per-path zero-coupon bonds (bond_exp in eqhip_device.hpp), run on the device
-through eqhip_bond_exp and checked against e^x in 256-bit arithmetic: below one ulp over the range the
-valuation kernels use and far beyond it, exact overflow to inf and underflow to 0, NaN propagated."""
+"""The FP64 exp of the per-path zero-coupon bonds (bond_exp and bond_exp_v in eqhip_device.hpp), run on
+the device through eqhip_bond_exp and checked against e^x in 256-bit arithmetic: below one ulp over the
+range the valuation kernels use and far beyond it, exact overflow to inf and underflow to 0, NaN
+propagated. The entry point returns NaN wherever the two forms differ in any bit, so every finite check
+below also checks that they agree."""
 import math
 
 import numpy as np
