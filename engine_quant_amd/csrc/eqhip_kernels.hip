@@ -672,8 +672,8 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     // family, so it carries nothing there; sPart is rewritten only by the next tile's C) and after
     // the last tile.
     auto reduce_sets = [&](uint64_t t) {
-        if (pg->set_mode == 0 && tid < pg->S * kPathsPerTile) {
-            const int s = tid >> 6;
+        if (pg->set_mode == 0 && wave < pg->S) {
+            const int s = wave;
 #pragma unroll
             for (int pp = 0; pp < PPL; ++pp) {
                 double v = 0.0;
