@@ -911,13 +911,31 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
             const int pb = islice_b;
             int pe = min(islice_e, pvalid);
             if (det) pe = (p0 == 0 && pb == 0) ? 1 : pb;  // grid index 0: path 0 only
-#pragma unroll 8  // eight class-leg reads in flight per slot
-            for (int p = pb; p < pe; ++p) {
-                const double2 c2 = col[p * CS];
-                const double sfl = sg * c2.x;  // (sign * F - (sign * K) A) 2^-512, exact sign flips and scaling
+            // paths in order, eight class-leg reads in flight, then four, two and one: a slice's
+            // remainder costs no loop overhead per path
+            auto paths = [&](int p, auto width) {
+                constexpr int W = decltype(width)::value;
+                double2 c2[W];
 #pragma unroll
-                for (int q = 0; q < TPI; ++q) acc[q] += fma_pos(tK[q], c2.y, sfl);
+                for (int u = 0; u < W; ++u) c2[u] = col[(p + u) * CS];
+#pragma unroll
+                for (int u = 0; u < W; ++u) {
+                    const double sfl = sg * c2[u].x;  // (sign * F - (sign * K) A) 2^-512, exact sign flips and scaling
+#pragma unroll
+                    for (int q = 0; q < TPI; ++q) acc[q] += fma_pos(tK[q], c2[u].y, sfl);
+                }
+            };
+            int p = pb;
+            for (; p + 8 <= pe; p += 8) paths(p, std::integral_constant<int, 8>{});
+            if (p + 4 <= pe) {
+                paths(p, std::integral_constant<int, 4>{});
+                p += 4;
             }
+            if (p + 2 <= pe) {
+                paths(p, std::integral_constant<int, 2>{});
+                p += 2;
+            }
+            if (p < pe) paths(p, std::integral_constant<int, 1>{});
             if (pg->path_mtm) {
                 for (int q = 0; q < icnt; ++q) {
                     const int j = ifirst + q;
@@ -1274,21 +1292,34 @@ template <int NB>
 __device__ void locate_rank(const unsigned int* hist, unsigned int* scan, unsigned long long rank,
                             unsigned int* s_bin, unsigned long long* s_before) {
     static_assert(NB % kSelectThreads == 0, "bins per thread");
-    const int tid = threadIdx.x;
+    constexpr int kWaves = kSelectThreads / 64;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int per = NB / kSelectThreads;
     unsigned int local = 0;
     for (int b = 0; b < per; ++b) local += hist[tid * per + b];
-    scan[tid] = local;
-    __syncthreads();
-    for (int off = 1; off < kSelectThreads; off <<= 1) {  // inclusive Hillis-Steele scan
-        unsigned int v = (tid >= off) ? scan[tid - off] : 0u;
-        __syncthreads();
-        scan[tid] += v;
-        __syncthreads();
+    // inclusive scan of the threads' counts: within each wave by shuffles, then over the wave totals
+    unsigned int incl = local;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned int v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
     }
-    const unsigned long long incl = scan[tid];
-    const unsigned long long excl = incl - local;
-    if (rank >= excl && rank < incl) {
+    if (lane == 63) scan[wave] = incl;
+    __syncthreads();
+    if (tid < 64) {
+        const unsigned int t = (lane < kWaves) ? scan[lane] : 0u;
+        unsigned int x = t;
+#pragma unroll
+        for (int off = 1; off < kWaves; off <<= 1) {
+            const unsigned int v = __shfl_up(x, off, 64);
+            if (lane >= off) x += v;
+        }
+        if (lane < kWaves) scan[kWaves + lane] = x - t;  // the counts of the earlier waves
+    }
+    __syncthreads();
+    const unsigned long long incl_all = (unsigned long long)scan[kWaves + wave] + incl;
+    const unsigned long long excl = incl_all - local;
+    if (rank >= excl && rank < incl_all) {
         unsigned long long cum = excl;
         int b = tid * per;
         for (;; ++b) {
@@ -1352,6 +1383,31 @@ struct TradeKeys {  // a trade's positive parts, formed from its class's legs as
     }
     __device__ double out(double v) const { return absN * v; }
 };
+
+// Sorts cand[0, nc) ascending in LDS (bitonic, padded with the largest key to a power of two).
+__device__ void sort_candidates(unsigned long long* cand, unsigned int nc) {
+    const int tid = threadIdx.x;
+    unsigned int m = 1;
+    while (m < nc) m <<= 1;
+    for (unsigned int i = nc + tid; i < m; i += kSelectThreads) cand[i] = ~0ull;
+    __syncthreads();
+    for (unsigned int size = 2; size <= m; size <<= 1) {
+        for (unsigned int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (unsigned int i = tid; i < m; i += kSelectThreads) {
+                const unsigned int j = i ^ stride;
+                if (j > i) {
+                    const bool up = (i & size) == 0;
+                    const unsigned long long a = cand[i], b = cand[j];
+                    if ((a > b) == up) {
+                        cand[i] = b;
+                        cand[j] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
 
 // EE (the mean, when ee is not null) and the rank-q order statistic of one row of n keys.
 template <class Keys>
@@ -1428,11 +1484,15 @@ __device__ void select_row(const Keys& keys, const uint64_t n, const uint64_t q,
         for (int j = 0; j < per; ++j) c += hist2[tid * per + j];
         if (c) atomicAdd(&hist[wb + (unsigned)(tid * per) / 1024u], c);
     }
-    for (int w = kSelectThreads / 2; w > 0; w >>= 1) {
-        if (tid < w) red[tid] = red[tid] + red[tid + w];
-        __syncthreads();
+    if (ee) {  // the EE sum's fixed-order tree (a row without EE skips its barriers)
+        for (int w = kSelectThreads / 2; w > 0; w >>= 1) {
+            if (tid < w) red[tid] = red[tid] + red[tid + w];
+            __syncthreads();
+        }
+        if (tid == 0) *ee = keys.out(red[0] / (double)n);
+    } else {
+        __syncthreads();  // the window totals are in the binade histogram
     }
-    if (tid == 0 && ee) *ee = keys.out(red[0] / (double)n);
     locate_rank<4096>(hist, scan, q, &s_bin, &s_before);
     if (tid == 0) {
         s_prefix = (unsigned long long)s_bin << 52;
@@ -1514,27 +1574,7 @@ __device__ void select_row(const Keys& keys, const uint64_t n, const uint64_t q,
         if ((key >> hi) == (prefix >> hi)) cand[atomicAdd(&s_ncand, 1u)] = key;
     });
     __syncthreads();
-    const unsigned int nc = s_ncand;
-    unsigned int m = 1;
-    while (m < nc) m <<= 1;
-    for (unsigned int i = nc + tid; i < m; i += kSelectThreads) cand[i] = ~0ull;
-    __syncthreads();
-    for (unsigned int size = 2; size <= m; size <<= 1) {
-        for (unsigned int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (unsigned int i = tid; i < m; i += kSelectThreads) {
-                const unsigned int j = i ^ stride;
-                if (j > i) {
-                    const bool up = (i & size) == 0;
-                    const unsigned long long a = cand[i], b = cand[j];
-                    if ((a > b) == up) {
-                        cand[i] = b;
-                        cand[j] = a;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
+    sort_candidates(cand, s_ncand);
     if (tid == 0) *pfe = keys.out(__longlong_as_double(cand[s_rank]));
 }
 

@@ -125,13 +125,13 @@ def test_plan_without_cube_refuses(ctx):
         plan.close()
 
 
-def test_exact_select_on_adversarial_rows(ctx):
-    """Rows of 200,003 paths that drive each branch of the select: the target binade inside the window
-    centred by the row's first 8192 keys, a window misplaced by those keys, a target prefix still
-    holding more than 4096 keys after the window (further 12-bit passes), ties at the target, a zero
-    target, and keys spread over 60 binades. The quantiles must be the oracle's, bit for bit."""
+@pytest.mark.parametrize("n", [200_003, 100_003])
+def test_exact_select_on_adversarial_rows(ctx, n):
+    """Rows that drive each branch of the select: the target binade inside the window centred by the
+    row's first 8192 keys, a window misplaced by those keys, a target prefix still holding more than
+    4096 keys after the window (further 12-bit passes), ties at the target, a zero target, and keys
+    spread over 60 binades, at two row lengths. The quantiles must be the oracle's, bit for bit."""
     rng = np.random.default_rng(29)
-    n = 200_003
     rows = [rng.normal(5e4, 1e5, n),                          # the common case
             np.exp2(rng.uniform(-30.0, 30.0, n))]              # few keys per binade
     r = rng.normal(1e6, 2e5, n)
