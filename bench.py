@@ -1092,6 +1092,10 @@ def main():
                      "traffic_source": os.path.relpath(profile, ROOT) if traffic else None,
                      "algorithmic_flops": flops, "algorithmic_exps": n_exp,
                      "algorithmic_unit": "2m+6 FLOPs per live (trade, path, date), m = remaining fixed flows",
+                     "frac_note": "achieved is the per-trade algorithm's work over the kernel's time; the "
+                                  "kernel shares each schedule class's bonds and annuities across its trades "
+                                  "and executes about a fifth of that work, so frac can exceed 1; "
+                                  "executed_frac_counters is the fraction of the FP64 peak the kernel executes",
                      "executed_flops_model": exec_flops, "executed_tflops_model": executed,
                      "executed_flops_counters": counted_flops,
                      "executed_tflops_counters": counted_tflops,

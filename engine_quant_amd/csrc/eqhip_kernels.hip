@@ -788,21 +788,30 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
             double carry[PPL];
 #pragma unroll
             for (int pp = 0; pp < PPL; ++pp) carry[pp] = 0.0;
+            // W totals from wave w2: the reads first, then the adds in wave order
+            auto totals = [&](int w2, auto width) {
+                constexpr int W = decltype(width)::value;
+                double t[W][PPL];
+#pragma unroll
+                for (int u = 0; u < W; ++u)
+#pragma unroll
+                    for (int pp = 0; pp < PPL; ++pp) t[u][pp] = sSeg[(w2 + u) * TW + lane + 64 * pp];
+#pragma unroll
+                for (int u = 0; u < W; ++u)
+#pragma unroll
+                    for (int pp = 0; pp < PPL; ++pp) carry[pp] += t[u][pp];
+            };
             int w2 = carry_lo;
-            for (; w2 + CB <= wave; w2 += CB) {
-                double t8[CB][PPL];
-#pragma unroll
-                for (int u = 0; u < CB; ++u)
-#pragma unroll
-                    for (int pp = 0; pp < PPL; ++pp) t8[u][pp] = sSeg[(w2 + u) * TW + lane + 64 * pp];
-#pragma unroll
-                for (int u = 0; u < CB; ++u)
-#pragma unroll
-                    for (int pp = 0; pp < PPL; ++pp) carry[pp] += t8[u][pp];
+            for (; w2 + CB <= wave; w2 += CB) totals(w2, std::integral_constant<int, CB>{});
+            if (CB > 4 && w2 + 4 <= wave) {  // the rest in steps of four, two and one
+                totals(w2, std::integral_constant<int, (CB > 4 ? 4 : 1)>{});
+                w2 += 4;
             }
-            for (; w2 < wave; ++w2)
-#pragma unroll
-                for (int pp = 0; pp < PPL; ++pp) carry[pp] += sSeg[w2 * TW + lane + 64 * pp];
+            if (CB > 2 && w2 + 2 <= wave) {
+                totals(w2, std::integral_constant<int, (CB > 2 ? 2 : 1)>{});
+                w2 += 2;
+            }
+            if (w2 < wave) totals(w2, std::integral_constant<int, 1>{});
             // the leading classes' annuities from pass 1 (the same values sC holds), then any further
             // carried flows by reading sC back
 #pragma unroll
@@ -842,25 +851,27 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                 double v[PPL];
 #pragma unroll
                 for (int pp = 0; pp < PPL; ++pp) v[pp] = 0.0;
-                int e = cb;
-                for (; e + FU <= ce; e += FU) {
-                    double2 c2[FU][PPL];
+                // W entries from entry e: the class-leg reads first, then the FMAs in entry order
+                auto entries = [&](int e, auto width) {
+                    constexpr int W = decltype(width)::value;
+                    double2 c2[W][PPL];
 #pragma unroll
-                    for (int u = 0; u < FU; ++u)
+                    for (int u = 0; u < W; ++u)
 #pragma unroll
                         for (int pp = 0; pp < PPL; ++pp) c2[u][pp] = sC[(lane + 64 * pp) * CS + scl[e + u]];
 #pragma unroll
-                    for (int u = 0; u < FU; ++u)
+                    for (int u = 0; u < W; ++u)
 #pragma unroll
                         for (int pp = 0; pp < PPL; ++pp)
                             v[pp] = __builtin_fma(-sbe[e + u], c2[u][pp].y, __builtin_fma(sal[e + u], c2[u][pp].x, v[pp]));
+                };
+                int e = cb;
+                for (; e + FU <= ce; e += FU) entries(e, std::integral_constant<int, FU>{});
+                if (FU > 2 && e + 2 <= ce) {  // the rest in steps of two and one
+                    entries(e, std::integral_constant<int, (FU > 2 ? 2 : 1)>{});
+                    e += 2;
                 }
-                for (; e < ce; ++e)
-#pragma unroll
-                    for (int pp = 0; pp < PPL; ++pp) {
-                        const double2 c2 = sC[(lane + 64 * pp) * CS + scl[e]];
-                        v[pp] = __builtin_fma(-sbe[e], c2.y, __builtin_fma(sal[e], c2.x, v[pp]));
-                    }
+                if (e < ce) entries(e, std::integral_constant<int, 1>{});
 #pragma unroll
                 for (int pp = 0; pp < PPL; ++pp) sPart[(s * kValueWaves + wave) * TW + lane + 64 * pp] = v[pp];
             }
