@@ -1294,7 +1294,9 @@ __global__ void reset_counter_kernel(unsigned int* counter) { *counter = 0u; }
 // kSelectWin binades around the order statistic of the row's first 8192 keys, the binade's top 10
 // mantissa bits. When the target binade lies in that window, one pass fixes 22 bits of the key;
 // otherwise 12-bit MSB radix passes narrow the prefix. Once at most kSelectCand candidates share
-// the prefix they are compacted into LDS and ranked by a bitonic sort.
+// the prefix they are compacted into LDS and ranked by a bitonic sort. The first pass also keeps in
+// LDS the keys of a bracket around the sample's order statistic; when the prefix's bin lies inside
+// it, the candidates come from there and the compaction pass over the row is skipped.
 // ---------------------------------------------------------------------------------------------
 namespace {
 
@@ -1360,9 +1362,29 @@ __device__ __forceinline__ void hist_add(unsigned int* hist, unsigned int b) {
     }
 }
 
-// Visits x[i] for i = start, start + T, start + 2T, ... in that order, with eight loads in flight
-// per thread (a pass over a row is latency-bound with one).
-constexpr int kSelectU = 8;
+// Appends key to buf (capacity cap) for the lanes where in holds, in lane order, with one LDS atomic
+// per wave; the count keeps growing past the capacity, so an overflow shows.
+__device__ __forceinline__ void wave_append(bool in, unsigned long long key, unsigned long long* buf, unsigned int cap,
+                                            unsigned int* count) {
+    const uint64_t m = __ballot(in);
+    if (m == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int lead = __ffsll((long long)m) - 1;
+    unsigned int base = 0u;
+    if (lane == lead) base = atomicAdd(count, (unsigned int)__popcll(m));
+    base = __shfl(base, lead, 64);
+    if (in) {
+        const unsigned int pos =
+            base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        if (pos < cap) buf[pos] = key;
+    }
+}
+
+// Visits x[i] for i = start, start + T, start + 2T, ... in that order, with four loads in flight
+// per thread (two blocks of 16 waves share a CU: the select kernels are compiled for 8 waves per SIMD,
+// 64 VGPRs, and a pass over a row is latency-bound with one load in flight).
+constexpr int kSelectU = 4;
+constexpr int kSelectS = 8;  // sample keys per thread: the row's first kSelectS * kSelectThreads keys
 template <class Keys, class F>
 __device__ __forceinline__ void select_pass(const Keys& keys, uint64_t n, uint64_t start, F f) {
     constexpr int U = kSelectU;
@@ -1431,39 +1453,77 @@ __device__ void select_row(const Keys& keys, const uint64_t n, const uint64_t q,
     unsigned int* hist2 = (unsigned int*)cand;
     static_assert(kSelectWin * 1024 * sizeof(unsigned int) <= sizeof(cand), "window histogram fits");
     __shared__ unsigned long long s_prefix, s_rank, s_before, s_min, s_max;
-    __shared__ unsigned int s_bin, s_count, s_ncand;
+    __shared__ unsigned int s_bin, s_count, s_ncand, s_low, s_nb;
+    // the keys of the first pass's bracket (see below)
+    __shared__ unsigned long long bkt[kSelectBracket];
 
     const int tid = threadIdx.x;
 
     for (int b = tid; b < 4096; b += kSelectThreads) hist[b] = 0u;
     for (int b = tid; b < kSelectWin * 1024; b += kSelectThreads) hist2[b] = 0u;
     __syncthreads();
-    // pass 1: EE sum, histogram of bits 63..52 and the window histogram. The first kSelectU keys of
+    // pass 1: EE sum, histogram of bits 63..52 and the window histogram. The first kSelectS keys of
     // every thread (x[tid + u T], the row's first U T keys) are binned first; the binade of their
     // rank-scaled order statistic centres the window.
-    constexpr uint64_t kSample = (uint64_t)kSelectU * kSelectThreads;
-    unsigned long long k0[kSelectU];
+    constexpr uint64_t kSample = (uint64_t)kSelectS * kSelectThreads;
+    unsigned long long k0[kSelectS];
 #pragma unroll
-    for (int u = 0; u < kSelectU; ++u) {
+    for (int u = 0; u < kSelectS; ++u) {
         const uint64_t i = (uint64_t)tid + (uint64_t)u * kSelectThreads;
         k0[u] = (i < n) ? keys(i) : 0ull;
     }
 #pragma unroll
-    for (int u = 0; u < kSelectU; ++u)
+    for (int u = 0; u < kSelectS; ++u)
         if ((uint64_t)tid + (uint64_t)u * kSelectThreads < n) hist_add(hist, (unsigned int)(k0[u] >> 52));
     __syncthreads();
     const uint64_t ns = n < kSample ? n : kSample;
     locate_rank<4096>(hist, scan, (q * ns) / n, &s_bin, &s_before);
     const unsigned int est = s_bin;
     // window [wb, wb + kSelectWin) of binades, inside bins 1..4095 (bin 0 holds zeros and subnormals)
-    const unsigned int wb = est > (unsigned)(kSelectWin / 2) ? min(est - kSelectWin / 2, 4096u - kSelectWin) : 1u;
+    unsigned int wb = est > (unsigned)(kSelectWin / 2) ? min(est - kSelectWin / 2, 4096u - kSelectWin) : 1u;
+    if (est == 0u) {
+        // the sample's order statistic is a zero: where the row's is positive (a row just past its zeros)
+        // it is among the smallest positive keys, so the window starts one binade below the sample's
+        // lowest positive binade
+        if (tid == 0) s_low = 4096u;
+        __syncthreads();
+        for (int b = tid; b < 4096; b += kSelectThreads)
+            if (b > 0 && hist[b] > 0u) atomicMin(&s_low, (unsigned int)b);
+        __syncthreads();
+        const unsigned int low = s_low;
+        if (low < 4096u) wb = min(low > 1u ? low - 1u : 1u, 4096u - kSelectWin);
+    }
+    // The bracket: the keys between the sample's order statistics kr ranks either side of its scaled
+    // rank (3.5 standard deviations of a sample quantile, fewer when the row is long enough that the
+    // bracket would overfill), widened to their histogram bins, are kept in LDS during the pass. When
+    // the bin the histograms locate the rank-q key in lies inside the bracket and the bracket kept every
+    // key, the candidates come from LDS and the compaction pass over the row is skipped.
+    const uint64_t rc = (q * ns) / n;
+    const double pq = ((double)q + 0.5) / (double)n;
+    const double sds = sqrt((double)ns * pq * (1.0 - pq));
+    const double kr = fmin(3.5 * sds + 1.0, (0.6 * kSelectBracket * (double)ns) / (2.0 * (double)n));
+    const bool use_bkt = kr >= sds + 1.0;
+    const uint64_t r_lo = (use_bkt && rc > (uint64_t)kr) ? rc - (uint64_t)kr : 0;
+    const uint64_t r_hi = use_bkt ? min(ns - 1, rc + (uint64_t)kr) : 0;
+    unsigned int blo_bin = 0u, bhi_bin = 0u;
+    unsigned long long blo_before = 0ull, bhi_before = 0ull;
+    if (use_bkt) {
+        __syncthreads();  // every thread has read est
+        locate_rank<4096>(hist, scan, r_lo, &s_bin, &s_before);
+        blo_bin = s_bin;
+        blo_before = s_before;
+        __syncthreads();
+        locate_rank<4096>(hist, scan, r_hi, &s_bin, &s_before);
+        bhi_bin = s_bin;
+        bhi_before = s_before;
+    }
     // A key inside the window is counted in the window histogram only (its binade's bin receives the
     // window's per-binade totals after the pass), so every key costs one LDS atomic; the sample keys
     // inside the window move there from the binade histogram.
     double sum = 0.0;
     unsigned int c0 = 0u;
 #pragma unroll
-    for (int u = 0; u < kSelectU; ++u) {
+    for (int u = 0; u < kSelectS; ++u) {
         if ((uint64_t)tid + (uint64_t)u * kSelectThreads < n) {
             sum += __longlong_as_double(k0[u]);
             const unsigned int b = (unsigned int)(k0[u] >> 52);
@@ -1472,6 +1532,32 @@ __device__ void select_row(const Keys& keys, const uint64_t n, const uint64_t q,
                 atomicSub(&hist[b], 1u);
                 hist_add(hist2, (w << 10) | (unsigned int)((k0[u] >> 42) & 1023u));
             }
+        }
+    }
+    // the bracket's bounds: the first key of r_lo's bin and the end of r_hi's, a window binade's bin
+    // narrowed by the sample's window histogram
+    unsigned long long b_lo = ~0ull, b_hi = 0ull;  // empty unless use_bkt
+    if (use_bkt) {
+        __syncthreads();  // the sample's window histogram is complete
+        auto bin_of = [&](unsigned int b, unsigned long long r, bool end) {
+            const unsigned int w = b - wb;
+            if (w < (unsigned)kSelectWin) {
+                locate_rank<1024>(hist2 + (w << 10), scan, r, &s_bin, &s_before);
+                const unsigned long long st = ((unsigned long long)b << 52) | ((unsigned long long)s_bin << 42);
+                __syncthreads();  // every thread has read s_bin
+                return end ? st + (1ull << 42) : st;
+            }
+            return end ? ((unsigned long long)(b + 1u) << 52) : ((unsigned long long)b << 52);
+        };
+        b_lo = bin_of(blo_bin, r_lo - blo_before, false);
+        b_hi = bin_of(bhi_bin, r_hi - bhi_before, true);
+        if (b_lo == 0ull) b_lo = 1ull;  // zeros are counted, never kept
+        if (tid == 0) s_nb = 0u;
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kSelectS; ++u) {
+            const bool valid = (uint64_t)tid + (uint64_t)u * kSelectThreads < n;
+            wave_append(valid && k0[u] >= b_lo && k0[u] < b_hi, k0[u], bkt, kSelectBracket, &s_nb);
         }
     }
     // keys in bin 0 (zero exposures, the common case) are counted per thread, not by LDS atomics
@@ -1483,6 +1569,7 @@ __device__ void select_row(const Keys& keys, const uint64_t n, const uint64_t q,
         if (b == 0u) ++c0;
         else if (w < (unsigned)kSelectWin) hist_add(hist2, (w << 10) | (unsigned int)((key >> 42) & 1023u));
         else hist_add(hist, b);
+        if (use_bkt) wave_append(key >= b_lo && key < b_hi, key, bkt, kSelectBracket, &s_nb);
     });
     if (c0) atomicAdd(&hist[0], c0);
     red[tid] = sum;
@@ -1577,25 +1664,35 @@ __device__ void select_row(const Keys& keys, const uint64_t n, const uint64_t q,
         if (tid == 0) *pfe = keys.out(__longlong_as_double(s_prefix));
         return;
     }
-    // compact the candidates sharing the prefix and rank them exactly
+    // compact the candidates sharing the prefix and rank them exactly: from the bracket when the
+    // prefix's bin lies inside it and the bracket kept every key, else by a pass over the row
     if (tid == 0) s_ncand = 0u;
     __syncthreads();
     const unsigned long long prefix = s_prefix;
-    select_pass(keys, n, tid, [&](unsigned long long key) {
-        if ((key >> hi) == (prefix >> hi)) cand[atomicAdd(&s_ncand, 1u)] = key;
-    });
+    const unsigned long long pend = prefix + (1ull << hi);
+    if (use_bkt && s_nb <= (unsigned)kSelectBracket && prefix >= b_lo && pend <= b_hi) {
+        const unsigned int nb = s_nb;
+        for (unsigned int j = tid; j < nb; j += kSelectThreads) {
+            const unsigned long long key = bkt[j];
+            if ((key >> hi) == (prefix >> hi)) cand[atomicAdd(&s_ncand, 1u)] = key;
+        }
+    } else {
+        select_pass(keys, n, tid, [&](unsigned long long key) {
+            if ((key >> hi) == (prefix >> hi)) cand[atomicAdd(&s_ncand, 1u)] = key;
+        });
+    }
     __syncthreads();
     sort_candidates(cand, s_ncand);
     if (tid == 0) *pfe = keys.out(__longlong_as_double(cand[s_rank]));
 }
 
-__global__ void __launch_bounds__(kSelectThreads) select_kernel(const SelectRow* rows) {
+__global__ void __launch_bounds__(kSelectThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) select_kernel(const SelectRow* rows) {
     const SelectRow row = rows[blockIdx.x];
     select_row(RowKeys{(const unsigned long long*)row.x}, row.n, row.q, row.ee, row.pfe);
 }
 
 // Per-trade PFE95 rows of one date group (each row's class legs written by the valuation).
-__global__ void __launch_bounds__(kSelectThreads) trade_select_kernel(const TradeSelectRow* rows) {
+__global__ void __launch_bounds__(kSelectThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) trade_select_kernel(const TradeSelectRow* rows) {
     const TradeSelectRow row = rows[blockIdx.x];
     select_row(TradeKeys{row.c, row.s, row.sK, row.absN}, row.n, row.q, nullptr, row.pfe);
 }

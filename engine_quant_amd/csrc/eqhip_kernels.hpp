@@ -28,6 +28,7 @@ constexpr int kMaxSplitSets = 8;     // netting sets whose class sums are split 
 constexpr int kSelectThreads = 1024;  // EE / PFE select block
 constexpr int kPvLdsPillars = 256;    // curves up to this many pillars are staged in LDS by pv_dv01_kernel
 constexpr int kSelectCand = 4096;     // candidates ranked in LDS by the select kernel
+constexpr int kSelectBracket = 2048;  // keys of the first pass's bracket kept in LDS by the select kernel
 constexpr int kSelectWin = 8;         // binades of the select kernel's first-pass mantissa histogram
 constexpr double kDateTolHost = 1e-9; // date matching tolerance, products/irs.rs:151-154,173
 
