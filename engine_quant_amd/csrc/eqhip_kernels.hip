@@ -408,8 +408,11 @@ __device__ void adjoint_path(const AdjArgs& g, uint64_t p, double acc[5]) {
     acc[4] += vsum;
 }
 
+// Compiled for four waves per SIMD (128 VGPRs): the reverse sweep waits on its checkpoint and bond
+// loads, and four waves hide them better than the two its unconstrained 190 VGPRs allow, the spills
+// to scratch included (5.35 -> 5.05 ms per adjoint step; three or five waves were slower).
 template <int DISC, int RNG>
-__global__ void __launch_bounds__(kAdjThreads) adjoint_kernel(AdjArgs g) {
+__global__ void __launch_bounds__(kAdjThreads) __attribute__((amdgpu_waves_per_eu(4))) adjoint_kernel(AdjArgs g) {
     __shared__ double red[5][kAdjThreads];
     const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
