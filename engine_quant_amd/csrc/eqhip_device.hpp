@@ -47,7 +47,13 @@ __device__ __forceinline__ double bond_exp(double x) {
     double p = __builtin_fma(q, r, 0.5);
     p = __builtin_fma(p, r, 1.0);
     p = __builtin_fma(p, r, 1.0);
-    p = (__builtin_fabs(x) > 1.0e9) ? 1.0 : p;
+    // beyond |x| = 1e9 the result is inf or 0 whatever p's low word holds: the high word of 1.0 is enough
+    // (one select), and a NaN x fails the test and propagates
+    {
+        const uint64_t pb = __builtin_bit_cast(uint64_t, p);
+        const uint32_t ph = (__builtin_fabs(x) > 1.0e9) ? 0x3ff00000u : (uint32_t)(pb >> 32);
+        p = __builtin_bit_cast(double, ((uint64_t)ph << 32) | (uint32_t)pb);
+    }
     return __builtin_amdgcn_ldexp(p, (int)n);  // the conversion saturates and maps NaN to 0
 }
 
@@ -75,7 +81,13 @@ __device__ __forceinline__ double bond_exp_v(double x) {
     double p = __builtin_fma(q, r, 0.5);
     p = __builtin_fma(p, r, 1.0);
     p = __builtin_fma(p, r, 1.0);
-    p = (__builtin_fabs(x) > 1.0e9) ? 1.0 : p;
+    // beyond |x| = 1e9 the result is inf or 0 whatever p's low word holds: the high word of 1.0 is enough
+    // (one select), and a NaN x fails the test and propagates
+    {
+        const uint64_t pb = __builtin_bit_cast(uint64_t, p);
+        const uint32_t ph = (__builtin_fabs(x) > 1.0e9) ? 0x3ff00000u : (uint32_t)(pb >> 32);
+        p = __builtin_bit_cast(double, ((uint64_t)ph << 32) | (uint32_t)pb);
+    }
     return __builtin_amdgcn_ldexp(p, (int)n);
 }
 
