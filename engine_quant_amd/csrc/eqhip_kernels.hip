@@ -603,12 +603,14 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
     // A: one list over the waves, the fixings of periods in progress (entry f < nfix, taken by wave f;
     // sF holds 1 / P(T_fix, T_next | r_fix) = exp(B r_fix) / A, so that a family start multiplies) and
     // then the zero-coupon bonds of the live cash-flow dates, for the tile starting at path pa, from
-    // the rates in the _next registers.
+    // the rates in the _next registers. A wave takes its bonds two at a time (two independent exponential
+    // chains).
     auto stage_a = [&](uint64_t pa) {
         if (NFAC == 2) {  // sF = 1 / P(T_fix, T_next | x_f, y_f), sP = P(t_k, T_d | x, y)
             const cptr<double> b2d = cst(pg->AB2 + (size_t)k * pg->D);
-            for (int e = wave; e < nfix + n_live; e += kValueWaves) {
-                if (e < nfix) {
+            int e = wave;
+            for (; e < nfix; e += kValueWaves) {
+                {
                     double c = fab_w.x, b1 = fab_w.y, b2 = fb2_w;
                     uint64_t row = 0;
                     if (e != wave) {  // more fixings than waves
@@ -628,17 +630,30 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                         }
                         sF[e * TW + lane + 64 * pp] = bond_exp_v(-((c - b1 * xf) - b2 * yf));
                     }
-                } else {
-                    const int d = dfirst + (e - nfix);
-#pragma unroll
-                    for (int pp = 0; pp < PPL; ++pp)
-                        sP[d * TW + lane + 64 * pp] = bond_exp_v((abd[2 * d] - abd[2 * d + 1] * r_next[pp]) - b2d[d] * y_next[pp]);
                 }
+            }
+            // the bonds two at a time: two independent exponential chains per wave
+            for (; e + kValueWaves < nfix + n_live; e += 2 * kValueWaves) {
+                const int d0 = dfirst + (e - nfix), d1 = d0 + kValueWaves;
+#pragma unroll
+                for (int pp = 0; pp < PPL; ++pp) {
+                    const double y0 = bond_exp_v((abd[2 * d0] - abd[2 * d0 + 1] * r_next[pp]) - b2d[d0] * y_next[pp]);
+                    const double y1 = bond_exp_v((abd[2 * d1] - abd[2 * d1 + 1] * r_next[pp]) - b2d[d1] * y_next[pp]);
+                    sP[d0 * TW + lane + 64 * pp] = y0;
+                    sP[d1 * TW + lane + 64 * pp] = y1;
+                }
+            }
+            if (e < nfix + n_live) {
+                const int d = dfirst + (e - nfix);
+#pragma unroll
+                for (int pp = 0; pp < PPL; ++pp)
+                    sP[d * TW + lane + 64 * pp] = bond_exp_v((abd[2 * d] - abd[2 * d + 1] * r_next[pp]) - b2d[d] * y_next[pp]);
             }
             return;
         }
-        for (int e = wave; e < nfix + n_live; e += kValueWaves) {
-            if (e < nfix) {
+        int e = wave;
+        for (; e < nfix; e += kValueWaves) {
+            {
                 double inv_a = inv_a_w, b = fab_w.y;
                 uint64_t row = 0;
                 if (e != wave) {  // more fixings than waves
@@ -654,11 +669,23 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
                     if (e != wave) rf = (gpa < n_paths) ? glb(pg->R)[row + gpa] : 0.0;
                     sF[e * TW + lane + 64 * pp] = inv_a * bond_exp_v(b * rf);
                 }
-            } else {
-                const int d = dfirst + (e - nfix);
-#pragma unroll
-                for (int pp = 0; pp < PPL; ++pp) sP[d * TW + lane + 64 * pp] = abd[2 * d] * bond_exp_v(-(abd[2 * d + 1] * r_next[pp]));
             }
+        }
+        // the bonds two at a time: two independent exponential chains per wave
+        for (; e + kValueWaves < nfix + n_live; e += 2 * kValueWaves) {
+            const int d0 = dfirst + (e - nfix), d1 = d0 + kValueWaves;
+#pragma unroll
+            for (int pp = 0; pp < PPL; ++pp) {
+                const double x0 = -(abd[2 * d0 + 1] * r_next[pp]), x1 = -(abd[2 * d1 + 1] * r_next[pp]);
+                const double y0 = bond_exp_v(x0), y1 = bond_exp_v(x1);
+                sP[d0 * TW + lane + 64 * pp] = abd[2 * d0] * y0;
+                sP[d1 * TW + lane + 64 * pp] = abd[2 * d1] * y1;
+            }
+        }
+        if (e < nfix + n_live) {
+            const int d = dfirst + (e - nfix);
+#pragma unroll
+            for (int pp = 0; pp < PPL; ++pp) sP[d * TW + lane + 64 * pp] = abd[2 * d] * bond_exp_v(-(abd[2 * d + 1] * r_next[pp]));
         }
     };
     // a netting-set value of this unit: stored as its positive part (and the signed value when asked),
