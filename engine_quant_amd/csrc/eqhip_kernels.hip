@@ -569,6 +569,15 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
 #pragma unroll
         for (int pp = 0; pp < PPL; ++pp) sC[(lane + 64 * pp) * CS + cst(pg->zcls)[zb + i]] = make_double2(0.0, 0.0);
 
+    // the first netting set's entries of this wave in the set step (the same on every tile)
+    int cb_set0 = 0, ce_set0 = 0;
+    if (pg->set_mode == 0 && pg->S > 0) {
+        const int beg = cst(pg->set_off)[0];
+        const int end = beg + cst(pg->set_nlive)[(size_t)k * pg->S];
+        const int per = (end - beg + kValueWaves - 1) / kValueWaves;
+        cb_set0 = beg + wave * per;
+        ce_set0 = min(end, cb_set0 + per);
+    }
     const uint64_t tile0 = (uint64_t)chunk * pg->tiles_per_block;
     uint64_t tile_end = tile0 + pg->tiles_per_block;
     if (tile_end > pg->n_tiles) tile_end = pg->n_tiles;
@@ -871,11 +880,14 @@ __device__ __forceinline__ void value_unit(const ValuePlan& plan, const int k, c
         if (pg->set_mode == 0) {
             for (int s = 0; s < pg->S; ++s) {
                 // the set's classes alive at t (a prefix of its entries), split evenly over the waves
-                const int beg = cst(pg->set_off)[s];
-                const int end = beg + cst(pg->set_nlive)[(size_t)k * pg->S + s];
-                const int per = (end - beg + kValueWaves - 1) / kValueWaves;
-                const int cb = beg + wave * per;
-                const int ce = min(end, cb + per);
+                int cb = cb_set0, ce = ce_set0;
+                if (s > 0) {
+                    const int beg = cst(pg->set_off)[s];
+                    const int end = beg + cst(pg->set_nlive)[(size_t)k * pg->S + s];
+                    const int per = (end - beg + kValueWaves - 1) / kValueWaves;
+                    cb = beg + wave * per;
+                    ce = min(end, cb + per);
+                }
                 const cptr<int> scl = cst(pg->set_cls);
                 const cptr<double> sal = cst(pg->set_alpha), sbe = cst(pg->set_beta);
                 double v[PPL];
