@@ -3,7 +3,8 @@ gradient, with optional emulation of the BF16 operand rounding of the GPU kernel
 
 TEST INFRASTRUCTURE ONLY: imported by ``tests/`` as the checker of the HIP forward / backward / Adam
 kernels. There is no reference counterpart: the reference's only regression method is the quadratic
-LSM fit of ``payoff/lsm.rs:73-107``; this network is a north-star addition.
+LSM fit of ``payoff/lsm.rs:73-107``; this network is an addition of this library (the Bermudan
+exposure surrogate).
 
 Parameter vector (float32 master copy on the GPU, same order here), N_PARAMS = 8641:
   W1 [64][4]  (columns: inputs u0, u1, u2, then the layer-1 bias)   offset 0
